@@ -1,0 +1,160 @@
+/*
+ * palial_hip.h -- C ABI of libpalial_hip.so, the MI355X (gfx950) tiled dense
+ * factorization library.  Plain C types only: a drop-in for the reference's
+ * algorithm layer (src/cholesky.h, src/lu.h, src/qr.h) and tile kernels
+ * (src/srclu/*.h, src/srcqr/*.h), with the reference's tile layout
+ * (include/descriptor.h, include/tile_address.h).
+ *
+ * Tile layout L: an N x N matrix (N % b == 0) is stored as (N/b)^2 contiguous
+ * b x b tiles; tile (m,n) starts at element b*b*(m + (N/b)*n) and is
+ * column-major inside with leading dimension b.
+ *
+ * Every entry point is synchronous: it returns after the device work it
+ * issued has completed.  One call at a time per process (calls are
+ * serialised by an internal mutex).  Return codes: 0 success; > 0 a
+ * LAPACK-style info (first failing pivot, 1-based global column); < 0 one of
+ * the PALIAL_ERR_* codes below.  The library never calls exit().
+ */
+#ifndef PALIAL_HIP_H
+#define PALIAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Byte-identical to the reference descriptor (include/descriptor.h:13-20):
+ * field offsets 0/8/16/24/32, sizeof 40 on x86-64.  Passed by value. */
+typedef struct matrix_descriptor_t {
+    int      tile_size;        /* b */
+    long int matrix_nelements; /* N*N */
+    long int tile_nelements;   /* b*b */
+    int      matrix_size;      /* N */
+    void    *matrix;           /* host or device pointer to layout-L storage */
+} MATRIX_desc;
+
+/* error codes */
+#define PALIAL_OK            0
+#define PALIAL_ERR_ARG      -1   /* bad argument (N % b != 0, NULL pointer, unsupported side/trans) */
+#define PALIAL_ERR_HIP      -2   /* a HIP runtime call failed */
+#define PALIAL_ERR_NOMEM    -3   /* device allocation failed */
+#define PALIAL_ERR_NODEV    -4   /* no usable gfx950 device */
+#define PALIAL_ERR_STATE    -5   /* library not initialised */
+#define PALIAL_ERR_NOCONV   -6   /* iterative refinement did not reach the requested accuracy */
+
+/* precision tier of the O(n^3) trailing updates */
+#define PALIAL_PREC_FP64  0
+#define PALIAL_PREC_FP32  1
+#define PALIAL_PREC_BF16  2
+#define PALIAL_PREC_FP8   3
+
+/* options; a NULL pointer means all defaults (zero-initialised struct) */
+typedef struct palial_opts {
+    int   device_matrix;  /* nonzero: MATRIX_desc.matrix (and S, B) are device pointers */
+    int   precision;      /* PALIAL_PREC_*; factorization calls honour FP64 only */
+    int   no_graph;       /* nonzero: launch stream-ordered instead of replaying a hipGraph */
+    int   no_lookahead;   /* nonzero: single stream, no panel look-ahead */
+    void *stream;         /* hipStream_t to run on (device_matrix only); NULL = library stream */
+    int   refine;         /* solves: 0 = LU-IR, 1 = GMRES-IR, 2 = auto */
+    int   max_iter;       /* solves: refinement steps (0 = 30) */
+    double tol;           /* solves: target backward error (0 = 1e-14) */
+    int   reserved[8];
+} palial_opts;
+
+/* per-call report; any pointer may be NULL */
+typedef struct palial_report {
+    double factor_ms;     /* device time of the factorization (hipEvents) */
+    double solve_ms;      /* device time of solve + refinement */
+    double h2d_ms;        /* host->device copies issued by this call */
+    double d2h_ms;        /* device->host copies issued by this call */
+    double gflop;         /* algorithmic flops of the factorization (N^3/3, 2N^3/3, 4N^3/3) / 1e9 */
+    double tflops;        /* gflop / factor_ms */
+    double backward_error;/* solves: ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+    int    info;          /* same as the return value when > 0 */
+    int    iterations;    /* solves: refinement steps taken */
+    long   ntasks;        /* tile tasks of the reference DAG this call covers */
+    long   nlaunches;     /* kernel launches issued (or graph nodes replayed) */
+    int    reserved[8];
+} palial_report;
+
+/* ---------------------------------------------------------------- lifecycle */
+int         palial_hip_init(int device);      /* select device, create streams/workspace */
+int         palial_hip_finalize(void);
+const char *palial_hip_strerror(int code);
+const char *palial_hip_last_error(void);      /* text of the last failure (HIP message etc.) */
+int         palial_hip_version(void);         /* 100*major + minor */
+
+/* device memory helpers for C callers that pass device_matrix = 1 */
+int palial_hip_malloc(void **dptr, size_t bytes);
+int palial_hip_free(void *dptr);
+int palial_hip_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int palial_hip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* ---------------------------------------------------------------- inputs */
+/* Seeded counter-based generators writing layout L (host or device per opts):
+ *   kind 0: S + shift*I, S symmetric, S_ij = U(seed, min(i,j), max(i,j))
+ *   kind 1: G + shift*I, G_ij = U(seed, i, j)
+ *   kind 2: reference populate.h generator (include/populate.h:35-66) made
+ *           reproducible: diagonal 173*(1 + U01), everything else 0.5
+ *   kind 3: identity (include/populate.h:48,58)
+ * U(seed,i,j) = (splitmix64(splitmix64(seed) + (i<<32 | j)) >> 11) * 2^-53 - 0.5 */
+int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift,
+                        const palial_opts *opts);
+
+/* ---------------------------------------------------------------- algorithms */
+/* replaces void cholesky(MATRIX_desc A)   (src/cholesky.h:12)
+ * Upper right-looking tiled Cholesky A = U^T U, in place; the strictly lower
+ * part of diagonal tiles and the strictly lower tiles keep their input. */
+int palial_hip_cholesky(MATRIX_desc A, const palial_opts *opts, palial_report *rep);
+
+/* replaces void lu(MATRIX_desc A)          (src/lu.h:11)
+ * Right-looking tiled LU without pivoting (src/srclu/dgetrf.h:20), in place:
+ * unit-lower L below the diagonal, U on and above.  ipiv (may be NULL)
+ * receives the identity permutation, 1-based, N entries. */
+int palial_hip_lu(MATRIX_desc A, int *ipiv, const palial_opts *opts, palial_report *rep);
+
+/* replaces void qr(MATRIX_desc A, MATRIX_desc S)   (src/qr.h:15)
+ * Flat-tree TS Householder QR: R and V in A, T factors in the upper triangles
+ * of S(k,k) and S(m,k); all other S entries are not written. */
+int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_report *rep);
+
+/* ---------------------------------------------------------------- tile kernels */
+/* Single-tile versions of the reference tile kernels; pointers follow
+ * opts->device_matrix.  The leading dimensions must equal n (tile storage). */
+int palial_hip_dpotrf(int n, double *A, int lda, const palial_opts *opts);      /* LAPACKE_dpotrf('U') at src/cholesky.h:40 */
+int palial_hip_dgetrf(int n, double *A, int lda, const palial_opts *opts);      /* PALIAL_dgetrf, src/srclu/dgetrf.h:11 */
+int palial_hip_dgeqrt(int n, int nb, double *A, int lda, double *T, int ldt,
+                      double *tau, double *work, const palial_opts *opts);       /* PALIAL_dgeqrt, src/srcqr/dgeqrt.h:9 */
+int palial_hip_dormqr(int side, int trans, int n, const double *V, int ldv,
+                      const double *T, int ldt, double *C, int ldc, double *work,
+                      int ldw, const palial_opts *opts);                         /* PALIAL_dormqr, src/srcqr/dormqr.h:9 */
+int palial_hip_dtsqrt(int n, double *A1, int lda1, double *A2, int lda2, double *T,
+                      int ldt, double *tau, double *work, const palial_opts *opts); /* PALIAL_dtsqrt, src/srcqr/dtsqrt.h:11 */
+int palial_hip_dtsmqr(int side, int trans, int M, int N, int K, int ib, double *A1,
+                      int lda1, double *A2, int lda2, const double *V, int ldv,
+                      const double *T, int ldt, double *work, int ldw,
+                      const palial_opts *opts);                                  /* PALIAL_dtsmqr, src/srcqr/dtsmqr.h:11 */
+
+/* The BLAS tile tasks of the Cholesky and LU DAGs, as the reference calls them:
+ *   PALIAL_TASK_CHOL_TRSM  C <- A^-T C, A upper          src/cholesky.h:73-84
+ *   PALIAL_TASK_CHOL_SYRK  C <- C - A^T A (upper of C)    src/cholesky.h:115-125
+ *   PALIAL_TASK_CHOL_GEMM  C <- C - A^T B                 src/cholesky.h:156-169
+ *   PALIAL_TASK_LU_TRSM_R  C <- C A^-1, A upper           src/lu.h:67-78
+ *   PALIAL_TASK_LU_TRSM_L  C <- A^-1 C, A unit lower      src/lu.h:109-120
+ *   PALIAL_TASK_LU_GEMM    C <- C - A B                   src/lu.h:151-164
+ * A, B, C are n x n tiles (ld = n); B is ignored by TRSM/SYRK. */
+#define PALIAL_TASK_CHOL_TRSM 1
+#define PALIAL_TASK_CHOL_SYRK 2
+#define PALIAL_TASK_CHOL_GEMM 3
+#define PALIAL_TASK_LU_TRSM_R 4
+#define PALIAL_TASK_LU_TRSM_L 5
+#define PALIAL_TASK_LU_GEMM   6
+int palial_hip_tile_task(int task, int n, const double *A, const double *B, double *C,
+                         const palial_opts *opts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PALIAL_HIP_H */

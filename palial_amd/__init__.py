@@ -1,0 +1,279 @@
+"""palial_amd -- MI355X (gfx950) tiled dense factorizations.
+
+Python mirror of the reference's algorithm interface over the C ABI of
+``libpalial_hip.so`` (include/palial_hip.h):
+
+    cholesky(A)   <- void cholesky(MATRIX_desc A)            src/cholesky.h:12
+    lu(A)         <- void lu(MATRIX_desc A)                  src/lu.h:11
+    qr(A, S)      <- void qr(MATRIX_desc A, MATRIX_desc S)   src/qr.h:15
+
+``A`` may be a MATRIX_desc, a flat float64 numpy array in tile layout L (host
+memory, factored in place), or a float64 CUDA/HIP torch tensor (device memory,
+factored in place).  Tile kernels (dpotrf, dgetrf, dgeqrt, dormqr, dtsqrt,
+dtsmqr and the BLAS tile tasks) take column-major n x n numpy tiles.
+
+There is no CPU fallback: every call goes to the HIP library, and importing
+the package fails loudly when ``libpalial_hip.so`` has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpalial_hip.so")
+DRIVER_PATH = os.path.join(_HERE, "bin", "palial")
+
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
+LEFT, RIGHT = 141, 142
+NOTRANS, TRANS = 111, 112
+TASK_CHOL_TRSM, TASK_CHOL_SYRK, TASK_CHOL_GEMM = 1, 2, 3
+TASK_LU_TRSM_R, TASK_LU_TRSM_L, TASK_LU_GEMM = 4, 5, 6
+GEN_SYM, GEN_DENSE, GEN_PALIAL, GEN_IDENTITY = 0, 1, 2, 3
+
+
+class MATRIX_desc(C.Structure):
+    """Byte-identical to the reference descriptor (include/descriptor.h:13-20)."""
+    _fields_ = [("tile_size", C.c_int), ("matrix_nelements", C.c_long),
+                ("tile_nelements", C.c_long), ("matrix_size", C.c_int),
+                ("matrix", C.c_void_p)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("device_matrix", C.c_int), ("precision", C.c_int), ("no_graph", C.c_int),
+                ("no_lookahead", C.c_int), ("stream", C.c_void_p), ("refine", C.c_int),
+                ("max_iter", C.c_int), ("tol", C.c_double), ("reserved", C.c_int * 8)]
+
+
+class Report(C.Structure):
+    _fields_ = [("factor_ms", C.c_double), ("solve_ms", C.c_double), ("h2d_ms", C.c_double),
+                ("d2h_ms", C.c_double), ("gflop", C.c_double), ("tflops", C.c_double),
+                ("backward_error", C.c_double), ("info", C.c_int), ("iterations", C.c_int),
+                ("ntasks", C.c_long), ("nlaunches", C.c_long), ("reserved", C.c_int * 8)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class PalialError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"palial error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_inited = False
+
+
+def lib():
+    """Load libpalial_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m palial_amd.build` first "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        dp = C.c_void_p
+        L.palial_hip_init.argtypes = [C.c_int]
+        L.palial_hip_strerror.restype = C.c_char_p
+        L.palial_hip_strerror.argtypes = [C.c_int]
+        L.palial_hip_last_error.restype = C.c_char_p
+        L.palial_hip_generate.argtypes = [MATRIX_desc, C.c_int, C.c_uint64, C.c_double, C.POINTER(Opts)]
+        L.palial_hip_cholesky.argtypes = [MATRIX_desc, C.POINTER(Opts), C.POINTER(Report)]
+        L.palial_hip_lu.argtypes = [MATRIX_desc, dp, C.POINTER(Opts), C.POINTER(Report)]
+        L.palial_hip_qr.argtypes = [MATRIX_desc, MATRIX_desc, C.POINTER(Opts), C.POINTER(Report)]
+        L.palial_hip_dpotrf.argtypes = [C.c_int, dp, C.c_int, C.POINTER(Opts)]
+        L.palial_hip_dgetrf.argtypes = [C.c_int, dp, C.c_int, C.POINTER(Opts)]
+        L.palial_hip_dgeqrt.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, dp, dp, C.POINTER(Opts)]
+        L.palial_hip_dormqr.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int,
+                                        dp, C.c_int, C.POINTER(Opts)]
+        L.palial_hip_dtsqrt.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int, dp, dp,
+                                        C.POINTER(Opts)]
+        L.palial_hip_dtsmqr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int,
+                                        dp, C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int, C.POINTER(Opts)]
+        L.palial_hip_tile_task.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(Opts)]
+        L.palial_hip_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.palial_hip_free.argtypes = [dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        L = lib()
+        raise PalialError(rc, f"{L.palial_hip_strerror(rc).decode()}: {L.palial_hip_last_error().decode()}")
+    return rc
+
+
+def init(device=0):
+    global _inited
+    if not _inited:
+        _check(lib().palial_hip_init(device))
+        _inited = True
+
+
+def finalize():
+    global _inited
+    if _inited:
+        lib().palial_hip_finalize()
+        _inited = False
+
+
+def _opts(device=False, no_graph=False, no_lookahead=False, precision=PREC_FP64, stream=None,
+          refine=0, max_iter=0, tol=0.0):
+    o = Opts()
+    o.device_matrix = int(bool(device))
+    o.no_graph = int(bool(no_graph))
+    o.no_lookahead = int(bool(no_lookahead))
+    o.precision = precision
+    o.stream = stream
+    o.refine = refine
+    o.max_iter = max_iter
+    o.tol = tol
+    return o
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _desc_of(A, N=None, b=None):
+    """Return (MATRIX_desc, on_device, keepalive)."""
+    if isinstance(A, MATRIX_desc):
+        return A, None, A
+    if _is_torch(A):
+        import torch
+        assert A.dtype == torch.float64 and A.is_contiguous()
+        N = N or int(round(A.numel() ** 0.5))
+        assert N * N == A.numel() and b
+        return MATRIX_desc(b, N * N, b * b, N, A.data_ptr()), A.is_cuda, A
+    a = A
+    assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+    N = N or int(round(a.size ** 0.5))
+    assert N * N == a.size and b
+    return MATRIX_desc(b, N * N, b * b, N, a.ctypes.data), False, a
+
+
+def _torch_stream(A):
+    import torch
+    return C.c_void_p(torch.cuda.current_stream(A.device).cuda_stream)
+
+
+def generate(A, kind, seed=0, shift=0.0, N=None, b=None):
+    """Fill A (layout L) with a seeded generator: GEN_SYM (S + shift I), GEN_DENSE
+    (G + shift I), GEN_PALIAL (include/populate.h:35-66, seeded), GEN_IDENTITY."""
+    init()
+    d, dev, keep = _desc_of(A, N, b)
+    o = _opts(device=dev)
+    if dev:
+        o.stream = _torch_stream(A)
+    _check(lib().palial_hip_generate(d, kind, seed, float(shift), C.byref(o)))
+    return A
+
+
+def _run(fn, A, extra, N, b, **kw):
+    init()
+    d, dev, keep = _desc_of(A, N, b)
+    o = _opts(device=dev if dev is not None else kw.pop("device", False), **kw)
+    if dev:
+        o.stream = _torch_stream(A)
+    rep = Report()
+    rc = _check(fn(d, *extra(dev), C.byref(o), C.byref(rep)))
+    return rc, rep
+
+
+def cholesky(A, b=None, N=None, **kw):
+    """Upper tiled Cholesky in place.  Returns (info, report)."""
+    return _run(lib().palial_hip_cholesky, A, lambda dev: (), N, b, **kw)
+
+
+def lu(A, b=None, N=None, ipiv=None, **kw):
+    """Tiled LU without pivoting in place.  Returns (info, report, ipiv)."""
+    n = N or (A.matrix_size if isinstance(A, MATRIX_desc) else int(round(
+        (A.numel() if _is_torch(A) else A.size) ** 0.5)))
+    piv = np.zeros(n, dtype=np.int32) if ipiv is None else ipiv
+    rc, rep = _run(lib().palial_hip_lu, A, lambda dev: (piv.ctypes.data,), N, b, **kw)
+    return rc, rep, piv
+
+
+def qr(A, S, b=None, N=None, **kw):
+    """Flat-tree tiled QR in place; T factors go to S.  Returns (info, report)."""
+    init()
+    dA, dev, ka = _desc_of(A, N, b)
+    dS, _, ks = _desc_of(S, N, b)
+    o = _opts(device=dev, **kw)
+    if dev:
+        o.stream = _torch_stream(A)
+    rep = Report()
+    rc = _check(lib().palial_hip_qr(dA, dS, C.byref(o), C.byref(rep)))
+    return rc, rep
+
+
+# ------------------------------------------------------------------ tile kernels
+def _f(a):
+    a = np.asfortranarray(np.array(a, dtype=np.float64, copy=True))
+    return a
+
+
+def dpotrf(A):
+    A = _f(A)
+    init()
+    info = _check(lib().palial_hip_dpotrf(A.shape[0], A.ctypes.data, A.shape[0], None))
+    return A, info
+
+
+def dgetrf(A):
+    A = _f(A)
+    init()
+    _check(lib().palial_hip_dgetrf(A.shape[0], A.ctypes.data, A.shape[0], None))
+    return A
+
+
+def dgeqrt(A):
+    A = _f(A)
+    n = A.shape[0]
+    T = np.zeros((n, n), order="F")
+    tau = np.zeros(n)
+    init()
+    _check(lib().palial_hip_dgeqrt(n, n, A.ctypes.data, n, T.ctypes.data, n, tau.ctypes.data, None, None))
+    return A, T, tau
+
+
+def dormqr(V, T, Cm, side=LEFT, trans=TRANS):
+    V, T, Cm = _f(V), _f(T), _f(Cm)
+    n = V.shape[0]
+    init()
+    _check(lib().palial_hip_dormqr(side, trans, n, V.ctypes.data, n, T.ctypes.data, n, Cm.ctypes.data, n,
+                                   None, n, None))
+    return Cm
+
+
+def dtsqrt(A1, A2):
+    A1, A2 = _f(A1), _f(A2)
+    n = A1.shape[0]
+    T = np.zeros((n, n), order="F")
+    tau = np.zeros(n)
+    init()
+    _check(lib().palial_hip_dtsqrt(n, A1.ctypes.data, n, A2.ctypes.data, n, T.ctypes.data, n,
+                                   tau.ctypes.data, None, None))
+    return A1, A2, T, tau
+
+
+def dtsmqr(A1, A2, V, T, side=LEFT, trans=TRANS):
+    A1, A2, V, T = _f(A1), _f(A2), _f(V), _f(T)
+    n = A1.shape[0]
+    init()
+    _check(lib().palial_hip_dtsmqr(side, trans, n, n, n, n, A1.ctypes.data, n, A2.ctypes.data, n,
+                                   V.ctypes.data, n, T.ctypes.data, n, None, n, None))
+    return A1, A2
+
+
+def tile_task(task, A, B, Cm):
+    A, Cm = _f(A), _f(Cm)
+    Bp = None
+    if B is not None:
+        B = _f(B)
+        Bp = B.ctypes.data
+    init()
+    _check(lib().palial_hip_tile_task(task, A.shape[0], A.ctypes.data, Bp, Cm.ctypes.data, None))
+    return Cm

@@ -1,0 +1,783 @@
+// api.hip -- C ABI of libpalial_hip.so (declared in include/palial_hip.h).
+//
+// The reference runs each algorithm as an OpenMP task DAG created by one
+// master thread (palial_src/main.c:124-159).  Here each DAG step becomes a
+// handful of batched launches on two HIP streams:
+//   panel stream : look-ahead update of the next panel, then its diagonal
+//                  factorization and panel TRSMs;
+//   update stream: the rest of the trailing update of the current step.
+// Cross-stream dependences are events.  The whole sequence is captured once
+// into a hipGraph per (algorithm, N, b, device pointer) and replayed.
+#include "internal.hpp"
+#include "../../include/palial_hip.h"
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace palial;
+
+namespace {
+
+struct Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long nodes = 0;
+};
+
+struct Ctx {
+    bool init = false;
+    int device = -1;
+    hipStream_t s_main = nullptr;   // update stream (and the caller-visible stream)
+    hipStream_t s_panel = nullptr;  // look-ahead panel stream
+    std::vector<hipEvent_t> events;
+    int *d_info = nullptr;
+    double *d_tau = nullptr;        // QR tau scratch (b doubles, overwritten per panel)
+    size_t tau_cap = 0;
+    double *d_w1 = nullptr, *d_w2 = nullptr;  // QR/ORMQR workspace tiles
+    size_t w_cap = 0;
+    std::map<std::tuple<int, void *, void *, int, int, int>, Graph> graphs;
+};
+
+Ctx g;
+std::mutex g_mu;
+thread_local std::string g_last;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(PALIAL_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+palial_opts default_opts()
+{
+    palial_opts o;
+    memset(&o, 0, sizeof o);
+    return o;
+}
+
+int ensure_init()
+{
+    if (g.init) return 0;
+    return fail(PALIAL_ERR_STATE, "palial_hip_init() has not been called");
+}
+
+int ensure_events(size_t n)
+{
+    while (g.events.size() < n) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        g.events.push_back(e);
+    }
+    return 0;
+}
+
+void drop_graphs()
+{
+    for (auto &kv : g.graphs) {
+        if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) hipGraphDestroy(kv.second.graph);
+    }
+    g.graphs.clear();
+}
+
+int ensure_ws(int b, int MT)
+{
+    const size_t need = (size_t)b * b * (size_t)(MT > 1 ? MT : 1);
+    if (g.w_cap < need || g.tau_cap < (size_t)b) drop_graphs();   // captured graphs hold the old pointers
+    if (g.w_cap < need) {
+        if (g.d_w1) hipFree(g.d_w1);
+        if (g.d_w2) hipFree(g.d_w2);
+        g.d_w1 = g.d_w2 = nullptr;
+        g.w_cap = 0;
+        if (hipMalloc(&g.d_w1, need * 8) != hipSuccess || hipMalloc(&g.d_w2, need * 8) != hipSuccess)
+            return fail(PALIAL_ERR_NOMEM, "workspace allocation of %zu bytes failed", 2 * need * 8);
+        g.w_cap = need;
+    }
+    if (g.tau_cap < (size_t)b) {
+        if (g.d_tau) hipFree(g.d_tau);
+        if (hipMalloc(&g.d_tau, (size_t)b * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "tau alloc failed");
+        g.tau_cap = b;
+    }
+    return 0;
+}
+
+int check_desc(const MATRIX_desc &A)
+{
+    if (!A.matrix) return fail(PALIAL_ERR_ARG, "descriptor has a NULL matrix pointer");
+    if (A.tile_size <= 0 || A.matrix_size <= 0)
+        return fail(PALIAL_ERR_ARG, "bad sizes N=%d b=%d", A.matrix_size, A.tile_size);
+    if (A.matrix_size % A.tile_size != 0)
+        return fail(PALIAL_ERR_ARG, "Tile size does not divide the matrix size (N=%d b=%d)", A.matrix_size,
+                    A.tile_size);
+    if (A.tile_size > 1024) return fail(PALIAL_ERR_ARG, "tile size %d above the supported 1024", A.tile_size);
+    return 0;
+}
+
+// ------------------------------------------------------------------ DAGs
+struct Launcher {
+    long launches = 0;
+    hipError_t err = hipSuccess;
+    void chk(hipError_t e)
+    {
+        launches++;
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    }
+};
+
+GemmArgs gargs(int mode, double *mat, int N, int b, int k, int T)
+{
+    GemmArgs a;
+    memset(&a, 0, sizeof a);
+    a.mode = mode;
+    a.mat = mat;
+    a.N = N;
+    a.b = b;
+    a.MT = N / b;
+    a.k = k;
+    a.T = T;
+    a.ntasks = (int)gemm_task_count(mode, T);
+    a.alpha = -1.0;
+    a.beta = 1.0;
+    return a;
+}
+
+TrsmArgs targs(double *mat, int N, int b, int k)
+{
+    TrsmArgs t;
+    memset(&t, 0, sizeof t);
+    const int MT = N / b;
+    t.diag = mat + tile_off(b, MT, k, k);
+    t.mat = mat;
+    t.N = N;
+    t.b = b;
+    t.MT = MT;
+    t.k = k;
+    t.first = k + 1;
+    t.ntiles = MT - k - 1;
+    return t;
+}
+
+// Cholesky, src/cholesky.h:15-176.  Look-ahead: step k's update of tile row
+// k+1 runs on the panel stream ahead of the rest, so POTRF/TRSM of step k+1
+// overlap the bulk of step k's trailing update.
+void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, int *dinfo,
+                  Launcher &L)
+{
+    const int MT = N / b;
+    if (!lookahead) {
+        for (int k = 0; k < MT; k++) {
+            L.chk(launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s0));
+            L.chk(launch_trsm(TR_CHOL, targs(dA, N, b, k), s0));
+            const int T = MT - k - 1;
+            if (T > 0) L.chk(launch_gemm(gargs(GM_CHOL_TRAIL, dA, N, b, k, T), true, false, s0));
+        }
+        return;
+    }
+    hipEvent_t *evP = g.events.data();          // panel k ready
+    hipEvent_t *evU = g.events.data() + MT + 1; // rest-update k done
+    hipEvent_t fork = g.events[2 * MT + 2];
+    L.chk(hipEventRecord(fork, s0));
+    L.chk(hipStreamWaitEvent(s1, fork, 0));
+    L.chk(launch_potrf(dA + tile_off(b, MT, 0, 0), b, dinfo, 0, s1));
+    L.chk(launch_trsm(TR_CHOL, targs(dA, N, b, 0), s1));
+    L.chk(hipEventRecord(evP[0], s1));
+    for (int k = 0; k < MT; k++) {
+        const int T = MT - k - 1;
+        if (T > 0) {
+            if (k > 0) L.chk(hipStreamWaitEvent(s1, evU[k - 1], 0));
+            L.chk(launch_gemm(gargs(GM_CHOL_ROW, dA, N, b, k, T), true, false, s1));
+            L.chk(launch_potrf(dA + tile_off(b, MT, k + 1, k + 1), b, dinfo, (k + 1) * b, s1));
+            L.chk(launch_trsm(TR_CHOL, targs(dA, N, b, k + 1), s1));
+            L.chk(hipEventRecord(evP[k + 1], s1));
+        }
+        L.chk(hipStreamWaitEvent(s0, evP[k], 0));
+        if (T > 1) L.chk(launch_gemm(gargs(GM_CHOL_REST, dA, N, b, k, T), true, false, s0));
+        L.chk(hipEventRecord(evU[k], s0));
+    }
+}
+
+// LU without pivoting, src/lu.h:14-173 (with correct tile dependences).
+void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, Launcher &L)
+{
+    const int MT = N / b;
+    auto panel = [&](int k, hipStream_t s) {
+        L.chk(launch_getrf(dA + tile_off(b, MT, k, k), b, s));
+        L.chk(launch_trsm(TR_LUR, targs(dA, N, b, k), s));
+        L.chk(launch_trsm(TR_LUL, targs(dA, N, b, k), s));
+    };
+    if (!lookahead) {
+        for (int k = 0; k < MT; k++) {
+            panel(k, s0);
+            const int T = MT - k - 1;
+            if (T > 0) L.chk(launch_gemm(gargs(GM_LU_TRAIL, dA, N, b, k, T), false, false, s0));
+        }
+        return;
+    }
+    hipEvent_t *evP = g.events.data();
+    hipEvent_t *evU = g.events.data() + MT + 1;
+    hipEvent_t fork = g.events[2 * MT + 2];
+    L.chk(hipEventRecord(fork, s0));
+    L.chk(hipStreamWaitEvent(s1, fork, 0));
+    panel(0, s1);
+    L.chk(hipEventRecord(evP[0], s1));
+    for (int k = 0; k < MT; k++) {
+        const int T = MT - k - 1;
+        if (T > 0) {
+            if (k > 0) L.chk(hipStreamWaitEvent(s1, evU[k - 1], 0));
+            L.chk(launch_gemm(gargs(GM_LU_PANEL, dA, N, b, k, T), false, false, s1));
+            panel(k + 1, s1);
+            L.chk(hipEventRecord(evP[k + 1], s1));
+        }
+        L.chk(hipStreamWaitEvent(s0, evP[k], 0));
+        if (T > 1) L.chk(launch_gemm(gargs(GM_LU_REST, dA, N, b, k, T), false, false, s0));
+        L.chk(hipEventRecord(evU[k], s0));
+    }
+}
+
+// Flat-tree tiled QR, src/qr.h:18-190.
+void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, Launcher &L)
+{
+    const int MT = N / b;
+    for (int k = 0; k < MT; k++) {
+        const int T = MT - k - 1;
+        L.chk(launch_geqrt(dA + tile_off(b, MT, k, k), dS + tile_off(b, MT, k, k), g.d_tau, b, s0));
+        if (T > 0) {
+            // PALIAL_dormqr on A(k,n), n > k: C -= V T^T V^T C
+            GemmArgs a = gargs(GM_ORM_W, dA, N, b, k, T);
+            a.smat = dS; a.w1 = g.d_w1; a.w2 = g.d_w2;
+            a.alpha = 1.0; a.beta = 0.0; a.maskA = MK_UNIT_LOWER;
+            L.chk(launch_gemm(a, true, false, s0));
+            a.mode = GM_ORM_TW; a.maskA = MK_UPPER;
+            L.chk(launch_gemm(a, true, false, s0));
+            a.mode = GM_ORM_VW; a.maskA = MK_UNIT_LOWER; a.alpha = -1.0; a.beta = 1.0;
+            L.chk(launch_gemm(a, false, false, s0));
+        }
+        for (int m = k + 1; m < MT; m++) {
+            L.chk(launch_tsqrt(dA + tile_off(b, MT, k, k), dA + tile_off(b, MT, m, k),
+                               dS + tile_off(b, MT, m, k), g.d_tau, b, s0));
+            if (T > 0) {
+                // PALIAL_dtsmqr on (A(k,n), A(m,n)), n > k
+                GemmArgs a = gargs(GM_QR_W, dA, N, b, k, T);
+                a.m = m; a.smat = dS; a.w1 = g.d_w1; a.w2 = g.d_w2;
+                a.alpha = 1.0; a.beta = 1.0;
+                L.chk(launch_gemm(a, true, false, s0));
+                a.mode = GM_QR_TW; a.maskA = MK_UPPER; a.beta = 0.0;
+                L.chk(launch_gemm(a, true, false, s0));
+                a.mode = GM_QR_VW; a.maskA = MK_NONE; a.alpha = -1.0; a.beta = 1.0;
+                L.chk(launch_gemm(a, false, false, s0));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ run helper
+enum Algo { ALG_CHOL = 0, ALG_LU = 1, ALG_QR = 2 };
+
+int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, palial_report *rep)
+{
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if ((rc = check_desc(A))) return rc;
+    if (S && (rc = check_desc(*S))) return rc;
+    if (S && (S->matrix_size != A.matrix_size || S->tile_size != A.tile_size))
+        return fail(PALIAL_ERR_ARG, "S must have the same N and b as A");
+    palial_opts o = o_in ? *o_in : default_opts();
+    const int N = A.matrix_size, b = A.tile_size, MT = N / b;
+    const size_t bytes = (size_t)N * N * 8;
+    hipStream_t s0 = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
+    hipStream_t s1 = g.s_panel;
+    const bool lookahead = !o.no_lookahead && MT > 1 && algo != ALG_QR;
+    const bool use_graph = !o.no_graph;
+    if ((rc = ensure_events(2 * (size_t)MT + 4))) return rc;
+    if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
+
+    double *dA = nullptr, *dS = nullptr;
+    bool owned = false;
+    hipEvent_t e0, e1, e2, e3;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    HIPCHK(hipEventCreate(&e3));
+    auto cleanup = [&]() {
+        if (owned) {
+            if (dA) hipFree(dA);
+            if (dS) hipFree(dS);
+        }
+        hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
+    };
+    if (o.device_matrix) {
+        dA = (double *)A.matrix;
+        dS = S ? (double *)S->matrix : nullptr;
+    } else {
+        owned = true;
+        if (hipMalloc(&dA, bytes) != hipSuccess) { cleanup(); return fail(PALIAL_ERR_NOMEM, "device alloc %zu B", bytes); }
+        if (S && hipMalloc(&dS, bytes) != hipSuccess) { cleanup(); return fail(PALIAL_ERR_NOMEM, "device alloc %zu B", bytes); }
+    }
+    hipEventRecord(e0, s0);
+    if (!o.device_matrix) {
+        hipMemcpyAsync(dA, A.matrix, bytes, hipMemcpyHostToDevice, s0);
+        if (S) hipMemcpyAsync(dS, S->matrix, bytes, hipMemcpyHostToDevice, s0);
+    }
+    hipMemsetAsync(g.d_info, 0, sizeof(int), s0);
+    hipEventRecord(e1, s0);
+
+    Launcher L;
+    auto issue = [&](hipStream_t st) {
+        if (algo == ALG_CHOL) dag_cholesky(dA, N, b, lookahead, st, s1, g.d_info, L);
+        else if (algo == ALG_LU) dag_lu(dA, N, b, lookahead, st, s1, L);
+        else dag_qr(dA, dS, N, b, st, L);
+    };
+    long nodes = 0;
+    if (use_graph) {
+        auto key = std::make_tuple(algo, (void *)dA, (void *)dS, N, b, (int)lookahead);
+        auto it = g.graphs.find(key);
+        if (it == g.graphs.end() || it->second.exec == nullptr) {
+            Graph G;
+            hipStream_t cs = g.s_main;   // capture on the library stream
+            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "begin capture: %s", hipGetErrorString(e)); }
+            issue(cs);
+            e = hipStreamEndCapture(cs, &G.graph);
+            if (e != hipSuccess || L.err != hipSuccess) {
+                cleanup();
+                return fail(PALIAL_ERR_HIP, "graph capture failed: %s / %s", hipGetErrorString(e),
+                            hipGetErrorString(L.err));
+            }
+            e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+            size_t n = 0;
+            hipGraphGetNodes(G.graph, nullptr, &n);
+            G.nodes = (long)n;
+            g.graphs[key] = G;
+            it = g.graphs.find(key);
+        }
+        // order the replay after the copies on s0
+        hipEvent_t gate = g.events[2 * MT + 3];
+        hipEventRecord(gate, s0);
+        hipStreamWaitEvent(g.s_main, gate, 0);
+        hipError_t e = hipGraphLaunch(it->second.exec, g.s_main);
+        if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "graph launch: %s", hipGetErrorString(e)); }
+        if (s0 != g.s_main) {
+            hipEventRecord(gate, g.s_main);
+            hipStreamWaitEvent(s0, gate, 0);
+        }
+        nodes = it->second.nodes;
+    } else {
+        issue(s0);
+        if (L.err != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "launch failed: %s", hipGetErrorString(L.err)); }
+        nodes = L.launches;
+    }
+    hipEventRecord(e2, s0);
+    int info = 0;
+    hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s0);
+    if (!o.device_matrix) {
+        hipMemcpyAsync(A.matrix, dA, bytes, hipMemcpyDeviceToHost, s0);
+        if (S) hipMemcpyAsync(S->matrix, dS, bytes, hipMemcpyDeviceToHost, s0);
+    }
+    hipEventRecord(e3, s0);
+    hipError_t e = hipStreamSynchronize(s0);
+    if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "synchronize: %s", hipGetErrorString(e)); }
+    float h2d = 0, fac = 0, d2h = 0;
+    hipEventElapsedTime(&h2d, e0, e1);
+    hipEventElapsedTime(&fac, e1, e2);
+    hipEventElapsedTime(&d2h, e2, e3);
+    if (rep) {
+        const double n = N;
+        const double flops = (algo == ALG_CHOL) ? n * n * n / 3 : (algo == ALG_LU) ? 2 * n * n * n / 3 : 4 * n * n * n / 3;
+        rep->factor_ms = fac;
+        rep->h2d_ms = h2d;
+        rep->d2h_ms = d2h;
+        rep->gflop = flops / 1e9;
+        rep->tflops = fac > 0 ? flops / (fac * 1e-3) / 1e12 : 0;
+        rep->info = info;
+        const long mt = MT;
+        rep->ntasks = (algo == ALG_CHOL) ? mt + mt * (mt - 1) + mt * (mt - 1) * (mt - 2) / 6
+                    : mt + mt * (mt - 1) + (mt - 1) * mt * (2 * mt - 1) / 6;
+        rep->nlaunches = nodes;
+    }
+    cleanup();
+    return info;
+}
+
+// single-tile staging: copy n x n tiles in/out when the caller passes host pointers
+struct Stage {
+    std::vector<std::pair<double *, const double *>> back;  // (host dst, device src)
+    std::vector<double *> owned;
+    size_t bytes;
+    bool dev;
+    Stage(size_t by, bool d) : bytes(by), dev(d) {}
+    ~Stage()
+    {
+        for (double *p : owned) hipFree(p);
+    }
+    double *in(const double *h, bool copy_back)
+    {
+        if (dev || !h) return (double *)h;
+        double *d = nullptr;
+        if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+        owned.push_back(d);
+        hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g.s_main);
+        if (copy_back) back.push_back({(double *)h, d});
+        return d;
+    }
+    double *scratch(size_t by)
+    {
+        double *d = nullptr;
+        if (hipMalloc(&d, by) != hipSuccess) return nullptr;
+        owned.push_back(d);
+        return d;
+    }
+    int finish()
+    {
+        for (auto &pr : back) hipMemcpyAsync(pr.first, pr.second, bytes, hipMemcpyDeviceToHost, g.s_main);
+        HIPCHK(hipStreamSynchronize(g.s_main));
+        return 0;
+    }
+};
+
+int check_tile(int n, int lda, const void *A)
+{
+    if (n < 0) return fail(PALIAL_ERR_ARG, "n < 0");
+    if (n > 1024) return fail(PALIAL_ERR_ARG, "tile size %d above 1024", n);
+    if (n > 0 && !A) return fail(PALIAL_ERR_ARG, "NULL tile pointer");
+    if (lda != n) return fail(PALIAL_ERR_ARG, "leading dimension %d must equal the tile size %d", lda, n);
+    return 0;
+}
+
+GemmArgs single(const double *A, const double *B, double *C, const double *D, double *E, int n,
+                double alpha, double beta, int mask)
+{
+    GemmArgs a;
+    memset(&a, 0, sizeof a);
+    a.mode = GM_SINGLE;
+    a.A = A; a.B = B; a.C = C; a.D = D; a.E = E;
+    a.N = n; a.b = n; a.MT = 1; a.T = 1; a.ntasks = 1;
+    a.alpha = alpha; a.beta = beta; a.maskA = mask;
+    return a;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+int palial_hip_version(void) { return 100; }
+
+const char *palial_hip_strerror(int code)
+{
+    switch (code) {
+    case PALIAL_OK: return "success";
+    case PALIAL_ERR_ARG: return "invalid argument";
+    case PALIAL_ERR_HIP: return "HIP runtime error";
+    case PALIAL_ERR_NOMEM: return "device out of memory";
+    case PALIAL_ERR_NODEV: return "no usable gfx950 device";
+    case PALIAL_ERR_STATE: return "library not initialised";
+    case PALIAL_ERR_NOCONV: return "iterative refinement did not converge";
+    default: return code > 0 ? "numerical failure (info > 0: non-positive pivot)" : "unknown error";
+    }
+}
+
+const char *palial_hip_last_error(void) { return g_last.c_str(); }
+
+int palial_hip_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g.init) return device == g.device || device < 0 ? 0 : fail(PALIAL_ERR_STATE, "already initialised on device %d", g.device);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(PALIAL_ERR_NODEV, "no HIP device visible");
+    if (device < 0) device = 0;
+    if (device >= n) return fail(PALIAL_ERR_NODEV, "device %d out of range (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PALIAL_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+    HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
+    panel_init();
+    g.device = device;
+    g.init = true;
+    return 0;
+}
+
+int palial_hip_finalize(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g.init) return 0;
+    hipDeviceSynchronize();
+    drop_graphs();
+    for (auto e : g.events) hipEventDestroy(e);
+    g.events.clear();
+    if (g.d_w1) hipFree(g.d_w1);
+    if (g.d_w2) hipFree(g.d_w2);
+    if (g.d_tau) hipFree(g.d_tau);
+    if (g.d_info) hipFree(g.d_info);
+    hipStreamDestroy(g.s_main);
+    hipStreamDestroy(g.s_panel);
+    g = Ctx();
+    return 0;
+}
+
+int palial_hip_malloc(void **p, size_t bytes)
+{
+    if (!p) return fail(PALIAL_ERR_ARG, "NULL out pointer");
+    if (hipMalloc(p, bytes) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    return 0;
+}
+int palial_hip_free(void *p)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    // graphs captured on this pointer are no longer valid
+    for (auto it = g.graphs.begin(); it != g.graphs.end();) {
+        if (std::get<1>(it->first) == p || std::get<2>(it->first) == p) {
+            if (it->second.exec) hipGraphExecDestroy(it->second.exec);
+            if (it->second.graph) hipGraphDestroy(it->second.graph);
+            it = g.graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    HIPCHK(hipFree(p));
+    return 0;
+}
+int palial_hip_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int palial_hip_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift, const palial_opts *o_in)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if ((rc = check_desc(A))) return rc;
+    if (kind < 0 || kind > 3) return fail(PALIAL_ERR_ARG, "unknown generator kind %d", kind);
+    palial_opts o = o_in ? *o_in : default_opts();
+    const int N = A.matrix_size, b = A.tile_size;
+    const size_t bytes = (size_t)N * N * 8;
+    hipStream_t s = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
+    double *d = (double *)A.matrix;
+    if (!o.device_matrix) {
+        if (hipMalloc(&d, bytes) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "device alloc");
+    }
+    hipError_t e = launch_generate(d, N, b, kind, seed, shift, s);
+    if (e == hipSuccess && !o.device_matrix) e = hipMemcpyAsync(A.matrix, d, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (!o.device_matrix) hipFree(d);
+    if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "generate: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int palial_hip_cholesky(MATRIX_desc A, const palial_opts *opts, palial_report *rep)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_algo(ALG_CHOL, A, nullptr, opts, rep);
+}
+
+int palial_hip_lu(MATRIX_desc A, int *ipiv, const palial_opts *opts, palial_report *rep)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = run_algo(ALG_LU, A, nullptr, opts, rep);
+    if (rc >= 0 && ipiv) {
+        // no row interchanges (src/srclu/dgetrf.h:20): identity, 1-based
+        const int N = A.matrix_size;
+        if (opts && opts->device_matrix) {
+            hipError_t e = launch_fill_identity_ipiv(ipiv, N, g.s_main);
+            if (e == hipSuccess) e = hipStreamSynchronize(g.s_main);
+            if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "ipiv: %s", hipGetErrorString(e));
+        } else {
+            for (int i = 0; i < N; i++) ipiv[i] = i + 1;
+        }
+    }
+    return rc;
+}
+
+int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_report *rep)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_algo(ALG_QR, A, &S, opts, rep);
+}
+
+// ---------------------------------------------------------------- tile kernels
+int palial_hip_dpotrf(int n, double *A, int lda, const palial_opts *opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_tile(n, lda, A))) return rc;
+    if (n == 0) return 0;
+    Stage st((size_t)n * n * 8, opts && opts->device_matrix);
+    double *d = st.in(A, true);
+    if (!d) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    hipMemsetAsync(g.d_info, 0, sizeof(int), g.s_main);
+    HIPCHK(launch_potrf(d, n, g.d_info, 0, g.s_main));
+    int info = 0;
+    hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, g.s_main);
+    if ((rc = st.finish())) return rc;
+    return info;
+}
+
+int palial_hip_dgetrf(int n, double *A, int lda, const palial_opts *opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_tile(n, lda, A))) return rc;
+    if (n == 0) return 0;
+    Stage st((size_t)n * n * 8, opts && opts->device_matrix);
+    double *d = st.in(A, true);
+    if (!d) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    HIPCHK(launch_getrf(d, n, g.s_main));
+    return st.finish();
+}
+
+int palial_hip_dgeqrt(int n, int nb, double *A, int lda, double *T, int ldt, double *tau, double *work,
+                      const palial_opts *opts)
+{
+    (void)work;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_tile(n, lda, A)) || (rc = check_tile(n, ldt, T))) return rc;
+    if (n == 0 || nb == 0) return 0;
+    if (nb != n) return fail(PALIAL_ERR_ARG, "inner blocking nb=%d: only nb = n (as every reference call) is supported", nb);
+    const bool dev = opts && opts->device_matrix;
+    Stage st((size_t)n * n * 8, dev);
+    double *dA = st.in(A, true), *dT = st.in(T, true);
+    double *dtau = dev ? tau : st.scratch((size_t)n * 8);
+    if (!dA || !dT || !dtau) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    HIPCHK(launch_geqrt(dA, dT, dtau, n, g.s_main));
+    if (!dev && tau) hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main);
+    return st.finish();
+}
+
+int palial_hip_dormqr(int side, int trans, int n, const double *V, int ldv, const double *T, int ldt, double *C,
+                      int ldc, double *work, int ldw, const palial_opts *opts)
+{
+    (void)work; (void)ldw;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_tile(n, ldv, V)) || (rc = check_tile(n, ldt, T)) ||
+        (rc = check_tile(n, ldc, C)))
+        return rc;
+    if (n == 0) return 0;
+    // src/srcqr/dormqr.h:18-29: only (Left, Trans) applies Q^T correctly there
+    if (!(side == 141 && trans != 111)) return fail(PALIAL_ERR_ARG, "only side=Left, trans=Trans is supported");
+    const bool dev = opts && opts->device_matrix;
+    Stage st((size_t)n * n * 8, dev);
+    double *dV = st.in(V, false), *dT = st.in(T, false), *dC = st.in(C, true);
+    double *w1 = st.scratch((size_t)n * n * 8), *w2 = st.scratch((size_t)n * n * 8);
+    if (!dV || !dT || !dC || !w1 || !w2) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    HIPCHK(launch_gemm(single(dV, dC, w1, nullptr, nullptr, n, 1.0, 0.0, MK_UNIT_LOWER), true, false, g.s_main));
+    HIPCHK(launch_gemm(single(dT, w1, w2, nullptr, nullptr, n, 1.0, 0.0, MK_UPPER), true, false, g.s_main));
+    HIPCHK(launch_gemm(single(dV, w2, dC, dC, nullptr, n, -1.0, 1.0, MK_UNIT_LOWER), false, false, g.s_main));
+    return st.finish();
+}
+
+int palial_hip_dtsqrt(int n, double *A1, int lda1, double *A2, int lda2, double *T, int ldt, double *tau,
+                      double *work, const palial_opts *opts)
+{
+    (void)work;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_tile(n, lda1, A1)) || (rc = check_tile(n, lda2, A2)) ||
+        (rc = check_tile(n, ldt, T)))
+        return rc;
+    if (n == 0) return 0;
+    const bool dev = opts && opts->device_matrix;
+    Stage st((size_t)n * n * 8, dev);
+    double *d1 = st.in(A1, true), *d2 = st.in(A2, true), *dT = st.in(T, true);
+    double *dtau = dev ? tau : st.scratch((size_t)n * 8);
+    if (!d1 || !d2 || !dT || !dtau) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    HIPCHK(launch_tsqrt(d1, d2, dT, dtau, n, g.s_main));
+    if (!dev && tau) hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main);
+    return st.finish();
+}
+
+int palial_hip_dtsmqr(int side, int trans, int M, int N, int K, int ib, double *A1, int lda1, double *A2,
+                      int lda2, const double *V, int ldv, const double *T, int ldt, double *work, int ldw,
+                      const palial_opts *opts)
+{
+    (void)work; (void)ldw;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if (M == 0 || N == 0 || K == 0 || ib == 0) return 0;
+    if (!(side == 141 && trans != 111)) return fail(PALIAL_ERR_ARG, "only side=Left, trans=Trans is supported");
+    if (!(M == N && N == K && K == ib)) return fail(PALIAL_ERR_ARG, "only M = N = K = ib (every reference call) is supported");
+    const int n = M;
+    if ((rc = check_tile(n, lda1, A1)) || (rc = check_tile(n, lda2, A2)) || (rc = check_tile(n, ldv, V)) ||
+        (rc = check_tile(n, ldt, T)))
+        return rc;
+    const bool dev = opts && opts->device_matrix;
+    Stage st((size_t)n * n * 8, dev);
+    double *d1 = st.in(A1, true), *d2 = st.in(A2, true), *dV = st.in(V, false), *dT = st.in(T, false);
+    double *w1 = st.scratch((size_t)n * n * 8), *w2 = st.scratch((size_t)n * n * 8);
+    if (!d1 || !d2 || !dV || !dT || !w1 || !w2) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    HIPCHK(launch_gemm(single(dV, d2, w1, d1, nullptr, n, 1.0, 1.0, MK_NONE), true, false, g.s_main));
+    HIPCHK(launch_gemm(single(dT, w1, w2, nullptr, d1, n, 1.0, 0.0, MK_UPPER), true, false, g.s_main));
+    HIPCHK(launch_gemm(single(dV, w2, d2, d2, nullptr, n, -1.0, 1.0, MK_NONE), false, false, g.s_main));
+    return st.finish();
+}
+
+int palial_hip_tile_task(int task, int n, const double *A, const double *B, double *C, const palial_opts *opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_tile(n, n, A)) || (rc = check_tile(n, n, C))) return rc;
+    const bool needB = (task == PALIAL_TASK_CHOL_GEMM || task == PALIAL_TASK_LU_GEMM);
+    if (needB && !B) return fail(PALIAL_ERR_ARG, "task %d needs B", task);
+    if (n == 0) return 0;
+    const bool dev = opts && opts->device_matrix;
+    Stage st((size_t)n * n * 8, dev);
+    double *dA = st.in(A, false), *dB = needB ? st.in(B, false) : nullptr, *dC = st.in(C, true);
+    if (!dA || !dC || (needB && !dB)) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    TrsmArgs t;
+    memset(&t, 0, sizeof t);
+    t.diag = dA; t.single = dC; t.N = n; t.b = n; t.MT = 1; t.ntiles = 1;
+    switch (task) {
+    case PALIAL_TASK_CHOL_TRSM: HIPCHK(launch_trsm(TR_CHOL, t, g.s_main)); break;
+    case PALIAL_TASK_LU_TRSM_R: HIPCHK(launch_trsm(TR_LUR, t, g.s_main)); break;
+    case PALIAL_TASK_LU_TRSM_L: HIPCHK(launch_trsm(TR_LUL, t, g.s_main)); break;
+    case PALIAL_TASK_CHOL_SYRK: {
+        GemmArgs a = single(dA, dA, dC, dC, nullptr, n, -1.0, 1.0, MK_NONE);
+        a.single_diag = 1;
+        HIPCHK(launch_gemm(a, true, false, g.s_main));
+        break;
+    }
+    case PALIAL_TASK_CHOL_GEMM:
+        HIPCHK(launch_gemm(single(dA, dB, dC, dC, nullptr, n, -1.0, 1.0, MK_NONE), true, false, g.s_main));
+        break;
+    case PALIAL_TASK_LU_GEMM:
+        HIPCHK(launch_gemm(single(dA, dB, dC, dC, nullptr, n, -1.0, 1.0, MK_NONE), false, false, g.s_main));
+        break;
+    default: return fail(PALIAL_ERR_ARG, "unknown tile task %d", task);
+    }
+    return st.finish();
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// internal.hpp -- shared declarations of libpalial_hip.so (host + device).
+//
+// Tile layout L (reference include/tile_address.h:15): tile (m,n) of an N x N
+// matrix with tile size b starts at element b*b*(m + (N/b)*n); inside a tile
+// element (r,c) is at r + c*b.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace palial {
+
+__host__ __device__ inline size_t tile_off(int b, int MT, int m, int n)
+{
+    return (size_t)b * (size_t)b * ((size_t)m + (size_t)MT * (size_t)n);
+}
+
+// ------------------------------------------------------------------ GEMM
+// Task enumerations of the batched tile GEMM kernel.
+enum GemmMode : int {
+    GM_SINGLE = 0,      // one task: C = alpha op(A) op(B) + beta D ; E -= alpha op(A) op(B)
+    GM_CHOL_TRAIL = 1,  // step k: C(n,m) -= A(k,n)^T A(k,m), k < n <= m < MT (diag: upper only)
+    GM_LU_TRAIL = 2,    // step k: C(m,n) -= A(m,k) A(k,n),  m, n > k
+    GM_QR_W = 3,        // TSMQR (k,m) over n>k: W_n = V^T A(m,n) + A(k,n), V = A(m,k)
+    GM_QR_TW = 4,       // W2_n = T^T W_n (T = S(m,k) upper);  A(k,n) -= W2_n
+    GM_QR_VW = 5,       // A(m,n) -= V W2_n
+    GM_ORM_W = 6,       // ORMQR k over n>k: W_n = V^T A(k,n), V = A(k,k) unit lower
+    GM_ORM_TW = 7,      // W2_n = T^T W_n (T = S(k,k) upper)
+    GM_ORM_VW = 8,      // A(k,n) -= V W2_n
+    GM_CHOL_ROW = 9,    // step k, only tiles (k+1, m), m >= k+1 (look-ahead part of GM_CHOL_TRAIL)
+    GM_CHOL_REST = 10,  // step k, tiles (n, m) with k+2 <= n <= m
+    GM_LU_PANEL = 11,   // step k, tiles (m, k+1) and (k+1, n) for m, n >= k+1 (look-ahead part)
+    GM_LU_REST = 12,    // step k, tiles (m, n) with m, n >= k+2
+};
+
+enum MaskA : int { MK_NONE = 0, MK_UPPER = 1, MK_UNIT_LOWER = 2 };
+
+struct GemmArgs {
+    int mode;            // GemmMode
+    // GM_SINGLE operands
+    const double *A;
+    const double *B;
+    double *C;
+    const double *D;
+    double *E;
+    // matrix modes
+    double *mat;         // layout-L matrix the tasks address
+    const double *smat;  // QR T-factor matrix (layout L)
+    double *w1;          // workspace tiles, indexed by task
+    double *w2;
+    int N, b, MT;
+    int k, m;            // step k; TS row m for the QR modes
+    int T;               // trailing tile count used by the enumeration
+    int ntasks;
+    double alpha, beta;
+    int maskA;
+    int single_diag;     // GM_SINGLE: output is a SYRK tile (upper triangle only)
+};
+
+// host launchers (k_gemm.hip)
+hipError_t launch_gemm(const GemmArgs &a, bool transA, bool transB, hipStream_t s);
+long gemm_task_count(int mode, int T);
+
+// ------------------------------------------------------------------ panels
+void panel_init();   // sets LDS limits of the panel kernels (call once, outside capture)
+hipError_t launch_potrf(double *tile, int b, int *info, int info_base, hipStream_t s);
+hipError_t launch_getrf(double *tile, int b, hipStream_t s);
+// TRSM variants: 0 = Cholesky (U^T X = B, columns), 1 = LU left unit lower
+// (L X = B, columns), 2 = LU right upper (X U = B, rows of the tile)
+enum TrsmKind : int { TR_CHOL = 0, TR_LUL = 1, TR_LUR = 2 };
+// Solves for tiles mat[tile_off(first + i*stride_m...)]; see k_panel.hip
+struct TrsmArgs {
+    const double *diag;  // the b x b diagonal tile
+    double *mat;         // layout-L matrix
+    double *single;      // single-tile target (ntiles == 1 && mat == nullptr)
+    int N, b, MT, k;
+    int first;           // first tile index along the panel (m or n)
+    int ntiles;
+};
+hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s);
+
+// QR panels
+hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s);
+hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s);
+
+// misc
+hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, double shift,
+                           hipStream_t s);
+hipError_t launch_fill_identity_ipiv(int *ipiv, int N, hipStream_t s);
+
+}  // namespace palial

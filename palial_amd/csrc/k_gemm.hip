@@ -1,0 +1,395 @@
+// k_gemm.hip -- batched FP64 tile GEMM on CDNA4 matrix cores.
+//
+// One launch performs every tile contraction of one DAG stage (e.g. all
+// GEMM+SYRK tasks of Cholesky step k, src/cholesky.h:107-173), instead of one
+// launch per tile task.  A workgroup owns a BM x BM block of one b x b output
+// tile; K (= b) is streamed through LDS in KB = 16 slices with register
+// prefetch; each wave accumulates a 64x64 (BM = 128) or 32x32 (BM = 32)
+// sub-block with v_mfma_f64_16x16x4_f64.
+//
+// MFMA 16x16x4 f64 operand layout (lane l of 64):
+//   A fragment: A[row = l % 16][k = l / 16]
+//   B fragment: B[k = l / 16][col = l % 16]
+//   D (4 doubles): D[row = 4*(l/16) + v][col = l % 16], v = 0..3
+#include "internal.hpp"
+
+namespace palial {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline double apply_mask(int mask, double v, int r, int c)
+{
+    // r, c: coordinates inside the stored tile
+    if (mask == MK_UPPER) return (r <= c) ? v : 0.0;
+    if (mask == MK_UNIT_LOWER) return (r > c) ? v : ((r == c) ? 1.0 : 0.0);
+    return v;
+}
+
+struct TaskPtrs {
+    const double *A, *B, *D;
+    double *C, *E;
+    bool diag;   // SYRK diagonal tile: only the upper triangle is written
+    bool valid;
+};
+
+__device__ inline void tri_decode(int idx, int &n, int &m)
+{
+    // idx = m(m+1)/2 + n with 0 <= n <= m
+    int mm = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while ((mm + 1) * (mm + 2) / 2 <= idx) mm++;
+    while (mm * (mm + 1) / 2 > idx) mm--;
+    m = mm;
+    n = idx - mm * (mm + 1) / 2;
+}
+
+__device__ inline TaskPtrs decode(const GemmArgs &p, int task)
+{
+    TaskPtrs t;
+    t.A = t.B = t.D = nullptr;
+    t.C = t.E = nullptr;
+    t.diag = false;
+    t.valid = true;
+    const int b = p.b, MT = p.MT, k = p.k;
+    const size_t bb = (size_t)b * b;
+    double *M = p.mat;
+    switch (p.mode) {
+    case GM_SINGLE:
+        t.A = p.A; t.B = p.B; t.C = p.C; t.D = p.D; t.E = p.E;
+        t.diag = p.single_diag != 0;
+        break;
+    case GM_CHOL_TRAIL:
+    case GM_CHOL_REST: {
+        int n, m;
+        tri_decode(task, n, m);
+        int off = (p.mode == GM_CHOL_TRAIL) ? k + 1 : k + 2;
+        n += off; m += off;
+        t.A = M + tile_off(b, MT, k, n);
+        t.B = M + tile_off(b, MT, k, m);
+        t.C = M + tile_off(b, MT, n, m);
+        t.D = t.C;
+        t.diag = (n == m);
+        break;
+    }
+    case GM_CHOL_ROW: {
+        int n = k + 1, m = k + 1 + task;
+        t.A = M + tile_off(b, MT, k, n);
+        t.B = M + tile_off(b, MT, k, m);
+        t.C = M + tile_off(b, MT, n, m);
+        t.D = t.C;
+        t.diag = (n == m);
+        break;
+    }
+    case GM_LU_TRAIL:
+    case GM_LU_REST: {
+        int off = (p.mode == GM_LU_TRAIL) ? k + 1 : k + 2;
+        int m = off + task % p.T, n = off + task / p.T;
+        t.A = M + tile_off(b, MT, m, k);
+        t.B = M + tile_off(b, MT, k, n);
+        t.C = M + tile_off(b, MT, m, n);
+        t.D = t.C;
+        break;
+    }
+    case GM_LU_PANEL: {
+        // tasks [0, T): column k+1, rows m = k+1..MT-1 ; [T, 2T-1): row k+1, cols n = k+2..
+        int m, n;
+        if (task < p.T) { m = k + 1 + task; n = k + 1; }
+        else { m = k + 1; n = k + 2 + (task - p.T); }
+        t.A = M + tile_off(b, MT, m, k);
+        t.B = M + tile_off(b, MT, k, n);
+        t.C = M + tile_off(b, MT, m, n);
+        t.D = t.C;
+        break;
+    }
+    case GM_QR_W: {
+        int n = k + 1 + task;
+        t.A = M + tile_off(b, MT, p.m, k);       // V (transposed)
+        t.B = M + tile_off(b, MT, p.m, n);       // A2
+        t.C = p.w1 + (size_t)task * bb;          // W
+        t.D = M + tile_off(b, MT, k, n);         // + A1
+        break;
+    }
+    case GM_QR_TW: {
+        int n = k + 1 + task;
+        t.A = p.smat + tile_off(b, MT, p.m, k);  // T (transposed, upper)
+        t.B = p.w1 + (size_t)task * bb;
+        t.C = p.w2 + (size_t)task * bb;
+        t.E = M + tile_off(b, MT, k, n);         // A1 -= T^T W
+        break;
+    }
+    case GM_QR_VW: {
+        int n = k + 1 + task;
+        t.A = M + tile_off(b, MT, p.m, k);       // V
+        t.B = p.w2 + (size_t)task * bb;
+        t.C = M + tile_off(b, MT, p.m, n);
+        t.D = t.C;
+        break;
+    }
+    case GM_ORM_W: {
+        int n = k + 1 + task;
+        t.A = M + tile_off(b, MT, k, k);         // V (transposed, unit lower)
+        t.B = M + tile_off(b, MT, k, n);
+        t.C = p.w1 + (size_t)task * bb;
+        break;
+    }
+    case GM_ORM_TW: {
+        t.A = p.smat + tile_off(b, MT, k, k);    // T (transposed, upper)
+        t.B = p.w1 + (size_t)task * bb;
+        t.C = p.w2 + (size_t)task * bb;
+        break;
+    }
+    case GM_ORM_VW: {
+        int n = k + 1 + task;
+        t.A = M + tile_off(b, MT, k, k);         // V (unit lower)
+        t.B = p.w2 + (size_t)task * bb;
+        t.C = M + tile_off(b, MT, k, n);
+        t.D = t.C;
+        break;
+    }
+    default:
+        t.valid = false;
+    }
+    return t;
+}
+
+// Workgroup index remap so that consecutive logical blocks run on the same
+// XCD (hardware dispatches workgroups round-robin over the 8 XCDs, each with
+// its own L2): blocks of one output tile then share that XCD's L2 copy of
+// their two operand tiles.
+__device__ inline int xcd_remap(int pid, int nwg)
+{
+    constexpr int NX = 8;
+    int q = nwg / NX, r = nwg % NX;
+    int xcd = pid % NX, local = pid / NX;
+    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+template <int BM, bool TA, bool TB>
+__global__ __launch_bounds__(2 * BM) void gemm_mfma_kernel(GemmArgs p)
+{
+    constexpr int NT = 2 * BM;          // threads per workgroup
+    constexpr int KB = 16;
+    constexpr int WM = (BM >= 64) ? 64 : BM;
+    constexpr int WAVES_M = BM / WM;
+    constexpr int FM = WM / 16;         // MFMA tiles per wave edge
+    constexpr int LD = BM + 16;         // LDS row stride (doubles): rows 128 B apart mod 256 B
+
+    __shared__ double sA[KB * LD];
+    __shared__ double sB[KB * LD];
+
+    const int b = p.b;
+    const int nbd = b / BM;
+    const int blk_per_task = nbd * nbd;
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int task = lid / blk_per_task;
+    const int blk = lid % blk_per_task;
+    const int bi = blk % nbd, bj = blk / nbd;
+    if (task >= p.ntasks) return;
+    TaskPtrs tp = decode(p, task);
+    if (!tp.valid) return;
+    if (tp.diag && bi > bj) return;     // strictly below the diagonal of a SYRK tile
+
+    const int i0 = bi * BM, j0 = bj * BM;
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int wm = w % WAVES_M, wn = w / WAVES_M;
+    const int mask = p.maskA;
+
+    // K range: triangular operands contribute only inside their triangle
+    int kbeg = 0, kend = b;
+    if (mask == MK_UPPER) {
+        if (TA) kend = min(b, i0 + BM);       // opA(i,k) = T(k,i), k <= i
+        else kbeg = i0;                       // opA(i,k) = T(i,k), i <= k
+    } else if (mask == MK_UNIT_LOWER) {
+        if (TA) kbeg = i0;                    // opA(i,k) = V(k,i), k >= i
+        else kend = min(b, i0 + BM);          // opA(i,k) = V(i,k), k <= i
+    }
+    kbeg = (kbeg / KB) * KB;
+
+    d4 acc[FM][FM];
+#pragma unroll
+    for (int a = 0; a < FM; a++)
+#pragma unroll
+        for (int c = 0; c < FM; c++) acc[a][c] = (d4){0.0, 0.0, 0.0, 0.0};
+
+    double ra[8], rb[8];
+
+    // ---- global -> registers
+    auto load_a = [&](int kb) {
+        if (TA) {   // opA(i,k) = X[k + i*b]: contiguous along k
+            const int row = t >> 1, kc = (t & 1) * 8;
+            const double2 *src = (const double2 *)(tp.A + (size_t)(i0 + row) * b + kb + kc);
+#pragma unroll
+            for (int q = 0; q < 4; q++) { double2 v = src[q]; ra[2 * q] = v.x; ra[2 * q + 1] = v.y; }
+            if (mask) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) ra[q] = apply_mask(mask, ra[q], kb + kc + q, i0 + row);
+            }
+        } else {    // opA(i,k) = X[i + k*b]: contiguous along i
+            const int kk = t / (BM / 8), ic = (t % (BM / 8)) * 8;
+            const double2 *src = (const double2 *)(tp.A + (size_t)(kb + kk) * b + i0 + ic);
+#pragma unroll
+            for (int q = 0; q < 4; q++) { double2 v = src[q]; ra[2 * q] = v.x; ra[2 * q + 1] = v.y; }
+            if (mask) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) ra[q] = apply_mask(mask, ra[q], i0 + ic + q, kb + kk);
+            }
+        }
+    };
+    auto load_b = [&](int kb) {
+        if (!TB) {  // opB(k,j) = Y[k + j*b]
+            const int col = t >> 1, kc = (t & 1) * 8;
+            const double2 *src = (const double2 *)(tp.B + (size_t)(j0 + col) * b + kb + kc);
+#pragma unroll
+            for (int q = 0; q < 4; q++) { double2 v = src[q]; rb[2 * q] = v.x; rb[2 * q + 1] = v.y; }
+        } else {    // opB(k,j) = Y[j + k*b]
+            const int kk = t / (BM / 8), jc = (t % (BM / 8)) * 8;
+            const double2 *src = (const double2 *)(tp.B + (size_t)(kb + kk) * b + j0 + jc);
+#pragma unroll
+            for (int q = 0; q < 4; q++) { double2 v = src[q]; rb[2 * q] = v.x; rb[2 * q + 1] = v.y; }
+        }
+    };
+    // ---- registers -> LDS ([k][mn] layout)
+    auto store_lds = [&]() {
+        if (TA) {
+            const int row = t >> 1, kc = (t & 1) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; q++) sA[(kc + q) * LD + row] = ra[q];
+        } else {
+            const int kk = t / (BM / 8), ic = (t % (BM / 8)) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; q++) sA[kk * LD + ic + q] = ra[q];
+        }
+        if (!TB) {
+            const int col = t >> 1, kc = (t & 1) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; q++) sB[(kc + q) * LD + col] = rb[q];
+        } else {
+            const int kk = t / (BM / 8), jc = (t % (BM / 8)) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; q++) sB[kk * LD + jc + q] = rb[q];
+        }
+    };
+
+    if (kbeg < kend) {
+        load_a(kbeg);
+        load_b(kbeg);
+        store_lds();
+        __syncthreads();
+        for (int kb = kbeg; kb < kend; kb += KB) {
+            const bool more = kb + KB < kend;
+            if (more) { load_a(kb + KB); load_b(kb + KB); }
+#pragma unroll
+            for (int kk = 0; kk < KB; kk += 4) {
+                double av[FM], bv[FM];
+                const int kr = kk + (lane >> 4);
+#pragma unroll
+                for (int a = 0; a < FM; a++) av[a] = sA[kr * LD + wm * WM + a * 16 + (lane & 15)];
+#pragma unroll
+                for (int c = 0; c < FM; c++) bv[c] = sB[kr * LD + wn * WM + c * 16 + (lane & 15)];
+#pragma unroll
+                for (int a = 0; a < FM; a++)
+#pragma unroll
+                    for (int c = 0; c < FM; c++)
+                        acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
+            }
+            __syncthreads();
+            if (more) { store_lds(); __syncthreads(); }
+        }
+    }
+
+    // ---- epilogue: C = alpha*acc + beta*D ; E -= alpha*acc
+    const double alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int a = 0; a < FM; a++) {
+#pragma unroll
+        for (int c = 0; c < FM; c++) {
+            const int j = j0 + wn * WM + c * 16 + (lane & 15);
+            const int ib = i0 + wm * WM + a * 16 + 4 * (lane >> 4);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = ib + v;
+                if (tp.diag && i > j) continue;
+                const size_t o = (size_t)i + (size_t)j * b;
+                const double r = alpha * acc[a][c][v];
+                double out = r;
+                if (tp.D) out += beta * tp.D[o];
+                tp.C[o] = out;
+                if (tp.E) tp.E[o] -= r;
+            }
+        }
+    }
+}
+
+// Scalar fallback for tile sizes that are not a multiple of 32 (tiny tests,
+// e.g. the reference's 4x4 matrix with b = 2).  One thread per output element.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_scalar_kernel(GemmArgs p)
+{
+    const int b = p.b;
+    const long per_task = (long)b * b;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int task = (int)(gid / per_task);
+    if (task >= p.ntasks) return;
+    const int e = (int)(gid % per_task);
+    const int i = e % b, j = e / b;
+    TaskPtrs tp = decode(p, task);
+    if (!tp.valid) return;
+    if (tp.diag && i > j) return;
+    double s = 0.0;
+    for (int k = 0; k < b; k++) {
+        double av = TA ? apply_mask(p.maskA, tp.A[k + (size_t)i * b], k, i)
+                       : apply_mask(p.maskA, tp.A[i + (size_t)k * b], i, k);
+        double bv = TB ? tp.B[j + (size_t)k * b] : tp.B[k + (size_t)j * b];
+        s += av * bv;
+    }
+    const size_t o = (size_t)i + (size_t)j * b;
+    const double r = p.alpha * s;
+    double out = r;
+    if (tp.D) out += p.beta * tp.D[o];
+    tp.C[o] = out;
+    if (tp.E) tp.E[o] -= r;
+}
+
+long gemm_task_count(int mode, int T)
+{
+    switch (mode) {
+    case GM_SINGLE: return 1;
+    case GM_CHOL_TRAIL: return (long)T * (T + 1) / 2;
+    case GM_CHOL_ROW: return T;
+    case GM_CHOL_REST: return (long)(T - 1) * T / 2;
+    case GM_LU_TRAIL: return (long)T * T;
+    case GM_LU_PANEL: return 2L * T - 1;
+    case GM_LU_REST: return (long)(T - 1) * (T - 1);
+    default: return T;  // QR / ORMQR modes: one task per trailing column tile
+    }
+}
+
+template <int BM>
+static hipError_t launch_mfma(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
+{
+    const long nbd = a.b / BM;
+    const long nblk = (long)a.ntasks * nbd * nbd;
+    if (nblk <= 0) return hipSuccess;
+    dim3 grid((unsigned)nblk), block(2 * BM);
+    if (ta && !tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, true, false>), grid, block, 0, s, a);
+    else if (!ta && !tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, false, false>), grid, block, 0, s, a);
+    else if (!ta && tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((gemm_mfma_kernel<BM, true, true>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
+{
+    if (a.ntasks <= 0) return hipSuccess;
+    if (a.b % 128 == 0) return launch_mfma<128>(a, ta, tb, s);
+    if (a.b % 32 == 0) return launch_mfma<32>(a, ta, tb, s);
+    const long total = (long)a.ntasks * a.b * a.b;
+    dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    if (ta && !tb) hipLaunchKernelGGL((gemm_scalar_kernel<true, false>), grid, block, 0, s, a);
+    else if (!ta && !tb) hipLaunchKernelGGL((gemm_scalar_kernel<false, false>), grid, block, 0, s, a);
+    else if (!ta && tb) hipLaunchKernelGGL((gemm_scalar_kernel<false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((gemm_scalar_kernel<true, true>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace palial

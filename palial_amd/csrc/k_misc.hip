@@ -1,0 +1,81 @@
+// k_misc.hip -- input generators and small helpers.
+//
+// The generators write layout L directly, walking the storage linearly so
+// that stores are coalesced; entry (i,j) of the global matrix depends only on
+// (seed, i, j), so host and device produce bit-identical matrices for any
+// tile size.  Rounding is pinned with explicit _rn intrinsics (no FMA
+// contraction), matching a plain C evaluation of the same expressions.
+#include "internal.hpp"
+
+namespace palial {
+
+__device__ inline uint64_t sm64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ inline double uniform(uint64_t seed, uint64_t i, uint64_t j)
+{
+    const uint64_t x = sm64(sm64(seed) + ((i << 32) | (j & 0xffffffffull)));
+    return __dadd_rn(__dmul_rn((double)(x >> 11), 0x1.0p-53), -0.5);
+}
+
+__global__ void generate_kernel(double *mat, int N, int b, int kind, uint64_t seed, double shift)
+{
+    const size_t total = (size_t)N * N;
+    const size_t bb = (size_t)b * b;
+    const int MT = N / b;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const size_t tile = e / bb, within = e % bb;
+        const int r = (int)(within % b), c = (int)(within / b);
+        const int m = (int)(tile % MT), n = (int)(tile / MT);
+        const uint64_t i = (uint64_t)m * b + r, j = (uint64_t)n * b + c;
+        double v;
+        switch (kind) {
+        case 0:
+            v = (i <= j) ? uniform(seed, i, j) : uniform(seed, j, i);
+            if (i == j) v = __dadd_rn(v, shift);
+            break;
+        case 1:
+            v = uniform(seed, i, j);
+            if (i == j) v = __dadd_rn(v, shift);
+            break;
+        case 2:
+            v = 0.5;
+            if (i == j) v = __dadd_rn(__dmul_rn(__dadd_rn(uniform(seed, i, i), 0.5), 173.0), 173.0);
+            break;
+        default:
+            v = (i == j) ? 1.0 : 0.0;
+        }
+        mat[e] = v;
+    }
+}
+
+hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, double shift, hipStream_t s)
+{
+    const size_t total = (size_t)N * N;
+    size_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mat, N, b, kind, seed, shift);
+    return hipGetLastError();
+}
+
+__global__ void iota_ipiv_kernel(int *ipiv, int N)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) ipiv[i] = i + 1;
+}
+
+hipError_t launch_fill_identity_ipiv(int *ipiv, int N, hipStream_t s)
+{
+    int blocks = (N + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(iota_ipiv_kernel, dim3(blocks), dim3(256), 0, s, ipiv, N);
+    return hipGetLastError();
+}
+
+}  // namespace palial
