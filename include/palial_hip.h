@@ -91,6 +91,18 @@ int palial_hip_malloc(void **dptr, size_t bytes);
 int palial_hip_free(void *dptr);
 int palial_hip_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int palial_hip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int palial_hip_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int palial_hip_synchronize(void);             /* hipDeviceSynchronize */
+
+/* Measurement hooks (no counterpart in the reference): per-kernel-class
+ * HIP-event timing of the following algorithm calls, which then launch
+ * stream-ordered instead of replaying a graph.  Classes:
+ *   0 trailing-update GEMM/SYRK, 1 diagonal-tile POTRF/GETRF/GEQRT,
+ *   2 panel TRSM, 3 TSQRT, 4 QR block-reflector GEMMs (ORMQR/TSMQR).
+ * flops / bytes are the algorithmic counts of the timed launches. */
+int palial_hip_profile(int enable);   /* also resets the counters */
+int palial_hip_profile_stats(int kclass, long *launches, double *total_ms, double *flops,
+                             double *bytes);
 
 /* ---------------------------------------------------------------- inputs */
 /* Seeded counter-based generators writing layout L (host or device per opts):

@@ -94,6 +94,11 @@ def lib():
         L.palial_hip_tile_task.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(Opts)]
         L.palial_hip_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.palial_hip_free.argtypes = [dp]
+        L.palial_hip_memcpy_h2d.argtypes = [dp, dp, C.c_size_t]
+        L.palial_hip_memcpy_d2h.argtypes = [dp, dp, C.c_size_t]
+        L.palial_hip_memcpy_d2d.argtypes = [dp, dp, C.c_size_t]
+        L.palial_hip_profile_stats.argtypes = [C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -277,3 +282,74 @@ def tile_task(task, A, B, Cm):
     init()
     _check(lib().palial_hip_tile_task(task, A.shape[0], A.ctypes.data, Bp, Cm.ctypes.data, None))
     return Cm
+
+
+# ------------------------------------------------------------------ device buffers
+class DeviceMatrix:
+    """An N x N layout-L matrix in device memory, owned by the library."""
+
+    def __init__(self, N, b):
+        init()
+        self.N, self.b = N, b
+        self.bytes = N * N * 8
+        p = C.c_void_p()
+        _check(lib().palial_hip_malloc(C.byref(p), self.bytes))
+        self.ptr = p.value
+
+    @property
+    def desc(self):
+        return MATRIX_desc(self.b, self.N * self.N, self.b * self.b, self.N, self.ptr)
+
+    def generate(self, kind, seed=0, shift=0.0):
+        o = _opts(device=True)
+        _check(lib().palial_hip_generate(self.desc, kind, seed, float(shift), C.byref(o)))
+
+    def copy_from(self, other):
+        _check(lib().palial_hip_memcpy_d2d(self.ptr, other.ptr, self.bytes))
+
+    def to_host(self):
+        out = np.empty(self.N * self.N)
+        _check(lib().palial_hip_memcpy_d2h(out.ctypes.data, self.ptr, self.bytes))
+        return out
+
+    def from_host(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(lib().palial_hip_memcpy_h2d(self.ptr, a.ctypes.data, self.bytes))
+
+    def free(self):
+        if self.ptr:
+            lib().palial_hip_free(self.ptr)
+            self.ptr = None
+
+
+def synchronize():
+    _check(lib().palial_hip_synchronize())
+
+
+def profile(enable):
+    init()
+    _check(lib().palial_hip_profile(int(bool(enable))))
+
+
+KCLASS = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4}
+
+
+def profile_stats(cls):
+    n = C.c_long(); ms = C.c_double(); fl = C.c_double(); by = C.c_double()
+    _check(lib().palial_hip_profile_stats(KCLASS.get(cls, cls), C.byref(n), C.byref(ms), C.byref(fl),
+                                          C.byref(by)))
+    return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
+
+def factor_device(algo, dm, S=None, **kw):
+    """Factor a DeviceMatrix in place (device-resident, no host copies)."""
+    o = _opts(device=True, **kw)
+    rep = Report()
+    if algo == "cholesky":
+        rc = lib().palial_hip_cholesky(dm.desc, C.byref(o), C.byref(rep))
+    elif algo == "lu":
+        rc = lib().palial_hip_lu(dm.desc, None, C.byref(o), C.byref(rep))
+    else:
+        rc = lib().palial_hip_qr(dm.desc, S.desc, C.byref(o), C.byref(rep))
+    _check(rc)
+    return rc, rep

@@ -133,16 +133,65 @@ int check_desc(const MATRIX_desc &A)
     return 0;
 }
 
+// ------------------------------------------------------------------ profiling
+// Per-kernel-class HIP-event timing (stream-ordered launches only): each
+// launch of a class is bracketed by two events on the stream it runs on.
+enum KClass : int { KC_GEMM = 0, KC_DIAG = 1, KC_TRSM = 2, KC_TSQRT = 3, KC_QRUPD = 4, KC_N = 5 };
+struct ProfRec { int cls; double flops; double bytes; hipEvent_t a, b; };
+struct Prof {
+    bool on = false;
+    std::vector<ProfRec> pending;
+    long launches[KC_N] = {0};
+    double ms[KC_N] = {0}, flops[KC_N] = {0}, bytes[KC_N] = {0};
+};
+Prof g_prof;
+
 // ------------------------------------------------------------------ DAGs
 struct Launcher {
     long launches = 0;
     hipError_t err = hipSuccess;
+    bool capturing = false;
     void chk(hipError_t e)
     {
         launches++;
         if (e != hipSuccess && err == hipSuccess) err = e;
     }
+    // run one kernel launch of class cls with its algorithmic flops / bytes
+    template <class F>
+    void run(int cls, double flops, double bytes, hipStream_t s, F &&f)
+    {
+        if (g_prof.on && !capturing) {
+            ProfRec r{cls, flops, bytes, nullptr, nullptr};
+            hipEventCreate(&r.a);
+            hipEventCreate(&r.b);
+            hipEventRecord(r.a, s);
+            chk(f());
+            hipEventRecord(r.b, s);
+            g_prof.pending.push_back(r);
+        } else {
+            chk(f());
+        }
+    }
 };
+
+void prof_collect()
+{
+    for (auto &r : g_prof.pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        g_prof.launches[r.cls]++;
+        g_prof.ms[r.cls] += ms;
+        g_prof.flops[r.cls] += r.flops;
+        g_prof.bytes[r.cls] += r.bytes;
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    g_prof.pending.clear();
+}
+
+// algorithmic work of the batched launches (leading terms, LAPACK counts)
+double b3(int b) { return (double)b * b * b; }
+double tile_bytes(int b) { return 8.0 * b * b; }
 
 GemmArgs gargs(int mode, double *mat, int N, int b, int k, int T)
 {
@@ -184,12 +233,30 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
                   Launcher &L)
 {
     const int MT = N / b;
+    auto potrf = [&](int k, hipStream_t s) {
+        L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s,
+              [&] { return launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s); });
+    };
+    auto trsm = [&](int k, hipStream_t s) {
+        const int T = MT - k - 1;
+        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s,
+              [&] { return launch_trsm(TR_CHOL, targs(dA, N, b, k), s); });
+    };
+    auto update = [&](int mode, int k, hipStream_t s) {
+        const int T = MT - k - 1;
+        GemmArgs a = gargs(mode, dA, N, b, k, T);
+        long ngemm, nsyrk;
+        if (mode == GM_CHOL_TRAIL) { nsyrk = T; ngemm = (long)T * (T - 1) / 2; }
+        else if (mode == GM_CHOL_ROW) { nsyrk = 1; ngemm = T - 1; }
+        else { nsyrk = T - 1; ngemm = (long)(T - 1) * (T - 2) / 2; }
+        L.run(KC_GEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk, 3.0 * (ngemm + nsyrk) * tile_bytes(b), s,
+              [&] { return launch_gemm(a, true, false, s); });
+    };
     if (!lookahead) {
         for (int k = 0; k < MT; k++) {
-            L.chk(launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s0));
-            L.chk(launch_trsm(TR_CHOL, targs(dA, N, b, k), s0));
-            const int T = MT - k - 1;
-            if (T > 0) L.chk(launch_gemm(gargs(GM_CHOL_TRAIL, dA, N, b, k, T), true, false, s0));
+            potrf(k, s0);
+            trsm(k, s0);
+            if (MT - k - 1 > 0) update(GM_CHOL_TRAIL, k, s0);
         }
         return;
     }
@@ -198,20 +265,20 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
     hipEvent_t fork = g.events[2 * MT + 2];
     L.chk(hipEventRecord(fork, s0));
     L.chk(hipStreamWaitEvent(s1, fork, 0));
-    L.chk(launch_potrf(dA + tile_off(b, MT, 0, 0), b, dinfo, 0, s1));
-    L.chk(launch_trsm(TR_CHOL, targs(dA, N, b, 0), s1));
+    potrf(0, s1);
+    trsm(0, s1);
     L.chk(hipEventRecord(evP[0], s1));
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
         if (T > 0) {
             if (k > 0) L.chk(hipStreamWaitEvent(s1, evU[k - 1], 0));
-            L.chk(launch_gemm(gargs(GM_CHOL_ROW, dA, N, b, k, T), true, false, s1));
-            L.chk(launch_potrf(dA + tile_off(b, MT, k + 1, k + 1), b, dinfo, (k + 1) * b, s1));
-            L.chk(launch_trsm(TR_CHOL, targs(dA, N, b, k + 1), s1));
+            update(GM_CHOL_ROW, k, s1);
+            potrf(k + 1, s1);
+            trsm(k + 1, s1);
             L.chk(hipEventRecord(evP[k + 1], s1));
         }
         L.chk(hipStreamWaitEvent(s0, evP[k], 0));
-        if (T > 1) L.chk(launch_gemm(gargs(GM_CHOL_REST, dA, N, b, k, T), true, false, s0));
+        if (T > 1) update(GM_CHOL_REST, k, s0);
         L.chk(hipEventRecord(evU[k], s0));
     }
 }
@@ -221,15 +288,25 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
 {
     const int MT = N / b;
     auto panel = [&](int k, hipStream_t s) {
-        L.chk(launch_getrf(dA + tile_off(b, MT, k, k), b, s));
-        L.chk(launch_trsm(TR_LUR, targs(dA, N, b, k), s));
-        L.chk(launch_trsm(TR_LUL, targs(dA, N, b, k), s));
+        const int T = MT - k - 1;
+        L.run(KC_DIAG, 2 * b3(b) / 3, 2 * tile_bytes(b), s,
+              [&] { return launch_getrf(dA + tile_off(b, MT, k, k), b, s); });
+        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s,
+              [&] { return launch_trsm(TR_LUR, targs(dA, N, b, k), s); });
+        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s,
+              [&] { return launch_trsm(TR_LUL, targs(dA, N, b, k), s); });
+    };
+    auto update = [&](int mode, int k, hipStream_t s) {
+        const int T = MT - k - 1;
+        GemmArgs a = gargs(mode, dA, N, b, k, T);
+        const double ntile = (double)a.ntasks;
+        L.run(KC_GEMM, 2 * b3(b) * ntile, 3.0 * ntile * tile_bytes(b), s,
+              [&] { return launch_gemm(a, false, false, s); });
     };
     if (!lookahead) {
         for (int k = 0; k < MT; k++) {
             panel(k, s0);
-            const int T = MT - k - 1;
-            if (T > 0) L.chk(launch_gemm(gargs(GM_LU_TRAIL, dA, N, b, k, T), false, false, s0));
+            if (MT - k - 1 > 0) update(GM_LU_TRAIL, k, s0);
         }
         return;
     }
@@ -244,12 +321,12 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
         const int T = MT - k - 1;
         if (T > 0) {
             if (k > 0) L.chk(hipStreamWaitEvent(s1, evU[k - 1], 0));
-            L.chk(launch_gemm(gargs(GM_LU_PANEL, dA, N, b, k, T), false, false, s1));
+            update(GM_LU_PANEL, k, s1);
             panel(k + 1, s1);
             L.chk(hipEventRecord(evP[k + 1], s1));
         }
         L.chk(hipStreamWaitEvent(s0, evP[k], 0));
-        if (T > 1) L.chk(launch_gemm(gargs(GM_LU_REST, dA, N, b, k, T), false, false, s0));
+        if (T > 1) update(GM_LU_REST, k, s0);
         L.chk(hipEventRecord(evU[k], s0));
     }
 }
@@ -260,31 +337,34 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, Launcher &L)
     const int MT = N / b;
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
-        L.chk(launch_geqrt(dA + tile_off(b, MT, k, k), dS + tile_off(b, MT, k, k), g.d_tau, b, s0));
+        L.run(KC_DIAG, 4 * b3(b) / 3, 3 * tile_bytes(b), s0,
+              [&] { return launch_geqrt(dA + tile_off(b, MT, k, k), dS + tile_off(b, MT, k, k), g.d_tau, b, s0); });
         if (T > 0) {
             // PALIAL_dormqr on A(k,n), n > k: C -= V T^T V^T C
             GemmArgs a = gargs(GM_ORM_W, dA, N, b, k, T);
             a.smat = dS; a.w1 = g.d_w1; a.w2 = g.d_w2;
             a.alpha = 1.0; a.beta = 0.0; a.maskA = MK_UNIT_LOWER;
-            L.chk(launch_gemm(a, true, false, s0));
+            L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
             a.mode = GM_ORM_TW; a.maskA = MK_UPPER;
-            L.chk(launch_gemm(a, true, false, s0));
+            L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
             a.mode = GM_ORM_VW; a.maskA = MK_UNIT_LOWER; a.alpha = -1.0; a.beta = 1.0;
-            L.chk(launch_gemm(a, false, false, s0));
+            L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, false, false, s0); });
         }
         for (int m = k + 1; m < MT; m++) {
-            L.chk(launch_tsqrt(dA + tile_off(b, MT, k, k), dA + tile_off(b, MT, m, k),
-                               dS + tile_off(b, MT, m, k), g.d_tau, b, s0));
+            L.run(KC_TSQRT, 2 * b3(b), 5 * tile_bytes(b), s0, [&] {
+                return launch_tsqrt(dA + tile_off(b, MT, k, k), dA + tile_off(b, MT, m, k),
+                                    dS + tile_off(b, MT, m, k), g.d_tau, b, s0);
+            });
             if (T > 0) {
                 // PALIAL_dtsmqr on (A(k,n), A(m,n)), n > k
                 GemmArgs a = gargs(GM_QR_W, dA, N, b, k, T);
                 a.m = m; a.smat = dS; a.w1 = g.d_w1; a.w2 = g.d_w2;
                 a.alpha = 1.0; a.beta = 1.0;
-                L.chk(launch_gemm(a, true, false, s0));
+                L.run(KC_QRUPD, 2 * T * b3(b), 4.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
                 a.mode = GM_QR_TW; a.maskA = MK_UPPER; a.beta = 0.0;
-                L.chk(launch_gemm(a, true, false, s0));
+                L.run(KC_QRUPD, T * b3(b), 4.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
                 a.mode = GM_QR_VW; a.maskA = MK_NONE; a.alpha = -1.0; a.beta = 1.0;
-                L.chk(launch_gemm(a, false, false, s0));
+                L.run(KC_QRUPD, 2 * T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, false, false, s0); });
             }
         }
     }
@@ -307,7 +387,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     hipStream_t s0 = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
     hipStream_t s1 = g.s_panel;
     const bool lookahead = !o.no_lookahead && MT > 1 && algo != ALG_QR;
-    const bool use_graph = !o.no_graph;
+    const bool use_graph = !o.no_graph && !g_prof.on;
     if ((rc = ensure_events(2 * (size_t)MT + 4))) return rc;
     if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
 
@@ -354,6 +434,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
         if (it == g.graphs.end() || it->second.exec == nullptr) {
             Graph G;
             hipStream_t cs = g.s_main;   // capture on the library stream
+            L.capturing = true;
             hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "begin capture: %s", hipGetErrorString(e)); }
             issue(cs);
@@ -401,6 +482,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     hipEventElapsedTime(&h2d, e0, e1);
     hipEventElapsedTime(&fac, e1, e2);
     hipEventElapsedTime(&d2h, e2, e3);
+    prof_collect();
     if (rep) {
         const double n = N;
         const double flops = (algo == ALG_CHOL) ? n * n * n / 3 : (algo == ALG_LU) ? 2 * n * n * n / 3 : 4 * n * n * n / 3;
@@ -539,6 +621,25 @@ int palial_hip_finalize(void)
     return 0;
 }
 
+int palial_hip_profile(int enable)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof.on = enable != 0;
+    for (int c = 0; c < KC_N; c++) { g_prof.launches[c] = 0; g_prof.ms[c] = g_prof.flops[c] = g_prof.bytes[c] = 0; }
+    return 0;
+}
+
+int palial_hip_profile_stats(int cls, long *launches, double *total_ms, double *flops, double *bytes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (cls < 0 || cls >= KC_N) return fail(PALIAL_ERR_ARG, "kernel class %d out of range", cls);
+    if (launches) *launches = g_prof.launches[cls];
+    if (total_ms) *total_ms = g_prof.ms[cls];
+    if (flops) *flops = g_prof.flops[cls];
+    if (bytes) *bytes = g_prof.bytes[cls];
+    return 0;
+}
+
 int palial_hip_malloc(void **p, size_t bytes)
 {
     if (!p) return fail(PALIAL_ERR_ARG, "NULL out pointer");
@@ -564,6 +665,16 @@ int palial_hip_free(void *p)
 int palial_hip_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int palial_hip_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return 0;
+}
+int palial_hip_synchronize(void)
+{
+    HIPCHK(hipDeviceSynchronize());
     return 0;
 }
 int palial_hip_memcpy_d2h(void *dst, const void *src, size_t bytes)

@@ -10,7 +10,8 @@
 // MFMA 16x16x4 f64 operand layout (lane l of 64):
 //   A fragment: A[row = l % 16][k = l / 16]
 //   B fragment: B[k = l / 16][col = l % 16]
-//   D (4 doubles): D[row = 4*(l/16) + v][col = l % 16], v = 0..3
+//   D (4 doubles): D[row = (l/16) + 4*v][col = l % 16], v = 0..3
+//   (the f64 variant interleaves rows across the four lane groups)
 #include "internal.hpp"
 
 namespace palial {
@@ -304,10 +305,10 @@ __global__ __launch_bounds__(2 * BM) void gemm_mfma_kernel(GemmArgs p)
 #pragma unroll
         for (int c = 0; c < FM; c++) {
             const int j = j0 + wn * WM + c * 16 + (lane & 15);
-            const int ib = i0 + wm * WM + a * 16 + 4 * (lane >> 4);
+            const int ib = i0 + wm * WM + a * 16 + (lane >> 4);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int i = ib + v;
+                const int i = ib + 4 * v;
                 if (tp.diag && i > j) continue;
                 const size_t o = (size_t)i + (size_t)j * b;
                 const double r = alpha * acc[a][c][v];

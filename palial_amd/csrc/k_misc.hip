@@ -7,6 +7,9 @@
 // contraction), matching a plain C evaluation of the same expressions.
 #include "internal.hpp"
 
+// bit-exact with a plain C evaluation: no multiply-add contraction here
+#pragma clang fp contract(off)
+
 namespace palial {
 
 __device__ inline uint64_t sm64(uint64_t z)
