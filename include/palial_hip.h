@@ -104,6 +104,14 @@ int palial_hip_profile(int enable);   /* also resets the counters */
 int palial_hip_profile_stats(int kclass, long *launches, double *total_ms, double *flops,
                              double *bytes);
 
+/* Test hook (no counterpart in the reference): enumerate the tile tasks of
+ * one batched trailing-update launch (mode = internal GEMM mode, step k, TS
+ * row m for the QR modes) without touching the GPU.  Writes 3 longs per task:
+ * element offsets of the A, B and output tiles from the matrix base (-1 none,
+ * -2 a workspace or T-factor tile).  Returns the task count, -1 if cap is too
+ * small. */
+int palial_hip_task_tiles(int mode, int MT, int b, int k, int m, long *out, int cap);
+
 /* ---------------------------------------------------------------- inputs */
 /* Seeded counter-based generators writing layout L (host or device per opts):
  *   kind 0: S + shift*I, S symmetric, S_ij = U(seed, min(i,j), max(i,j))

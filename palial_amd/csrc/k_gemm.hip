@@ -13,6 +13,7 @@
 //   D (4 doubles): D[row = (l/16) + 4*v][col = l % 16], v = 0..3
 //   (the f64 variant interleaves rows across the four lane groups)
 #include "internal.hpp"
+#include <cstring>
 
 namespace palial {
 
@@ -33,7 +34,7 @@ struct TaskPtrs {
     bool valid;
 };
 
-__device__ inline void tri_decode(int idx, int &n, int &m)
+__host__ __device__ inline void tri_decode(int idx, int &n, int &m)
 {
     // idx = m(m+1)/2 + n with 0 <= n <= m
     int mm = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
@@ -43,7 +44,7 @@ __device__ inline void tri_decode(int idx, int &n, int &m)
     n = idx - mm * (mm + 1) / 2;
 }
 
-__device__ inline TaskPtrs decode(const GemmArgs &p, int task)
+__host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
 {
     TaskPtrs t;
     t.A = t.B = t.D = nullptr;
@@ -82,8 +83,10 @@ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
     }
     case GM_LU_TRAIL:
     case GM_LU_REST: {
-        int off = (p.mode == GM_LU_TRAIL) ? k + 1 : k + 2;
-        int m = off + task % p.T, n = off + task / p.T;
+        // GM_LU_TRAIL covers T x T tiles from k+1, GM_LU_REST (T-1) x (T-1) from k+2
+        const int off = (p.mode == GM_LU_TRAIL) ? k + 1 : k + 2;
+        const int rows = (p.mode == GM_LU_TRAIL) ? p.T : p.T - 1;
+        int m = off + task % rows, n = off + task / rows;
         t.A = M + tile_off(b, MT, m, k);
         t.B = M + tile_off(b, MT, k, n);
         t.C = M + tile_off(b, MT, m, n);
@@ -350,6 +353,49 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(GemmArgs p)
     tp.C[o] = out;
     if (tp.E) tp.E[o] -= r;
 }
+
+}  // namespace palial
+
+// Host-side view of the task enumeration (test hook): for each task of a
+// batched launch, the element offsets of its A, B and output tiles relative to
+// the matrix base (or -1 when the operand is not a matrix tile).  Lets the CPU
+// tests prove every launch stays inside the N x N buffer.
+extern "C" int palial_hip_task_tiles(int mode, int MT, int b, int k, int m, long *out, int cap)
+{
+    using namespace palial;
+    GemmArgs a;
+    memset(&a, 0, sizeof a);
+    a.mode = mode;
+    a.b = b;
+    a.MT = MT;
+    a.N = MT * b;
+    a.k = k;
+    a.m = m;
+    a.T = MT - k - 1;
+    a.ntasks = (int)gemm_task_count(mode, a.T);
+    // a fake base far from zero so offsets are recoverable
+    const size_t base = (size_t)1 << 40;
+    a.mat = (double *)(base * 8);
+    a.smat = (const double *)(base * 8 * 4);
+    a.w1 = (double *)(base * 8 * 8);
+    a.w2 = (double *)(base * 8 * 12);
+    int n = 0;
+    auto rel = [&](const double *p) -> long {
+        if (!p) return -1;
+        const long d = (long)((const double *)p - (const double *)a.mat);
+        return (d >= 0 && d < (long)base) ? d : -2;   // -2: a workspace / S tile
+    };
+    for (int t = 0; t < a.ntasks; t++) {
+        TaskPtrs tp = decode(a, t);
+        if (n + 3 > cap) return -1;
+        out[n++] = rel(tp.A);
+        out[n++] = rel(tp.B);
+        out[n++] = rel(tp.C);
+    }
+    return a.ntasks;
+}
+
+namespace palial {
 
 long gemm_task_count(int mode, int T)
 {

@@ -3,12 +3,11 @@
 // The generators write layout L directly, walking the storage linearly so
 // that stores are coalesced; entry (i,j) of the global matrix depends only on
 // (seed, i, j), so host and device produce bit-identical matrices for any
-// tile size.  Rounding is pinned with explicit _rn intrinsics (no FMA
-// contraction), matching a plain C evaluation of the same expressions.
+// tile size.
 #include "internal.hpp"
 
-// bit-exact with a plain C evaluation: no multiply-add contraction here
-#pragma clang fp contract(off)
+// Built with -ffp-contract=off (see palial_amd/build.py) so that every
+// multiply and add rounds separately, as in a plain C evaluation.
 
 namespace palial {
 
@@ -23,7 +22,7 @@ __device__ inline uint64_t sm64(uint64_t z)
 __device__ inline double uniform(uint64_t seed, uint64_t i, uint64_t j)
 {
     const uint64_t x = sm64(sm64(seed) + ((i << 32) | (j & 0xffffffffull)));
-    return __dadd_rn(__dmul_rn((double)(x >> 11), 0x1.0p-53), -0.5);
+    return (double)(x >> 11) * 0x1.0p-53 - 0.5;
 }
 
 __global__ void generate_kernel(double *mat, int N, int b, int kind, uint64_t seed, double shift)
@@ -41,15 +40,15 @@ __global__ void generate_kernel(double *mat, int N, int b, int kind, uint64_t se
         switch (kind) {
         case 0:
             v = (i <= j) ? uniform(seed, i, j) : uniform(seed, j, i);
-            if (i == j) v = __dadd_rn(v, shift);
+            if (i == j) v = v + shift;
             break;
         case 1:
             v = uniform(seed, i, j);
-            if (i == j) v = __dadd_rn(v, shift);
+            if (i == j) v = v + shift;
             break;
         case 2:
             v = 0.5;
-            if (i == j) v = __dadd_rn(__dmul_rn(__dadd_rn(uniform(seed, i, i), 0.5), 173.0), 173.0);
+            if (i == j) v = (uniform(seed, i, i) + 0.5) * 173.0 + 173.0;
             break;
         default:
             v = (i == j) ? 1.0 : 0.0;
