@@ -93,6 +93,7 @@ int palial_hip_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int palial_hip_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int palial_hip_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int palial_hip_synchronize(void);             /* hipDeviceSynchronize */
+int palial_hip_copy(void *dst, const void *src, size_t bytes);  /* device copy by the library's own streaming kernel */
 
 /* Measurement hooks (no counterpart in the reference): per-kernel-class
  * HIP-event timing of the following algorithm calls, which then launch

@@ -97,6 +97,7 @@ def lib():
         L.palial_hip_memcpy_h2d.argtypes = [dp, dp, C.c_size_t]
         L.palial_hip_memcpy_d2h.argtypes = [dp, dp, C.c_size_t]
         L.palial_hip_memcpy_d2d.argtypes = [dp, dp, C.c_size_t]
+        L.palial_hip_copy.argtypes = [dp, dp, C.c_size_t]
         L.palial_hip_profile_stats.argtypes = [C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
@@ -305,7 +306,7 @@ class DeviceMatrix:
         _check(lib().palial_hip_generate(self.desc, kind, seed, float(shift), C.byref(o)))
 
     def copy_from(self, other):
-        _check(lib().palial_hip_memcpy_d2d(self.ptr, other.ptr, self.bytes))
+        _check(lib().palial_hip_copy(self.ptr, other.ptr, self.bytes))
 
     def to_host(self):
         out = np.empty(self.N * self.N)

@@ -672,6 +672,15 @@ int palial_hip_memcpy_d2d(void *dst, const void *src, size_t bytes)
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return 0;
 }
+int palial_hip_copy(void *dst, const void *src, size_t bytes)
+{
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if (bytes % 16) return fail(PALIAL_ERR_ARG, "copy size must be a multiple of 16 bytes");
+    HIPCHK(launch_copy(dst, src, bytes, g.s_main));
+    HIPCHK(hipStreamSynchronize(g.s_main));
+    return 0;
+}
 int palial_hip_synchronize(void)
 {
     HIPCHK(hipDeviceSynchronize());

@@ -88,5 +88,6 @@ hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, h
 hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, double shift,
                            hipStream_t s);
 hipError_t launch_fill_identity_ipiv(int *ipiv, int N, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
 }  // namespace palial

@@ -66,6 +66,21 @@ hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, d
     return hipGetLastError();
 }
 
+// Plain streaming copy (16-byte vectors): the reference point for the HBM
+// byte counters (its algorithmic traffic is exactly 2 x bytes).
+__global__ void copy_kernel(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n2)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    const size_t n2 = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, s, (double2 *)dst, (const double2 *)src, n2);
+    return hipGetLastError();
+}
+
 __global__ void iota_ipiv_kernel(int *ipiv, int N)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) ipiv[i] = i + 1;
