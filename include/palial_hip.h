@@ -113,6 +113,11 @@ int palial_hip_profile_stats(int kclass, long *launches, double *total_ms, doubl
  * small. */
 int palial_hip_task_tiles(int mode, int MT, int b, int k, int m, long *out, int cap);
 
+/* Test hook (no counterpart in the reference): average device time of one
+ * batched trailing-update launch of step k (algo 0 Cholesky, 1 LU) over reps
+ * launches on a device-resident matrix (its contents are overwritten). */
+int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms);
+
 /* ---------------------------------------------------------------- inputs */
 /* Seeded counter-based generators writing layout L (host or device per opts):
  *   kind 0: S + shift*I, S symmetric, S_ij = U(seed, min(i,j), max(i,j))

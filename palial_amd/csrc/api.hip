@@ -42,6 +42,8 @@ struct Ctx {
     size_t tau_cap = 0;
     double *d_w1 = nullptr, *d_w2 = nullptr;  // QR/ORMQR workspace tiles
     size_t w_cap = 0;
+    double *d_dinv = nullptr;       // inverses of the 32x32 diagonal blocks (U then L)
+    size_t dinv_cap = 0;
     std::map<std::tuple<int, void *, void *, int, int, int>, Graph> graphs;
 };
 
@@ -117,6 +119,19 @@ int ensure_ws(int b, int MT)
         if (g.d_tau) hipFree(g.d_tau);
         if (hipMalloc(&g.d_tau, (size_t)b * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "tau alloc failed");
         g.tau_cap = b;
+    }
+    return 0;
+}
+
+int ensure_dinv(int b)
+{
+    const size_t need = 2 * (size_t)32 * b;
+    if (g.dinv_cap < need) {
+        drop_graphs();
+        if (g.d_dinv) hipFree(g.d_dinv);
+        g.d_dinv = nullptr;
+        if (hipMalloc(&g.d_dinv, need * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "dinv alloc failed");
+        g.dinv_cap = need;
     }
     return 0;
 }
@@ -233,14 +248,19 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
                   Launcher &L)
 {
     const int MT = N / b;
+    const bool v2 = panel2_ok(b);
     auto potrf = [&](int k, hipStream_t s) {
-        L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s,
-              [&] { return launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s); });
+        L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s, [&] {
+            return v2 ? launch_potrf2(dA + tile_off(b, MT, k, k), b, dinfo, k * b, g.d_dinv, s)
+                      : launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s);
+        });
     };
     auto trsm = [&](int k, hipStream_t s) {
         const int T = MT - k - 1;
-        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s,
-              [&] { return launch_trsm(TR_CHOL, targs(dA, N, b, k), s); });
+        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] {
+            return v2 ? launch_trsm2(TR_CHOL, targs(dA, N, b, k), g.d_dinv, s)
+                      : launch_trsm(TR_CHOL, targs(dA, N, b, k), s);
+        });
     };
     auto update = [&](int mode, int k, hipStream_t s) {
         const int T = MT - k - 1;
@@ -287,14 +307,20 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
 void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, Launcher &L)
 {
     const int MT = N / b;
+    const bool v2 = panel2_ok(b);
+    double *dU = g.d_dinv, *dL = g.d_dinv + (size_t)32 * b;
     auto panel = [&](int k, hipStream_t s) {
         const int T = MT - k - 1;
-        L.run(KC_DIAG, 2 * b3(b) / 3, 2 * tile_bytes(b), s,
-              [&] { return launch_getrf(dA + tile_off(b, MT, k, k), b, s); });
-        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s,
-              [&] { return launch_trsm(TR_LUR, targs(dA, N, b, k), s); });
-        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s,
-              [&] { return launch_trsm(TR_LUL, targs(dA, N, b, k), s); });
+        L.run(KC_DIAG, 2 * b3(b) / 3, 2 * tile_bytes(b), s, [&] {
+            return v2 ? launch_getrf2(dA + tile_off(b, MT, k, k), b, dU, dL, s)
+                      : launch_getrf(dA + tile_off(b, MT, k, k), b, s);
+        });
+        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] {
+            return v2 ? launch_trsm2(TR_LUR, targs(dA, N, b, k), dU, s) : launch_trsm(TR_LUR, targs(dA, N, b, k), s);
+        });
+        L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] {
+            return v2 ? launch_trsm2(TR_LUL, targs(dA, N, b, k), dL, s) : launch_trsm(TR_LUL, targs(dA, N, b, k), s);
+        });
     };
     auto update = [&](int mode, int k, hipStream_t s) {
         const int T = MT - k - 1;
@@ -390,6 +416,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     const bool use_graph = !o.no_graph && !g_prof.on;
     if ((rc = ensure_events(2 * (size_t)MT + 4))) return rc;
     if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
+    if ((rc = ensure_dinv(b))) return rc;
 
     double *dA = nullptr, *dS = nullptr;
     bool owned = false;
@@ -598,6 +625,11 @@ int palial_hip_init(int device)
     HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
     panel_init();
+    panel2_init();
+    {
+        hipError_t e = hipGetLastError();   // attribute calls must not leave a sticky error
+        if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "kernel attribute setup: %s", hipGetErrorString(e));
+    }
     g.device = device;
     g.init = true;
     return 0;
@@ -614,6 +646,7 @@ int palial_hip_finalize(void)
     if (g.d_w1) hipFree(g.d_w1);
     if (g.d_w2) hipFree(g.d_w2);
     if (g.d_tau) hipFree(g.d_tau);
+    if (g.d_dinv) hipFree(g.d_dinv);
     if (g.d_info) hipFree(g.d_info);
     hipStreamDestroy(g.s_main);
     hipStreamDestroy(g.s_panel);
@@ -637,6 +670,30 @@ int palial_hip_profile_stats(int cls, long *launches, double *total_ms, double *
     if (total_ms) *total_ms = g_prof.ms[cls];
     if (flops) *flops = g_prof.flops[cls];
     if (bytes) *bytes = g_prof.bytes[cls];
+    return 0;
+}
+
+int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_desc(A))) return rc;
+    const int N = A.matrix_size, b = A.tile_size, MT = N / b, T = MT - k - 1;
+    if (k < 0 || T <= 0 || reps <= 0) return fail(PALIAL_ERR_ARG, "bad step %d or reps %d", k, reps);
+    GemmArgs a = gargs(algo == ALG_CHOL ? GM_CHOL_TRAIL : GM_LU_TRAIL, (double *)A.matrix, N, b, k, T);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(launch_gemm(a, algo == ALG_CHOL, false, g.s_main));   // warm
+    HIPCHK(hipEventRecord(e0, g.s_main));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_gemm(a, algo == ALG_CHOL, false, g.s_main));
+    HIPCHK(hipEventRecord(e1, g.s_main));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ms) *ms = t / reps;
     return 0;
 }
 
@@ -756,7 +813,9 @@ int palial_hip_dpotrf(int n, double *A, int lda, const palial_opts *opts)
     double *d = st.in(A, true);
     if (!d) return fail(PALIAL_ERR_NOMEM, "tile alloc");
     hipMemsetAsync(g.d_info, 0, sizeof(int), g.s_main);
-    HIPCHK(launch_potrf(d, n, g.d_info, 0, g.s_main));
+    if ((rc = ensure_dinv(n))) return rc;
+    if (panel2_ok(n)) HIPCHK(launch_potrf2(d, n, g.d_info, 0, g.d_dinv, g.s_main));
+    else HIPCHK(launch_potrf(d, n, g.d_info, 0, g.s_main));
     int info = 0;
     hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, g.s_main);
     if ((rc = st.finish())) return rc;
@@ -772,7 +831,9 @@ int palial_hip_dgetrf(int n, double *A, int lda, const palial_opts *opts)
     Stage st((size_t)n * n * 8, opts && opts->device_matrix);
     double *d = st.in(A, true);
     if (!d) return fail(PALIAL_ERR_NOMEM, "tile alloc");
-    HIPCHK(launch_getrf(d, n, g.s_main));
+    if ((rc = ensure_dinv(n))) return rc;
+    if (panel2_ok(n)) HIPCHK(launch_getrf2(d, n, g.d_dinv, g.d_dinv + (size_t)32 * n, g.s_main));
+    else HIPCHK(launch_getrf(d, n, g.s_main));
     return st.finish();
 }
 

@@ -80,6 +80,15 @@ struct TrsmArgs {
 };
 hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s);
 
+// MFMA panel kernels for b % 32 == 0, b <= 512 (k_panel2.hip).  dinv holds
+// the inverses of the 32x32 diagonal blocks of the factored tile (b/32 blocks
+// of 32*32 doubles, column-major).
+bool panel2_ok(int b);
+void panel2_init();
+hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s);
+hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipStream_t s);
+hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s);
+
 // QR panels
 hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s);
 hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s);

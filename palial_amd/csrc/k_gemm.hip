@@ -14,6 +14,7 @@
 //   (the f64 variant interleaves rows across the four lane groups)
 #include "internal.hpp"
 #include <cstring>
+#include <cstdlib>
 
 namespace palial {
 
@@ -167,18 +168,21 @@ __device__ inline int xcd_remap(int pid, int nwg)
     return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <int BM, bool TA, bool TB>
-__global__ __launch_bounds__(2 * BM) void gemm_mfma_kernel(GemmArgs p)
+template <int BM, int KB, int MINB, bool TA, bool TB>
+__global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
 {
     constexpr int NT = 2 * BM;          // threads per workgroup
-    constexpr int KB = 16;
     constexpr int WM = (BM >= 64) ? 64 : BM;
     constexpr int WAVES_M = BM / WM;
     constexpr int FM = WM / 16;         // MFMA tiles per wave edge
     constexpr int LD = BM + 16;         // LDS row stride (doubles): rows 128 B apart mod 256 B
+    constexpr int EPT = BM * KB / NT;   // operand elements per thread per K slice (= KB/2)
+    constexpr int BUF = KB * LD;        // one LDS buffer of one operand
 
-    __shared__ double sA[KB * LD];
-    __shared__ double sB[KB * LD];
+    // double-buffered LDS: slice t is computed from buffer t&1 while slice t+1
+    // is staged into the other buffer; one barrier per slice
+    __shared__ double sA[2 * BUF];
+    __shared__ double sB[2 * BUF];
 
     const int b = p.b;
     const int nbd = b / BM;
@@ -215,89 +219,110 @@ __global__ __launch_bounds__(2 * BM) void gemm_mfma_kernel(GemmArgs p)
 #pragma unroll
         for (int c = 0; c < FM; c++) acc[a][c] = (d4){0.0, 0.0, 0.0, 0.0};
 
-    double ra[8], rb[8];
+    double ra[EPT], rb[EPT];
 
-    // ---- global -> registers
+    // ---- global -> registers.  K-contiguous operands: each thread loads EPT
+    // consecutive k of one row; MN-contiguous: chunks of 8 along the row.
     auto load_a = [&](int kb) {
-        if (TA) {   // opA(i,k) = X[k + i*b]: contiguous along k
-            const int row = t >> 1, kc = (t & 1) * 8;
+        if (TA) {   // opA(i,k) = X[k + i*b]
+            const int row = t >> 1, kc = (t & 1) * EPT;
             const double2 *src = (const double2 *)(tp.A + (size_t)(i0 + row) * b + kb + kc);
 #pragma unroll
-            for (int q = 0; q < 4; q++) { double2 v = src[q]; ra[2 * q] = v.x; ra[2 * q + 1] = v.y; }
+            for (int q = 0; q < EPT / 2; q++) { double2 v = src[q]; ra[2 * q] = v.x; ra[2 * q + 1] = v.y; }
             if (mask) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) ra[q] = apply_mask(mask, ra[q], kb + kc + q, i0 + row);
+                for (int q = 0; q < EPT; q++) ra[q] = apply_mask(mask, ra[q], kb + kc + q, i0 + row);
             }
-        } else {    // opA(i,k) = X[i + k*b]: contiguous along i
-            const int kk = t / (BM / 8), ic = (t % (BM / 8)) * 8;
-            const double2 *src = (const double2 *)(tp.A + (size_t)(kb + kk) * b + i0 + ic);
+        } else {    // opA(i,k) = X[i + k*b]
 #pragma unroll
-            for (int q = 0; q < 4; q++) { double2 v = src[q]; ra[2 * q] = v.x; ra[2 * q + 1] = v.y; }
-            if (mask) {
+            for (int c = 0; c < EPT / 8; c++) {
+                const int id = t + c * NT;
+                const int kk = id / (BM / 8), ic = (id % (BM / 8)) * 8;
+                const double2 *src = (const double2 *)(tp.A + (size_t)(kb + kk) * b + i0 + ic);
 #pragma unroll
-                for (int q = 0; q < 8; q++) ra[q] = apply_mask(mask, ra[q], i0 + ic + q, kb + kk);
+                for (int q = 0; q < 4; q++) { double2 v = src[q]; ra[8 * c + 2 * q] = v.x; ra[8 * c + 2 * q + 1] = v.y; }
+                if (mask) {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) ra[8 * c + q] = apply_mask(mask, ra[8 * c + q], i0 + ic + q, kb + kk);
+                }
             }
         }
     };
     auto load_b = [&](int kb) {
         if (!TB) {  // opB(k,j) = Y[k + j*b]
-            const int col = t >> 1, kc = (t & 1) * 8;
+            const int col = t >> 1, kc = (t & 1) * EPT;
             const double2 *src = (const double2 *)(tp.B + (size_t)(j0 + col) * b + kb + kc);
 #pragma unroll
-            for (int q = 0; q < 4; q++) { double2 v = src[q]; rb[2 * q] = v.x; rb[2 * q + 1] = v.y; }
+            for (int q = 0; q < EPT / 2; q++) { double2 v = src[q]; rb[2 * q] = v.x; rb[2 * q + 1] = v.y; }
         } else {    // opB(k,j) = Y[j + k*b]
-            const int kk = t / (BM / 8), jc = (t % (BM / 8)) * 8;
-            const double2 *src = (const double2 *)(tp.B + (size_t)(kb + kk) * b + j0 + jc);
 #pragma unroll
-            for (int q = 0; q < 4; q++) { double2 v = src[q]; rb[2 * q] = v.x; rb[2 * q + 1] = v.y; }
+            for (int c = 0; c < EPT / 8; c++) {
+                const int id = t + c * NT;
+                const int kk = id / (BM / 8), jc = (id % (BM / 8)) * 8;
+                const double2 *src = (const double2 *)(tp.B + (size_t)(kb + kk) * b + j0 + jc);
+#pragma unroll
+                for (int q = 0; q < 4; q++) { double2 v = src[q]; rb[8 * c + 2 * q] = v.x; rb[8 * c + 2 * q + 1] = v.y; }
+            }
         }
     };
     // ---- registers -> LDS ([k][mn] layout)
-    auto store_lds = [&]() {
+    auto store_lds = [&](int buf) {
+        double *a = sA + buf * BUF, *bb = sB + buf * BUF;
         if (TA) {
-            const int row = t >> 1, kc = (t & 1) * 8;
+            const int row = t >> 1, kc = (t & 1) * EPT;
 #pragma unroll
-            for (int q = 0; q < 8; q++) sA[(kc + q) * LD + row] = ra[q];
+            for (int q = 0; q < EPT; q++) a[(kc + q) * LD + row] = ra[q];
         } else {
-            const int kk = t / (BM / 8), ic = (t % (BM / 8)) * 8;
 #pragma unroll
-            for (int q = 0; q < 8; q++) sA[kk * LD + ic + q] = ra[q];
+            for (int c = 0; c < EPT / 8; c++) {
+                const int id = t + c * NT;
+                const int kk = id / (BM / 8), ic = (id % (BM / 8)) * 8;
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[kk * LD + ic + q] = ra[8 * c + q];
+            }
         }
         if (!TB) {
-            const int col = t >> 1, kc = (t & 1) * 8;
+            const int col = t >> 1, kc = (t & 1) * EPT;
 #pragma unroll
-            for (int q = 0; q < 8; q++) sB[(kc + q) * LD + col] = rb[q];
+            for (int q = 0; q < EPT; q++) bb[(kc + q) * LD + col] = rb[q];
         } else {
-            const int kk = t / (BM / 8), jc = (t % (BM / 8)) * 8;
 #pragma unroll
-            for (int q = 0; q < 8; q++) sB[kk * LD + jc + q] = rb[q];
+            for (int c = 0; c < EPT / 8; c++) {
+                const int id = t + c * NT;
+                const int kk = id / (BM / 8), jc = (id % (BM / 8)) * 8;
+#pragma unroll
+                for (int q = 0; q < 8; q++) bb[kk * LD + jc + q] = rb[8 * c + q];
+            }
         }
     };
 
     if (kbeg < kend) {
         load_a(kbeg);
         load_b(kbeg);
-        store_lds();
+        store_lds(0);
         __syncthreads();
+        int buf = 0;
         for (int kb = kbeg; kb < kend; kb += KB) {
             const bool more = kb + KB < kend;
             if (more) { load_a(kb + KB); load_b(kb + KB); }
+            const double *a = sA + buf * BUF, *bb = sB + buf * BUF;
 #pragma unroll
             for (int kk = 0; kk < KB; kk += 4) {
                 double av[FM], bv[FM];
                 const int kr = kk + (lane >> 4);
 #pragma unroll
-                for (int a = 0; a < FM; a++) av[a] = sA[kr * LD + wm * WM + a * 16 + (lane & 15)];
+                for (int x = 0; x < FM; x++) av[x] = a[kr * LD + wm * WM + x * 16 + (lane & 15)];
 #pragma unroll
-                for (int c = 0; c < FM; c++) bv[c] = sB[kr * LD + wn * WM + c * 16 + (lane & 15)];
+                for (int c = 0; c < FM; c++) bv[c] = bb[kr * LD + wn * WM + c * 16 + (lane & 15)];
 #pragma unroll
-                for (int a = 0; a < FM; a++)
+                for (int x = 0; x < FM; x++)
 #pragma unroll
                     for (int c = 0; c < FM; c++)
-                        acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
+                        acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[c], acc[x][c], 0, 0, 0);
             }
+            if (more) store_lds(buf ^ 1);
             __syncthreads();
-            if (more) { store_lds(); __syncthreads(); }
+            buf ^= 1;
         }
     }
 
@@ -411,25 +436,41 @@ long gemm_task_count(int mode, int T)
     }
 }
 
-template <int BM>
+template <int BM, int KB, int MINB>
 static hipError_t launch_mfma(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
 {
     const long nbd = a.b / BM;
     const long nblk = (long)a.ntasks * nbd * nbd;
     if (nblk <= 0) return hipSuccess;
     dim3 grid((unsigned)nblk), block(2 * BM);
-    if (ta && !tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, true, false>), grid, block, 0, s, a);
-    else if (!ta && !tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, false, false>), grid, block, 0, s, a);
-    else if (!ta && tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((gemm_mfma_kernel<BM, true, true>), grid, block, 0, s, a);
+    if (ta && !tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, KB, MINB, true, false>), grid, block, 0, s, a);
+    else if (!ta && !tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, KB, MINB, false, false>), grid, block, 0, s, a);
+    else if (!ta && tb) hipLaunchKernelGGL((gemm_mfma_kernel<BM, KB, MINB, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((gemm_mfma_kernel<BM, KB, MINB, true, true>), grid, block, 0, s, a);
     return hipGetLastError();
+}
+
+static int gemm_variant()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("PALIAL_GEMM_VARIANT");
+        v = e ? atoi(e) : 0;
+    }
+    return v;
 }
 
 hipError_t launch_gemm(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
 {
     if (a.ntasks <= 0) return hipSuccess;
-    if (a.b % 128 == 0) return launch_mfma<128>(a, ta, tb, s);
-    if (a.b % 32 == 0) return launch_mfma<32>(a, ta, tb, s);
+    if (a.b % 128 == 0) {
+        // 2 waves per SIMD (<= 256 registers per lane, LDS 2 x 73.7 KB per CU)
+        switch (gemm_variant()) {
+        case 9: return launch_mfma<128, 16, 1>(a, ta, tb, s);   // 1 wave per SIMD, for comparison
+        default: return launch_mfma<128, 16, 2>(a, ta, tb, s);
+        }
+    }
+    if (a.b % 32 == 0) return launch_mfma<32, 16, 1>(a, ta, tb, s);
     const long total = (long)a.ntasks * a.b * a.b;
     dim3 grid((unsigned)((total + 255) / 256)), block(256);
     if (ta && !tb) hipLaunchKernelGGL((gemm_scalar_kernel<true, false>), grid, block, 0, s, a);

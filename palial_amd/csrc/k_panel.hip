@@ -410,7 +410,7 @@ static bool g_attr_set = false;
 void panel_init()
 {
     if (g_attr_set) return;
-    const int lim = 160 * 1024 - 4096;
+    const int lim = 150 * 1024;   // largest dynamic request of these kernels; static LDS <= 2 KB
     hipFuncSetAttribute((const void *)potrf_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     hipFuncSetAttribute((const void *)potrf_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     hipFuncSetAttribute((const void *)getrf_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);

@@ -1,0 +1,591 @@
+// k_panel2.hip -- MFMA panel kernels for tiles with b % 32 == 0 and b <= 512.
+//
+//   potrf2_kernel : upper Cholesky of one tile (LAPACKE_dpotrf('U'),
+//                   src/cholesky.h:40) in 32-column blocks.  For each block:
+//                   the 32x32 diagonal block is factored by one wave (one
+//                   column per lane, shuffles), inverted (U_D^-1, kept in LDS
+//                   and written to `dinv` for the panel TRSM), the 32-row panel
+//                   is solved as U_D^-T P with v_mfma_f64_16x16x4_f64, and the
+//                   rank-32 trailing SYRK runs out of the LDS copy of P.
+//   getrf2_kernel : LU without pivoting of one tile (PALIAL_dgetrf,
+//                   src/srclu/dgetrf.h:11), same blocking; writes U_D^-1 and
+//                   L_D^-1 of every diagonal block.  A zero or sub-sfmin pivot
+//                   in a block switches that block's L panel to the
+//                   reference's per-element scaling rule
+//                   (src/srclu/dgetrf_no_pivot.h:21-39).
+//   trsm2_kernel  : the panel TRSMs of one DAG step (src/cholesky.h:73-84,
+//                   src/lu.h:67-78, src/lu.h:109-120): each workgroup solves a
+//                   32-wide strip of one tile held in LDS by right-looking
+//                   blocked substitution; diagonal blocks are applied through
+//                   the inverses written by potrf2/getrf2, off-diagonal
+//                   updates are MFMA.
+#include "internal.hpp"
+#include <float.h>
+
+namespace palial {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int P2_THREADS = 512;           // TRSM strips
+constexpr int P2_WAVES = P2_THREADS / 64;
+constexpr int PF_THREADS = 512;           // diagonal-tile factorizations
+constexpr int PF_WAVES = PF_THREADS / 64;
+constexpr int PF_G = 4;                   // POTRF trailing-update tiles in flight per wave
+constexpr int GF_G = 2;                   // GETRF trailing-update tiles per job (one row tile)
+constexpr int NB2 = 32;            // inner block
+constexpr int LDD = NB2 + 1;       // LDS stride of 32x32 blocks
+
+__device__ inline d4 mfma16(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ inline d4 zero4() { return (d4){0.0, 0.0, 0.0, 0.0}; }
+
+// triangular index e -> (bi <= bj), e = bj(bj+1)/2 + bi
+__device__ inline void tri_ij(int e, int &bi, int &bj)
+{
+    int j = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while ((j + 1) * (j + 2) / 2 <= e) j++;
+    while (j * (j + 1) / 2 > e) j--;
+    bj = j;
+    bi = e - j * (j + 1) / 2;
+}
+
+// Parallel inverse of a 32x32 triangular block by recursive doubling, called
+// by all threads of the workgroup (contains barriers).  X (ld LDD) receives
+// the inverse, sT (32*32) is scratch.  At level s every 2s-block
+//   upper: [[X11, X12], [0, X22]],  X12 = -X11 U12 X22
+//   unit lower: [[X11, 0], [X21, X22]],  X21 = -X22 L21 X11
+// is completed from its two already inverted s-blocks.
+template <bool UPPER>
+__device__ void tri_inverse32(const double *M, double *X, double *sT, int nthreads)
+{
+    const int t = threadIdx.x;
+    for (int e = t; e < NB2 * NB2; e += nthreads) {
+        const int r = e % NB2, c = e / NB2;
+        X[r + c * LDD] = (r == c) ? (UPPER ? 1.0 / M[r + r * LDD] : 1.0) : 0.0;
+    }
+    __syncthreads();
+    for (int sz = 1; sz < NB2; sz *= 2) {
+        const int nel = (NB2 / (2 * sz)) * sz * sz;
+        // T = U12 X22 (upper) or L21 X11 (lower), one element per thread
+        for (int e = t; e < nel; e += nthreads) {
+            const int pair = e / (sz * sz), ij = e % (sz * sz), i = ij % sz, j = ij / sz;
+            const int r0 = pair * 2 * sz;
+            double acc = 0.0;
+            if (UPPER) {
+                for (int p = 0; p <= j; p++) acc += M[(r0 + i) + (r0 + sz + p) * LDD] * X[(r0 + sz + p) + (r0 + sz + j) * LDD];
+            } else {
+                for (int p = j; p < sz; p++) acc += M[(r0 + sz + i) + (r0 + p) * LDD] * X[(r0 + p) + (r0 + j) * LDD];
+            }
+            sT[e] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < nel; e += nthreads) {
+            const int pair = e / (sz * sz), ij = e % (sz * sz), i = ij % sz, j = ij / sz;
+            const int r0 = pair * 2 * sz;
+            const double *T = sT + pair * sz * sz;
+            double acc = 0.0;
+            if (UPPER) {
+                for (int p = i; p < sz; p++) acc += X[(r0 + i) + (r0 + p) * LDD] * T[p + j * sz];
+                X[(r0 + i) + (r0 + sz + j) * LDD] = -acc;
+            } else {
+                for (int p = 0; p <= i; p++) acc += X[(r0 + sz + i) + (r0 + sz + p) * LDD] * T[p + j * sz];
+                X[(r0 + sz + i) + (r0 + j) * LDD] = -acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Copy the NB x (b - base) row panel A[j0:j0+NB, base:b] into sP, eight
+// independent global loads in flight per thread before the LDS stores.
+__device__ inline void stage_panel(const double *A, int b, int j0, int base, double *sP, int LDP, int nthreads)
+{
+    const int total = NB2 * (b - base);
+    for (int e0 = threadIdx.x; e0 < total; e0 += nthreads * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * nthreads;
+            v[u] = (e < total) ? A[(size_t)(j0 + e % NB2) + (size_t)(base + e / NB2) * b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * nthreads;
+            if (e < total) sP[(e % NB2) * LDP + base + e / NB2] = v[u];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- POTRF
+__global__ __launch_bounds__(PF_THREADS) void potrf2_kernel(double *A, int b, int *info, int info_base,
+                                                            double *dinv)
+{
+    constexpr int NB = NB2;
+    const int LDP = b + 16;               // panel rows 128 B apart modulo 256 B
+    extern __shared__ double sP[];        // NB x LDP: sP[q*LDP + c], c = absolute column
+    __shared__ double sU[NB * LDD];
+    __shared__ double sI[NB * LDD];
+    double *sT = sP;                      // inverse scratch: the panel is not live yet
+    __shared__ int s_fail;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) s_fail = 0;
+    __syncthreads();
+
+    for (int j0 = 0; j0 < b; j0 += NB) {
+        // 1. diagonal block, one element pair (rows r0, r0+16 of column c) per
+        //    thread; right-looking Cholesky through LDS (LAPACK dpotf2 order:
+        //    sqrt of the pivot, row scaled by its reciprocal, rank-1 update)
+        {
+            const bool own = t < 16 * NB;
+            const int c = t / 16, r0 = t % 16;
+            double v0 = 0.0, v1 = 0.0;
+            if (own) {
+                v0 = (r0 <= c) ? A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b] : 0.0;
+                v1 = (r0 + 16 <= c) ? A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b] : 0.0;
+                sU[r0 + c * LDD] = v0;
+                sU[r0 + 16 + c * LDD] = v1;
+            }
+            __syncthreads();
+            for (int j = 0; j < NB; j++) {
+                const double piv = sU[j + j * LDD];
+                if (!(piv > 0.0)) {               // not positive definite: LAPACK info = j+1
+                    if (t == 0) { s_fail = 1; if (info) *info = info_base + j0 + j + 1; }
+                    break;
+                }
+                const double d = sqrt(piv), rd = 1.0 / d;
+                if (own && c >= j) {              // row j of U: U(j,j) = d, U(j,c) = a(j,c)/d
+                    if (r0 == j) { v0 = (c == j) ? d : v0 * rd; sU[r0 + c * LDD] = v0; }
+                    if (r0 + 16 == j) { v1 = (c == j) ? d : v1 * rd; sU[r0 + 16 + c * LDD] = v1; }
+                }
+                __syncthreads();
+                if (own && c > j) {               // a(r,c) -= U(j,r) U(j,c) for j < r <= c
+                    const double u = sU[j + c * LDD];
+                    if (r0 > j && r0 <= c) { v0 -= sU[j + r0 * LDD] * u; sU[r0 + c * LDD] = v0; }
+                    if (r0 + 16 > j && r0 + 16 <= c) { v1 -= sU[j + (r0 + 16) * LDD] * u; sU[r0 + 16 + c * LDD] = v1; }
+                }
+                __syncthreads();
+            }
+            if (own) {
+                if (r0 <= c) A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b] = v0;
+                if (r0 + 16 <= c) A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b] = v1;
+            }
+        }
+        __syncthreads();
+        if (s_fail) return;
+        tri_inverse32<true>(sU, sI, sT, PF_THREADS);
+        if (dinv)
+            for (int e = t; e < NB * NB; e += PF_THREADS)
+                dinv[(size_t)(j0 / NB) * NB * NB + e] = sI[(e % NB) + (e / NB) * LDD];
+        const int base = j0 + NB, rest = b - base;
+        // 3. stage the raw row panel A[j0:j0+NB, base:b]
+        stage_panel(A, b, j0, base, sP, LDP, PF_THREADS);
+        __syncthreads();
+        if (rest == 0) break;
+        // 4. P <- U_D^{-T} P: a wave owns 16-column tiles (both 16-row halves)
+#pragma unroll 1
+        for (int ct = w; ct < rest / 16; ct += PF_WAVES) {
+            const int col = base + ct * 16 + (lane & 15);
+            d4 acc0 = zero4(), acc1 = zero4();
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+                const double bv = sP[q * LDP + col];
+                acc0 = mfma16(sI[q + (lane & 15) * LDD], bv, acc0);         // opA(i,q) = Uinv(q,i)
+                acc1 = mfma16(sI[q + (16 + (lane & 15)) * LDD], bv, acc1);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = (lane >> 4) + 4 * v;
+                sP[i * LDP + col] = acc0[v];
+                sP[(i + 16) * LDP + col] = acc1[v];
+                A[(size_t)(j0 + i) + (size_t)col * b] = acc0[v];
+                A[(size_t)(j0 + i + 16) + (size_t)col * b] = acc1[v];
+            }
+        }
+        __syncthreads();
+        // 5. trailing SYRK on the upper triangle of A[base:b, base:b], 16x16 tiles.
+        //    Each wave walks groups of PF_G tiles; the C tiles of the next group
+        //    are loaded while the MFMAs of the current group run.
+        const int nt = rest / 16;
+        const int ntri = nt * (nt + 1) / 2;
+        const int stride = PF_WAVES * PF_G;
+        auto load_group = [&](int e0, double (&cv)[PF_G][4]) {
+#pragma unroll
+            for (int g = 0; g < PF_G; g++) {
+                if (e0 + g < ntri) {
+                    int bi, bj;
+                    tri_ij(e0 + g, bi, bj);
+                    const int j = base + 16 * bj + (lane & 15);
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        cv[g][v] = A[(size_t)(base + 16 * bi + (lane >> 4) + 4 * v) + (size_t)j * b];
+                }
+            }
+        };
+        auto run_group = [&](int e0, const double (&cv)[PF_G][4]) {
+            int ti[PF_G], tj[PF_G];
+            d4 acc[PF_G];
+#pragma unroll
+            for (int g = 0; g < PF_G; g++) {
+                acc[g] = zero4();
+                ti[g] = -1;
+                tj[g] = 0;
+                if (e0 + g < ntri) tri_ij(e0 + g, ti[g], tj[g]);
+            }
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+#pragma unroll
+                for (int g = 0; g < PF_G; g++)
+                    if (ti[g] >= 0)
+                        acc[g] = mfma16(sP[q * LDP + base + 16 * ti[g] + (lane & 15)],
+                                        sP[q * LDP + base + 16 * tj[g] + (lane & 15)], acc[g]);
+            }
+#pragma unroll
+            for (int g = 0; g < PF_G; g++) {
+                if (ti[g] < 0) continue;
+                const int j = base + 16 * tj[g] + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = base + 16 * ti[g] + (lane >> 4) + 4 * v;
+                    if (ti[g] < tj[g] || i <= j) A[(size_t)i + (size_t)j * b] = cv[g][v] - acc[g][v];
+                }
+            }
+        };
+        {
+            double cva[PF_G][4], cvb[PF_G][4];
+            int e0 = w * PF_G;
+            if (e0 < ntri) load_group(e0, cva);
+#pragma unroll 1
+            while (e0 < ntri) {
+                if (e0 + stride < ntri) load_group(e0 + stride, cvb);
+                run_group(e0, cva);
+                e0 += stride;
+                if (e0 >= ntri) break;
+                if (e0 + stride < ntri) load_group(e0 + stride, cva);
+                run_group(e0, cvb);
+                e0 += stride;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- GETRF
+__device__ inline double piv_scale2(double x, double alpha)
+{
+    if (alpha == 0.0) return x;
+    if (fabs(alpha) > DBL_MIN) return x * (1.0 / alpha);
+    return x / alpha;
+}
+
+__global__ __launch_bounds__(PF_THREADS) void getrf2_kernel(double *A, int b, double *dinvU, double *dinvL)
+{
+    constexpr int NB = NB2;
+    const int LDP = b + 16;
+    extern __shared__ double sP[];        // U panel rows: sP[q*LDP + c]
+    __shared__ double sD[NB * LDD];       // factored diagonal block (L strict lower, U upper)
+    __shared__ double sUi[NB * LDD];
+    __shared__ double sLi[NB * LDD];
+    double *sT = sP;                      // inverse scratch: the panel is not live yet
+    __shared__ int s_slow;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+
+    for (int j0 = 0; j0 < b; j0 += NB) {
+        // 1. diagonal block, one element pair per thread (rows r0 and r0+16 of
+        //    column c); right-looking no-pivot LU through LDS, two barriers per
+        //    column (src/srclu/dgetrf_no_pivot.h:18-55 on the 32x32 block)
+        {
+            const bool own = t < 16 * NB;
+            const int c = t / 16, r0 = t % 16;
+            double v0 = 0.0, v1 = 0.0;
+            if (own) {
+                v0 = A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b];
+                v1 = A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b];
+                sD[r0 + c * LDD] = v0;
+                sD[r0 + 16 + c * LDD] = v1;
+            }
+            if (t == 0) s_slow = 0;
+            __syncthreads();
+            for (int i = 0; i < NB; i++) {
+                const double alpha = sD[i + i * LDD];
+                if (own && c == i) {
+                    if (r0 > i) { v0 = piv_scale2(v0, alpha); sD[r0 + i * LDD] = v0; }
+                    if (r0 + 16 > i) { v1 = piv_scale2(v1, alpha); sD[r0 + 16 + i * LDD] = v1; }
+                }
+                if (t == 0 && !(fabs(alpha) > DBL_MIN)) s_slow = 1;
+                __syncthreads();
+                if (own && c > i) {
+                    const double u = sD[i + c * LDD];
+                    if (r0 > i) { v0 -= sD[r0 + i * LDD] * u; sD[r0 + c * LDD] = v0; }
+                    if (r0 + 16 > i) { v1 -= sD[r0 + 16 + i * LDD] * u; sD[r0 + 16 + c * LDD] = v1; }
+                }
+                __syncthreads();
+            }
+            if (own) {
+                A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b] = v0;
+                A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b] = v1;
+            }
+        }
+        tri_inverse32<true>(sD, sUi, sT, PF_THREADS);
+        tri_inverse32<false>(sD, sLi, sT, PF_THREADS);
+        for (int e = t; e < NB * NB; e += PF_THREADS) {
+            const int r = e % NB, c = e / NB;
+            if (dinvU) dinvU[(size_t)(j0 / NB) * NB * NB + e] = sUi[r + c * LDD];
+            if (dinvL) dinvL[(size_t)(j0 / NB) * NB * NB + e] = sLi[r + c * LDD];
+        }
+        const int base = j0 + NB, rest = b - base;
+        stage_panel(A, b, j0, base, sP, LDP, PF_THREADS);
+        __syncthreads();
+        if (rest == 0) break;
+        const int nt = rest / 16;
+        // U panel: Y = L_D^{-1} A[j0:j0+NB, base:b]   (tiles 0 .. nt-1)
+        // L panel: X = A[base:b, j0:j0+NB] U_D^{-1}   (tiles nt .. 2nt-1, two 16-col halves each)
+#pragma unroll 1
+        for (int job = w; job < 2 * nt; job += PF_WAVES) {
+            if (job < nt) {
+                const int col = base + job * 16 + (lane & 15);
+                d4 acc0 = zero4(), acc1 = zero4();
+#pragma unroll
+                for (int k4 = 0; k4 < NB; k4 += 4) {
+                    const int q = k4 + (lane >> 4);
+                    const double bv = sP[q * LDP + col];
+                    acc0 = mfma16(sLi[(lane & 15) + q * LDD], bv, acc0);          // opA(i,q) = Linv(i,q)
+                    acc1 = mfma16(sLi[(16 + (lane & 15)) + q * LDD], bv, acc1);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = (lane >> 4) + 4 * v;
+                    sP[i * LDP + col] = acc0[v];
+                    sP[(i + 16) * LDP + col] = acc1[v];
+                    A[(size_t)(j0 + i) + (size_t)col * b] = acc0[v];
+                    A[(size_t)(j0 + i + 16) + (size_t)col * b] = acc1[v];
+                }
+            } else if (!s_slow) {
+                const int i0 = base + (job - nt) * 16;
+                d4 acc0 = zero4(), acc1 = zero4();
+#pragma unroll
+                for (int k4 = 0; k4 < NB; k4 += 4) {
+                    const int q = k4 + (lane >> 4);
+                    const double av = A[(size_t)(i0 + (lane & 15)) + (size_t)(j0 + q) * b];
+                    acc0 = mfma16(av, sUi[q + (lane & 15) * LDD], acc0);            // opB(q,c) = Uinv(q,c)
+                    acc1 = mfma16(av, sUi[q + (16 + (lane & 15)) * LDD], acc1);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = i0 + (lane >> 4) + 4 * v;
+                    A[(size_t)i + (size_t)(j0 + (lane & 15)) * b] = acc0[v];
+                    A[(size_t)i + (size_t)(j0 + 16 + (lane & 15)) * b] = acc1[v];
+                }
+            } else if (lane < 16) {
+                // reference scaling rule, one row per lane: x U_D = a (rare path:
+                // a zero or sub-sfmin pivot in this block)
+                const int i = base + (job - nt) * 16 + lane;
+                double *row = A + (size_t)i + (size_t)j0 * b;
+                for (int q = 0; q < NB; q++) {
+                    double s = row[(size_t)q * b];
+                    for (int p = 0; p < q; p++) s -= row[(size_t)p * b] * sD[p + q * LDD];
+                    row[(size_t)q * b] = piv_scale2(s, sD[q + q * LDD]);
+                }
+            }
+        }
+        __syncthreads();
+        // trailing: A[base:, base:] -= L_panel U_panel.  A job is one 16-row tile
+        // row and GF_G consecutive 16-column tiles: the L-panel fragment (global)
+        // is loaded once, the C tiles are loaded before the MFMAs.
+        const int ngrp = (nt + GF_G - 1) / GF_G;
+#pragma unroll 1
+        for (int job = w; job < nt * ngrp; job += PF_WAVES) {
+            const int bi = job / ngrp, bj0 = (job % ngrp) * GF_G;
+            const int i0 = base + 16 * bi;
+            double av[NB / 4];
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4)
+                av[k4 / 4] = A[(size_t)(i0 + (lane & 15)) + (size_t)(j0 + k4 + (lane >> 4)) * b];
+            double cv[GF_G][4];
+            d4 acc[GF_G];
+#pragma unroll
+            for (int g = 0; g < GF_G; g++) {
+                acc[g] = zero4();
+                if (bj0 + g < nt) {
+                    const int j = base + 16 * (bj0 + g) + (lane & 15);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) cv[g][v] = A[(size_t)(i0 + (lane >> 4) + 4 * v) + (size_t)j * b];
+                }
+            }
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+#pragma unroll
+                for (int g = 0; g < GF_G; g++)
+                    if (bj0 + g < nt)
+                        acc[g] = mfma16(av[k4 / 4], sP[q * LDP + base + 16 * (bj0 + g) + (lane & 15)], acc[g]);
+            }
+#pragma unroll
+            for (int g = 0; g < GF_G; g++) {
+                if (bj0 + g >= nt) continue;
+                const int j = base + 16 * (bj0 + g) + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; v++) A[(size_t)(i0 + (lane >> 4) + 4 * v) + (size_t)j * b] = cv[g][v] - acc[g][v];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- TRSM
+// Strip layout in LDS: row i, column c at i*32 + (c ^ 16*(i&1)) (odd rows
+// swap their 16-column halves so four consecutive rows read by one MFMA
+// fragment load fall on distinct bank halves).
+__device__ inline int xidx(int i, int c) { return i * NB2 + (c ^ ((i & 1) << 4)); }
+
+template <int KIND>
+__global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const double *dinv)
+{
+    constexpr int NB = NB2, SW = NB2;
+    extern __shared__ double sX[];
+    const int b = p.b;
+    const int strips = b / SW;
+    const int tix = blockIdx.x / strips, strip = blockIdx.x % strips;
+    const int c0 = strip * SW;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    double *X;
+    if (p.mat) {
+        const int idx = p.first + tix;
+        X = (KIND == TR_LUR) ? p.mat + tile_off(b, p.MT, idx, p.k) : p.mat + tile_off(b, p.MT, p.k, idx);
+    } else {
+        X = p.single;
+    }
+    const double *D = p.diag;
+    // M(i,q): CHOL/LUR M = U^T -> D[q + i*b];  LUL M = L -> D[i + q*b]
+    for (int e0 = t; e0 < b * SW; e0 += P2_THREADS * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * P2_THREADS;
+            if (e < b * SW)
+                v[u] = (KIND == TR_LUR) ? X[(size_t)(c0 + e % SW) + (size_t)(e / SW) * b]
+                                        : X[(size_t)(e % b) + (size_t)(c0 + e / b) * b];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * P2_THREADS;
+            if (e < b * SW) {
+                if (KIND == TR_LUR) sX[xidx(e / SW, e % SW)] = v[u];
+                else sX[xidx(e % b, e / b)] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+
+    for (int r0 = 0; r0 < b; r0 += NB) {
+        const double *Mi = dinv + (size_t)(r0 / NB) * NB * NB;   // inverse of the diagonal block
+        // (a) X_r <- M_rr^{-1} X_r : waves 0..3 own one 16x16 output tile each
+        d4 acc = zero4();
+        const int ti = w & 1, tc = w >> 1;
+        if (w < 4) {
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+                const int i = ti * 16 + (lane & 15);
+                // CHOL/LUR: M_rr^{-1} = (U_rr^{-1})^T -> Uinv(q,i);  LUL: Linv(i,q)
+                const double av = (KIND == TR_LUL) ? Mi[i + q * NB] : Mi[q + i * NB];
+                acc = mfma16(av, sX[xidx(r0 + q, tc * 16 + (lane & 15))], acc);
+            }
+        }
+        __syncthreads();
+        if (w < 4) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                sX[xidx(r0 + ti * 16 + (lane >> 4) + 4 * v, tc * 16 + (lane & 15))] = acc[v];
+        }
+        __syncthreads();
+        // (b) rows below: X[i] -= M(i, r-block) X_r.  A wave owns row tiles
+        //     it = w, w+8, ... (both 16-column halves); the M fragments of all
+        //     its row tiles are loaded before any MFMA.
+        const int below = (b - r0 - NB) / 16;
+        constexpr int MAXT = 4;                 // row tiles per wave for b <= 512
+        double av[MAXT][NB / 4];
+#pragma unroll
+        for (int m = 0; m < MAXT; m++) {
+            const int it = w + m * P2_WAVES;
+            if (it < below) {
+                const int i = r0 + NB + it * 16 + (lane & 15);
+#pragma unroll
+                for (int k4 = 0; k4 < NB; k4 += 4) {
+                    const int q = k4 + (lane >> 4);
+                    av[m][k4 / 4] = (KIND == TR_LUL) ? D[(size_t)i + (size_t)(r0 + q) * b] : D[(size_t)(r0 + q) + (size_t)i * b];
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MAXT; m++) {
+            const int it = w + m * P2_WAVES;
+            if (it >= below) continue;
+            const int i0 = r0 + NB + it * 16;
+            d4 u0 = zero4(), u1 = zero4();
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+                u0 = mfma16(av[m][k4 / 4], sX[xidx(r0 + q, (lane & 15))], u0);
+                u1 = mfma16(av[m][k4 / 4], sX[xidx(r0 + q, 16 + (lane & 15))], u1);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                sX[xidx(i0 + (lane >> 4) + 4 * v, (lane & 15))] -= u0[v];
+                sX[xidx(i0 + (lane >> 4) + 4 * v, 16 + (lane & 15))] -= u1[v];
+            }
+        }
+        __syncthreads();
+    }
+
+    for (int e = t; e < b * SW; e += P2_THREADS) {
+        if (KIND == TR_LUR) X[(size_t)(c0 + e % SW) + (size_t)(e / SW) * b] = sX[xidx(e / SW, e % SW)];
+        else X[(size_t)(e % b) + (size_t)(c0 + e / b) * b] = sX[xidx(e % b, e / b)];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+bool panel2_ok(int b) { return b % NB2 == 0 && b >= NB2 && b <= 512; }
+
+void panel2_init()
+{
+    // dynamic LDS at the largest supported tile (b = 512); static LDS comes on top
+    const int lds_panel = NB2 * (512 + 16) * 8;   // 135168 B
+    const int lds_trsm = 512 * NB2 * 8;           // 131072 B
+    hipFuncSetAttribute((const void *)potrf2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_panel);
+    hipFuncSetAttribute((const void *)getrf2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_panel);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_CHOL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+}
+
+hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s)
+{
+    const size_t lds = (size_t)NB2 * (b + 16) * 8;
+    hipLaunchKernelGGL(potrf2_kernel, dim3(1), dim3(PF_THREADS), lds, s, tile, b, info, info_base, dinv);
+    return hipGetLastError();
+}
+
+hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipStream_t s)
+{
+    const size_t lds = (size_t)NB2 * (b + 16) * 8;
+    hipLaunchKernelGGL(getrf2_kernel, dim3(1), dim3(PF_THREADS), lds, s, tile, b, dinvU, dinvL);
+    return hipGetLastError();
+}
+
+hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    dim3 grid((unsigned)(a.ntiles * (a.b / NB2))), block(P2_THREADS);
+    const size_t lds = (size_t)a.b * NB2 * 8;
+    if (kind == TR_CHOL) hipLaunchKernelGGL(trsm2_kernel<TR_CHOL>, grid, block, lds, s, a, dinv);
+    else if (kind == TR_LUL) hipLaunchKernelGGL(trsm2_kernel<TR_LUL>, grid, block, lds, s, a, dinv);
+    else hipLaunchKernelGGL(trsm2_kernel<TR_LUR>, grid, block, lds, s, a, dinv);
+    return hipGetLastError();
+}
+
+}  // namespace palial

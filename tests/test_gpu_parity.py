@@ -114,7 +114,7 @@ def test_qr_tiles(hip, orc, n):
 
 
 # ------------------------------------------------------------------ full algorithms
-SIZES = [(4, 2), (256, 64), (512, 128), (1024, 256), (2048, 256)]
+SIZES = [(4, 2), (256, 64), (512, 128), (1024, 256), (2048, 256), (2048, 512), (2048, 1024), (480, 96)]
 
 
 @pytest.mark.parametrize("N,b", SIZES)
