@@ -146,6 +146,22 @@ int palial_hip_lu(MATRIX_desc A, int *ipiv, const palial_opts *opts, palial_repo
  * of S(k,k) and S(m,k); all other S entries are not written. */
 int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_report *rep);
 
+/* ---------------------------------------------------------------- solve */
+/* No counterpart in the reference (which only factors): solve A X = B for
+ * nrhs <= 16 right-hand sides with iterative refinement to the backward
+ * error opts->tol (default 1e-14), eta = max_j ||b_j - A x_j||_inf /
+ * (||A||_inf ||x_j||_inf + ||b_j||_inf).  algo 0: tiled Cholesky (A SPD),
+ * 1: tiled LU without pivoting.  A (layout L, FP64) is not modified: a copy
+ * is factored with its O(n^3) trailing updates in opts->precision (FP64,
+ * FP32 or BF16 operands with FP32 accumulation); the diagonal-tile
+ * factorizations, the residual b - A x and the solution update stay FP64.
+ * B (column-major, leading dimension ldb >= N) holds the right-hand sides on
+ * entry and the solution on return.  Requires b % 64 == 0 and b <= 512.
+ * Returns PALIAL_ERR_NOCONV (with the reached backward error in rep) when
+ * opts->max_iter refinement steps do not reach the tolerance. */
+int palial_hip_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palial_opts *opts,
+                     palial_report *rep);
+
 /* ---------------------------------------------------------------- tile kernels */
 /* Single-tile versions of the reference tile kernels; pointers follow
  * opts->device_matrix.  The leading dimensions must equal n (tile storage). */

@@ -98,6 +98,8 @@ def lib():
         L.palial_hip_memcpy_d2h.argtypes = [dp, dp, C.c_size_t]
         L.palial_hip_memcpy_d2d.argtypes = [dp, dp, C.c_size_t]
         L.palial_hip_copy.argtypes = [dp, dp, C.c_size_t]
+        L.palial_hip_solve.argtypes = [C.c_int, MATRIX_desc, dp, C.c_int, C.c_int, C.POINTER(Opts),
+                                       C.POINTER(Report)]
         L.palial_hip_profile_stats.argtypes = [C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
@@ -354,3 +356,33 @@ def factor_device(algo, dm, S=None, **kw):
         rc = lib().palial_hip_qr(dm.desc, S.desc, C.byref(o), C.byref(rep))
     _check(rc)
     return rc, rep
+
+
+# ------------------------------------------------------------------ solve
+def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, max_iter=0, device=False,
+          check=True):
+    """Solve A X = B with iterative refinement (no reference counterpart).
+
+    A: layout-L matrix (numpy flat array, torch tensor or DeviceMatrix), left
+    unmodified.  B: N x nrhs column-major right-hand sides (numpy, copied;
+    or a device pointer with device=True).  Returns (X, report, rc)."""
+    init()
+    if isinstance(A, DeviceMatrix):
+        d, dev = A.desc, True
+    else:
+        d, dev, keep = _desc_of(A, N, b)
+    o = _opts(device=bool(dev), precision=precision, tol=tol, max_iter=max_iter)
+    rep = Report()
+    code = {"cholesky": 0, "lu": 1}[algo]
+    if dev and not isinstance(B, np.ndarray):
+        ptr, nrhs, ldb = B
+        rc = lib().palial_hip_solve(code, d, ptr, nrhs, ldb, C.byref(o), C.byref(rep))
+        X = None
+    else:
+        X = np.asfortranarray(np.array(B, dtype=np.float64, copy=True).reshape(d.matrix_size, -1))
+        if dev:
+            raise ValueError("host right-hand sides need a host matrix")
+        rc = lib().palial_hip_solve(code, d, X.ctypes.data, X.shape[1], X.shape[0], C.byref(o), C.byref(rep))
+    if check and rc < 0:
+        _check(rc)
+    return X, rep, rc

@@ -44,6 +44,13 @@ struct Ctx {
     size_t w_cap = 0;
     double *d_dinv = nullptr;       // inverses of the 32x32 diagonal blocks (U then L)
     size_t dinv_cap = 0;
+    // solve workspace: factor copy F, diagonal-tile inverses, vectors
+    double *d_F = nullptr;
+    size_t F_cap = 0;
+    double *d_inv = nullptr;
+    size_t inv_cap = 0;
+    double *d_vec = nullptr;
+    size_t vec_cap = 0;
     std::map<std::tuple<int, void *, void *, int, int, int>, Graph> graphs;
 };
 
@@ -123,9 +130,11 @@ int ensure_ws(int b, int MT)
     return 0;
 }
 
-int ensure_dinv(int b)
+// inverses of the 32x32 diagonal blocks of every diagonal tile: tile k holds
+// U blocks at k*64*b and L blocks at k*64*b + 32*b (doubles)
+int ensure_dinv(int b, int MT)
 {
-    const size_t need = 2 * (size_t)32 * b;
+    const size_t need = (size_t)MT * 64 * b;
     if (g.dinv_cap < need) {
         drop_graphs();
         if (g.d_dinv) hipFree(g.d_dinv);
@@ -245,20 +254,22 @@ TrsmArgs targs(double *mat, int N, int b, int k)
 // k+1 runs on the panel stream ahead of the rest, so POTRF/TRSM of step k+1
 // overlap the bulk of step k's trailing update.
 void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, int *dinfo,
-                  Launcher &L)
+                  Launcher &L, int prec = 0)
 {
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
+    const bool lp = prec > 0 && b % 128 == 0;
+    auto dinvU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
     auto potrf = [&](int k, hipStream_t s) {
         L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s, [&] {
-            return v2 ? launch_potrf2(dA + tile_off(b, MT, k, k), b, dinfo, k * b, g.d_dinv, s)
+            return v2 ? launch_potrf2(dA + tile_off(b, MT, k, k), b, dinfo, k * b, dinvU(k), s)
                       : launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s);
         });
     };
     auto trsm = [&](int k, hipStream_t s) {
         const int T = MT - k - 1;
         L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] {
-            return v2 ? launch_trsm2(TR_CHOL, targs(dA, N, b, k), g.d_dinv, s)
+            return v2 ? launch_trsm2(TR_CHOL, targs(dA, N, b, k), dinvU(k), s)
                       : launch_trsm(TR_CHOL, targs(dA, N, b, k), s);
         });
     };
@@ -270,7 +281,7 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
         else if (mode == GM_CHOL_ROW) { nsyrk = 1; ngemm = T - 1; }
         else { nsyrk = T - 1; ngemm = (long)(T - 1) * (T - 2) / 2; }
         L.run(KC_GEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk, 3.0 * (ngemm + nsyrk) * tile_bytes(b), s,
-              [&] { return launch_gemm(a, true, false, s); });
+              [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
     };
     if (!lookahead) {
         for (int k = 0; k < MT; k++) {
@@ -304,13 +315,14 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
 }
 
 // LU without pivoting, src/lu.h:14-173 (with correct tile dependences).
-void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, Launcher &L)
+void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, Launcher &L, int prec = 0)
 {
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
-    double *dU = g.d_dinv, *dL = g.d_dinv + (size_t)32 * b;
+    const bool lp = prec > 0 && b % 128 == 0;
     auto panel = [&](int k, hipStream_t s) {
         const int T = MT - k - 1;
+        double *dU = g.d_dinv + (size_t)k * 64 * b, *dL = dU + (size_t)32 * b;
         L.run(KC_DIAG, 2 * b3(b) / 3, 2 * tile_bytes(b), s, [&] {
             return v2 ? launch_getrf2(dA + tile_off(b, MT, k, k), b, dU, dL, s)
                       : launch_getrf(dA + tile_off(b, MT, k, k), b, s);
@@ -327,7 +339,7 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
         GemmArgs a = gargs(mode, dA, N, b, k, T);
         const double ntile = (double)a.ntasks;
         L.run(KC_GEMM, 2 * b3(b) * ntile, 3.0 * ntile * tile_bytes(b), s,
-              [&] { return launch_gemm(a, false, false, s); });
+              [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
     };
     if (!lookahead) {
         for (int k = 0; k < MT; k++) {
@@ -396,6 +408,163 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, Launcher &L)
     }
 }
 
+// ------------------------------------------------------------------ solve
+int grow(double **p, size_t *cap, size_t need)
+{
+    if (*cap >= need) return 0;
+    drop_graphs();
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, need * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "solve workspace of %zu bytes", need * 8);
+    *cap = need;
+    return 0;
+}
+
+// Triangular sweeps with the factored tiles F and the inverted diagonal tiles:
+//   Cholesky: U^T y = r, U x = y;  LU: L y = r, U x = y.
+// r is destroyed (used as the running right-hand side of the forward sweep).
+void sweeps(int algo, const double *F, const double *Uinv, const double *Linv, double *r, double *y, double *x,
+            int N, int b, int nrhs, hipStream_t s, Launcher &L)
+{
+    const int MT = N / b;
+    GemvArgs a;
+    memset(&a, 0, sizeof a);
+    a.mat = F; a.N = N; a.b = b; a.ld = N; a.nrhs = nrhs;
+    for (int k = 0; k < MT; k++) {
+        a.k = k;
+        a.mode = GV_DIAG; a.inv = (algo == 0) ? Uinv : Linv; a.trans = (algo == 0); a.vin = r; a.vout = y; a.ntasks = 1;
+        L.chk(launch_tile_gemv(a, s));
+        if (k + 1 < MT) {
+            a.mode = (algo == 0) ? GV_CHOL_FWD : GV_LU_FWD; a.trans = (algo == 0); a.vin = y; a.vout = r;
+            a.ntasks = MT - k - 1;
+            L.chk(launch_tile_gemv(a, s));
+        }
+    }
+    for (int k = MT - 1; k >= 0; k--) {
+        a.k = k;
+        a.mode = GV_DIAG; a.inv = Uinv; a.trans = 0; a.vin = y; a.vout = x; a.ntasks = 1;
+        L.chk(launch_tile_gemv(a, s));
+        if (k > 0) {
+            a.mode = GV_BWD; a.trans = 0; a.vin = x; a.vout = y; a.ntasks = k;
+            L.chk(launch_tile_gemv(a, s));
+        }
+    }
+}
+
+int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palial_opts *o_in, palial_report *rep)
+{
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_desc(A))) return rc;
+    palial_opts o = o_in ? *o_in : default_opts();
+    const int N = A.matrix_size, b = A.tile_size, MT = N / b;
+    if (!panel2_ok(b) || b % 64) return fail(PALIAL_ERR_ARG, "solve needs b %% 64 == 0 and b <= 512 (b=%d)", b);
+    if (nrhs < 1 || nrhs > 16) return fail(PALIAL_ERR_ARG, "nrhs=%d outside 1..16", nrhs);
+    if (!B || ldb < N) return fail(PALIAL_ERR_ARG, "bad right-hand side (ldb=%d)", ldb);
+    if (o.precision < 0 || o.precision > 2) return fail(PALIAL_ERR_ARG, "precision %d not available", o.precision);
+    const double tol = o.tol > 0 ? o.tol : 1e-14;
+    const int max_iter = o.max_iter > 0 ? o.max_iter : 30;
+    const size_t nn = (size_t)N * N, nv = (size_t)N * nrhs;
+    hipStream_t s = g.s_main;
+    if ((rc = ensure_events(2 * (size_t)MT + 4)) || (rc = ensure_dinv(b, MT))) return rc;
+    if ((rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
+    if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
+    if ((rc = grow(&g.d_vec, &g.vec_cap, 6 * nv + 2 * (size_t)N + 64))) return rc;
+    double *X = g.d_vec, *R = X + nv, *Y = R + nv, *D = Y + nv, *Bd = D + nv, *R2 = Bd + nv;
+    double *rowsum = R2 + nv, *red = rowsum + N;     // red: [0]=||A||, [1..]=col maxima
+    double *Uinv = g.d_inv, *Linv = g.d_inv + (size_t)MT * b * b;
+
+    // inputs on the device
+    double *dA = (double *)A.matrix;
+    bool ownA = false;
+    if (!o.device_matrix) {
+        if (hipMalloc(&dA, nn * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "device copy of A");
+        ownA = true;
+        HIPCHK(hipMemcpyAsync(dA, A.matrix, nn * 8, hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipMemcpy2DAsync(Bd, (size_t)N * 8, B, (size_t)ldb * 8, (size_t)N * 8, nrhs,
+                            o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    auto done = [&](int code) {
+        hipStreamSynchronize(s);
+        if (ownA) hipFree(dA);
+        hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
+        return code;
+    };
+    Launcher L;
+    HIPCHK(hipEventRecord(e0, s));
+    // factorization of a copy (A stays FP64 and intact for the residuals)
+    L.chk(launch_copy(g.d_F, dA, nn * 8, s));
+    hipMemsetAsync(g.d_info, 0, sizeof(int), s);
+    if (algo == 0) dag_cholesky(g.d_F, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision);
+    else dag_lu(g.d_F, N, b, MT > 1, s, g.s_panel, L, o.precision);
+    L.chk(launch_trtri(g.d_F, N, b, g.d_dinv, Uinv, true, s));
+    if (algo == 1) L.chk(launch_trtri(g.d_F, N, b, g.d_dinv, Linv, false, s));
+    HIPCHK(hipEventRecord(e1, s));
+    if (L.err != hipSuccess) return done(fail(PALIAL_ERR_HIP, "factorization launch: %s", hipGetErrorString(L.err)));
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    // ||A||_inf, ||b||_inf
+    L.chk(launch_norm_inf(dA, N, b, rowsum, red, s));
+    L.chk(launch_colmax(Bd, N, N, nrhs, red + 1, s));
+    // x0 = F^{-1} b
+    L.chk(launch_copy(R, Bd, nv * 8, s));
+    sweeps(algo, g.d_F, Uinv, Linv, R, Y, X, N, b, nrhs, s, L);
+    double hred[40];
+    HIPCHK(hipMemcpyAsync(hred, red, (1 + nrhs) * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info > 0) return done(fail(info, "factorization failed at column %d", info));
+    const double normA = hred[0];
+    double normB[16];
+    for (int j = 0; j < nrhs; j++) normB[j] = hred[1 + j];
+    double eta = 0.0;
+    int it = 0;
+    for (;; it++) {
+        // r = b - A x ; eta = max_j ||r_j|| / (||A|| ||x_j|| + ||b_j||)
+        L.chk(launch_residual(dA, N, b, X, Bd, R, N, nrhs, s));
+        L.chk(launch_colmax(R, N, N, nrhs, red + 1, s));
+        L.chk(launch_colmax(X, N, N, nrhs, red + 1 + nrhs, s));
+        HIPCHK(hipMemcpyAsync(hred, red + 1, 2 * nrhs * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (L.err != hipSuccess) return done(fail(PALIAL_ERR_HIP, "refinement launch: %s", hipGetErrorString(L.err)));
+        eta = 0.0;
+        for (int j = 0; j < nrhs; j++) {
+            const double den = normA * hred[nrhs + j] + normB[j];
+            const double ej = den > 0 ? hred[j] / den : 0.0;
+            if (!(ej <= eta)) eta = ej;      // NaN propagates
+        }
+        if (eta <= tol || it >= max_iter) break;
+        // d = F^{-1} r ; x += d
+        L.chk(launch_copy(R2, R, nv * 8, s));
+        sweeps(algo, g.d_F, Uinv, Linv, R2, Y, D, N, b, nrhs, s, L);
+        L.chk(launch_axpy(X, D, N, N, nrhs, s));
+    }
+    HIPCHK(hipEventRecord(e2, s));
+    HIPCHK(hipMemcpy2DAsync(B, (size_t)ldb * 8, X, (size_t)N * 8, (size_t)N * 8, nrhs,
+                            o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float fms = 0, sms = 0;
+    hipEventElapsedTime(&fms, e0, e1);
+    hipEventElapsedTime(&sms, e1, e2);
+    prof_collect();
+    if (rep) {
+        const double n = N;
+        rep->factor_ms = fms;
+        rep->solve_ms = sms;
+        rep->gflop = (algo == 0 ? n * n * n / 3 : 2 * n * n * n / 3) / 1e9;
+        rep->tflops = (algo == 0 ? n * n * n / 3 : 2 * n * n * n / 3) / ((fms + sms) * 1e-3) / 1e12;
+        rep->backward_error = eta;
+        rep->iterations = it;
+        rep->info = info;
+        rep->nlaunches = L.launches;
+    }
+    if (!(eta <= tol)) return done(fail(PALIAL_ERR_NOCONV, "refinement stopped at backward error %.3e after %d steps", eta, it));
+    return done(0);
+}
+
 // ------------------------------------------------------------------ run helper
 enum Algo { ALG_CHOL = 0, ALG_LU = 1, ALG_QR = 2 };
 
@@ -416,7 +585,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     const bool use_graph = !o.no_graph && !g_prof.on;
     if ((rc = ensure_events(2 * (size_t)MT + 4))) return rc;
     if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
-    if ((rc = ensure_dinv(b))) return rc;
+    if ((rc = ensure_dinv(b, MT))) return rc;
 
     double *dA = nullptr, *dS = nullptr;
     bool owned = false;
@@ -647,6 +816,9 @@ int palial_hip_finalize(void)
     if (g.d_w2) hipFree(g.d_w2);
     if (g.d_tau) hipFree(g.d_tau);
     if (g.d_dinv) hipFree(g.d_dinv);
+    if (g.d_F) hipFree(g.d_F);
+    if (g.d_inv) hipFree(g.d_inv);
+    if (g.d_vec) hipFree(g.d_vec);
     if (g.d_info) hipFree(g.d_info);
     hipStreamDestroy(g.s_main);
     hipStreamDestroy(g.s_panel);
@@ -796,6 +968,14 @@ int palial_hip_lu(MATRIX_desc A, int *ipiv, const palial_opts *opts, palial_repo
     return rc;
 }
 
+int palial_hip_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palial_opts *opts,
+                     palial_report *rep)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (algo != 0 && algo != 1) return fail(PALIAL_ERR_ARG, "algo must be 0 (Cholesky) or 1 (LU)");
+    return run_solve(algo, A, B, nrhs, ldb, opts, rep);
+}
+
 int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_report *rep)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -813,7 +993,7 @@ int palial_hip_dpotrf(int n, double *A, int lda, const palial_opts *opts)
     double *d = st.in(A, true);
     if (!d) return fail(PALIAL_ERR_NOMEM, "tile alloc");
     hipMemsetAsync(g.d_info, 0, sizeof(int), g.s_main);
-    if ((rc = ensure_dinv(n))) return rc;
+    if ((rc = ensure_dinv(n, 1))) return rc;
     if (panel2_ok(n)) HIPCHK(launch_potrf2(d, n, g.d_info, 0, g.d_dinv, g.s_main));
     else HIPCHK(launch_potrf(d, n, g.d_info, 0, g.s_main));
     int info = 0;
@@ -831,7 +1011,7 @@ int palial_hip_dgetrf(int n, double *A, int lda, const palial_opts *opts)
     Stage st((size_t)n * n * 8, opts && opts->device_matrix);
     double *d = st.in(A, true);
     if (!d) return fail(PALIAL_ERR_NOMEM, "tile alloc");
-    if ((rc = ensure_dinv(n))) return rc;
+    if ((rc = ensure_dinv(n, 1))) return rc;
     if (panel2_ok(n)) HIPCHK(launch_getrf2(d, n, g.d_dinv, g.d_dinv + (size_t)32 * n, g.s_main));
     else HIPCHK(launch_getrf(d, n, g.s_main));
     return st.finish();

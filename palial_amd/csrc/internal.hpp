@@ -89,6 +89,27 @@ hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *
 hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipStream_t s);
 hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s);
 
+// low-precision trailing updates (k_gemm_lp.hip): prec 1 = FP32, 2 = BF16
+hipError_t launch_gemm_lp(const GemmArgs &a, bool transA, bool transB, int prec, hipStream_t s);
+
+// solve path (k_solve.hip)
+enum GemvMode : int { GV_DIAG = 0, GV_CHOL_FWD = 1, GV_LU_FWD = 2, GV_BWD = 3 };
+struct GemvArgs {
+    int mode;
+    const double *mat;   // factored matrix (layout L)
+    const double *inv;   // inverses of the diagonal tiles, tile k at k*b*b
+    const double *vin;   // input vector block (N x nrhs, leading dimension ld)
+    double *vout;        // output vector block
+    int N, b, k, ld, nrhs, ntasks, trans;
+};
+hipError_t launch_trtri(const double *mat, int N, int b, const double *dinv, double *inv, bool upper, hipStream_t s);
+hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s);
+hipError_t launch_residual(const double *A, int N, int b, const double *x, const double *rhs, double *r, int ld,
+                           int nrhs, hipStream_t s);
+hipError_t launch_norm_inf(const double *A, int N, int b, double *rowsum, double *out, hipStream_t s);
+hipError_t launch_colmax(const double *v, int N, int ld, int nrhs, double *out, hipStream_t s);
+hipError_t launch_axpy(double *x, const double *d, int N, int ld, int nrhs, hipStream_t s);
+
 // QR panels
 hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s);
 hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s);

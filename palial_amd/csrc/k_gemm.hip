@@ -12,161 +12,13 @@
 //   B fragment: B[k = l / 16][col = l % 16]
 //   D (4 doubles): D[row = (l/16) + 4*v][col = l % 16], v = 0..3
 //   (the f64 variant interleaves rows across the four lane groups)
-#include "internal.hpp"
+#include "gemm_common.hpp"
 #include <cstring>
 #include <cstdlib>
 
 namespace palial {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ inline double apply_mask(int mask, double v, int r, int c)
-{
-    // r, c: coordinates inside the stored tile
-    if (mask == MK_UPPER) return (r <= c) ? v : 0.0;
-    if (mask == MK_UNIT_LOWER) return (r > c) ? v : ((r == c) ? 1.0 : 0.0);
-    return v;
-}
-
-struct TaskPtrs {
-    const double *A, *B, *D;
-    double *C, *E;
-    bool diag;   // SYRK diagonal tile: only the upper triangle is written
-    bool valid;
-};
-
-__host__ __device__ inline void tri_decode(int idx, int &n, int &m)
-{
-    // idx = m(m+1)/2 + n with 0 <= n <= m
-    int mm = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-    while ((mm + 1) * (mm + 2) / 2 <= idx) mm++;
-    while (mm * (mm + 1) / 2 > idx) mm--;
-    m = mm;
-    n = idx - mm * (mm + 1) / 2;
-}
-
-__host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
-{
-    TaskPtrs t;
-    t.A = t.B = t.D = nullptr;
-    t.C = t.E = nullptr;
-    t.diag = false;
-    t.valid = true;
-    const int b = p.b, MT = p.MT, k = p.k;
-    const size_t bb = (size_t)b * b;
-    double *M = p.mat;
-    switch (p.mode) {
-    case GM_SINGLE:
-        t.A = p.A; t.B = p.B; t.C = p.C; t.D = p.D; t.E = p.E;
-        t.diag = p.single_diag != 0;
-        break;
-    case GM_CHOL_TRAIL:
-    case GM_CHOL_REST: {
-        int n, m;
-        tri_decode(task, n, m);
-        int off = (p.mode == GM_CHOL_TRAIL) ? k + 1 : k + 2;
-        n += off; m += off;
-        t.A = M + tile_off(b, MT, k, n);
-        t.B = M + tile_off(b, MT, k, m);
-        t.C = M + tile_off(b, MT, n, m);
-        t.D = t.C;
-        t.diag = (n == m);
-        break;
-    }
-    case GM_CHOL_ROW: {
-        int n = k + 1, m = k + 1 + task;
-        t.A = M + tile_off(b, MT, k, n);
-        t.B = M + tile_off(b, MT, k, m);
-        t.C = M + tile_off(b, MT, n, m);
-        t.D = t.C;
-        t.diag = (n == m);
-        break;
-    }
-    case GM_LU_TRAIL:
-    case GM_LU_REST: {
-        // GM_LU_TRAIL covers T x T tiles from k+1, GM_LU_REST (T-1) x (T-1) from k+2
-        const int off = (p.mode == GM_LU_TRAIL) ? k + 1 : k + 2;
-        const int rows = (p.mode == GM_LU_TRAIL) ? p.T : p.T - 1;
-        int m = off + task % rows, n = off + task / rows;
-        t.A = M + tile_off(b, MT, m, k);
-        t.B = M + tile_off(b, MT, k, n);
-        t.C = M + tile_off(b, MT, m, n);
-        t.D = t.C;
-        break;
-    }
-    case GM_LU_PANEL: {
-        // tasks [0, T): column k+1, rows m = k+1..MT-1 ; [T, 2T-1): row k+1, cols n = k+2..
-        int m, n;
-        if (task < p.T) { m = k + 1 + task; n = k + 1; }
-        else { m = k + 1; n = k + 2 + (task - p.T); }
-        t.A = M + tile_off(b, MT, m, k);
-        t.B = M + tile_off(b, MT, k, n);
-        t.C = M + tile_off(b, MT, m, n);
-        t.D = t.C;
-        break;
-    }
-    case GM_QR_W: {
-        int n = k + 1 + task;
-        t.A = M + tile_off(b, MT, p.m, k);       // V (transposed)
-        t.B = M + tile_off(b, MT, p.m, n);       // A2
-        t.C = p.w1 + (size_t)task * bb;          // W
-        t.D = M + tile_off(b, MT, k, n);         // + A1
-        break;
-    }
-    case GM_QR_TW: {
-        int n = k + 1 + task;
-        t.A = p.smat + tile_off(b, MT, p.m, k);  // T (transposed, upper)
-        t.B = p.w1 + (size_t)task * bb;
-        t.C = p.w2 + (size_t)task * bb;
-        t.E = M + tile_off(b, MT, k, n);         // A1 -= T^T W
-        break;
-    }
-    case GM_QR_VW: {
-        int n = k + 1 + task;
-        t.A = M + tile_off(b, MT, p.m, k);       // V
-        t.B = p.w2 + (size_t)task * bb;
-        t.C = M + tile_off(b, MT, p.m, n);
-        t.D = t.C;
-        break;
-    }
-    case GM_ORM_W: {
-        int n = k + 1 + task;
-        t.A = M + tile_off(b, MT, k, k);         // V (transposed, unit lower)
-        t.B = M + tile_off(b, MT, k, n);
-        t.C = p.w1 + (size_t)task * bb;
-        break;
-    }
-    case GM_ORM_TW: {
-        t.A = p.smat + tile_off(b, MT, k, k);    // T (transposed, upper)
-        t.B = p.w1 + (size_t)task * bb;
-        t.C = p.w2 + (size_t)task * bb;
-        break;
-    }
-    case GM_ORM_VW: {
-        int n = k + 1 + task;
-        t.A = M + tile_off(b, MT, k, k);         // V (unit lower)
-        t.B = p.w2 + (size_t)task * bb;
-        t.C = M + tile_off(b, MT, k, n);
-        t.D = t.C;
-        break;
-    }
-    default:
-        t.valid = false;
-    }
-    return t;
-}
-
-// Workgroup index remap so that consecutive logical blocks run on the same
-// XCD (hardware dispatches workgroups round-robin over the 8 XCDs, each with
-// its own L2): blocks of one output tile then share that XCD's L2 copy of
-// their two operand tiles.
-__device__ inline int xcd_remap(int pid, int nwg)
-{
-    constexpr int NX = 8;
-    int q = nwg / NX, r = nwg % NX;
-    int xcd = pid % NX, local = pid / NX;
-    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
-}
 
 template <int BM, int KB, int MINB, bool TA, bool TB>
 __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)

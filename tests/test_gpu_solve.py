@@ -1,0 +1,56 @@
+"""GPU tests of the solve path (no reference counterpart: the reference only
+factors).  The FP64 oracle here is numpy's dense solve and the closed-form
+backward error; every precision tier must reach backward error <= 1e-14."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def spd_cond(N, kappa, seed):
+    """A = Q diag(s) Q^T, Q from the QR of a seeded Gaussian, s geometric from 1 to 1/kappa."""
+    r = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(r.standard_normal((N, N)))
+    s = np.geomspace(1.0, 1.0 / kappa, N)
+    return (Q * s) @ Q.T
+
+
+def backward_error(A, X, B):
+    R = B - A @ X
+    return max(np.abs(R[:, j]).max() / (np.abs(A).sum(1).max() * np.abs(X[:, j]).max() + np.abs(B[:, j]).max())
+               for j in range(B.shape[1]))
+
+
+@pytest.mark.parametrize("algo", ["cholesky", "lu"])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("N,b,nrhs", [(512, 128, 1), (1024, 256, 16), (768, 128, 3)])
+def test_solve_diag_dominant(hip, orc, algo, prec, N, b, nrhs):
+    a = orc.gen("sym" if algo == "cholesky" else "dense", N, b, seed=1, shift=N)
+    A = orc.to_dense(a, N, b)
+    B = np.random.default_rng(2).standard_normal((N, nrhs))
+    X, rep, rc = hip.solve(a, B, algo=algo, precision=prec, b=b)
+    assert rc == 0, (rc, rep.backward_error)
+    assert rep.backward_error <= 1e-14
+    assert backward_error(A, X, B) <= 2e-14
+    assert np.abs(X - np.linalg.solve(A, B)).max() / np.abs(X).max() < 1e-11
+    if prec == 0:
+        assert rep.iterations <= 1
+
+
+@pytest.mark.parametrize("prec,kappa", [(1, 1e2), (1, 1e5), (2, 1e2), (0, 1e8)])
+def test_solve_conditioned_spd(hip, orc, prec, kappa):
+    N, b = 512, 128
+    A = spd_cond(N, kappa, 5)
+    a = orc.from_dense(A, N, b)
+    B = np.random.default_rng(6).standard_normal((N, 1))
+    X, rep, rc = hip.solve(a, B, algo="cholesky", precision=prec, b=b, max_iter=60)
+    assert rc == 0, (rc, rep.backward_error, rep.iterations)
+    assert backward_error(A, X, B) <= 2e-14
+
+
+def test_solve_leaves_A_untouched(hip, orc):
+    N, b = 256, 64
+    a = orc.gen("sym", N, b, seed=3, shift=N)
+    keep = a.copy()
+    hip.solve(a, np.ones((N, 1)), algo="cholesky", precision=1, b=b)
+    assert np.array_equal(a, keep)
