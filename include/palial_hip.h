@@ -129,6 +129,13 @@ int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms)
 int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift,
                         const palial_opts *opts);
 
+/* A = A1 (x) A2 (Kronecker product; A1 n1 x n1, A2 n2 x n2 column-major host
+ * arrays, n1*n2 = N) written in layout L.  With A_i = Q_i D_i Q_i^T this is
+ * Q diag(s) Q^T for Q = Q1 (x) Q2 and s = diag(D1) (x) diag(D2): a geometric
+ * spectrum from 1 to 1/kappa factorises exactly this way. */
+int palial_hip_generate_kron(MATRIX_desc A, const double *A1, int n1, const double *A2, int n2,
+                             const palial_opts *opts);
+
 /* ---------------------------------------------------------------- algorithms */
 /* replaces void cholesky(MATRIX_desc A)   (src/cholesky.h:12)
  * Upper right-looking tiled Cholesky A = U^T U, in place; the strictly lower

@@ -98,6 +98,7 @@ def lib():
         L.palial_hip_memcpy_d2h.argtypes = [dp, dp, C.c_size_t]
         L.palial_hip_memcpy_d2d.argtypes = [dp, dp, C.c_size_t]
         L.palial_hip_copy.argtypes = [dp, dp, C.c_size_t]
+        L.palial_hip_generate_kron.argtypes = [MATRIX_desc, dp, C.c_int, dp, C.c_int, C.POINTER(Opts)]
         L.palial_hip_solve.argtypes = [C.c_int, MATRIX_desc, dp, C.c_int, C.c_int, C.POINTER(Opts),
                                        C.POINTER(Report)]
         L.palial_hip_profile_stats.argtypes = [C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double),
@@ -307,6 +308,13 @@ class DeviceMatrix:
         o = _opts(device=True)
         _check(lib().palial_hip_generate(self.desc, kind, seed, float(shift), C.byref(o)))
 
+    def generate_kron(self, A1, A2):
+        A1 = np.asfortranarray(A1, dtype=np.float64)
+        A2 = np.asfortranarray(A2, dtype=np.float64)
+        o = _opts(device=True)
+        _check(lib().palial_hip_generate_kron(self.desc, A1.ctypes.data, A1.shape[0], A2.ctypes.data, A2.shape[0],
+                                              C.byref(o)))
+
     def copy_from(self, other):
         _check(lib().palial_hip_copy(self.ptr, other.ptr, self.bytes))
 
@@ -386,3 +394,23 @@ def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, m
     if check and rc < 0:
         _check(rc)
     return X, rep, rc
+
+
+def kron_spd_factors(N, kappa, seed=0, n1=None):
+    """Factors of A = Q diag(s) Q^T with Q = Q1 (x) Q2 (Householder QR of seeded
+    Gaussians) and s geometric from 1 to 1/kappa: returns (A1, A2) with
+    A = A1 (x) A2.  s_k = r^k, k = k1*n2 + k2, splits as r^(n2 k1) * r^k2."""
+    if n1 is None:
+        n1 = 1 << ((N.bit_length() - 1) // 2)
+        while N % n1:
+            n1 //= 2
+    n2 = N // n1
+    rng = np.random.default_rng(seed)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n1, n1)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n2, n2)))
+    r = kappa ** (-1.0 / (N - 1)) if N > 1 else 1.0
+    d1 = r ** (n2 * np.arange(n1))
+    d2 = r ** np.arange(n2)
+    A1 = (Q1 * d1) @ Q1.T
+    A2 = (Q2 * d2) @ Q2.T
+    return (A1 + A1.T) / 2, (A2 + A2.T) / 2

@@ -944,6 +944,33 @@ int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift, co
     return 0;
 }
 
+int palial_hip_generate_kron(MATRIX_desc A, const double *A1, int n1, const double *A2, int n2,
+                             const palial_opts *o_in)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_desc(A))) return rc;
+    if (!A1 || !A2 || n1 <= 0 || n2 <= 0 || (long)n1 * n2 != A.matrix_size)
+        return fail(PALIAL_ERR_ARG, "n1*n2 must equal N");
+    palial_opts o = o_in ? *o_in : default_opts();
+    const int N = A.matrix_size, b = A.tile_size;
+    const size_t bytes = (size_t)N * N * 8;
+    double *d = (double *)A.matrix, *d1 = nullptr, *d2 = nullptr;
+    if (hipMalloc(&d1, (size_t)n1 * n1 * 8) != hipSuccess || hipMalloc(&d2, (size_t)n2 * n2 * 8) != hipSuccess)
+        return fail(PALIAL_ERR_NOMEM, "kron factors");
+    if (!o.device_matrix && hipMalloc(&d, bytes) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "device alloc");
+    hipError_t e = hipMemcpy(d1, A1, (size_t)n1 * n1 * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d2, A2, (size_t)n2 * n2 * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_kron(d, N, b, d1, n1, d2, n2, g.s_main);
+    if (e == hipSuccess && !o.device_matrix) e = hipMemcpyAsync(A.matrix, d, bytes, hipMemcpyDeviceToHost, g.s_main);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.s_main);
+    hipFree(d1);
+    hipFree(d2);
+    if (!o.device_matrix) hipFree(d);
+    if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "generate_kron: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int palial_hip_cholesky(MATRIX_desc A, const palial_opts *opts, palial_report *rep)
 {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -119,5 +119,6 @@ hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, d
                            hipStream_t s);
 hipError_t launch_fill_identity_ipiv(int *ipiv, int N, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2, hipStream_t s);
 
 }  // namespace palial

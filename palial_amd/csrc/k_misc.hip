@@ -81,6 +81,28 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
     return hipGetLastError();
 }
 
+// A(i,j) = A1(i / n2, j / n2) * A2(i % n2, j % n2): the Kronecker product of
+// two small dense column-major matrices, written in layout L.
+__global__ void kron_kernel(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2)
+{
+    const size_t total = (size_t)N * N;
+    const size_t bb = (size_t)b * b;
+    const int MT = N / b;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t tile = e / bb, within = e % bb;
+        const int r = (int)(within % b), c = (int)(within / b);
+        const int m = (int)(tile % MT), n = (int)(tile / MT);
+        const int i = m * b + r, j = n * b + c;
+        mat[e] = A1[(size_t)(i / n2) + (size_t)(j / n2) * n1] * A2[(size_t)(i % n2) + (size_t)(j % n2) * n2];
+    }
+}
+
+hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2, hipStream_t s)
+{
+    hipLaunchKernelGGL(kron_kernel, dim3(65536), dim3(256), 0, s, mat, N, b, A1, n1, A2, n2);
+    return hipGetLastError();
+}
+
 __global__ void iota_ipiv_kernel(int *ipiv, int N)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) ipiv[i] = i + 1;
