@@ -60,7 +60,8 @@ typedef struct palial_opts {
     int   refine;         /* solves: 0 = LU-IR, 1 = GMRES-IR, 2 = auto */
     int   max_iter;       /* solves: refinement steps (0 = 30) */
     double tol;           /* solves: target backward error (0 = 1e-14) */
-    int   reserved[8];
+    int   factor_tile;    /* solves: tile size of the internal factorization (0 = automatic) */
+    int   reserved[7];
 } palial_opts;
 
 /* per-call report; any pointer may be NULL */

@@ -42,7 +42,7 @@ class MATRIX_desc(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("device_matrix", C.c_int), ("precision", C.c_int), ("no_graph", C.c_int),
                 ("no_lookahead", C.c_int), ("stream", C.c_void_p), ("refine", C.c_int),
-                ("max_iter", C.c_int), ("tol", C.c_double), ("reserved", C.c_int * 8)]
+                ("max_iter", C.c_int), ("tol", C.c_double), ("factor_tile", C.c_int), ("reserved", C.c_int * 7)]
 
 
 class Report(C.Structure):
@@ -129,8 +129,9 @@ def finalize():
 
 
 def _opts(device=False, no_graph=False, no_lookahead=False, precision=PREC_FP64, stream=None,
-          refine=0, max_iter=0, tol=0.0):
+          refine=0, max_iter=0, tol=0.0, factor_tile=0):
     o = Opts()
+    o.factor_tile = factor_tile
     o.device_matrix = int(bool(device))
     o.no_graph = int(bool(no_graph))
     o.no_lookahead = int(bool(no_lookahead))
@@ -368,7 +369,7 @@ def factor_device(algo, dm, S=None, **kw):
 
 # ------------------------------------------------------------------ solve
 def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, max_iter=0, device=False,
-          check=True):
+          check=True, factor_tile=0, refine=0):
     """Solve A X = B with iterative refinement (no reference counterpart).
 
     A: layout-L matrix (numpy flat array, torch tensor or DeviceMatrix), left
@@ -379,7 +380,8 @@ def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, m
         d, dev = A.desc, True
     else:
         d, dev, keep = _desc_of(A, N, b)
-    o = _opts(device=bool(dev), precision=precision, tol=tol, max_iter=max_iter)
+    o = _opts(device=bool(dev), precision=precision, tol=tol, max_iter=max_iter, factor_tile=factor_tile,
+              refine=refine)
     rep = Report()
     code = {"cholesky": 0, "lu": 1}[algo]
     if dev and not isinstance(B, np.ndarray):

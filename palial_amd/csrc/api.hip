@@ -36,6 +36,8 @@ struct Ctx {
     int device = -1;
     hipStream_t s_main = nullptr;   // update stream (and the caller-visible stream)
     hipStream_t s_panel = nullptr;  // look-ahead panel stream
+    hipStream_t s_upd = nullptr;    // update stream of the solve path (s_main unless CUs are split)
+    int cu_split = 0;               // CUs reserved for the panel stream (0: no split)
     std::vector<hipEvent_t> events;
     int *d_info = nullptr;
     double *d_tau = nullptr;        // QR tau scratch (b doubles, overwritten per panel)
@@ -457,21 +459,29 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     int rc;
     if ((rc = ensure_init()) || (rc = check_desc(A))) return rc;
     palial_opts o = o_in ? *o_in : default_opts();
-    const int N = A.matrix_size, b = A.tile_size, MT = N / b;
-    if (!panel2_ok(b) || b % 64) return fail(PALIAL_ERR_ARG, "solve needs b %% 64 == 0 and b <= 512 (b=%d)", b);
+    const int N = A.matrix_size, bA = A.tile_size;
+    // tile size of the internal factorization: the copy of A that is factored
+    // is re-laid out, so it need not be the caller's b
+    int b = o.factor_tile > 0 ? o.factor_tile : (bA > 256 && N % 256 == 0 ? 256 : bA);
+    if (N % b) return fail(PALIAL_ERR_ARG, "factor tile %d does not divide N=%d", b, N);
+    const int MT = N / b;
+    if (!panel2_ok(b) || b % 64) return fail(PALIAL_ERR_ARG, "solve needs a factor tile with b %% 64 == 0, b <= 512 (b=%d)", b);
+    if (bA % 64) return fail(PALIAL_ERR_ARG, "solve needs the matrix tile size %% 64 == 0 (b=%d)", bA);
     if (nrhs < 1 || nrhs > 16) return fail(PALIAL_ERR_ARG, "nrhs=%d outside 1..16", nrhs);
     if (!B || ldb < N) return fail(PALIAL_ERR_ARG, "bad right-hand side (ldb=%d)", ldb);
     if (o.precision < 0 || o.precision > 2) return fail(PALIAL_ERR_ARG, "precision %d not available", o.precision);
     const double tol = o.tol > 0 ? o.tol : 1e-14;
     const int max_iter = o.max_iter > 0 ? o.max_iter : 30;
     const size_t nn = (size_t)N * N, nv = (size_t)N * nrhs;
-    hipStream_t s = g.s_main;
+    hipStream_t s = g.s_upd;
     if ((rc = ensure_events(2 * (size_t)MT + 4)) || (rc = ensure_dinv(b, MT))) return rc;
     if ((rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
     if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
-    if ((rc = grow(&g.d_vec, &g.vec_cap, 6 * nv + 2 * (size_t)N + 64))) return rc;
+    const size_t npart = residual_scratch(N, bA, nrhs);
+    if ((rc = grow(&g.d_vec, &g.vec_cap, 6 * nv + 2 * (size_t)N + 64 + npart))) return rc;
     double *X = g.d_vec, *R = X + nv, *Y = R + nv, *D = Y + nv, *Bd = D + nv, *R2 = Bd + nv;
     double *rowsum = R2 + nv, *red = rowsum + N;     // red: [0]=||A||, [1..]=col maxima
+    double *part = red + 64;
     double *Uinv = g.d_inv, *Linv = g.d_inv + (size_t)MT * b * b;
 
     // inputs on the device
@@ -497,7 +507,8 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     Launcher L;
     HIPCHK(hipEventRecord(e0, s));
     // factorization of a copy (A stays FP64 and intact for the residuals)
-    L.chk(launch_copy(g.d_F, dA, nn * 8, s));
+    if (b == bA) L.chk(launch_copy(g.d_F, dA, nn * 8, s));
+    else L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
     hipMemsetAsync(g.d_info, 0, sizeof(int), s);
     if (algo == 0) dag_cholesky(g.d_F, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision);
     else dag_lu(g.d_F, N, b, MT > 1, s, g.s_panel, L, o.precision);
@@ -508,7 +519,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     int info = 0;
     HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
     // ||A||_inf, ||b||_inf
-    L.chk(launch_norm_inf(dA, N, b, rowsum, red, s));
+    L.chk(launch_norm_inf(dA, N, bA, rowsum, red, s));
     L.chk(launch_colmax(Bd, N, N, nrhs, red + 1, s));
     // x0 = F^{-1} b
     L.chk(launch_copy(R, Bd, nv * 8, s));
@@ -524,7 +535,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     int it = 0;
     for (;; it++) {
         // r = b - A x ; eta = max_j ||r_j|| / (||A|| ||x_j|| + ||b_j||)
-        L.chk(launch_residual(dA, N, b, X, Bd, R, N, nrhs, s));
+        L.chk(launch_residual(dA, N, bA, X, Bd, R, N, nrhs, part, s));
         L.chk(launch_colmax(R, N, N, nrhs, red + 1, s));
         L.chk(launch_colmax(X, N, N, nrhs, red + 1 + nrhs, s));
         HIPCHK(hipMemcpyAsync(hred, red + 1, 2 * nrhs * 8, hipMemcpyDeviceToHost, s));
@@ -791,7 +802,22 @@ int palial_hip_init(int device)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(PALIAL_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
     HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
+    // PALIAL_CU_SPLIT=n: the solve path's panel stream runs on n reserved CUs
+    // and its update stream on the others, so panel kernels never wait for
+    // update workgroups to drain a CU (graph replays do not keep CU masks)
+    const char *split = getenv("PALIAL_CU_SPLIT");
+    g.cu_split = split ? atoi(split) : 0;
+    if (g.cu_split > 0 && g.cu_split < prop.multiProcessorCount) {
+        const int ncu = prop.multiProcessorCount, words = (ncu + 31) / 32;
+        std::vector<uint32_t> mp(words, 0), mu(words, 0);
+        for (int cu = 0; cu < ncu; cu++) (cu < g.cu_split ? mp : mu)[cu / 32] |= 1u << (cu % 32);
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_panel, words, mp.data()));
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_upd, words, mu.data()));
+    } else {
+        g.cu_split = 0;
+        HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
+        g.s_upd = g.s_main;
+    }
     HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
     panel_init();
     panel2_init();
@@ -820,6 +846,7 @@ int palial_hip_finalize(void)
     if (g.d_inv) hipFree(g.d_inv);
     if (g.d_vec) hipFree(g.d_vec);
     if (g.d_info) hipFree(g.d_info);
+    if (g.s_upd && g.s_upd != g.s_main) hipStreamDestroy(g.s_upd);
     hipStreamDestroy(g.s_main);
     hipStreamDestroy(g.s_panel);
     g = Ctx();

@@ -105,7 +105,8 @@ struct GemvArgs {
 hipError_t launch_trtri(const double *mat, int N, int b, const double *dinv, double *inv, bool upper, hipStream_t s);
 hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s);
 hipError_t launch_residual(const double *A, int N, int b, const double *x, const double *rhs, double *r, int ld,
-                           int nrhs, hipStream_t s);
+                           int nrhs, double *part, hipStream_t s);
+size_t residual_scratch(int N, int b, int nrhs);   // doubles of `part` launch_residual needs
 hipError_t launch_norm_inf(const double *A, int N, int b, double *rowsum, double *out, hipStream_t s);
 hipError_t launch_colmax(const double *v, int N, int ld, int nrhs, double *out, hipStream_t s);
 hipError_t launch_axpy(double *x, const double *d, int N, int ld, int nrhs, hipStream_t s);
@@ -119,6 +120,7 @@ hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, d
                            hipStream_t s);
 hipError_t launch_fill_identity_ipiv(int *ipiv, int N, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
+hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N, hipStream_t s);
 hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2, hipStream_t s);
 
 }  // namespace palial

@@ -103,6 +103,27 @@ hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, cons
     return hipGetLastError();
 }
 
+// Copy an N x N matrix from layout L with tile size b1 to layout L with tile
+// size b2 (the destination is walked linearly).
+__global__ void relayout_kernel(double *__restrict__ dst, int b2, const double *__restrict__ src, int b1, int N)
+{
+    const size_t total = (size_t)N * N;
+    const size_t bb2 = (size_t)b2 * b2;
+    const int MT1 = N / b1, MT2 = N / b2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t tile = e / bb2, within = e % bb2;
+        const int i = (int)(tile % MT2) * b2 + (int)(within % b2);
+        const int j = (int)(tile / MT2) * b2 + (int)(within / b2);
+        dst[e] = src[tile_off(b1, MT1, i / b1, j / b1) + (size_t)(i % b1) + (size_t)(j % b1) * b1];
+    }
+}
+
+hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N, hipStream_t s)
+{
+    hipLaunchKernelGGL(relayout_kernel, dim3(65536), dim3(256), 0, s, dst, b2, src, b1, N);
+    return hipGetLastError();
+}
+
 __global__ void iota_ipiv_kernel(int *ipiv, int N)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) ipiv[i] = i + 1;

@@ -42,6 +42,14 @@ __device__ inline d4 mfma16(double a, double b, d4 c)
 
 __device__ inline d4 zero4() { return (d4){0.0, 0.0, 0.0, 0.0}; }
 
+// value of v held by lane `l` of the wave (broadcast through SGPRs)
+__device__ inline double readlane_d(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
 // triangular index e -> (bi <= bj), e = bj(bj+1)/2 + bi
 __device__ inline void tri_ij(int e, int &bi, int &bj)
 {
@@ -128,54 +136,51 @@ __global__ __launch_bounds__(PF_THREADS) void potrf2_kernel(double *A, int b, in
     extern __shared__ double sP[];        // NB x LDP: sP[q*LDP + c], c = absolute column
     __shared__ double sU[NB * LDD];
     __shared__ double sI[NB * LDD];
-    double *sT = sP;                      // inverse scratch: the panel is not live yet
     __shared__ int s_fail;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_fail = 0;
     __syncthreads();
 
     for (int j0 = 0; j0 < b; j0 += NB) {
-        // 1. diagonal block, one element pair (rows r0, r0+16 of column c) per
-        //    thread; right-looking Cholesky through LDS (LAPACK dpotf2 order:
-        //    sqrt of the pivot, row scaled by its reciprocal, rank-1 update)
-        {
-            const bool own = t < 16 * NB;
-            const int c = t / 16, r0 = t % 16;
-            double v0 = 0.0, v1 = 0.0;
-            if (own) {
-                v0 = (r0 <= c) ? A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b] : 0.0;
-                v1 = (r0 + 16 <= c) ? A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b] : 0.0;
-                sU[r0 + c * LDD] = v0;
-                sU[r0 + 16 + c * LDD] = v1;
-            }
-            __syncthreads();
+        // 1. diagonal block: staged in LDS (upper part, zeros below), factored
+        //    by wave 0 alone with one lane per column (dpotf2 order: sqrt of
+        //    the pivot, row scaled by its reciprocal, rank-1 update); the
+        //    single wave orders its own LDS accesses, so no barriers inside
+        for (int e = t; e < NB * NB; e += PF_THREADS) {
+            const int r = e % NB, c = e / NB;
+            sU[r + c * LDD] = (r <= c) ? A[(size_t)(j0 + r) + (size_t)(j0 + c) * b] : 0.0;
+        }
+        __syncthreads();
+        if (w == 0) {
+            const int c = lane;
             for (int j = 0; j < NB; j++) {
                 const double piv = sU[j + j * LDD];
-                if (!(piv > 0.0)) {               // not positive definite: LAPACK info = j+1
-                    if (t == 0) { s_fail = 1; if (info) *info = info_base + j0 + j + 1; }
+                if (!(piv > 0.0)) {
+                    if (lane == 0) { s_fail = 1; if (info) *info = info_base + j0 + j + 1; }
                     break;
                 }
                 const double d = sqrt(piv), rd = 1.0 / d;
-                if (own && c >= j) {              // row j of U: U(j,j) = d, U(j,c) = a(j,c)/d
-                    if (r0 == j) { v0 = (c == j) ? d : v0 * rd; sU[r0 + c * LDD] = v0; }
-                    if (r0 + 16 == j) { v1 = (c == j) ? d : v1 * rd; sU[r0 + 16 + c * LDD] = v1; }
+                __builtin_amdgcn_wave_barrier();
+                if (c == j) sU[j + j * LDD] = d;
+                else if (c > j && c < NB) sU[j + c * LDD] *= rd;
+                __builtin_amdgcn_s_waitcnt(0xc07f);
+                __builtin_amdgcn_wave_barrier();
+                if (c > j && c < NB) {
+                    const double ujc = sU[j + c * LDD];
+                    for (int r = j + 1; r <= c; r++) sU[r + c * LDD] -= sU[j + r * LDD] * ujc;
                 }
-                __syncthreads();
-                if (own && c > j) {               // a(r,c) -= U(j,r) U(j,c) for j < r <= c
-                    const double u = sU[j + c * LDD];
-                    if (r0 > j && r0 <= c) { v0 -= sU[j + r0 * LDD] * u; sU[r0 + c * LDD] = v0; }
-                    if (r0 + 16 > j && r0 + 16 <= c) { v1 -= sU[j + (r0 + 16) * LDD] * u; sU[r0 + 16 + c * LDD] = v1; }
-                }
-                __syncthreads();
-            }
-            if (own) {
-                if (r0 <= c) A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b] = v0;
-                if (r0 + 16 <= c) A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b] = v1;
+                __builtin_amdgcn_s_waitcnt(0xc07f);
+                __builtin_amdgcn_wave_barrier();
             }
         }
         __syncthreads();
+        for (int e = t; e < NB * NB; e += PF_THREADS) {
+            const int r = e % NB, c = e / NB;
+            if (r <= c) A[(size_t)(j0 + r) + (size_t)(j0 + c) * b] = sU[r + c * LDD];
+        }
         if (s_fail) return;
-        tri_inverse32<true>(sU, sI, sT, PF_THREADS);
+        // 2. U_D^{-1} by recursive doubling (all threads), kept in LDS and in dinv
+        tri_inverse32<true>(sU, sI, sP, PF_THREADS);
         if (dinv)
             for (int e = t; e < NB * NB; e += PF_THREADS)
                 dinv[(size_t)(j0 / NB) * NB * NB + e] = sI[(e % NB) + (e / NB) * LDD];

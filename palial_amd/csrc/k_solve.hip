@@ -9,7 +9,8 @@
 //   tile_gemv      : batched out -= op(T) v (or out = op(T) v) over a list
 //                    of b x b tiles of layout L, nrhs <= 16 right-hand
 //                    sides; the two launches of one sweep step.
-//   residual_kernel: r = b - A x over the FP64 layout-L matrix (HBM-bound).
+//   residual       : r = b - A x over the FP64 layout-L matrix (HBM-bound),
+//                    chunked partial sums + a deterministic reduction.
 //   rowsum_abs     : row sums of |A| (for ||A||_inf).
 //   colmax_abs     : per-column max |v| of an N x nrhs block.
 //   axpy_cols      : x += d.
@@ -98,145 +99,198 @@ hipError_t launch_trtri(const double *mat, int N, int b, const double *dinv, dou
 }
 
 // ---------------------------------------------------------------- tile GEMV
-// For each task t: out_t(0:b, 0:nrhs) (+)= sign * op(T_t) v(0:b, 0:nrhs).
-// Tiles and vector segments are addressed from a small task description.
+// For each task t: out_t(0:b, 0:nrhs) = op(T_t) v  (GV_DIAG) or
+// out_t -= op(T_t) v (updates).  A workgroup (256 threads) owns 16 output
+// rows of one tile; the input segment (b x nrhs) sits in LDS.
+//   op = N: out(r) = sum_c T(r, c) v(c): 16 row lanes x 16 column groups,
+//           a column of 16 rows is one 128-byte read
+//   op = T: out(r) = sum_c T(c, r) v(c): a wave owns 4 rows; its 64 lanes
+//           read 64 consecutive c of a row (512 bytes) and reduce by shuffles
 constexpr int GV_THREADS = 256;
-constexpr int GV_ROWS = 64;          // output rows per workgroup
+constexpr int GV_ROWS = 16;          // output rows per workgroup
 constexpr int GV_MAXR = 16;          // right-hand sides
 
+template <int NR>
 __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 {
-    __shared__ double sv[512 * GV_MAXR];            // input segment, b <= 512
-    __shared__ double sred[4][GV_ROWS * GV_MAXR];
-    const int b = p.b, nrhs = p.nrhs;
+    extern __shared__ double sm[];
+    double *sv = sm;                          // b x NR input segment: sv[c + j*b]
+    double *sred = sm + (size_t)p.b * NR;     // 16 groups x 16 rows x NR partial sums
+    const int b = p.b;
     const int rb_per_tile = b / GV_ROWS;
     const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
     const int t = threadIdx.x;
-    // task -> (tile pointer, input segment index, output segment index)
     const double *T;
     int vin, vout;
     {
         const int MT = p.N / b, k = p.k;
         switch (p.mode) {
-        case GV_DIAG:           // out_k = op(Minv_k) in_k
-            T = p.inv + (size_t)k * b * b; vin = k; vout = k; break;
-        case GV_CHOL_FWD:       // r_m -= U_km^T y_k, m = k+1+task
-            T = p.mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
-        case GV_LU_FWD:         // r_m -= L_mk y_k
-            T = p.mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
-        default:                // GV_BWD: y_j -= U_jk x_k, j = task < k
-            T = p.mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
+        case GV_DIAG: T = p.inv + (size_t)k * b * b; vin = k; vout = k; break;
+        case GV_CHOL_FWD: T = p.mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
+        case GV_LU_FWD: T = p.mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
+        default: T = p.mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
         }
     }
     const double *vi = p.vin + (size_t)vin * b;
     double *vo = p.vout + (size_t)vout * b;
-    for (int e = t; e < b * nrhs; e += GV_THREADS) {
+    for (int e = t; e < b * NR; e += GV_THREADS) {
         const int i = e % b, c = e / b;
-        sv[i + c * 512] = vi[(size_t)i + (size_t)c * p.ld];
+        sv[e] = (c < p.nrhs) ? vi[(size_t)i + (size_t)c * p.ld] : 0.0;   // padded right-hand sides are zero
     }
     __syncthreads();
     const int r0 = rb * GV_ROWS;
-    double acc[GV_MAXR];
+    double acc[NR];
 #pragma unroll
-    for (int c = 0; c < GV_MAXR; c++) acc[c] = 0.0;
-    const int q = t / GV_ROWS;           // K quarter
+    for (int j = 0; j < NR; j++) acc[j] = 0.0;
     if (!p.trans) {
-        // out(r) = sum_c T(r0 + r, c) v(c): thread (r, quarter), columns contiguous in r
-        const int r = t % GV_ROWS;
-        const int cb = q * (b / 4), ce = cb + b / 4;
+        const int r = t % GV_ROWS, g = t / GV_ROWS;       // g: column group, columns c = g, g+16, ...
+        const double *Tr = T + r0 + r;
 #pragma unroll 8
-        for (int c = cb; c < ce; c++) {
-            const double a = T[(size_t)(r0 + r) + (size_t)c * b];
+        for (int c = g; c < b; c += 16) {
+            const double a = Tr[(size_t)c * b];
 #pragma unroll
-            for (int j = 0; j < GV_MAXR; j++)
-                if (j < nrhs) acc[j] += a * sv[c + j * 512];
+            for (int j = 0; j < NR; j++) acc[j] += a * sv[c + j * b];
         }
-        for (int j = 0; j < nrhs; j++) sred[q][r + j * GV_ROWS] = acc[j];
+#pragma unroll
+        for (int j = 0; j < NR; j++) sred[(g * GV_ROWS + r) * NR + j] = acc[j];
+        __syncthreads();
+        if (t < GV_ROWS * NR && t / GV_ROWS < p.nrhs) {
+            const int rr = t % GV_ROWS, j = t / GV_ROWS;
+            double s = 0.0;
+            for (int q = 0; q < 16; q++) s += sred[(q * GV_ROWS + rr) * NR + j];
+            double *o = vo + (size_t)(r0 + rr) + (size_t)j * p.ld;
+            if (p.mode == GV_DIAG) *o = s;
+            else *o -= s;
+        }
     } else {
-        // out(r) = sum_c T(c, r0 + r) v(c): a wave walks 16 output rows, lanes split c
         const int lane = t & 63, wv = t >> 6;
-        for (int rr = 0; rr < GV_ROWS / 4; rr++) {
-            const int r = wv * (GV_ROWS / 4) + rr;
-            double part[GV_MAXR];
+        double part[4][NR];
 #pragma unroll
-            for (int j = 0; j < GV_MAXR; j++) part[j] = 0.0;
-            for (int c = lane; c < b; c += 64) {
-                const double a = T[(size_t)c + (size_t)(r0 + r) * b];
+        for (int x = 0; x < 4; x++)
 #pragma unroll
-                for (int j = 0; j < GV_MAXR; j++)
-                    if (j < nrhs) part[j] += a * sv[c + j * 512];
-            }
+            for (int j = 0; j < NR; j++) part[x][j] = 0.0;
+        for (int c0 = 0; c0 < b; c0 += 64) {
+            const int c = c0 + lane;
+            double a[4];
 #pragma unroll
-            for (int j = 0; j < GV_MAXR; j++) {
-                if (j >= nrhs) break;
-                double v = part[j];
+            for (int x = 0; x < 4; x++) a[x] = T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int j = 0; j < NR; j++) part[x][j] += a[x] * sv[c + j * b];
+        }
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int j = 0; j < NR; j++) {
+                double v = part[x][j];
 #pragma unroll
                 for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-                if (lane == 0) sred[0][r + j * GV_ROWS] = v;
+                part[x][j] = v;
             }
+        if (lane < 4 * NR && lane % NR < p.nrhs) {
+            const int x = lane / NR, j = lane % NR;
+            double s = 0.0;
+#pragma unroll
+            for (int xx = 0; xx < 4; xx++)
+#pragma unroll
+                for (int jj = 0; jj < NR; jj++)
+                    if (xx == x && jj == j) s = part[xx][jj];
+            double *o = vo + (size_t)(r0 + wv * 4 + x) + (size_t)j * p.ld;
+            if (p.mode == GV_DIAG) *o = s;
+            else *o -= s;
         }
-        for (int e = t; e < GV_ROWS * nrhs; e += GV_THREADS) { sred[1][e] = 0.0; sred[2][e] = 0.0; sred[3][e] = 0.0; }
-    }
-    __syncthreads();
-    for (int e = t; e < GV_ROWS * nrhs; e += GV_THREADS) {
-        const int r = e % GV_ROWS, j = e / GV_ROWS;
-        const double s = sred[0][e] + sred[1][e] + sred[2][e] + sred[3][e];
-        double *o = vo + (size_t)(r0 + r) + (size_t)j * p.ld;
-        if (p.mode == GV_DIAG) *o = s;
-        else *o -= s;
     }
 }
 
 hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
 {
     if (a.ntasks <= 0) return hipSuccess;
-    if (a.b > 512 || a.b % GV_ROWS || a.nrhs > GV_MAXR || a.nrhs < 1) return hipErrorInvalidValue;
+    if (a.b > 512 || a.b % 64 || a.nrhs > GV_MAXR || a.nrhs < 1) return hipErrorInvalidValue;
     dim3 grid((unsigned)(a.ntasks * (a.b / GV_ROWS))), block(GV_THREADS);
-    hipLaunchKernelGGL(tile_gemv_kernel, grid, block, 0, s, a);
+    auto lds = [&](int nr) { return ((size_t)a.b * nr + 16 * GV_ROWS * nr) * 8; };
+    if (a.nrhs == 1) hipLaunchKernelGGL(tile_gemv_kernel<1>, grid, block, lds(1), s, a);
+    else if (a.nrhs <= 4) hipLaunchKernelGGL(tile_gemv_kernel<4>, grid, block, lds(4), s, a);
+    else hipLaunchKernelGGL(tile_gemv_kernel<16>, grid, block, lds(16), s, a);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- residual
-// r = b - A x for the FP64 layout-L matrix; a workgroup owns 64 rows of one
-// tile row and walks every column (4 column phases per workgroup).
-__global__ __launch_bounds__(256) void residual_kernel(const double *A, int N, int bt, const double *x,
-                                                       const double *rhs, double *r, int ld, int nrhs)
+// r = b - A x for the FP64 layout-L matrix in two deterministic passes:
+// partial sums over chunks of RES_CHUNK tile columns, then the reduction.
+constexpr int RES_CHUNK = 8;
+
+template <int NR>
+__global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, int N, int bt, const double *x,
+                                                               int ld, int nrhs, double *part)
 {
-    __shared__ double sred[4][64 * GV_MAXR];
+    __shared__ double sred[4][64 * NR];
     const int MT = N / bt;
     const int rows_per = bt / 64;
-    const int m = blockIdx.x / rows_per, r0 = (blockIdx.x % rows_per) * 64;
+    const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
+    const int rowblk = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    const int m = rowblk / rows_per, r0 = (rowblk % rows_per) * 64;
     const int t = threadIdx.x, rr = t % 64, q = t / 64;
-    double acc[GV_MAXR];
+    double acc[NR];
 #pragma unroll
-    for (int j = 0; j < GV_MAXR; j++) acc[j] = 0.0;
-    for (int n = 0; n < MT; n++) {
+    for (int j = 0; j < NR; j++) acc[j] = 0.0;
+    const int n1 = min(MT, (chunk + 1) * RES_CHUNK);
+    for (int n = chunk * RES_CHUNK; n < n1; n++) {
         const double *T = A + tile_off(bt, MT, m, n) + r0 + rr;
         const double *xs = x + (size_t)n * bt;
 #pragma unroll 8
         for (int c = q; c < bt; c += 4) {
             const double a = T[(size_t)c * bt];
 #pragma unroll
-            for (int j = 0; j < GV_MAXR; j++)
+            for (int j = 0; j < NR; j++)
                 if (j < nrhs) acc[j] += a * xs[(size_t)c + (size_t)j * ld];
         }
     }
-    for (int j = 0; j < nrhs; j++) sred[q][rr + j * 64] = acc[j];
+#pragma unroll
+    for (int j = 0; j < NR; j++) sred[q][rr + j * 64] = acc[j];
     __syncthreads();
     for (int e = t; e < 64 * nrhs; e += 256) {
         const int i = e % 64, j = e / 64;
-        const size_t o = (size_t)m * bt + r0 + i + (size_t)j * ld;
-        r[o] = rhs[o] - (sred[0][e] + sred[1][e] + sred[2][e] + sred[3][e]);
+        // part layout: [chunk][j][row]
+        part[((size_t)chunk * NR + j) * N + (size_t)m * bt + r0 + i] = sred[0][e] + sred[1][e] + sred[2][e] + sred[3][e];
+    }
+}
+
+template <int NR>
+__global__ void residual_reduce_kernel(const double *part, int nchunk, const double *rhs, double *r, int N, int ld,
+                                       int nrhs)
+{
+    const size_t total = (size_t)N * nrhs;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e % N, j = e / N;
+        double s = 0.0;
+        for (int c = 0; c < nchunk; c++) s += part[((size_t)c * NR + j) * N + i];
+        r[i + j * ld] = rhs[i + j * ld] - s;
     }
 }
 
 hipError_t launch_residual(const double *A, int N, int b, const double *x, const double *rhs, double *r, int ld,
-                           int nrhs, hipStream_t s)
+                           int nrhs, double *part, hipStream_t s)
 {
     if (b % 64 || nrhs > GV_MAXR || nrhs < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((N / b) * (b / 64))), dim3(256), 0, s, A, N, b, x, rhs, r, ld,
-                       nrhs);
+    const int MT = N / b;
+    const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
+    dim3 grid((unsigned)(MT * (b / 64) * nchunk)), block(256);
+    if (nrhs == 1) {
+        hipLaunchKernelGGL(residual_partial_kernel<1>, grid, block, 0, s, A, N, b, x, ld, nrhs, part);
+        hipLaunchKernelGGL(residual_reduce_kernel<1>, dim3(1024), dim3(256), 0, s, part, nchunk, rhs, r, N, ld, nrhs);
+    } else {
+        hipLaunchKernelGGL(residual_partial_kernel<16>, grid, block, 0, s, A, N, b, x, ld, nrhs, part);
+        hipLaunchKernelGGL(residual_reduce_kernel<16>, dim3(1024), dim3(256), 0, s, part, nchunk, rhs, r, N, ld, nrhs);
+    }
     return hipGetLastError();
+}
+
+size_t residual_scratch(int N, int b, int nrhs)
+{
+    const int MT = N / b;
+    const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
+    return (size_t)nchunk * (nrhs == 1 ? 1 : 16) * N;
 }
 
 // ||A||_inf: row sums of |A|, then max (two passes)

@@ -54,3 +54,16 @@ def test_solve_leaves_A_untouched(hip, orc):
     keep = a.copy()
     hip.solve(a, np.ones((N, 1)), algo="cholesky", precision=1, b=b)
     assert np.array_equal(a, keep)
+
+
+@pytest.mark.parametrize("algo", ["cholesky", "lu"])
+@pytest.mark.parametrize("b,ft", [(256, 128), (128, 256), (256, 64)])
+def test_solve_factor_tile_relayout(hip, orc, algo, b, ft):
+    """The solve factors a re-laid-out copy with its own tile size."""
+    N = 1024
+    a = orc.gen("sym" if algo == "cholesky" else "dense", N, b, seed=4, shift=N)
+    A = orc.to_dense(a, N, b)
+    B = np.random.default_rng(7).standard_normal((N, 2))
+    for prec in (0, 2):
+        X, rep, rc = hip.solve(a, B, algo=algo, precision=prec, b=b, factor_tile=ft)
+        assert rc == 0 and backward_error(A, X, B) <= 2e-14
