@@ -119,6 +119,15 @@ int palial_hip_task_tiles(int mode, int MT, int b, int k, int m, long *out, int 
  * launches on a device-resident matrix (its contents are overwritten). */
 int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms);
 
+/* Test hook (no counterpart in the reference): applies the trailing update of
+ * step k (algo 0 Cholesky: A_ij -= A_ki^T A_kj on the upper triangle, 1 LU:
+ * A_ij -= A_ik A_kj, all tiles i, j > k) reps times to a device-resident
+ * matrix.  prec 0 uses FP64 operands; 1 (FP32) and 2 (BF16) round the panel
+ * of step k once into packed K-contiguous copies and accumulate in FP32, as
+ * the mixed-precision solves do (b % 128 == 0).  *ms: average time per rep
+ * including the packing. */
+int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double *ms);
+
 /* ---------------------------------------------------------------- inputs */
 /* Seeded counter-based generators writing layout L (host or device per opts):
  *   kind 0: S + shift*I, S symmetric, S_ij = U(seed, min(i,j), max(i,j))

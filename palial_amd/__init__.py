@@ -103,6 +103,7 @@ def lib():
                                        C.POINTER(Report)]
         L.palial_hip_profile_stats.argtypes = [C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.palial_hip_update.argtypes = [C.c_int, MATRIX_desc, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -343,7 +344,7 @@ def profile(enable):
     _check(lib().palial_hip_profile(int(bool(enable))))
 
 
-KCLASS = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4}
+KCLASS = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4, "pack": 5}
 
 
 def profile_stats(cls):
@@ -351,6 +352,14 @@ def profile_stats(cls):
     _check(lib().palial_hip_profile_stats(KCLASS.get(cls, cls), C.byref(n), C.byref(ms), C.byref(fl),
                                           C.byref(by)))
     return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
+
+def update(algo, dm, k, precision=PREC_FP64, reps=1):
+    """Apply the trailing update of step k to a DeviceMatrix reps times (test
+    hook); returns the average time per rep in ms."""
+    ms = C.c_double()
+    _check(lib().palial_hip_update(0 if algo == "cholesky" else 1, dm.desc, k, precision, reps, C.byref(ms)))
+    return ms.value
 
 
 def factor_device(algo, dm, S=None, **kw):

@@ -3,11 +3,17 @@
 // The reference runs each algorithm as an OpenMP task DAG created by one
 // master thread (palial_src/main.c:124-159).  Here each DAG step becomes a
 // handful of batched launches on two HIP streams:
-//   panel stream : look-ahead update of the next panel, then its diagonal
-//                  factorization and panel TRSMs;
-//   update stream: the rest of the trailing update of the current step.
-// Cross-stream dependences are events.  The whole sequence is captured once
-// into a hipGraph per (algorithm, N, b, device pointer) and replayed.
+//   chain stream : the diagonal chain -- factorization of tile (k,k), TRSM of
+//                  tile (k,k+1), update of tile (k+1,k+1) by row k, then the
+//                  factorization of tile (k+1,k+1);
+//   bulk stream  : the TRSMs of the rest of panel k and the trailing update of
+//                  every other tile, on CUs masked away from a few reserved
+//                  CUs, so the chain's single-workgroup kernels never wait for
+//                  a CU to drain.
+// Cross-stream dependences are events.  Without a CU reservation the sequence
+// is captured once into a hipGraph per (algorithm, N, b, device pointer) and
+// replayed (graph replays do not keep stream CU masks, so a reserved schedule
+// is launched eagerly).
 #include "internal.hpp"
 #include "../../include/palial_hip.h"
 
@@ -35,9 +41,11 @@ struct Ctx {
     bool init = false;
     int device = -1;
     hipStream_t s_main = nullptr;   // update stream (and the caller-visible stream)
-    hipStream_t s_panel = nullptr;  // look-ahead panel stream
-    hipStream_t s_upd = nullptr;    // update stream of the solve path (s_main unless CUs are split)
-    int cu_split = 0;               // CUs reserved for the panel stream (0: no split)
+    hipStream_t s_panel = nullptr;  // look-ahead panel / diagonal-chain stream
+    hipStream_t s_bulk = nullptr;   // bulk stream of the chain schedule: CU-masked away from
+                                    // the reserved CUs (s_main when nothing is reserved)
+    int reserve = 0;                // CUs kept free of bulk work for the diagonal chain
+    int dag = 1;                    // 1: diagonal-chain schedule, 0: row look-ahead
     std::vector<hipEvent_t> events;
     int *d_info = nullptr;
     double *d_tau = nullptr;        // QR tau scratch (b doubles, overwritten per panel)
@@ -46,6 +54,8 @@ struct Ctx {
     size_t w_cap = 0;
     double *d_dinv = nullptr;       // inverses of the 32x32 diagonal blocks (U then L)
     size_t dinv_cap = 0;
+    void *d_pack = nullptr;         // low-precision packed panels of one step (U then L)
+    size_t pack_cap = 0;            // bytes
     // solve workspace: factor copy F, diagonal-tile inverses, vectors
     double *d_F = nullptr;
     size_t F_cap = 0;
@@ -147,6 +157,22 @@ int ensure_dinv(int b, int MT)
     return 0;
 }
 
+// packed low-precision panel operands of one step: row panel at 0, column
+// panel at MT*b*b*4 bytes (4 bytes per element covers FP32 and BF16)
+int ensure_pack(int b, int MT)
+{
+    const size_t need = 2 * (size_t)MT * b * b * 4;
+    if (g.pack_cap < need) {
+        drop_graphs();
+        if (g.d_pack) hipFree(g.d_pack);
+        g.d_pack = nullptr;
+        g.pack_cap = 0;
+        if (hipMalloc(&g.d_pack, need) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "packed panel alloc failed");
+        g.pack_cap = need;
+    }
+    return 0;
+}
+
 int check_desc(const MATRIX_desc &A)
 {
     if (!A.matrix) return fail(PALIAL_ERR_ARG, "descriptor has a NULL matrix pointer");
@@ -162,7 +188,7 @@ int check_desc(const MATRIX_desc &A)
 // ------------------------------------------------------------------ profiling
 // Per-kernel-class HIP-event timing (stream-ordered launches only): each
 // launch of a class is bracketed by two events on the stream it runs on.
-enum KClass : int { KC_GEMM = 0, KC_DIAG = 1, KC_TRSM = 2, KC_TSQRT = 3, KC_QRUPD = 4, KC_N = 5 };
+enum KClass : int { KC_GEMM = 0, KC_DIAG = 1, KC_TRSM = 2, KC_TSQRT = 3, KC_QRUPD = 4, KC_PACK = 5, KC_N = 6 };
 struct ProfRec { int cls; double flops; double bytes; hipEvent_t a, b; };
 struct Prof {
     bool on = false;
@@ -252,6 +278,178 @@ TrsmArgs targs(double *mat, int N, int b, int k)
     return t;
 }
 
+// events of one factorization: [0, MT+1) panel k ready, [MT+1, 2MT+2) update
+// k done, 2MT+2 fork, 2MT+3 graph gate, [2MT+4, 3MT+5) chain TRSM k done
+size_t events_needed(int MT) { return 3 * (size_t)MT + 6; }
+
+GemmArgs single_tile(const double *A, const double *B, double *C, int b, bool syrk)
+{
+    GemmArgs a;
+    memset(&a, 0, sizeof a);
+    a.mode = GM_SINGLE;
+    a.A = A; a.B = B; a.C = C; a.D = C;
+    a.N = b; a.b = b; a.MT = 1; a.T = 1; a.ntasks = 1;
+    a.alpha = -1.0; a.beta = 1.0;
+    a.single_diag = syrk ? 1 : 0;
+    return a;
+}
+
+// Diagonal-chain schedule of the upper Cholesky (src/cholesky.h:15-176).
+//   chain (sc): P(k) -> T1(k) TRSM of tile (k,k+1) -> S1(k) SYRK of tile
+//               (k+1,k+1) by row k -> P(k+1)
+//   bulk  (sb): TB(k) TRSM of tiles (k, k+2..) -> R(k) update of every tile
+//               (i,j), k < i <= j, except (k+1,k+1)
+// Before T1(k) the chain waits for R(k-1), which brings tiles (k,k+1) and
+// (k+1,k+1) up to date with rows < k; R(k) waits for T1(k).  Every tile
+// therefore receives the updates of rows 0..k in increasing k as in the
+// reference, with the same kernels: only the grouping into launches differs.
+void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, int *dinfo,
+                        Launcher &L, int prec)
+{
+    const int MT = N / b;
+    const bool v2 = panel2_ok(b);
+    const bool lp = prec > 0 && b % 128 == 0;
+    const bool pk = lp && g.d_pack && g.pack_cap >= 2 * (size_t)MT * b * b * 4;
+    auto dinvU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
+    auto potrf = [&](int k, hipStream_t s) {
+        L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s, [&] {
+            return v2 ? launch_potrf2(dA + tile_off(b, MT, k, k), b, dinfo, k * b, dinvU(k), s)
+                      : launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s);
+        });
+    };
+    auto trsm = [&](int k, int first, int n, hipStream_t s) {
+        TrsmArgs t = targs(dA, N, b, k);
+        t.first = first;
+        t.ntiles = n;
+        L.run(KC_TRSM, n * b3(b), 3.0 * n * tile_bytes(b), s, [&] {
+            return v2 ? launch_trsm2(TR_CHOL, t, dinvU(k), s) : launch_trsm(TR_CHOL, t, s);
+        });
+    };
+    auto gemm = [&](const GemmArgs &a, double flops, double tiles, hipStream_t s) {
+        L.run(KC_GEMM, flops, 3.0 * tiles * tile_bytes(b), s,
+              [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
+    };
+    hipEvent_t *evP = g.events.data(), *evR = g.events.data() + MT + 1, *evT = g.events.data() + 2 * MT + 4;
+    hipEvent_t fork = g.events[2 * MT + 2];
+    L.chk(hipEventRecord(fork, s0));
+    L.chk(hipStreamWaitEvent(sc, fork, 0));
+    if (sb != s0) L.chk(hipStreamWaitEvent(sb, fork, 0));
+    potrf(0, sc);
+    L.chk(hipEventRecord(evP[0], sc));
+    for (int k = 0; k < MT; k++) {
+        const int T = MT - k - 1;
+        if (T > 0) {
+            if (k > 0) L.chk(hipStreamWaitEvent(sc, evR[k - 1], 0));
+            trsm(k, k + 1, 1, sc);
+            L.chk(hipEventRecord(evT[k], sc));
+            gemm(single_tile(dA + tile_off(b, MT, k, k + 1), dA + tile_off(b, MT, k, k + 1),
+                             dA + tile_off(b, MT, k + 1, k + 1), b, true),
+                 b3(b), 1, sc);
+            potrf(k + 1, sc);
+            L.chk(hipEventRecord(evP[k + 1], sc));
+        }
+        L.chk(hipStreamWaitEvent(sb, evP[k], 0));
+        if (T > 1) trsm(k, k + 2, T - 1, sb);
+        if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
+        if (T > 1) {
+            GemmArgs a = gargs(GM_CHOL_TRAIL, dA, N, b, k, T);
+            a.task0 = 1;                 // task 0 is tile (k+1,k+1), done by the chain
+            a.ntasks -= 1;
+            const double ngemm = (double)T * (T - 1) / 2, nsyrk = T - 1;
+            if (pk) {
+                // row k rounded once; the update streams the packed copy
+                L.run(KC_PACK, 0, T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5)), sb,
+                      [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, g.d_pack, sb); });
+                a.pa = a.pb = g.d_pack;
+                L.run(KC_GEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk, 3.0 * (ngemm + nsyrk) * tile_bytes(b), sb,
+                      [&] { return launch_gemm_packed(a, prec, sb); });
+            } else {
+                gemm(a, 2 * b3(b) * ngemm + b3(b) * nsyrk, ngemm + nsyrk, sb);
+            }
+        }
+        L.chk(hipEventRecord(evR[k], sb));
+    }
+    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evR[MT - 1], 0));
+}
+
+// Diagonal-chain schedule of the no-pivot LU (src/lu.h:14-173), same shape:
+// the chain solves tiles (k,k+1) and (k+1,k) and updates (k+1,k+1); the bulk
+// solves the rest of row and column k and updates every other trailing tile.
+void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, Launcher &L, int prec)
+{
+    const int MT = N / b;
+    const bool v2 = panel2_ok(b);
+    const bool lp = prec > 0 && b % 128 == 0;
+    const bool pk = lp && g.d_pack && g.pack_cap >= 2 * (size_t)MT * b * b * 4;
+    auto dU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
+    auto dL = [&](int k) { return g.d_dinv + (size_t)k * 64 * b + (size_t)32 * b; };
+    auto getrf = [&](int k, hipStream_t s) {
+        L.run(KC_DIAG, 2 * b3(b) / 3, 2 * tile_bytes(b), s, [&] {
+            return v2 ? launch_getrf2(dA + tile_off(b, MT, k, k), b, dU(k), dL(k), s)
+                      : launch_getrf(dA + tile_off(b, MT, k, k), b, s);
+        });
+    };
+    auto trsm = [&](int kind, int k, int first, int n, hipStream_t s) {
+        TrsmArgs t = targs(dA, N, b, k);
+        t.first = first;
+        t.ntiles = n;
+        L.run(KC_TRSM, n * b3(b), 3.0 * n * tile_bytes(b), s, [&] {
+            return v2 ? launch_trsm2(kind, t, kind == TR_LUR ? dU(k) : dL(k), s) : launch_trsm(kind, t, s);
+        });
+    };
+    auto gemm = [&](const GemmArgs &a, double tiles, hipStream_t s) {
+        L.run(KC_GEMM, 2 * b3(b) * tiles, 3.0 * tiles * tile_bytes(b), s,
+              [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
+    };
+    hipEvent_t *evP = g.events.data(), *evR = g.events.data() + MT + 1, *evT = g.events.data() + 2 * MT + 4;
+    hipEvent_t fork = g.events[2 * MT + 2];
+    L.chk(hipEventRecord(fork, s0));
+    L.chk(hipStreamWaitEvent(sc, fork, 0));
+    if (sb != s0) L.chk(hipStreamWaitEvent(sb, fork, 0));
+    getrf(0, sc);
+    L.chk(hipEventRecord(evP[0], sc));
+    for (int k = 0; k < MT; k++) {
+        const int T = MT - k - 1;
+        if (T > 0) {
+            if (k > 0) L.chk(hipStreamWaitEvent(sc, evR[k - 1], 0));
+            trsm(TR_LUL, k, k + 1, 1, sc);
+            trsm(TR_LUR, k, k + 1, 1, sc);
+            L.chk(hipEventRecord(evT[k], sc));
+            gemm(single_tile(dA + tile_off(b, MT, k + 1, k), dA + tile_off(b, MT, k, k + 1),
+                             dA + tile_off(b, MT, k + 1, k + 1), b, false),
+                 1, sc);
+            getrf(k + 1, sc);
+            L.chk(hipEventRecord(evP[k + 1], sc));
+        }
+        L.chk(hipStreamWaitEvent(sb, evP[k], 0));
+        if (T > 1) {
+            trsm(TR_LUL, k, k + 2, T - 1, sb);
+            trsm(TR_LUR, k, k + 2, T - 1, sb);
+        }
+        if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
+        if (T > 1) {
+            GemmArgs a = gargs(GM_LU_TRAIL, dA, N, b, k, T);
+            a.task0 = 1;                 // task 0 is tile (k+1,k+1), done by the chain
+            a.ntasks -= 1;
+            if (pk) {
+                char *pU = (char *)g.d_pack, *pL = pU + (size_t)MT * b * b * 4;
+                const double pb = T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5));
+                L.run(KC_PACK, 0, pb, sb, [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, pU, sb); });
+                L.run(KC_PACK, 0, pb, sb, [&] { return launch_pack(dA, N, b, k, k + 1, T, true, prec, pL, sb); });
+                a.pa = pL;
+                a.pb = pU;
+                const double nt = (double)T * T - 1;
+                L.run(KC_GEMM, 2 * b3(b) * nt, 3.0 * nt * tile_bytes(b), sb,
+                      [&] { return launch_gemm_packed(a, prec, sb); });
+            } else {
+                gemm(a, (double)T * T - 1, sb);
+            }
+        }
+        L.chk(hipEventRecord(evR[k], sb));
+    }
+    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evR[MT - 1], 0));
+}
+
 // Cholesky, src/cholesky.h:15-176.  Look-ahead: step k's update of tile row
 // k+1 runs on the panel stream ahead of the rest, so POTRF/TRSM of step k+1
 // overlap the bulk of step k's trailing update.
@@ -285,6 +483,10 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
         L.run(KC_GEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk, 3.0 * (ngemm + nsyrk) * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
     };
+    if (lookahead && g.dag == 1) {
+        dag_cholesky_chain(dA, N, b, s0, s1, g.s_bulk, dinfo, L, prec);
+        return;
+    }
     if (!lookahead) {
         for (int k = 0; k < MT; k++) {
             potrf(k, s0);
@@ -343,6 +545,10 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
         L.run(KC_GEMM, 2 * b3(b) * ntile, 3.0 * ntile * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
     };
+    if (lookahead && g.dag == 1) {
+        dag_lu_chain(dA, N, b, s0, s1, g.s_bulk, L, prec);
+        return;
+    }
     if (!lookahead) {
         for (int k = 0; k < MT; k++) {
             panel(k, s0);
@@ -473,8 +679,9 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     const double tol = o.tol > 0 ? o.tol : 1e-14;
     const int max_iter = o.max_iter > 0 ? o.max_iter : 30;
     const size_t nn = (size_t)N * N, nv = (size_t)N * nrhs;
-    hipStream_t s = g.s_upd;
-    if ((rc = ensure_events(2 * (size_t)MT + 4)) || (rc = ensure_dinv(b, MT))) return rc;
+    hipStream_t s = g.s_main;
+    if ((rc = ensure_events(events_needed(MT))) || (rc = ensure_dinv(b, MT))) return rc;
+    if (o.precision > 0 && b % 128 == 0 && (rc = ensure_pack(b, MT))) return rc;
     if ((rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
     if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
     const size_t npart = residual_scratch(N, bA, nrhs);
@@ -593,8 +800,8 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     hipStream_t s0 = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
     hipStream_t s1 = g.s_panel;
     const bool lookahead = !o.no_lookahead && MT > 1 && algo != ALG_QR;
-    const bool use_graph = !o.no_graph && !g_prof.on;
-    if ((rc = ensure_events(2 * (size_t)MT + 4))) return rc;
+    const bool use_graph = !o.no_graph && !g_prof.on && !(lookahead && g.dag == 1 && g.reserve > 0);
+    if ((rc = ensure_events(events_needed(MT)))) return rc;
     if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
     if ((rc = ensure_dinv(b, MT))) return rc;
 
@@ -802,21 +1009,22 @@ int palial_hip_init(int device)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(PALIAL_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
     HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
-    // PALIAL_CU_SPLIT=n: the solve path's panel stream runs on n reserved CUs
-    // and its update stream on the others, so panel kernels never wait for
-    // update workgroups to drain a CU (graph replays do not keep CU masks)
-    const char *split = getenv("PALIAL_CU_SPLIT");
-    g.cu_split = split ? atoi(split) : 0;
-    if (g.cu_split > 0 && g.cu_split < prop.multiProcessorCount) {
+    // PALIAL_RESERVE_CUS=n (default 16): the bulk stream of the chain schedule
+    // runs on all CUs but the first n, which stay free for the diagonal chain.
+    // PALIAL_DAG=0 selects the row look-ahead schedule instead.
+    const char *res = getenv("PALIAL_RESERVE_CUS");
+    const char *dag = getenv("PALIAL_DAG");
+    g.dag = dag ? atoi(dag) : 1;
+    g.reserve = res ? atoi(res) : 16;
+    HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
+    if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
         const int ncu = prop.multiProcessorCount, words = (ncu + 31) / 32;
-        std::vector<uint32_t> mp(words, 0), mu(words, 0);
-        for (int cu = 0; cu < ncu; cu++) (cu < g.cu_split ? mp : mu)[cu / 32] |= 1u << (cu % 32);
-        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_panel, words, mp.data()));
-        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_upd, words, mu.data()));
+        std::vector<uint32_t> mu(words, 0);
+        for (int cu = g.reserve; cu < ncu; cu++) mu[cu / 32] |= 1u << (cu % 32);
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_bulk, words, mu.data()));
     } else {
-        g.cu_split = 0;
-        HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
-        g.s_upd = g.s_main;
+        g.reserve = 0;
+        g.s_bulk = g.s_main;
     }
     HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
     panel_init();
@@ -842,11 +1050,12 @@ int palial_hip_finalize(void)
     if (g.d_w2) hipFree(g.d_w2);
     if (g.d_tau) hipFree(g.d_tau);
     if (g.d_dinv) hipFree(g.d_dinv);
+    if (g.d_pack) hipFree(g.d_pack);
     if (g.d_F) hipFree(g.d_F);
     if (g.d_inv) hipFree(g.d_inv);
     if (g.d_vec) hipFree(g.d_vec);
     if (g.d_info) hipFree(g.d_info);
-    if (g.s_upd && g.s_upd != g.s_main) hipStreamDestroy(g.s_upd);
+    if (g.s_bulk && g.s_bulk != g.s_main) hipStreamDestroy(g.s_bulk);
     hipStreamDestroy(g.s_main);
     hipStreamDestroy(g.s_panel);
     g = Ctx();
@@ -887,6 +1096,45 @@ int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms)
     HIPCHK(hipEventRecord(e0, g.s_main));
     for (int r = 0; r < reps; r++) HIPCHK(launch_gemm(a, algo == ALG_CHOL, false, g.s_main));
     HIPCHK(hipEventRecord(e1, g.s_main));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ms) *ms = t / reps;
+    return 0;
+}
+
+int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double *ms)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_desc(A))) return rc;
+    const int N = A.matrix_size, b = A.tile_size, MT = N / b, T = MT - k - 1;
+    if (algo != ALG_CHOL && algo != ALG_LU) return fail(PALIAL_ERR_ARG, "algo %d has no trailing update", algo);
+    if (k < 0 || T <= 0 || reps <= 0) return fail(PALIAL_ERR_ARG, "bad step %d or reps %d", k, reps);
+    if (prec < 0 || prec > 2) return fail(PALIAL_ERR_ARG, "precision %d not available", prec);
+    if (prec > 0 && (b % 128 != 0)) return fail(PALIAL_ERR_ARG, "packed updates need b %% 128 == 0 (b=%d)", b);
+    if (prec > 0 && (rc = ensure_pack(b, MT))) return rc;
+    double *dA = (double *)A.matrix;
+    GemmArgs a = gargs(algo == ALG_CHOL ? GM_CHOL_TRAIL : GM_LU_TRAIL, dA, N, b, k, T);
+    char *pU = (char *)g.d_pack, *pL = pU + (size_t)MT * b * b * 4;
+    a.pa = algo == ALG_CHOL ? (const void *)pU : (const void *)pL;
+    a.pb = pU;
+    hipStream_t s = g.s_main;
+    auto once = [&]() -> hipError_t {
+        if (prec == 0) return launch_gemm(a, algo == ALG_CHOL, false, s);
+        hipError_t e = launch_pack(dA, N, b, k, k + 1, T, false, prec, pU, s);
+        if (e == hipSuccess && algo == ALG_LU) e = launch_pack(dA, N, b, k, k + 1, T, true, prec, pL, s);
+        if (e == hipSuccess) e = launch_gemm_packed(a, prec, s);
+        return e;
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) HIPCHK(once());
+    HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float t = 0;
     hipEventElapsedTime(&t, e0, e1);

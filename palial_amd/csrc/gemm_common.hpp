@@ -18,6 +18,7 @@ struct TaskPtrs {
     double *C, *E;
     bool diag;   // SYRK diagonal tile: only the upper triangle is written
     bool valid;
+    int ia, ib;  // panel positions of the A / B operand tiles (packed operands), -1: none
 };
 
 __host__ __device__ inline void tri_decode(int idx, int &n, int &m)
@@ -32,11 +33,13 @@ __host__ __device__ inline void tri_decode(int idx, int &n, int &m)
 
 __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
 {
+    task += p.task0;
     TaskPtrs t;
     t.A = t.B = t.D = nullptr;
     t.C = t.E = nullptr;
     t.diag = false;
     t.valid = true;
+    t.ia = t.ib = -1;
     const int b = p.b, MT = p.MT, k = p.k;
     const size_t bb = (size_t)b * b;
     double *M = p.mat;
@@ -56,6 +59,8 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.C = M + tile_off(b, MT, n, m);
         t.D = t.C;
         t.diag = (n == m);
+        t.ia = n - k - 1;
+        t.ib = m - k - 1;
         break;
     }
     case GM_CHOL_ROW: {
@@ -65,6 +70,8 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.C = M + tile_off(b, MT, n, m);
         t.D = t.C;
         t.diag = (n == m);
+        t.ia = 0;
+        t.ib = m - k - 1;
         break;
     }
     case GM_LU_TRAIL:
@@ -77,6 +84,8 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.B = M + tile_off(b, MT, k, n);
         t.C = M + tile_off(b, MT, m, n);
         t.D = t.C;
+        t.ia = m - k - 1;
+        t.ib = n - k - 1;
         break;
     }
     case GM_LU_PANEL: {
@@ -88,6 +97,8 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.B = M + tile_off(b, MT, k, n);
         t.C = M + tile_off(b, MT, m, n);
         t.D = t.C;
+        t.ia = m - k - 1;
+        t.ib = n - k - 1;
         break;
     }
     case GM_QR_W: {

@@ -53,6 +53,9 @@ struct GemmArgs {
     int k, m;            // step k; TS row m for the QR modes
     int T;               // trailing tile count used by the enumeration
     int ntasks;
+    int task0;           // first task of the enumeration this launch covers
+    const void *pa;      // packed low-precision panel operands (launch_gemm_packed):
+    const void *pb;      // tile t of the panel at t*b*b elements, K-contiguous
     double alpha, beta;
     int maskA;
     int single_diag;     // GM_SINGLE: output is a SYRK tile (upper triangle only)
@@ -90,6 +93,13 @@ hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipS
 hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s);
 
 // low-precision trailing updates (k_gemm_lp.hip): prec 1 = FP32, 2 = BF16
+// Panel tiles of step k rounded to `prec` (1 FP32, 2 BF16), K-contiguous:
+// row-panel tiles (k, first..first+n-1) keep their layout, column-panel tiles
+// (first.., k) are transposed; tile j lands at dst + (j-k-1)*b*b elements.
+hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n, bool colpanel, int prec,
+                       void *dst, hipStream_t s);
+// Trailing update from packed operands (a.pa / a.pb); TA/TB are implied.
+hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s);
 hipError_t launch_gemm_lp(const GemmArgs &a, bool transA, bool transB, int prec, hipStream_t s);
 
 // solve path (k_solve.hip)

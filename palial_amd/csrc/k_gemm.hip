@@ -12,6 +12,8 @@
 //   B fragment: B[k = l / 16][col = l % 16]
 //   D (4 doubles): D[row = (l/16) + 4*v][col = l % 16], v = 0..3
 //   (the f64 variant interleaves rows across the four lane groups)
+// The operands are passed swapped (D = B^T A^T), so lane l's accumulators sit
+// in row l%16 of C and the epilogue moves 128-byte row runs.
 #include "gemm_common.hpp"
 #include <cstring>
 #include <cstdlib>
@@ -169,8 +171,8 @@ __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
 #pragma unroll
                 for (int x = 0; x < FM; x++)
 #pragma unroll
-                    for (int c = 0; c < FM; c++)
-                        acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[c], acc[x][c], 0, 0, 0);
+                    for (int c = 0; c < FM; c++)   // operands swapped: acc holds C^T fragments
+                        acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[c], av[x], acc[x][c], 0, 0, 0);
             }
             if (more) store_lds(buf ^ 1);
             __syncthreads();
@@ -178,26 +180,46 @@ __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
         }
     }
 
-    // ---- epilogue: C = alpha*acc + beta*D ; E -= alpha*acc
+    // ---- epilogue: C = alpha*acc + beta*D ; E -= alpha*acc.
+    // acc[x][c][v] = C(i0 + wm*WM + 16x + l%16, j0 + wn*WM + 16c + l/16 + 4v):
+    // 16 lanes cover 16 consecutive rows (128 B) of one column.  D (which may
+    // be C itself) is read for a whole fragment row before any store, so the
+    // loads are in flight together instead of one round trip each.  E (the
+    // QR updates only; never aliased with C or D) follows in its own pass.
     const double alpha = p.alpha, beta = p.beta;
 #pragma unroll
-    for (int a = 0; a < FM; a++) {
+    for (int x = 0; x < FM; x++) {
+        const int i = i0 + wm * WM + x * 16 + (lane & 15);
+        double dv[FM][4];
 #pragma unroll
-        for (int c = 0; c < FM; c++) {
-            const int j = j0 + wn * WM + c * 16 + (lane & 15);
-            const int ib = i0 + wm * WM + a * 16 + (lane >> 4);
+        for (int c = 0; c < FM; c++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int i = ib + 4 * v;
-                if (tp.diag && i > j) continue;
-                const size_t o = (size_t)i + (size_t)j * b;
-                const double r = alpha * acc[a][c][v];
-                double out = r;
-                if (tp.D) out += beta * tp.D[o];
-                tp.C[o] = out;
-                if (tp.E) tp.E[o] -= r;
+                const int j = j0 + wn * WM + c * 16 + (lane >> 4) + 4 * v;
+                dv[c][v] = (tp.D && !(tp.diag && i > j)) ? tp.D[(size_t)i + (size_t)j * b] : 0.0;
             }
-        }
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + wn * WM + c * 16 + (lane >> 4) + 4 * v;
+                if (tp.diag && i > j) continue;
+                const double r = alpha * acc[x][c][v];
+                tp.C[(size_t)i + (size_t)j * b] = tp.D ? r + beta * dv[c][v] : r;
+            }
+    }
+    if (tp.E) {
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+            for (int c = 0; c < FM; c++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = i0 + wm * WM + x * 16 + (lane & 15);
+                    const int j = j0 + wn * WM + c * 16 + (lane >> 4) + 4 * v;
+                    if (tp.diag && i > j) continue;
+                    tp.E[(size_t)i + (size_t)j * b] -= alpha * acc[x][c][v];
+                }
     }
 }
 

@@ -14,6 +14,7 @@
 //   f32 16x16x4 : A[row = l%16][k = l/16], B[k = l/16][col = l%16]
 //   bf16 16x16x32: A[row = l%16][k = 8*(l/16) .. +7], B[k = 8*(l/16) .. +7][col = l%16]
 //   f32 result  : D[row = 4*(l/16) + v][col = l%16]
+// (operands are passed swapped, see lp_epilogue)
 #include "gemm_common.hpp"
 
 namespace palial {
@@ -26,6 +27,51 @@ __device__ inline unsigned short f2bf(float f)
     // round to nearest even (inputs are finite)
     const unsigned int u = __float_as_uint(f);
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Operands are fed to the MFMA swapped, so acc[x][c][v] holds
+// C(i0 + wm*WM + 16x + l%16, j0 + wn*WM + 16c + 4(l/16) + v): 16 lanes cover
+// 16 consecutive rows (128 B) of one column of C.  D (C itself in the
+// trailing updates) is read for a whole fragment row before any store, and E
+// (never aliased with C or D) is updated in its own pass.
+template <int FM, int WM>
+__device__ inline void lp_epilogue(const TaskPtrs &tp, const f4 (&acc)[FM][FM], int i0, int j0, int wm, int wn,
+                                   int lane, int b, double alpha, double beta)
+{
+#pragma unroll
+    for (int x = 0; x < FM; x++) {
+        const int i = i0 + wm * WM + x * 16 + (lane & 15);
+        double dv[FM][4];
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v;
+                dv[c][v] = (tp.D && !(tp.diag && i > j)) ? tp.D[(size_t)i + (size_t)j * b] : 0.0;
+            }
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v;
+                if (tp.diag && i > j) continue;
+                const double r = alpha * (double)acc[x][c][v];
+                tp.C[(size_t)i + (size_t)j * b] = tp.D ? r + beta * dv[c][v] : r;
+            }
+    }
+    if (tp.E) {
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+            for (int c = 0; c < FM; c++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = i0 + wm * WM + x * 16 + (lane & 15);
+                    const int j = j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v;
+                    if (tp.diag && i > j) continue;
+                    tp.E[(size_t)i + (size_t)j * b] -= alpha * (double)acc[x][c][v];
+                }
+    }
 }
 
 // ------------------------------------------------------------------ FP32
@@ -126,31 +172,14 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_kernel(GemmArgs p)
             for (int x = 0; x < FM; x++)
 #pragma unroll
                 for (int c = 0; c < FM; c++)
-                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[x], bv[c], acc[x][c], 0, 0, 0);
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[c], av[x], acc[x][c], 0, 0, 0);
         }
         if (more) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
 
-    const double alpha = p.alpha, beta = p.beta;
-#pragma unroll
-    for (int a = 0; a < FM; a++)
-#pragma unroll
-        for (int c = 0; c < FM; c++) {
-            const int j = j0 + wn * WM + c * 16 + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int i = i0 + wm * WM + a * 16 + 4 * (lane >> 4) + v;
-                if (tp.diag && i > j) continue;
-                const size_t o = (size_t)i + (size_t)j * b;
-                const double r = alpha * (double)acc[a][c][v];
-                double out = r;
-                if (tp.D) out += beta * tp.D[o];
-                tp.C[o] = out;
-                if (tp.E) tp.E[o] -= r;
-            }
-        }
+    lp_epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
 }
 
 // ------------------------------------------------------------------ BF16
@@ -267,30 +296,13 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(GemmArgs p)
         for (int x = 0; x < FM; x++)
 #pragma unroll
             for (int c = 0; c < FM; c++)
-                acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[x], bv[c], acc[x][c], 0, 0, 0);
+                acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], av[x], acc[x][c], 0, 0, 0);
         if (more) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
 
-    const double alpha = p.alpha, beta = p.beta;
-#pragma unroll
-    for (int a = 0; a < FM; a++)
-#pragma unroll
-        for (int c = 0; c < FM; c++) {
-            const int j = j0 + wn * WM + c * 16 + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int i = i0 + wm * WM + a * 16 + 4 * (lane >> 4) + v;
-                if (tp.diag && i > j) continue;
-                const size_t o = (size_t)i + (size_t)j * b;
-                const double r = alpha * (double)acc[a][c][v];
-                double out = r;
-                if (tp.D) out += beta * tp.D[o];
-                tp.C[o] = out;
-                if (tp.E) tp.E[o] -= r;
-            }
-        }
+    lp_epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
 }
 
 hipError_t launch_gemm_lp(const GemmArgs &a, bool ta, bool tb, int prec, hipStream_t s)

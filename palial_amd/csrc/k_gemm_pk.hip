@@ -1,0 +1,304 @@
+// k_gemm_pk.hip -- trailing updates of the mixed-precision factorizations
+// from packed low-precision panels.
+//
+// The panel of step k (row tiles U_kj of Cholesky, L_ik and U_kj of LU) is
+// rounded once to BF16 or FP32 by pack_kernel and stored K-contiguous, so
+// the update GEMM streams 2 (BF16) or 4 (FP32) bytes per operand element
+// instead of re-reading and converting FP64 tiles in every workgroup:
+//   row-panel tile (k, j) : element (q, c) at q + c*b   (its own layout)
+//   col-panel tile (i, k) : element (r, q) at q + r*b   (transposed)
+// with q the K index.  The FP64 output tile receives C = alpha*acc + beta*D.
+//
+// MFMA operands are swapped (D = B^T A^T) so that each lane's accumulator
+// column maps to 16 consecutive rows of the column-major C tile: every load
+// and store of the FP64 epilogue covers 4 full 128-byte row runs.
+//
+// Operand layouts (lane l of 64):
+//   bf16 16x16x32: A[row = l%16][k = 8*(l/16) .. +7], B[k = 8*(l/16) .. +7][col = l%16]
+//   f32  16x16x4 : A[row = l%16][k = l/16],           B[k = l/16][col = l%16]
+//   f32 result   : D[row = 4*(l/16) + v][col = l%16]
+#include "gemm_common.hpp"
+
+namespace palial {
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ inline unsigned short to_bf16(double d)
+{
+    // FP64 -> FP32 (nearest) -> BF16 (nearest even); inputs are finite
+    const unsigned int u = __float_as_uint((float)d);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// C = alpha*acc + beta*D with acc[x][c][v] = C(i0 + wm*WM + 16x + l%16,
+// j0 + wn*WM + 16c + 4(l/16) + v).  D (C itself in the trailing updates) is
+// read for a whole fragment row before any store, so those loads are in
+// flight together instead of one dependent round trip each.
+template <int FM, int WM>
+__device__ inline void epilogue(const TaskPtrs &tp, const f4 (&acc)[FM][FM], int i0, int j0, int wm, int wn,
+                                int lane, int b, double alpha, double beta)
+{
+#pragma unroll
+    for (int x = 0; x < FM; x++) {
+        const int i = i0 + wm * WM + x * 16 + (lane & 15);
+        double dv[FM][4];
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v;
+                dv[c][v] = (tp.D && !(tp.diag && i > j)) ? tp.D[(size_t)i + (size_t)j * b] : 0.0;
+            }
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v;
+                if (tp.diag && i > j) continue;
+                const double r = alpha * (double)acc[x][c][v];
+                tp.C[(size_t)i + (size_t)j * b] = tp.D ? r + beta * dv[c][v] : r;
+            }
+    }
+}
+
+// One workgroup per 64 x 64 sub-block of one panel tile.
+template <int PREC>
+__global__ __launch_bounds__(256) void pack_kernel(const double *mat, int b, int MT, int k, int first, int colp,
+                                                   void *dst)
+{
+    __shared__ float sT[64][65];
+    const int nsb = b / 64, per = nsb * nsb;
+    const int tix = blockIdx.x / per, sbk = blockIdx.x % per;
+    const int r0 = (sbk % nsb) * 64, c0 = (sbk / nsb) * 64;
+    const int idx = first + tix;
+    const double *src = colp ? mat + tile_off(b, MT, idx, k) : mat + tile_off(b, MT, k, idx);
+    const size_t dbase = (size_t)(idx - k - 1) * b * b;
+    const int t = threadIdx.x;
+    if (!colp) {
+        // same layout: 4 consecutive rows of one column per thread and pass
+#pragma unroll
+        for (int pass = 0; pass < 4; pass++) {
+            const int e = t + pass * 256;
+            const int r = (e % 16) * 4, c = e / 16;
+            const size_t o = (size_t)(r0 + r) + (size_t)(c0 + c) * b;
+            const double2 x = *(const double2 *)(src + o), y = *(const double2 *)(src + o + 2);
+            if (PREC == 2) {
+                ushort4 v;
+                v.x = to_bf16(x.x); v.y = to_bf16(x.y); v.z = to_bf16(y.x); v.w = to_bf16(y.y);
+                *(ushort4 *)((unsigned short *)dst + dbase + o) = v;
+            } else {
+                *(float4 *)((float *)dst + dbase + o) = make_float4((float)x.x, (float)x.y, (float)y.x, (float)y.y);
+            }
+        }
+    } else {
+        // transpose through LDS: read along rows r, write along q
+#pragma unroll
+        for (int pass = 0; pass < 16; pass++) {
+            const int e = t + pass * 256;
+            const int r = e % 64, q = e / 64;
+            sT[q][r] = (float)src[(size_t)(r0 + r) + (size_t)(c0 + q) * b];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pass = 0; pass < 16; pass++) {
+            const int e = t + pass * 256;
+            const int q = e % 64, r = e / 64;
+            const size_t o = dbase + (size_t)(c0 + q) + (size_t)(r0 + r) * b;
+            // BF16 of the FP32 rounding, as to_bf16() does
+            if (PREC == 2) {
+                const unsigned int u = __float_as_uint(sT[q][r]);
+                ((unsigned short *)dst)[o] = (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+            } else {
+                ((float *)dst)[o] = sT[q][r];
+            }
+        }
+    }
+}
+
+// 256 threads, 128 x 128 output block, 2 x 2 waves of 64 x 64, KB = 64.
+__global__ __launch_bounds__(256, 2) void gemm_bf16p_kernel(GemmArgs p)
+{
+    constexpr int BM = 128, KB = 64, WM = 64, FM = 4;
+    constexpr int LDK = KB + 8;           // bf16 per LDS row ([mn][k], k contiguous)
+    constexpr int BUF = BM * LDK;
+    __shared__ unsigned short sA[2 * BUF];
+    __shared__ unsigned short sB[2 * BUF];
+
+    const int b = p.b;
+    const int nbd = b / BM;
+    const int blk_per_task = nbd * nbd;
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int task = lid / blk_per_task, blk = lid % blk_per_task;
+    const int bi = blk % nbd, bj = blk / nbd;
+    if (task >= p.ntasks) return;
+    const TaskPtrs tp = decode(p, task);
+    if (!tp.valid) return;
+    if (tp.diag && bi > bj) return;
+    const int i0 = bi * BM, j0 = bj * BM;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    const unsigned short *PA = (const unsigned short *)p.pa + (size_t)tp.ia * b * b;
+    const unsigned short *PB = (const unsigned short *)p.pb + (size_t)tp.ib * b * b;
+
+    f4 acc[FM][FM];
+#pragma unroll
+    for (int a = 0; a < FM; a++)
+#pragma unroll
+        for (int c = 0; c < FM; c++) acc[a][c] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    // each thread stages 32 consecutive k (64 B) of one row of each operand
+    const int row = t >> 1, kc = (t & 1) * 32;
+    uint4 ra[4], rb[4];
+    auto load = [&](int kb) {
+        const uint4 *a = (const uint4 *)(PA + (size_t)(i0 + row) * b + kb + kc);
+        const uint4 *bb = (const uint4 *)(PB + (size_t)(j0 + row) * b + kb + kc);
+#pragma unroll
+        for (int q = 0; q < 4; q++) { ra[q] = a[q]; rb[q] = bb[q]; }
+    };
+    auto store = [&](int buf) {
+        uint4 *a = (uint4 *)(sA + buf * BUF + row * LDK + kc);
+        uint4 *bb = (uint4 *)(sB + buf * BUF + row * LDK + kc);
+#pragma unroll
+        for (int q = 0; q < 4; q++) { a[q] = ra[q]; bb[q] = rb[q]; }
+    };
+
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int kb = 0; kb < b; kb += KB) {
+        const bool more = kb + KB < b;
+        if (more) load(kb + KB);
+        const unsigned short *a = sA + buf * BUF, *bb = sB + buf * BUF;
+#pragma unroll
+        for (int ks = 0; ks < KB; ks += 32) {
+            const int ko = ks + 8 * (lane >> 4);
+            bf8 av[FM], bv[FM];
+#pragma unroll
+            for (int x = 0; x < FM; x++) av[x] = *(const bf8 *)(a + (wm * WM + x * 16 + (lane & 15)) * LDK + ko);
+#pragma unroll
+            for (int c = 0; c < FM; c++) bv[c] = *(const bf8 *)(bb + (wn * WM + c * 16 + (lane & 15)) * LDK + ko);
+#pragma unroll
+            for (int x = 0; x < FM; x++)
+#pragma unroll
+                for (int c = 0; c < FM; c++)
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], av[x], acc[x][c], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    // acc[x][c][v] = C(i0 + wm*64 + 16x + l%16, j0 + wn*64 + 16c + 4(l/16) + v)
+    epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
+}
+
+// FP32 operands, v_mfma_f32_16x16x4_f32; same blocking with KB = 32.
+__global__ __launch_bounds__(256, 2) void gemm_f32p_kernel(GemmArgs p)
+{
+    constexpr int BM = 128, KB = 32, WM = 64, FM = 4;
+    constexpr int LDK = KB + 4;           // floats; 16 rows x 4 k of a fragment hit distinct banks
+    constexpr int BUF = BM * LDK;
+    __shared__ float sA[2 * BUF];
+    __shared__ float sB[2 * BUF];
+
+    const int b = p.b;
+    const int nbd = b / BM;
+    const int blk_per_task = nbd * nbd;
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int task = lid / blk_per_task, blk = lid % blk_per_task;
+    const int bi = blk % nbd, bj = blk / nbd;
+    if (task >= p.ntasks) return;
+    const TaskPtrs tp = decode(p, task);
+    if (!tp.valid) return;
+    if (tp.diag && bi > bj) return;
+    const int i0 = bi * BM, j0 = bj * BM;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    const float *PA = (const float *)p.pa + (size_t)tp.ia * b * b;
+    const float *PB = (const float *)p.pb + (size_t)tp.ib * b * b;
+
+    f4 acc[FM][FM];
+#pragma unroll
+    for (int a = 0; a < FM; a++)
+#pragma unroll
+        for (int c = 0; c < FM; c++) acc[a][c] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    const int row = t >> 1, kc = (t & 1) * 16;
+    float4 ra[4], rb[4];
+    auto load = [&](int kb) {
+        const float4 *a = (const float4 *)(PA + (size_t)(i0 + row) * b + kb + kc);
+        const float4 *bb = (const float4 *)(PB + (size_t)(j0 + row) * b + kb + kc);
+#pragma unroll
+        for (int q = 0; q < 4; q++) { ra[q] = a[q]; rb[q] = bb[q]; }
+    };
+    auto store = [&](int buf) {
+        float4 *a = (float4 *)(sA + buf * BUF + row * LDK + kc);
+        float4 *bb = (float4 *)(sB + buf * BUF + row * LDK + kc);
+#pragma unroll
+        for (int q = 0; q < 4; q++) { a[q] = ra[q]; bb[q] = rb[q]; }
+    };
+
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int kb = 0; kb < b; kb += KB) {
+        const bool more = kb + KB < b;
+        if (more) load(kb + KB);
+        const float *a = sA + buf * BUF, *bb = sB + buf * BUF;
+#pragma unroll
+        for (int ks = 0; ks < KB; ks += 4) {
+            const int ko = ks + (lane >> 4);
+            float av[FM], bv[FM];
+#pragma unroll
+            for (int x = 0; x < FM; x++) av[x] = a[(wm * WM + x * 16 + (lane & 15)) * LDK + ko];
+#pragma unroll
+            for (int c = 0; c < FM; c++) bv[c] = bb[(wn * WM + c * 16 + (lane & 15)) * LDK + ko];
+#pragma unroll
+            for (int x = 0; x < FM; x++)
+#pragma unroll
+                for (int c = 0; c < FM; c++)
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[c], av[x], acc[x][c], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
+}
+
+}  // namespace
+
+hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n, bool colpanel, int prec,
+                       void *dst, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (b % 64 != 0 || first <= k || first + n > N / b || (prec != 1 && prec != 2)) return hipErrorInvalidValue;
+    const int nsb = b / 64;
+    dim3 grid((unsigned)(n * nsb * nsb)), block(256);
+    if (prec == 2) hipLaunchKernelGGL(pack_kernel<2>, grid, block, 0, s, mat, b, N / b, k, first, colpanel ? 1 : 0, dst);
+    else hipLaunchKernelGGL(pack_kernel<1>, grid, block, 0, s, mat, b, N / b, k, first, colpanel ? 1 : 0, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
+{
+    if (a.ntasks <= 0) return hipSuccess;
+    if (a.b % 128 != 0 || a.maskA != MK_NONE || !a.pa || !a.pb) return hipErrorInvalidValue;
+    if (a.mode != GM_CHOL_TRAIL && a.mode != GM_CHOL_REST && a.mode != GM_CHOL_ROW && a.mode != GM_LU_TRAIL &&
+        a.mode != GM_LU_REST && a.mode != GM_LU_PANEL)
+        return hipErrorInvalidValue;
+    const long nbd = a.b / 128;
+    dim3 grid((unsigned)((long)a.ntasks * nbd * nbd)), block(256);
+    if (prec == 2) hipLaunchKernelGGL(gemm_bf16p_kernel, grid, block, 0, s, a);
+    else if (prec == 1) hipLaunchKernelGGL(gemm_f32p_kernel, grid, block, 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace palial

@@ -3,7 +3,7 @@
 //   potrf2_kernel : upper Cholesky of one tile (LAPACKE_dpotrf('U'),
 //                   src/cholesky.h:40) in 32-column blocks.  For each block:
 //                   the 32x32 diagonal block is factored by one wave (one
-//                   column per lane, shuffles), inverted (U_D^-1, kept in LDS
+//                   column per lane, readlane broadcasts), inverted (U_D^-1, kept in LDS
 //                   and written to `dinv` for the panel TRSM), the 32-row panel
 //                   is solved as U_D^-T P with v_mfma_f64_16x16x4_f64, and the
 //                   rank-32 trailing SYRK runs out of the LDS copy of P.
@@ -143,35 +143,44 @@ __global__ __launch_bounds__(PF_THREADS) void potrf2_kernel(double *A, int b, in
 
     for (int j0 = 0; j0 < b; j0 += NB) {
         // 1. diagonal block: staged in LDS (upper part, zeros below), factored
-        //    by wave 0 alone with one lane per column (dpotf2 order: sqrt of
-        //    the pivot, row scaled by its reciprocal, rank-1 update); the
-        //    single wave orders its own LDS accesses, so no barriers inside
+        //    by wave 0 in registers, lane c holding column c (dpotf2 order:
+        //    sqrt of the pivot, row scaled by its reciprocal, rank-1 update).
+        //    Row j of U reaches every lane through v_readlane (uniform SGPR
+        //    operands), so the 32 dependent steps need no LDS round trips.
+        //    Entries below the diagonal become garbage and are never read.
         for (int e = t; e < NB * NB; e += PF_THREADS) {
             const int r = e % NB, c = e / NB;
             sU[r + c * LDD] = (r <= c) ? A[(size_t)(j0 + r) + (size_t)(j0 + c) * b] : 0.0;
         }
         __syncthreads();
         if (w == 0) {
-            const int c = lane;
+            const int c = lane & (NB - 1);
+            double x[NB];
+#pragma unroll
+            for (int r = 0; r < NB; r++) x[r] = sU[r + c * LDD];
+            int bad = -1;   // wave-uniform; a failed pivot freezes the block (no break:
+                            // the loop must unroll fully to keep x[] in registers)
+#pragma unroll
             for (int j = 0; j < NB; j++) {
-                const double piv = sU[j + j * LDD];
-                if (!(piv > 0.0)) {
-                    if (lane == 0) { s_fail = 1; if (info) *info = info_base + j0 + j + 1; }
-                    break;
+                if (bad < 0) {
+                    const double piv = readlane_d(x[j], j);
+                    if (!(piv > 0.0)) {
+                        bad = j;
+                    } else {
+                        const double d = sqrt(piv), rd = 1.0 / d;
+                        x[j] = (c == j) ? d : x[j] * rd;
+#pragma unroll
+                        for (int r = j + 1; r < NB; r++) x[r] -= readlane_d(x[j], r) * x[j];
+                    }
                 }
-                const double d = sqrt(piv), rd = 1.0 / d;
-                __builtin_amdgcn_wave_barrier();
-                if (c == j) sU[j + j * LDD] = d;
-                else if (c > j && c < NB) sU[j + c * LDD] *= rd;
-                __builtin_amdgcn_s_waitcnt(0xc07f);
-                __builtin_amdgcn_wave_barrier();
-                if (c > j && c < NB) {
-                    const double ujc = sU[j + c * LDD];
-                    for (int r = j + 1; r <= c; r++) sU[r + c * LDD] -= sU[j + r * LDD] * ujc;
-                }
-                __builtin_amdgcn_s_waitcnt(0xc07f);
-                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_sched_barrier(0);   // keep each step's broadcasts local
             }
+            if (lane < NB) {
+#pragma unroll
+                for (int r = 0; r < NB; r++)
+                    if (r <= c) sU[r + c * LDD] = x[r];
+            }
+            if (bad >= 0 && lane == 0) { s_fail = 1; if (info) *info = info_base + j0 + bad + 1; }
         }
         __syncthreads();
         for (int e = t; e < NB * NB; e += PF_THREADS) {
@@ -223,10 +232,10 @@ __global__ __launch_bounds__(PF_THREADS) void potrf2_kernel(double *A, int b, in
                 if (e0 + g < ntri) {
                     int bi, bj;
                     tri_ij(e0 + g, bi, bj);
-                    const int j = base + 16 * bj + (lane & 15);
+                    const int i = base + 16 * bi + (lane & 15);
 #pragma unroll
                     for (int v = 0; v < 4; v++)
-                        cv[g][v] = A[(size_t)(base + 16 * bi + (lane >> 4) + 4 * v) + (size_t)j * b];
+                        cv[g][v] = A[(size_t)i + (size_t)(base + 16 * bj + (lane >> 4) + 4 * v) * b];
                 }
             }
         };
@@ -245,17 +254,17 @@ __global__ __launch_bounds__(PF_THREADS) void potrf2_kernel(double *A, int b, in
                 const int q = k4 + (lane >> 4);
 #pragma unroll
                 for (int g = 0; g < PF_G; g++)
-                    if (ti[g] >= 0)
-                        acc[g] = mfma16(sP[q * LDP + base + 16 * ti[g] + (lane & 15)],
-                                        sP[q * LDP + base + 16 * tj[g] + (lane & 15)], acc[g]);
+                    if (ti[g] >= 0)   // operands swapped: D holds the tile transposed
+                        acc[g] = mfma16(sP[q * LDP + base + 16 * tj[g] + (lane & 15)],
+                                        sP[q * LDP + base + 16 * ti[g] + (lane & 15)], acc[g]);
             }
 #pragma unroll
             for (int g = 0; g < PF_G; g++) {
                 if (ti[g] < 0) continue;
-                const int j = base + 16 * tj[g] + (lane & 15);
+                const int i = base + 16 * ti[g] + (lane & 15);
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int i = base + 16 * ti[g] + (lane >> 4) + 4 * v;
+                    const int j = base + 16 * tj[g] + (lane >> 4) + 4 * v;
                     if (ti[g] < tj[g] || i <= j) A[(size_t)i + (size_t)j * b] = cv[g][v] - acc[g][v];
                 }
             }
@@ -300,41 +309,42 @@ __global__ __launch_bounds__(PF_THREADS) void getrf2_kernel(double *A, int b, do
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
 
     for (int j0 = 0; j0 < b; j0 += NB) {
-        // 1. diagonal block, one element pair per thread (rows r0 and r0+16 of
-        //    column c); right-looking no-pivot LU through LDS, two barriers per
-        //    column (src/srclu/dgetrf_no_pivot.h:18-55 on the 32x32 block)
-        {
-            const bool own = t < 16 * NB;
-            const int c = t / 16, r0 = t % 16;
-            double v0 = 0.0, v1 = 0.0;
-            if (own) {
-                v0 = A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b];
-                v1 = A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b];
-                sD[r0 + c * LDD] = v0;
-                sD[r0 + 16 + c * LDD] = v1;
-            }
-            if (t == 0) s_slow = 0;
-            __syncthreads();
+        // 1. diagonal block, factored by wave 0 in registers with lane r holding
+        //    row r: right-looking no-pivot LU (src/srclu/dgetrf_no_pivot.h:18-55
+        //    on the 32x32 block).  Column i's multipliers are scaled in
+        //    parallel across lanes (one piv_scale2 per lane); row i of U
+        //    reaches the other lanes through v_readlane.
+        for (int e = t; e < NB * NB; e += PF_THREADS)
+            sD[(e % NB) + (e / NB) * LDD] = A[(size_t)(j0 + e % NB) + (size_t)(j0 + e / NB) * b];
+        __syncthreads();
+        if (w == 0) {
+            const int r = lane & (NB - 1);
+            double x[NB];
+#pragma unroll
+            for (int c = 0; c < NB; c++) x[c] = sD[r + c * LDD];
+            bool slow = false;
+#pragma unroll
             for (int i = 0; i < NB; i++) {
-                const double alpha = sD[i + i * LDD];
-                if (own && c == i) {
-                    if (r0 > i) { v0 = piv_scale2(v0, alpha); sD[r0 + i * LDD] = v0; }
-                    if (r0 + 16 > i) { v1 = piv_scale2(v1, alpha); sD[r0 + 16 + i * LDD] = v1; }
+                const double alpha = readlane_d(x[i], i);
+                slow |= !(fabs(alpha) > DBL_MIN);
+                if (r > i) x[i] = piv_scale2(x[i], alpha);
+                const double l = x[i];
+#pragma unroll
+                for (int c = i + 1; c < NB; c++) {
+                    const double u = readlane_d(x[c], i);
+                    if (r > i) x[c] -= l * u;
                 }
-                if (t == 0 && !(fabs(alpha) > DBL_MIN)) s_slow = 1;
-                __syncthreads();
-                if (own && c > i) {
-                    const double u = sD[i + c * LDD];
-                    if (r0 > i) { v0 -= sD[r0 + i * LDD] * u; sD[r0 + c * LDD] = v0; }
-                    if (r0 + 16 > i) { v1 -= sD[r0 + 16 + i * LDD] * u; sD[r0 + 16 + c * LDD] = v1; }
-                }
-                __syncthreads();
+                __builtin_amdgcn_sched_barrier(0);   // keep each step's broadcasts local
             }
-            if (own) {
-                A[(size_t)(j0 + r0) + (size_t)(j0 + c) * b] = v0;
-                A[(size_t)(j0 + r0 + 16) + (size_t)(j0 + c) * b] = v1;
+            if (lane < NB) {
+#pragma unroll
+                for (int c = 0; c < NB; c++) sD[r + c * LDD] = x[c];
             }
+            if (lane == 0) s_slow = slow ? 1 : 0;
         }
+        __syncthreads();
+        for (int e = t; e < NB * NB; e += PF_THREADS)
+            A[(size_t)(j0 + e % NB) + (size_t)(j0 + e / NB) * b] = sD[(e % NB) + (e / NB) * LDD];
         tri_inverse32<true>(sD, sUi, sT, PF_THREADS);
         tri_inverse32<false>(sD, sLi, sT, PF_THREADS);
         for (int e = t; e < NB * NB; e += PF_THREADS) {
@@ -376,14 +386,15 @@ __global__ __launch_bounds__(PF_THREADS) void getrf2_kernel(double *A, int b, do
                 for (int k4 = 0; k4 < NB; k4 += 4) {
                     const int q = k4 + (lane >> 4);
                     const double av = A[(size_t)(i0 + (lane & 15)) + (size_t)(j0 + q) * b];
-                    acc0 = mfma16(av, sUi[q + (lane & 15) * LDD], acc0);            // opB(q,c) = Uinv(q,c)
-                    acc1 = mfma16(av, sUi[q + (16 + (lane & 15)) * LDD], acc1);
+                    // D = X^T (rows of X along lanes): opA(c,q) = Uinv(q,c), opB(q,i) = A(i,q)
+                    acc0 = mfma16(sUi[q + (lane & 15) * LDD], av, acc0);
+                    acc1 = mfma16(sUi[q + (16 + (lane & 15)) * LDD], av, acc1);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int i = i0 + (lane >> 4) + 4 * v;
-                    A[(size_t)i + (size_t)(j0 + (lane & 15)) * b] = acc0[v];
-                    A[(size_t)i + (size_t)(j0 + 16 + (lane & 15)) * b] = acc1[v];
+                    const int i = i0 + (lane & 15), cc = j0 + (lane >> 4) + 4 * v;
+                    A[(size_t)i + (size_t)cc * b] = acc0[v];
+                    A[(size_t)i + (size_t)(cc + 16) * b] = acc1[v];
                 }
             } else if (lane < 16) {
                 // reference scaling rule, one row per lane: x U_D = a (rare path:
@@ -416,25 +427,29 @@ __global__ __launch_bounds__(PF_THREADS) void getrf2_kernel(double *A, int b, do
             for (int g = 0; g < GF_G; g++) {
                 acc[g] = zero4();
                 if (bj0 + g < nt) {
-                    const int j = base + 16 * (bj0 + g) + (lane & 15);
 #pragma unroll
-                    for (int v = 0; v < 4; v++) cv[g][v] = A[(size_t)(i0 + (lane >> 4) + 4 * v) + (size_t)j * b];
+                    for (int v = 0; v < 4; v++)
+                        cv[g][v] = A[(size_t)(i0 + (lane & 15)) +
+                                     (size_t)(base + 16 * (bj0 + g) + (lane >> 4) + 4 * v) * b];
                 }
             }
+            // operands swapped so that D holds the C tile transposed: 16 lanes
+            // then cover 16 consecutive rows (128 B) of one column of C
 #pragma unroll
             for (int k4 = 0; k4 < NB; k4 += 4) {
                 const int q = k4 + (lane >> 4);
 #pragma unroll
                 for (int g = 0; g < GF_G; g++)
                     if (bj0 + g < nt)
-                        acc[g] = mfma16(av[k4 / 4], sP[q * LDP + base + 16 * (bj0 + g) + (lane & 15)], acc[g]);
+                        acc[g] = mfma16(sP[q * LDP + base + 16 * (bj0 + g) + (lane & 15)], av[k4 / 4], acc[g]);
             }
 #pragma unroll
             for (int g = 0; g < GF_G; g++) {
                 if (bj0 + g >= nt) continue;
-                const int j = base + 16 * (bj0 + g) + (lane & 15);
 #pragma unroll
-                for (int v = 0; v < 4; v++) A[(size_t)(i0 + (lane >> 4) + 4 * v) + (size_t)j * b] = cv[g][v] - acc[g][v];
+                for (int v = 0; v < 4; v++)
+                    A[(size_t)(i0 + (lane & 15)) + (size_t)(base + 16 * (bj0 + g) + (lane >> 4) + 4 * v) * b] =
+                        cv[g][v] - acc[g][v];
             }
         }
         __syncthreads();

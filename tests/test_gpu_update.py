@@ -1,0 +1,71 @@
+"""GPU tests of one trailing update (the O(n^3) part of a factorization step)
+through the palial_hip_update test hook, in every precision tier.
+
+The model is numpy in FP64 on operands rounded exactly as the device rounds
+them (FP64 -> FP32 nearest, then BF16 nearest-even): the only remaining
+difference is the FP32 accumulation order, bounded by b * 2^-24 * sum |a||b|.
+The FP64 tier is checked with the FP64 unit roundoff in place of 2^-24.  The Cholesky update
+touches only tiles i <= j (and the upper triangle of diagonal tiles), as
+src/cholesky.h:86-170 does; the LU update touches every trailing tile
+(src/lu.h:122-170)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def round_fp32(x):
+    return x.astype(np.float32).astype(np.float64)
+
+
+def round_bf16(x):
+    u = x.astype(np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+ROUND = {0: lambda x: x, 1: round_fp32, 2: round_bf16}
+TOL = {0: 4.0 * 2.0 ** -53, 1: 4.0 * 2.0 ** -24, 2: 4.0 * 2.0 ** -24}
+
+
+@pytest.mark.parametrize("algo", ["cholesky", "lu"])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("N,b,k", [(1024, 256, 0), (1024, 256, 1), (1536, 512, 0), (768, 128, 2)])
+def test_trailing_update(hip, orc, algo, prec, N, b, k):
+    a = orc.gen("dense", N, b, seed=11 + k, shift=0.0)
+    A = orc.to_dense(a, N, b)
+    dm = hip.DeviceMatrix(N, b)
+    try:
+        dm.from_host(a)
+        hip.update(algo, dm, k, precision=prec)
+        out = orc.to_dense(dm.to_host(), N, b)
+    finally:
+        dm.free()
+    r = ROUND[prec]
+    s0, s1 = k * b, (k + 1) * b
+    exp = A.copy()
+    if algo == "cholesky":
+        U = r(A[s0:s1, s1:])
+        upd = U.T @ U
+        scale = np.abs(U).T @ np.abs(U)
+        # upper triangle of the trailing block, tile by tile (i <= j)
+        mask = np.zeros((N - s1, N - s1), dtype=bool)
+        idx = np.arange(N - s1)
+        ti = idx // b
+        same = ti[:, None] == ti[None, :]
+        mask |= ti[:, None] < ti[None, :]
+        mask |= same & (idx[:, None] <= idx[None, :])
+    else:
+        Lc = r(A[s1:, s0:s1])
+        Ur = r(A[s0:s1, s1:])
+        upd = Lc @ Ur
+        scale = np.abs(Lc) @ np.abs(Ur)
+        mask = np.ones((N - s1, N - s1), dtype=bool)
+    exp[s1:, s1:] = np.where(mask, A[s1:, s1:] - upd, A[s1:, s1:])
+    err = np.abs(out - exp)
+    # untouched regions are bitwise unchanged
+    assert np.array_equal(out[:s1, :], A[:s1, :])
+    assert np.array_equal(out[:, :s1], A[:, :s1])
+    assert np.array_equal(out[s1:, s1:][~mask], A[s1:, s1:][~mask])
+    bound = TOL[prec] * b * scale[mask] + 1e-300
+    assert (err[s1:, s1:][mask] <= bound).all(), float((err[s1:, s1:][mask] / bound).max())
