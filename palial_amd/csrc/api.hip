@@ -61,6 +61,8 @@ struct Ctx {
     size_t F_cap = 0;
     double *d_inv = nullptr;
     size_t inv_cap = 0;
+    double *d_F32 = nullptr;        // FP32 copies of the factor and the tile inverses
+    size_t F32_cap = 0;             // (in doubles: the buffer is reused as floats)
     double *d_vec = nullptr;
     size_t vec_cap = 0;
     std::map<std::tuple<int, void *, void *, int, int, int>, Graph> graphs;
@@ -629,19 +631,21 @@ int grow(double **p, size_t *cap, size_t need)
     return 0;
 }
 
-// Triangular sweeps with the factored tiles F and the inverted diagonal tiles:
+// Triangular sweeps with the factored tiles F and the inverted diagonal tiles
+// (Cholesky: Uinv holds U_kk^-T; LU: U_kk^-1 and L_kk^-1):
 //   Cholesky: U^T y = r, U x = y;  LU: L y = r, U x = y.
 // r is destroyed (used as the running right-hand side of the forward sweep).
-void sweeps(int algo, const double *F, const double *Uinv, const double *Linv, double *r, double *y, double *x,
-            int N, int b, int nrhs, hipStream_t s, Launcher &L)
+// f32: F, Uinv, Linv point at FP32 copies (mixed-precision correction solves)
+void sweeps(int algo, const void *F, const void *Uinv, const void *Linv, double *r, double *y, double *x,
+            int N, int b, int nrhs, bool f32, hipStream_t s, Launcher &L)
 {
     const int MT = N / b;
     GemvArgs a;
     memset(&a, 0, sizeof a);
-    a.mat = F; a.N = N; a.b = b; a.ld = N; a.nrhs = nrhs;
+    a.mat = F; a.N = N; a.b = b; a.ld = N; a.nrhs = nrhs; a.f32 = f32 ? 1 : 0;
     for (int k = 0; k < MT; k++) {
         a.k = k;
-        a.mode = GV_DIAG; a.inv = (algo == 0) ? Uinv : Linv; a.trans = (algo == 0); a.vin = r; a.vout = y; a.ntasks = 1;
+        a.mode = GV_DIAG; a.inv = (algo == 0) ? Uinv : Linv; a.trans = 0; a.vin = r; a.vout = y; a.ntasks = 1;
         L.chk(launch_tile_gemv(a, s));
         if (k + 1 < MT) {
             a.mode = (algo == 0) ? GV_CHOL_FWD : GV_LU_FWD; a.trans = (algo == 0); a.vin = y; a.vout = r;
@@ -651,7 +655,7 @@ void sweeps(int algo, const double *F, const double *Uinv, const double *Linv, d
     }
     for (int k = MT - 1; k >= 0; k--) {
         a.k = k;
-        a.mode = GV_DIAG; a.inv = Uinv; a.trans = 0; a.vin = y; a.vout = x; a.ntasks = 1;
+        a.mode = GV_DIAG; a.inv = Uinv; a.trans = (algo == 0); a.vin = y; a.vout = x; a.ntasks = 1;
         L.chk(launch_tile_gemv(a, s));
         if (k > 0) {
             a.mode = GV_BWD; a.trans = 0; a.vin = x; a.vout = y; a.ntasks = k;
@@ -684,6 +688,11 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     if (o.precision > 0 && b % 128 == 0 && (rc = ensure_pack(b, MT))) return rc;
     if ((rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
     if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
+    // mixed tiers: the correction solves stream FP32 copies (half the bytes);
+    // the factor itself carries BF16/FP32-level errors, so this costs nothing
+    const bool f32 = o.precision > 0;
+    if (f32 && (rc = grow(&g.d_F32, &g.F32_cap, (nn + (size_t)(algo == 0 ? 1 : 2) * MT * b * b) / 2 + 2))) return rc;
+    float *F32 = (float *)g.d_F32, *Inv32 = F32 + nn;
     const size_t npart = residual_scratch(N, bA, nrhs);
     if ((rc = grow(&g.d_vec, &g.vec_cap, 6 * nv + 2 * (size_t)N + 64 + npart))) return rc;
     double *X = g.d_vec, *R = X + nv, *Y = R + nv, *D = Y + nv, *Bd = D + nv, *R2 = Bd + nv;
@@ -719,45 +728,68 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     hipMemsetAsync(g.d_info, 0, sizeof(int), s);
     if (algo == 0) dag_cholesky(g.d_F, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision);
     else dag_lu(g.d_F, N, b, MT > 1, s, g.s_panel, L, o.precision);
-    L.chk(launch_trtri(g.d_F, N, b, g.d_dinv, Uinv, true, s));
-    if (algo == 1) L.chk(launch_trtri(g.d_F, N, b, g.d_dinv, Linv, false, s));
+    // inverses of the diagonal tiles: one batched strip TRSM per factor with the
+    // identity as right-hand side (Cholesky: U^-T; LU: U^-1 and L^-1)
+    auto inverses = [&](int kind, double *out, const double *dbase) {
+        TrsmArgs t;
+        memset(&t, 0, sizeof t);
+        t.mat = g.d_F; t.single = out; t.N = N; t.b = b; t.MT = MT; t.ntiles = MT;
+        t.batch = 1; t.ident = 1; t.dinv_stride = 64L * b;
+        L.chk(launch_trsm2(kind, t, dbase, s));
+    };
+    if (algo == 0) {
+        inverses(TR_CHOL, Uinv, g.d_dinv);
+    } else {
+        inverses(TR_LUR, Uinv, g.d_dinv);
+        inverses(TR_LUL, Linv, g.d_dinv + (size_t)32 * b);
+    }
+    if (f32) {
+        L.chk(launch_to_f32(g.d_F, F32, N, b, algo == 0, 0, s));
+        L.chk(launch_to_f32(Uinv, Inv32, 0, b, false, (size_t)(algo == 0 ? 1 : 2) * MT * b * b, s));
+    }
+    const void *sF = f32 ? (const void *)F32 : (const void *)g.d_F;
+    const void *sU = f32 ? (const void *)Inv32 : (const void *)Uinv;
+    const void *sL = f32 ? (const void *)(Inv32 + (size_t)MT * b * b) : (const void *)Linv;
     HIPCHK(hipEventRecord(e1, s));
     if (L.err != hipSuccess) return done(fail(PALIAL_ERR_HIP, "factorization launch: %s", hipGetErrorString(L.err)));
     int info = 0;
     HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-    // ||A||_inf, ||b||_inf
-    L.chk(launch_norm_inf(dA, N, bA, rowsum, red, s));
+    // ||b||_inf now; ||A||_inf comes with the first residual (same pass over A)
     L.chk(launch_colmax(Bd, N, N, nrhs, red + 1, s));
     // x0 = F^{-1} b
     L.chk(launch_copy(R, Bd, nv * 8, s));
-    sweeps(algo, g.d_F, Uinv, Linv, R, Y, X, N, b, nrhs, s, L);
-    double hred[40];
-    HIPCHK(hipMemcpyAsync(hred, red, (1 + nrhs) * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (info > 0) return done(fail(info, "factorization failed at column %d", info));
-    const double normA = hred[0];
-    double normB[16];
-    for (int j = 0; j < nrhs; j++) normB[j] = hred[1 + j];
+    sweeps(algo, sF, sU, sL, R, Y, X, N, b, nrhs, f32, s, L);
+    // the Cholesky residual reads only the upper tiles of A (the symmetric
+    // matrix the factorization sees)
+    const bool sym = (algo == 0 && nrhs == 1);
+    double hred[64];
+    double normA = 0.0, normB[16];
     double eta = 0.0;
     int it = 0;
     for (;; it++) {
         // r = b - A x ; eta = max_j ||r_j|| / (||A|| ||x_j|| + ||b_j||)
-        L.chk(launch_residual(dA, N, bA, X, Bd, R, N, nrhs, part, s));
-        L.chk(launch_colmax(R, N, N, nrhs, red + 1, s));
-        L.chk(launch_colmax(X, N, N, nrhs, red + 1 + nrhs, s));
-        HIPCHK(hipMemcpyAsync(hred, red + 1, 2 * nrhs * 8, hipMemcpyDeviceToHost, s));
+        L.chk(launch_residual(dA, N, bA, X, Bd, R, N, nrhs, part, sym, it == 0 ? rowsum : nullptr, s));
+        if (it == 0) L.chk(launch_colmax(rowsum, N, N, 1, red, s));
+        L.chk(launch_colmax(R, N, N, nrhs, red + 1 + nrhs, s));
+        L.chk(launch_colmax(X, N, N, nrhs, red + 1 + 2 * nrhs, s));
+        HIPCHK(hipMemcpyAsync(hred, red, (1 + 3 * nrhs) * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (L.err != hipSuccess) return done(fail(PALIAL_ERR_HIP, "refinement launch: %s", hipGetErrorString(L.err)));
+        if (it == 0) {
+            if (info > 0) return done(fail(info, "factorization failed at column %d", info));
+            normA = hred[0];
+            for (int j = 0; j < nrhs; j++) normB[j] = hred[1 + j];
+        }
         eta = 0.0;
         for (int j = 0; j < nrhs; j++) {
-            const double den = normA * hred[nrhs + j] + normB[j];
-            const double ej = den > 0 ? hred[j] / den : 0.0;
+            const double den = normA * hred[1 + 2 * nrhs + j] + normB[j];
+            const double ej = den > 0 ? hred[1 + nrhs + j] / den : 0.0;
             if (!(ej <= eta)) eta = ej;      // NaN propagates
         }
         if (eta <= tol || it >= max_iter) break;
         // d = F^{-1} r ; x += d
         L.chk(launch_copy(R2, R, nv * 8, s));
-        sweeps(algo, g.d_F, Uinv, Linv, R2, Y, D, N, b, nrhs, s, L);
+        sweeps(algo, sF, sU, sL, R2, Y, D, N, b, nrhs, f32, s, L);
         L.chk(launch_axpy(X, D, N, N, nrhs, s));
     }
     HIPCHK(hipEventRecord(e2, s));
@@ -1053,6 +1085,7 @@ int palial_hip_finalize(void)
     if (g.d_pack) hipFree(g.d_pack);
     if (g.d_F) hipFree(g.d_F);
     if (g.d_inv) hipFree(g.d_inv);
+    if (g.d_F32) hipFree(g.d_F32);
     if (g.d_vec) hipFree(g.d_vec);
     if (g.d_info) hipFree(g.d_info);
     if (g.s_bulk && g.s_bulk != g.s_main) hipStreamDestroy(g.s_bulk);

@@ -80,6 +80,11 @@ struct TrsmArgs {
     int N, b, MT, k;
     int first;           // first tile index along the panel (m or n)
     int ntiles;
+    // batch mode (trsm2 only): tile t of the launch solves with diagonal tile
+    // (t, t) of mat and dinv + t*dinv_stride, in place on single + t*b*b;
+    // ident: the right-hand side is the identity (computes the tile inverse)
+    int batch, ident;
+    long dinv_stride;
 };
 hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s);
 
@@ -106,18 +111,19 @@ hipError_t launch_gemm_lp(const GemmArgs &a, bool transA, bool transB, int prec,
 enum GemvMode : int { GV_DIAG = 0, GV_CHOL_FWD = 1, GV_LU_FWD = 2, GV_BWD = 3 };
 struct GemvArgs {
     int mode;
-    const double *mat;   // factored matrix (layout L)
-    const double *inv;   // inverses of the diagonal tiles, tile k at k*b*b
+    const void *mat;     // factored matrix (layout L), double or float (f32)
+    const void *inv;     // inverses of the diagonal tiles, tile k at k*b*b
     const double *vin;   // input vector block (N x nrhs, leading dimension ld)
     double *vout;        // output vector block
     int N, b, k, ld, nrhs, ntasks, trans;
+    int f32;             // mat / inv hold FP32 copies
 };
-hipError_t launch_trtri(const double *mat, int N, int b, const double *dinv, double *inv, bool upper, hipStream_t s);
 hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s);
+// FP32 copy of a layout-L matrix (N > 0; upper_only: tiles m <= n) or of n doubles (N == 0)
+hipError_t launch_to_f32(const double *src, float *dst, int N, int b, bool upper_only, size_t n, hipStream_t s);
 hipError_t launch_residual(const double *A, int N, int b, const double *x, const double *rhs, double *r, int ld,
-                           int nrhs, double *part, hipStream_t s);
+                           int nrhs, double *part, bool sym, double *rowsum, hipStream_t s);
 size_t residual_scratch(int N, int b, int nrhs);   // doubles of `part` launch_residual needs
-hipError_t launch_norm_inf(const double *A, int N, int b, double *rowsum, double *out, hipStream_t s);
 hipError_t launch_colmax(const double *v, int N, int ld, int nrhs, double *out, hipStream_t s);
 hipError_t launch_axpy(double *x, const double *d, int N, int ld, int nrhs, hipStream_t s);
 

@@ -18,6 +18,7 @@
 //   f32  16x16x4 : A[row = l%16][k = l/16],           B[k = l/16][col = l%16]
 //   f32 result   : D[row = 4*(l/16) + v][col = l%16]
 #include "gemm_common.hpp"
+#include <cstdlib>
 
 namespace palial {
 
@@ -118,12 +119,17 @@ __global__ __launch_bounds__(256) void pack_kernel(const double *mat, int b, int
     }
 }
 
-// 256 threads, 128 x 128 output block, 2 x 2 waves of 64 x 64, KB = 64.
-__global__ __launch_bounds__(256, 2) void gemm_bf16p_kernel(GemmArgs p)
+// 256 threads, 128 x 128 output block, 2 x 2 waves of 64 x 64, K slices of
+// KB.  The first PF fragment rows of D are loaded before the K loop and the
+// rest one row ahead in the epilogue, so the HBM latency of the C tile hides
+// under the MFMA work instead of following it.
+template <int KB, int MINB, int PF>
+__global__ __launch_bounds__(256, MINB) void gemm_bf16p_kernel(GemmArgs p)
 {
-    constexpr int BM = 128, KB = 64, WM = 64, FM = 4;
+    constexpr int BM = 128, WM = 64, FM = 4;
     constexpr int LDK = KB + 8;           // bf16 per LDS row ([mn][k], k contiguous)
     constexpr int BUF = BM * LDK;
+    constexpr int CH = KB / 16;           // 16-byte chunks (8 bf16) per thread and operand: KB/2 k values
     __shared__ unsigned short sA[2 * BUF];
     __shared__ unsigned short sB[2 * BUF];
 
@@ -143,29 +149,46 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16p_kernel(GemmArgs p)
     const unsigned short *PA = (const unsigned short *)p.pa + (size_t)tp.ia * b * b;
     const unsigned short *PB = (const unsigned short *)p.pb + (size_t)tp.ib * b * b;
 
+    // acc[x][c][v] = C(i0 + wm*64 + 16x + l%16, j0 + wn*64 + 16c + 4(l/16) + v)
+    auto cidx = [&](int x, int c, int v) -> size_t {
+        return (size_t)(i0 + wm * WM + x * 16 + (lane & 15)) + (size_t)(j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v) * b;
+    };
+    auto live = [&](int x, int c, int v) {
+        return !(tp.diag && i0 + wm * WM + x * 16 + (lane & 15) > j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v);
+    };
+    auto ldD = [&](int x, double (&dv)[FM][4]) {
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) dv[c][v] = (tp.D && live(x, c, v)) ? tp.D[cidx(x, c, v)] : 0.0;
+    };
+
     f4 acc[FM][FM];
 #pragma unroll
     for (int a = 0; a < FM; a++)
 #pragma unroll
         for (int c = 0; c < FM; c++) acc[a][c] = (f4){0.f, 0.f, 0.f, 0.f};
 
-    // each thread stages 32 consecutive k (64 B) of one row of each operand
-    const int row = t >> 1, kc = (t & 1) * 32;
-    uint4 ra[4], rb[4];
+    // each thread stages KB/2 consecutive k of one row of each operand
+    const int row = t >> 1, kc = (t & 1) * (KB / 2);
+    uint4 ra[CH], rb[CH];
     auto load = [&](int kb) {
         const uint4 *a = (const uint4 *)(PA + (size_t)(i0 + row) * b + kb + kc);
         const uint4 *bb = (const uint4 *)(PB + (size_t)(j0 + row) * b + kb + kc);
 #pragma unroll
-        for (int q = 0; q < 4; q++) { ra[q] = a[q]; rb[q] = bb[q]; }
+        for (int q = 0; q < CH; q++) { ra[q] = a[q]; rb[q] = bb[q]; }
     };
     auto store = [&](int buf) {
         uint4 *a = (uint4 *)(sA + buf * BUF + row * LDK + kc);
         uint4 *bb = (uint4 *)(sB + buf * BUF + row * LDK + kc);
 #pragma unroll
-        for (int q = 0; q < 4; q++) { a[q] = ra[q]; bb[q] = rb[q]; }
+        for (int q = 0; q < CH; q++) { a[q] = ra[q]; bb[q] = rb[q]; }
     };
 
     load(0);
+    double d0[FM][4], d1[FM][4], d2[FM][4];
+    if (PF >= 1) ldD(0, d0);
+    if (PF >= 2) ldD(1, d1);
     store(0);
     __syncthreads();
     int buf = 0;
@@ -192,8 +215,26 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16p_kernel(GemmArgs p)
         buf ^= 1;
     }
 
-    // acc[x][c][v] = C(i0 + wm*64 + 16x + l%16, j0 + wn*64 + 16c + 4(l/16) + v)
-    epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
+    const double alpha = p.alpha, beta = p.beta;
+    auto stC = [&](int x, const double (&dv)[FM][4]) {
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (!live(x, c, v)) continue;
+                const double r = alpha * (double)acc[x][c][v];
+                tp.C[cidx(x, c, v)] = tp.D ? r + beta * dv[c][v] : r;
+            }
+    };
+    // rows 0..3 with a rolling prefetch of the next row's D (FM == 4)
+    if (PF < 1) ldD(0, d0);
+    if (PF < 2) ldD(1, d1);
+    ldD(2, d2);
+    stC(0, d0);
+    ldD(3, d0);
+    stC(1, d1);
+    stC(2, d2);
+    stC(3, d0);
 }
 
 // FP32 operands, v_mfma_f32_16x16x4_f32; same blocking with KB = 32.
@@ -295,7 +336,16 @@ hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
         return hipErrorInvalidValue;
     const long nbd = a.b / 128;
     dim3 grid((unsigned)((long)a.ntasks * nbd * nbd)), block(256);
-    if (prec == 2) hipLaunchKernelGGL(gemm_bf16p_kernel, grid, block, 0, s, a);
+    if (prec == 2) {
+        static const int var = [] {
+            const char *e = getenv("PALIAL_PK_VARIANT");
+            return e ? atoi(e) : 0;
+        }();
+        if (var == 1) hipLaunchKernelGGL((gemm_bf16p_kernel<32, 3, 0>), grid, block, 0, s, a);
+        else if (var == 2) hipLaunchKernelGGL((gemm_bf16p_kernel<64, 2, 0>), grid, block, 0, s, a);
+        else if (var == 3) hipLaunchKernelGGL((gemm_bf16p_kernel<64, 2, 2>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gemm_bf16p_kernel<32, 2, 2>), grid, block, 0, s, a);
+    }
     else if (prec == 1) hipLaunchKernelGGL(gemm_f32p_kernel, grid, block, 0, s, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();

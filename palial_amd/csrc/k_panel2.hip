@@ -473,22 +473,29 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
     const int c0 = strip * SW;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     double *X;
-    if (p.mat) {
+    const double *D = p.diag;
+    if (p.batch) {
+        D = p.mat + tile_off(b, p.MT, tix, tix);
+        X = p.single + (size_t)tix * b * b;
+        dinv += (size_t)tix * p.dinv_stride;
+    } else if (p.mat) {
         const int idx = p.first + tix;
         X = (KIND == TR_LUR) ? p.mat + tile_off(b, p.MT, idx, p.k) : p.mat + tile_off(b, p.MT, p.k, idx);
     } else {
         X = p.single;
     }
-    const double *D = p.diag;
     // M(i,q): CHOL/LUR M = U^T -> D[q + i*b];  LUL M = L -> D[i + q*b]
     for (int e0 = t; e0 < b * SW; e0 += P2_THREADS * 8) {
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int e = e0 + u * P2_THREADS;
-            if (e < b * SW)
-                v[u] = (KIND == TR_LUR) ? X[(size_t)(c0 + e % SW) + (size_t)(e / SW) * b]
-                                        : X[(size_t)(e % b) + (size_t)(c0 + e / b) * b];
+            if (e < b * SW) {
+                // element (row, col) of the tile this strip holds
+                const int row = (KIND == TR_LUR) ? c0 + e % SW : e % b;
+                const int col = (KIND == TR_LUR) ? e / SW : c0 + e / b;
+                v[u] = p.ident ? (row == col ? 1.0 : 0.0) : X[(size_t)row + (size_t)col * b];
+            }
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {

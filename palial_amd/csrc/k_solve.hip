@@ -1,102 +1,20 @@
 // k_solve.hip -- kernels of the solve / iterative-refinement path.
 //
-//   trtri_kernel   : full inverse of every diagonal tile of a factored
-//                    matrix (upper U_kk^-1 or unit-lower L_kk^-1), one
-//                    workgroup per tile, from the 32x32 block inverses the
-//                    panel kernels left in dinv.  Runs once after the
-//                    factorization, off its critical path, and turns each
-//                    diagonal solve of the triangular sweeps into a GEMV.
 //   tile_gemv      : batched out -= op(T) v (or out = op(T) v) over a list
 //                    of b x b tiles of layout L, nrhs <= 16 right-hand
 //                    sides; the two launches of one sweep step.
 //   residual       : r = b - A x over the FP64 layout-L matrix (HBM-bound),
-//                    chunked partial sums + a deterministic reduction.
-//   rowsum_abs     : row sums of |A| (for ||A||_inf).
+//                    partial sums in fixed slots + a deterministic reduction;
+//                    optionally the row sums of |A| (||A||_inf) in the same
+//                    pass; symmetric matrices read only their upper tiles.
 //   colmax_abs     : per-column max |v| of an N x nrhs block.
 //   axpy_cols      : x += d.
 #include "internal.hpp"
+#include "gemm_common.hpp"
 
 namespace palial {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ inline d4 mfma16s(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-
-// ---------------------------------------------------------------- TRTRI
-// X = U^{-1} for an upper-triangular b x b tile, column block by column
-// block (32 wide):  X[0:j, j] = -X[0:j, 0:j] (U[0:j, j] X_jj),  X_jj = U_jj^{-1}.
-// X = L^{-1} for a unit-lower tile, column blocks from the last:
-//   X[j+1:, j] = -X[j+1:, j+1:] (L[j+1:, j] X_jj).
-// Everything below (upper) / above (lower) the block diagonal is zero.
-constexpr int TT_THREADS = 256;
-
-template <bool UPPER>
-__global__ __launch_bounds__(TT_THREADS) void trtri_kernel(const double *mat, int N, int b, const double *dinv,
-                                                          double *inv)
-{
-    constexpr int NB = 32;
-    __shared__ double sW[512 * NB];      // W = U[0:j,j] X_jj (upper) or L[j+1:,j] X_jj (lower), b <= 512
-    const int k = blockIdx.x;
-    const int MT = N / b;
-    const double *T = mat + tile_off(b, MT, k, k);
-    double *X = inv + (size_t)k * b * b;
-    const double *Dk = dinv + (size_t)k * 2 * 32 * b + (UPPER ? 0 : (size_t)32 * b);
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int nbk = b / NB;
-    // zero + diagonal blocks
-    for (size_t e = t; e < (size_t)b * b; e += TT_THREADS) {
-        const int r = (int)(e % b), c = (int)(e / b);
-        double v = 0.0;
-        if (r / NB == c / NB) v = Dk[(size_t)(r / NB) * NB * NB + (r % NB) + (size_t)(c % NB) * NB];
-        X[e] = v;
-    }
-    __syncthreads();
-    for (int step = 1; step < nbk; step++) {
-        const int j = UPPER ? step : nbk - 1 - step;       // column block being completed
-        const int c0 = j * NB;
-        const int r_lo = UPPER ? 0 : c0 + NB, r_hi = UPPER ? c0 : b;   // off-diagonal rows of column block j
-        const int nr = r_hi - r_lo;
-        const double *Xjj = Dk + (size_t)j * NB * NB;
-        // W(r, c) = sum_q T(r_lo + r, c0 + q) Xjj(q, c)
-        for (int e = t; e < nr * NB; e += TT_THREADS) {
-            const int r = e % nr, c = e / nr;
-            double s = 0.0;
-            if (UPPER) {
-                for (int q = 0; q <= c; q++) s += T[(size_t)(r_lo + r) + (size_t)(c0 + q) * b] * Xjj[q + c * NB];
-            } else {
-                for (int q = c; q < NB; q++) s += T[(size_t)(r_lo + r) + (size_t)(c0 + q) * b] * Xjj[q + c * NB];
-            }
-            sW[r + c * 512] = s;
-        }
-        __syncthreads();
-        // X[r_lo:r_hi, j] = -X[r_lo:r_hi, r_lo:r_hi] W : 16x16 output tiles, MFMA,
-        // K restricted to the triangle of X
-        const int ntr = nr / 16;
-        for (int e = w; e < ntr * 2; e += TT_THREADS / 64) {
-            const int it = e >> 1, ct = e & 1;
-            const int i0 = r_lo + it * 16;
-            d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
-            const int kbeg = UPPER ? i0 : r_lo, kend = UPPER ? r_hi : i0 + 16;
-            for (int k4 = kbeg; k4 < kend; k4 += 4) {
-                const int q = k4 + (lane >> 4);
-                acc = mfma16s(X[(size_t)(i0 + (lane & 15)) + (size_t)q * b], sW[(q - r_lo) + (ct * 16 + (lane & 15)) * 512], acc);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                X[(size_t)(i0 + (lane >> 4) + 4 * v) + (size_t)(c0 + ct * 16 + (lane & 15)) * b] = -acc[v];
-        }
-        __syncthreads();
-    }
-}
-
-hipError_t launch_trtri(const double *mat, int N, int b, const double *dinv, double *inv, bool upper, hipStream_t s)
-{
-    if (b > 512 || b % 32) return hipErrorInvalidValue;
-    const int MT = N / b;
-    if (upper) hipLaunchKernelGGL(trtri_kernel<true>, dim3(MT), dim3(TT_THREADS), 0, s, mat, N, b, dinv, inv);
-    else hipLaunchKernelGGL(trtri_kernel<false>, dim3(MT), dim3(TT_THREADS), 0, s, mat, N, b, dinv, inv);
-    return hipGetLastError();
-}
 
 // ---------------------------------------------------------------- tile GEMV
 // For each task t: out_t(0:b, 0:nrhs) = op(T_t) v  (GV_DIAG) or
@@ -110,7 +28,9 @@ constexpr int GV_THREADS = 256;
 constexpr int GV_ROWS = 16;          // output rows per workgroup
 constexpr int GV_MAXR = 16;          // right-hand sides
 
-template <int NR>
+// E: element type of the tiles (double, or float for the FP32 copies the
+// mixed-precision correction solves stream); sums are accumulated in FP64.
+template <int NR, typename E>
 __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 {
     extern __shared__ double sm[];
@@ -120,15 +40,17 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
     const int rb_per_tile = b / GV_ROWS;
     const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
     const int t = threadIdx.x;
-    const double *T;
+    const E *mat = (const E *)p.mat;
+    const E *inv = (const E *)p.inv;
+    const E *T;
     int vin, vout;
     {
         const int MT = p.N / b, k = p.k;
         switch (p.mode) {
-        case GV_DIAG: T = p.inv + (size_t)k * b * b; vin = k; vout = k; break;
-        case GV_CHOL_FWD: T = p.mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
-        case GV_LU_FWD: T = p.mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
-        default: T = p.mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
+        case GV_DIAG: T = inv + (size_t)k * b * b; vin = k; vout = k; break;
+        case GV_CHOL_FWD: T = mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
+        case GV_LU_FWD: T = mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
+        default: T = mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
         }
     }
     const double *vi = p.vin + (size_t)vin * b;
@@ -144,10 +66,10 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
     for (int j = 0; j < NR; j++) acc[j] = 0.0;
     if (!p.trans) {
         const int r = t % GV_ROWS, g = t / GV_ROWS;       // g: column group, columns c = g, g+16, ...
-        const double *Tr = T + r0 + r;
+        const E *Tr = T + r0 + r;
 #pragma unroll 8
         for (int c = g; c < b; c += 16) {
-            const double a = Tr[(size_t)c * b];
+            const double a = (double)Tr[(size_t)c * b];
 #pragma unroll
             for (int j = 0; j < NR; j++) acc[j] += a * sv[c + j * b];
         }
@@ -173,7 +95,7 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
             const int c = c0 + lane;
             double a[4];
 #pragma unroll
-            for (int x = 0; x < 4; x++) a[x] = T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
+            for (int x = 0; x < 4; x++) a[x] = (double)T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
 #pragma unroll
             for (int x = 0; x < 4; x++)
 #pragma unroll
@@ -209,29 +131,70 @@ hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
     if (a.b > 512 || a.b % 64 || a.nrhs > GV_MAXR || a.nrhs < 1) return hipErrorInvalidValue;
     dim3 grid((unsigned)(a.ntasks * (a.b / GV_ROWS))), block(GV_THREADS);
     auto lds = [&](int nr) { return ((size_t)a.b * nr + 16 * GV_ROWS * nr) * 8; };
-    if (a.nrhs == 1) hipLaunchKernelGGL(tile_gemv_kernel<1>, grid, block, lds(1), s, a);
-    else if (a.nrhs <= 4) hipLaunchKernelGGL(tile_gemv_kernel<4>, grid, block, lds(4), s, a);
-    else hipLaunchKernelGGL(tile_gemv_kernel<16>, grid, block, lds(16), s, a);
+    if (a.f32) {
+        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, float>), grid, block, lds(1), s, a);
+        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, float>), grid, block, lds(4), s, a);
+        else hipLaunchKernelGGL((tile_gemv_kernel<16, float>), grid, block, lds(16), s, a);
+    } else {
+        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, double>), grid, block, lds(1), s, a);
+        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, double>), grid, block, lds(4), s, a);
+        else hipLaunchKernelGGL((tile_gemv_kernel<16, double>), grid, block, lds(16), s, a);
+    }
+    return hipGetLastError();
+}
+
+// FP64 -> FP32 copy of the tiles the sweeps read (upper_only: tiles (m, n),
+// m <= n, as the Cholesky sweeps use) or of a plain array (N = 0: n elements)
+__global__ __launch_bounds__(256) void to_f32_kernel(const double *src, float *dst, int N, int b, int upper_only,
+                                                     size_t n)
+{
+    const size_t total = N ? (size_t)N * N : n;
+    const size_t bb = (size_t)b * b;
+    const int MT = N ? N / b : 1;
+    for (size_t e = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; e < total;
+         e += (size_t)gridDim.x * blockDim.x * 4) {
+        if (N && upper_only) {
+            // layout L: tile (m, n) occupies [bb*(m + MT*n), +bb)
+            const size_t tile = e / bb;
+            const int m = (int)(tile % MT), nn = (int)(tile / MT);
+            if (m > nn) continue;
+        }
+        const double2 x = *(const double2 *)(src + e), y = *(const double2 *)(src + e + 2);
+        *(float4 *)(dst + e) = make_float4((float)x.x, (float)x.y, (float)y.x, (float)y.y);
+    }
+}
+
+hipError_t launch_to_f32(const double *src, float *dst, int N, int b, bool upper_only, size_t n, hipStream_t s)
+{
+    const size_t total = N ? (size_t)N * N : n;
+    if (total % 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(to_f32_kernel, dim3(8192), dim3(256), 0, s, src, dst, N, b, upper_only ? 1 : 0, n);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- residual
-// r = b - A x for the FP64 layout-L matrix in two deterministic passes:
-// partial sums over chunks of RES_CHUNK tile columns, then the reduction.
+// r = b - A x for the FP64 layout-L matrix, in two deterministic passes:
+// partial sums into fixed slots, then a reduction over the slots in order.
+// NORM additionally accumulates the row sums of |A| (for ||A||_inf) in the
+// same pass over A.
+//
+// General matrices: a workgroup covers 64 rows of one tile row and a chunk
+// of RES_CHUNK tile columns; slot = chunk.
 constexpr int RES_CHUNK = 8;
 
-template <int NR>
+template <int NR, bool NORM>
 __global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, int N, int bt, const double *x,
-                                                               int ld, int nrhs, double *part)
+                                                               int ld, int nrhs, double *part, double *partn)
 {
     __shared__ double sred[4][64 * NR];
+    __shared__ double snrm[4][64];
     const int MT = N / bt;
     const int rows_per = bt / 64;
     const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
     const int rowblk = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
     const int m = rowblk / rows_per, r0 = (rowblk % rows_per) * 64;
     const int t = threadIdx.x, rr = t % 64, q = t / 64;
-    double acc[NR];
+    double acc[NR], nacc = 0.0;
 #pragma unroll
     for (int j = 0; j < NR; j++) acc[j] = 0.0;
     const int n1 = min(MT, (chunk + 1) * RES_CHUNK);
@@ -241,6 +204,7 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, 
 #pragma unroll 8
         for (int c = q; c < bt; c += 4) {
             const double a = T[(size_t)c * bt];
+            if (NORM) nacc += fabs(a);
 #pragma unroll
             for (int j = 0; j < NR; j++)
                 if (j < nrhs) acc[j] += a * xs[(size_t)c + (size_t)j * ld];
@@ -248,40 +212,150 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, 
     }
 #pragma unroll
     for (int j = 0; j < NR; j++) sred[q][rr + j * 64] = acc[j];
+    if (NORM) snrm[q][rr] = nacc;
     __syncthreads();
     for (int e = t; e < 64 * nrhs; e += 256) {
         const int i = e % 64, j = e / 64;
-        // part layout: [chunk][j][row]
+        // part layout: [slot][j][row]
         part[((size_t)chunk * NR + j) * N + (size_t)m * bt + r0 + i] = sred[0][e] + sred[1][e] + sred[2][e] + sred[3][e];
+    }
+    if (NORM && t < 64)
+        partn[(size_t)chunk * N + (size_t)m * bt + r0 + t] = snrm[0][t] + snrm[1][t] + snrm[2][t] + snrm[3][t];
+}
+
+// Symmetric A given by its upper tiles (the Cholesky solves; one right-hand
+// side): a workgroup per tile (i, j), i <= j, reads the tile once and
+// produces A_ij x_j (slot j of row block i) and A_ij^T x_i (slot i of row
+// block j); the diagonal tile uses its upper triangle for both and adds the
+// two into slot i.  Lane l of wave w holds rows l + 64g of the tile for the
+// row groups g = w, w+4 (b <= 512); the transposed sums reduce across lanes.
+template <bool NORM>
+__global__ __launch_bounds__(256) void residual_sym_kernel(const double *A, int N, int bt, const double *x,
+                                                           double *part, double *partn)
+{
+    __shared__ double sxi[512], sxj[512];
+    __shared__ double scol[4][512];
+    __shared__ double sncol[4][512];
+    const int MT = N / bt;
+    int ti, tj;
+    tri_decode((int)blockIdx.x, ti, tj);                  // ti <= tj
+    const double *T = A + tile_off(bt, MT, ti, tj);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int e = t; e < bt; e += 256) {
+        sxi[e] = x[(size_t)ti * bt + e];
+        sxj[e] = x[(size_t)tj * bt + e];
+    }
+    __syncthreads();
+    const int ngrp = bt / 64;
+    const bool diag = (ti == tj);
+    double accd[2] = {0.0, 0.0}, nd[2] = {0.0, 0.0}, xr[2] = {0.0, 0.0};
+    int row[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        row[u] = lane + 64 * (w + 4 * u);
+        if (w + 4 * u < ngrp) xr[u] = sxi[row[u]];
+    }
+    for (int c = 0; c < bt; c++) {
+        double s = 0.0, sn = 0.0;
+        const double xc = sxj[c];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (w + 4 * u >= ngrp) continue;
+            const double a = T[(size_t)row[u] + (size_t)c * bt];
+            const bool dir = !diag || row[u] <= c, tr = !diag || row[u] < c;
+            if (dir) {
+                accd[u] += a * xc;
+                if (NORM) nd[u] += fabs(a);
+            }
+            if (tr) {
+                s += a * xr[u];
+                if (NORM) sn += fabs(a);
+            }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            s += __shfl_xor(s, o);
+            if (NORM) sn += __shfl_xor(sn, o);
+        }
+        if (lane == 0) {
+            scol[w][c] = s;
+            if (NORM) sncol[w][c] = sn;
+        }
+    }
+    __syncthreads();
+    // transposed sums of column c (fixed wave order), then the stores
+    for (int c = t; c < bt; c += 256) {
+        scol[0][c] = scol[0][c] + scol[1][c] + scol[2][c] + scol[3][c];
+        if (NORM) sncol[0][c] = sncol[0][c] + sncol[1][c] + sncol[2][c] + sncol[3][c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        if (w + 4 * u >= ngrp) continue;
+        const int r = row[u];
+        double v = accd[u], vn = nd[u];
+        if (diag) {
+            v += scol[0][r];
+            if (NORM) vn += sncol[0][r];
+        }
+        part[(size_t)tj * N + (size_t)ti * bt + r] = v;
+        if (NORM) partn[(size_t)tj * N + (size_t)ti * bt + r] = vn;
+    }
+    if (!diag) {
+        for (int c = t; c < bt; c += 256) {
+            part[(size_t)ti * N + (size_t)tj * bt + c] = scol[0][c];
+            if (NORM) partn[(size_t)ti * N + (size_t)tj * bt + c] = sncol[0][c];
+        }
     }
 }
 
 template <int NR>
-__global__ void residual_reduce_kernel(const double *part, int nchunk, const double *rhs, double *r, int N, int ld,
-                                       int nrhs)
+__global__ void residual_reduce_kernel(const double *part, const double *partn, int nslot, const double *rhs,
+                                       double *r, double *rowsum, int N, int ld, int nrhs)
 {
     const size_t total = (size_t)N * nrhs;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
         const size_t i = e % N, j = e / N;
         double s = 0.0;
-        for (int c = 0; c < nchunk; c++) s += part[((size_t)c * NR + j) * N + i];
+        for (int c = 0; c < nslot; c++) s += part[((size_t)c * NR + j) * N + i];
         r[i + j * ld] = rhs[i + j * ld] - s;
+        if (rowsum && j == 0) {
+            double sn = 0.0;
+            for (int c = 0; c < nslot; c++) sn += partn[(size_t)c * N + i];
+            rowsum[i] = sn;
+        }
     }
 }
 
 hipError_t launch_residual(const double *A, int N, int b, const double *x, const double *rhs, double *r, int ld,
-                           int nrhs, double *part, hipStream_t s)
+                           int nrhs, double *part, bool sym, double *rowsum, hipStream_t s)
 {
-    if (b % 64 || nrhs > GV_MAXR || nrhs < 1) return hipErrorInvalidValue;
+    if (b % 64 || b > 512 || nrhs > GV_MAXR || nrhs < 1) return hipErrorInvalidValue;
     const int MT = N / b;
+    if (sym && nrhs == 1) {
+        double *partn = part + (size_t)MT * N;
+        dim3 grid((unsigned)(MT * (MT + 1) / 2)), block(256);
+        if (rowsum) hipLaunchKernelGGL(residual_sym_kernel<true>, grid, block, 0, s, A, N, b, x, part, partn);
+        else hipLaunchKernelGGL(residual_sym_kernel<false>, grid, block, 0, s, A, N, b, x, part, partn);
+        hipLaunchKernelGGL(residual_reduce_kernel<1>, dim3(1024), dim3(256), 0, s, part, partn, MT, rhs, r, rowsum,
+                           N, ld, nrhs);
+        return hipGetLastError();
+    }
+    if (sym) return hipErrorInvalidValue;   // symmetric residual: one right-hand side
     const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
     dim3 grid((unsigned)(MT * (b / 64) * nchunk)), block(256);
+    const int NRk = (nrhs == 1) ? 1 : 16;
+    double *partn = part + (size_t)nchunk * NRk * N;
     if (nrhs == 1) {
-        hipLaunchKernelGGL(residual_partial_kernel<1>, grid, block, 0, s, A, N, b, x, ld, nrhs, part);
-        hipLaunchKernelGGL(residual_reduce_kernel<1>, dim3(1024), dim3(256), 0, s, part, nchunk, rhs, r, N, ld, nrhs);
+        if (rowsum) hipLaunchKernelGGL((residual_partial_kernel<1, true>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
+        else hipLaunchKernelGGL((residual_partial_kernel<1, false>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
+        hipLaunchKernelGGL(residual_reduce_kernel<1>, dim3(1024), dim3(256), 0, s, part, partn, nchunk, rhs, r, rowsum,
+                           N, ld, nrhs);
     } else {
-        hipLaunchKernelGGL(residual_partial_kernel<16>, grid, block, 0, s, A, N, b, x, ld, nrhs, part);
-        hipLaunchKernelGGL(residual_reduce_kernel<16>, dim3(1024), dim3(256), 0, s, part, nchunk, rhs, r, N, ld, nrhs);
+        if (rowsum) hipLaunchKernelGGL((residual_partial_kernel<16, true>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
+        else hipLaunchKernelGGL((residual_partial_kernel<16, false>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
+        hipLaunchKernelGGL(residual_reduce_kernel<16>, dim3(1024), dim3(256), 0, s, part, partn, nchunk, rhs, r,
+                           rowsum, N, ld, nrhs);
     }
     return hipGetLastError();
 }
@@ -290,25 +364,9 @@ size_t residual_scratch(int N, int b, int nrhs)
 {
     const int MT = N / b;
     const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
-    return (size_t)nchunk * (nrhs == 1 ? 1 : 16) * N;
-}
-
-// ||A||_inf: row sums of |A|, then max (two passes)
-__global__ __launch_bounds__(256) void rowsum_abs_kernel(const double *A, int N, int bt, double *rs)
-{
-    __shared__ double sred[4][64];
-    const int MT = N / bt;
-    const int rows_per = bt / 64;
-    const int m = blockIdx.x / rows_per, r0 = (blockIdx.x % rows_per) * 64;
-    const int t = threadIdx.x, rr = t % 64, q = t / 64;
-    double acc = 0.0;
-    for (int n = 0; n < MT; n++) {
-        const double *T = A + tile_off(bt, MT, m, n) + r0 + rr;
-        for (int c = q; c < bt; c += 4) acc += fabs(T[(size_t)c * bt]);
-    }
-    sred[q][rr] = acc;
-    __syncthreads();
-    if (t < 64) rs[(size_t)m * bt + r0 + t] = sred[0][t] + sred[1][t] + sred[2][t] + sred[3][t];
+    const size_t chunked = (size_t)nchunk * ((nrhs == 1 ? 1 : 16) + 1) * N;
+    const size_t sym = 2 * (size_t)MT * N;
+    return chunked > sym ? chunked : sym;
 }
 
 // per-column max |v| of an N x nrhs block (one workgroup per column)
@@ -325,14 +383,6 @@ __global__ __launch_bounds__(256) void colmax_abs_kernel(const double *v, int N,
         __syncthreads();
     }
     if (t == 0) out[j] = sred[0];
-}
-
-hipError_t launch_norm_inf(const double *A, int N, int b, double *rowsum, double *out, hipStream_t s)
-{
-    if (b % 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(rowsum_abs_kernel, dim3((unsigned)((N / b) * (b / 64))), dim3(256), 0, s, A, N, b, rowsum);
-    hipLaunchKernelGGL(colmax_abs_kernel, dim3(1), dim3(256), 0, s, rowsum, N, N, out);
-    return hipGetLastError();
 }
 
 hipError_t launch_colmax(const double *v, int N, int ld, int nrhs, double *out, hipStream_t s)
