@@ -26,14 +26,19 @@ extern "C" {
 #endif
 
 /* Byte-identical to the reference descriptor (include/descriptor.h:13-20):
- * field offsets 0/8/16/24/32, sizeof 40 on x86-64.  Passed by value. */
-typedef struct matrix_descriptor_t {
+ * field offsets 0/8/16/24/32, sizeof 40 on x86-64.  Passed by value.  A
+ * translation unit that already has the reference's typedef defines
+ * PALIAL_HIP_MATRIX_DESC before including this header. */
+#ifndef PALIAL_HIP_MATRIX_DESC
+#define PALIAL_HIP_MATRIX_DESC
+typedef struct palial_matrix_desc {
     int      tile_size;        /* b */
     long int matrix_nelements; /* N*N */
     long int tile_nelements;   /* b*b */
     int      matrix_size;      /* N */
     void    *matrix;           /* host or device pointer to layout-L storage */
 } MATRIX_desc;
+#endif
 
 /* error codes */
 #define PALIAL_OK            0
@@ -136,6 +141,10 @@ int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double
  *           reproducible: diagonal 173*(1 + U01), everything else 0.5
  *   kind 3: identity (include/populate.h:48,58)
  * U(seed,i,j) = (splitmix64(splitmix64(seed) + (i<<32 | j)) >> 11) * 2^-53 - 0.5 */
+#define PALIAL_GEN_SYM 0
+#define PALIAL_GEN_DENSE 1
+#define PALIAL_GEN_PALIAL 2
+#define PALIAL_GEN_IDENTITY 3
 int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift,
                         const palial_opts *opts);
 
@@ -180,6 +189,12 @@ int palial_hip_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, cons
                      palial_report *rep);
 
 /* ---------------------------------------------------------------- tile kernels */
+/* side / trans values of the reference (src/common.h:9-24; equal to the CBLAS
+ * enum values it passes through) */
+#define PALIAL_NOTRANS 111
+#define PALIAL_TRANS   112
+#define PALIAL_LEFT    141
+#define PALIAL_RIGHT   142
 /* Single-tile versions of the reference tile kernels; pointers follow
  * opts->device_matrix.  The leading dimensions must equal n (tile storage). */
 int palial_hip_dpotrf(int n, double *A, int lda, const palial_opts *opts);      /* LAPACKE_dpotrf('U') at src/cholesky.h:40 */

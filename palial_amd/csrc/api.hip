@@ -1382,7 +1382,7 @@ int palial_hip_dormqr(int side, int trans, int n, const double *V, int ldv, cons
         return rc;
     if (n == 0) return 0;
     // src/srcqr/dormqr.h:18-29: only (Left, Trans) applies Q^T correctly there
-    if (!(side == 141 && trans != 111)) return fail(PALIAL_ERR_ARG, "only side=Left, trans=Trans is supported");
+    if (!(side == PALIAL_LEFT && trans != PALIAL_NOTRANS)) return fail(PALIAL_ERR_ARG, "only side=Left, trans=Trans is supported");
     const bool dev = opts && opts->device_matrix;
     Stage st((size_t)n * n * 8, dev);
     double *dV = st.in(V, false), *dT = st.in(T, false), *dC = st.in(C, true);
@@ -1423,7 +1423,7 @@ int palial_hip_dtsmqr(int side, int trans, int M, int N, int K, int ib, double *
     int rc;
     if ((rc = ensure_init())) return rc;
     if (M == 0 || N == 0 || K == 0 || ib == 0) return 0;
-    if (!(side == 141 && trans != 111)) return fail(PALIAL_ERR_ARG, "only side=Left, trans=Trans is supported");
+    if (!(side == PALIAL_LEFT && trans != PALIAL_NOTRANS)) return fail(PALIAL_ERR_ARG, "only side=Left, trans=Trans is supported");
     if (!(M == N && N == K && K == ib)) return fail(PALIAL_ERR_ARG, "only M = N = K = ib (every reference call) is supported");
     const int n = M;
     if ((rc = check_tile(n, lda1, A1)) || (rc = check_tile(n, lda2, A2)) || (rc = check_tile(n, ldv, V)) ||
