@@ -13,11 +13,14 @@ triangular solves + every refinement step, with A and b resident in HBM.
 Prints ONE JSON line.  ``value`` is the Cholesky solve's effective
 throughput N^3/3 / time-to-solution of the fastest tier that reaches 1e-14
 (TFLOP/s); the FP64 tier and the LU solves are reported beside it.
-``roofline`` describes the dominant kernel (the batched FP64 trailing-update
-GEMM) from live HIP-event timing of every launch in one profiled plain FP64
-factorization (A = S + N*I); ``cpu_baseline`` times the CPU oracle's OpenMP
-task DAG (OpenBLAS tile kernels, one BLAS thread per task) on a bounded
-sample.
+``roofline`` describes the dominant kernel of the headline solve (the packed
+low-precision trailing update, HBM-bound on the FP64 C tiles) from live
+HIP-event timing of every launch in one profiled solve; ``roofline_fp64_gemm``
+does the same for the FP64 update GEMM over one profiled plain FP64
+factorization (A = S + N*I).  ``traffic`` is the measured HBM bytes per
+launch from profiles/traffic.json.  ``cpu_baseline`` times the CPU oracle's
+OpenMP task DAG (OpenBLAS tile kernels, one BLAS thread per task) on a
+bounded sample.
 """
 import argparse
 import json
@@ -106,12 +109,39 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
                 if best is None or (r["converged"], -r["ms"]) > (best["converged"], -best["ms"]):
                     best = r
             out[f"{algo}_{name}"] = best
+    # profiled pass of the headline solve (fastest converged tier of the first
+    # algorithm): HIP events around every launch, on the stream it runs on
+    lead = algos[0]
+    conv = {k: v for k, v in out.items() if k.startswith(lead + "_") and v["converged"]}
+    prof = None
+    if conv:
+        key = min(conv, key=lambda k: conv[k]["ms"])
+        prec = {"fp64": 0, "fp32": 1, "bf16": 2}[key.split("_")[1]]
+        P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8))
+        P.synchronize()
+        P.profile(True)
+        P.solve(A, (dX, 1, N), algo=lead, precision=prec, device=True, max_iter=40, check=False,
+                factor_tile=conv[key]["factor_tile"])
+        prof = {c: P.profile_stats(c) for c in ("gemm", "pkgemm", "diag", "trsm", "pack")}
+        prof["tier"] = key
+        P.profile(False)
     P.lib().palial_hip_free(dX)
     A.free()
-    return out
+    return out, prof
 
 
-def cpu_baseline(N=16384, b=512):
+def load_traffic():
+    """Measured HBM bytes per launch of the dominant kernels (rocprofv3 FETCH_SIZE
+    and WRITE_SIZE, calibrated on a copy kernel), from profiles/traffic.json."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def cpu_baseline(N=32768, b=512):
     import oracle
     ob = oracle.use_openblas()
     oracle.openblas_threads(1)
@@ -142,7 +172,7 @@ def main():
     ap.add_argument("--b", type=int, default=512)
     ap.add_argument("--algos", default="cholesky,lu")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-n", type=int, default=32768)
     ap.add_argument("--kappa", type=float, default=1e2)
     ap.add_argument("--tiers", default="0,1,2")
     ap.add_argument("--factor-tiles", default="256,512", help="internal factorization tile sizes tried per tier")
@@ -157,19 +187,37 @@ def main():
     algos = args.algos.split(",")
     tiers = [int(x) for x in args.tiers.split(",")]
     fts = [int(x) for x in args.factor_tiles.split(",")]
-    solves = run_solves(P, args.n, args.b, args.kappa, args.steps, args.warmup, tiers, algos, fts)
+    solves, sprof = run_solves(P, args.n, args.b, args.kappa, args.steps, args.warmup, tiers, algos, fts)
     res = {}
     if not args.no_factor:
         for algo in algos:
             res[algo] = run_algo(P, algo, args.n, args.b, args.steps, args.warmup)
-    head = res.get("cholesky") or next(iter(res.values()), {"kernels": {"gemm": {"flops": 0, "ms": 0, "launches": 0}}})
-    g = head["kernels"]["gemm"]
-    achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
-    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TF, 4), "traffic": None,
-            "kernel": "gemm_mfma_kernel<128,true,false> (Cholesky trailing update, all launches of one factorization)",
-            "launches": g["launches"], "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4),
-            "flops_per_launch": g["flops"] / max(1, g["launches"])}
+    traffic = load_traffic()
+    head = res.get("cholesky") or next(iter(res.values()), None)
+    roof64 = None
+    if head:
+        g = head["kernels"]["gemm"]
+        achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+        t64 = traffic.get("gemm_mfma_kernel")
+        roof64 = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
+                  "frac": round(achieved / FP64_PEAK_TF, 4),
+                  "traffic": t64["hbm_bytes_per_launch"] if t64 else None,
+                  "kernel": "gemm_mfma_kernel<128,16,2,true,false> (FP64 Cholesky trailing update, every launch "
+                            "of one single-stream factorization)",
+                  "launches": g["launches"], "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4),
+                  "flops_per_launch": g["flops"] / max(1, g["launches"])}
+    roof = roof64
+    if sprof and sprof["pkgemm"]["launches"] > 0:
+        k = sprof["pkgemm"]
+        gbs = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+        tb = traffic.get("gemm_bf16p_kernel")
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": tb["hbm_bytes_per_launch"] if tb else None,
+                "kernel": f"gemm_bf16p_kernel (packed low-precision trailing update of the {sprof['tier']} solve; "
+                          "algorithmic bytes = FP64 C read + write + packed panel once)",
+                "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(1, k["launches"]), 4),
+                "bytes_per_launch": k["bytes"] / max(1, k["launches"]),
+                "tflops": round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 1)}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.b)
     lead = algos[0]
     conv = {k: v for k, v in solves.items() if k.startswith(lead + "_") and v["converged"]}
@@ -204,11 +252,14 @@ def main():
         "backward_error": best["backward_error"],
         "solves": solves,
         "roofline": roof,
+        "roofline_fp64_gemm": roof64,
+        "solve_kernels": {c: v for c, v in (sprof or {}).items() if c != "tier"},
         "cpu_baseline": cpu,
         "results": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in res.items()},
     }
     print(json.dumps(out), flush=True)
+    P.finalize()
     return 0
 
 

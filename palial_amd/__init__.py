@@ -344,7 +344,7 @@ def profile(enable):
     _check(lib().palial_hip_profile(int(bool(enable))))
 
 
-KCLASS = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4, "pack": 5}
+KCLASS = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4, "pack": 5, "pkgemm": 6}
 
 
 def profile_stats(cls):

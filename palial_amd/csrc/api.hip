@@ -190,7 +190,10 @@ int check_desc(const MATRIX_desc &A)
 // ------------------------------------------------------------------ profiling
 // Per-kernel-class HIP-event timing (stream-ordered launches only): each
 // launch of a class is bracketed by two events on the stream it runs on.
-enum KClass : int { KC_GEMM = 0, KC_DIAG = 1, KC_TRSM = 2, KC_TSQRT = 3, KC_QRUPD = 4, KC_PACK = 5, KC_N = 6 };
+// KC_PKGEMM: trailing updates from packed low-precision panels; their bytes
+// are the algorithmic HBM traffic (FP64 C read + write, the packed panel once)
+enum KClass : int { KC_GEMM = 0, KC_DIAG = 1, KC_TRSM = 2, KC_TSQRT = 3, KC_QRUPD = 4, KC_PACK = 5, KC_PKGEMM = 6,
+                    KC_N = 7 };
 struct ProfRec { int cls; double flops; double bytes; hipEvent_t a, b; };
 struct Prof {
     bool on = false;
@@ -363,7 +366,9 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
                 L.run(KC_PACK, 0, T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5)), sb,
                       [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, g.d_pack, sb); });
                 a.pa = a.pb = g.d_pack;
-                L.run(KC_GEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk, 3.0 * (ngemm + nsyrk) * tile_bytes(b), sb,
+                const double esz = prec == 2 ? 2.0 : 4.0;
+                L.run(KC_PKGEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk,
+                      2.0 * (ngemm + nsyrk) * tile_bytes(b) + T * esz * b * b, sb,
                       [&] { return launch_gemm_packed(a, prec, sb); });
             } else {
                 gemm(a, 2 * b3(b) * ngemm + b3(b) * nsyrk, ngemm + nsyrk, sb);
@@ -441,7 +446,8 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
                 a.pa = pL;
                 a.pb = pU;
                 const double nt = (double)T * T - 1;
-                L.run(KC_GEMM, 2 * b3(b) * nt, 3.0 * nt * tile_bytes(b), sb,
+                const double esz = prec == 2 ? 2.0 : 4.0;
+                L.run(KC_PKGEMM, 2 * b3(b) * nt, 2.0 * nt * tile_bytes(b) + 2.0 * T * esz * b * b, sb,
                       [&] { return launch_gemm_packed(a, prec, sb); });
             } else {
                 gemm(a, (double)T * T - 1, sb);

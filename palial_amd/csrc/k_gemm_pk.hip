@@ -237,6 +237,132 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16p_kernel(GemmArgs p)
     stC(3, d0);
 }
 
+// BF16 update with asynchronous global -> LDS loads (global_load_lds_dwordx4,
+// gfx950): NST LDS stages of KB = 32 K values, up to NST-1 slices in flight
+// per workgroup, one barrier per slice.  A slice of one operand is 128 rows x
+// 64 B; each wave's DMA instruction fills 1 KB of LDS (16 rows, lane l at byte
+// 16*l), and lane l fetches row 16j + l/4, 16-byte chunk (l%4) ^ ((row/4)%4)
+// of that row: the XOR swizzle makes each MFMA fragment read (16 rows x one
+// chunk per 16 lanes) hit 16 distinct 16-byte bank groups.
+// Waits are explicit (vmcnt counts this wave's DMA instructions, 4 per
+// slice) and the barriers carry no fence, so loads of later slices stay in
+// flight across them.
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ inline void wait_vm(int n)
+{
+    // s_waitcnt vmcnt(n) (expcnt and lgkmcnt left unconstrained); n in {0, 4, 8}
+    if (n >= 8) __builtin_amdgcn_s_waitcnt(0x0F78);
+    else if (n >= 4) __builtin_amdgcn_s_waitcnt(0x0F74);
+    else __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+
+template <int MINB>
+__global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
+{
+    constexpr int BM = 128, KB = 32, WM = 64, FM = 4, NST = 4;
+    constexpr int SLICE = BM * KB * 2;                  // bytes of one operand slice
+    __shared__ __attribute__((aligned(1024))) char lds[NST][2][SLICE];
+
+    const int b = p.b;
+    const int nbd = b / BM;
+    const int blk_per_task = nbd * nbd;
+    const int lid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int task = lid / blk_per_task, blk = lid % blk_per_task;
+    const int bi = blk % nbd, bj = blk / nbd;
+    if (task >= p.ntasks) return;
+    const TaskPtrs tp = decode(p, task);
+    if (!tp.valid) return;
+    if (tp.diag && bi > bj) return;
+    const int i0 = bi * BM, j0 = bj * BM;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    const unsigned short *PA = (const unsigned short *)p.pa + (size_t)tp.ia * b * b;
+    const unsigned short *PB = (const unsigned short *)p.pb + (size_t)tp.ib * b * b;
+    const int NS = b / KB;
+
+    auto issue = [&](int s) {
+        const int st = s % NST, kb = s * KB;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int j = w * 2 + q;                    // 1 KB chunk of the slice
+            const int row = 16 * j + (lane >> 2);
+            const int c = (lane & 3) ^ ((row >> 2) & 3);
+            __builtin_amdgcn_global_load_lds((const void *)(PA + (size_t)(i0 + row) * b + kb + c * 8),
+                                             (lds_void_t *)(lds[st][0] + j * 1024), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(PB + (size_t)(j0 + row) * b + kb + c * 8),
+                                             (lds_void_t *)(lds[st][1] + j * 1024), 16, 0, 0);
+        }
+    };
+    auto frag = [&](const char *base, int row) -> bf8 {
+        const int c = lane >> 4;
+        return *(const bf8 *)(base + (row * 4 + (c ^ ((row >> 2) & 3))) * 16);
+    };
+
+    f4 acc[FM][FM];
+#pragma unroll
+    for (int a = 0; a < FM; a++)
+#pragma unroll
+        for (int c = 0; c < FM; c++) acc[a][c] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    issue(0);
+    if (NS > 1) issue(1);
+    if (NS > 2) issue(2);
+    for (int s = 0; s < NS; s++) {
+        wait_vm(s + 2 < NS ? 8 : (s + 1 < NS ? 4 : 0));   // slice s has landed (this wave's part)
+        __builtin_amdgcn_s_barrier();                     // ... and every wave's part
+        asm volatile("" ::: "memory");
+        if (s + 3 < NS) issue(s + 3);                     // into the stage read at s-1 (all waves are past it)
+        const char *A = lds[s % NST][0], *B = lds[s % NST][1];
+        bf8 av[FM], bv[FM];
+#pragma unroll
+        for (int x = 0; x < FM; x++) av[x] = frag(A, wm * WM + x * 16 + (lane & 15));
+#pragma unroll
+        for (int c = 0; c < FM; c++) bv[c] = frag(B, wn * WM + c * 16 + (lane & 15));
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+            for (int c = 0; c < FM; c++)
+                acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], av[x], acc[x][c], 0, 0, 0);
+        asm volatile("" ::: "memory");
+    }
+
+    // acc[x][c][v] = C(i0 + wm*64 + 16x + l%16, j0 + wn*64 + 16c + 4(l/16) + v);
+    // D rows are loaded one fragment row ahead of the stores
+    auto cidx = [&](int x, int c, int v) -> size_t {
+        return (size_t)(i0 + wm * WM + x * 16 + (lane & 15)) + (size_t)(j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v) * b;
+    };
+    auto live = [&](int x, int c, int v) {
+        return !(tp.diag && i0 + wm * WM + x * 16 + (lane & 15) > j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v);
+    };
+    auto ldD = [&](int x, double (&dv)[FM][4]) {
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) dv[c][v] = (tp.D && live(x, c, v)) ? tp.D[cidx(x, c, v)] : 0.0;
+    };
+    const double alpha = p.alpha, beta = p.beta;
+    auto stC = [&](int x, const double (&dv)[FM][4]) {
+#pragma unroll
+        for (int c = 0; c < FM; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (!live(x, c, v)) continue;
+                const double r = alpha * (double)acc[x][c][v];
+                tp.C[cidx(x, c, v)] = tp.D ? r + beta * dv[c][v] : r;
+            }
+    };
+    double d0[FM][4], d1[FM][4];
+    ldD(0, d0);
+    ldD(1, d1);
+    stC(0, d0);
+    ldD(2, d0);
+    stC(1, d1);
+    ldD(3, d1);
+    stC(2, d0);
+    stC(3, d1);
+}
+
 // FP32 operands, v_mfma_f32_16x16x4_f32; same blocking with KB = 32.
 __global__ __launch_bounds__(256, 2) void gemm_f32p_kernel(GemmArgs p)
 {
@@ -341,10 +467,11 @@ hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
             const char *e = getenv("PALIAL_PK_VARIANT");
             return e ? atoi(e) : 0;
         }();
-        if (var == 1) hipLaunchKernelGGL((gemm_bf16p_kernel<32, 3, 0>), grid, block, 0, s, a);
+        if (var == 0) hipLaunchKernelGGL((gemm_bf16d_kernel<2>), grid, block, 0, s, a);
+        else if (var == 1) hipLaunchKernelGGL((gemm_bf16p_kernel<32, 3, 0>), grid, block, 0, s, a);
         else if (var == 2) hipLaunchKernelGGL((gemm_bf16p_kernel<64, 2, 0>), grid, block, 0, s, a);
         else if (var == 3) hipLaunchKernelGGL((gemm_bf16p_kernel<64, 2, 2>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((gemm_bf16p_kernel<32, 2, 2>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gemm_bf16p_kernel<32, 2, 2>), grid, block, 0, s, a);   // var 4
     }
     else if (prec == 1) hipLaunchKernelGGL(gemm_f32p_kernel, grid, block, 0, s, a);
     else return hipErrorInvalidValue;
