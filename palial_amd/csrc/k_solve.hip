@@ -227,49 +227,50 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, 
 // side): a workgroup per tile (i, j), i <= j, reads the tile once and
 // produces A_ij x_j (slot j of row block i) and A_ij^T x_i (slot i of row
 // block j); the diagonal tile uses its upper triangle for both and adds the
-// two into slot i.  Lane l of wave w holds rows l + 64g of the tile for the
-// row groups g = w, w+4 (b <= 512); the transposed sums reduce across lanes.
+// two into slot i.  Wave w owns the columns c = w, w+4, ...; lane l holds
+// rows l + 64q (q < b/64), so a column costs b/64 coalesced loads per lane
+// and one cross-lane reduction (its transposed sum, owned by this wave).
+// The direct partials of the four waves are combined in a fixed order.
 template <bool NORM>
 __global__ __launch_bounds__(256) void residual_sym_kernel(const double *A, int N, int bt, const double *x,
                                                            double *part, double *partn)
 {
-    __shared__ double sxi[512], sxj[512];
-    __shared__ double scol[4][512];
-    __shared__ double sncol[4][512];
+    __shared__ double sxj[512];
+    __shared__ double scol[512], sncol[512];
+    __shared__ double sdir[4][512], sndir[4][512];
     const int MT = N / bt;
     int ti, tj;
     tri_decode((int)blockIdx.x, ti, tj);                  // ti <= tj
     const double *T = A + tile_off(bt, MT, ti, tj);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    for (int e = t; e < bt; e += 256) {
-        sxi[e] = x[(size_t)ti * bt + e];
-        sxj[e] = x[(size_t)tj * bt + e];
+    for (int e = t; e < bt; e += 256) sxj[e] = x[(size_t)tj * bt + e];
+    const int nq = bt / 64;
+    const bool diag = (ti == tj);
+    double xr[8], accd[8], nd[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        xr[q] = (q < nq) ? x[(size_t)ti * bt + lane + 64 * q] : 0.0;
+        accd[q] = 0.0;
+        nd[q] = 0.0;
     }
     __syncthreads();
-    const int ngrp = bt / 64;
-    const bool diag = (ti == tj);
-    double accd[2] = {0.0, 0.0}, nd[2] = {0.0, 0.0}, xr[2] = {0.0, 0.0};
-    int row[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-        row[u] = lane + 64 * (w + 4 * u);
-        if (w + 4 * u < ngrp) xr[u] = sxi[row[u]];
-    }
-    for (int c = 0; c < bt; c++) {
-        double s = 0.0, sn = 0.0;
+    for (int c = w; c < bt; c += 4) {
         const double xc = sxj[c];
+        const double *col = T + (size_t)c * bt + lane;
+        double a[8];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-            if (w + 4 * u >= ngrp) continue;
-            const double a = T[(size_t)row[u] + (size_t)c * bt];
-            const bool dir = !diag || row[u] <= c, tr = !diag || row[u] < c;
-            if (dir) {
-                accd[u] += a * xc;
-                if (NORM) nd[u] += fabs(a);
-            }
-            if (tr) {
-                s += a * xr[u];
-                if (NORM) sn += fabs(a);
+        for (int q = 0; q < 8; q++) a[q] = (q < nq) ? col[64 * q] : 0.0;
+        double s = 0.0, sn = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int r = lane + 64 * q;
+            const bool dir = !diag || r <= c, tr = !diag || r < c;
+            const double ad = dir ? a[q] : 0.0, at = tr ? a[q] : 0.0;
+            accd[q] += ad * xc;
+            s += at * xr[q];
+            if (NORM) {
+                nd[q] += fabs(ad);
+                sn += fabs(at);
             }
         }
 #pragma unroll
@@ -278,34 +279,29 @@ __global__ __launch_bounds__(256) void residual_sym_kernel(const double *A, int 
             if (NORM) sn += __shfl_xor(sn, o);
         }
         if (lane == 0) {
-            scol[w][c] = s;
-            if (NORM) sncol[w][c] = sn;
+            scol[c] = s;
+            if (NORM) sncol[c] = sn;
         }
     }
-    __syncthreads();
-    // transposed sums of column c (fixed wave order), then the stores
-    for (int c = t; c < bt; c += 256) {
-        scol[0][c] = scol[0][c] + scol[1][c] + scol[2][c] + scol[3][c];
-        if (NORM) sncol[0][c] = sncol[0][c] + sncol[1][c] + sncol[2][c] + sncol[3][c];
-    }
-    __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-        if (w + 4 * u >= ngrp) continue;
-        const int r = row[u];
-        double v = accd[u], vn = nd[u];
+    for (int q = 0; q < 8; q++)
+        if (q < nq) {
+            sdir[w][lane + 64 * q] = accd[q];
+            if (NORM) sndir[w][lane + 64 * q] = nd[q];
+        }
+    __syncthreads();
+    for (int r = t; r < bt; r += 256) {
+        double v = sdir[0][r] + sdir[1][r] + sdir[2][r] + sdir[3][r];
+        double vn = NORM ? sndir[0][r] + sndir[1][r] + sndir[2][r] + sndir[3][r] : 0.0;
         if (diag) {
-            v += scol[0][r];
-            if (NORM) vn += sncol[0][r];
+            v += scol[r];
+            if (NORM) vn += sncol[r];
+        } else {
+            part[(size_t)ti * N + (size_t)tj * bt + r] = scol[r];
+            if (NORM) partn[(size_t)ti * N + (size_t)tj * bt + r] = sncol[r];
         }
         part[(size_t)tj * N + (size_t)ti * bt + r] = v;
         if (NORM) partn[(size_t)tj * N + (size_t)ti * bt + r] = vn;
-    }
-    if (!diag) {
-        for (int c = t; c < bt; c += 256) {
-            part[(size_t)ti * N + (size_t)tj * bt + c] = scol[0][c];
-            if (NORM) partn[(size_t)ti * N + (size_t)tj * bt + c] = sncol[0][c];
-        }
     }
 }
 
