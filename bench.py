@@ -210,10 +210,10 @@ def main():
     if sprof and sprof["pkgemm"]["launches"] > 0:
         k = sprof["pkgemm"]
         gbs = k["bytes"] / (k["ms"] * 1e-3) / 1e9
-        tb = traffic.get("gemm_bf16p_kernel")
+        tb = traffic.get("gemm_bf16d_kernel")
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": tb["hbm_bytes_per_launch"] if tb else None,
-                "kernel": f"gemm_bf16p_kernel (packed low-precision trailing update of the {sprof['tier']} solve; "
+                "kernel": f"gemm_bf16d_kernel (packed low-precision trailing update of the {sprof['tier']} solve; "
                           "algorithmic bytes = FP64 C read + write + packed panel once)",
                 "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(1, k["launches"]), 4),
                 "bytes_per_launch": k["bytes"] / max(1, k["launches"]),
