@@ -130,6 +130,37 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
     return out, prof
 
 
+def run_batched(P, count=10000, steps=3, seed=0):
+    """Batched small factorizations: `count` matrices with n ~ UniformInt[16, 512]
+    (seeded), even i Cholesky of S + n I, odd i LU of G + n I, one N(0,1)
+    right-hand side each, factored and solved in one launch per kind; FP64
+    and FP32.  Inputs are regenerated before each timed run."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = rng.integers(16, 513, count).astype(np.int32)
+    kind = (np.arange(count) % 2).astype(np.int32)
+    rhs = rng.standard_normal(int(n.sum()))
+    out = {}
+    for prec, name in ((0, "fp64"), (1, "fp32")):
+        bt = P.Batch(n, kind, precision=prec)
+        times = []
+        flat = rhs.astype(bt.dtype)
+        for i in range(steps + 1):
+            bt.generate(seed)
+            P._check(P.lib().palial_hip_memcpy_h2d(bt.B, flat.ctypes.data, flat.nbytes))
+            P.synchronize()
+            rep = bt.factor(solve=True)
+            if i > 0:
+                times.append(rep.factor_ms)
+        fails = int((bt.info != 0).sum())
+        bt.free()
+        t = sorted(times)[len(times) // 2]
+        out[name] = {"ms": round(t, 3), "gflop": round(rep.gflop, 2), "tflops": round(rep.gflop / t, 3),
+                     "failures": fails}
+        log(f"[bench] batched {count} matrices {name}: {t:.2f} ms ({rep.gflop / t:.2f} TFLOP/s factor+solve)")
+    return {"matrices": count, "n": "UniformInt[16,512] seed 0, even Cholesky / odd LU, 1 RHS", **out}
+
+
 def load_traffic():
     """Measured HBM bytes per launch of the dominant kernels (rocprofv3 FETCH_SIZE
     and WRITE_SIZE, calibrated on a copy kernel), from profiles/traffic.json."""
@@ -177,6 +208,7 @@ def main():
     ap.add_argument("--tiers", default="0,1,2")
     ap.add_argument("--factor-tiles", default="256,512", help="internal factorization tile sizes tried per tier")
     ap.add_argument("--no-factor", action="store_true", help="skip the plain factorization runs")
+    ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -218,6 +250,7 @@ def main():
                 "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(1, k["launches"]), 4),
                 "bytes_per_launch": k["bytes"] / max(1, k["launches"]),
                 "tflops": round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 1)}
+    batched = run_batched(P, args.batched) if args.batched > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.b)
     lead = algos[0]
     conv = {k: v for k, v in solves.items() if k.startswith(lead + "_") and v["converged"]}
@@ -254,6 +287,7 @@ def main():
         "roofline": roof,
         "roofline_fp64_gemm": roof64,
         "solve_kernels": {c: v for c, v in (sprof or {}).items() if c != "tier"},
+        "batched": batched,
         "cpu_baseline": cpu,
         "results": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in res.items()},

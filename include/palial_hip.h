@@ -188,6 +188,27 @@ int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_
 int palial_hip_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palial_opts *opts,
                      palial_report *rep);
 
+/* ---------------------------------------------------------------- batched */
+/* Many independent small factorizations in one launch per kind (no reference
+ * counterpart; for callers with batches of small systems).  Matrix i has
+ * order n[i] (1..512), is column-major with leading dimension n[i] and
+ * starts at element off[i] of A; kind[i] 0 = Cholesky (A = U^T U, U in the
+ * upper triangle, strict lower part untouched, as palial_hip_dpotrf), 1 = LU
+ * without pivoting (as palial_hip_dgetrf).  prec PALIAL_PREC_FP64 (A and B
+ * hold double) or PALIAL_PREC_FP32 (float; factored with FP64 arithmetic,
+ * rounded on output).  B (optional) holds one right-hand side per matrix at
+ * element boff[i]; it is overwritten by the solution.  info[i] (optional):
+ * 0, or the first non-positive pivot (Cholesky, 1-based).  n, off, kind,
+ * boff, info are host arrays; A and B follow opts->device_matrix.
+ * rep->factor_ms is the device time of the launches. */
+int palial_hip_batched(int count, const int *n, const long *off, const int *kind, void *A, void *B,
+                       const long *boff, int *info, int prec, const palial_opts *opts, palial_report *rep);
+
+/* Seeded inputs for a batch: kind 0 -> S + n I (S symmetric), kind 1 ->
+ * G + n I, entries U(seed + i, r, c) as palial_hip_generate. */
+int palial_hip_batched_generate(int count, const int *n, const long *off, const int *kind, void *A,
+                                uint64_t seed, int prec, const palial_opts *opts);
+
 /* ---------------------------------------------------------------- tile kernels */
 /* side / trans values of the reference (src/common.h:9-24; equal to the CBLAS
  * enum values it passes through) */

@@ -104,6 +104,9 @@ def lib():
         L.palial_hip_profile_stats.argtypes = [C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.palial_hip_update.argtypes = [C.c_int, MATRIX_desc, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.palial_hip_batched.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_int, C.POINTER(Opts),
+                                         C.POINTER(Report)]
+        L.palial_hip_batched_generate.argtypes = [C.c_int, dp, dp, dp, dp, C.c_uint64, C.c_int, C.POINTER(Opts)]
         _lib = L
     return _lib
 
@@ -360,6 +363,71 @@ def update(algo, dm, k, precision=PREC_FP64, reps=1):
     ms = C.c_double()
     _check(lib().palial_hip_update(0 if algo == "cholesky" else 1, dm.desc, k, precision, reps, C.byref(ms)))
     return ms.value
+
+
+class Batch:
+    """A batch of independent small matrices (orders n[i] <= 512; kind[i] 0 =
+    Cholesky, 1 = LU without pivoting) stored back to back on the device,
+    column-major, with one right-hand side each (palial_hip_batched)."""
+
+    def __init__(self, n, kind, precision=PREC_FP64):
+        init()
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.kind = np.ascontiguousarray(kind, dtype=np.int32)
+        self.count = len(self.n)
+        self.precision = precision
+        self.dtype = np.float32 if precision == PREC_FP32 else np.float64
+        sq = self.n.astype(np.int64) ** 2
+        self.off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sq)[:-1]]), dtype=np.int64)
+        self.boff = np.ascontiguousarray(np.concatenate([[0], np.cumsum(self.n.astype(np.int64))[:-1]]),
+                                         dtype=np.int64)
+        self.na, self.nb = int(sq.sum()), int(self.n.sum())
+        es = np.dtype(self.dtype).itemsize
+        pa, pb = C.c_void_p(), C.c_void_p()
+        _check(lib().palial_hip_malloc(C.byref(pa), self.na * es))
+        _check(lib().palial_hip_malloc(C.byref(pb), self.nb * es))
+        self.A, self.B = pa.value, pb.value
+        self.info = np.zeros(self.count, dtype=np.int32)
+
+    def generate(self, seed=0):
+        o = _opts(device=True)
+        _check(lib().palial_hip_batched_generate(self.count, self.n.ctypes.data, self.off.ctypes.data,
+                                                 self.kind.ctypes.data, self.A, seed, self.precision, C.byref(o)))
+
+    def set(self, mats=None, rhs=None):
+        """Upload matrices (list of n x n arrays) and / or right-hand sides (list of vectors)."""
+        if mats is not None:
+            flat = np.concatenate([np.asarray(m, dtype=self.dtype).ravel(order="F") for m in mats])
+            _check(lib().palial_hip_memcpy_h2d(self.A, flat.ctypes.data, flat.nbytes))
+        if rhs is not None:
+            flat = np.concatenate([np.asarray(v, dtype=self.dtype).ravel() for v in rhs])
+            _check(lib().palial_hip_memcpy_h2d(self.B, flat.ctypes.data, flat.nbytes))
+
+    def get(self):
+        """(list of n x n matrices, list of vectors) from the device."""
+        a = np.empty(self.na, dtype=self.dtype)
+        b = np.empty(self.nb, dtype=self.dtype)
+        _check(lib().palial_hip_memcpy_d2h(a.ctypes.data, self.A, a.nbytes))
+        _check(lib().palial_hip_memcpy_d2h(b.ctypes.data, self.B, b.nbytes))
+        mats = [a[o:o + m * m].reshape(m, m, order="F") for o, m in zip(self.off, self.n)]
+        vecs = [b[o:o + m] for o, m in zip(self.boff, self.n)]
+        return mats, vecs
+
+    def factor(self, solve=True):
+        """Factor every matrix in place (and solve its right-hand side); returns the report."""
+        o = _opts(device=True)
+        rep = Report()
+        _check(lib().palial_hip_batched(self.count, self.n.ctypes.data, self.off.ctypes.data,
+                                        self.kind.ctypes.data, self.A, self.B if solve else None,
+                                        self.boff.ctypes.data if solve else None, self.info.ctypes.data,
+                                        self.precision, C.byref(o), C.byref(rep)))
+        return rep
+
+    def free(self):
+        for p in (self.A, self.B):
+            if p:
+                lib().palial_hip_free(p)
+        self.A = self.B = None
 
 
 def factor_device(algo, dm, S=None, **kw):

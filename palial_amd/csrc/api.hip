@@ -17,6 +17,7 @@
 #include "internal.hpp"
 #include "../../include/palial_hip.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -61,6 +62,10 @@ struct Ctx {
     size_t F_cap = 0;
     double *d_inv = nullptr;
     size_t inv_cap = 0;
+    void *d_bidx = nullptr;         // batched factorizations: per-matrix index arrays
+    size_t bidx_cap = 0;            // bytes
+    double *d_bwork = nullptr;      // batched factorizations: padded FP64 workspace
+    size_t bwork_cap = 0;           // doubles
     double *d_F32 = nullptr;        // FP32 copies of the factor and the tile inverses
     size_t F32_cap = 0;             // (in doubles: the buffer is reused as floats)
     double *d_vec = nullptr;
@@ -1092,6 +1097,8 @@ int palial_hip_finalize(void)
     if (g.d_F) hipFree(g.d_F);
     if (g.d_inv) hipFree(g.d_inv);
     if (g.d_F32) hipFree(g.d_F32);
+    if (g.d_bidx) hipFree(g.d_bidx);
+    if (g.d_bwork) hipFree(g.d_bwork);
     if (g.d_vec) hipFree(g.d_vec);
     if (g.d_info) hipFree(g.d_info);
     if (g.s_bulk && g.s_bulk != g.s_main) hipStreamDestroy(g.s_bulk);
@@ -1180,6 +1187,170 @@ int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (ms) *ms = t / reps;
+    return 0;
+}
+
+// ------------------------------------------------------------------ batched
+namespace {
+
+int grow_bytes(void **p, size_t *cap, size_t need)
+{
+    if (*cap >= need) return 0;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, need) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "batch workspace of %zu bytes", need);
+    *cap = need;
+    return 0;
+}
+
+struct BatchPlan {
+    std::vector<int> orderC, orderL;
+    std::vector<long> woff;
+    size_t work = 0, a_ext = 0, b_ext = 0;
+    int bmaxC = 32, bmaxL = 32;
+};
+
+int batch_plan(int count, const int *n, const long *off, const int *kind, const long *boff, BatchPlan &P)
+{
+    if (count <= 0) return fail(PALIAL_ERR_ARG, "batch count %d", count);
+    if (!n || !off || !kind) return fail(PALIAL_ERR_ARG, "NULL batch description array");
+    P.woff.resize(count);
+    for (int i = 0; i < count; i++) {
+        if (n[i] < 1 || n[i] > 512) return fail(PALIAL_ERR_ARG, "matrix %d: order %d outside 1..512", i, n[i]);
+        if (kind[i] != 0 && kind[i] != 1) return fail(PALIAL_ERR_ARG, "matrix %d: kind %d", i, kind[i]);
+        if (off[i] < 0 || (boff && boff[i] < 0)) return fail(PALIAL_ERR_ARG, "matrix %d: negative offset", i);
+        const int b = (n[i] + 31) / 32 * 32;
+        P.woff[i] = (long)P.work;
+        P.work += (size_t)b * b;
+        P.a_ext = std::max(P.a_ext, (size_t)off[i] + (size_t)n[i] * n[i]);
+        if (boff) P.b_ext = std::max(P.b_ext, (size_t)boff[i] + (size_t)n[i]);
+        (kind[i] == 0 ? P.orderC : P.orderL).push_back(i);
+        int &bm = kind[i] == 0 ? P.bmaxC : P.bmaxL;
+        bm = std::max(bm, b);
+    }
+    auto larger = [&](int a, int b) { return n[a] > n[b]; };
+    std::stable_sort(P.orderC.begin(), P.orderC.end(), larger);
+    std::stable_sort(P.orderL.begin(), P.orderL.end(), larger);
+    return 0;
+}
+
+}  // namespace
+
+int palial_hip_batched(int count, const int *n, const long *off, const int *kind, void *A, void *B,
+                       const long *boff, int *info, int prec, const palial_opts *o_in, palial_report *rep)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    palial_opts o = o_in ? *o_in : default_opts();
+    if (prec != PALIAL_PREC_FP64 && prec != PALIAL_PREC_FP32) return fail(PALIAL_ERR_ARG, "batched precision %d", prec);
+    if (!A) return fail(PALIAL_ERR_ARG, "NULL matrix storage");
+    if (B && !boff) return fail(PALIAL_ERR_ARG, "right-hand sides need boff");
+    BatchPlan P;
+    if ((rc = batch_plan(count, n, off, kind, B ? boff : nullptr, P))) return rc;
+    const size_t es = prec == PALIAL_PREC_FP32 ? 4 : 8;
+    // device index arrays: n | off | woff | boff | orderC | orderL | info
+    const size_t ibytes = (size_t)count * (4 + 8 + 8 + 8 + 4 + 4 + 4) + 64;
+    if ((rc = grow_bytes(&g.d_bidx, &g.bidx_cap, ibytes))) return rc;
+    if ((rc = grow(&g.d_bwork, &g.bwork_cap, P.work))) return rc;
+    char *base = (char *)g.d_bidx;
+    long *d_off = (long *)base, *d_woff = d_off + count, *d_boff = d_woff + count;
+    int *d_n = (int *)(d_boff + count), *d_oc = d_n + count, *d_ol = d_oc + P.orderC.size(), *d_info = d_ol + P.orderL.size();
+    hipStream_t s = g.s_main;
+    std::vector<long> hb(count, 0);
+    if (B) for (int i = 0; i < count; i++) hb[i] = boff[i];
+    HIPCHK(hipMemcpyAsync(d_off, off, (size_t)count * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_woff, P.woff.data(), (size_t)count * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_boff, hb.data(), (size_t)count * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_n, n, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    if (!P.orderC.empty()) HIPCHK(hipMemcpyAsync(d_oc, P.orderC.data(), P.orderC.size() * 4, hipMemcpyHostToDevice, s));
+    if (!P.orderL.empty()) HIPCHK(hipMemcpyAsync(d_ol, P.orderL.data(), P.orderL.size() * 4, hipMemcpyHostToDevice, s));
+    void *dA = A, *dB = B;
+    bool own = false;
+    if (!o.device_matrix) {
+        own = true;
+        dA = dB = nullptr;
+        if (hipMalloc(&dA, P.a_ext * es) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "batch matrices");
+        if (B && hipMalloc(&dB, P.b_ext * es) != hipSuccess) { hipFree(dA); return fail(PALIAL_ERR_NOMEM, "batch vectors"); }
+        HIPCHK(hipMemcpyAsync(dA, A, P.a_ext * es, hipMemcpyHostToDevice, s));
+        if (B) HIPCHK(hipMemcpyAsync(dB, B, P.b_ext * es, hipMemcpyHostToDevice, s));
+    }
+    if ((rc = ensure_events(4))) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    BatchArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = d_n; a.off = d_off; a.woff = d_woff; a.boff = d_boff; a.A = dA; a.B = dB;
+    a.work = g.d_bwork; a.info = d_info; a.f32 = (prec == PALIAL_PREC_FP32);
+    HIPCHK(hipEventRecord(e0, s));
+    // the Cholesky and LU sub-batches run side by side on the two streams
+    hipEvent_t fork = g.events[0], join = g.events[1];
+    HIPCHK(hipEventRecord(fork, s));
+    HIPCHK(hipStreamWaitEvent(g.s_panel, fork, 0));
+    BatchArgs ac = a, al = a;
+    ac.count = (int)P.orderC.size(); ac.order = d_oc;
+    al.count = (int)P.orderL.size(); al.order = d_ol;
+    HIPCHK(launch_batch(ac, true, P.bmaxC, s));
+    HIPCHK(launch_batch(al, false, P.bmaxL, g.s_panel));
+    HIPCHK(hipEventRecord(join, g.s_panel));
+    HIPCHK(hipStreamWaitEvent(s, join, 0));
+    HIPCHK(hipEventRecord(e1, s));
+    if (info) HIPCHK(hipMemcpyAsync(info, d_info, (size_t)count * 4, hipMemcpyDeviceToHost, s));
+    if (own) {
+        HIPCHK(hipMemcpyAsync(A, dA, P.a_ext * es, hipMemcpyDeviceToHost, s));
+        if (B) HIPCHK(hipMemcpyAsync(B, dB, P.b_ext * es, hipMemcpyDeviceToHost, s));
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (own) {
+        hipFree(dA);
+        if (dB) hipFree(dB);
+    }
+    if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "batched factorization: %s", hipGetErrorString(e));
+    if (rep) {
+        double fl = 0;
+        for (int i = 0; i < count; i++) fl += (kind[i] == 0 ? 1.0 : 2.0) * n[i] * (double)n[i] * n[i] / 3.0;
+        rep->factor_ms = ms;
+        rep->gflop = fl / 1e9;
+        rep->tflops = ms > 0 ? fl / (ms * 1e-3) / 1e12 : 0.0;
+        rep->ntasks = count;
+        rep->nlaunches = 2;
+    }
+    return 0;
+}
+
+int palial_hip_batched_generate(int count, const int *n, const long *off, const int *kind, void *A, uint64_t seed,
+                                int prec, const palial_opts *o_in)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    palial_opts o = o_in ? *o_in : default_opts();
+    if (prec != PALIAL_PREC_FP64 && prec != PALIAL_PREC_FP32) return fail(PALIAL_ERR_ARG, "batched precision %d", prec);
+    if (!A) return fail(PALIAL_ERR_ARG, "NULL matrix storage");
+    BatchPlan P;
+    if ((rc = batch_plan(count, n, off, kind, nullptr, P))) return rc;
+    const size_t es = prec == PALIAL_PREC_FP32 ? 4 : 8;
+    const size_t ibytes = (size_t)count * (4 + 8 + 4) + 64;
+    if ((rc = grow_bytes(&g.d_bidx, &g.bidx_cap, std::max(ibytes, (size_t)count * 36 + 64)))) return rc;
+    long *d_off = (long *)g.d_bidx;
+    int *d_n = (int *)(d_off + count), *d_k = d_n + count;
+    hipStream_t s = g.s_main;
+    HIPCHK(hipMemcpyAsync(d_off, off, (size_t)count * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_n, n, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_k, kind, (size_t)count * 4, hipMemcpyHostToDevice, s));
+    void *dA = A;
+    if (!o.device_matrix && hipMalloc(&dA, P.a_ext * es) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "batch matrices");
+    HIPCHK(launch_batch_generate(d_n, d_off, d_k, dA, count, prec == PALIAL_PREC_FP32, seed, s));
+    if (!o.device_matrix) HIPCHK(hipMemcpyAsync(A, dA, P.a_ext * es, hipMemcpyDeviceToHost, s));
+    const hipError_t e = hipStreamSynchronize(s);
+    if (!o.device_matrix) hipFree(dA);
+    if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "batch generate: %s", hipGetErrorString(e));
     return 0;
 }
 

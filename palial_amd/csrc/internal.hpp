@@ -93,6 +93,30 @@ hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s);
 // of 32*32 doubles, column-major).
 bool panel2_ok(int b);
 void panel2_init();
+// Batched small factorizations (k_panel2.hip): one launch per kind; matrix
+// i (order n[i] <= 512, column-major, ld n[i]) at A + off[i] is copied into
+// a 32-padded FP64 workspace (identity padding), factored there by the tile
+// kernels' code (kind 0 Cholesky, 1 LU without pivoting), optionally solved
+// for one right-hand side at B + boff[i], and copied back (rounded to FP32
+// when f32).  All index arrays are device arrays.
+struct BatchArgs {
+    int count;
+    const int *order;    // matrices of this launch's kind, processing order (largest first)
+    const int *n;
+    const long *off;
+    const long *woff;    // workspace offsets (doubles)
+    const long *boff;
+    void *A;
+    void *B;             // nullptr: factor only
+    double *work;
+    int *info;
+    int f32;
+};
+hipError_t launch_batch(const BatchArgs &a, bool chol, int bmax, hipStream_t s);
+// seeded batch inputs: kind 0 -> S + n I (S symmetric), 1 -> G + n I; seed of matrix i: seed + i
+hipError_t launch_batch_generate(const int *n, const long *off, const int *kind, void *A, int count, int f32,
+                                 uint64_t seed, hipStream_t s);
+
 hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s);
 hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipStream_t s);
 hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s);

@@ -57,6 +57,33 @@ __global__ void generate_kernel(double *mat, int N, int b, int kind, uint64_t se
     }
 }
 
+// Batch inputs: matrix i (column-major, order n[i]) from seed + i; kind 0:
+// S + n I with S_rc = U(seed+i, min(r,c), max(r,c)); kind 1: G + n I with
+// G_rc = U(seed+i, r, c).  One workgroup per matrix.
+__global__ void batch_generate_kernel(const int *n, const long *off, const int *kind, void *A, int f32,
+                                      uint64_t seed)
+{
+    const int i = blockIdx.x;
+    const int m = n[i];
+    const uint64_t sd = seed + (uint64_t)i;
+    const size_t o = (size_t)off[i];
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int r = e % m, c = e / m;
+        double v = (kind[i] == 0) ? ((r <= c) ? uniform(sd, r, c) : uniform(sd, c, r)) : uniform(sd, r, c);
+        if (r == c) v += (double)m;
+        if (f32) ((float *)A)[o + e] = (float)v;
+        else ((double *)A)[o + e] = v;
+    }
+}
+
+hipError_t launch_batch_generate(const int *n, const long *off, const int *kind, void *A, int count, int f32,
+                                 uint64_t seed, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_generate_kernel, dim3((unsigned)count), dim3(256), 0, s, n, off, kind, A, f32, seed);
+    return hipGetLastError();
+}
+
 hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, double shift, hipStream_t s)
 {
     const size_t total = (size_t)N * N;

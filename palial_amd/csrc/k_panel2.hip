@@ -128,14 +128,18 @@ __device__ inline void stage_panel(const double *A, int b, int j0, int base, dou
 }
 
 // ---------------------------------------------------------------- POTRF
-__global__ __launch_bounds__(PF_THREADS) void potrf2_kernel(double *A, int b, int *info, int info_base,
-                                                            double *dinv)
+// LDS of the tile factorizations, carved from one dynamic buffer so that a
+// kernel running both (the batched one) stays within 160 KB:
+//   sP: NB x (b+16) panel rows, then 2 (POTRF) or 3 (GETRF) 32x33 blocks
+__host__ __device__ constexpr size_t panel_lds_bytes(int b, int nblk) { return ((size_t)NB2 * (b + 16) + (size_t)nblk * NB2 * LDD) * 8; }
+
+__device__ __forceinline__ void potrf2_tile(double *A, int b, int *info, int info_base, double *dinv, double *smem)
 {
     constexpr int NB = NB2;
     const int LDP = b + 16;               // panel rows 128 B apart modulo 256 B
-    extern __shared__ double sP[];        // NB x LDP: sP[q*LDP + c], c = absolute column
-    __shared__ double sU[NB * LDD];
-    __shared__ double sI[NB * LDD];
+    double *sP = smem;                    // NB x LDP: sP[q*LDP + c], c = absolute column
+    double *sU = smem + (size_t)NB * LDP;
+    double *sI = sU + NB * LDD;
     __shared__ int s_fail;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_fail = 0;
@@ -296,14 +300,14 @@ __device__ inline double piv_scale2(double x, double alpha)
     return x / alpha;
 }
 
-__global__ __launch_bounds__(PF_THREADS) void getrf2_kernel(double *A, int b, double *dinvU, double *dinvL)
+__device__ __forceinline__ void getrf2_tile(double *A, int b, double *dinvU, double *dinvL, double *smem)
 {
     constexpr int NB = NB2;
     const int LDP = b + 16;
-    extern __shared__ double sP[];        // U panel rows: sP[q*LDP + c]
-    __shared__ double sD[NB * LDD];       // factored diagonal block (L strict lower, U upper)
-    __shared__ double sUi[NB * LDD];
-    __shared__ double sLi[NB * LDD];
+    double *sP = smem;                    // U panel rows: sP[q*LDP + c]
+    double *sD = smem + (size_t)NB * LDP; // factored diagonal block (L strict lower, U upper)
+    double *sUi = sD + NB * LDD;
+    double *sLi = sUi + NB * LDD;
     double *sT = sP;                      // inverse scratch: the panel is not live yet
     __shared__ int s_slow;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -456,6 +460,136 @@ __global__ __launch_bounds__(PF_THREADS) void getrf2_kernel(double *A, int b, do
     }
 }
 
+__global__ __launch_bounds__(PF_THREADS) void potrf2_kernel(double *A, int b, int *info, int info_base,
+                                                            double *dinv)
+{
+    extern __shared__ double smem[];
+    potrf2_tile(A, b, info, info_base, dinv, smem);
+}
+
+__global__ __launch_bounds__(PF_THREADS) void getrf2_kernel(double *A, int b, double *dinvU, double *dinvL)
+{
+    extern __shared__ double smem[];
+    getrf2_tile(A, b, dinvU, dinvL, smem);
+}
+
+// ---------------------------------------------------------------- batched
+// 32x32 triangular solves by one wave, lane i (< 32) holding v_i and
+// reg[j] = M(i, j): forward for lower M, backward for upper M.
+__device__ inline double tri32_fwd(double v, const double (&reg)[NB2], bool unit, int lane)
+{
+#pragma unroll
+    for (int j = 0; j < NB2; j++) {
+        double yj = __shfl(v, j);
+        if (!unit) yj /= __shfl(reg[j], j);
+        if (lane == j) v = yj;
+        else if (lane > j) v -= reg[j] * yj;
+    }
+    return v;
+}
+
+__device__ inline double tri32_bwd(double v, const double (&reg)[NB2], int lane)
+{
+#pragma unroll
+    for (int j = NB2 - 1; j >= 0; j--) {
+        const double xj = __shfl(v, j) / __shfl(reg[j], j);
+        if (lane == j) v = xj;
+        else if (lane < j) v -= reg[j] * xj;
+    }
+    return v;
+}
+
+// One workgroup per matrix; workgroup g takes the g-th matrix of the order
+// list (largest first: the dispatcher hands out workgroups in index order,
+// which approximates longest-processing-time scheduling).  No counters, so
+// the launch can be captured and replayed.
+template <bool CHOL>
+__global__ __launch_bounds__(PF_THREADS) void batch_kernel(BatchArgs p)
+{
+    extern __shared__ double smem[];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    {
+        const int i = p.order[blockIdx.x];
+        const int n = p.n[i], b = (n + NB2 - 1) / NB2 * NB2;
+        double *W = p.work + p.woff[i];
+        const size_t off = (size_t)p.off[i];
+        for (int e = t; e < b * b; e += PF_THREADS) {
+            const int r = e % b, c = e / b;
+            double v = (r == c) ? 1.0 : 0.0;
+            if (r < n && c < n)
+                v = p.f32 ? (double)((const float *)p.A)[off + r + (size_t)c * n]
+                          : ((const double *)p.A)[off + r + (size_t)c * n];
+            W[e] = v;
+        }
+        if (t == 0) p.info[i] = 0;
+        __syncthreads();
+        constexpr bool chol = CHOL;   // the launch holds one kind only (order lists per kind)
+        if (chol) potrf2_tile(W, b, p.info + i, 0, nullptr, smem);
+        else getrf2_tile(W, b, nullptr, nullptr, smem);
+        __syncthreads();
+        if (p.B) {
+            // x in LDS (padding rows are 0); forward then backward blocked
+            // substitution with 32-row blocks
+            double *sx = smem;
+            const size_t bo = (size_t)p.boff[i];
+            for (int e = t; e < b; e += PF_THREADS)
+                sx[e] = (e < n) ? (p.f32 ? (double)((const float *)p.B)[bo + e] : ((const double *)p.B)[bo + e]) : 0.0;
+            __syncthreads();
+            // forward: Cholesky U^T y = r (lower operand U^T), LU L y = r (unit lower)
+            for (int j0 = 0; j0 < b; j0 += NB2) {
+                if (w == 0) {
+                    double reg[NB2];
+                    const int li = lane & (NB2 - 1);
+#pragma unroll
+                    for (int j = 0; j < NB2; j++)
+                        reg[j] = chol ? W[(size_t)(j0 + j) + (size_t)(j0 + li) * b]    // U^T(i,j) = U(j,i)
+                                      : W[(size_t)(j0 + li) + (size_t)(j0 + j) * b];   // L(i,j)
+                    const double v = tri32_fwd(lane < NB2 ? sx[j0 + lane] : 0.0, reg, !chol, lane);
+                    if (lane < NB2) sx[j0 + lane] = v;
+                }
+                __syncthreads();
+                for (int r = j0 + NB2 + t; r < b; r += PF_THREADS) {
+                    double acc = 0.0;
+#pragma unroll 8
+                    for (int q = 0; q < NB2; q++)
+                        acc += (chol ? W[(size_t)(j0 + q) + (size_t)r * b] : W[(size_t)r + (size_t)(j0 + q) * b]) * sx[j0 + q];
+                    sx[r] -= acc;
+                }
+                __syncthreads();
+            }
+            // backward: U x = y
+            for (int j0 = b - NB2; j0 >= 0; j0 -= NB2) {
+                if (w == 0) {
+                    double reg[NB2];
+                    const int li = lane & (NB2 - 1);
+#pragma unroll
+                    for (int j = 0; j < NB2; j++) reg[j] = W[(size_t)(j0 + li) + (size_t)(j0 + j) * b];   // U(i,j)
+                    const double v = tri32_bwd(lane < NB2 ? sx[j0 + lane] : 0.0, reg, lane);
+                    if (lane < NB2) sx[j0 + lane] = v;
+                }
+                __syncthreads();
+                for (int r = t; r < j0; r += PF_THREADS) {
+                    double acc = 0.0;
+#pragma unroll 8
+                    for (int q = 0; q < NB2; q++) acc += W[(size_t)r + (size_t)(j0 + q) * b] * sx[j0 + q];
+                    sx[r] -= acc;
+                }
+                __syncthreads();
+            }
+            for (int e = t; e < n; e += PF_THREADS) {
+                if (p.f32) ((float *)p.B)[bo + e] = (float)sx[e];
+                else ((double *)p.B)[bo + e] = sx[e];
+            }
+        }
+        for (int e = t; e < n * n; e += PF_THREADS) {
+            const int r = e % n, c = e / n;
+            const double v = W[(size_t)r + (size_t)c * b];
+            if (p.f32) ((float *)p.A)[off + e] = (float)v;
+            else ((double *)p.A)[off + e] = v;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- TRSM
 // Strip layout in LDS: row i, column c at i*32 + (c ^ 16*(i&1)) (odd rows
 // swap their 16-column halves so four consecutive rows read by one MFMA
@@ -580,11 +714,16 @@ bool panel2_ok(int b) { return b % NB2 == 0 && b >= NB2 && b <= 512; }
 
 void panel2_init()
 {
-    // dynamic LDS at the largest supported tile (b = 512); static LDS comes on top
-    const int lds_panel = NB2 * (512 + 16) * 8;   // 135168 B
+    // dynamic LDS at the largest supported tile (b = 512)
     const int lds_trsm = 512 * NB2 * 8;           // 131072 B
-    hipFuncSetAttribute((const void *)potrf2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_panel);
-    hipFuncSetAttribute((const void *)getrf2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_panel);
+    hipFuncSetAttribute((const void *)potrf2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)panel_lds_bytes(512, 2));
+    hipFuncSetAttribute((const void *)getrf2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)panel_lds_bytes(512, 3));
+    hipFuncSetAttribute((const void *)batch_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)panel_lds_bytes(512, 3));
+    hipFuncSetAttribute((const void *)batch_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)panel_lds_bytes(512, 3));
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_CHOL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
@@ -592,15 +731,25 @@ void panel2_init()
 
 hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s)
 {
-    const size_t lds = (size_t)NB2 * (b + 16) * 8;
+    const size_t lds = panel_lds_bytes(b, 2);
     hipLaunchKernelGGL(potrf2_kernel, dim3(1), dim3(PF_THREADS), lds, s, tile, b, info, info_base, dinv);
     return hipGetLastError();
 }
 
 hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipStream_t s)
 {
-    const size_t lds = (size_t)NB2 * (b + 16) * 8;
+    const size_t lds = panel_lds_bytes(b, 3);
     hipLaunchKernelGGL(getrf2_kernel, dim3(1), dim3(PF_THREADS), lds, s, tile, b, dinvU, dinvL);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchArgs &a, bool chol, int bmax, hipStream_t s)
+{
+    if (a.count <= 0) return hipSuccess;
+    if (bmax <= 0 || bmax > 512 || bmax % NB2) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)a.count), block(PF_THREADS);
+    if (chol) hipLaunchKernelGGL(batch_kernel<true>, grid, block, panel_lds_bytes(bmax, 3), s, a);
+    else hipLaunchKernelGGL(batch_kernel<false>, grid, block, panel_lds_bytes(bmax, 3), s, a);
     return hipGetLastError();
 }
 
