@@ -104,10 +104,16 @@ int palial_hip_copy(void *dst, const void *src, size_t bytes);  /* device copy b
 /* Measurement hooks (no counterpart in the reference): per-kernel-class
  * HIP-event timing of the following algorithm calls, which then launch
  * stream-ordered instead of replaying a graph.  Classes:
- *   0 trailing-update GEMM/SYRK, 1 diagonal-tile POTRF/GETRF/GEQRT,
- *   2 panel TRSM, 3 TSQRT, 4 QR block-reflector GEMMs (ORMQR/TSMQR).
+ *   0 trailing-update GEMM/SYRK (FP64, or low-precision with FP64 operands),
+ *   1 diagonal-tile POTRF/GETRF/GEQRT, 2 panel TRSM, 3 TSQRT,
+ *   4 QR block-reflector GEMMs (ORMQR/TSMQR), 5 panel packing (mixed tiers),
+ *   6 trailing update from packed low-precision panels.
  * flops / bytes are the algorithmic counts of the timed launches. */
-int palial_hip_profile(int enable);   /* also resets the counters */
+int palial_hip_profile(int enable);   /* 1: per-class totals, 2: also one record per launch; resets */
+/* Per-launch records kept under palial_hip_profile(2): 6 doubles each --
+ * kernel class, tile size b, precision tier, algorithmic flops, algorithmic
+ * bytes, device ms.  Copies at most cap records; returns how many exist. */
+long palial_hip_profile_records(double *out, long cap);
 int palial_hip_profile_stats(int kclass, long *launches, double *total_ms, double *flops,
                              double *bytes);
 
@@ -208,6 +214,23 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
  * G + n I, entries U(seed + i, r, c) as palial_hip_generate. */
 int palial_hip_batched_generate(int count, const int *n, const long *off, const int *kind, void *A,
                                 uint64_t seed, int prec, const palial_opts *opts);
+
+/* ---------------------------------------------------------------- cost model */
+/* Task-time cost model (no reference counterpart; for the autotuner,
+ * palial_amd/autotune.py): a 3-layer perceptron with 64 hidden units in FP32,
+ * y = relu(relu(x W1 + b1) W2 + b2) . w3 + b3, nfeat <= 16 input features.
+ * params: W1 [nfeat x 64] row by row, b1 [64], W2 [64 x 64] row by row, b2 [64],
+ * w3 [64], b3 [1]; palial_hip_mlp_nparams gives the count.  X is n x nfeat
+ * row-major.  Pointers follow opts->device_matrix. */
+int palial_hip_mlp_nparams(int nfeat);
+int palial_hip_mlp_forward(int n, int nfeat, const float *X, const float *params, float *y,
+                           const palial_opts *opts);
+/* Adam on the mean squared error, mini-batches of 128 samples in order, fresh
+ * optimizer state per call; loss[epoch] = mean squared error of the epoch
+ * (before its steps complete).  epochs == 0: grad receives the full-batch
+ * gradient of the mean squared error at params (params unchanged). */
+int palial_hip_mlp_train(int n, int nfeat, const float *X, const float *y, float *params, int epochs,
+                         float lr, float *loss, float *grad, const palial_opts *opts);
 
 /* ---------------------------------------------------------------- tile kernels */
 /* side / trans values of the reference (src/common.h:9-24; equal to the CBLAS

@@ -107,6 +107,11 @@ def lib():
         L.palial_hip_batched.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_int, C.POINTER(Opts),
                                          C.POINTER(Report)]
         L.palial_hip_batched_generate.argtypes = [C.c_int, dp, dp, dp, dp, C.c_uint64, C.c_int, C.POINTER(Opts)]
+        L.palial_hip_mlp_forward.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(Opts)]
+        L.palial_hip_mlp_train.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_float, dp, dp,
+                                           C.POINTER(Opts)]
+        L.palial_hip_profile_records.restype = C.c_long
+        L.palial_hip_profile_records.argtypes = [dp, C.c_long]
         _lib = L
     return _lib
 
@@ -343,8 +348,9 @@ def synchronize():
 
 
 def profile(enable):
+    """0 off, 1 per-class totals, 2 also per-launch records (profile_records())."""
     init()
-    _check(lib().palial_hip_profile(int(bool(enable))))
+    _check(lib().palial_hip_profile(int(enable)))
 
 
 KCLASS = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4, "pack": 5, "pkgemm": 6}
@@ -428,6 +434,16 @@ class Batch:
             if p:
                 lib().palial_hip_free(p)
         self.A = self.B = None
+
+
+def profile_records():
+    """Per-launch records kept under profile(2): (n, 6) array of (class, b,
+    precision, flops, bytes, ms)."""
+    n = lib().palial_hip_profile_records(None, 0)
+    out = np.zeros((max(n, 0), 6))
+    if n > 0:
+        lib().palial_hip_profile_records(out.ctypes.data, n)
+    return out
 
 
 def factor_device(algo, dm, S=None, **kw):

@@ -18,6 +18,7 @@
 #include "../../include/palial_hip.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -199,10 +200,12 @@ int check_desc(const MATRIX_desc &A)
 // are the algorithmic HBM traffic (FP64 C read + write, the packed panel once)
 enum KClass : int { KC_GEMM = 0, KC_DIAG = 1, KC_TRSM = 2, KC_TSQRT = 3, KC_QRUPD = 4, KC_PACK = 5, KC_PKGEMM = 6,
                     KC_N = 7 };
-struct ProfRec { int cls; double flops; double bytes; hipEvent_t a, b; };
+struct ProfRec { int cls; double flops; double bytes; int b, prec; hipEvent_t a, b_ev; };
 struct Prof {
     bool on = false;
+    bool keep = false;                    // also keep one record per launch
     std::vector<ProfRec> pending;
+    std::vector<std::array<double, 6>> records;   // class, b, precision, flops, bytes, ms
     long launches[KC_N] = {0};
     double ms[KC_N] = {0}, flops[KC_N] = {0}, bytes[KC_N] = {0};
 };
@@ -213,6 +216,7 @@ struct Launcher {
     long launches = 0;
     hipError_t err = hipSuccess;
     bool capturing = false;
+    int b = 0, prec = 0;   // context of the records: tile size and precision tier
     void chk(hipError_t e)
     {
         launches++;
@@ -223,12 +227,12 @@ struct Launcher {
     void run(int cls, double flops, double bytes, hipStream_t s, F &&f)
     {
         if (g_prof.on && !capturing) {
-            ProfRec r{cls, flops, bytes, nullptr, nullptr};
+            ProfRec r{cls, flops, bytes, b, prec, nullptr, nullptr};
             hipEventCreate(&r.a);
-            hipEventCreate(&r.b);
+            hipEventCreate(&r.b_ev);
             hipEventRecord(r.a, s);
             chk(f());
-            hipEventRecord(r.b, s);
+            hipEventRecord(r.b_ev, s);
             g_prof.pending.push_back(r);
         } else {
             chk(f());
@@ -240,13 +244,14 @@ void prof_collect()
 {
     for (auto &r : g_prof.pending) {
         float ms = 0;
-        hipEventElapsedTime(&ms, r.a, r.b);
+        hipEventElapsedTime(&ms, r.a, r.b_ev);
         g_prof.launches[r.cls]++;
         g_prof.ms[r.cls] += ms;
         g_prof.flops[r.cls] += r.flops;
         g_prof.bytes[r.cls] += r.bytes;
+        if (g_prof.keep) g_prof.records.push_back({(double)r.cls, (double)r.b, (double)r.prec, r.flops, r.bytes, (double)ms});
         hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        hipEventDestroy(r.b_ev);
     }
     g_prof.pending.clear();
 }
@@ -316,6 +321,8 @@ GemmArgs single_tile(const double *A, const double *B, double *C, int b, bool sy
 void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, int *dinfo,
                         Launcher &L, int prec)
 {
+    L.b = b;
+    L.prec = prec;
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
@@ -389,6 +396,8 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
 // solves the rest of row and column k and updates every other trailing tile.
 void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, Launcher &L, int prec)
 {
+    L.b = b;
+    L.prec = prec;
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
@@ -469,6 +478,8 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
 void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, int *dinfo,
                   Launcher &L, int prec = 0)
 {
+    L.b = b;
+    L.prec = prec;
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
@@ -534,6 +545,8 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
 // LU without pivoting, src/lu.h:14-173 (with correct tile dependences).
 void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, Launcher &L, int prec = 0)
 {
+    L.b = b;
+    L.prec = prec;
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
@@ -593,6 +606,8 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
 // Flat-tree tiled QR, src/qr.h:18-190.
 void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, Launcher &L)
 {
+    L.b = b;
+    L.prec = 0;
     const int MT = N / b;
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
@@ -1072,6 +1087,7 @@ int palial_hip_init(int device)
     HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
     panel_init();
     panel2_init();
+    mlp_init();
     {
         hipError_t e = hipGetLastError();   // attribute calls must not leave a sticky error
         if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "kernel attribute setup: %s", hipGetErrorString(e));
@@ -1112,8 +1128,20 @@ int palial_hip_profile(int enable)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     g_prof.on = enable != 0;
+    g_prof.keep = enable == 2;
+    g_prof.records.clear();
     for (int c = 0; c < KC_N; c++) { g_prof.launches[c] = 0; g_prof.ms[c] = g_prof.flops[c] = g_prof.bytes[c] = 0; }
     return 0;
+}
+
+long palial_hip_profile_records(double *out, long cap)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    const long n = (long)g_prof.records.size();
+    if (out)
+        for (long i = 0; i < n && i < cap; i++)
+            for (int f = 0; f < 6; f++) out[i * 6 + f] = g_prof.records[i][f];
+    return n;
 }
 
 int palial_hip_profile_stats(int cls, long *launches, double *total_ms, double *flops, double *bytes)
@@ -1351,6 +1379,71 @@ int palial_hip_batched_generate(int count, const int *n, const long *off, const 
     const hipError_t e = hipStreamSynchronize(s);
     if (!o.device_matrix) hipFree(dA);
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "batch generate: %s", hipGetErrorString(e));
+    return 0;
+}
+
+// ------------------------------------------------------------------ cost model
+int palial_hip_mlp_nparams(int nfeat) { return (nfeat >= 1 && nfeat <= 16) ? mlp_nparams(nfeat) : -1; }
+
+int palial_hip_mlp_forward(int n, int nfeat, const float *X, const float *params, float *y, const palial_opts *o_in)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if (n < 0 || nfeat < 1 || nfeat > 16 || !X || !params || !y) return fail(PALIAL_ERR_ARG, "bad cost-model arguments");
+    palial_opts o = o_in ? *o_in : default_opts();
+    const int np = mlp_nparams(nfeat);
+    const float *dX = X, *dP = params;
+    float *dY = y, *buf = nullptr;
+    hipStream_t s = g.s_main;
+    if (!o.device_matrix) {
+        if (hipMalloc(&buf, ((size_t)n * nfeat + np + n) * 4 + 64) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "cost-model buffers");
+        float *bx = buf, *bp = bx + (size_t)n * nfeat, *by = bp + np;
+        HIPCHK(hipMemcpyAsync(bx, X, (size_t)n * nfeat * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(bp, params, (size_t)np * 4, hipMemcpyHostToDevice, s));
+        dX = bx; dP = bp; dY = by;
+    }
+    HIPCHK(launch_mlp_forward(n, nfeat, dX, dP, dY, s));
+    if (!o.device_matrix) HIPCHK(hipMemcpyAsync(y, dY, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    const hipError_t e = hipStreamSynchronize(s);
+    if (buf) hipFree(buf);
+    if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "cost-model forward: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int palial_hip_mlp_train(int n, int nfeat, const float *X, const float *y, float *params, int epochs, float lr,
+                         float *loss, float *grad, const palial_opts *o_in)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if (n < 1 || nfeat < 1 || nfeat > 16 || epochs < 0 || !X || !y || !params || (epochs == 0 && !grad))
+        return fail(PALIAL_ERR_ARG, "bad cost-model arguments");
+    palial_opts o = o_in ? *o_in : default_opts();
+    const int np = mlp_nparams(nfeat);
+    const int nl = epochs > 0 ? epochs : 1;
+    // device layout: X | y | params | adam m | adam v | loss | grad
+    float *buf = nullptr;
+    const size_t tot = (size_t)n * nfeat + n + 4 * (size_t)np + nl + 64;
+    if (hipMalloc(&buf, tot * 4) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "cost-model buffers");
+    float *bx = buf, *by = bx + (size_t)n * nfeat, *bp = by + n, *bm = bp + np, *bv = bm + np, *bl = bv + np, *bg = bl + nl;
+    hipStream_t s = g.s_main;
+    const hipMemcpyKind in = o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind out = o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    auto done = [&](int code) { hipStreamSynchronize(s); hipFree(buf); return code; };
+    if (hipMemcpyAsync(bx, X, (size_t)n * nfeat * 4, in, s) != hipSuccess ||
+        hipMemcpyAsync(by, y, (size_t)n * 4, in, s) != hipSuccess ||
+        hipMemcpyAsync(bp, params, (size_t)np * 4, in, s) != hipSuccess ||
+        hipMemsetAsync(bm, 0, 2 * (size_t)np * 4, s) != hipSuccess)
+        return done(fail(PALIAL_ERR_HIP, "cost-model copies"));
+    if (launch_mlp_train(n, nfeat, bx, by, bp, bm, bv, epochs, lr, bl, bg, s) != hipSuccess)
+        return done(fail(PALIAL_ERR_HIP, "cost-model training launch"));
+    if (epochs > 0) hipMemcpyAsync(params, bp, (size_t)np * 4, out, s);
+    if (grad) hipMemcpyAsync(grad, bg, (size_t)np * 4, out, s);
+    if (loss) hipMemcpyAsync(loss, bl, (size_t)nl * 4, out, s);
+    const hipError_t e = hipStreamSynchronize(s);
+    hipFree(buf);
+    if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "cost-model training: %s", hipGetErrorString(e));
     return 0;
 }
 

@@ -117,6 +117,13 @@ hipError_t launch_batch(const BatchArgs &a, bool chol, int bmax, hipStream_t s);
 hipError_t launch_batch_generate(const int *n, const long *off, const int *kind, void *A, int count, int f32,
                                  uint64_t seed, hipStream_t s);
 
+// MLP cost model (k_mlp.hip)
+int mlp_nparams(int F);
+void mlp_init();
+hipError_t launch_mlp_forward(int n, int F, const float *X, const float *P, float *Y, hipStream_t s);
+hipError_t launch_mlp_train(int n, int F, const float *X, const float *Y, float *P, float *m, float *v, int epochs,
+                            float lr, float *loss, float *grad, hipStream_t s);
+
 hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s);
 hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipStream_t s);
 hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s);
