@@ -62,7 +62,7 @@ typedef struct palial_opts {
     int   no_graph;       /* nonzero: launch stream-ordered instead of replaying a hipGraph */
     int   no_lookahead;   /* nonzero: single stream, no panel look-ahead */
     void *stream;         /* hipStream_t to run on (device_matrix only); NULL = library stream */
-    int   refine;         /* solves: 0 = LU-IR, 1 = GMRES-IR, 2 = auto */
+    int   refine;         /* solves: 0 = auto (LU-IR, GMRES-IR once LU-IR stalls), 1 = GMRES-IR, 2 = LU-IR only */
     int   max_iter;       /* solves: refinement steps (0 = 30) */
     double tol;           /* solves: target backward error (0 = 1e-14) */
     int   factor_tile;    /* solves: tile size of the internal factorization (0 = automatic) */
@@ -82,7 +82,8 @@ typedef struct palial_report {
     int    iterations;    /* solves: refinement steps taken */
     long   ntasks;        /* tile tasks of the reference DAG this call covers */
     long   nlaunches;     /* kernel launches issued (or graph nodes replayed) */
-    int    reserved[8];
+    int    gmres_iterations; /* solves: GMRES inner iterations over all GMRES-IR steps */
+    int    reserved[7];
 } palial_report;
 
 /* ---------------------------------------------------------------- lifecycle */

@@ -49,7 +49,8 @@ class Report(C.Structure):
     _fields_ = [("factor_ms", C.c_double), ("solve_ms", C.c_double), ("h2d_ms", C.c_double),
                 ("d2h_ms", C.c_double), ("gflop", C.c_double), ("tflops", C.c_double),
                 ("backward_error", C.c_double), ("info", C.c_int), ("iterations", C.c_int),
-                ("ntasks", C.c_long), ("nlaunches", C.c_long), ("reserved", C.c_int * 8)]
+                ("ntasks", C.c_long), ("nlaunches", C.c_long), ("gmres_iterations", C.c_int),
+                ("reserved", C.c_int * 7)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -67,6 +67,8 @@ struct Ctx {
     size_t bidx_cap = 0;            // bytes
     double *d_bwork = nullptr;      // batched factorizations: padded FP64 workspace
     size_t bwork_cap = 0;           // doubles
+    double *d_gm = nullptr;         // GMRES-IR: Krylov basis and vectors
+    size_t gm_cap = 0;
     double *d_F32 = nullptr;        // FP32 copies of the factor and the tile inverses
     size_t F32_cap = 0;             // (in doubles: the buffer is reused as floats)
     double *d_vec = nullptr;
@@ -790,8 +792,97 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     const bool sym = (algo == 0 && nrhs == 1);
     double hred[64];
     double normA = 0.0, normB[16];
-    double eta = 0.0;
-    int it = 0;
+    double eta = 0.0, eta_prev = 0.0;
+    int it = 0, stall = 0, gmres_total = 0;
+    int mode = (o.refine == 1) ? 1 : 0;      // 0 LU-IR, 1 GMRES-IR
+    // GMRES workspace: Krylov basis V (N x (GM+1)), two vectors, a zero
+    // right-hand side, the Hessenberg column on the device
+    constexpr int GM = 30;
+    const double gm_tol = 1e-8;              // inner relative residual per refinement step
+    double *V = nullptr, *Wv = nullptr, *T1 = nullptr, *Z0 = nullptr, *hd = nullptr;
+    if (o.refine != 2) {
+        if ((rc = grow(&g.d_gm, &g.gm_cap, (size_t)N * (GM + 4) + 2 * (GM + 2)))) return done(rc);
+        V = g.d_gm; Wv = V + (size_t)N * (GM + 1); T1 = Wv + N; Z0 = T1 + N; hd = Z0 + N;
+        HIPCHK(hipMemsetAsync(Z0, 0, (size_t)N * 8, s));
+    }
+    auto dots = [&](int k, const double *w, double *out) -> int {
+        L.chk(launch_multi_dot(V, (size_t)N, k, w, N, hd, s));
+        if (hipMemcpyAsync(out, hd, (size_t)k * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail(PALIAL_ERR_HIP, "GMRES reduction");
+        return 0;
+    };
+    auto norm2 = [&](const double *w, double &out) -> int {
+        L.chk(launch_multi_dot(w, (size_t)N, 1, w, N, hd, s));
+        if (hipMemcpyAsync(&out, hd, 8, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            return fail(PALIAL_ERR_HIP, "GMRES norm");
+        out = sqrt(out);
+        return 0;
+    };
+    // d = solution of A d = r (one column) by GMRES on F^-1 A d = F^-1 r, d0 = 0
+    auto gmres = [&](const double *rj, double *dj, int &inner) -> int {
+        int err;
+        std::vector<double> H((size_t)(GM + 1) * GM, 0.0), cs(GM, 0.0), sn(GM, 0.0), gv(GM + 1, 0.0), h(GM + 1);
+        L.chk(launch_copy(T1, rj, (size_t)N * 8, s));
+        sweeps(algo, sF, sU, sL, T1, Y, V, N, b, 1, f32, s, L);          // v0 = F^-1 r
+        double beta;
+        if ((err = norm2(V, beta))) return err;
+        inner = 0;
+        if (!(beta > 0.0)) {
+            L.chk(launch_vh_update(V, (size_t)N, 0, hd, dj, 0.0, N, s));     // d = 0
+            return 0;
+        }
+        L.chk(launch_vh_update(V, (size_t)N, 0, hd, V, 1.0 / beta, N, s));
+        gv[0] = beta;
+        int k = 0;
+        for (; k < GM; k++) {
+            double *vk = V + (size_t)k * N, *vn = V + (size_t)(k + 1) * N;
+            // w = F^-1 A v_k  (the residual kernel with a zero right-hand side gives -A v_k)
+            L.chk(launch_residual(dA, N, bA, vk, Z0, T1, N, 1, part, sym, nullptr, s));
+            sweeps(algo, sF, sU, sL, T1, Y, Wv, N, b, 1, f32, s, L);
+            L.chk(launch_vh_update(V, (size_t)N, 0, hd, Wv, -1.0, N, s));
+            // classical Gram-Schmidt, twice
+            std::vector<double> h2(k + 1);
+            if ((err = dots(k + 1, Wv, h.data()))) return err;
+            L.chk(launch_vh_update(V, (size_t)N, k + 1, hd, Wv, 1.0, N, s));
+            if ((err = dots(k + 1, Wv, h2.data()))) return err;
+            L.chk(launch_vh_update(V, (size_t)N, k + 1, hd, Wv, 1.0, N, s));
+            for (int i = 0; i <= k; i++) h[i] += h2[i];
+            double hn;
+            if ((err = norm2(Wv, hn))) return err;
+            h[k + 1] = hn;
+            if (hn > 0.0) L.chk(launch_vh_update(V, (size_t)N, 0, hd, Wv, 1.0 / hn, N, s));
+            L.chk(launch_copy(vn, Wv, (size_t)N * 8, s));
+            // Givens rotations on the new Hessenberg column
+            for (int i = 0; i < k; i++) {
+                const double a = cs[i] * h[i] + sn[i] * h[i + 1];
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                h[i] = a;
+            }
+            const double r = hypot(h[k], h[k + 1]);
+            cs[k] = r > 0 ? h[k] / r : 1.0;
+            sn[k] = r > 0 ? h[k + 1] / r : 0.0;
+            h[k] = r;
+            h[k + 1] = 0.0;
+            gv[k + 1] = -sn[k] * gv[k];
+            gv[k] = cs[k] * gv[k];
+            for (int i = 0; i <= k; i++) H[(size_t)i + (size_t)k * (GM + 1)] = h[i];
+            inner++;
+            if (fabs(gv[k + 1]) <= gm_tol * beta || !(hn > 0.0)) { k++; break; }
+        }
+        // y = R^-1 g, d = V y
+        std::vector<double> y(k, 0.0);
+        for (int i = k - 1; i >= 0; i--) {
+            double a = gv[i];
+            for (int c = i + 1; c < k; c++) a -= H[(size_t)i + (size_t)c * (GM + 1)] * y[c];
+            y[i] = a / H[(size_t)i + (size_t)i * (GM + 1)];
+        }
+        for (int i = 0; i < k; i++) y[i] = -y[i];
+        if (hipMemcpyAsync(hd, y.data(), (size_t)k * 8, hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail(PALIAL_ERR_HIP, "GMRES coefficients");
+        L.chk(launch_vh_update(V, (size_t)N, k, hd, dj, 0.0, N, s));     // d = 0*d - V(-y)
+        return 0;
+    };
     for (;; it++) {
         // r = b - A x ; eta = max_j ||r_j|| / (||A|| ||x_j|| + ||b_j||)
         L.chk(launch_residual(dA, N, bA, X, Bd, R, N, nrhs, part, sym, it == 0 ? rowsum : nullptr, s));
@@ -813,9 +904,25 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
             if (!(ej <= eta)) eta = ej;      // NaN propagates
         }
         if (eta <= tol || it >= max_iter) break;
-        // d = F^{-1} r ; x += d
-        L.chk(launch_copy(R2, R, nv * 8, s));
-        sweeps(algo, sF, sU, sL, R2, Y, D, N, b, nrhs, f32, s, L);
+        // auto: LU-IR until it fails twice in a row to halve eta, then GMRES-IR
+        if (o.refine == 0 && mode == 0 && it > 0 && !(eta <= 0.5 * eta_prev)) {
+            if (++stall >= 2) mode = 1;
+        } else {
+            stall = 0;
+        }
+        eta_prev = eta;
+        if (mode == 1) {
+            // GMRES-IR: A d = r by GMRES preconditioned on the left with F^-1
+            for (int j = 0; j < nrhs; j++) {
+                int inner = 0;
+                if ((rc = gmres(R + (size_t)j * N, D + (size_t)j * N, inner))) return done(rc);
+                gmres_total += inner;
+            }
+        } else {
+            // LU-IR: d = F^{-1} r
+            L.chk(launch_copy(R2, R, nv * 8, s));
+            sweeps(algo, sF, sU, sL, R2, Y, D, N, b, nrhs, f32, s, L);
+        }
         L.chk(launch_axpy(X, D, N, N, nrhs, s));
     }
     HIPCHK(hipEventRecord(e2, s));
@@ -834,6 +941,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         rep->tflops = (algo == 0 ? n * n * n / 3 : 2 * n * n * n / 3) / ((fms + sms) * 1e-3) / 1e12;
         rep->backward_error = eta;
         rep->iterations = it;
+        rep->gmres_iterations = gmres_total;
         rep->info = info;
         rep->nlaunches = L.launches;
     }
@@ -1113,6 +1221,7 @@ int palial_hip_finalize(void)
     if (g.d_F) hipFree(g.d_F);
     if (g.d_inv) hipFree(g.d_inv);
     if (g.d_F32) hipFree(g.d_F32);
+    if (g.d_gm) hipFree(g.d_gm);
     if (g.d_bidx) hipFree(g.d_bidx);
     if (g.d_bwork) hipFree(g.d_bwork);
     if (g.d_vec) hipFree(g.d_vec);

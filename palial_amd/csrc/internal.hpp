@@ -157,6 +157,10 @@ hipError_t launch_residual(const double *A, int N, int b, const double *x, const
 size_t residual_scratch(int N, int b, int nrhs);   // doubles of `part` launch_residual needs
 hipError_t launch_colmax(const double *v, int N, int ld, int nrhs, double *out, hipStream_t s);
 hipError_t launch_axpy(double *x, const double *d, int N, int ld, int nrhs, hipStream_t s);
+// GMRES: out[i] = V(:,i).w (i < k); w = alpha*w - V(:,0:k) h
+hipError_t launch_multi_dot(const double *V, size_t ldv, int k, const double *w, int N, double *out, hipStream_t s);
+hipError_t launch_vh_update(const double *V, size_t ldv, int k, const double *h, double *w, double alpha, int N,
+                            hipStream_t s);
 
 // QR panels
 hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s);

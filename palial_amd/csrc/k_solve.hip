@@ -387,6 +387,51 @@ hipError_t launch_colmax(const double *v, int N, int ld, int nrhs, double *out, 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- GMRES helpers
+// out[i] = V(:, i) . w for i < k (one workgroup per column, fixed-order
+// reduction); V has leading dimension ldv
+__global__ __launch_bounds__(256) void multi_dot_kernel(const double *V, size_t ldv, const double *w, int N,
+                                                        double *out)
+{
+    __shared__ double sred[256];
+    const int i = blockIdx.x, t = threadIdx.x;
+    const double *v = V + (size_t)i * ldv;
+    double a = 0.0;
+    for (int r = t; r < N; r += 256) a += v[r] * w[r];
+    sred[t] = a;
+    __syncthreads();
+    for (int s = 128; s >= 1; s >>= 1) {
+        if (t < s) sred[t] += sred[t + s];
+        __syncthreads();
+    }
+    if (t == 0) out[i] = sred[0];
+}
+
+// w = alpha * w - V(:, 0:k) h  (h on the device; alpha = 0: w is not read)
+__global__ void vh_update_kernel(const double *V, size_t ldv, int k, const double *h, double *w, double alpha,
+                                 int N)
+{
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+        double a = (alpha == 0.0) ? 0.0 : alpha * w[r];   // alpha = 0 ignores w entirely
+        for (int i = 0; i < k; i++) a -= V[(size_t)i * ldv + r] * h[i];
+        w[r] = a;
+    }
+}
+
+hipError_t launch_multi_dot(const double *V, size_t ldv, int k, const double *w, int N, double *out, hipStream_t s)
+{
+    if (k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(multi_dot_kernel, dim3((unsigned)k), dim3(256), 0, s, V, ldv, w, N, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_vh_update(const double *V, size_t ldv, int k, const double *h, double *w, double alpha, int N,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(vh_update_kernel, dim3(512), dim3(256), 0, s, V, ldv, k, h, w, alpha, N);
+    return hipGetLastError();
+}
+
 __global__ void axpy_cols_kernel(double *x, const double *d, int N, int ld, int nrhs)
 {
     const size_t total = (size_t)N * nrhs;

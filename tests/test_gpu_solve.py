@@ -67,3 +67,50 @@ def test_solve_factor_tile_relayout(hip, orc, algo, b, ft):
     for prec in (0, 2):
         X, rep, rc = hip.solve(a, B, algo=algo, precision=prec, b=b, factor_tile=ft)
         assert rc == 0 and backward_error(A, X, B) <= 2e-14
+
+
+def gen_cond(N, kappa, seed, sym):
+    """sym: A = Q diag(s) Q^T, Q from the QR of a seeded Gaussian.  Otherwise
+    A = L0 diag(s) U0 with unit triangular L0, U0 (small random off-diagonal
+    parts, well conditioned): its no-pivot LU is L0 and diag(s) U0, so the
+    factorization without pivoting is stable and cond(A) ~ kappa.  s is
+    geometric from 1 to 1/kappa, in a seeded random order."""
+    r = np.random.default_rng(seed)
+    s = r.permutation(np.geomspace(1.0, 1.0 / kappa, N))
+    if sym:
+        Q, _ = np.linalg.qr(r.standard_normal((N, N)))
+        return (Q * s) @ Q.T
+    L0 = np.eye(N) + np.tril(r.standard_normal((N, N)), -1) * (0.5 / np.sqrt(N))
+    U0 = np.eye(N) + np.triu(r.standard_normal((N, N)), 1) * (0.5 / np.sqrt(N))
+    return (L0 * s) @ U0
+
+
+@pytest.mark.parametrize("algo,prec,kappa", [("cholesky", 1, 1e6), ("cholesky", 2, 1e3), ("lu", 1, 1e6),
+                                             ("lu", 2, 1e3)])
+@pytest.mark.parametrize("refine", [1, 0])
+def test_gmres_ir_ill_conditioned(hip, orc, algo, prec, kappa, refine):
+    """GMRES-IR (refine=1) and the automatic switch (refine=0) reach 1e-14
+    where the low-precision factor alone contracts slowly or not at all."""
+    N, b = 512, 128
+    A = gen_cond(N, kappa, 8, algo == "cholesky")
+    a = orc.from_dense(A, N, b)
+    B = np.random.default_rng(9).standard_normal((N, 1))
+    X, rep, rc = hip.solve(a, B, algo=algo, precision=prec, b=b, max_iter=40, refine=refine)
+    assert rc == 0, (rc, rep.backward_error, rep.iterations, rep.gmres_iterations)
+    assert backward_error(A, X, B) <= 2e-14
+    if refine == 1:
+        assert rep.gmres_iterations >= rep.iterations
+    assert np.abs(X - np.linalg.solve(A, B)).max() <= kappa * 1e-13 * np.abs(X).max()
+
+
+def test_gmres_ir_well_conditioned(hip, orc):
+    N, b = 512, 128
+    a = orc.gen("dense", N, b, seed=4, shift=N)
+    A = orc.to_dense(a, N, b)
+    B = np.random.default_rng(2).standard_normal((N, 2))
+    X, rep, rc = hip.solve(a, B, algo="lu", precision=2, b=b, refine=1)
+    assert rc == 0 and rep.iterations <= 3 and rep.gmres_iterations >= 1
+    assert backward_error(A, X, B) <= 2e-14
+    # LU-IR only: never uses GMRES
+    X2, rep2, rc2 = hip.solve(a, B, algo="lu", precision=2, b=b, refine=2)
+    assert rc2 == 0 and rep2.gmres_iterations == 0
