@@ -124,64 +124,71 @@ class MLP:
         return loss
 
 
-def launches(algo, N, b, prec=0):
+def launches(algo, N, b, prec=0, split=True):
     """Launches of one factorization in the diagonal-chain schedule:
-    list of (k, stream, class, flops, bytes) with stream 'chain' or 'bulk'.
-    Mirrors dag_cholesky_chain / dag_lu_chain (palial_amd/csrc/api.hip);
-    the mixed tiers use packed panels when b % 128 == 0."""
+    list of (k, stream, class, flops, bytes) with stream 'chain', 'aux'
+    (row/column TRSMs and packing of step k), 'first' (update of tile row
+    k+1 and tile (k+2,k+2)) or 'rest' (the other trailing tiles).  Mirrors
+    dag_cholesky_chain / dag_lu_chain (palial_amd/csrc/api.hip) with the split
+    update (split=False: one update launch per step, labelled 'first'); the
+    mixed tiers use packed panels when b % 128 == 0."""
     MT = N // b
     b3, tb = float(b) ** 3, 8.0 * b * b
     packed = prec > 0 and b % 128 == 0
     esz = 2.0 if prec == 2 else 4.0
+    pfac = 1.0 + (0.25 if prec == 2 else 0.5)
     out = []
+    chol = algo == "cholesky"
     for k in range(MT):
         T = MT - k - 1
-        if algo == "cholesky":
+        if chol:
             out.append((k, "chain", CLASSES["diag"], b3 / 3, 2 * tb))
-            if T > 0:
-                out.append((k, "chain", CLASSES["trsm"], b3, 3 * tb))
-                out.append((k, "chain", CLASSES["gemm"], b3, 3 * tb))
-            if T > 1:
-                out.append((k, "bulk", CLASSES["trsm"], (T - 1) * b3, 3.0 * (T - 1) * tb))
-                ng, ns = T * (T - 1) / 2.0, T - 1.0
-                if packed:
-                    out.append((k, "bulk", CLASSES["pack"], 0.0, T * tb * (1.0 + (0.25 if prec == 2 else 0.5))))
-                    out.append((k, "bulk", CLASSES["pkgemm"], 2 * b3 * ng + b3 * ns, 2.0 * (ng + ns) * tb + T * esz * b * b))
-                else:
-                    out.append((k, "bulk", CLASSES["gemm"], 2 * b3 * ng + b3 * ns, 3.0 * (ng + ns) * tb))
         else:
             out.append((k, "chain", CLASSES["diag"], 2 * b3 / 3, 2 * tb))
-            if T > 0:
+        if T > 0:
+            for _ in range(1 if chol else 2):
                 out.append((k, "chain", CLASSES["trsm"], b3, 3 * tb))
-                out.append((k, "chain", CLASSES["trsm"], b3, 3 * tb))
-                out.append((k, "chain", CLASSES["gemm"], 2 * b3, 3 * tb))
-            if T > 1:
-                out.append((k, "bulk", CLASSES["trsm"], (T - 1) * b3, 3.0 * (T - 1) * tb))
-                out.append((k, "bulk", CLASSES["trsm"], (T - 1) * b3, 3.0 * (T - 1) * tb))
-                nt = T * T - 1.0
-                if packed:
-                    pb = T * tb * (1.0 + (0.25 if prec == 2 else 0.5))
-                    out.append((k, "bulk", CLASSES["pack"], 0.0, pb))
-                    out.append((k, "bulk", CLASSES["pack"], 0.0, pb))
-                    out.append((k, "bulk", CLASSES["pkgemm"], 2 * b3 * nt, 2.0 * nt * tb + 2.0 * T * esz * b * b))
-                else:
-                    out.append((k, "bulk", CLASSES["gemm"], 2 * b3 * nt, 3.0 * nt * tb))
+            out.append((k, "chain", CLASSES["gemm"], b3 if chol else 2 * b3, 3 * tb))
+        if T <= 1:
+            continue
+        for _ in range(1 if chol else 2):
+            out.append((k, "aux", CLASSES["trsm"], (T - 1) * b3, 3.0 * (T - 1) * tb))
+        if packed:
+            for _ in range(1 if chol else 2):
+                out.append((k, "aux", CLASSES["pack"], 0.0, T * tb * pfac))
+        # (gemm tiles, syrk tiles, packed panel tiles read) of each update launch
+        if chol:
+            parts = [("first", T - 1.0, 1.0, T), ("rest", (T - 1) * (T - 2) / 2.0, T - 2.0, T - 1)] if split \
+                else [("first", T * (T - 1) / 2.0, T - 1.0, T)]
+        else:
+            parts = [("first", 2.0 * T - 1, 0.0, T), ("rest", (T - 1.0) ** 2 - 1, 0.0, T - 1)] if split \
+                else [("first", T * T - 1.0, 0.0, T)]
+        for st, ng, ns, npan in parts:
+            if ng + ns <= 0:
+                continue
+            fl = 2 * b3 * ng + b3 * ns
+            if packed:
+                by = 2.0 * (ng + ns) * tb + (1 if chol else 2) * npan * esz * b * b
+                out.append((k, st, CLASSES["pkgemm"], fl, by))
+            else:
+                out.append((k, st, CLASSES["gemm"], fl, 3.0 * (ng + ns) * tb))
     return out
 
 
 def predict_factor_ms(model, algo, N, b, prec=0):
-    """Predicted factorization time: chain step k+1 overlaps bulk step k."""
+    """Predicted factorization time of the split chain schedule: step k's
+    first update launch, then its remaining update beside step k+1's row
+    TRSMs and packing, against the chain's step (both bounded below by it)."""
     L = launches(algo, N, b, prec)
     recs = np.array([(c, b, prec, f, by, 1.0) for (_, _, c, f, by) in L])
     X, _ = featurize(recs)
     ms = np.exp(model.predict_log_ms(X).astype(np.float64))
     MT = N // b
-    chain = np.zeros(MT + 1)
-    bulk = np.zeros(MT + 1)
+    acc = {st: np.zeros(MT + 1) for st in ("chain", "aux", "first", "rest")}
     for (k, st, _, _, _), t in zip(L, ms):
-        (chain if st == "chain" else bulk)[k] += t
-    # chain launches of step k produce P(k+1); they run beside bulk step k
-    total = 0.0
+        acc[st][k] += t
+    total = acc["aux"][0]
     for k in range(MT):
-        total += max(chain[k], bulk[k])
+        bulk = acc["first"][k] + max(acc["rest"][k], acc["aux"][k + 1])
+        total += max(acc["chain"][k], bulk)
     return float(total)

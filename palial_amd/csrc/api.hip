@@ -46,8 +46,11 @@ struct Ctx {
     hipStream_t s_panel = nullptr;  // look-ahead panel / diagonal-chain stream
     hipStream_t s_bulk = nullptr;   // bulk stream of the chain schedule: CU-masked away from
                                     // the reserved CUs (s_main when nothing is reserved)
+    hipStream_t s_aux = nullptr;    // chain schedule with the split update: the row/column
+                                    // TRSMs and panel packing (CU mask of s_bulk)
     int reserve = 0;                // CUs kept free of bulk work for the diagonal chain
-    int dag = 1;                    // 1: diagonal-chain schedule, 0: row look-ahead
+    int dag = 2;                    // 2: diagonal chain + split update, 1: diagonal chain,
+                                    // 0: row look-ahead
     std::vector<hipEvent_t> events;
     int *d_info = nullptr;
     double *d_tau = nullptr;        // QR tau scratch (b doubles, overwritten per panel)
@@ -56,7 +59,10 @@ struct Ctx {
     size_t w_cap = 0;
     double *d_dinv = nullptr;       // inverses of the 32x32 diagonal blocks (U then L)
     size_t dinv_cap = 0;
-    void *d_pack = nullptr;         // low-precision packed panels of one step (U then L)
+    double *d_sub = nullptr;        // one diagonal tile as sub-tiles (potrf_tile)
+    size_t sub_cap = 0;             // doubles
+    int subtile = 1;                // factor diagonal tiles of b >= 256 as 128-sub-tiles
+    void *d_pack = nullptr;         // low-precision packed panels of two steps (U then L each)
     size_t pack_cap = 0;            // bytes
     // solve workspace: factor copy F, diagonal-tile inverses, vectors
     double *d_F = nullptr;
@@ -164,14 +170,27 @@ int ensure_dinv(int b, int MT)
         if (hipMalloc(&g.d_dinv, need * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "dinv alloc failed");
         g.dinv_cap = need;
     }
+    const size_t sub = (size_t)b * b;
+    if (g.subtile && b >= 256 && b % 128 == 0 && g.sub_cap < sub) {
+        drop_graphs();
+        if (g.d_sub) hipFree(g.d_sub);
+        g.d_sub = nullptr;
+        g.sub_cap = 0;
+        if (hipMalloc(&g.d_sub, sub * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "sub-tile alloc failed");
+        g.sub_cap = sub;
+    }
     return 0;
 }
 
-// packed low-precision panel operands of one step: row panel at 0, column
-// panel at MT*b*b*4 bytes (4 bytes per element covers FP32 and BF16)
+
+// packed low-precision panel operands of two consecutive steps (step k uses
+// set k % 2): row panel at 0, column panel at MT*b*b*4 bytes of a set, 4 bytes
+// per element covering FP32 and BF16
+size_t pack_set_bytes(int b, int MT) { return 2 * (size_t)MT * b * b * 4; }
+
 int ensure_pack(int b, int MT)
 {
-    const size_t need = 2 * (size_t)MT * b * b * 4;
+    const size_t need = 2 * pack_set_bytes(b, MT);
     if (g.pack_cap < need) {
         drop_graphs();
         if (g.d_pack) hipFree(g.d_pack);
@@ -295,9 +314,41 @@ TrsmArgs targs(double *mat, int N, int b, int k)
     return t;
 }
 
+// Cholesky of a diagonal tile of the DAG (LAPACKE_dpotrf('U'),
+// src/cholesky.h:40), writing the 32x32 block inverses to dinv.  For b >= 256
+// (b % 128 == 0) the tile is copied out as a grid of 128-sub-tiles and
+// factored by the tiled algorithm itself: the one-workgroup kernel runs only
+// on the 128-diagonal sub-tiles, and the sub-tile TRSMs and SYRK/GEMM updates
+// spread over many workgroups instead of one CU.  info receives the first
+// failing global column (1-based) as with the one-kernel factorization.
+hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s)
+{
+    constexpr int bs = 128;
+    if (!g.subtile || b < 256 || b % bs != 0 || !dinv || !g.d_sub || g.sub_cap < (size_t)b * b)
+        return launch_potrf2(tile, b, info, info_base, dinv, s);
+    const int ms = b / bs;
+    hipError_t e;
+    if ((e = launch_tile_split(g.d_sub, tile, b, bs, true, s)) != hipSuccess) return e;
+    for (int j = 0; j < ms; j++) {
+        double *dj = dinv + (size_t)j * bs * 32;    // the bs/32 block inverses of sub-tile j
+        if ((e = launch_potrf2(g.d_sub + tile_off(bs, ms, j, j), bs, info, info_base + j * bs, dj, s)) != hipSuccess)
+            return e;
+        if (j + 1 == ms) break;
+        TrsmArgs t = targs(g.d_sub, b, bs, j);
+        t.first = j + 1;
+        t.ntiles = ms - j - 1;
+        if ((e = launch_trsm2(TR_CHOL, t, dj, s)) != hipSuccess) return e;
+        if ((e = launch_gemm_fine(gargs(GM_CHOL_TRAIL, g.d_sub, b, bs, j, ms - j - 1), true, false, s)) != hipSuccess)
+            return e;
+    }
+    return launch_tile_split(g.d_sub, tile, b, bs, false, s);
+}
+
 // events of one factorization: [0, MT+1) panel k ready, [MT+1, 2MT+2) update
-// k done, 2MT+2 fork, 2MT+3 graph gate, [2MT+4, 3MT+5) chain TRSM k done
-size_t events_needed(int MT) { return 3 * (size_t)MT + 6; }
+// k done (split update: its first launch), 2MT+2 fork, 2MT+3 graph gate,
+// [2MT+4, 3MT+5) chain TRSM k done, [3MT+6, 4MT+7) row/column TRSM and packing
+// of step k done, 4MT+7 end of the bulk stream
+size_t events_needed(int MT) { return 4 * (size_t)MT + 8; }
 
 GemmArgs single_tile(const double *A, const double *B, double *C, int b, bool syrk)
 {
@@ -320,19 +371,30 @@ GemmArgs single_tile(const double *A, const double *B, double *C, int b, bool sy
 // (k+1,k+1) up to date with rows < k; R(k) waits for T1(k).  Every tile
 // therefore receives the updates of rows 0..k in increasing k as in the
 // reference, with the same kernels: only the grouping into launches differs.
-void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, int *dinfo,
-                        Launcher &L, int prec)
+//
+// Split update (sa != nullptr): R(k) is launched as F(k), the rest of tile
+// row k+1 and tile (k+2,k+2), then E(k), every other tile.  TB(k) and the
+// packing of row k run on a third stream sa after F(k-1) and P(k), i.e.
+// beside E(k-1), so they leave the bulk stream's critical path; the chain
+// waits for F(k-1) only.  The bulk order F(k-1) E(k-1) F(k) E(k) keeps each
+// tile's updates in increasing k, and packed set k % 2 is rewritten by step
+// k+2's packing only after F(k+1), which follows E(k) on the bulk stream.
+void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, hipStream_t sa,
+                        int *dinfo, Launcher &L, int prec)
 {
     L.b = b;
     L.prec = prec;
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
-    const bool pk = lp && g.d_pack && g.pack_cap >= 2 * (size_t)MT * b * b * 4;
+    const bool split = sa != nullptr;
+    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * pack_set_bytes(b, MT);
+    const double esz = prec == 2 ? 2.0 : 4.0;
+    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * pack_set_bytes(b, MT) : 0); };
     auto dinvU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
     auto potrf = [&](int k, hipStream_t s) {
         L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s, [&] {
-            return v2 ? launch_potrf2(dA + tile_off(b, MT, k, k), b, dinfo, k * b, dinvU(k), s)
+            return v2 ? potrf_tile(dA + tile_off(b, MT, k, k), b, dinfo, k * b, dinvU(k), s)
                       : launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s);
         });
     };
@@ -348,11 +410,31 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
         L.run(KC_GEMM, flops, 3.0 * tiles * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
     };
-    hipEvent_t *evP = g.events.data(), *evR = g.events.data() + MT + 1, *evT = g.events.data() + 2 * MT + 4;
-    hipEvent_t fork = g.events[2 * MT + 2];
+    // trailing update of step k over tasks [1, count) of `mode`: ngemm GEMM and
+    // nsyrk SYRK tiles, reading npanel tiles of the packed row k
+    auto update = [&](int mode, int k, double ngemm, double nsyrk, double npanel) {
+        const int T = MT - k - 1;
+        GemmArgs a = gargs(mode, dA, N, b, k, T);
+        a.task0 = 1;                     // task 0 is tile (k+1,k+1) (chain) or (k+2,k+2) (F(k))
+        a.ntasks -= 1;
+        if (a.ntasks <= 0) return;
+        const double fl = 2 * b3(b) * ngemm + b3(b) * nsyrk;
+        if (pk) {
+            a.pa = a.pb = pset(k);
+            L.run(KC_PKGEMM, fl, 2.0 * (ngemm + nsyrk) * tile_bytes(b) + npanel * esz * b * b, sb,
+                  [&] { return launch_gemm_packed(a, prec, sb); });
+        } else {
+            gemm(a, fl, ngemm + nsyrk, sb);
+        }
+    };
+    hipEvent_t *ev = g.events.data();
+    hipEvent_t *evP = ev, *evR = ev + MT + 1, *evT = ev + 2 * MT + 4, *evB = ev + 3 * MT + 6;
+    hipEvent_t fork = ev[2 * MT + 2], evEnd = ev[4 * MT + 7];
+    hipStream_t st = split ? sa : sb;    // stream of TB(k) and the packing
     L.chk(hipEventRecord(fork, s0));
     L.chk(hipStreamWaitEvent(sc, fork, 0));
     if (sb != s0) L.chk(hipStreamWaitEvent(sb, fork, 0));
+    if (split) L.chk(hipStreamWaitEvent(sa, fork, 0));
     potrf(0, sc);
     L.chk(hipEventRecord(evP[0], sc));
     for (int k = 0; k < MT; k++) {
@@ -367,43 +449,50 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
             potrf(k + 1, sc);
             L.chk(hipEventRecord(evP[k + 1], sc));
         }
-        L.chk(hipStreamWaitEvent(sb, evP[k], 0));
-        if (T > 1) trsm(k, k + 2, T - 1, sb);
-        if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
-        if (T > 1) {
-            GemmArgs a = gargs(GM_CHOL_TRAIL, dA, N, b, k, T);
-            a.task0 = 1;                 // task 0 is tile (k+1,k+1), done by the chain
-            a.ntasks -= 1;
-            const double ngemm = (double)T * (T - 1) / 2, nsyrk = T - 1;
-            if (pk) {
-                // row k rounded once; the update streams the packed copy
-                L.run(KC_PACK, 0, T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5)), sb,
-                      [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, g.d_pack, sb); });
-                a.pa = a.pb = g.d_pack;
-                const double esz = prec == 2 ? 2.0 : 4.0;
-                L.run(KC_PKGEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk,
-                      2.0 * (ngemm + nsyrk) * tile_bytes(b) + T * esz * b * b, sb,
-                      [&] { return launch_gemm_packed(a, prec, sb); });
-            } else {
-                gemm(a, 2 * b3(b) * ngemm + b3(b) * nsyrk, ngemm + nsyrk, sb);
-            }
+        if (split && k > 0) L.chk(hipStreamWaitEvent(sa, evR[k - 1], 0));
+        L.chk(hipStreamWaitEvent(st, evP[k], 0));
+        if (T > 1) trsm(k, k + 2, T - 1, st);
+        if (T > 1 && pk) {
+            // row k rounded once; the update streams the packed copy
+            L.chk(hipStreamWaitEvent(st, evT[k], 0));
+            L.run(KC_PACK, 0, T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5)), st,
+                  [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, pset(k), st); });
         }
-        L.chk(hipEventRecord(evR[k], sb));
+        if (split) {
+            L.chk(hipEventRecord(evB[k], sa));
+            L.chk(hipStreamWaitEvent(sb, evB[k], 0));
+        }
+        if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
+        if (!split) {
+            if (T > 1) update(GM_CHOL_TRAIL, k, (double)T * (T - 1) / 2, T - 1, T);
+            L.chk(hipEventRecord(evR[k], sb));
+        } else {
+            if (T > 1) update(GM_CHOL_FIRST, k, T - 1, 1, T);
+            L.chk(hipEventRecord(evR[k], sb));
+            if (T > 2) update(GM_CHOL_REST, k, (double)(T - 1) * (T - 2) / 2, T - 2, T - 1);
+        }
     }
-    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evR[MT - 1], 0));
+    L.chk(hipEventRecord(evEnd, sb));
+    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evEnd, 0));
 }
 
 // Diagonal-chain schedule of the no-pivot LU (src/lu.h:14-173), same shape:
 // the chain solves tiles (k,k+1) and (k+1,k) and updates (k+1,k+1); the bulk
 // solves the rest of row and column k and updates every other trailing tile.
-void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, Launcher &L, int prec)
+// Split update as for the Cholesky: F(k) is tile row and column k+1 and tile
+// (k+2,k+2), E(k) the tiles (m,n), m,n >= k+2, but (k+2,k+2).
+void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, hipStream_t sa,
+                  Launcher &L, int prec)
 {
     L.b = b;
     L.prec = prec;
     const int MT = N / b;
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
-    const bool pk = lp && g.d_pack && g.pack_cap >= 2 * (size_t)MT * b * b * 4;
+    const bool split = sa != nullptr;
+    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * pack_set_bytes(b, MT);
+    const double esz = prec == 2 ? 2.0 : 4.0;
+    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * pack_set_bytes(b, MT) : 0); };
     auto dU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
     auto dL = [&](int k) { return g.d_dinv + (size_t)k * 64 * b + (size_t)32 * b; };
     auto getrf = [&](int k, hipStream_t s) {
@@ -424,11 +513,33 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
         L.run(KC_GEMM, 2 * b3(b) * tiles, 3.0 * tiles * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
     };
-    hipEvent_t *evP = g.events.data(), *evR = g.events.data() + MT + 1, *evT = g.events.data() + 2 * MT + 4;
-    hipEvent_t fork = g.events[2 * MT + 2];
+    // trailing update of step k over tasks [1, count) of `mode`, reading npanel
+    // tiles of each packed panel
+    auto update = [&](int mode, int k, double npanel) {
+        const int T = MT - k - 1;
+        GemmArgs a = gargs(mode, dA, N, b, k, T);
+        a.task0 = 1;                     // task 0 is tile (k+1,k+1) (chain) or (k+2,k+2) (F(k))
+        a.ntasks -= 1;
+        if (a.ntasks <= 0) return;
+        const double nt = a.ntasks;
+        if (pk) {
+            char *pU = pset(k), *pL = pU + (size_t)MT * b * b * 4;
+            a.pa = pL;
+            a.pb = pU;
+            L.run(KC_PKGEMM, 2 * b3(b) * nt, 2.0 * nt * tile_bytes(b) + 2.0 * npanel * esz * b * b, sb,
+                  [&] { return launch_gemm_packed(a, prec, sb); });
+        } else {
+            gemm(a, nt, sb);
+        }
+    };
+    hipEvent_t *ev = g.events.data();
+    hipEvent_t *evP = ev, *evR = ev + MT + 1, *evT = ev + 2 * MT + 4, *evB = ev + 3 * MT + 6;
+    hipEvent_t fork = ev[2 * MT + 2], evEnd = ev[4 * MT + 7];
+    hipStream_t st = split ? sa : sb;
     L.chk(hipEventRecord(fork, s0));
     L.chk(hipStreamWaitEvent(sc, fork, 0));
     if (sb != s0) L.chk(hipStreamWaitEvent(sb, fork, 0));
+    if (split) L.chk(hipStreamWaitEvent(sa, fork, 0));
     getrf(0, sc);
     L.chk(hipEventRecord(evP[0], sc));
     for (int k = 0; k < MT; k++) {
@@ -444,34 +555,35 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
             getrf(k + 1, sc);
             L.chk(hipEventRecord(evP[k + 1], sc));
         }
-        L.chk(hipStreamWaitEvent(sb, evP[k], 0));
+        if (split && k > 0) L.chk(hipStreamWaitEvent(sa, evR[k - 1], 0));
+        L.chk(hipStreamWaitEvent(st, evP[k], 0));
         if (T > 1) {
-            trsm(TR_LUL, k, k + 2, T - 1, sb);
-            trsm(TR_LUR, k, k + 2, T - 1, sb);
+            trsm(TR_LUL, k, k + 2, T - 1, st);
+            trsm(TR_LUR, k, k + 2, T - 1, st);
+        }
+        if (T > 1 && pk) {
+            L.chk(hipStreamWaitEvent(st, evT[k], 0));
+            char *pU = pset(k), *pL = pU + (size_t)MT * b * b * 4;
+            const double pb = T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5));
+            L.run(KC_PACK, 0, pb, st, [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, pU, st); });
+            L.run(KC_PACK, 0, pb, st, [&] { return launch_pack(dA, N, b, k, k + 1, T, true, prec, pL, st); });
+        }
+        if (split) {
+            L.chk(hipEventRecord(evB[k], sa));
+            L.chk(hipStreamWaitEvent(sb, evB[k], 0));
         }
         if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
-        if (T > 1) {
-            GemmArgs a = gargs(GM_LU_TRAIL, dA, N, b, k, T);
-            a.task0 = 1;                 // task 0 is tile (k+1,k+1), done by the chain
-            a.ntasks -= 1;
-            if (pk) {
-                char *pU = (char *)g.d_pack, *pL = pU + (size_t)MT * b * b * 4;
-                const double pb = T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5));
-                L.run(KC_PACK, 0, pb, sb, [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, pU, sb); });
-                L.run(KC_PACK, 0, pb, sb, [&] { return launch_pack(dA, N, b, k, k + 1, T, true, prec, pL, sb); });
-                a.pa = pL;
-                a.pb = pU;
-                const double nt = (double)T * T - 1;
-                const double esz = prec == 2 ? 2.0 : 4.0;
-                L.run(KC_PKGEMM, 2 * b3(b) * nt, 2.0 * nt * tile_bytes(b) + 2.0 * T * esz * b * b, sb,
-                      [&] { return launch_gemm_packed(a, prec, sb); });
-            } else {
-                gemm(a, (double)T * T - 1, sb);
-            }
+        if (!split) {
+            if (T > 1) update(GM_LU_TRAIL, k, T);
+            L.chk(hipEventRecord(evR[k], sb));
+        } else {
+            if (T > 1) update(GM_LU_FIRST, k, T);
+            L.chk(hipEventRecord(evR[k], sb));
+            if (T > 2) update(GM_LU_REST, k, T - 1);
         }
-        L.chk(hipEventRecord(evR[k], sb));
     }
-    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evR[MT - 1], 0));
+    L.chk(hipEventRecord(evEnd, sb));
+    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evEnd, 0));
 }
 
 // Cholesky, src/cholesky.h:15-176.  Look-ahead: step k's update of tile row
@@ -488,7 +600,7 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
     auto dinvU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
     auto potrf = [&](int k, hipStream_t s) {
         L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s, [&] {
-            return v2 ? launch_potrf2(dA + tile_off(b, MT, k, k), b, dinfo, k * b, dinvU(k), s)
+            return v2 ? potrf_tile(dA + tile_off(b, MT, k, k), b, dinfo, k * b, dinvU(k), s)
                       : launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s);
         });
     };
@@ -509,8 +621,8 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
         L.run(KC_GEMM, 2 * b3(b) * ngemm + b3(b) * nsyrk, 3.0 * (ngemm + nsyrk) * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
     };
-    if (lookahead && g.dag == 1) {
-        dag_cholesky_chain(dA, N, b, s0, s1, g.s_bulk, dinfo, L, prec);
+    if (lookahead && g.dag >= 1) {
+        dag_cholesky_chain(dA, N, b, s0, s1, g.s_bulk, g.dag == 2 ? g.s_aux : nullptr, dinfo, L, prec);
         return;
     }
     if (!lookahead) {
@@ -573,8 +685,8 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
         L.run(KC_GEMM, 2 * b3(b) * ntile, 3.0 * ntile * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
     };
-    if (lookahead && g.dag == 1) {
-        dag_lu_chain(dA, N, b, s0, s1, g.s_bulk, L, prec);
+    if (lookahead && g.dag >= 1) {
+        dag_lu_chain(dA, N, b, s0, s1, g.s_bulk, g.dag == 2 ? g.s_aux : nullptr, L, prec);
         return;
     }
     if (!lookahead) {
@@ -966,7 +1078,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     hipStream_t s0 = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
     hipStream_t s1 = g.s_panel;
     const bool lookahead = !o.no_lookahead && MT > 1 && algo != ALG_QR;
-    const bool use_graph = !o.no_graph && !g_prof.on && !(lookahead && g.dag == 1 && g.reserve > 0);
+    const bool use_graph = !o.no_graph && !g_prof.on && !(lookahead && g.dag >= 1 && g.reserve > 0);
     if ((rc = ensure_events(events_needed(MT)))) return rc;
     if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
     if ((rc = ensure_dinv(b, MT))) return rc;
@@ -1174,23 +1286,30 @@ int palial_hip_init(int device)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(PALIAL_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
-    HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
     // PALIAL_RESERVE_CUS=n (default 16): the bulk stream of the chain schedule
     // runs on all CUs but the first n, which stay free for the diagonal chain.
-    // PALIAL_DAG=0 selects the row look-ahead schedule instead.
+    // PALIAL_DAG=1 selects the chain schedule with one update launch per
+    // step, PALIAL_DAG=0 the row look-ahead schedule.
     const char *res = getenv("PALIAL_RESERVE_CUS");
     const char *dag = getenv("PALIAL_DAG");
-    g.dag = dag ? atoi(dag) : 1;
+    g.dag = dag ? atoi(dag) : 2;
+    if (g.dag < 0 || g.dag > 2) return fail(PALIAL_ERR_ARG, "PALIAL_DAG=%s: expected 0, 1 or 2", dag);
     g.reserve = res ? atoi(res) : 16;
+    // PALIAL_SUBTILE=0 factors diagonal tiles with the one-workgroup kernel only
+    const char *sub = getenv("PALIAL_SUBTILE");
+    g.subtile = sub ? atoi(sub) : 1;
+    HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
     if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
         const int ncu = prop.multiProcessorCount, words = (ncu + 31) / 32;
         std::vector<uint32_t> mu(words, 0);
         for (int cu = g.reserve; cu < ncu; cu++) mu[cu / 32] |= 1u << (cu % 32);
         HIPCHK(hipExtStreamCreateWithCUMask(&g.s_bulk, words, mu.data()));
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_aux, words, mu.data()));
     } else {
         g.reserve = 0;
         g.s_bulk = g.s_main;
+        HIPCHK(hipStreamCreateWithFlags(&g.s_aux, hipStreamNonBlocking));
     }
     HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
     panel_init();
@@ -1217,6 +1336,7 @@ int palial_hip_finalize(void)
     if (g.d_w2) hipFree(g.d_w2);
     if (g.d_tau) hipFree(g.d_tau);
     if (g.d_dinv) hipFree(g.d_dinv);
+    if (g.d_sub) hipFree(g.d_sub);
     if (g.d_pack) hipFree(g.d_pack);
     if (g.d_F) hipFree(g.d_F);
     if (g.d_inv) hipFree(g.d_inv);
@@ -1227,6 +1347,7 @@ int palial_hip_finalize(void)
     if (g.d_vec) hipFree(g.d_vec);
     if (g.d_info) hipFree(g.d_info);
     if (g.s_bulk && g.s_bulk != g.s_main) hipStreamDestroy(g.s_bulk);
+    if (g.s_aux) hipStreamDestroy(g.s_aux);
     hipStreamDestroy(g.s_main);
     hipStreamDestroy(g.s_panel);
     g = Ctx();

@@ -63,14 +63,17 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.ib = m - k - 1;
         break;
     }
-    case GM_CHOL_ROW: {
-        int n = k + 1, m = k + 1 + task;
+    case GM_CHOL_ROW:
+    case GM_CHOL_FIRST: {
+        // GM_CHOL_FIRST: task T is the diagonal tile (k+2, k+2)
+        const bool last = p.mode == GM_CHOL_FIRST && task == p.T;
+        int n = last ? k + 2 : k + 1, m = last ? k + 2 : k + 1 + task;
         t.A = M + tile_off(b, MT, k, n);
         t.B = M + tile_off(b, MT, k, m);
         t.C = M + tile_off(b, MT, n, m);
         t.D = t.C;
         t.diag = (n == m);
-        t.ia = 0;
+        t.ia = n - k - 1;
         t.ib = m - k - 1;
         break;
     }
@@ -88,11 +91,14 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.ib = n - k - 1;
         break;
     }
-    case GM_LU_PANEL: {
+    case GM_LU_PANEL:
+    case GM_LU_FIRST: {
         // tasks [0, T): column k+1, rows m = k+1..MT-1 ; [T, 2T-1): row k+1, cols n = k+2..
+        // GM_LU_FIRST: task 2T-1 is tile (k+2, k+2)
         int m, n;
         if (task < p.T) { m = k + 1 + task; n = k + 1; }
-        else { m = k + 1; n = k + 2 + (task - p.T); }
+        else if (task < 2 * p.T - 1) { m = k + 1; n = k + 2 + (task - p.T); }
+        else { m = k + 2; n = k + 2; }
         t.A = M + tile_off(b, MT, m, k);
         t.B = M + tile_off(b, MT, k, n);
         t.C = M + tile_off(b, MT, m, n);

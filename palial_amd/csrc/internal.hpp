@@ -32,6 +32,8 @@ enum GemmMode : int {
     GM_CHOL_REST = 10,  // step k, tiles (n, m) with k+2 <= n <= m
     GM_LU_PANEL = 11,   // step k, tiles (m, k+1) and (k+1, n) for m, n >= k+1 (look-ahead part)
     GM_LU_REST = 12,    // step k, tiles (m, n) with m, n >= k+2
+    GM_CHOL_FIRST = 13, // GM_CHOL_ROW followed by tile (k+2, k+2) (task T)
+    GM_LU_FIRST = 14,   // GM_LU_PANEL followed by tile (k+2, k+2) (task 2T-1)
 };
 
 enum MaskA : int { MK_NONE = 0, MK_UPPER = 1, MK_UNIT_LOWER = 2 };
@@ -63,6 +65,9 @@ struct GemmArgs {
 
 // host launchers (k_gemm.hip)
 hipError_t launch_gemm(const GemmArgs &a, bool transA, bool transB, hipStream_t s);
+// same with 32 x 32 output blocks of one wave each (b % 32 == 0): many
+// workgroups for launches of few tiles
+hipError_t launch_gemm_fine(const GemmArgs &a, bool transA, bool transB, hipStream_t s);
 long gemm_task_count(int mode, int T);
 
 // ------------------------------------------------------------------ panels
@@ -172,6 +177,8 @@ hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, d
 hipError_t launch_fill_identity_ipiv(int *ipiv, int N, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N, hipStream_t s);
+// b x b column-major tile <-> its bs-sub-tiles in layout L (to_sub: tile -> sub)
+hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s);
 hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2, hipStream_t s);
 
 }  // namespace palial

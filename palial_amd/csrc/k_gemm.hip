@@ -306,6 +306,8 @@ long gemm_task_count(int mode, int T)
     case GM_LU_TRAIL: return (long)T * T;
     case GM_LU_PANEL: return 2L * T - 1;
     case GM_LU_REST: return (long)(T - 1) * (T - 1);
+    case GM_CHOL_FIRST: return T + 1;
+    case GM_LU_FIRST: return 2L * T;
     default: return T;  // QR / ORMQR modes: one task per trailing column tile
     }
 }
@@ -332,6 +334,13 @@ static int gemm_variant()
         v = e ? atoi(e) : 0;
     }
     return v;
+}
+
+hipError_t launch_gemm_fine(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
+{
+    if (a.ntasks <= 0) return hipSuccess;
+    if (a.b % 32 != 0) return launch_gemm(a, ta, tb, s);
+    return launch_mfma<32, 16, 1>(a, ta, tb, s);
 }
 
 hipError_t launch_gemm(const GemmArgs &a, bool ta, bool tb, hipStream_t s)

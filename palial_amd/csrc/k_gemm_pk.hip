@@ -458,7 +458,7 @@ hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
     if (a.ntasks <= 0) return hipSuccess;
     if (a.b % 128 != 0 || a.maskA != MK_NONE || !a.pa || !a.pb) return hipErrorInvalidValue;
     if (a.mode != GM_CHOL_TRAIL && a.mode != GM_CHOL_REST && a.mode != GM_CHOL_ROW && a.mode != GM_LU_TRAIL &&
-        a.mode != GM_LU_REST && a.mode != GM_LU_PANEL)
+        a.mode != GM_LU_REST && a.mode != GM_LU_PANEL && a.mode != GM_CHOL_FIRST && a.mode != GM_LU_FIRST)
         return hipErrorInvalidValue;
     const long nbd = a.b / 128;
     dim3 grid((unsigned)((long)a.ntasks * nbd * nbd)), block(256);

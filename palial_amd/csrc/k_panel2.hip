@@ -184,7 +184,8 @@ __device__ __forceinline__ void potrf2_tile(double *A, int b, int *info, int inf
                 for (int r = 0; r < NB; r++)
                     if (r <= c) sU[r + c * LDD] = x[r];
             }
-            if (bad >= 0 && lane == 0) { s_fail = 1; if (info) *info = info_base + j0 + bad + 1; }
+            // the first failure of a factorization is kept (later tiles factor garbage)
+            if (bad >= 0 && lane == 0) { s_fail = 1; if (info) atomicCAS(info, 0, info_base + j0 + bad + 1); }
         }
         __syncthreads();
         for (int e = t; e < NB * NB; e += PF_THREADS) {

@@ -47,4 +47,7 @@ def test_launch_flops_sum_to_tiled_count(algo, prec):
         # GETRF + 2 TRSM per pair + GEMM over the trailing squares (src/lu.h)
         exp = MT * 2 * b3 / 3 + 2 * pairs * b3 + sum((MT - k - 1) ** 2 for k in range(MT)) * 2 * b3
     assert np.isclose(tot, exp, rtol=1e-12)
-    assert {st for (_, st, _, _, _) in L} == {"chain", "bulk"}
+    assert {st for (_, st, _, _, _) in L} == {"chain", "aux", "first", "rest"}
+    # one update launch per step: same flops
+    tot1 = sum(f for (_, _, _, f, _) in cm.launches(algo, N, b, prec, split=False))
+    assert np.isclose(tot1, exp, rtol=1e-12)

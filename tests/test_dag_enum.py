@@ -9,6 +9,7 @@ import pytest
 
 CHOL_TRAIL, LU_TRAIL, QR_W, QR_TW, QR_VW, ORM_W, ORM_TW, ORM_VW = 1, 2, 3, 4, 5, 6, 7, 8
 CHOL_ROW, CHOL_REST, LU_PANEL, LU_REST = 9, 10, 11, 12
+CHOL_FIRST, LU_FIRST = 13, 14
 
 
 def tasks(mode, MT, k, m=0, b=4):
@@ -54,6 +55,28 @@ def test_lu_step_tasks(MT):
         rest = tasks(LU_REST, MT, k) if MT - k - 1 > 1 else []
         assert set(pan) | set(rest) == expect and not set(pan) & set(rest)
         assert all(t[2][0] >= k + 2 and t[2][1] >= k + 2 for t in rest)
+
+
+@pytest.mark.parametrize("MT", [3, 4, 7])
+def test_split_update_tasks(MT):
+    """Split trailing update of the chain schedule: the chain updates tile
+    (k+1,k+1); launch FIRST (from task 1) the rest of tile row k+1 (LU: row and
+    column k+1) plus tile (k+2,k+2); launch REST (from task 1) the remainder.
+    Together they are step k's update, each tile once."""
+    for k in range(MT - 2):
+        chain = {(k + 1, k + 1)}
+        for full_mode, first_mode, rest_mode in ((CHOL_TRAIL, CHOL_FIRST, CHOL_REST), (LU_TRAIL, LU_FIRST, LU_REST)):
+            full = tasks(full_mode, MT, k)
+            first = tasks(first_mode, MT, k)
+            rest = tasks(rest_mode, MT, k)
+            assert first[0][2] == (k + 1, k + 1) and rest[0][2] == (k + 2, k + 2)   # task 0: skipped by task0 = 1
+            first, rest = first[1:], rest[1:]
+            assert (k + 2, k + 2) in [t[2] for t in first]
+            outs = [t[2] for t in first] + [t[2] for t in rest]
+            assert len(outs) == len(set(outs)) and not chain & set(outs)
+            assert set(first) | set(rest) | {t for t in full if t[2] in chain} == set(full)
+            if full_mode == CHOL_TRAIL:
+                assert all(t[2][0] == k + 1 or t[2] == (k + 2, k + 2) for t in first)
 
 
 @pytest.mark.parametrize("MT", [2, 4, 6])
