@@ -136,7 +136,9 @@ int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms)
  * A_ij -= A_ik A_kj, all tiles i, j > k) reps times to a device-resident
  * matrix.  prec 0 uses FP64 operands; 1 (FP32) and 2 (BF16) round the panel
  * of step k once into packed K-contiguous copies and accumulate in FP32, as
- * the mixed-precision solves do (b % 128 == 0).  *ms: average time per rep
+ * the mixed-precision solves do (b % 128 == 0).  prec | 8 (with 1 or 2): the
+ * matrix is held as FP32 tiles in the first N*N*4 bytes of the buffer, as the
+ * mixed-precision solves' working copy is.  *ms: average time per rep
  * including the packing. */
 int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double *ms);
 

@@ -124,21 +124,27 @@ class MLP:
         return loss
 
 
-def launches(algo, N, b, prec=0, split=True):
+def launches(algo, N, b, prec=0, split=True, fs=None):
     """Launches of one factorization in the diagonal-chain schedule:
     list of (k, stream, class, flops, bytes) with stream 'chain', 'aux'
     (row/column TRSMs and packing of step k), 'first' (update of tile row
     k+1 and tile (k+2,k+2)) or 'rest' (the other trailing tiles).  Mirrors
     dag_cholesky_chain / dag_lu_chain (palial_amd/csrc/api.hip) with the split
     update (split=False: one update launch per step, labelled 'first'); the
-    mixed tiers use packed panels when b % 128 == 0."""
+    mixed tiers use packed panels when b % 128 == 0.  fs: the solve's FP32
+    working copy (default: packed panels on the split schedule), with FP32
+    tiles in the byte counts and the chain's diagonal update on the packed
+    kernel."""
     MT = N // b
     b3, tb = float(b) ** 3, 8.0 * b * b
     packed = prec > 0 and b % 128 == 0
+    if fs is None:
+        fs = packed and split and MT > 1
+    sbt = 4.0 * b * b if fs else tb             # bytes of one stored tile
     esz = 2.0 if prec == 2 else 4.0
-    pfac = 1.0 + (0.25 if prec == 2 else 0.5)
     out = []
     chol = algo == "cholesky"
+    npan = 1 if chol else 2                      # packed panels per step
     for k in range(MT):
         T = MT - k - 1
         if chol:
@@ -146,16 +152,21 @@ def launches(algo, N, b, prec=0, split=True):
         else:
             out.append((k, "chain", CLASSES["diag"], 2 * b3 / 3, 2 * tb))
         if T > 0:
-            for _ in range(1 if chol else 2):
-                out.append((k, "chain", CLASSES["trsm"], b3, 3 * tb))
-            out.append((k, "chain", CLASSES["gemm"], b3 if chol else 2 * b3, 3 * tb))
+            for _ in range(npan):
+                out.append((k, "chain", CLASSES["trsm"], b3, 3 * sbt))
+            if fs:
+                for _ in range(npan):
+                    out.append((k, "chain", CLASSES["pack"], 0.0, sbt + esz * b * b))
+                out.append((k, "chain", CLASSES["pkgemm"], b3 if chol else 2 * b3, 2 * sbt + npan * esz * b * b))
+            else:
+                out.append((k, "chain", CLASSES["gemm"], b3 if chol else 2 * b3, 3 * tb))
         if T <= 1:
             continue
-        for _ in range(1 if chol else 2):
-            out.append((k, "aux", CLASSES["trsm"], (T - 1) * b3, 3.0 * (T - 1) * tb))
+        for _ in range(npan):
+            out.append((k, "aux", CLASSES["trsm"], (T - 1) * b3, 3.0 * (T - 1) * sbt))
         if packed:
-            for _ in range(1 if chol else 2):
-                out.append((k, "aux", CLASSES["pack"], 0.0, T * tb * pfac))
+            for _ in range(npan):
+                out.append((k, "aux", CLASSES["pack"], 0.0, T * (sbt + esz * b * b)))
         # (gemm tiles, syrk tiles, packed panel tiles read) of each update launch
         if chol:
             parts = [("first", T - 1.0, 1.0, T), ("rest", (T - 1) * (T - 2) / 2.0, T - 2.0, T - 1)] if split \
@@ -163,13 +174,12 @@ def launches(algo, N, b, prec=0, split=True):
         else:
             parts = [("first", 2.0 * T - 1, 0.0, T), ("rest", (T - 1.0) ** 2 - 1, 0.0, T - 1)] if split \
                 else [("first", T * T - 1.0, 0.0, T)]
-        for st, ng, ns, npan in parts:
+        for st, ng, ns, np_ in parts:
             if ng + ns <= 0:
                 continue
             fl = 2 * b3 * ng + b3 * ns
             if packed:
-                by = 2.0 * (ng + ns) * tb + (1 if chol else 2) * npan * esz * b * b
-                out.append((k, st, CLASSES["pkgemm"], fl, by))
+                out.append((k, st, CLASSES["pkgemm"], fl, 2.0 * (ng + ns) * sbt + npan * np_ * esz * b * b))
             else:
                 out.append((k, st, CLASSES["gemm"], fl, 3.0 * (ng + ns) * tb))
     return out

@@ -62,6 +62,7 @@ struct Ctx {
     double *d_sub = nullptr;        // one diagonal tile as sub-tiles (potrf_tile)
     size_t sub_cap = 0;             // doubles
     int subtile = 1;                // factor diagonal tiles of b >= 256 as 128-sub-tiles
+    int f32work = 1;                // mixed-tier solves factor an FP32 working copy
     void *d_pack = nullptr;         // low-precision packed panels of two steps (U then L each)
     size_t pack_cap = 0;            // bytes
     // solve workspace: factor copy F, diagonal-tile inverses, vectors
@@ -190,7 +191,8 @@ size_t pack_set_bytes(int b, int MT) { return 2 * (size_t)MT * b * b * 4; }
 
 int ensure_pack(int b, int MT)
 {
-    const size_t need = 2 * pack_set_bytes(b, MT);
+    // two step sets, then the one or two packed tiles of the chain's S1 (FP32 working copy)
+    const size_t need = 2 * pack_set_bytes(b, MT) + 2 * (size_t)b * b * 4;
     if (g.pack_cap < need) {
         drop_graphs();
         if (g.d_pack) hipFree(g.d_pack);
@@ -321,14 +323,19 @@ TrsmArgs targs(double *mat, int N, int b, int k)
 // on the 128-diagonal sub-tiles, and the sub-tile TRSMs and SYRK/GEMM updates
 // spread over many workgroups instead of one CU.  info receives the first
 // failing global column (1-based) as with the one-kernel factorization.
-hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s)
+// tile32: the tile is FP32 (the working copy of a mixed-precision solve); it
+// is factored in FP64 in the copy and rounded back.
+hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s, bool tile32 = false)
 {
-    constexpr int bs = 128;
-    if (!g.subtile || b < 256 || b % bs != 0 || !dinv || !g.d_sub || g.sub_cap < (size_t)b * b)
+    const bool sub = g.subtile && b >= 256 && b % 128 == 0;
+    const bool copy = (sub || tile32) && dinv && g.d_sub && g.sub_cap >= (size_t)b * b;
+    if (!copy) {
+        if (tile32) return hipErrorInvalidValue;
         return launch_potrf2(tile, b, info, info_base, dinv, s);
-    const int ms = b / bs;
+    }
+    const int bs = sub ? 128 : b, ms = b / bs;
     hipError_t e;
-    if ((e = launch_tile_split(g.d_sub, tile, b, bs, true, s)) != hipSuccess) return e;
+    if ((e = launch_tile_split(g.d_sub, tile, b, bs, true, s, tile32)) != hipSuccess) return e;
     for (int j = 0; j < ms; j++) {
         double *dj = dinv + (size_t)j * bs * 32;    // the bs/32 block inverses of sub-tile j
         if ((e = launch_potrf2(g.d_sub + tile_off(bs, ms, j, j), bs, info, info_base + j * bs, dj, s)) != hipSuccess)
@@ -341,7 +348,19 @@ hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *din
         if ((e = launch_gemm_fine(gargs(GM_CHOL_TRAIL, g.d_sub, b, bs, j, ms - j - 1), true, false, s)) != hipSuccess)
             return e;
     }
-    return launch_tile_split(g.d_sub, tile, b, bs, false, s);
+    return launch_tile_split(g.d_sub, tile, b, bs, false, s, tile32);
+}
+
+// LU of a diagonal tile (PALIAL_dgetrf, src/srclu/dgetrf.h:11); tile32 as
+// for potrf_tile (factored in FP64 in the copy).
+hipError_t getrf_tile(double *tile, int b, double *dU, double *dL, hipStream_t s, bool tile32 = false)
+{
+    if (!tile32) return launch_getrf2(tile, b, dU, dL, s);
+    if (!g.d_sub || g.sub_cap < (size_t)b * b) return hipErrorInvalidValue;
+    hipError_t e;
+    if ((e = launch_tile_split(g.d_sub, tile, b, b, true, s, true)) != hipSuccess) return e;
+    if ((e = launch_getrf2(g.d_sub, b, dU, dL, s)) != hipSuccess) return e;
+    return launch_tile_split(g.d_sub, tile, b, b, false, s, true);
 }
 
 // events of one factorization: [0, MT+1) panel k ready, [MT+1, 2MT+2) update
@@ -379,8 +398,13 @@ GemmArgs single_tile(const double *A, const double *B, double *C, int b, bool sy
 // waits for F(k-1) only.  The bulk order F(k-1) E(k-1) F(k) E(k) keeps each
 // tile's updates in increasing k, and packed set k % 2 is rewritten by step
 // k+2's packing only after F(k+1), which follows E(k) on the bulk stream.
+//
+// fs (mixed tiers with packed panels): dA holds FP32 tiles (element offsets
+// of the FP64 layout).  Diagonal tiles are factored in FP64 copies, the TRSMs
+// solve in FP64 and round back, the updates read and write FP32 C tiles, and
+// S1 packs tile (k,k+1) and runs on the packed kernel like the bulk.
 void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, hipStream_t sa,
-                        int *dinfo, Launcher &L, int prec)
+                        int *dinfo, Launcher &L, int prec, bool fs = false)
 {
     L.b = b;
     L.prec = prec;
@@ -388,21 +412,33 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
     const bool split = sa != nullptr;
-    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * pack_set_bytes(b, MT);
+    const size_t psb = pack_set_bytes(b, MT);
+    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * psb;
+    if (fs && !(pk && split && v2 && g.pack_cap >= 2 * psb + 2 * (size_t)b * b * 4)) {
+        L.chk(hipErrorInvalidValue);     // the FP32 working copy needs the packed split schedule
+        return;
+    }
     const double esz = prec == 2 ? 2.0 : 4.0;
-    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * pack_set_bytes(b, MT) : 0); };
+    const double sbt = fs ? 4.0 * b * b : tile_bytes(b);      // bytes of one stored tile
+    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * psb : 0); };
+    char *pk1 = (char *)g.d_pack + 2 * psb;                    // S1's packed tile (fs)
+    auto tile = [&](int m, int n) -> double * {
+        return fs ? (double *)(void *)((float *)(void *)dA + tile_off(b, MT, m, n)) : dA + tile_off(b, MT, m, n);
+    };
     auto dinvU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
     auto potrf = [&](int k, hipStream_t s) {
         L.run(KC_DIAG, b3(b) / 3, 2 * tile_bytes(b), s, [&] {
-            return v2 ? potrf_tile(dA + tile_off(b, MT, k, k), b, dinfo, k * b, dinvU(k), s)
+            return v2 ? potrf_tile(tile(k, k), b, dinfo, k * b, dinvU(k), s, fs)
                       : launch_potrf(dA + tile_off(b, MT, k, k), b, dinfo, k * b, s);
         });
     };
     auto trsm = [&](int k, int first, int n, hipStream_t s) {
         TrsmArgs t = targs(dA, N, b, k);
+        t.diag = tile(k, k);
         t.first = first;
         t.ntiles = n;
-        L.run(KC_TRSM, n * b3(b), 3.0 * n * tile_bytes(b), s, [&] {
+        t.f32 = fs ? 1 : 0;
+        L.run(KC_TRSM, n * b3(b), 3.0 * n * sbt, s, [&] {
             return v2 ? launch_trsm2(TR_CHOL, t, dinvU(k), s) : launch_trsm(TR_CHOL, t, s);
         });
     };
@@ -410,21 +446,27 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
         L.run(KC_GEMM, flops, 3.0 * tiles * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
     };
-    // trailing update of step k over tasks [1, count) of `mode`: ngemm GEMM and
-    // nsyrk SYRK tiles, reading npanel tiles of the packed row k
-    auto update = [&](int mode, int k, double ngemm, double nsyrk, double npanel) {
+    auto pack = [&](int k, int n, void *dst, hipStream_t s) {
+        L.run(KC_PACK, 0, n * (sbt + esz * b * b), s,
+              [&] { return launch_pack(dA, N, b, k, k + 1, n, false, prec, dst, s, fs); });
+    };
+    // trailing update of step k over tasks [task0, count) of `mode`: ngemm GEMM
+    // and nsyrk SYRK tiles, reading npanel tiles of the packed row k
+    auto update = [&](int mode, int k, int task0, int ntasks, double ngemm, double nsyrk, double npanel, void *panel,
+                      hipStream_t s) {
         const int T = MT - k - 1;
         GemmArgs a = gargs(mode, dA, N, b, k, T);
-        a.task0 = 1;                     // task 0 is tile (k+1,k+1) (chain) or (k+2,k+2) (F(k))
-        a.ntasks -= 1;
+        a.task0 = task0;
+        a.ntasks = ntasks;
+        a.f32c = fs ? 1 : 0;
         if (a.ntasks <= 0) return;
         const double fl = 2 * b3(b) * ngemm + b3(b) * nsyrk;
         if (pk) {
-            a.pa = a.pb = pset(k);
-            L.run(KC_PKGEMM, fl, 2.0 * (ngemm + nsyrk) * tile_bytes(b) + npanel * esz * b * b, sb,
-                  [&] { return launch_gemm_packed(a, prec, sb); });
+            a.pa = a.pb = panel;
+            L.run(KC_PKGEMM, fl, 2.0 * (ngemm + nsyrk) * sbt + npanel * esz * b * b, s,
+                  [&] { return launch_gemm_packed(a, prec, s); });
         } else {
-            gemm(a, fl, ngemm + nsyrk, sb);
+            gemm(a, fl, ngemm + nsyrk, s);
         }
     };
     hipEvent_t *ev = g.events.data();
@@ -443,9 +485,15 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
             if (k > 0) L.chk(hipStreamWaitEvent(sc, evR[k - 1], 0));
             trsm(k, k + 1, 1, sc);
             L.chk(hipEventRecord(evT[k], sc));
-            gemm(single_tile(dA + tile_off(b, MT, k, k + 1), dA + tile_off(b, MT, k, k + 1),
-                             dA + tile_off(b, MT, k + 1, k + 1), b, true),
-                 b3(b), 1, sc);
+            if (fs) {
+                // GM_CHOL_ROW task 0 is tile (k+1,k+1) from packed tile 0 of row k
+                pack(k, 1, pk1, sc);
+                update(GM_CHOL_ROW, k, 0, 1, 0, 1, 1, pk1, sc);
+            } else {
+                gemm(single_tile(dA + tile_off(b, MT, k, k + 1), dA + tile_off(b, MT, k, k + 1),
+                                 dA + tile_off(b, MT, k + 1, k + 1), b, true),
+                     b3(b), 1, sc);
+            }
             potrf(k + 1, sc);
             L.chk(hipEventRecord(evP[k + 1], sc));
         }
@@ -455,21 +503,21 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
         if (T > 1 && pk) {
             // row k rounded once; the update streams the packed copy
             L.chk(hipStreamWaitEvent(st, evT[k], 0));
-            L.run(KC_PACK, 0, T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5)), st,
-                  [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, pset(k), st); });
+            pack(k, T, pset(k), st);
         }
         if (split) {
             L.chk(hipEventRecord(evB[k], sa));
             L.chk(hipStreamWaitEvent(sb, evB[k], 0));
         }
         if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
+        // task 0 of each enumeration is tile (k+1,k+1) (the chain's) or (k+2,k+2) (F(k)'s)
         if (!split) {
-            if (T > 1) update(GM_CHOL_TRAIL, k, (double)T * (T - 1) / 2, T - 1, T);
+            if (T > 1) update(GM_CHOL_TRAIL, k, 1, T * (T + 1) / 2 - 1, (double)T * (T - 1) / 2, T - 1, T, pset(k), sb);
             L.chk(hipEventRecord(evR[k], sb));
         } else {
-            if (T > 1) update(GM_CHOL_FIRST, k, T - 1, 1, T);
+            if (T > 1) update(GM_CHOL_FIRST, k, 1, T, T - 1, 1, T, pset(k), sb);
             L.chk(hipEventRecord(evR[k], sb));
-            if (T > 2) update(GM_CHOL_REST, k, (double)(T - 1) * (T - 2) / 2, T - 2, T - 1);
+            if (T > 2) update(GM_CHOL_REST, k, 1, (T - 1) * T / 2 - 1, (double)(T - 1) * (T - 2) / 2, T - 2, T - 1, pset(k), sb);
         }
     }
     L.chk(hipEventRecord(evEnd, sb));
@@ -480,9 +528,10 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
 // the chain solves tiles (k,k+1) and (k+1,k) and updates (k+1,k+1); the bulk
 // solves the rest of row and column k and updates every other trailing tile.
 // Split update as for the Cholesky: F(k) is tile row and column k+1 and tile
-// (k+2,k+2), E(k) the tiles (m,n), m,n >= k+2, but (k+2,k+2).
+// (k+2,k+2), E(k) the tiles (m,n), m,n >= k+2, but (k+2,k+2).  fs as for the
+// Cholesky.
 void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipStream_t sb, hipStream_t sa,
-                  Launcher &L, int prec)
+                  Launcher &L, int prec, bool fs = false)
 {
     L.b = b;
     L.prec = prec;
@@ -490,22 +539,33 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
     const bool v2 = panel2_ok(b);
     const bool lp = prec > 0 && b % 128 == 0;
     const bool split = sa != nullptr;
-    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * pack_set_bytes(b, MT);
+    const size_t psb = pack_set_bytes(b, MT);
+    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * psb;
+    if (fs && !(pk && split && v2 && g.pack_cap >= 2 * psb + 2 * (size_t)b * b * 4)) {
+        L.chk(hipErrorInvalidValue);
+        return;
+    }
     const double esz = prec == 2 ? 2.0 : 4.0;
-    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * pack_set_bytes(b, MT) : 0); };
+    const double sbt = fs ? 4.0 * b * b : tile_bytes(b);
+    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * psb : 0); };
+    char *pk1U = (char *)g.d_pack + 2 * psb, *pk1L = pk1U + (size_t)b * b * 4;   // S1's packed tiles (fs)
+    auto tile = [&](int m, int n) -> double * {
+        return fs ? (double *)(void *)((float *)(void *)dA + tile_off(b, MT, m, n)) : dA + tile_off(b, MT, m, n);
+    };
     auto dU = [&](int k) { return g.d_dinv + (size_t)k * 64 * b; };
     auto dL = [&](int k) { return g.d_dinv + (size_t)k * 64 * b + (size_t)32 * b; };
     auto getrf = [&](int k, hipStream_t s) {
         L.run(KC_DIAG, 2 * b3(b) / 3, 2 * tile_bytes(b), s, [&] {
-            return v2 ? launch_getrf2(dA + tile_off(b, MT, k, k), b, dU(k), dL(k), s)
-                      : launch_getrf(dA + tile_off(b, MT, k, k), b, s);
+            return v2 ? getrf_tile(tile(k, k), b, dU(k), dL(k), s, fs) : launch_getrf(dA + tile_off(b, MT, k, k), b, s);
         });
     };
     auto trsm = [&](int kind, int k, int first, int n, hipStream_t s) {
         TrsmArgs t = targs(dA, N, b, k);
+        t.diag = tile(k, k);
         t.first = first;
         t.ntiles = n;
-        L.run(KC_TRSM, n * b3(b), 3.0 * n * tile_bytes(b), s, [&] {
+        t.f32 = fs ? 1 : 0;
+        L.run(KC_TRSM, n * b3(b), 3.0 * n * sbt, s, [&] {
             return v2 ? launch_trsm2(kind, t, kind == TR_LUR ? dU(k) : dL(k), s) : launch_trsm(kind, t, s);
         });
     };
@@ -513,25 +573,33 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
         L.run(KC_GEMM, 2 * b3(b) * tiles, 3.0 * tiles * tile_bytes(b), s,
               [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
     };
-    // trailing update of step k over tasks [1, count) of `mode`, reading npanel
-    // tiles of each packed panel
-    auto update = [&](int mode, int k, double npanel) {
+    // row (U) and column (L, transposed) panels of step k, n tiles from k+1
+    auto pack = [&](int k, int n, char *pU, char *pL, hipStream_t s) {
+        const double pb = n * (sbt + esz * b * b);
+        L.run(KC_PACK, 0, pb, s, [&] { return launch_pack(dA, N, b, k, k + 1, n, false, prec, pU, s, fs); });
+        L.run(KC_PACK, 0, pb, s, [&] { return launch_pack(dA, N, b, k, k + 1, n, true, prec, pL, s, fs); });
+    };
+    // trailing update of step k over tasks [task0, task0 + ntasks) of `mode`,
+    // reading npanel tiles of each packed panel
+    auto update = [&](int mode, int k, int task0, int ntasks, double npanel, char *pU, char *pL, hipStream_t s) {
         const int T = MT - k - 1;
         GemmArgs a = gargs(mode, dA, N, b, k, T);
-        a.task0 = 1;                     // task 0 is tile (k+1,k+1) (chain) or (k+2,k+2) (F(k))
-        a.ntasks -= 1;
+        a.task0 = task0;
+        a.ntasks = ntasks;
+        a.f32c = fs ? 1 : 0;
         if (a.ntasks <= 0) return;
         const double nt = a.ntasks;
         if (pk) {
-            char *pU = pset(k), *pL = pU + (size_t)MT * b * b * 4;
             a.pa = pL;
             a.pb = pU;
-            L.run(KC_PKGEMM, 2 * b3(b) * nt, 2.0 * nt * tile_bytes(b) + 2.0 * npanel * esz * b * b, sb,
-                  [&] { return launch_gemm_packed(a, prec, sb); });
+            L.run(KC_PKGEMM, 2 * b3(b) * nt, 2.0 * nt * sbt + 2.0 * npanel * esz * b * b, s,
+                  [&] { return launch_gemm_packed(a, prec, s); });
         } else {
-            gemm(a, nt, sb);
+            gemm(a, nt, s);
         }
     };
+    auto psetU = [&](int k) { return pset(k); };
+    auto psetL = [&](int k) { return pset(k) + (size_t)MT * b * b * 4; };
     hipEvent_t *ev = g.events.data();
     hipEvent_t *evP = ev, *evR = ev + MT + 1, *evT = ev + 2 * MT + 4, *evB = ev + 3 * MT + 6;
     hipEvent_t fork = ev[2 * MT + 2], evEnd = ev[4 * MT + 7];
@@ -549,9 +617,15 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
             trsm(TR_LUL, k, k + 1, 1, sc);
             trsm(TR_LUR, k, k + 1, 1, sc);
             L.chk(hipEventRecord(evT[k], sc));
-            gemm(single_tile(dA + tile_off(b, MT, k + 1, k), dA + tile_off(b, MT, k, k + 1),
-                             dA + tile_off(b, MT, k + 1, k + 1), b, false),
-                 1, sc);
+            if (fs) {
+                // GM_LU_PANEL task 0 is tile (k+1,k+1) from packed tiles 0 of both panels
+                pack(k, 1, pk1U, pk1L, sc);
+                update(GM_LU_PANEL, k, 0, 1, 1, pk1U, pk1L, sc);
+            } else {
+                gemm(single_tile(dA + tile_off(b, MT, k + 1, k), dA + tile_off(b, MT, k, k + 1),
+                                 dA + tile_off(b, MT, k + 1, k + 1), b, false),
+                     1, sc);
+            }
             getrf(k + 1, sc);
             L.chk(hipEventRecord(evP[k + 1], sc));
         }
@@ -563,23 +637,21 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
         }
         if (T > 1 && pk) {
             L.chk(hipStreamWaitEvent(st, evT[k], 0));
-            char *pU = pset(k), *pL = pU + (size_t)MT * b * b * 4;
-            const double pb = T * tile_bytes(b) * (1.0 + (prec == 2 ? 0.25 : 0.5));
-            L.run(KC_PACK, 0, pb, st, [&] { return launch_pack(dA, N, b, k, k + 1, T, false, prec, pU, st); });
-            L.run(KC_PACK, 0, pb, st, [&] { return launch_pack(dA, N, b, k, k + 1, T, true, prec, pL, st); });
+            pack(k, T, psetU(k), psetL(k), st);
         }
         if (split) {
             L.chk(hipEventRecord(evB[k], sa));
             L.chk(hipStreamWaitEvent(sb, evB[k], 0));
         }
         if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
+        // task 0 of each enumeration is tile (k+1,k+1) (the chain's) or (k+2,k+2) (F(k)'s)
         if (!split) {
-            if (T > 1) update(GM_LU_TRAIL, k, T);
+            if (T > 1) update(GM_LU_TRAIL, k, 1, T * T - 1, T, psetU(k), psetL(k), sb);
             L.chk(hipEventRecord(evR[k], sb));
         } else {
-            if (T > 1) update(GM_LU_FIRST, k, T);
+            if (T > 1) update(GM_LU_FIRST, k, 1, 2 * T - 1, T, psetU(k), psetL(k), sb);
             L.chk(hipEventRecord(evR[k], sb));
-            if (T > 2) update(GM_LU_REST, k, T - 1);
+            if (T > 2) update(GM_LU_REST, k, 1, (T - 1) * (T - 1) - 1, T - 1, psetU(k), psetL(k), sb);
         }
     }
     L.chk(hipEventRecord(evEnd, sb));
@@ -590,8 +662,12 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
 // k+1 runs on the panel stream ahead of the rest, so POTRF/TRSM of step k+1
 // overlap the bulk of step k's trailing update.
 void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, int *dinfo,
-                  Launcher &L, int prec = 0)
+                  Launcher &L, int prec = 0, bool fs = false)
 {
+    if (fs && !(lookahead && g.dag == 2)) {
+        L.chk(hipErrorInvalidValue);     // the FP32 working copy runs on the split chain schedule only
+        return;
+    }
     L.b = b;
     L.prec = prec;
     const int MT = N / b;
@@ -622,7 +698,7 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
               [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
     };
     if (lookahead && g.dag >= 1) {
-        dag_cholesky_chain(dA, N, b, s0, s1, g.s_bulk, g.dag == 2 ? g.s_aux : nullptr, dinfo, L, prec);
+        dag_cholesky_chain(dA, N, b, s0, s1, g.s_bulk, g.dag == 2 ? g.s_aux : nullptr, dinfo, L, prec, fs);
         return;
     }
     if (!lookahead) {
@@ -657,8 +733,13 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
 }
 
 // LU without pivoting, src/lu.h:14-173 (with correct tile dependences).
-void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, Launcher &L, int prec = 0)
+void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_t s1, Launcher &L, int prec = 0,
+            bool fs = false)
 {
+    if (fs && !(lookahead && g.dag == 2)) {
+        L.chk(hipErrorInvalidValue);
+        return;
+    }
     L.b = b;
     L.prec = prec;
     const int MT = N / b;
@@ -686,7 +767,7 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
               [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
     };
     if (lookahead && g.dag >= 1) {
-        dag_lu_chain(dA, N, b, s0, s1, g.s_bulk, g.dag == 2 ? g.s_aux : nullptr, L, prec);
+        dag_lu_chain(dA, N, b, s0, s1, g.s_bulk, g.dag == 2 ? g.s_aux : nullptr, L, prec, fs);
         return;
     }
     if (!lookahead) {
@@ -826,11 +907,16 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     hipStream_t s = g.s_main;
     if ((rc = ensure_events(events_needed(MT))) || (rc = ensure_dinv(b, MT))) return rc;
     if (o.precision > 0 && b % 128 == 0 && (rc = ensure_pack(b, MT))) return rc;
-    if ((rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
-    if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
     // mixed tiers: the correction solves stream FP32 copies (half the bytes);
     // the factor itself carries BF16/FP32-level errors, so this costs nothing
     const bool f32 = o.precision > 0;
+    // ... and on the split chain schedule with packed panels the factorization
+    // itself runs on the FP32 copy (trailing updates move 8 instead of 16 bytes
+    // per element; diagonal tiles are factored in FP64 copies)
+    const bool fs = f32 && g.f32work && b % 128 == 0 && MT > 1 && g.dag == 2;
+    if ((!fs || b != bA) && (rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
+    if (fs && (rc = grow(&g.d_sub, &g.sub_cap, (size_t)b * b))) return rc;
+    if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
     if (f32 && (rc = grow(&g.d_F32, &g.F32_cap, (nn + (size_t)(algo == 0 ? 1 : 2) * MT * b * b) / 2 + 2))) return rc;
     float *F32 = (float *)g.d_F32, *Inv32 = F32 + nn;
     const size_t npart = residual_scratch(N, bA, nrhs);
@@ -863,18 +949,26 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     Launcher L;
     HIPCHK(hipEventRecord(e0, s));
     // factorization of a copy (A stays FP64 and intact for the residuals)
-    if (b == bA) L.chk(launch_copy(g.d_F, dA, nn * 8, s));
-    else L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
+    double *Fw = fs ? (double *)(void *)F32 : g.d_F;      // the matrix the DAG factors
+    if (fs) {
+        // FP32 copy (Cholesky: upper tiles only, the ones the factorization reads)
+        if (b != bA) L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
+        L.chk(launch_to_f32(b == bA ? dA : g.d_F, F32, N, b, algo == 0, 0, s));
+    } else if (b == bA) {
+        L.chk(launch_copy(g.d_F, dA, nn * 8, s));
+    } else {
+        L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
+    }
     hipMemsetAsync(g.d_info, 0, sizeof(int), s);
-    if (algo == 0) dag_cholesky(g.d_F, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision);
-    else dag_lu(g.d_F, N, b, MT > 1, s, g.s_panel, L, o.precision);
+    if (algo == 0) dag_cholesky(Fw, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision, fs);
+    else dag_lu(Fw, N, b, MT > 1, s, g.s_panel, L, o.precision, fs);
     // inverses of the diagonal tiles: one batched strip TRSM per factor with the
     // identity as right-hand side (Cholesky: U^-T; LU: U^-1 and L^-1)
     auto inverses = [&](int kind, double *out, const double *dbase) {
         TrsmArgs t;
         memset(&t, 0, sizeof t);
-        t.mat = g.d_F; t.single = out; t.N = N; t.b = b; t.MT = MT; t.ntiles = MT;
-        t.batch = 1; t.ident = 1; t.dinv_stride = 64L * b;
+        t.mat = Fw; t.single = out; t.N = N; t.b = b; t.MT = MT; t.ntiles = MT;
+        t.batch = 1; t.ident = 1; t.dinv_stride = 64L * b; t.f32 = fs ? 1 : 0;
         L.chk(launch_trsm2(kind, t, dbase, s));
     };
     if (algo == 0) {
@@ -884,7 +978,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         inverses(TR_LUL, Linv, g.d_dinv + (size_t)32 * b);
     }
     if (f32) {
-        L.chk(launch_to_f32(g.d_F, F32, N, b, algo == 0, 0, s));
+        if (!fs) L.chk(launch_to_f32(g.d_F, F32, N, b, algo == 0, 0, s));
         L.chk(launch_to_f32(Uinv, Inv32, 0, b, false, (size_t)(algo == 0 ? 1 : 2) * MT * b * b, s));
     }
     const void *sF = f32 ? (const void *)F32 : (const void *)g.d_F;
@@ -1298,6 +1392,9 @@ int palial_hip_init(int device)
     // PALIAL_SUBTILE=0 factors diagonal tiles with the one-workgroup kernel only
     const char *sub = getenv("PALIAL_SUBTILE");
     g.subtile = sub ? atoi(sub) : 1;
+    // PALIAL_F32WORK=0 keeps the FP64 working copy in the mixed-tier solves
+    const char *fw = getenv("PALIAL_F32WORK");
+    g.f32work = fw ? atoi(fw) : 1;
     HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
     if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
@@ -1417,19 +1514,22 @@ int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double
     const int N = A.matrix_size, b = A.tile_size, MT = N / b, T = MT - k - 1;
     if (algo != ALG_CHOL && algo != ALG_LU) return fail(PALIAL_ERR_ARG, "algo %d has no trailing update", algo);
     if (k < 0 || T <= 0 || reps <= 0) return fail(PALIAL_ERR_ARG, "bad step %d or reps %d", k, reps);
-    if (prec < 0 || prec > 2) return fail(PALIAL_ERR_ARG, "precision %d not available", prec);
+    const bool w32 = (prec & 8) != 0;    // FP32 tiles (the first N*N*4 bytes of the buffer)
+    prec &= 7;
+    if (prec < 0 || prec > 2 || (w32 && prec == 0)) return fail(PALIAL_ERR_ARG, "precision %d not available", prec);
     if (prec > 0 && (b % 128 != 0)) return fail(PALIAL_ERR_ARG, "packed updates need b %% 128 == 0 (b=%d)", b);
     if (prec > 0 && (rc = ensure_pack(b, MT))) return rc;
     double *dA = (double *)A.matrix;
     GemmArgs a = gargs(algo == ALG_CHOL ? GM_CHOL_TRAIL : GM_LU_TRAIL, dA, N, b, k, T);
+    a.f32c = w32 ? 1 : 0;
     char *pU = (char *)g.d_pack, *pL = pU + (size_t)MT * b * b * 4;
     a.pa = algo == ALG_CHOL ? (const void *)pU : (const void *)pL;
     a.pb = pU;
     hipStream_t s = g.s_main;
     auto once = [&]() -> hipError_t {
         if (prec == 0) return launch_gemm(a, algo == ALG_CHOL, false, s);
-        hipError_t e = launch_pack(dA, N, b, k, k + 1, T, false, prec, pU, s);
-        if (e == hipSuccess && algo == ALG_LU) e = launch_pack(dA, N, b, k, k + 1, T, true, prec, pL, s);
+        hipError_t e = launch_pack(dA, N, b, k, k + 1, T, false, prec, pU, s, w32);
+        if (e == hipSuccess && algo == ALG_LU) e = launch_pack(dA, N, b, k, k + 1, T, true, prec, pL, s, w32);
         if (e == hipSuccess) e = launch_gemm_packed(a, prec, s);
         return e;
     };

@@ -61,6 +61,7 @@ struct GemmArgs {
     double alpha, beta;
     int maskA;
     int single_diag;     // GM_SINGLE: output is a SYRK tile (upper triangle only)
+    int f32c;            // packed updates: C and D are FP32 tiles at the same element offsets from mat
 };
 
 // host launchers (k_gemm.hip)
@@ -90,6 +91,7 @@ struct TrsmArgs {
     // ident: the right-hand side is the identity (computes the tile inverse)
     int batch, ident;
     long dinv_stride;
+    int f32;             // trsm2: the tiles of mat and diag are FP32 (single stays FP64)
 };
 hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s);
 
@@ -137,8 +139,9 @@ hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStre
 // Panel tiles of step k rounded to `prec` (1 FP32, 2 BF16), K-contiguous:
 // row-panel tiles (k, first..first+n-1) keep their layout, column-panel tiles
 // (first.., k) are transposed; tile j lands at dst + (j-k-1)*b*b elements.
+// src32: mat holds FP32 tiles (the FP32 working copy; same element offsets)
 hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n, bool colpanel, int prec,
-                       void *dst, hipStream_t s);
+                       void *dst, hipStream_t s, bool src32 = false);
 // Trailing update from packed operands (a.pa / a.pb); TA/TB are implied.
 hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s);
 hipError_t launch_gemm_lp(const GemmArgs &a, bool transA, bool transB, int prec, hipStream_t s);
@@ -178,7 +181,8 @@ hipError_t launch_fill_identity_ipiv(int *ipiv, int N, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N, hipStream_t s);
 // b x b column-major tile <-> its bs-sub-tiles in layout L (to_sub: tile -> sub)
-hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s);
+hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s,
+                             bool tile32 = false);   // tile32: the tile is FP32 (sub stays FP64)
 hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2, hipStream_t s);
 
 }  // namespace palial

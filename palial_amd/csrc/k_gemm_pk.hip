@@ -34,14 +34,33 @@ __device__ inline unsigned short to_bf16(double d)
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// Output storage: FP64 tiles, or (F32C) FP32 tiles at the same element
+// offsets from the matrix base (the FP32 working copy of the mixed-precision
+// solves).  Task pointers are formed on the FP64 view of the base, so the
+// element offset of a pointer is its distance from p.mat.
+template <bool F32C> struct Store {
+    const double *base;
+    __device__ double ld(const double *ptr, size_t i) const
+    {
+        if (F32C) return (double)((const float *)(const void *)base)[(size_t)(ptr - base) + i];
+        return ptr[i];
+    }
+    __device__ void st(double *ptr, size_t i, double v) const
+    {
+        if (F32C) ((float *)(void *)const_cast<double *>(base))[(size_t)(ptr - base) + i] = (float)v;
+        else ptr[i] = v;
+    }
+};
+
 // C = alpha*acc + beta*D with acc[x][c][v] = C(i0 + wm*WM + 16x + l%16,
 // j0 + wn*WM + 16c + 4(l/16) + v).  D (C itself in the trailing updates) is
 // read for a whole fragment row before any store, so those loads are in
 // flight together instead of one dependent round trip each.
-template <int FM, int WM>
+template <int FM, int WM, bool F32C>
 __device__ inline void epilogue(const TaskPtrs &tp, const f4 (&acc)[FM][FM], int i0, int j0, int wm, int wn,
-                                int lane, int b, double alpha, double beta)
+                                int lane, int b, double alpha, double beta, const double *base)
 {
+    const Store<F32C> S{base};
 #pragma unroll
     for (int x = 0; x < FM; x++) {
         const int i = i0 + wm * WM + x * 16 + (lane & 15);
@@ -51,7 +70,7 @@ __device__ inline void epilogue(const TaskPtrs &tp, const f4 (&acc)[FM][FM], int
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int j = j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v;
-                dv[c][v] = (tp.D && !(tp.diag && i > j)) ? tp.D[(size_t)i + (size_t)j * b] : 0.0;
+                dv[c][v] = (tp.D && !(tp.diag && i > j)) ? S.ld(tp.D, (size_t)i + (size_t)j * b) : 0.0;
             }
 #pragma unroll
         for (int c = 0; c < FM; c++)
@@ -60,14 +79,14 @@ __device__ inline void epilogue(const TaskPtrs &tp, const f4 (&acc)[FM][FM], int
                 const int j = j0 + wn * WM + c * 16 + 4 * (lane >> 4) + v;
                 if (tp.diag && i > j) continue;
                 const double r = alpha * (double)acc[x][c][v];
-                tp.C[(size_t)i + (size_t)j * b] = tp.D ? r + beta * dv[c][v] : r;
+                S.st(tp.C, (size_t)i + (size_t)j * b, tp.D ? r + beta * dv[c][v] : r);
             }
     }
 }
 
 // One workgroup per 64 x 64 sub-block of one panel tile.
-template <int PREC>
-__global__ __launch_bounds__(256) void pack_kernel(const double *mat, int b, int MT, int k, int first, int colp,
+template <int PREC, typename ST>
+__global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT, int k, int first, int colp,
                                                    void *dst)
 {
     __shared__ float sT[64][65];
@@ -75,7 +94,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const double *mat, int b, int
     const int tix = blockIdx.x / per, sbk = blockIdx.x % per;
     const int r0 = (sbk % nsb) * 64, c0 = (sbk / nsb) * 64;
     const int idx = first + tix;
-    const double *src = colp ? mat + tile_off(b, MT, idx, k) : mat + tile_off(b, MT, k, idx);
+    const ST *src = colp ? mat + tile_off(b, MT, idx, k) : mat + tile_off(b, MT, k, idx);
     const size_t dbase = (size_t)(idx - k - 1) * b * b;
     const int t = threadIdx.x;
     if (!colp) {
@@ -85,7 +104,15 @@ __global__ __launch_bounds__(256) void pack_kernel(const double *mat, int b, int
             const int e = t + pass * 256;
             const int r = (e % 16) * 4, c = e / 16;
             const size_t o = (size_t)(r0 + r) + (size_t)(c0 + c) * b;
-            const double2 x = *(const double2 *)(src + o), y = *(const double2 *)(src + o + 2);
+            double2 x, y;
+            if constexpr (sizeof(ST) == 4) {
+                const float4 f = *(const float4 *)(src + o);
+                x = make_double2(f.x, f.y);
+                y = make_double2(f.z, f.w);
+            } else {
+                x = *(const double2 *)(src + o);
+                y = *(const double2 *)(src + o + 2);
+            }
             if (PREC == 2) {
                 ushort4 v;
                 v.x = to_bf16(x.x); v.y = to_bf16(x.y); v.z = to_bf16(y.x); v.w = to_bf16(y.y);
@@ -257,9 +284,10 @@ __device__ inline void wait_vm(int n)
     else __builtin_amdgcn_s_waitcnt(0x0F70);
 }
 
-template <int MINB>
+template <int MINB, bool F32C>
 __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 {
+    const Store<F32C> S{p.mat};
     constexpr int BM = 128, KB = 32, WM = 64, FM = 4, NST = 4;
     constexpr int SLICE = BM * KB * 2;                  // bytes of one operand slice
     __shared__ __attribute__((aligned(1024))) char lds[NST][2][SLICE];
@@ -327,6 +355,56 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
         asm volatile("" ::: "memory");
     }
 
+    if constexpr (F32C) {
+        // FP32 tiles: 16 lanes of one accumulator column would cover only 64
+        // bytes, so each wave stages half of its 64 x 64 block (32 rows) in
+        // LDS, column by column, and moves it as float4 runs: 8 lanes write one
+        // full 128-byte line (32 rows of a column), one instruction 8 columns.
+        // Lower-triangle elements of a diagonal (SYRK) tile keep their value.
+        constexpr int LDC = 36;                          // floats per staged column (banks spread)
+        __syncthreads();                                 // all waves are done with the operand stages
+        float *sc = (float *)(void *)&lds[0][0][0] + w * 64 * LDC;
+        float *Cf = (float *)(void *)const_cast<double *>(p.mat) + (size_t)(tp.C - p.mat);
+        const float *Df = tp.D ? (const float *)(const void *)p.mat + (size_t)(tp.D - p.mat) : nullptr;
+        const float alpha = (float)p.alpha, beta = (float)p.beta;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int r0 = i0 + wm * WM + 32 * h + 4 * (lane & 7);      // first of this lane's 4 rows
+            float4 dv[8];
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int col = j0 + wn * WM + 8 * it + (lane >> 3);
+                dv[it] = Df ? *(const float4 *)(Df + (size_t)r0 + (size_t)col * b) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int c = 0; c < FM; c++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        sc[(c * 16 + 4 * (lane >> 4) + v) * LDC + x * 16 + (lane & 15)] = acc[2 * h + x][c][v];
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int cl = 8 * it + (lane >> 3), col = j0 + wn * WM + cl;
+                const float4 a = *(const float4 *)(sc + cl * LDC + 4 * (lane & 7));
+                float4 o;
+                o.x = alpha * a.x + beta * dv[it].x;
+                o.y = alpha * a.y + beta * dv[it].y;
+                o.z = alpha * a.z + beta * dv[it].z;
+                o.w = alpha * a.w + beta * dv[it].w;
+                if (tp.diag) {
+                    if (r0 > col) o.x = dv[it].x;
+                    if (r0 + 1 > col) o.y = dv[it].y;
+                    if (r0 + 2 > col) o.z = dv[it].z;
+                    if (r0 + 3 > col) o.w = dv[it].w;
+                }
+                *(float4 *)(Cf + (size_t)r0 + (size_t)col * b) = o;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     // acc[x][c][v] = C(i0 + wm*64 + 16x + l%16, j0 + wn*64 + 16c + 4(l/16) + v);
     // D rows are loaded one fragment row ahead of the stores
     auto cidx = [&](int x, int c, int v) -> size_t {
@@ -339,7 +417,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 #pragma unroll
         for (int c = 0; c < FM; c++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) dv[c][v] = (tp.D && live(x, c, v)) ? tp.D[cidx(x, c, v)] : 0.0;
+            for (int v = 0; v < 4; v++) dv[c][v] = (tp.D && live(x, c, v)) ? S.ld(tp.D, cidx(x, c, v)) : 0.0;
     };
     const double alpha = p.alpha, beta = p.beta;
     auto stC = [&](int x, const double (&dv)[FM][4]) {
@@ -349,7 +427,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
             for (int v = 0; v < 4; v++) {
                 if (!live(x, c, v)) continue;
                 const double r = alpha * (double)acc[x][c][v];
-                tp.C[cidx(x, c, v)] = tp.D ? r + beta * dv[c][v] : r;
+                S.st(tp.C, cidx(x, c, v), tp.D ? r + beta * dv[c][v] : r);
             }
     };
     double d0[FM][4], d1[FM][4];
@@ -364,6 +442,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 }
 
 // FP32 operands, v_mfma_f32_16x16x4_f32; same blocking with KB = 32.
+template <bool F32C>
 __global__ __launch_bounds__(256, 2) void gemm_f32p_kernel(GemmArgs p)
 {
     constexpr int BM = 128, KB = 32, WM = 64, FM = 4;
@@ -436,20 +515,23 @@ __global__ __launch_bounds__(256, 2) void gemm_f32p_kernel(GemmArgs p)
         buf ^= 1;
     }
 
-    epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
+    epilogue<FM, WM, F32C>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta, p.mat);
 }
 
 }  // namespace
 
 hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n, bool colpanel, int prec,
-                       void *dst, hipStream_t s)
+                       void *dst, hipStream_t s, bool src32)
 {
     if (n <= 0) return hipSuccess;
     if (b % 64 != 0 || first <= k || first + n > N / b || (prec != 1 && prec != 2)) return hipErrorInvalidValue;
-    const int nsb = b / 64;
+    const int nsb = b / 64, cp = colpanel ? 1 : 0;
     dim3 grid((unsigned)(n * nsb * nsb)), block(256);
-    if (prec == 2) hipLaunchKernelGGL(pack_kernel<2>, grid, block, 0, s, mat, b, N / b, k, first, colpanel ? 1 : 0, dst);
-    else hipLaunchKernelGGL(pack_kernel<1>, grid, block, 0, s, mat, b, N / b, k, first, colpanel ? 1 : 0, dst);
+    const float *m32 = (const float *)(const void *)mat;
+    if (prec == 2 && src32) hipLaunchKernelGGL((pack_kernel<2, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
+    else if (prec == 2) hipLaunchKernelGGL((pack_kernel<2, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
+    else if (src32) hipLaunchKernelGGL((pack_kernel<1, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
+    else hipLaunchKernelGGL((pack_kernel<1, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
     return hipGetLastError();
 }
 
@@ -467,13 +549,18 @@ hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
             const char *e = getenv("PALIAL_PK_VARIANT");
             return e ? atoi(e) : 0;
         }();
-        if (var == 0) hipLaunchKernelGGL((gemm_bf16d_kernel<2>), grid, block, 0, s, a);
+        if (a.f32c) {
+            if (var != 0) return hipErrorInvalidValue;       // FP32 output only in the default kernel
+            hipLaunchKernelGGL((gemm_bf16d_kernel<2, true>), grid, block, 0, s, a);
+        }
+        else if (var == 0) hipLaunchKernelGGL((gemm_bf16d_kernel<2, false>), grid, block, 0, s, a);
         else if (var == 1) hipLaunchKernelGGL((gemm_bf16p_kernel<32, 3, 0>), grid, block, 0, s, a);
         else if (var == 2) hipLaunchKernelGGL((gemm_bf16p_kernel<64, 2, 0>), grid, block, 0, s, a);
         else if (var == 3) hipLaunchKernelGGL((gemm_bf16p_kernel<64, 2, 2>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((gemm_bf16p_kernel<32, 2, 2>), grid, block, 0, s, a);   // var 4
     }
-    else if (prec == 1) hipLaunchKernelGGL(gemm_f32p_kernel, grid, block, 0, s, a);
+    else if (prec == 1 && a.f32c) hipLaunchKernelGGL((gemm_f32p_kernel<true>), grid, block, 0, s, a);
+    else if (prec == 1) hipLaunchKernelGGL((gemm_f32p_kernel<false>), grid, block, 0, s, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -153,23 +153,28 @@ hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N
 
 // One b x b column-major tile <-> its (b/bs) x (b/bs) sub-tiles in layout L
 // (sub-tile (i,j) at bs*bs*(i + (b/bs)*j), column-major inside).
-__global__ void tile_split_kernel(double *__restrict__ sub, double *__restrict__ tile, int b, int bs, int to_sub)
+template <typename TT>
+__global__ void tile_split_kernel(double *__restrict__ sub, TT *__restrict__ tile, int b, int bs, int to_sub)
 {
     const int n = b * b, ms = b / bs;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         const int r = e % b, c = e / b;
         const size_t so = tile_off(bs, ms, r / bs, c / bs) + (size_t)(r % bs) + (size_t)(c % bs) * bs;
-        if (to_sub) sub[so] = tile[e];
-        else tile[e] = sub[so];
+        if (to_sub) sub[so] = (double)tile[e];
+        else tile[e] = (TT)sub[so];
     }
 }
 
-hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s)
+hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s, bool tile32)
 {
     if (b <= 0 || bs <= 0 || b % bs != 0) return hipErrorInvalidValue;
     int blocks = (b * b + 1023) / 1024;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(tile_split_kernel, dim3(blocks), dim3(256), 0, s, sub, tile, b, bs, to_sub ? 1 : 0);
+    if (tile32)
+        hipLaunchKernelGGL(tile_split_kernel<float>, dim3(blocks), dim3(256), 0, s, sub, (float *)(void *)tile, b, bs,
+                           to_sub ? 1 : 0);
+    else
+        hipLaunchKernelGGL(tile_split_kernel<double>, dim3(blocks), dim3(256), 0, s, sub, tile, b, bs, to_sub ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@
 //                   updates are MFMA.
 #include "internal.hpp"
 #include <float.h>
+#include <type_traits>
 
 namespace palial {
 
@@ -597,9 +598,12 @@ __global__ __launch_bounds__(PF_THREADS) void batch_kernel(BatchArgs p)
 // fragment load fall on distinct bank halves).
 __device__ inline int xidx(int i, int c) { return i * NB2 + (c ^ ((i & 1) << 4)); }
 
-template <int KIND>
+// F32: the tiles of p.mat and p.diag are FP32 (element offsets as for FP64
+// tiles); the strip is solved in FP64 in LDS and rounded back on the store.
+template <int KIND, bool F32>
 __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const double *dinv)
 {
+    using DT = typename std::conditional<F32, float, double>::type;
     constexpr int NB = NB2, SW = NB2;
     extern __shared__ double sX[];
     const int b = p.b;
@@ -607,18 +611,27 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
     const int tix = blockIdx.x / strips, strip = blockIdx.x % strips;
     const int c0 = strip * SW;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    double *X;
-    const double *D = p.diag;
+    const DT *const mat = (const DT *)(const void *)p.mat;
+    double *X = nullptr;
+    DT *X32 = nullptr;                  // the strip's tile when it lives in an FP32 matrix
+    const DT *D = (const DT *)(const void *)p.diag;
     if (p.batch) {
-        D = p.mat + tile_off(b, p.MT, tix, tix);
+        D = mat + tile_off(b, p.MT, tix, tix);
         X = p.single + (size_t)tix * b * b;
         dinv += (size_t)tix * p.dinv_stride;
     } else if (p.mat) {
         const int idx = p.first + tix;
-        X = (KIND == TR_LUR) ? p.mat + tile_off(b, p.MT, idx, p.k) : p.mat + tile_off(b, p.MT, p.k, idx);
+        const size_t xo = (KIND == TR_LUR) ? tile_off(b, p.MT, idx, p.k) : tile_off(b, p.MT, p.k, idx);
+        if (F32) X32 = (DT *)(void *)p.mat + xo;
+        else X = p.mat + xo;
     } else {
         X = p.single;
     }
+    auto ldx = [&](size_t i) -> double { return (F32 && X32) ? (double)X32[i] : X[i]; };
+    auto stx = [&](size_t i, double v) {
+        if (F32 && X32) X32[i] = (DT)v;
+        else X[i] = v;
+    };
     // M(i,q): CHOL/LUR M = U^T -> D[q + i*b];  LUL M = L -> D[i + q*b]
     for (int e0 = t; e0 < b * SW; e0 += P2_THREADS * 8) {
         double v[8];
@@ -629,7 +642,7 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
                 // element (row, col) of the tile this strip holds
                 const int row = (KIND == TR_LUR) ? c0 + e % SW : e % b;
                 const int col = (KIND == TR_LUR) ? e / SW : c0 + e / b;
-                v[u] = p.ident ? (row == col ? 1.0 : 0.0) : X[(size_t)row + (size_t)col * b];
+                v[u] = p.ident ? (row == col ? 1.0 : 0.0) : ldx((size_t)row + (size_t)col * b);
             }
         }
 #pragma unroll
@@ -679,7 +692,8 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
 #pragma unroll
                 for (int k4 = 0; k4 < NB; k4 += 4) {
                     const int q = k4 + (lane >> 4);
-                    av[m][k4 / 4] = (KIND == TR_LUL) ? D[(size_t)i + (size_t)(r0 + q) * b] : D[(size_t)(r0 + q) + (size_t)i * b];
+                    av[m][k4 / 4] = (KIND == TR_LUL) ? (double)D[(size_t)i + (size_t)(r0 + q) * b]
+                                                     : (double)D[(size_t)(r0 + q) + (size_t)i * b];
                 }
             }
         }
@@ -705,8 +719,8 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
     }
 
     for (int e = t; e < b * SW; e += P2_THREADS) {
-        if (KIND == TR_LUR) X[(size_t)(c0 + e % SW) + (size_t)(e / SW) * b] = sX[xidx(e / SW, e % SW)];
-        else X[(size_t)(e % b) + (size_t)(c0 + e / b) * b] = sX[xidx(e % b, e / b)];
+        if (KIND == TR_LUR) stx((size_t)(c0 + e % SW) + (size_t)(e / SW) * b, sX[xidx(e / SW, e % SW)]);
+        else stx((size_t)(e % b) + (size_t)(c0 + e / b) * b, sX[xidx(e % b, e / b)]);
     }
 }
 
@@ -725,9 +739,12 @@ void panel2_init()
                         (int)panel_lds_bytes(512, 3));
     hipFuncSetAttribute((const void *)batch_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)panel_lds_bytes(512, 3));
-    hipFuncSetAttribute((const void *)trsm2_kernel<TR_CHOL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
-    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
-    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_CHOL, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUL, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUR, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_CHOL, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUL, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUR, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
 }
 
 hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s)
@@ -759,9 +776,15 @@ hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStre
     if (a.ntiles <= 0) return hipSuccess;
     dim3 grid((unsigned)(a.ntiles * (a.b / NB2))), block(P2_THREADS);
     const size_t lds = (size_t)a.b * NB2 * 8;
-    if (kind == TR_CHOL) hipLaunchKernelGGL(trsm2_kernel<TR_CHOL>, grid, block, lds, s, a, dinv);
-    else if (kind == TR_LUL) hipLaunchKernelGGL(trsm2_kernel<TR_LUL>, grid, block, lds, s, a, dinv);
-    else hipLaunchKernelGGL(trsm2_kernel<TR_LUR>, grid, block, lds, s, a, dinv);
+    if (a.f32) {
+        if (kind == TR_CHOL) hipLaunchKernelGGL((trsm2_kernel<TR_CHOL, true>), grid, block, lds, s, a, dinv);
+        else if (kind == TR_LUL) hipLaunchKernelGGL((trsm2_kernel<TR_LUL, true>), grid, block, lds, s, a, dinv);
+        else hipLaunchKernelGGL((trsm2_kernel<TR_LUR, true>), grid, block, lds, s, a, dinv);
+    } else {
+        if (kind == TR_CHOL) hipLaunchKernelGGL((trsm2_kernel<TR_CHOL, false>), grid, block, lds, s, a, dinv);
+        else if (kind == TR_LUL) hipLaunchKernelGGL((trsm2_kernel<TR_LUL, false>), grid, block, lds, s, a, dinv);
+        else hipLaunchKernelGGL((trsm2_kernel<TR_LUR, false>), grid, block, lds, s, a, dinv);
+    }
     return hipGetLastError();
 }
 

@@ -184,27 +184,30 @@ def test_graph_and_streams_bitwise(hip, orc, algo):
 def test_schedules_bitwise(hip, orc, algo):
     """The split-update chain schedule (default), the one-launch chain schedule,
     the row look-ahead schedule, with the bulk streams CU-masked (no graph) and
-    without a reserve (captured graph), give bitwise identical FP64 factors;
-    the two chain schedules give bitwise identical solutions on the packed BF16
-    tier (double-buffered panels in the split schedule)."""
+    without a reserve (captured graph), give bitwise identical FP64 factors.
+    On the packed BF16 tier the solutions agree bitwise within each working
+    storage: the FP32 working copy (split schedule, either reserve) and the
+    FP64 one (both chain schedules)."""
     import os
     N, b = 1280, 256                       # MT = 5: every step kind, packed panels (b % 128 == 0)
     a = orc.gen("sym" if algo == "cholesky" else "dense", N, b, seed=5, shift=N)
     rhs = np.random.default_rng(2).standard_normal((N, 1))
-    keep = {k: os.environ.get(k) for k in ("PALIAL_DAG", "PALIAL_RESERVE_CUS")}
-    facs, sols = [], []
+    names = ("PALIAL_DAG", "PALIAL_RESERVE_CUS", "PALIAL_F32WORK")
+    keep = {k: os.environ.get(k) for k in names}
+    facs, sols = [], {}
     try:
-        for dag, res in [("2", "16"), ("1", "16"), ("0", "16"), ("2", "0"), ("1", "0")]:
+        for dag, res, fw in [("2", "16", "1"), ("1", "16", "1"), ("0", "16", "1"), ("2", "0", "1"), ("1", "0", "1"),
+                             ("2", "16", "0")]:
             hip.finalize()
-            os.environ["PALIAL_DAG"], os.environ["PALIAL_RESERVE_CUS"] = dag, res
+            os.environ.update(dict(zip(names, (dag, res, fw))))
             hip.init(0)
             x = a.copy()
             getattr(hip, algo)(x, b=b)
             facs.append(x)
             if dag != "0":                 # the row look-ahead uses the unpacked BF16 kernel
                 sol, rep, rc = hip.solve(a, rhs, algo=algo, precision=hip.PREC_BF16, b=b, factor_tile=b)
-                assert rc == 0, (dag, res)
-                sols.append(sol)
+                assert rc == 0 and rep.backward_error <= 1e-14, (dag, res, fw)
+                sols.setdefault(dag == "2" and fw == "1", []).append(sol)
     finally:
         hip.finalize()
         for k, v in keep.items():
@@ -215,5 +218,7 @@ def test_schedules_bitwise(hip, orc, algo):
         hip.init(0)
     for i in range(1, len(facs)):
         assert np.array_equal(facs[i].view(np.uint64), facs[0].view(np.uint64)), i
-    for i in range(1, len(sols)):
-        assert np.array_equal(sols[i].view(np.uint64), sols[0].view(np.uint64)), i
+    assert len(sols[True]) == 2 and len(sols[False]) == 3
+    for group in sols.values():
+        for i in range(1, len(group)):
+            assert np.array_equal(group[i].view(np.uint64), group[0].view(np.uint64)), i

@@ -69,3 +69,42 @@ def test_trailing_update(hip, orc, algo, prec, N, b, k):
     assert np.array_equal(out[s1:, s1:][~mask], A[s1:, s1:][~mask])
     bound = TOL[prec] * b * scale[mask] + 1e-300
     assert (err[s1:, s1:][mask] <= bound).all(), float((err[s1:, s1:][mask] / bound).max())
+
+
+@pytest.mark.parametrize("algo", ["cholesky", "lu"])
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("N,b,k", [(1024, 256, 0), (1536, 512, 1)])
+def test_trailing_update_fp32_storage(hip, orc, algo, prec, N, b, k):
+    """The same update on FP32 tiles (the mixed-precision solves' working copy,
+    hook flag prec | 8): operands rounded from the FP32 values, the result
+    rounded to FP32 once more; untouched elements keep their FP32 bits."""
+    a32 = orc.gen("dense", N, b, seed=21 + k, shift=0.0).astype(np.float32)
+    buf = np.zeros(N * N, dtype=np.float64)
+    buf.view(np.float32)[:N * N] = a32
+    dm = hip.DeviceMatrix(N, b)
+    try:
+        dm.from_host(buf)
+        hip.update(algo, dm, k, precision=prec | 8)
+        out32 = dm.to_host().view(np.float32)[:N * N].copy()
+    finally:
+        dm.free()
+    A = orc.to_dense(a32.astype(np.float64), N, b)
+    out = orc.to_dense(out32.astype(np.float64), N, b)
+    r = ROUND[prec]
+    s0, s1 = k * b, (k + 1) * b
+    if algo == "cholesky":
+        U = r(A[s0:s1, s1:])
+        upd, scale = U.T @ U, np.abs(U).T @ np.abs(U)
+        idx = np.arange(N - s1)
+        ti = idx // b
+        mask = (ti[:, None] < ti[None, :]) | ((ti[:, None] == ti[None, :]) & (idx[:, None] <= idx[None, :]))
+    else:
+        Lc, Ur = r(A[s1:, s0:s1]), r(A[s0:s1, s1:])
+        upd, scale = Lc @ Ur, np.abs(Lc) @ np.abs(Ur)
+        mask = np.ones((N - s1, N - s1), dtype=bool)
+    exp = A[s1:, s1:] - upd
+    assert np.array_equal(out[:s1, :], A[:s1, :]) and np.array_equal(out[:, :s1], A[:, :s1])
+    assert np.array_equal(out[s1:, s1:][~mask], A[s1:, s1:][~mask])
+    err = np.abs(out[s1:, s1:] - exp)[mask]
+    bound = TOL[prec] * b * scale[mask] + 2.0 ** -24 * np.abs(exp[mask]) + 1e-300
+    assert (err <= bound).all(), float((err / bound).max())
