@@ -22,6 +22,7 @@
 #include "internal.hpp"
 #include <float.h>
 #include <type_traits>
+#include <cstdlib>
 
 namespace palial {
 
@@ -656,31 +657,25 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
     }
     __syncthreads();
 
-    for (int r0 = 0; r0 < b; r0 += NB) {
+    // fragments of the inverse of diagonal block r0/NB for phase (a), loaded
+    // one block ahead (waves 0..3)
+    const int ti = w & 1, tc = w >> 1;
+    double mi[NB / 4];
+    auto load_mi = [&](int r0) {
         const double *Mi = dinv + (size_t)(r0 / NB) * NB * NB;   // inverse of the diagonal block
-        // (a) X_r <- M_rr^{-1} X_r : waves 0..3 own one 16x16 output tile each
-        d4 acc = zero4();
-        const int ti = w & 1, tc = w >> 1;
-        if (w < 4) {
 #pragma unroll
-            for (int k4 = 0; k4 < NB; k4 += 4) {
-                const int q = k4 + (lane >> 4);
-                const int i = ti * 16 + (lane & 15);
-                // CHOL/LUR: M_rr^{-1} = (U_rr^{-1})^T -> Uinv(q,i);  LUL: Linv(i,q)
-                const double av = (KIND == TR_LUL) ? Mi[i + q * NB] : Mi[q + i * NB];
-                acc = mfma16(av, sX[xidx(r0 + q, tc * 16 + (lane & 15))], acc);
-            }
+        for (int k4 = 0; k4 < NB; k4 += 4) {
+            const int q = k4 + (lane >> 4);
+            const int i = ti * 16 + (lane & 15);
+            // CHOL/LUR: M_rr^{-1} = (U_rr^{-1})^T -> Uinv(q,i);  LUL: Linv(i,q)
+            mi[k4 / 4] = (KIND == TR_LUL) ? Mi[i + q * NB] : Mi[q + i * NB];
         }
-        __syncthreads();
-        if (w < 4) {
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                sX[xidx(r0 + ti * 16 + (lane >> 4) + 4 * v, tc * 16 + (lane & 15))] = acc[v];
-        }
-        __syncthreads();
-        // (b) rows below: X[i] -= M(i, r-block) X_r.  A wave owns row tiles
-        //     it = w, w+8, ... (both 16-column halves); the M fragments of all
-        //     its row tiles are loaded before any MFMA.
+    };
+    if (w < 4) load_mi(0);
+    for (int r0 = 0; r0 < b; r0 += NB) {
+        // the M fragments of phase (b) do not depend on the solve: their loads
+        // are issued first and land during phase (a) and its barriers.  A wave
+        // owns row tiles it = w, w+8, ... below the block (both 16-column halves).
         const int below = (b - r0 - NB) / 16;
         constexpr int MAXT = 4;                 // row tiles per wave for b <= 512
         double av[MAXT][NB / 4];
@@ -697,6 +692,24 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
                 }
             }
         }
+        // (a) X_r <- M_rr^{-1} X_r : waves 0..3 own one 16x16 output tile each
+        d4 acc = zero4();
+        if (w < 4) {
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+                acc = mfma16(mi[k4 / 4], sX[xidx(r0 + q, tc * 16 + (lane & 15))], acc);
+            }
+            if (r0 + NB < b) load_mi(r0 + NB);
+        }
+        __syncthreads();
+        if (w < 4) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                sX[xidx(r0 + ti * 16 + (lane >> 4) + 4 * v, tc * 16 + (lane & 15))] = acc[v];
+        }
+        __syncthreads();
+        // (b) rows below: X[i] -= M(i, r-block) X_r
 #pragma unroll
         for (int m = 0; m < MAXT; m++) {
             const int it = w + m * P2_WAVES;
@@ -724,6 +737,143 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
     }
 }
 
+// FP32 variant for the mixed-precision tiers' FP32 working copy: the strip
+// is solved in FP32 in LDS (half the LDS, two workgroups per CU) with
+// v_mfma_f32_16x16x4_f32 (twice the FP64 rate).  The block inverses stay FP64
+// in memory and are rounded on load; D and the matrix tiles are FP32.  Same
+// blocking as trsm2_kernel; the f32 MFMA result D[4(l/16)+v][l%16] differs
+// from the f64 one only in its row mapping.  Batch (inverse) mode writes FP64.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ inline f4v mfma16f(float a, float b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(P2_THREADS, 2) void trsm2f_kernel(TrsmArgs p, const double *dinv)
+{
+    constexpr int NB = NB2, SW = NB2;
+    extern __shared__ double sX_raw[];
+    float *sX = (float *)sX_raw;
+    const int b = p.b;
+    const int strips = b / SW;
+    const int tix = blockIdx.x / strips, strip = blockIdx.x % strips;
+    const int c0 = strip * SW;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const float *const mat = (const float *)(const void *)p.mat;
+    double *Xd = nullptr;               // FP64 output (batch mode / single tile)
+    float *X32 = nullptr;               // the strip's tile in the FP32 matrix
+    const float *D = (const float *)(const void *)p.diag;
+    if (p.batch) {
+        D = mat + tile_off(b, p.MT, tix, tix);
+        Xd = p.single + (size_t)tix * b * b;
+        dinv += (size_t)tix * p.dinv_stride;
+    } else if (p.mat) {
+        const int idx = p.first + tix;
+        X32 = (float *)(void *)p.mat + ((KIND == TR_LUR) ? tile_off(b, p.MT, idx, p.k) : tile_off(b, p.MT, p.k, idx));
+    } else {
+        Xd = p.single;
+    }
+    for (int e0 = t; e0 < b * SW; e0 += P2_THREADS * 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * P2_THREADS;
+            if (e < b * SW) {
+                const int row = (KIND == TR_LUR) ? c0 + e % SW : e % b;
+                const int col = (KIND == TR_LUR) ? e / SW : c0 + e / b;
+                const size_t o = (size_t)row + (size_t)col * b;
+                v[u] = p.ident ? (row == col ? 1.f : 0.f) : (X32 ? X32[o] : (float)Xd[o]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + u * P2_THREADS;
+            if (e < b * SW) {
+                if (KIND == TR_LUR) sX[xidx(e / SW, e % SW)] = v[u];
+                else sX[xidx(e % b, e / b)] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+
+    const int ti = w & 1, tc = w >> 1;
+    float mi[NB / 4];
+    auto load_mi = [&](int r0) {
+        const double *Mi = dinv + (size_t)(r0 / NB) * NB * NB;
+#pragma unroll
+        for (int k4 = 0; k4 < NB; k4 += 4) {
+            const int q = k4 + (lane >> 4);
+            const int i = ti * 16 + (lane & 15);
+            mi[k4 / 4] = (float)((KIND == TR_LUL) ? Mi[i + q * NB] : Mi[q + i * NB]);
+        }
+    };
+    if (w < 4) load_mi(0);
+    for (int r0 = 0; r0 < b; r0 += NB) {
+        const int below = (b - r0 - NB) / 16;
+        constexpr int MAXT = 4;
+        float av[MAXT][NB / 4];
+#pragma unroll
+        for (int m = 0; m < MAXT; m++) {
+            const int it = w + m * P2_WAVES;
+            if (it < below) {
+                const int i = r0 + NB + it * 16 + (lane & 15);
+#pragma unroll
+                for (int k4 = 0; k4 < NB; k4 += 4) {
+                    const int q = k4 + (lane >> 4);
+                    av[m][k4 / 4] = (KIND == TR_LUL) ? D[(size_t)i + (size_t)(r0 + q) * b] : D[(size_t)(r0 + q) + (size_t)i * b];
+                }
+            }
+        }
+        // (a) X_r <- M_rr^{-1} X_r : waves 0..3 own one 16x16 output tile each
+        f4v acc = (f4v){0.f, 0.f, 0.f, 0.f};
+        if (w < 4) {
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+                acc = mfma16f(mi[k4 / 4], sX[xidx(r0 + q, tc * 16 + (lane & 15))], acc);
+            }
+            if (r0 + NB < b) load_mi(r0 + NB);
+        }
+        __syncthreads();
+        if (w < 4) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                sX[xidx(r0 + ti * 16 + 4 * (lane >> 4) + v, tc * 16 + (lane & 15))] = acc[v];
+        }
+        __syncthreads();
+        // (b) rows below: X[i] -= M(i, r-block) X_r
+#pragma unroll
+        for (int m = 0; m < MAXT; m++) {
+            const int it = w + m * P2_WAVES;
+            if (it >= below) continue;
+            const int i0 = r0 + NB + it * 16;
+            f4v u0 = (f4v){0.f, 0.f, 0.f, 0.f}, u1 = u0;
+#pragma unroll
+            for (int k4 = 0; k4 < NB; k4 += 4) {
+                const int q = k4 + (lane >> 4);
+                u0 = mfma16f(av[m][k4 / 4], sX[xidx(r0 + q, (lane & 15))], u0);
+                u1 = mfma16f(av[m][k4 / 4], sX[xidx(r0 + q, 16 + (lane & 15))], u1);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                sX[xidx(i0 + 4 * (lane >> 4) + v, (lane & 15))] -= u0[v];
+                sX[xidx(i0 + 4 * (lane >> 4) + v, 16 + (lane & 15))] -= u1[v];
+            }
+        }
+        __syncthreads();
+    }
+
+    for (int e = t; e < b * SW; e += P2_THREADS) {
+        const size_t o = (KIND == TR_LUR) ? (size_t)(c0 + e % SW) + (size_t)(e / SW) * b
+                                          : (size_t)(e % b) + (size_t)(c0 + e / b) * b;
+        const float v = (KIND == TR_LUR) ? sX[xidx(e / SW, e % SW)] : sX[xidx(e % b, e / b)];
+        if (X32) X32[o] = v;
+        else Xd[o] = (double)v;
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 bool panel2_ok(int b) { return b % NB2 == 0 && b >= NB2 && b <= 512; }
 
@@ -745,6 +895,9 @@ void panel2_init()
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_CHOL, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUL, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUR, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
+    hipFuncSetAttribute((const void *)trsm2f_kernel<TR_CHOL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm / 2);
+    hipFuncSetAttribute((const void *)trsm2f_kernel<TR_LUL>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm / 2);
+    hipFuncSetAttribute((const void *)trsm2f_kernel<TR_LUR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm / 2);
 }
 
 hipError_t launch_potrf2(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s)
@@ -777,7 +930,16 @@ hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStre
     dim3 grid((unsigned)(a.ntiles * (a.b / NB2))), block(P2_THREADS);
     const size_t lds = (size_t)a.b * NB2 * 8;
     if (a.f32) {
-        if (kind == TR_CHOL) hipLaunchKernelGGL((trsm2_kernel<TR_CHOL, true>), grid, block, lds, s, a, dinv);
+        // FP32 tiles: solved in FP32 (PALIAL_TRSM_F64=1 keeps FP64 arithmetic)
+        static const bool f64 = [] {
+            const char *e = getenv("PALIAL_TRSM_F64");
+            return e && atoi(e) != 0;
+        }();
+        const size_t lds32 = (size_t)a.b * NB2 * 4;
+        if (!f64 && kind == TR_CHOL) hipLaunchKernelGGL(trsm2f_kernel<TR_CHOL>, grid, block, lds32, s, a, dinv);
+        else if (!f64 && kind == TR_LUL) hipLaunchKernelGGL(trsm2f_kernel<TR_LUL>, grid, block, lds32, s, a, dinv);
+        else if (!f64) hipLaunchKernelGGL(trsm2f_kernel<TR_LUR>, grid, block, lds32, s, a, dinv);
+        else if (kind == TR_CHOL) hipLaunchKernelGGL((trsm2_kernel<TR_CHOL, true>), grid, block, lds, s, a, dinv);
         else if (kind == TR_LUL) hipLaunchKernelGGL((trsm2_kernel<TR_LUL, true>), grid, block, lds, s, a, dinv);
         else hipLaunchKernelGGL((trsm2_kernel<TR_LUR, true>), grid, block, lds, s, a, dinv);
     } else {
