@@ -351,16 +351,36 @@ hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *din
     return launch_tile_split(g.d_sub, tile, b, bs, false, s, tile32);
 }
 
-// LU of a diagonal tile (PALIAL_dgetrf, src/srclu/dgetrf.h:11); tile32 as
-// for potrf_tile (factored in FP64 in the copy).
+// LU of a diagonal tile (PALIAL_dgetrf, src/srclu/dgetrf.h:11) writing the
+// 32x32 block inverses of U (dU) and L (dL); sub-tiles and tile32 as for
+// potrf_tile.  Inside a sub-tile the one-workgroup kernel keeps the
+// reference's small-pivot scaling rule; the rows of the sub-tiles below are
+// solved with the block inverses, as the DAG solves the tiles below a
+// diagonal tile.
 hipError_t getrf_tile(double *tile, int b, double *dU, double *dL, hipStream_t s, bool tile32 = false)
 {
-    if (!tile32) return launch_getrf2(tile, b, dU, dL, s);
-    if (!g.d_sub || g.sub_cap < (size_t)b * b) return hipErrorInvalidValue;
+    const bool sub = g.subtile && b >= 256 && b % 128 == 0;
+    const bool copy = (sub || tile32) && g.d_sub && g.sub_cap >= (size_t)b * b;
+    if (!copy) {
+        if (tile32) return hipErrorInvalidValue;
+        return launch_getrf2(tile, b, dU, dL, s);
+    }
+    const int bs = sub ? 128 : b, ms = b / bs;
     hipError_t e;
-    if ((e = launch_tile_split(g.d_sub, tile, b, b, true, s, true)) != hipSuccess) return e;
-    if ((e = launch_getrf2(g.d_sub, b, dU, dL, s)) != hipSuccess) return e;
-    return launch_tile_split(g.d_sub, tile, b, b, false, s, true);
+    if ((e = launch_tile_split(g.d_sub, tile, b, bs, true, s, tile32)) != hipSuccess) return e;
+    for (int j = 0; j < ms; j++) {
+        double *uj = dU + (size_t)j * bs * 32, *lj = dL + (size_t)j * bs * 32;
+        if ((e = launch_getrf2(g.d_sub + tile_off(bs, ms, j, j), bs, uj, lj, s)) != hipSuccess) return e;
+        if (j + 1 == ms) break;
+        TrsmArgs t = targs(g.d_sub, b, bs, j);
+        t.first = j + 1;
+        t.ntiles = ms - j - 1;
+        if ((e = launch_trsm2(TR_LUL, t, lj, s)) != hipSuccess) return e;    // row j: L_jj X = A_jn
+        if ((e = launch_trsm2(TR_LUR, t, uj, s)) != hipSuccess) return e;    // column j: X U_jj = A_mj
+        if ((e = launch_gemm_fine(gargs(GM_LU_TRAIL, g.d_sub, b, bs, j, ms - j - 1), false, false, s)) != hipSuccess)
+            return e;
+    }
+    return launch_tile_split(g.d_sub, tile, b, bs, false, s, tile32);
 }
 
 // events of one factorization: [0, MT+1) panel k ready, [MT+1, 2MT+2) update
@@ -749,7 +769,7 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
         const int T = MT - k - 1;
         double *dU = g.d_dinv + (size_t)k * 64 * b, *dL = dU + (size_t)32 * b;
         L.run(KC_DIAG, 2 * b3(b) / 3, 2 * tile_bytes(b), s, [&] {
-            return v2 ? launch_getrf2(dA + tile_off(b, MT, k, k), b, dU, dL, s)
+            return v2 ? getrf_tile(dA + tile_off(b, MT, k, k), b, dU, dL, s)
                       : launch_getrf(dA + tile_off(b, MT, k, k), b, s);
         });
         L.run(KC_TRSM, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] {

@@ -173,13 +173,19 @@ __device__ __forceinline__ void potrf2_tile(double *A, int b, int *info, int inf
                     if (!(piv > 0.0)) {
                         bad = j;
                     } else {
-                        const double d = sqrt(piv), rd = 1.0 / d;
-                        x[j] = (c == j) ? d : x[j] * rd;
+                        // 1/sqrt(piv): hardware estimate refined by two Newton steps
+                        double rs = __builtin_amdgcn_rsq(piv);
+                        rs = rs * (1.5 - 0.5 * piv * rs * rs);
+                        rs = rs * (1.5 - 0.5 * piv * rs * rs);
+                        x[j] = (c == j) ? piv * rs : x[j] * rs;
+                        // row j of U into SGPRs first, then the rank-1 update
+                        double u[NB];
 #pragma unroll
-                        for (int r = j + 1; r < NB; r++) x[r] -= readlane_d(x[j], r) * x[j];
+                        for (int r = j + 1; r < NB; r++) u[r] = readlane_d(x[j], r);
+#pragma unroll
+                        for (int r = j + 1; r < NB; r++) x[r] -= u[r] * x[j];
                     }
                 }
-                __builtin_amdgcn_sched_barrier(0);   // keep each step's broadcasts local
             }
             if (lane < NB) {
 #pragma unroll
@@ -336,12 +342,13 @@ __device__ __forceinline__ void getrf2_tile(double *A, int b, double *dinvU, dou
                 slow |= !(fabs(alpha) > DBL_MIN);
                 if (r > i) x[i] = piv_scale2(x[i], alpha);
                 const double l = x[i];
+                // row i of U into SGPRs first, then the rank-1 update
+                double u[NB];
 #pragma unroll
-                for (int c = i + 1; c < NB; c++) {
-                    const double u = readlane_d(x[c], i);
-                    if (r > i) x[c] -= l * u;
-                }
-                __builtin_amdgcn_sched_barrier(0);   // keep each step's broadcasts local
+                for (int c = i + 1; c < NB; c++) u[c] = readlane_d(x[c], i);
+#pragma unroll
+                for (int c = i + 1; c < NB; c++)
+                    if (r > i) x[c] -= l * u[c];
             }
             if (lane < NB) {
 #pragma unroll
