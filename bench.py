@@ -5,16 +5,18 @@ N = 32768, b = 512, one N(0,1) right-hand side, A = Q diag(s) Q^T with s
 geometric from 1 to 1/kappa (kappa = 1e2 by default) and Q = Q1 (x) Q2 from
 Householder QR of seeded Gaussians (a geometric spectrum factorises over the
 Kronecker index, so A = A1 (x) A2 is generated on the device in O(N^2)).
-Each precision tier factors a device copy of A (trailing updates in FP64,
-FP32 or BF16 with FP32 accumulation; panels FP64) and refines in FP64 until
-the backward error reaches 1e-14; the timed region covers copy + factor +
-triangular solves + every refinement step, with A and b resident in HBM.
+Each precision tier factors a device copy of A and refines in FP64 until the
+backward error reaches 1e-14.  The FP64 tier factors an FP64 copy; the mixed
+tiers factor an FP32 copy (trailing updates from BF16 or FP32 packed panels
+with FP32 accumulation, panel TRSMs in FP32, diagonal tiles in FP64).  The
+timed region covers copy + factor + triangular solves + every refinement
+step, with A and b resident in HBM.
 
 Prints ONE JSON line.  ``value`` is the Cholesky solve's effective
 throughput N^3/3 / time-to-solution of the fastest tier that reaches 1e-14
 (TFLOP/s); the FP64 tier and the LU solves are reported beside it.
 ``roofline`` describes the dominant kernel of the headline solve (the packed
-low-precision trailing update, HBM-bound on the FP64 C tiles) from live
+low-precision trailing update, HBM-bound on the FP32 C tiles) from live
 HIP-event timing of every launch in one profiled solve; ``roofline_fp64_gemm``
 does the same for the FP64 update GEMM over one profiled plain FP64
 factorization (A = S + N*I).  ``traffic`` is the measured HBM bytes per
@@ -246,7 +248,7 @@ def main():
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": tb["hbm_bytes_per_launch"] if tb else None,
                 "kernel": f"gemm_bf16d_kernel (packed low-precision trailing update of the {sprof['tier']} solve; "
-                          "algorithmic bytes = FP64 C read + write + packed panel once)",
+                          "algorithmic bytes = FP32 C read + write + packed panel once)",
                 "launches": k["launches"], "avg_launch_ms": round(k["ms"] / max(1, k["launches"]), 4),
                 "bytes_per_launch": k["bytes"] / max(1, k["launches"]),
                 "tflops": round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 1)}
@@ -279,7 +281,8 @@ def main():
         "config": {"workload": f"{lead} solve to backward error 1e-14 (time-to-solution incl. factorization "
                                f"and FP64 refinement), N={args.n}, b={args.b}, kappa={args.kappa:g}, 1 RHS, "
                                f"tier {tier}", "N": args.n, "b": args.b, "kappa": args.kappa, "nrhs": 1,
-                   "parallelism": "single GPU, 2 HIP streams"},
+                   "parallelism": "single GPU, 3 HIP streams (diagonal chain, trailing update, "
+                                  "row TRSM and packing)"},
         "pct_fp64_peak": best["pct_fp64_peak"],
         "speedup_vs_fp64": best["speedup_vs_fp64"],
         "backward_error": best["backward_error"],

@@ -1400,7 +1400,7 @@ int palial_hip_init(int device)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(PALIAL_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
-    // PALIAL_RESERVE_CUS=n (default 16): the bulk stream of the chain schedule
+    // PALIAL_RESERVE_CUS=n (default 8): the bulk stream of the chain schedule
     // runs on all CUs but the first n, which stay free for the diagonal chain.
     // PALIAL_DAG=1 selects the chain schedule with one update launch per
     // step, PALIAL_DAG=0 the row look-ahead schedule.
@@ -1408,7 +1408,7 @@ int palial_hip_init(int device)
     const char *dag = getenv("PALIAL_DAG");
     g.dag = dag ? atoi(dag) : 2;
     if (g.dag < 0 || g.dag > 2) return fail(PALIAL_ERR_ARG, "PALIAL_DAG=%s: expected 0, 1 or 2", dag);
-    g.reserve = res ? atoi(res) : 16;
+    g.reserve = res ? atoi(res) : 8;
     // PALIAL_SUBTILE=0 factors diagonal tiles with the one-workgroup kernel only
     const char *sub = getenv("PALIAL_SUBTILE");
     g.subtile = sub ? atoi(sub) : 1;
