@@ -87,7 +87,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
     out = {}
     for algo in algos:
         for prec in tiers:
-            name = {0: "fp64", 1: "fp32", 2: "bf16"}[prec]
+            name = {0: "fp64", 1: "fp32", 2: "bf16", 4: "bf16x3"}[prec]
             best = None
             for ft in factor_tiles:
                 times, rep, rc = [], None, 0
@@ -118,7 +118,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
     prof = None
     if conv:
         key = min(conv, key=lambda k: conv[k]["ms"])
-        prec = {"fp64": 0, "fp32": 1, "bf16": 2}[key.split("_")[1]]
+        prec = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 4}[key.split("_")[1]]
         P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8))
         P.synchronize()
         P.profile(True)
@@ -207,7 +207,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=32768)
     ap.add_argument("--kappa", type=float, default=1e2)
-    ap.add_argument("--tiers", default="0,1,2")
+    ap.add_argument("--tiers", default="0,1,2,4")
     ap.add_argument("--factor-tiles", default="256,512", help="internal factorization tile sizes tried per tier")
     ap.add_argument("--no-factor", action="store_true", help="skip the plain factorization runs")
     ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")

@@ -53,7 +53,9 @@ typedef struct palial_matrix_desc {
 #define PALIAL_PREC_FP64  0
 #define PALIAL_PREC_FP32  1
 #define PALIAL_PREC_BF16  2
-#define PALIAL_PREC_FP8   3
+#define PALIAL_PREC_FP8   3   /* reserved, not built */
+#define PALIAL_PREC_BF16X3 4  /* split BF16: each panel element as hi + lo BF16 terms, three MFMA
+                                 products (hi*hi + hi*lo + lo*hi), ~2^-16 relative per product */
 
 /* options; a NULL pointer means all defaults (zero-initialised struct) */
 typedef struct palial_opts {
@@ -134,9 +136,10 @@ int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms)
 /* Test hook (no counterpart in the reference): applies the trailing update of
  * step k (algo 0 Cholesky: A_ij -= A_ki^T A_kj on the upper triangle, 1 LU:
  * A_ij -= A_ik A_kj, all tiles i, j > k) reps times to a device-resident
- * matrix.  prec 0 uses FP64 operands; 1 (FP32) and 2 (BF16) round the panel
- * of step k once into packed K-contiguous copies and accumulate in FP32, as
- * the mixed-precision solves do (b % 128 == 0).  prec | 8 (with 1 or 2): the
+ * matrix.  prec 0 uses FP64 operands; 1 (FP32), 2 (BF16) and 4 (split BF16)
+ * round the panel of step k once into packed K-contiguous copies and
+ * accumulate in FP32, as the mixed-precision solves do (b % 128 == 0).
+ * prec | 8 (with 1, 2 or 4): the
  * matrix is held as FP32 tiles in the first N*N*4 bytes of the buffer, as the
  * mixed-precision solves' working copy is.  *ms: average time per rep
  * including the packing. */
@@ -188,8 +191,11 @@ int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_
  * (||A||_inf ||x_j||_inf + ||b_j||_inf).  algo 0: tiled Cholesky (A SPD),
  * 1: tiled LU without pivoting.  A (layout L, FP64) is not modified: a copy
  * is factored with its O(n^3) trailing updates in opts->precision (FP64,
- * FP32 or BF16 operands with FP32 accumulation); the diagonal-tile
- * factorizations, the residual b - A x and the solution update stay FP64.
+ * FP32, BF16 or split-BF16 operands with FP32 accumulation; the mixed tiers
+ * factor an FP32 copy with FP32 panel solves when the factor tile is a
+ * multiple of 128); the diagonal-tile factorizations, the residual b - A x
+ * and the solution update stay FP64.  PALIAL_PREC_BF16X3 needs a factor tile
+ * multiple of 128, N above it, and the default schedule.
  * B (column-major, leading dimension ldb >= N) holds the right-hand sides on
  * entry and the solution on return.  Requires b % 64 == 0 and b <= 512.
  * Returns PALIAL_ERR_NOCONV (with the reached backward error in rep) when

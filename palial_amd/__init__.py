@@ -24,7 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpalial_hip.so")
 DRIVER_PATH = os.path.join(_HERE, "bin", "palial")
 
-PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP8, PREC_BF16X3 = 0, 1, 2, 3, 4
 LEFT, RIGHT = 141, 142
 NOTRANS, TRANS = 111, 112
 TASK_CHOL_TRSM, TASK_CHOL_SYRK, TASK_CHOL_GEMM = 1, 2, 3

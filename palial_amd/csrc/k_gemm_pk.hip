@@ -34,6 +34,19 @@ __device__ inline unsigned short to_bf16(double d)
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// BF16 (nearest even) of an FP32 value, and the split x = hi + lo with both
+// terms BF16 (lo = the BF16 of the exact FP32 remainder x - hi)
+__device__ inline unsigned short bf16_bits(float f)
+{
+    const unsigned int u = __float_as_uint(f);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline void bf16_split(float x, unsigned short &hi, unsigned short &lo)
+{
+    hi = bf16_bits(x);
+    lo = bf16_bits(x - __uint_as_float((unsigned int)hi << 16));
+}
+
 // Output storage: FP64 tiles, or (F32C) FP32 tiles at the same element
 // offsets from the matrix base (the FP32 working copy of the mixed-precision
 // solves).  Task pointers are formed on the FP64 view of the base, so the
@@ -95,7 +108,8 @@ __global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT,
     const int r0 = (sbk % nsb) * 64, c0 = (sbk / nsb) * 64;
     const int idx = first + tix;
     const ST *src = colp ? mat + tile_off(b, MT, idx, k) : mat + tile_off(b, MT, k, idx);
-    const size_t dbase = (size_t)(idx - k - 1) * b * b;
+    // PREC 4 (split BF16): tile t holds hi at t*2b^2 and lo at t*2b^2 + b^2 (BF16 elements)
+    const size_t dbase = (size_t)(idx - k - 1) * b * b * (PREC == 4 ? 2 : 1);
     const int t = threadIdx.x;
     if (!colp) {
         // same layout: 4 consecutive rows of one column per thread and pass
@@ -117,6 +131,12 @@ __global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT,
                 ushort4 v;
                 v.x = to_bf16(x.x); v.y = to_bf16(x.y); v.z = to_bf16(y.x); v.w = to_bf16(y.y);
                 *(ushort4 *)((unsigned short *)dst + dbase + o) = v;
+            } else if (PREC == 4) {
+                ushort4 h, l;
+                bf16_split((float)x.x, h.x, l.x); bf16_split((float)x.y, h.y, l.y);
+                bf16_split((float)y.x, h.z, l.z); bf16_split((float)y.y, h.w, l.w);
+                *(ushort4 *)((unsigned short *)dst + dbase + o) = h;
+                *(ushort4 *)((unsigned short *)dst + dbase + (size_t)b * b + o) = l;
             } else {
                 *(float4 *)((float *)dst + dbase + o) = make_float4((float)x.x, (float)x.y, (float)y.x, (float)y.y);
             }
@@ -137,8 +157,12 @@ __global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT,
             const size_t o = dbase + (size_t)(c0 + q) + (size_t)(r0 + r) * b;
             // BF16 of the FP32 rounding, as to_bf16() does
             if (PREC == 2) {
-                const unsigned int u = __float_as_uint(sT[q][r]);
-                ((unsigned short *)dst)[o] = (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+                ((unsigned short *)dst)[o] = bf16_bits(sT[q][r]);
+            } else if (PREC == 4) {
+                unsigned short h, l;
+                bf16_split(sT[q][r], h, l);
+                ((unsigned short *)dst)[o] = h;
+                ((unsigned short *)dst)[o + (size_t)b * b] = l;
             } else {
                 ((float *)dst)[o] = sT[q][r];
             }
@@ -278,19 +302,27 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 
 __device__ inline void wait_vm(int n)
 {
-    // s_waitcnt vmcnt(n) (expcnt and lgkmcnt left unconstrained); n in {0, 4, 8}
-    if (n >= 8) __builtin_amdgcn_s_waitcnt(0x0F78);
+    // s_waitcnt vmcnt(n) (expcnt and lgkmcnt left unconstrained); n in {0, 4, 8, 16}
+    // (vmcnt bits [3:0] and [15:14])
+    if (n >= 16) __builtin_amdgcn_s_waitcnt(0x4F70);
+    else if (n >= 8) __builtin_amdgcn_s_waitcnt(0x0F78);
     else if (n >= 4) __builtin_amdgcn_s_waitcnt(0x0F74);
     else __builtin_amdgcn_s_waitcnt(0x0F70);
 }
 
-template <int MINB, bool F32C>
+// X3 (split BF16 panels, tile t = hi at 2t*b^2, lo at (2t+1)*b^2 elements):
+// each stage also holds the lo slices and acc += B_hi A_hi + B_lo A_hi + B_hi A_lo,
+// products good to ~2^-16 instead of 2^-8 at three times the MFMA work.
+// NST stages, NST-1 of them in flight; LDS = NST * (2 or 4) * 8 KB.
+template <int MINB, bool F32C, bool X3 = false, int NST = 4>
 __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 {
     const Store<F32C> S{p.mat};
-    constexpr int BM = 128, KB = 32, WM = 64, FM = 4, NST = 4;
+    constexpr int BM = 128, KB = 32, WM = 64, FM = 4, AHEAD = NST - 1;
+    constexpr int NOPS = X3 ? 4 : 2;                    // operand slices per stage (A, B[, A lo, B lo])
+    constexpr int DPS = 2 * NOPS;                       // DMA instructions per wave and stage
     constexpr int SLICE = BM * KB * 2;                  // bytes of one operand slice
-    __shared__ __attribute__((aligned(1024))) char lds[NST][2][SLICE];
+    __shared__ __attribute__((aligned(1024))) char lds[NST][NOPS][SLICE];
 
     const int b = p.b;
     const int nbd = b / BM;
@@ -305,8 +337,9 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
-    const unsigned short *PA = (const unsigned short *)p.pa + (size_t)tp.ia * b * b;
-    const unsigned short *PB = (const unsigned short *)p.pb + (size_t)tp.ib * b * b;
+    const size_t tstride = (size_t)b * b * (X3 ? 2 : 1);
+    const unsigned short *PA = (const unsigned short *)p.pa + (size_t)tp.ia * tstride;
+    const unsigned short *PB = (const unsigned short *)p.pb + (size_t)tp.ib * tstride;
     const int NS = b / KB;
 
     auto issue = [&](int s) {
@@ -320,6 +353,13 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
                                              (lds_void_t *)(lds[st][0] + j * 1024), 16, 0, 0);
             __builtin_amdgcn_global_load_lds((const void *)(PB + (size_t)(j0 + row) * b + kb + c * 8),
                                              (lds_void_t *)(lds[st][1] + j * 1024), 16, 0, 0);
+            if (X3) {
+                const size_t lo = (size_t)b * b;
+                __builtin_amdgcn_global_load_lds((const void *)(PA + lo + (size_t)(i0 + row) * b + kb + c * 8),
+                                                 (lds_void_t *)(lds[st][NOPS - 2] + j * 1024), 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void *)(PB + lo + (size_t)(j0 + row) * b + kb + c * 8),
+                                                 (lds_void_t *)(lds[st][NOPS - 1] + j * 1024), 16, 0, 0);
+            }
         }
     };
     auto frag = [&](const char *base, int row) -> bf8 {
@@ -333,20 +373,37 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 #pragma unroll
         for (int c = 0; c < FM; c++) acc[a][c] = (f4){0.f, 0.f, 0.f, 0.f};
 
-    issue(0);
-    if (NS > 1) issue(1);
-    if (NS > 2) issue(2);
+#pragma unroll
+    for (int q = 0; q < AHEAD; q++)
+        if (q < NS) issue(q);
     for (int s = 0; s < NS; s++) {
-        wait_vm(s + 2 < NS ? 8 : (s + 1 < NS ? 4 : 0));   // slice s has landed (this wave's part)
+        // slice s has landed (this wave's part): stages s+1 .. s+AHEAD-1 may still be in flight
+        const int later = min(AHEAD - 1, NS - 1 - s);
+        wait_vm(later * DPS);
         __builtin_amdgcn_s_barrier();                     // ... and every wave's part
         asm volatile("" ::: "memory");
-        if (s + 3 < NS) issue(s + 3);                     // into the stage read at s-1 (all waves are past it)
+        if (s + AHEAD < NS) issue(s + AHEAD);             // into the stage read at s-1 (all waves are past it)
         const char *A = lds[s % NST][0], *B = lds[s % NST][1];
         bf8 av[FM], bv[FM];
 #pragma unroll
         for (int x = 0; x < FM; x++) av[x] = frag(A, wm * WM + x * 16 + (lane & 15));
 #pragma unroll
         for (int c = 0; c < FM; c++) bv[c] = frag(B, wn * WM + c * 16 + (lane & 15));
+        if constexpr (X3) {
+            const char *Al = lds[s % NST][NOPS - 2], *Bl = lds[s % NST][NOPS - 1];
+            bf8 al[FM], bl[FM];
+#pragma unroll
+            for (int x = 0; x < FM; x++) al[x] = frag(Al, wm * WM + x * 16 + (lane & 15));
+#pragma unroll
+            for (int c = 0; c < FM; c++) bl[c] = frag(Bl, wn * WM + c * 16 + (lane & 15));
+#pragma unroll
+            for (int x = 0; x < FM; x++)
+#pragma unroll
+                for (int c = 0; c < FM; c++) {
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[c], av[x], acc[x][c], 0, 0, 0);
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], al[x], acc[x][c], 0, 0, 0);
+                }
+        }
 #pragma unroll
         for (int x = 0; x < FM; x++)
 #pragma unroll
@@ -524,11 +581,14 @@ hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n,
                        void *dst, hipStream_t s, bool src32)
 {
     if (n <= 0) return hipSuccess;
-    if (b % 64 != 0 || first <= k || first + n > N / b || (prec != 1 && prec != 2)) return hipErrorInvalidValue;
+    if (b % 64 != 0 || first <= k || first + n > N / b || (prec != 1 && prec != 2 && prec != 4))
+        return hipErrorInvalidValue;
     const int nsb = b / 64, cp = colpanel ? 1 : 0;
     dim3 grid((unsigned)(n * nsb * nsb)), block(256);
     const float *m32 = (const float *)(const void *)mat;
-    if (prec == 2 && src32) hipLaunchKernelGGL((pack_kernel<2, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
+    if (prec == 4 && src32) hipLaunchKernelGGL((pack_kernel<4, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
+    else if (prec == 4) hipLaunchKernelGGL((pack_kernel<4, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
+    else if (prec == 2 && src32) hipLaunchKernelGGL((pack_kernel<2, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
     else if (prec == 2) hipLaunchKernelGGL((pack_kernel<2, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
     else if (src32) hipLaunchKernelGGL((pack_kernel<1, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
     else hipLaunchKernelGGL((pack_kernel<1, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
@@ -544,7 +604,12 @@ hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
         return hipErrorInvalidValue;
     const long nbd = a.b / 128;
     dim3 grid((unsigned)((long)a.ntasks * nbd * nbd)), block(256);
-    if (prec == 2) {
+    if (prec == 4) {
+        // two stages (64 KB LDS, two workgroups per CU)
+        if (a.f32c) hipLaunchKernelGGL((gemm_bf16d_kernel<2, true, true, 2>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gemm_bf16d_kernel<2, false, true, 2>), grid, block, 0, s, a);
+    }
+    else if (prec == 2) {
         static const int var = [] {
             const char *e = getenv("PALIAL_PK_VARIANT");
             return e ? atoi(e) : 0;

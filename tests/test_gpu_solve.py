@@ -22,7 +22,7 @@ def backward_error(A, X, B):
 
 
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 4])
 @pytest.mark.parametrize("N,b,nrhs", [(512, 128, 1), (1024, 256, 16), (768, 128, 3)])
 def test_solve_diag_dominant(hip, orc, algo, prec, N, b, nrhs):
     a = orc.gen("sym" if algo == "cholesky" else "dense", N, b, seed=1, shift=N)
@@ -37,7 +37,7 @@ def test_solve_diag_dominant(hip, orc, algo, prec, N, b, nrhs):
         assert rep.iterations <= 1
 
 
-@pytest.mark.parametrize("prec,kappa", [(1, 1e2), (1, 1e5), (2, 1e2), (0, 1e8)])
+@pytest.mark.parametrize("prec,kappa", [(1, 1e2), (1, 1e5), (2, 1e2), (4, 1e2), (4, 1e5), (0, 1e8)])
 def test_solve_conditioned_spd(hip, orc, prec, kappa):
     N, b = 512, 128
     A = spd_cond(N, kappa, 5)
@@ -114,3 +114,28 @@ def test_gmres_ir_well_conditioned(hip, orc):
     # LU-IR only: never uses GMRES
     X2, rep2, rc2 = hip.solve(a, B, algo="lu", precision=2, b=b, refine=2)
     assert rc2 == 0 and rep2.gmres_iterations == 0
+
+
+def test_split_bf16_needs_working_copy(hip, orc):
+    """The split-BF16 tier runs only on the FP32 working copy: a factor tile that
+    is not a multiple of 128 is refused with an argument error."""
+    N, b = 256, 64
+    a = orc.gen("sym", N, b, seed=1, shift=N)
+    B = np.ones((N, 1))
+    with pytest.raises(hip.PalialError):
+        hip.solve(a, B, algo="cholesky", precision=hip.PREC_BF16X3, b=b)
+
+
+def test_split_bf16_fewer_refinement_steps(hip, orc):
+    """Split-BF16 products (~2^-16) leave a factor accurate enough that LU-IR
+    needs fewer steps than with plain BF16 at the same condition number."""
+    N, b = 1024, 256
+    A = spd_cond(N, 1e3, 9)
+    a = orc.from_dense(A, N, b)
+    B = np.random.default_rng(3).standard_normal((N, 1))
+    its = {}
+    for prec in (2, 4):
+        X, rep, rc = hip.solve(a, B, algo="cholesky", precision=prec, b=b, refine=2, max_iter=60)
+        assert rc == 0 and backward_error(A, X, B) <= 2e-14
+        its[prec] = rep.iterations
+    assert its[4] < its[2], its

@@ -24,12 +24,38 @@ def round_bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def split_bf16(x):
+    """Split BF16 panels (precision 4): hi = BF16(FP32(x)), lo = BF16(FP32(x) - hi)."""
+    hi = round_bf16(x)
+    lo = round_bf16(round_fp32(x) - hi)
+    return hi, lo
+
+
 ROUND = {0: lambda x: x, 1: round_fp32, 2: round_bf16}
-TOL = {0: 4.0 * 2.0 ** -53, 1: 4.0 * 2.0 ** -24, 2: 4.0 * 2.0 ** -24}
+TOL = {0: 4.0 * 2.0 ** -53, 1: 4.0 * 2.0 ** -24, 2: 4.0 * 2.0 ** -24, 4: 4.0 * 2.0 ** -24}
+
+
+def model_update(A, s0, s1, algo, prec):
+    """Step-k update with operands as the device rounds them; the split tier
+    sums hi*hi + hi*lo + lo*hi (lo*lo is not formed)."""
+    if algo == "cholesky":
+        P = A[s0:s1, s1:]
+        if prec == 4:
+            h, l = split_bf16(P)
+            return h.T @ h + h.T @ l + l.T @ h, np.abs(P).T @ np.abs(P)
+        U = ROUND[prec](P)
+        return U.T @ U, np.abs(U).T @ np.abs(U)
+    Lc, Ur = A[s1:, s0:s1], A[s0:s1, s1:]
+    if prec == 4:
+        lh, ll = split_bf16(Lc)
+        uh, ul = split_bf16(Ur)
+        return lh @ uh + lh @ ul + ll @ uh, np.abs(Lc) @ np.abs(Ur)
+    Lr, Urr = ROUND[prec](Lc), ROUND[prec](Ur)
+    return Lr @ Urr, np.abs(Lr) @ np.abs(Urr)
 
 
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 4])
 @pytest.mark.parametrize("N,b,k", [(1024, 256, 0), (1024, 256, 1), (1536, 512, 0), (768, 128, 2)])
 def test_trailing_update(hip, orc, algo, prec, N, b, k):
     a = orc.gen("dense", N, b, seed=11 + k, shift=0.0)
@@ -41,13 +67,10 @@ def test_trailing_update(hip, orc, algo, prec, N, b, k):
         out = orc.to_dense(dm.to_host(), N, b)
     finally:
         dm.free()
-    r = ROUND[prec]
     s0, s1 = k * b, (k + 1) * b
     exp = A.copy()
+    upd, scale = model_update(A, s0, s1, algo, prec)
     if algo == "cholesky":
-        U = r(A[s0:s1, s1:])
-        upd = U.T @ U
-        scale = np.abs(U).T @ np.abs(U)
         # upper triangle of the trailing block, tile by tile (i <= j)
         mask = np.zeros((N - s1, N - s1), dtype=bool)
         idx = np.arange(N - s1)
@@ -56,10 +79,6 @@ def test_trailing_update(hip, orc, algo, prec, N, b, k):
         mask |= ti[:, None] < ti[None, :]
         mask |= same & (idx[:, None] <= idx[None, :])
     else:
-        Lc = r(A[s1:, s0:s1])
-        Ur = r(A[s0:s1, s1:])
-        upd = Lc @ Ur
-        scale = np.abs(Lc) @ np.abs(Ur)
         mask = np.ones((N - s1, N - s1), dtype=bool)
     exp[s1:, s1:] = np.where(mask, A[s1:, s1:] - upd, A[s1:, s1:])
     err = np.abs(out - exp)
@@ -72,7 +91,7 @@ def test_trailing_update(hip, orc, algo, prec, N, b, k):
 
 
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
-@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("prec", [1, 2, 4])
 @pytest.mark.parametrize("N,b,k", [(1024, 256, 0), (1536, 512, 1)])
 def test_trailing_update_fp32_storage(hip, orc, algo, prec, N, b, k):
     """The same update on FP32 tiles (the mixed-precision solves' working copy,
@@ -90,17 +109,13 @@ def test_trailing_update_fp32_storage(hip, orc, algo, prec, N, b, k):
         dm.free()
     A = orc.to_dense(a32.astype(np.float64), N, b)
     out = orc.to_dense(out32.astype(np.float64), N, b)
-    r = ROUND[prec]
     s0, s1 = k * b, (k + 1) * b
+    upd, scale = model_update(A, s0, s1, algo, prec)
     if algo == "cholesky":
-        U = r(A[s0:s1, s1:])
-        upd, scale = U.T @ U, np.abs(U).T @ np.abs(U)
         idx = np.arange(N - s1)
         ti = idx // b
         mask = (ti[:, None] < ti[None, :]) | ((ti[:, None] == ti[None, :]) & (idx[:, None] <= idx[None, :]))
     else:
-        Lc, Ur = r(A[s1:, s0:s1]), r(A[s0:s1, s1:])
-        upd, scale = Lc @ Ur, np.abs(Lc) @ np.abs(Ur)
         mask = np.ones((N - s1, N - s1), dtype=bool)
     exp = A[s1:, s1:] - upd
     assert np.array_equal(out[:s1, :], A[:s1, :]) and np.array_equal(out[:, :s1], A[:, :s1])
