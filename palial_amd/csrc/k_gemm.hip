@@ -340,7 +340,7 @@ hipError_t launch_gemm_fine(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
 {
     if (a.ntasks <= 0) return hipSuccess;
     if (a.b % 32 != 0) return launch_gemm(a, ta, tb, s);
-    return launch_mfma<32, 16, 1>(a, ta, tb, s);
+    return launch_mfma<32, 32, 1>(a, ta, tb, s);      // K slices of 32: half the barriers
 }
 
 hipError_t launch_gemm(const GemmArgs &a, bool ta, bool tb, hipStream_t s)

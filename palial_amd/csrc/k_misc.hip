@@ -168,8 +168,9 @@ __global__ void tile_split_kernel(double *__restrict__ sub, TT *__restrict__ til
 hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s, bool tile32)
 {
     if (b <= 0 || bs <= 0 || b % bs != 0) return hipErrorInvalidValue;
-    int blocks = (b * b + 1023) / 1024;
-    if (blocks > 1024) blocks = 1024;
+    // one element per thread: the copy is latency-bound at this size
+    int blocks = (b * b + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
     if (tile32)
         hipLaunchKernelGGL(tile_split_kernel<float>, dim3(blocks), dim3(256), 0, s, sub, (float *)(void *)tile, b, bs,
                            to_sub ? 1 : 0);

@@ -18,10 +18,11 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------- tile GEMV
 // For each task t: out_t(0:b, 0:nrhs) = op(T_t) v  (GV_DIAG) or
-// out_t -= op(T_t) v (updates).  A workgroup (256 threads) owns 16 output
+// out_t -= op(T_t) v (updates).  A workgroup (256 threads) owns ROWS output
 // rows of one tile; the input segment (b x nrhs) sits in LDS.
-//   op = N: out(r) = sum_c T(r, c) v(c): 16 row lanes x 16 column groups,
-//           a column of 16 rows is one 128-byte read
+//   op = N: out(r) = sum_c T(r, c) v(c): ROWS row lanes x 256/ROWS column
+//           groups; ROWS = 16 for FP64 and 32 for FP32 tiles, so that a
+//           column piece is one full 128-byte line
 //   op = T: out(r) = sum_c T(c, r) v(c): a wave owns 4 rows; its 64 lanes
 //           read 64 consecutive c of a row (512 bytes) and reduce by shuffles
 constexpr int GV_THREADS = 256;
@@ -30,14 +31,15 @@ constexpr int GV_MAXR = 16;          // right-hand sides
 
 // E: element type of the tiles (double, or float for the FP32 copies the
 // mixed-precision correction solves stream); sums are accumulated in FP64.
-template <int NR, typename E>
+template <int NR, typename E, int ROWS>
 __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 {
+    constexpr int G = GV_THREADS / ROWS;      // column groups (op = N)
     extern __shared__ double sm[];
     double *sv = sm;                          // b x NR input segment: sv[c + j*b]
-    double *sred = sm + (size_t)p.b * NR;     // 16 groups x 16 rows x NR partial sums
+    double *sred = sm + (size_t)p.b * NR;     // G groups x ROWS rows x NR partial sums
     const int b = p.b;
-    const int rb_per_tile = b / GV_ROWS;
+    const int rb_per_tile = b / ROWS;
     const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
     const int t = threadIdx.x;
     const E *mat = (const E *)p.mat;
@@ -60,31 +62,31 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
         sv[e] = (c < p.nrhs) ? vi[(size_t)i + (size_t)c * p.ld] : 0.0;   // padded right-hand sides are zero
     }
     __syncthreads();
-    const int r0 = rb * GV_ROWS;
+    const int r0 = rb * ROWS;
     double acc[NR];
 #pragma unroll
     for (int j = 0; j < NR; j++) acc[j] = 0.0;
     if (!p.trans) {
-        const int r = t % GV_ROWS, g = t / GV_ROWS;       // g: column group, columns c = g, g+16, ...
+        const int r = t % ROWS, g = t / ROWS;             // g: column group, columns c = g, g+G, ...
         const E *Tr = T + r0 + r;
 #pragma unroll 8
-        for (int c = g; c < b; c += 16) {
+        for (int c = g; c < b; c += G) {
             const double a = (double)Tr[(size_t)c * b];
 #pragma unroll
             for (int j = 0; j < NR; j++) acc[j] += a * sv[c + j * b];
         }
 #pragma unroll
-        for (int j = 0; j < NR; j++) sred[(g * GV_ROWS + r) * NR + j] = acc[j];
+        for (int j = 0; j < NR; j++) sred[(g * ROWS + r) * NR + j] = acc[j];
         __syncthreads();
-        if (t < GV_ROWS * NR && t / GV_ROWS < p.nrhs) {
-            const int rr = t % GV_ROWS, j = t / GV_ROWS;
+        for (int e = t; e < ROWS * p.nrhs; e += GV_THREADS) {   // ROWS * NR may exceed the threads
+            const int rr = e % ROWS, j = e / ROWS;
             double s = 0.0;
-            for (int q = 0; q < 16; q++) s += sred[(q * GV_ROWS + rr) * NR + j];
+            for (int q = 0; q < G; q++) s += sred[(q * ROWS + rr) * NR + j];
             double *o = vo + (size_t)(r0 + rr) + (size_t)j * p.ld;
             if (p.mode == GV_DIAG) *o = s;
             else *o -= s;
         }
-    } else {
+    } else if constexpr (ROWS == 16) {           // op = T: 4 waves x 4 rows (launched with ROWS = 16 only)
         const int lane = t & 63, wv = t >> 6;
         double part[4][NR];
 #pragma unroll
@@ -129,16 +131,22 @@ hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
 {
     if (a.ntasks <= 0) return hipSuccess;
     if (a.b > 512 || a.b % 64 || a.nrhs > GV_MAXR || a.nrhs < 1) return hipErrorInvalidValue;
-    dim3 grid((unsigned)(a.ntasks * (a.b / GV_ROWS))), block(GV_THREADS);
-    auto lds = [&](int nr) { return ((size_t)a.b * nr + 16 * GV_ROWS * nr) * 8; };
-    if (a.f32) {
-        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, float>), grid, block, lds(1), s, a);
-        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, float>), grid, block, lds(4), s, a);
-        else hipLaunchKernelGGL((tile_gemv_kernel<16, float>), grid, block, lds(16), s, a);
+    // FP32 tiles read by columns (op = N): 32 rows per workgroup (128-byte column pieces)
+    const bool wide = a.f32 && !a.trans;
+    dim3 grid((unsigned)(a.ntasks * (a.b / (wide ? 32 : GV_ROWS)))), block(GV_THREADS);
+    auto lds = [&](int nr) { return ((size_t)a.b * nr + (size_t)GV_THREADS * nr) * 8; };
+    if (wide) {
+        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, float, 32>), grid, block, lds(1), s, a);
+        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, float, 32>), grid, block, lds(4), s, a);
+        else hipLaunchKernelGGL((tile_gemv_kernel<16, float, 32>), grid, block, lds(16), s, a);
+    } else if (a.f32) {
+        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, float, 16>), grid, block, lds(1), s, a);
+        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, float, 16>), grid, block, lds(4), s, a);
+        else hipLaunchKernelGGL((tile_gemv_kernel<16, float, 16>), grid, block, lds(16), s, a);
     } else {
-        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, double>), grid, block, lds(1), s, a);
-        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, double>), grid, block, lds(4), s, a);
-        else hipLaunchKernelGGL((tile_gemv_kernel<16, double>), grid, block, lds(16), s, a);
+        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, double, 16>), grid, block, lds(1), s, a);
+        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, double, 16>), grid, block, lds(4), s, a);
+        else hipLaunchKernelGGL((tile_gemv_kernel<16, double, 16>), grid, block, lds(16), s, a);
     }
     return hipGetLastError();
 }
