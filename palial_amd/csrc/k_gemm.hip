@@ -187,10 +187,8 @@ __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
     // loads are in flight together instead of one round trip each.  E (the
     // QR updates only; never aliased with C or D) follows in its own pass.
     const double alpha = p.alpha, beta = p.beta;
-#pragma unroll
-    for (int x = 0; x < FM; x++) {
+    auto ldD = [&](int x, double (&dv)[FM][4]) {
         const int i = i0 + wm * WM + x * 16 + (lane & 15);
-        double dv[FM][4];
 #pragma unroll
         for (int c = 0; c < FM; c++)
 #pragma unroll
@@ -198,6 +196,9 @@ __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
                 const int j = j0 + wn * WM + c * 16 + (lane >> 4) + 4 * v;
                 dv[c][v] = (tp.D && !(tp.diag && i > j)) ? tp.D[(size_t)i + (size_t)j * b] : 0.0;
             }
+    };
+    auto stC = [&](int x, const double (&dv)[FM][4]) {
+        const int i = i0 + wm * WM + x * 16 + (lane & 15);
 #pragma unroll
         for (int c = 0; c < FM; c++)
 #pragma unroll
@@ -207,6 +208,26 @@ __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
                 const double r = alpha * acc[x][c][v];
                 tp.C[(size_t)i + (size_t)j * b] = tp.D ? r + beta * dv[c][v] : r;
             }
+    };
+    if constexpr (FM == 4) {
+        // two fragment rows of D in flight ahead of the stores (D may be C: a
+        // row is read completely before any of its elements is written)
+        double d0[FM][4], d1[FM][4];
+        ldD(0, d0);
+        ldD(1, d1);
+        stC(0, d0);
+        ldD(2, d0);
+        stC(1, d1);
+        ldD(3, d1);
+        stC(2, d0);
+        stC(3, d1);
+    } else {
+#pragma unroll
+        for (int x = 0; x < FM; x++) {
+            double dv[FM][4];
+            ldD(x, dv);
+            stC(x, dv);
+        }
     }
     if (tp.E) {
 #pragma unroll
