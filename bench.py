@@ -132,6 +132,69 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
     return out, prof
 
 
+def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16),
+              tiers=(0, 1, 2, 4), solve_n=32768):
+    """Survey configs 2 and 3 beside the headline (opt-in, --sweep):
+    FP64 factorizations of Cholesky (S + N I), LU (G + N I) and QR (G) at
+    b = 512 for each N (device time of one factorization after a warm-up),
+    and Cholesky solves at N = solve_n for each kappa, nrhs and tier."""
+    import ctypes as C
+    import numpy as np
+    fac = {}
+    for algo in ("cholesky", "lu", "qr"):
+        flops_f = {"cholesky": 1 / 3, "lu": 2 / 3, "qr": 4 / 3}[algo]
+        for N in ns:
+            A0, A = P.DeviceMatrix(N, b), P.DeviceMatrix(N, b)
+            S = P.DeviceMatrix(N, b) if algo == "qr" else None
+            A0.generate(P.GEN_SYM if algo == "cholesky" else P.GEN_DENSE, seed=0,
+                        shift=0.0 if algo == "qr" else float(N))
+            ms = []
+            for i in range(2):
+                A.copy_from(A0)
+                info, rep = P.factor_device(algo, A, S=S)
+                ms.append(rep.factor_ms)
+            t = ms[-1]
+            tf = flops_f * N ** 3 / (t * 1e-3) / 1e12
+            fac[f"{algo}_{N}"] = {"factor_ms": round(t, 2), "tflops": round(tf, 2),
+                                  "pct_fp64_peak": round(100 * tf / FP64_PEAK_TF, 1), "info": info}
+            log(f"[sweep] {algo} N={N} b={b}: {t:.1f} ms, {tf:.1f} TF/s ({100 * tf / FP64_PEAK_TF:.1f}% of FP64 peak)")
+            A0.free(); A.free()
+            if S is not None:
+                S.free()
+    sol = {}
+    N = solve_n
+    A = P.DeviceMatrix(N, b)
+    L = P.lib()
+    dX = C.c_void_p()
+    P._check(L.palial_hip_malloc(C.byref(dX), N * 8 * max(nrhss)))
+    for kappa in kappas:
+        A1, A2 = P.kron_spd_factors(N, kappa, seed=0)
+        A.generate_kron(A1, A2)
+        for nrhs in nrhss:
+            rhs = np.random.default_rng(1).standard_normal((nrhs, N))      # column j = row j (column-major)
+            for prec in tiers:
+                name = {0: "fp64", 1: "fp32", 2: "bf16", 4: "bf16x3"}[prec]
+                res = None
+                for i in range(2):
+                    P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8 * nrhs))
+                    P.synchronize()
+                    t0 = time.perf_counter()
+                    _, rep, rc = P.solve(A, (dX, nrhs, N), algo="cholesky", precision=prec, device=True,
+                                         max_iter=60, check=False, factor_tile=b)
+                    t1 = time.perf_counter()
+                    res = {"ms": round((t1 - t0) * 1e3, 2), "factor_ms": round(rep.factor_ms, 2),
+                           "refine_ms": round(rep.solve_ms, 2), "iterations": rep.iterations,
+                           "gmres_iterations": rep.gmres_iterations, "backward_error": rep.backward_error,
+                           "converged": rc == 0}
+                sol[f"kappa{kappa:g}_nrhs{nrhs}_{name}"] = res
+                log(f"[sweep] cholesky solve N={N} kappa={kappa:g} nrhs={nrhs} {name}: {res['ms']:.1f} ms, "
+                    f"{res['iterations']} it (+{res['gmres_iterations']} GMRES), eta={res['backward_error']:.2e}, "
+                    f"converged={res['converged']}")
+    L.palial_hip_free(dX)
+    A.free()
+    return {"factorizations_fp64_b512": fac, "cholesky_solves": sol}
+
+
 def run_batched(P, count=10000, steps=3, seed=0):
     """Batched small factorizations: `count` matrices with n ~ UniformInt[16, 512]
     (seeded), even i Cholesky of S + n I, odd i LU of G + n I, one N(0,1)
@@ -211,6 +274,8 @@ def main():
     ap.add_argument("--factor-tiles", default="256,512", help="internal factorization tile sizes tried per tier")
     ap.add_argument("--no-factor", action="store_true", help="skip the plain factorization runs")
     ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only the survey's size / condition-number sweep (one JSON line with key 'sweep')")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -218,6 +283,10 @@ def main():
         return 2
     import palial_amd as P
     P.init(0)
+    if args.sweep:
+        print(json.dumps({"sweep": run_sweep(P, b=args.b)}), flush=True)
+        P.finalize()
+        return 0
     algos = args.algos.split(",")
     tiers = [int(x) for x in args.tiers.split(",")]
     fts = [int(x) for x in args.factor_tiles.split(",")]

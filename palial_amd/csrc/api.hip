@@ -56,6 +56,8 @@ struct Ctx {
     double *d_tau = nullptr;        // QR tau scratch (b doubles, overwritten per panel)
     size_t tau_cap = 0;
     double *d_w1 = nullptr, *d_w2 = nullptr;  // QR/ORMQR workspace tiles
+    double *d_qg = nullptr;                    // b x b Gram workspace of the blocked QR panels
+    size_t qg_cap = 0;
     size_t w_cap = 0;
     double *d_dinv = nullptr;       // inverses of the 32x32 diagonal blocks (U then L)
     size_t dinv_cap = 0;
@@ -150,6 +152,15 @@ int ensure_ws(int b, int MT)
         if (hipMalloc(&g.d_w1, need * 8) != hipSuccess || hipMalloc(&g.d_w2, need * 8) != hipSuccess)
             return fail(PALIAL_ERR_NOMEM, "workspace allocation of %zu bytes failed", 2 * need * 8);
         g.w_cap = need;
+    }
+    if (g.qg_cap < (size_t)b * b) {
+        drop_graphs();
+        if (g.d_qg) hipFree(g.d_qg);
+        g.d_qg = nullptr;
+        g.qg_cap = 0;
+        if (hipMalloc(&g.d_qg, (size_t)b * b * 8) != hipSuccess)
+            return fail(PALIAL_ERR_NOMEM, "QR Gram workspace of %zu bytes failed", (size_t)b * b * 8);
+        g.qg_cap = (size_t)b * b;
     }
     if (g.tau_cap < (size_t)b) {
         if (g.d_tau) hipFree(g.d_tau);
@@ -827,7 +838,7 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, Launcher &L)
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
         L.run(KC_DIAG, 4 * b3(b) / 3, 3 * tile_bytes(b), s0,
-              [&] { return launch_geqrt(dA + tile_off(b, MT, k, k), dS + tile_off(b, MT, k, k), g.d_tau, b, s0); });
+              [&] { return launch_geqrt(dA + tile_off(b, MT, k, k), dS + tile_off(b, MT, k, k), g.d_tau, b, s0, g.d_qg); });
         if (T > 0) {
             // PALIAL_dormqr on A(k,n), n > k: C -= V T^T V^T C
             GemmArgs a = gargs(GM_ORM_W, dA, N, b, k, T);
@@ -842,7 +853,7 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, Launcher &L)
         for (int m = k + 1; m < MT; m++) {
             L.run(KC_TSQRT, 2 * b3(b), 5 * tile_bytes(b), s0, [&] {
                 return launch_tsqrt(dA + tile_off(b, MT, k, k), dA + tile_off(b, MT, m, k),
-                                    dS + tile_off(b, MT, m, k), g.d_tau, b, s0);
+                                    dS + tile_off(b, MT, m, k), g.d_tau, b, s0, g.d_qg);
             });
             if (T > 0) {
                 // PALIAL_dtsmqr on (A(k,n), A(m,n)), n > k
@@ -1455,6 +1466,7 @@ int palial_hip_finalize(void)
     g.events.clear();
     if (g.d_w1) hipFree(g.d_w1);
     if (g.d_w2) hipFree(g.d_w2);
+    if (g.d_qg) hipFree(g.d_qg);
     if (g.d_tau) hipFree(g.d_tau);
     if (g.d_dinv) hipFree(g.d_dinv);
     if (g.d_sub) hipFree(g.d_sub);
@@ -1991,7 +2003,9 @@ int palial_hip_dgeqrt(int n, int nb, double *A, int lda, double *T, int ldt, dou
     double *dA = st.in(A, true), *dT = st.in(T, true);
     double *dtau = dev ? tau : st.scratch((size_t)n * 8);
     if (!dA || !dT || !dtau) return fail(PALIAL_ERR_NOMEM, "tile alloc");
-    HIPCHK(launch_geqrt(dA, dT, dtau, n, g.s_main));
+    double *dG = st.scratch((size_t)n * n * 8);
+    if (!dG) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    HIPCHK(launch_geqrt(dA, dT, dtau, n, g.s_main, dG));
     if (!dev && tau) hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main);
     return st.finish();
 }
@@ -2034,7 +2048,9 @@ int palial_hip_dtsqrt(int n, double *A1, int lda1, double *A2, int lda2, double 
     double *d1 = st.in(A1, true), *d2 = st.in(A2, true), *dT = st.in(T, true);
     double *dtau = dev ? tau : st.scratch((size_t)n * 8);
     if (!d1 || !d2 || !dT || !dtau) return fail(PALIAL_ERR_NOMEM, "tile alloc");
-    HIPCHK(launch_tsqrt(d1, d2, dT, dtau, n, g.s_main));
+    double *dG = st.scratch((size_t)n * n * 8);
+    if (!dG) return fail(PALIAL_ERR_NOMEM, "tile alloc");
+    HIPCHK(launch_tsqrt(d1, d2, dT, dtau, n, g.s_main, dG));
     if (!dev && tau) hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main);
     return st.finish();
 }

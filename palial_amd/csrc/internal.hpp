@@ -170,9 +170,10 @@ hipError_t launch_multi_dot(const double *V, size_t ldv, int k, const double *w,
 hipError_t launch_vh_update(const double *V, size_t ldv, int k, const double *h, double *w, double alpha, int N,
                             hipStream_t s);
 
-// QR panels
-hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s);
-hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s);
+// QR panels.  G: b x b workspace (Gram blocks of the panel reflectors); with
+// G != nullptr and b % 32 == 0, 32 <= b <= 512 the blocked kernels run
+hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s, double *G = nullptr);
+hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s, double *G = nullptr);
 
 // misc
 hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, double shift,

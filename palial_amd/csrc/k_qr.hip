@@ -195,14 +195,350 @@ __global__ __launch_bounds__(QR_THREADS) void tsqrt_kernel(double *A1, double *A
     }
 }
 
-hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s)
+// ------------------------------------------------------------------ blocked
+// For b % 32 == 0, b <= 512 the tile is factored in panels of 32 columns:
+//   qr_panel_kernel  : dgeqr2 / the dtsqrt column loop over the 32 panel
+//                      columns (one workgroup, panel in registers), writing V,
+//                      the head rows, tau and the diagonal 32 x 32 block of T;
+//   qr_apply_kernel  : the panel's block reflector I - Y Tpp Y^T applied to the
+//                      trailing columns in 32-column chunks (FP64 MFMA), the
+//                      Gram blocks G(q, p) = Y_q^T Y_p of earlier panels, and
+//                      the off-diagonal T blocks of the previous panel by
+//                      T(q, pc) = -(sum_r T(q, r) G(r, pc)) T(pc, pc).
+// The result is the reference's T (src/srcqr/dtsqrt.h:65-86, dlarft 'F','C'):
+// T is fixed by V and tau, T(0:i,i) = -tau_i T(0:i,0:i) Y(:,0:i)^T Y(:,i), and
+// the block formula evaluates the same recurrence 32 columns at a time.
+//
+// Rows of one panel's reflectors ("stacked rows"): 32 head rows, then nt tail rows.
+//   GE (geqrt of tile A1): head = tile rows c0..c0+31, tail = rows c0+32..b-1;
+//      reflector i has 1 at head row i, v below, 0 above.
+//   TS (tsqrt of [A1; A2]): head = rows c0..c0+31 of R (A1), tail = A2 rows
+//      0..b-1; reflector i is e_i on the head and column i of A2 on the tail.
+namespace {
+
+constexpr int QP = 32;
+constexpr int QP_THREADS = 512;    // thread (j, g): panel column j = t / 16, row group g = t % 16
+constexpr int QA_THREADS = 256;    // four waves, one 16 x 16 quarter of a 32 x 32 block each
+
+typedef double qd4 __attribute__((ext_vector_type(4)));
+
+__device__ inline qd4 qmfma(double a, double b, qd4 c)
 {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ inline double sum16(double v)
+{
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 1);
+    return v;
+}
+
+template <bool GE>
+__global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double *A2, double *T, double *tau_out,
+                                                              int b, int c0)
+{
+    __shared__ double sH[QP][QP + 1];   // head rows: sH[h][c] = (head row h, panel column c)
+    __shared__ double sG[QP][QP + 1];   // sG[q][i] = Y_q^T Y_i, q < i
+    __shared__ double sT[QP][QP + 1];   // diagonal block of T
+    __shared__ double sv[512];          // tail of the current reflector
+    __shared__ double sTau[QP], sScal[2];
+    const int t = threadIdx.x, j = t >> 4, g = t & 15;
+    const int nt = GE ? b - c0 - QP : b;
+    const int ns = nt >> 4;
+    double *tail = GE ? A1 + c0 + QP : A2;
+    double *head = A1 + c0;
+    const size_t cj = (size_t)(c0 + j) * b + g;
+    double a[32];
+#pragma unroll
+    for (int s = 0; s < 32; s++) a[s] = s < ns ? tail[cj + 16 * s] : 0.0;
+    for (int e = t; e < QP * QP; e += QP_THREADS) {
+        const int h = e & 31, c = e >> 5;
+        sH[h][c] = head[(size_t)(c0 + c) * b + h];
+    }
+    // T column c of the diagonal block from G(:, c) and the T columns before it (thread r < 32 owns row r)
+    auto tcol = [&](int c) {
+        const int r = t;
+        double s0 = 0.0, s1 = 0.0;
+        int q = r;
+        for (; q + 1 < c; q += 2) {
+            s0 += sT[r][q] * sG[q][c];
+            s1 += sT[r][q + 1] * sG[q + 1][c];
+        }
+        if (q < c) s0 += sT[r][q] * sG[q][c];
+        const double tc = sTau[c];
+        sT[r][c] = r < c ? (tc == 0.0 ? 0.0 : -tc * (s0 + s1)) : (r == c ? tc : 0.0);
+    };
+    __syncthreads();
+    for (int i = 0; i < QP; i++) {
+        if (j == i) {
+            double p = 0.0;
+#pragma unroll
+            for (int s = 0; s < 32; s++) p += a[s] * a[s];
+            if (GE) {
+                if (g > i) p += sH[g][i] * sH[g][i];
+                if (g + 16 > i) p += sH[g + 16][i] * sH[g + 16][i];
+            }
+            p = sum16(p);
+            if (g == (i & 15)) {   // owner of head row i
+                double tau, beta, scal;
+                householder(sH[i][i], p, tau, beta, scal);
+                sH[i][i] = beta;
+                sTau[i] = tau;
+                sScal[i & 1] = scal;
+                tau_out[c0 + i] = tau;
+            }
+        }
+        __syncthreads();
+        const double tau = sTau[i];
+        if (j == i) {
+            if (tau != 0.0) {
+                const double scal = sScal[i & 1];
+#pragma unroll
+                for (int s = 0; s < 32; s++) a[s] *= scal;
+                if (GE) {
+                    if (g > i) sH[g][i] *= scal;
+                    if (g + 16 > i) sH[g + 16][i] *= scal;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 32; s++)
+                if (s < ns) sv[g + 16 * s] = a[s];
+        }
+        if (t < QP && i > 0) tcol(i - 1);
+        __syncthreads();
+        if (j != i) {
+            double v[32], d = 0.0;
+#pragma unroll
+            for (int s = 0; s < 32; s++) {
+                v[s] = s < ns ? sv[g + 16 * s] : 0.0;
+                d += v[s] * a[s];
+            }
+            double vh0 = 0.0, vh1 = 0.0;
+            if (GE) {
+                if (g > i) { vh0 = sH[g][i]; d += vh0 * sH[g][j]; }
+                if (g + 16 > i) { vh1 = sH[g + 16][i]; d += vh1 * sH[g + 16][j]; }
+            }
+            d = sum16(d);
+            if (j > i) {
+                const double f = tau * (sH[i][j] + d);
+#pragma unroll
+                for (int s = 0; s < 32; s++) a[s] -= f * v[s];
+                if (GE) {
+                    if (g > i) sH[g][j] -= f * vh0;
+                    if (g + 16 > i) sH[g + 16][j] -= f * vh1;
+                }
+                if (g == 0) sH[i][j] -= f;
+            } else if (g == 0) {
+                sG[j][i] = GE ? sH[i][j] + d : d;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < QP) tcol(QP - 1);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 32; s++)
+        if (s < ns) tail[cj + 16 * s] = a[s];
+    for (int e = t; e < QP * QP; e += QP_THREADS) {
+        const int r = e & 31, c = e >> 5;
+        if (GE || r <= c) head[(size_t)(c0 + c) * b + r] = sH[r][c];
+        if (r <= c) T[(size_t)(c0 + r) + (size_t)(c0 + c) * b] = sT[r][c];
+    }
+}
+
+// acc += Y(0:n, 0:16)^T X(0:n, 0:16) for column-major Y, X (ld b), n % 16 == 0.
+// Lane l takes column l % 16 and the four consecutive rows 4 (l / 16) + s of
+// each 16-row group, so that both operands are read as 32-byte pieces.
+__device__ inline qd4 gram16(const double *Y, const double *X, int n, int b, qd4 acc)
+{
+    const int l = threadIdx.x & 63;
+    const size_t o = (size_t)(l & 15) * b + 4 * (l >> 4);
+    const double *y = Y + o, *x = X + o;
+    int r = 0;
+    for (; r + 32 <= n; r += 32) {
+        const qd4 y0 = *(const qd4 *)(y + r), x0 = *(const qd4 *)(x + r);
+        const qd4 y1 = *(const qd4 *)(y + r + 16), x1 = *(const qd4 *)(x + r + 16);
+        acc = qmfma(y0.x, x0.x, acc); acc = qmfma(y0.y, x0.y, acc);
+        acc = qmfma(y0.z, x0.z, acc); acc = qmfma(y0.w, x0.w, acc);
+        acc = qmfma(y1.x, x1.x, acc); acc = qmfma(y1.y, x1.y, acc);
+        acc = qmfma(y1.z, x1.z, acc); acc = qmfma(y1.w, x1.w, acc);
+    }
+    if (r < n) {
+        const qd4 y0 = *(const qd4 *)(y + r), x0 = *(const qd4 *)(x + r);
+        acc = qmfma(y0.x, x0.x, acc); acc = qmfma(y0.y, x0.y, acc);
+        acc = qmfma(y0.z, x0.z, acc); acc = qmfma(y0.w, x0.w, acc);
+    }
+    return acc;
+}
+
+// Role A (blocks [0, na)): chunk x = columns cc = c0 + 32 + 32 x:
+//   W = Yp^T X;  W = Tpp^T W;  X -= Yp W          (dlarfb / dtsqrt's gemv + ger)
+// Role B (blocks [na, na + nbk)): q < p: G(32q + j, c0 + i) = Y_q(:, j)^T Y_p(:, i)
+// Role C (the rest): row chunk q < pc of T column block pc (cq = 32 pc):
+//   T(q, pc) = -(sum_{r=q}^{pc-1} T(q, r) G(r, pc)) T(pc, pc)
+template <bool GE>
+__global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double *A2, double *T, double *G, int b,
+                                                              int p, int na, int nbk, int pc)
+{
+    __shared__ double sW[QP][QP + 1];
+    __shared__ double sW2[QP][QP + 1];
+    __shared__ double sTp[QP][QP + 1];
+    __shared__ double sY[QP][QP + 1];
+    const int t = threadIdx.x, l = t & 63, w = t >> 6, wi = w & 1, wj = w >> 1;
+    const int c0 = QP * p;
+    const int x = blockIdx.x;
+    const int nt = GE ? b - c0 - QP : b;
+    double *head = A1 + c0;                        // stacked row 0 (column offsets added)
+    double *tail = GE ? A1 + c0 + QP : A2;         // stacked row 32
+    const size_t ldb = b;
+    if (x < na + nbk) {
+        const bool roleA = x < na;
+        const int xc = roleA ? c0 + QP + QP * x : QP * (x - na);   // first column of X
+        qd4 acc = {0.0, 0.0, 0.0, 0.0};
+        // head rows: Yp unit lower (GE) or identity (TS)
+        if (GE) {
+            const int c = l & 15, kk = l >> 4;
+            const int ic = 16 * wi + c;                 // panel column of the A operand
+            for (int r = 0; r < QP; r += 16) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const int h = r + 4 * kk + s;
+                    const double yv = h > ic ? head[(size_t)(c0 + ic) * ldb + h] : (h == ic ? 1.0 : 0.0);
+                    const double xv = head[(size_t)(xc + 16 * wj + c) * ldb + h];
+                    acc = qmfma(yv, xv, acc);
+                }
+            }
+        }
+        acc = gram16(tail + (size_t)(c0 + 16 * wi) * ldb, tail + (size_t)(xc + 16 * wj) * ldb, nt, b, acc);
+        // D layout: lane l, value v -> (i = 16 wi + l / 16 + 4 v, j = 16 wj + l % 16)
+        if (!roleA) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = 16 * wi + (l >> 4) + 4 * v, jj = 16 * wj + (l & 15);
+                G[(size_t)(xc + jj) + (size_t)(c0 + i) * ldb] = acc[v];
+            }
+            return;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * wi + (l >> 4) + 4 * v, jj = 16 * wj + (l & 15);
+            double wv = acc[v];
+            if (!GE) wv += head[(size_t)(xc + jj) * ldb + i];   // identity head of Yp
+            sW[i][jj] = wv;
+        }
+        for (int e = t; e < QP * QP; e += QA_THREADS) {
+            const int r = e & 31, c = e >> 5;
+            sTp[r][c] = r <= c ? T[(size_t)(c0 + r) + (size_t)(c0 + c) * ldb] : 0.0;
+            if (GE) sY[r][c] = r > c ? head[(size_t)(c0 + c) * ldb + r] : (r == c ? 1.0 : 0.0);
+        }
+        __syncthreads();
+        // W2 = Tpp^T W
+        for (int e = t; e < QP * QP; e += QA_THREADS) {
+            const int i = e >> 5, jj = e & 31;
+            double s = 0.0;
+            for (int k = 0; k <= i; k++) s += sTp[k][i] * sW[k][jj];
+            sW2[i][jj] = s;
+        }
+        __syncthreads();
+        // head rows: X(h, j) -= (Yp_head W2)(h, j)
+        for (int e = t; e < QP * QP; e += QA_THREADS) {
+            const int h = e & 31, jj = e >> 5;
+            double s = sW2[h][jj];
+            if (GE)
+                for (int k = 0; k < h; k++) s += sY[h][k] * sW2[k][jj];
+            head[(size_t)(xc + jj) * ldb + h] -= s;
+        }
+        // tail rows: X(r, j) -= sum_k Yp(r, k) W2(k, j), as D'[j][r] = sum_k W2(k, j) Yp(r, k)
+        const int nrb = nt >> 4;
+        for (int blk = w; blk < 2 * nrb; blk += 4) {
+            const int rb = blk >> 1, j0 = 16 * (blk & 1);
+            qd4 d = {0.0, 0.0, 0.0, 0.0};
+            const double *yp = tail + (size_t)16 * rb + (l & 15);
+#pragma unroll
+            for (int k0 = 0; k0 < QP; k0 += 4) {
+                const int k = k0 + (l >> 4);
+                d = qmfma(sW2[k][j0 + (l & 15)], yp[(size_t)(c0 + k) * ldb], d);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int jj = j0 + (l >> 4) + 4 * v;
+                tail[(size_t)(xc + jj) * ldb + 16 * rb + (l & 15)] -= d[v];
+            }
+        }
+        return;
+    }
+    // role C
+    const int q = x - na - nbk, cq = QP * pc;
+    qd4 acc = {0.0, 0.0, 0.0, 0.0};
+    {
+        const int c = l & 15, kk = l >> 4;
+        const int ir = 16 * wi + c;                      // row inside chunk q
+        const double *trow = T + (size_t)(QP * q + ir);
+        const double *gcol = G + (size_t)(cq + 16 * wj + c) * ldb + 4 * kk;
+        for (int k0 = QP * q; k0 < cq; k0 += 16) {
+            const qd4 gv = *(const qd4 *)(gcol + k0);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int kc = k0 + 4 * kk + s;                 // column of T
+                double tv = trow[(size_t)kc * ldb];
+                if (kc < QP * q + ir) tv = 0.0;                 // lower part of the diagonal block
+                acc = qmfma(tv, gv[s], acc);
+            }
+        }
+    }
+    // acc: lane l, v -> Z(i = 16 wi + l % 16 ... ) -- operand A was T rows: D[i][j], i = l / 16 + 4 v
+#pragma unroll
+    for (int v = 0; v < 4; v++) sW[16 * wi + (l >> 4) + 4 * v][16 * wj + (l & 15)] = acc[v];
+    for (int e = t; e < QP * QP; e += QA_THREADS) {
+        const int r = e & 31, c = e >> 5;
+        sTp[r][c] = r <= c ? T[(size_t)(cq + r) + (size_t)(cq + c) * ldb] : 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < QP * QP; e += QA_THREADS) {
+        const int i = e & 31, jj = e >> 5;
+        double s = 0.0;
+        for (int k = 0; k <= jj; k++) s += sW[i][k] * sTp[k][jj];
+        T[(size_t)(QP * q + i) + (size_t)(cq + jj) * ldb] = -s;
+    }
+}
+
+hipError_t launch_blocked(bool ge, double *A1, double *A2, double *T, double *tau, double *G, int b, hipStream_t s)
+{
+    const int P = b / QP;
+    for (int p = 0; p <= P; p++) {
+        if (p < P) {
+            if (ge) hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(1), dim3(QP_THREADS), 0, s, A1, A2, T, tau, b, QP * p);
+            else hipLaunchKernelGGL(qr_panel_kernel<false>, dim3(1), dim3(QP_THREADS), 0, s, A1, A2, T, tau, b, QP * p);
+        }
+        const int na = p < P ? P - 1 - p : 0, nbk = p < P ? p : 0, nc = p > 1 ? p - 1 : 0;
+        if (na + nbk + nc == 0) continue;
+        if (ge)
+            hipLaunchKernelGGL(qr_apply_kernel<true>, dim3(na + nbk + nc), dim3(QA_THREADS), 0, s, A1, A2, T, G, b, p,
+                               na, nbk, p - 1);
+        else
+            hipLaunchKernelGGL(qr_apply_kernel<false>, dim3(na + nbk + nc), dim3(QA_THREADS), 0, s, A1, A2, T, G, b,
+                               p, na, nbk, p - 1);
+    }
+    return hipGetLastError();
+}
+
+bool blocked_ok(int b, const double *G) { return G && b % QP == 0 && b >= QP && b <= 512; }
+
+}  // namespace
+
+hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s, double *G)
+{
+    if (blocked_ok(b, G)) return launch_blocked(true, A, nullptr, T, tau, G, b, s);
     hipLaunchKernelGGL(geqrt_kernel, dim3(1), dim3(QR_THREADS), (size_t)2 * b * 8, s, A, T, tau, b);
     return hipGetLastError();
 }
 
-hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s)
+hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s, double *G)
 {
+    if (blocked_ok(b, G)) return launch_blocked(false, A1, A2, T, tau, G, b, s);
     hipLaunchKernelGGL(tsqrt_kernel, dim3(1), dim3(QR_THREADS), (size_t)b * 8, s, A1, A2, T, tau, b);
     return hipGetLastError();
 }

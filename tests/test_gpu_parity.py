@@ -94,7 +94,7 @@ def test_dgetrf_tile(hip, orc, n):
     assert rel(out, ref) < TOL
 
 
-@pytest.mark.parametrize("n", [4, 32, 64, 256])
+@pytest.mark.parametrize("n", [4, 32, 64, 96, 256, 512])
 def test_qr_tiles(hip, orc, n):
     a = rand_tile(n, 31)
     A, T, tau = hip.dgeqrt(a)
@@ -111,6 +111,28 @@ def test_qr_tiles(hip, orc, n):
     o1, o2 = hip.dtsmqr(c1, c2, A2r, T2r)
     r1, r2 = orc.tile_tsmqr(c1, c2, A2r, T2r)
     assert rel(o1, r1) < TOL and rel(o2, r2) < TOL
+
+
+@pytest.mark.parametrize("n", [64, 160])
+def test_qr_tiles_zero_reflectors(hip, orc, n):
+    """Columns whose Householder vector is zero give tau = 0 (dlarfg with a
+    zero x), an identity reflector and a zero row and column of T."""
+    a = np.triu(rand_tile(n, 41))
+    a[:, 40:] = rand_tile(n, 42)[:, 40:]          # first 40 columns already upper triangular
+    A, T, tau = hip.dgeqrt(a)
+    Ar, Tr, taur = orc.tile_geqrt(a)
+    assert np.all(taur[:40] == 0) and np.all(tau[:40] == 0)
+    assert rel(A, Ar) < TOL and rel(np.triu(T), np.triu(Tr)) < TOL and rel(tau, taur) < TOL
+    a2 = rand_tile(n, 43)
+    a2[:, :10] = 0.0
+    a2[:, 33] = 0.0                               # nonzero again after the first updates
+    r = np.triu(Ar)
+    A1, A2, T2, tau2 = hip.dtsqrt(r, a2)
+    A1r, A2r, T2r, tau2r = orc.tile_tsqrt(r, a2)
+    assert np.all(tau2r[:10] == 0) and np.all(tau2[:10] == 0)
+    assert rel(A1, A1r) < TOL and rel(A2, A2r) < TOL and rel(np.triu(T2), np.triu(T2r)) < TOL
+    assert rel(tau2, tau2r) < TOL
+    assert np.array_equal(np.tril(A1, -1), np.tril(r, -1))   # below R: untouched
 
 
 # ------------------------------------------------------------------ full algorithms
@@ -147,7 +169,7 @@ def test_lu(hip, orc, N, b):
     assert orc.resid_lu(orc.to_dense(a, N, b), orc.to_dense(out, N, b)) < 10
 
 
-@pytest.mark.parametrize("N,b", [(4, 2), (256, 64), (512, 128), (1024, 256)])
+@pytest.mark.parametrize("N,b", [(4, 2), (256, 64), (512, 128), (1024, 256), (480, 96), (2048, 512)])
 def test_qr(hip, orc, N, b):
     a = orc.gen("dense", N, b, seed=0, shift=0.0)
     out = a.copy()
