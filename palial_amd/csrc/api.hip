@@ -48,15 +48,16 @@ struct Ctx {
                                     // the reserved CUs (s_main when nothing is reserved)
     hipStream_t s_aux = nullptr;    // chain schedule with the split update: the row/column
                                     // TRSMs and panel packing (CU mask of s_bulk)
+    hipStream_t s_qr[3] = {};       // QR: further panel-chain streams (steps k mod 4 = 1..3)
     int reserve = 0;                // CUs kept free of bulk work for the diagonal chain
     int dag = 2;                    // 2: diagonal chain + split update, 1: diagonal chain,
                                     // 0: row look-ahead
     std::vector<hipEvent_t> events;
     int *d_info = nullptr;
-    double *d_tau = nullptr;        // QR tau scratch (b doubles, overwritten per panel)
+    double *d_tau = nullptr;        // QR tau scratch (b doubles per panel-chain stream)
     size_t tau_cap = 0;
-    double *d_w1 = nullptr, *d_w2 = nullptr;  // QR/ORMQR workspace tiles
-    double *d_qg = nullptr;                    // b x b Gram workspace of the blocked QR panels
+    double *d_w1 = nullptr, *d_w2 = nullptr;  // QR/ORMQR workspace tiles (MT tiles per bulk stream)
+    double *d_qg = nullptr;                    // b x b Gram workspace of the blocked QR panels, per chain
     size_t qg_cap = 0;
     size_t w_cap = 0;
     double *d_dinv = nullptr;       // inverses of the 32x32 diagonal blocks (U then L)
@@ -140,10 +141,13 @@ void drop_graphs()
     g.graphs.clear();
 }
 
+constexpr int QR_CHAINS = 4;   // panel-chain streams of the QR schedule (step k on chain k % 4)
+constexpr int QR_BULK = 2;     // update streams (step k on bulk k % 2)
+
 int ensure_ws(int b, int MT)
 {
-    const size_t need = (size_t)b * b * (size_t)(MT > 1 ? MT : 1);
-    if (g.w_cap < need || g.tau_cap < (size_t)b) drop_graphs();   // captured graphs hold the old pointers
+    const size_t need = (size_t)QR_BULK * b * b * (size_t)(MT > 1 ? MT : 1);
+    if (g.w_cap < need || g.tau_cap < (size_t)QR_CHAINS * b) drop_graphs();   // captured graphs hold the old pointers
     if (g.w_cap < need) {
         if (g.d_w1) hipFree(g.d_w1);
         if (g.d_w2) hipFree(g.d_w2);
@@ -153,19 +157,22 @@ int ensure_ws(int b, int MT)
             return fail(PALIAL_ERR_NOMEM, "workspace allocation of %zu bytes failed", 2 * need * 8);
         g.w_cap = need;
     }
-    if (g.qg_cap < (size_t)b * b) {
+    if (g.qg_cap < (size_t)QR_CHAINS * b * b) {
         drop_graphs();
         if (g.d_qg) hipFree(g.d_qg);
         g.d_qg = nullptr;
         g.qg_cap = 0;
-        if (hipMalloc(&g.d_qg, (size_t)b * b * 8) != hipSuccess)
-            return fail(PALIAL_ERR_NOMEM, "QR Gram workspace of %zu bytes failed", (size_t)b * b * 8);
-        g.qg_cap = (size_t)b * b;
+        if (hipMalloc(&g.d_qg, (size_t)QR_CHAINS * b * b * 8) != hipSuccess)
+            return fail(PALIAL_ERR_NOMEM, "QR Gram workspace of %zu bytes failed", (size_t)QR_CHAINS * b * b * 8);
+        g.qg_cap = (size_t)QR_CHAINS * b * b;
     }
-    if (g.tau_cap < (size_t)b) {
+    if (g.tau_cap < (size_t)QR_CHAINS * b) {
         if (g.d_tau) hipFree(g.d_tau);
-        if (hipMalloc(&g.d_tau, (size_t)b * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "tau alloc failed");
-        g.tau_cap = b;
+        g.d_tau = nullptr;
+        g.tau_cap = 0;
+        if (hipMalloc(&g.d_tau, (size_t)QR_CHAINS * b * 8) != hipSuccess)
+            return fail(PALIAL_ERR_NOMEM, "tau alloc failed");
+        g.tau_cap = (size_t)QR_CHAINS * b;
     }
     return 0;
 }
@@ -830,44 +837,105 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
 }
 
 // Flat-tree tiled QR, src/qr.h:18-190.
-void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, Launcher &L)
+// two == false: every task on s0 in the reference's loop order.
+// two == true: step k's panel chain (GEQRT(k), then TSQRT(k, m) in m order)
+// runs on chain stream k % 4 and its updates (ORMQR(k), then TSMQR(k, m) in
+// m order) on bulk stream k % 2, each stream with its own workspaces:
+//   GEQRT(k) and TSQRT(k, m) wait for TSMQR(k-1, k) / TSMQR(k-1, m) (tile final),
+//   ORMQR(k) waits for GEQRT(k), TSMQR(k, m) for TSQRT(k, m).
+// Tasks that may run at the same time touch disjoint tiles (ORMQR(k) reads
+// only the unit-lower V of A(k,k) while TSQRT(k, m) rewrites its upper R;
+// TSMQR(k, m) and TSMQR(k+1, m') write rows {k, m} and {k+1, m'}, m' < m),
+// so up to four steps' panel chains progress together and the factors are
+// the same bits as in the reference's order.
+void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1, bool two, Launcher &L)
 {
     L.b = b;
     L.prec = 0;
     const int MT = N / b;
-    for (int k = 0; k < MT; k++) {
+    const size_t wstride = (size_t)b * b * (size_t)MT;   // one bulk stream's W tiles
+    auto geqrt = [&](int k, hipStream_t s, int c) {
+        L.run(KC_DIAG, 4 * b3(b) / 3, 3 * tile_bytes(b), s, [&] {
+            return launch_geqrt(dA + tile_off(b, MT, k, k), dS + tile_off(b, MT, k, k), g.d_tau + (size_t)c * b, b, s,
+                                g.d_qg + (size_t)c * b * b);
+        });
+    };
+    auto ormqr = [&](int k, hipStream_t s, int u) {   // PALIAL_dormqr on A(k,n), n > k: C -= V T^T V^T C
         const int T = MT - k - 1;
-        L.run(KC_DIAG, 4 * b3(b) / 3, 3 * tile_bytes(b), s0,
-              [&] { return launch_geqrt(dA + tile_off(b, MT, k, k), dS + tile_off(b, MT, k, k), g.d_tau, b, s0, g.d_qg); });
-        if (T > 0) {
-            // PALIAL_dormqr on A(k,n), n > k: C -= V T^T V^T C
-            GemmArgs a = gargs(GM_ORM_W, dA, N, b, k, T);
-            a.smat = dS; a.w1 = g.d_w1; a.w2 = g.d_w2;
-            a.alpha = 1.0; a.beta = 0.0; a.maskA = MK_UNIT_LOWER;
-            L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
-            a.mode = GM_ORM_TW; a.maskA = MK_UPPER;
-            L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
-            a.mode = GM_ORM_VW; a.maskA = MK_UNIT_LOWER; a.alpha = -1.0; a.beta = 1.0;
-            L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, false, false, s0); });
-        }
-        for (int m = k + 1; m < MT; m++) {
-            L.run(KC_TSQRT, 2 * b3(b), 5 * tile_bytes(b), s0, [&] {
-                return launch_tsqrt(dA + tile_off(b, MT, k, k), dA + tile_off(b, MT, m, k),
-                                    dS + tile_off(b, MT, m, k), g.d_tau, b, s0, g.d_qg);
-            });
-            if (T > 0) {
-                // PALIAL_dtsmqr on (A(k,n), A(m,n)), n > k
-                GemmArgs a = gargs(GM_QR_W, dA, N, b, k, T);
-                a.m = m; a.smat = dS; a.w1 = g.d_w1; a.w2 = g.d_w2;
-                a.alpha = 1.0; a.beta = 1.0;
-                L.run(KC_QRUPD, 2 * T * b3(b), 4.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
-                a.mode = GM_QR_TW; a.maskA = MK_UPPER; a.beta = 0.0;
-                L.run(KC_QRUPD, T * b3(b), 4.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, true, false, s0); });
-                a.mode = GM_QR_VW; a.maskA = MK_NONE; a.alpha = -1.0; a.beta = 1.0;
-                L.run(KC_QRUPD, 2 * T * b3(b), 3.0 * T * tile_bytes(b), s0, [&] { return launch_gemm(a, false, false, s0); });
+        GemmArgs a = gargs(GM_ORM_W, dA, N, b, k, T);
+        a.smat = dS; a.w1 = g.d_w1 + u * wstride; a.w2 = g.d_w2 + u * wstride;
+        a.alpha = 1.0; a.beta = 0.0; a.maskA = MK_UNIT_LOWER;
+        L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, true, false, s); });
+        a.mode = GM_ORM_TW; a.maskA = MK_UPPER;
+        L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, true, false, s); });
+        a.mode = GM_ORM_VW; a.maskA = MK_UNIT_LOWER; a.alpha = -1.0; a.beta = 1.0;
+        L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, false, false, s); });
+    };
+    auto tsqrt = [&](int k, int m, hipStream_t s, int c) {
+        L.run(KC_TSQRT, 2 * b3(b), 5 * tile_bytes(b), s, [&] {
+            return launch_tsqrt(dA + tile_off(b, MT, k, k), dA + tile_off(b, MT, m, k), dS + tile_off(b, MT, m, k),
+                                g.d_tau + (size_t)c * b, b, s, g.d_qg + (size_t)c * b * b);
+        });
+    };
+    auto tsmqr = [&](int k, int m, hipStream_t s, int u) {   // PALIAL_dtsmqr on (A(k,n), A(m,n)), n > k
+        const int T = MT - k - 1;
+        GemmArgs a = gargs(GM_QR_W, dA, N, b, k, T);
+        a.m = m; a.smat = dS; a.w1 = g.d_w1 + u * wstride; a.w2 = g.d_w2 + u * wstride;
+        a.alpha = 1.0; a.beta = 1.0;
+        L.run(KC_QRUPD, 2 * T * b3(b), 4.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, true, false, s); });
+        a.mode = GM_QR_TW; a.maskA = MK_UPPER; a.beta = 0.0;
+        L.run(KC_QRUPD, T * b3(b), 4.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, true, false, s); });
+        a.mode = GM_QR_VW; a.maskA = MK_NONE; a.alpha = -1.0; a.beta = 1.0;
+        L.run(KC_QRUPD, 2 * T * b3(b), 3.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, false, false, s); });
+    };
+    if (!two) {
+        for (int k = 0; k < MT; k++) {
+            geqrt(k, s0, 0);
+            if (k + 1 < MT) ormqr(k, s0, 0);
+            for (int m = k + 1; m < MT; m++) {
+                tsqrt(k, m, s0, 0);
+                tsmqr(k, m, s0, 0);
             }
         }
+        return;
     }
+    const hipStream_t chain[QR_CHAINS] = {s1, g.s_qr[0], g.s_qr[1], g.s_qr[2]};
+    const hipStream_t bulk[QR_BULK] = {g.s_bulk, g.s_aux};
+    hipEvent_t *E = g.events.data();                 // E[(k & 1) MT + m]: TSMQR(k, m) done
+    hipEvent_t evG = g.events[2 * MT], evQ = g.events[2 * MT + 1], fork = g.events[2 * MT + 2];
+    hipEvent_t *join = g.events.data() + 2 * MT + 3;   // QR_CHAINS + QR_BULK events
+    L.chk(hipEventRecord(fork, s0));
+    for (hipStream_t s : chain) L.chk(hipStreamWaitEvent(s, fork, 0));
+    for (hipStream_t s : bulk)
+        if (s != s0) L.chk(hipStreamWaitEvent(s, fork, 0));
+    for (int k = 0; k < MT; k++) {
+        const int c = k % QR_CHAINS, u = k % QR_BULK;
+        hipStream_t sc = chain[c], sb = bulk[u];
+        if (k > 0) L.chk(hipStreamWaitEvent(sc, E[((k - 1) & 1) * MT + k], 0));
+        geqrt(k, sc, c);
+        if (k + 1 == MT) break;
+        L.chk(hipEventRecord(evG, sc));
+        L.chk(hipStreamWaitEvent(sb, evG, 0));
+        ormqr(k, sb, u);
+        for (int m = k + 1; m < MT; m++) {
+            if (k > 0) L.chk(hipStreamWaitEvent(sc, E[((k - 1) & 1) * MT + m], 0));
+            tsqrt(k, m, sc, c);
+            L.chk(hipEventRecord(evQ, sc));
+            L.chk(hipStreamWaitEvent(sb, evQ, 0));
+            tsmqr(k, m, sb, u);
+            L.chk(hipEventRecord(E[(k & 1) * MT + m], sb));
+        }
+    }
+    int nj = 0;
+    for (hipStream_t s : chain) {
+        L.chk(hipEventRecord(join[nj], s));
+        L.chk(hipStreamWaitEvent(s0, join[nj++], 0));
+    }
+    for (hipStream_t s : bulk)
+        if (s != s0) {
+            L.chk(hipEventRecord(join[nj], s));
+            L.chk(hipStreamWaitEvent(s0, join[nj++], 0));
+        }
 }
 
 // ------------------------------------------------------------------ solve
@@ -1207,7 +1275,9 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     hipStream_t s0 = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
     hipStream_t s1 = g.s_panel;
     const bool lookahead = !o.no_lookahead && MT > 1 && algo != ALG_QR;
-    const bool use_graph = !o.no_graph && !g_prof.on && !(lookahead && g.dag >= 1 && g.reserve > 0);
+    // QR's two-stream schedule runs eagerly (a graph replay drops the bulk stream's CU mask)
+    const bool qr_two = algo == ALG_QR && !o.no_lookahead && MT > 1;
+    const bool use_graph = !o.no_graph && !g_prof.on && !(lookahead && g.dag >= 1 && g.reserve > 0) && !qr_two;
     if ((rc = ensure_events(events_needed(MT)))) return rc;
     if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
     if ((rc = ensure_dinv(b, MT))) return rc;
@@ -1246,7 +1316,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     auto issue = [&](hipStream_t st) {
         if (algo == ALG_CHOL) dag_cholesky(dA, N, b, lookahead, st, s1, g.d_info, L);
         else if (algo == ALG_LU) dag_lu(dA, N, b, lookahead, st, s1, L);
-        else dag_qr(dA, dS, N, b, st, L);
+        else dag_qr(dA, dS, N, b, st, s1, qr_two, L);
     };
     long nodes = 0;
     if (use_graph) {
@@ -1432,6 +1502,7 @@ int palial_hip_init(int device)
     g.f32work = fw ? atoi(fw) : 1;
     HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
+    for (auto &q : g.s_qr) HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
     if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
         const int ncu = prop.multiProcessorCount, words = (ncu + 31) / 32;
         std::vector<uint32_t> mu(words, 0);
@@ -1483,6 +1554,7 @@ int palial_hip_finalize(void)
     if (g.s_aux) hipStreamDestroy(g.s_aux);
     hipStreamDestroy(g.s_main);
     hipStreamDestroy(g.s_panel);
+    for (auto q : g.s_qr) if (q) hipStreamDestroy(q);
     g = Ctx();
     return 0;
 }

@@ -227,24 +227,43 @@ __device__ inline qd4 qmfma(double a, double b, qd4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// DPP move of a double inside rows of 16 lanes
+template <int CTRL>
+__device__ inline double dpp_d(double v)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 16 lanes of a DPP row: pairs, quads (quad_perm), then the
+// half-row and row mirrors.  Each step adds two values that the partner lane
+// adds in the other order, so every lane ends with the same bits.
 __device__ inline double sum16(double v)
 {
-    v += __shfl_xor(v, 8);
-    v += __shfl_xor(v, 4);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 1);
+    v += dpp_d<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);   // row_half_mirror
+    v += dpp_d<0x140>(v);   // row_mirror
     return v;
 }
 
+// One barrier per column: after the update of step i, the threads of column
+// i+1 already hold that column's final values and build reflector i+1 (norm,
+// tau, scaled v into the other half of the double-buffered sv) while the rest
+// finish step i.  Column c of the diagonal T block is formed in step c+1 by
+// rows (thread group j computes T(j, c), its 16 lanes splitting the sum).
 template <bool GE>
 __global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double *A2, double *T, double *tau_out,
                                                               int b, int c0)
 {
+    constexpr int SVLD = 34;            // sv row stride: thread g's tail rows g + 16 s at [g * SVLD + s]
     __shared__ double sH[QP][QP + 1];   // head rows: sH[h][c] = (head row h, panel column c)
     __shared__ double sG[QP][QP + 1];   // sG[q][i] = Y_q^T Y_i, q < i
-    __shared__ double sT[QP][QP + 1];   // diagonal block of T
-    __shared__ double sv[512];          // tail of the current reflector
-    __shared__ double sTau[QP], sScal[2];
+    __shared__ double sT[QP][QP + 1];   // diagonal block of T (upper part)
+    __shared__ __align__(16) double sv[2][16 * SVLD];
+    __shared__ double sTau[QP];
     const int t = threadIdx.x, j = t >> 4, g = t & 15;
     const int nt = GE ? b - c0 - QP : b;
     const int ns = nt >> 4;
@@ -258,64 +277,82 @@ __global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double
         const int h = e & 31, c = e >> 5;
         sH[h][c] = head[(size_t)(c0 + c) * b + h];
     }
-    // T column c of the diagonal block from G(:, c) and the T columns before it (thread r < 32 owns row r)
-    auto tcol = [&](int c) {
-        const int r = t;
-        double s0 = 0.0, s1 = 0.0;
-        int q = r;
-        for (; q + 1 < c; q += 2) {
-            s0 += sT[r][q] * sG[q][c];
-            s1 += sT[r][q + 1] * sG[q + 1][c];
-        }
-        if (q < c) s0 += sT[r][q] * sG[q][c];
-        const double tc = sTau[c];
-        sT[r][c] = r < c ? (tc == 0.0 ? 0.0 : -tc * (s0 + s1)) : (r == c ? tc : 0.0);
-    };
     __syncthreads();
-    for (int i = 0; i < QP; i++) {
-        if (j == i) {
-            double p = 0.0;
+
+    // reflector of column c, by the 16 threads of column c
+    auto reflect = [&](int c) {
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
-            for (int s = 0; s < 32; s++) p += a[s] * a[s];
+        for (int s = 0; s < 32; s += 4) {
+            p0 += a[s] * a[s];
+            p1 += a[s + 1] * a[s + 1];
+            p2 += a[s + 2] * a[s + 2];
+            p3 += a[s + 3] * a[s + 3];
+        }
+        double p = (p0 + p1) + (p2 + p3);
+        if (GE) {
+            if (g > c) p += sH[g][c] * sH[g][c];
+            if (g + 16 > c) p += sH[g + 16][c] * sH[g + 16][c];
+        }
+        p = sum16(p);
+        const int own = c & 15;   // lane holding head row c
+        const double alpha = __shfl(g == own ? sH[c][c] : 0.0, (t & 48) | own);
+        double tau, beta, scal;
+        householder(alpha, p, tau, beta, scal);
+        if (g == own) {
+            sH[c][c] = beta;
+            sTau[c] = tau;
+            tau_out[c0 + c] = tau;
+        }
+        if (tau != 0.0) {
+#pragma unroll
+            for (int s = 0; s < 32; s++) a[s] *= scal;
             if (GE) {
-                if (g > i) p += sH[g][i] * sH[g][i];
-                if (g + 16 > i) p += sH[g + 16][i] * sH[g + 16][i];
-            }
-            p = sum16(p);
-            if (g == (i & 15)) {   // owner of head row i
-                double tau, beta, scal;
-                householder(sH[i][i], p, tau, beta, scal);
-                sH[i][i] = beta;
-                sTau[i] = tau;
-                sScal[i & 1] = scal;
-                tau_out[c0 + i] = tau;
+                if (g > c) sH[g][c] *= scal;
+                if (g + 16 > c) sH[g + 16][c] *= scal;
             }
         }
+        double *w = sv[c & 1] + g * SVLD;
+#pragma unroll
+        for (int s = 0; s < 32; s += 2)
+            if (s < ns) *(double2 *)(w + s) = double2{a[s], a[s + 1]};
+    };
+    // T(j, c) for the thread's row j <= c (G(:, c) and T(j, 0:c) complete)
+    auto tcol = [&](int c) {
+        if (j < c) {
+            double s = 0.0;
+            for (int q = j + g; q < c; q += 16) s += sT[j][q] * sG[q][c];
+            s = sum16(s);
+            const double tc = sTau[c];
+            if (g == 0) sT[j][c] = tc == 0.0 ? 0.0 : -tc * s;
+        } else if (j == c && g == 0) {
+            sT[c][c] = sTau[c];
+        }
+    };
+
+    if (j == 0) reflect(0);
+    for (int i = 0; i < QP; i++) {
         __syncthreads();
         const double tau = sTau[i];
-        if (j == i) {
-            if (tau != 0.0) {
-                const double scal = sScal[i & 1];
-#pragma unroll
-                for (int s = 0; s < 32; s++) a[s] *= scal;
-                if (GE) {
-                    if (g > i) sH[g][i] *= scal;
-                    if (g + 16 > i) sH[g + 16][i] *= scal;
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < 32; s++)
-                if (s < ns) sv[g + 16 * s] = a[s];
-        }
-        if (t < QP && i > 0) tcol(i - 1);
-        __syncthreads();
+        if (i > 0) tcol(i - 1);
         if (j != i) {
-            double v[32], d = 0.0;
+            const double *vp = sv[i & 1] + g * SVLD;
+            double v[32];
 #pragma unroll
-            for (int s = 0; s < 32; s++) {
-                v[s] = s < ns ? sv[g + 16 * s] : 0.0;
-                d += v[s] * a[s];
+            for (int s = 0; s < 32; s += 2) {
+                double2 x = s < ns ? *(const double2 *)(vp + s) : double2{0.0, 0.0};
+                v[s] = x.x;
+                v[s + 1] = x.y;
             }
+            double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+            for (int s = 0; s < 32; s += 4) {
+                d0 += v[s] * a[s];
+                d1 += v[s + 1] * a[s + 1];
+                d2 += v[s + 2] * a[s + 2];
+                d3 += v[s + 3] * a[s + 3];
+            }
+            double d = (d0 + d1) + (d2 + d3);
             double vh0 = 0.0, vh1 = 0.0;
             if (GE) {
                 if (g > i) { vh0 = sH[g][i]; d += vh0 * sH[g][j]; }
@@ -331,13 +368,14 @@ __global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double
                     if (g + 16 > i) sH[g + 16][j] -= f * vh1;
                 }
                 if (g == 0) sH[i][j] -= f;
+                if (j == i + 1) reflect(i + 1);
             } else if (g == 0) {
                 sG[j][i] = GE ? sH[i][j] + d : d;
             }
         }
     }
     __syncthreads();
-    if (t < QP) tcol(QP - 1);
+    tcol(QP - 1);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 32; s++)
@@ -354,22 +392,33 @@ __global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double
 // each 16-row group, so that both operands are read as 32-byte pieces.
 __device__ inline qd4 gram16(const double *Y, const double *X, int n, int b, qd4 acc)
 {
+    constexpr int D = 4;   // 16-row groups in flight
     const int l = threadIdx.x & 63;
     const size_t o = (size_t)(l & 15) * b + 4 * (l >> 4);
     const double *y = Y + o, *x = X + o;
-    int r = 0;
-    for (; r + 32 <= n; r += 32) {
-        const qd4 y0 = *(const qd4 *)(y + r), x0 = *(const qd4 *)(x + r);
-        const qd4 y1 = *(const qd4 *)(y + r + 16), x1 = *(const qd4 *)(x + r + 16);
-        acc = qmfma(y0.x, x0.x, acc); acc = qmfma(y0.y, x0.y, acc);
-        acc = qmfma(y0.z, x0.z, acc); acc = qmfma(y0.w, x0.w, acc);
-        acc = qmfma(y1.x, x1.x, acc); acc = qmfma(y1.y, x1.y, acc);
-        acc = qmfma(y1.z, x1.z, acc); acc = qmfma(y1.w, x1.w, acc);
-    }
-    if (r < n) {
-        const qd4 y0 = *(const qd4 *)(y + r), x0 = *(const qd4 *)(x + r);
-        acc = qmfma(y0.x, x0.x, acc); acc = qmfma(y0.y, x0.y, acc);
-        acc = qmfma(y0.z, x0.z, acc); acc = qmfma(y0.w, x0.w, acc);
+    const int m = n >> 4;
+    qd4 yb[D], xb[D];
+#pragma unroll
+    for (int u = 0; u < D; u++)
+        if (u < m) {
+            yb[u] = *(const qd4 *)(y + 16 * u);
+            xb[u] = *(const qd4 *)(x + 16 * u);
+        }
+    for (int r = 0; r < m; r += D) {
+#pragma unroll
+        for (int u = 0; u < D; u++) {
+            if (r + u < m) {
+                const qd4 yv = yb[u], xv = xb[u];
+                if (r + u + D < m) {
+                    yb[u] = *(const qd4 *)(y + 16 * (r + u + D));
+                    xb[u] = *(const qd4 *)(x + 16 * (r + u + D));
+                }
+                acc = qmfma(yv.x, xv.x, acc);
+                acc = qmfma(yv.y, xv.y, acc);
+                acc = qmfma(yv.z, xv.z, acc);
+                acc = qmfma(yv.w, xv.w, acc);
+            }
+        }
     }
     return acc;
 }
@@ -451,22 +500,39 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
                 for (int k = 0; k < h; k++) s += sY[h][k] * sW2[k][jj];
             head[(size_t)(xc + jj) * ldb + h] -= s;
         }
-        // tail rows: X(r, j) -= sum_k Yp(r, k) W2(k, j), as D'[j][r] = sum_k W2(k, j) Yp(r, k)
-        const int nrb = nt >> 4;
-        for (int blk = w; blk < 2 * nrb; blk += 4) {
-            const int rb = blk >> 1, j0 = 16 * (blk & 1);
+        // tail rows: X(r, j) -= sum_k Yp(r, k) W2(k, j), as D'[j][r] = sum_k W2(k, j) Yp(r, k).
+        // Wave w: columns j0 = 16 (w & 1), row blocks (w >> 1) + 2 m; the next
+        // block's operands are loaded before the current block's MFMAs.
+        const int nrb = nt >> 4, j0 = 16 * (w & 1);
+        double wa[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) wa[k] = sW2[4 * k + (l >> 4)][j0 + (l & 15)];
+        const double *yp = tail + (size_t)c0 * ldb + (size_t)(l >> 4) * ldb + (l & 15);
+        double *xp = tail + (size_t)(xc + j0 + (l >> 4)) * ldb + (l & 15);
+        int rb = w >> 1;
+        double yb[8], xv[4];
+        if (rb < nrb) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) yb[k] = yp[(size_t)4 * k * ldb + 16 * rb];
+#pragma unroll
+            for (int v = 0; v < 4; v++) xv[v] = xp[(size_t)4 * v * ldb + 16 * rb];
+        }
+        for (; rb < nrb; rb += 2) {
             qd4 d = {0.0, 0.0, 0.0, 0.0};
-            const double *yp = tail + (size_t)16 * rb + (l & 15);
 #pragma unroll
-            for (int k0 = 0; k0 < QP; k0 += 4) {
-                const int k = k0 + (l >> 4);
-                d = qmfma(sW2[k][j0 + (l & 15)], yp[(size_t)(c0 + k) * ldb], d);
+            for (int k = 0; k < 8; k++) d = qmfma(wa[k], yb[k], d);
+            double xo[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) xo[v] = xv[v] - d[v];
+            const int rn = rb + 2;
+            if (rn < nrb) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) yb[k] = yp[(size_t)4 * k * ldb + 16 * rn];
+#pragma unroll
+                for (int v = 0; v < 4; v++) xv[v] = xp[(size_t)4 * v * ldb + 16 * rn];
             }
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int jj = j0 + (l >> 4) + 4 * v;
-                tail[(size_t)(xc + jj) * ldb + 16 * rb + (l & 15)] -= d[v];
-            }
+            for (int v = 0; v < 4; v++) xp[(size_t)4 * v * ldb + 16 * rb] = xo[v];
         }
         return;
     }
@@ -478,18 +544,42 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
         const int ir = 16 * wi + c;                      // row inside chunk q
         const double *trow = T + (size_t)(QP * q + ir);
         const double *gcol = G + (size_t)(cq + 16 * wj + c) * ldb + 4 * kk;
-        for (int k0 = QP * q; k0 < cq; k0 += 16) {
-            const qd4 gv = *(const qd4 *)(gcol + k0);
+        const int kb = QP * q, m = (cq - kb) >> 4;     // 16-column groups of T's row chunk
+        constexpr int D = 2;
+        qd4 gb[D];
+        double tb[D][4];
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int kc = k0 + 4 * kk + s;                 // column of T
-                double tv = trow[(size_t)kc * ldb];
-                if (kc < QP * q + ir) tv = 0.0;                 // lower part of the diagonal block
-                acc = qmfma(tv, gv[s], acc);
+        for (int u = 0; u < D; u++)
+            if (u < m) {
+                gb[u] = *(const qd4 *)(gcol + kb + 16 * u);
+#pragma unroll
+                for (int s = 0; s < 4; s++) tb[u][s] = trow[(size_t)(kb + 16 * u + 4 * kk + s) * ldb];
+            }
+        for (int r = 0; r < m; r += D) {
+#pragma unroll
+            for (int u = 0; u < D; u++) {
+                if (r + u < m) {
+                    const int k0 = kb + 16 * (r + u);
+                    const qd4 gv = gb[u];
+                    double tv[4];
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        tv[s] = tb[u][s];
+                        if (k0 + 4 * kk + s < kb + ir) tv[s] = 0.0;   // lower part of the diagonal block
+                    }
+                    if (r + u + D < m) {
+                        const int kn = k0 + 16 * D;
+                        gb[u] = *(const qd4 *)(gcol + kn);
+#pragma unroll
+                        for (int s = 0; s < 4; s++) tb[u][s] = trow[(size_t)(kn + 4 * kk + s) * ldb];
+                    }
+#pragma unroll
+                    for (int s = 0; s < 4; s++) acc = qmfma(tv[s], gv[s], acc);
+                }
             }
         }
     }
-    // acc: lane l, v -> Z(i = 16 wi + l % 16 ... ) -- operand A was T rows: D[i][j], i = l / 16 + 4 v
+    // D layout: lane l, value v -> Z(16 wi + l / 16 + 4 v, 16 wj + l % 16)
 #pragma unroll
     for (int v = 0; v < 4; v++) sW[16 * wi + (l >> 4) + 4 * v][16 * wj + (l & 15)] = acc[v];
     for (int e = t; e < QP * QP; e += QA_THREADS) {

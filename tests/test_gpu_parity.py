@@ -187,6 +187,24 @@ def test_qr(hip, orc, N, b):
                 assert rel(np.triu(t), np.triu(tr)) < TOL, (m, k)
 
 
+@pytest.mark.parametrize("N,b", [(1024, 128), (2048, 256)])
+def test_qr_schedules_bitwise(hip, orc, N, b):
+    """The two-stream QR schedule (panel chain beside the updates), the
+    in-order single-stream schedule and its graph replay give the same bits
+    in A and in the T factors S."""
+    a = orc.gen("dense", N, b, seed=5, shift=0.0)
+    outs = []
+    for kw in [{}, {"no_lookahead": True}, {"no_lookahead": True, "no_graph": True}, {}]:
+        x, s = a.copy(), np.zeros(N * N)
+        hip.qr(x, s, b=b, **kw)
+        outs.append((x, s))
+    for x, s in outs[1:]:
+        assert np.array_equal(x.view(np.uint64), outs[0][0].view(np.uint64))
+        assert np.array_equal(s.view(np.uint64), outs[0][1].view(np.uint64))
+    ref, sref = orc.qr(a, N, b)
+    assert rel(outs[0][0], ref) < TOL
+
+
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
 def test_graph_and_streams_bitwise(hip, orc, algo):
     """Graph replay, stream-ordered launches and the no-look-ahead schedule give
