@@ -48,7 +48,12 @@ struct Ctx {
                                     // the reserved CUs (s_main when nothing is reserved)
     hipStream_t s_aux = nullptr;    // chain schedule with the split update: the row/column
                                     // TRSMs and panel packing (CU mask of s_bulk)
-    hipStream_t s_qr[3] = {};       // QR: further panel-chain streams (steps k mod 4 = 1..3)
+    hipStream_t s_qr[8] = {};       // QR panel-chain streams (all CUs)
+    hipStream_t s_qrb[8] = {};      // QR update streams: CU-masked away from the first qr_reserve CUs
+    int qr_streams = 8;             // QR: panel chains / update streams in use (1..8)
+    int qr_reserve = 0;             // QR: CUs kept for the panel chains
+    bool qr_up = false;             // s_qr / s_qrb exist (created by the first QR factorization)
+    int ncu = 0;
     int reserve = 0;                // CUs kept free of bulk work for the diagonal chain
     int dag = 2;                    // 2: diagonal chain + split update, 1: diagonal chain,
                                     // 0: row look-ahead
@@ -141,11 +146,11 @@ void drop_graphs()
     g.graphs.clear();
 }
 
-constexpr int QR_CHAINS = 4;   // panel-chain streams of the QR schedule (step k on chain k % 4)
-constexpr int QR_BULK = 2;     // update streams (step k on bulk k % 2)
+constexpr int QR_MAXS = 8;     // most panel-chain (and update) streams of the QR schedule
 
 int ensure_ws(int b, int MT)
 {
+    const int QR_CHAINS = g.qr_streams, QR_BULK = g.qr_streams;
     const size_t need = (size_t)QR_BULK * b * b * (size_t)(MT > 1 ? MT : 1);
     if (g.w_cap < need || g.tau_cap < (size_t)QR_CHAINS * b) drop_graphs();   // captured graphs hold the old pointers
     if (g.w_cap < need) {
@@ -405,7 +410,7 @@ hipError_t getrf_tile(double *tile, int b, double *dU, double *dL, hipStream_t s
 // k done (split update: its first launch), 2MT+2 fork, 2MT+3 graph gate,
 // [2MT+4, 3MT+5) chain TRSM k done, [3MT+6, 4MT+7) row/column TRSM and packing
 // of step k done, 4MT+7 end of the bulk stream
-size_t events_needed(int MT) { return 4 * (size_t)MT + 8; }
+size_t events_needed(int MT) { return 4 * (size_t)MT + 24; }   // QR: 2 MT + 3 + 2 * 8 at most
 
 GemmArgs single_tile(const double *A, const double *B, double *C, int b, bool syrk)
 {
@@ -836,18 +841,49 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
     }
 }
 
+// The QR schedule's streams are created at the first QR factorization and
+// kept until palial_hip_finalize.  All of them carry a CU mask, which gives
+// each its own hardware queue (streams without one share the runtime's few
+// queues and would run one after the other): the panel chains may use every
+// CU, the update streams stay off the first qr_reserve CUs.
+int qr_streams_up()
+{
+    if (g.qr_up) return 0;
+    g.qr_up = true;   // qr_streams_down() releases whatever was created, also after a failure
+    const int words = (g.ncu + 31) / 32;
+    std::vector<uint32_t> all(words, 0), upd(words, 0);
+    for (int cu = 0; cu < g.ncu; cu++) all[cu / 32] |= 1u << (cu % 32);
+    for (int cu = g.qr_reserve; cu < g.ncu; cu++) upd[cu / 32] |= 1u << (cu % 32);
+    for (int i = 0; i < g.qr_streams; i++) {
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_qr[i], words, all.data()));
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_qrb[i], words, upd.data()));
+    }
+    return 0;
+}
+
+void qr_streams_down()
+{
+    if (!g.qr_up) return;
+    for (auto &q : g.s_qr)
+        if (q) { hipStreamDestroy(q); q = nullptr; }
+    for (auto &q : g.s_qrb)
+        if (q) { hipStreamDestroy(q); q = nullptr; }
+    g.qr_up = false;
+}
+
 // Flat-tree tiled QR, src/qr.h:18-190.
 // two == false: every task on s0 in the reference's loop order.
-// two == true: step k's panel chain (GEQRT(k), then TSQRT(k, m) in m order)
-// runs on chain stream k % 4 and its updates (ORMQR(k), then TSMQR(k, m) in
-// m order) on bulk stream k % 2, each stream with its own workspaces:
+// two == true: with ns = PALIAL_QR_STREAMS (default 8), step k's panel chain
+// (GEQRT(k), then TSQRT(k, m) in m order) runs on chain stream k % ns and its
+// updates (ORMQR(k), then TSMQR(k, m) in m order) on update stream k % ns,
+// each stream with its own workspaces:
 //   GEQRT(k) and TSQRT(k, m) wait for TSMQR(k-1, k) / TSMQR(k-1, m) (tile final),
 //   ORMQR(k) waits for GEQRT(k), TSMQR(k, m) for TSQRT(k, m).
 // Tasks that may run at the same time touch disjoint tiles (ORMQR(k) reads
 // only the unit-lower V of A(k,k) while TSQRT(k, m) rewrites its upper R;
 // TSMQR(k, m) and TSMQR(k+1, m') write rows {k, m} and {k+1, m'}, m' < m),
-// so up to four steps' panel chains progress together and the factors are
-// the same bits as in the reference's order.
+// so up to ns steps progress together and the factors are the same bits as
+// in the reference's order.
 void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1, bool two, Launcher &L)
 {
     L.b = b;
@@ -899,17 +935,19 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1
         }
         return;
     }
-    const hipStream_t chain[QR_CHAINS] = {s1, g.s_qr[0], g.s_qr[1], g.s_qr[2]};
-    const hipStream_t bulk[QR_BULK] = {g.s_bulk, g.s_aux};
+    (void)s1;
+    const hipStream_t *chain = g.s_qr, *bulk = g.s_qrb;
+    const int ns = g.qr_streams;
     hipEvent_t *E = g.events.data();                 // E[(k & 1) MT + m]: TSMQR(k, m) done
     hipEvent_t evG = g.events[2 * MT], evQ = g.events[2 * MT + 1], fork = g.events[2 * MT + 2];
-    hipEvent_t *join = g.events.data() + 2 * MT + 3;   // QR_CHAINS + QR_BULK events
+    hipEvent_t *join = g.events.data() + 2 * MT + 3;   // 2 ns events
     L.chk(hipEventRecord(fork, s0));
-    for (hipStream_t s : chain) L.chk(hipStreamWaitEvent(s, fork, 0));
-    for (hipStream_t s : bulk)
-        if (s != s0) L.chk(hipStreamWaitEvent(s, fork, 0));
+    for (int i = 0; i < ns; i++) {
+        L.chk(hipStreamWaitEvent(chain[i], fork, 0));
+        L.chk(hipStreamWaitEvent(bulk[i], fork, 0));
+    }
     for (int k = 0; k < MT; k++) {
-        const int c = k % QR_CHAINS, u = k % QR_BULK;
+        const int c = k % ns, u = k % ns;
         hipStream_t sc = chain[c], sb = bulk[u];
         if (k > 0) L.chk(hipStreamWaitEvent(sc, E[((k - 1) & 1) * MT + k], 0));
         geqrt(k, sc, c);
@@ -926,16 +964,12 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1
             L.chk(hipEventRecord(E[(k & 1) * MT + m], sb));
         }
     }
-    int nj = 0;
-    for (hipStream_t s : chain) {
-        L.chk(hipEventRecord(join[nj], s));
-        L.chk(hipStreamWaitEvent(s0, join[nj++], 0));
+    for (int i = 0; i < ns; i++) {
+        L.chk(hipEventRecord(join[2 * i], chain[i]));
+        L.chk(hipStreamWaitEvent(s0, join[2 * i], 0));
+        L.chk(hipEventRecord(join[2 * i + 1], bulk[i]));
+        L.chk(hipStreamWaitEvent(s0, join[2 * i + 1], 0));
     }
-    for (hipStream_t s : bulk)
-        if (s != s0) {
-            L.chk(hipEventRecord(join[nj], s));
-            L.chk(hipStreamWaitEvent(s0, join[nj++], 0));
-        }
 }
 
 // ------------------------------------------------------------------ solve
@@ -1312,6 +1346,11 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     hipMemsetAsync(g.d_info, 0, sizeof(int), s0);
     hipEventRecord(e1, s0);
 
+    if (qr_two && (rc = qr_streams_up())) {
+        qr_streams_down();
+        cleanup();
+        return rc;
+    }
     Launcher L;
     auto issue = [&](hipStream_t st) {
         if (algo == ALG_CHOL) dag_cholesky(dA, N, b, lookahead, st, s1, g.d_info, L);
@@ -1502,7 +1541,6 @@ int palial_hip_init(int device)
     g.f32work = fw ? atoi(fw) : 1;
     HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
-    for (auto &q : g.s_qr) HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
     if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
         const int ncu = prop.multiProcessorCount, words = (ncu + 31) / 32;
         std::vector<uint32_t> mu(words, 0);
@@ -1514,6 +1552,16 @@ int palial_hip_init(int device)
         g.s_bulk = g.s_main;
         HIPCHK(hipStreamCreateWithFlags(&g.s_aux, hipStreamNonBlocking));
     }
+    // QR schedule: PALIAL_QR_STREAMS panel chains and as many update streams;
+    // the update streams stay off the first PALIAL_QR_RESERVE CUs
+    const char *qs = getenv("PALIAL_QR_STREAMS");
+    g.qr_streams = qs ? atoi(qs) : 8;
+    if (g.qr_streams < 1) g.qr_streams = 1;
+    if (g.qr_streams > QR_MAXS) g.qr_streams = QR_MAXS;
+    const char *qres = getenv("PALIAL_QR_RESERVE");
+    g.qr_reserve = qres ? atoi(qres) : 32;
+    if (g.qr_reserve < 0 || g.qr_reserve >= prop.multiProcessorCount / 2) g.qr_reserve = 0;
+    g.ncu = prop.multiProcessorCount;
     HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
     panel_init();
     panel2_init();
@@ -1533,6 +1581,7 @@ int palial_hip_finalize(void)
     if (!g.init) return 0;
     hipDeviceSynchronize();
     drop_graphs();
+    qr_streams_down();
     for (auto e : g.events) hipEventDestroy(e);
     g.events.clear();
     if (g.d_w1) hipFree(g.d_w1);
@@ -1554,7 +1603,6 @@ int palial_hip_finalize(void)
     if (g.s_aux) hipStreamDestroy(g.s_aux);
     hipStreamDestroy(g.s_main);
     hipStreamDestroy(g.s_panel);
-    for (auto q : g.s_qr) if (q) hipStreamDestroy(q);
     g = Ctx();
     return 0;
 }

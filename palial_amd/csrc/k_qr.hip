@@ -249,137 +249,160 @@ __device__ inline double sum16(double v)
     return v;
 }
 
-// One barrier per column: after the update of step i, the threads of column
-// i+1 already hold that column's final values and build reflector i+1 (norm,
-// tau, scaled v into the other half of the double-buffered sv) while the rest
-// finish step i.  Column c of the diagonal T block is formed in step c+1 by
-// rows (thread group j computes T(j, c), its 16 lanes splitting the sum).
+__device__ inline double readlane_d(double v, int lane)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wave, the same bits in every lane.
+__device__ inline double wave_sum64(double v)
+{
+    v = sum16(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+// Thread layout: wave w owns panel columns w, w + 8, w + 16, w + 24; lane g
+// holds tail rows g + 64 s (s < 8) of them in registers, so that the
+// reflector's tail is read from LDS once per row set (8 values per thread)
+// and the branches on the column index are uniform per wave.
+// One barrier per column: after the update of step i, the wave holding
+// column i+1 builds reflector i+1 (norm, tau, scaled v into the other half of
+// the double-buffered sv) while the other waves finish step i.  Column c of
+// the diagonal T block is formed in step c+1, 16 lanes per row of T.
 template <bool GE>
 __global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double *A2, double *T, double *tau_out,
                                                               int b, int c0)
 {
-    constexpr int SVLD = 34;            // sv row stride: thread g's tail rows g + 16 s at [g * SVLD + s]
+    constexpr int SV = 10;              // sv stride per lane (80 B: b128 reads without bank conflicts)
     __shared__ double sH[QP][QP + 1];   // head rows: sH[h][c] = (head row h, panel column c)
     __shared__ double sG[QP][QP + 1];   // sG[q][i] = Y_q^T Y_i, q < i
     __shared__ double sT[QP][QP + 1];   // diagonal block of T (upper part)
-    __shared__ __align__(16) double sv[2][16 * SVLD];
+    __shared__ __align__(16) double sv[2][64 * SV];
     __shared__ double sTau[QP];
-    const int t = threadIdx.x, j = t >> 4, g = t & 15;
+    const int t = threadIdx.x, g = t & 63, w = t >> 6;
     const int nt = GE ? b - c0 - QP : b;
-    const int ns = nt >> 4;
     double *tail = GE ? A1 + c0 + QP : A2;
     double *head = A1 + c0;
-    const size_t cj = (size_t)(c0 + j) * b + g;
-    double a[32];
+    double a[4][8];
 #pragma unroll
-    for (int s = 0; s < 32; s++) a[s] = s < ns ? tail[cj + 16 * s] : 0.0;
+    for (int q = 0; q < 4; q++) {
+        const double *src = tail + (size_t)(c0 + w + 8 * q) * b;
+#pragma unroll
+        for (int s = 0; s < 8; s++) a[q][s] = g + 64 * s < nt ? src[g + 64 * s] : 0.0;
+    }
     for (int e = t; e < QP * QP; e += QP_THREADS) {
         const int h = e & 31, c = e >> 5;
         sH[h][c] = head[(size_t)(c0 + c) * b + h];
     }
     __syncthreads();
 
-    // reflector of column c, by the 16 threads of column c
+    // reflector of column c = w + 8 qc, by its wave
     auto reflect = [&](int c) {
-        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        const int qc = c >> 3;
 #pragma unroll
-        for (int s = 0; s < 32; s += 4) {
-            p0 += a[s] * a[s];
-            p1 += a[s + 1] * a[s + 1];
-            p2 += a[s + 2] * a[s + 2];
-            p3 += a[s + 3] * a[s + 3];
-        }
-        double p = (p0 + p1) + (p2 + p3);
-        if (GE) {
-            if (g > c) p += sH[g][c] * sH[g][c];
-            if (g + 16 > c) p += sH[g + 16][c] * sH[g + 16][c];
-        }
-        p = sum16(p);
-        const int own = c & 15;   // lane holding head row c
-        const double alpha = __shfl(g == own ? sH[c][c] : 0.0, (t & 48) | own);
-        double tau, beta, scal;
-        householder(alpha, p, tau, beta, scal);
-        if (g == own) {
-            sH[c][c] = beta;
-            sTau[c] = tau;
-            tau_out[c0 + c] = tau;
-        }
-        if (tau != 0.0) {
+        for (int q = 0; q < 4; q++) {
+            if (q != qc) continue;
+            double p0 = 0.0, p1 = 0.0;
 #pragma unroll
-            for (int s = 0; s < 32; s++) a[s] *= scal;
-            if (GE) {
-                if (g > c) sH[g][c] *= scal;
-                if (g + 16 > c) sH[g + 16][c] *= scal;
+            for (int s = 0; s < 8; s += 2) {
+                p0 += a[q][s] * a[q][s];
+                p1 += a[q][s + 1] * a[q][s + 1];
             }
-        }
-        double *w = sv[c & 1] + g * SVLD;
+            double p = p0 + p1;
+            if (GE && g < QP && g > c) p += sH[g][c] * sH[g][c];
+            p = wave_sum64(p);
+            const double alpha = readlane_d(g == c ? sH[c][c] : 0.0, c);
+            double tau, beta, scal;
+            householder(alpha, p, tau, beta, scal);
+            if (g == c) {
+                sH[c][c] = beta;
+                sTau[c] = tau;
+                tau_out[c0 + c] = tau;
+            }
+            if (tau != 0.0) {
 #pragma unroll
-        for (int s = 0; s < 32; s += 2)
-            if (s < ns) *(double2 *)(w + s) = double2{a[s], a[s + 1]};
+                for (int s = 0; s < 8; s++) a[q][s] *= scal;
+                if (GE && g < QP && g > c) sH[g][c] *= scal;
+            }
+            double *dst = sv[c & 1] + g * SV;
+#pragma unroll
+            for (int s = 0; s < 8; s += 2) *(double2 *)(dst + s) = double2{a[q][s], a[q][s + 1]};
+        }
     };
-    // T(j, c) for the thread's row j <= c (G(:, c) and T(j, 0:c) complete)
+    // T(jr, c) for rows jr <= c (G(:, c) and T(jr, 0:c) complete); 16 lanes per row
     auto tcol = [&](int c) {
-        if (j < c) {
+        const int jr = 4 * w + (g >> 4), gl = g & 15;
+        if (jr < c) {
             double s = 0.0;
-            for (int q = j + g; q < c; q += 16) s += sT[j][q] * sG[q][c];
+            for (int q = jr + gl; q < c; q += 16) s += sT[jr][q] * sG[q][c];
             s = sum16(s);
             const double tc = sTau[c];
-            if (g == 0) sT[j][c] = tc == 0.0 ? 0.0 : -tc * s;
-        } else if (j == c && g == 0) {
+            if (gl == 0) sT[jr][c] = tc == 0.0 ? 0.0 : -tc * s;
+        } else if (jr == c && gl == 0) {
             sT[c][c] = sTau[c];
         }
     };
 
-    if (j == 0) reflect(0);
+    if (w == 0) reflect(0);
     for (int i = 0; i < QP; i++) {
         __syncthreads();
         const double tau = sTau[i];
         if (i > 0) tcol(i - 1);
-        if (j != i) {
-            const double *vp = sv[i & 1] + g * SVLD;
-            double v[32];
+        const double *vp = sv[i & 1] + g * SV;
+        double v[8];
 #pragma unroll
-            for (int s = 0; s < 32; s += 2) {
-                double2 x = s < ns ? *(const double2 *)(vp + s) : double2{0.0, 0.0};
-                v[s] = x.x;
-                v[s + 1] = x.y;
-            }
-            double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+        for (int s = 0; s < 8; s += 2) {
+            const double2 x = *(const double2 *)(vp + s);
+            v[s] = x.x;
+            v[s + 1] = x.y;
+        }
+        double d[4];
 #pragma unroll
-            for (int s = 0; s < 32; s += 4) {
-                d0 += v[s] * a[s];
-                d1 += v[s + 1] * a[s + 1];
-                d2 += v[s + 2] * a[s + 2];
-                d3 += v[s + 3] * a[s + 3];
-            }
-            double d = (d0 + d1) + (d2 + d3);
-            double vh0 = 0.0, vh1 = 0.0;
-            if (GE) {
-                if (g > i) { vh0 = sH[g][i]; d += vh0 * sH[g][j]; }
-                if (g + 16 > i) { vh1 = sH[g + 16][i]; d += vh1 * sH[g + 16][j]; }
-            }
-            d = sum16(d);
-            if (j > i) {
-                const double f = tau * (sH[i][j] + d);
+        for (int q = 0; q < 4; q++) {
+            double d0 = 0.0, d1 = 0.0;
 #pragma unroll
-                for (int s = 0; s < 32; s++) a[s] -= f * v[s];
-                if (GE) {
-                    if (g > i) sH[g][j] -= f * vh0;
-                    if (g + 16 > i) sH[g + 16][j] -= f * vh1;
-                }
-                if (g == 0) sH[i][j] -= f;
-                if (j == i + 1) reflect(i + 1);
+            for (int s = 0; s < 8; s += 2) {
+                d0 += v[s] * a[q][s];
+                d1 += v[s + 1] * a[q][s + 1];
+            }
+            d[q] = d0 + d1;
+        }
+        const bool hrow = GE && g < QP && g > i;   // head row of reflector i below its 1
+        const double vh = hrow ? sH[g][i] : 0.0;
+        if (hrow) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) d[q] += vh * sH[g][w + 8 * q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int jc = w + 8 * q;
+            if (jc == i) continue;
+            const double dq = wave_sum64(d[q]);
+            if (jc > i) {
+                const double f = tau * (sH[i][jc] + dq);
+#pragma unroll
+                for (int s = 0; s < 8; s++) a[q][s] -= f * v[s];
+                if (hrow) sH[g][jc] -= f * vh;
+                if (g == 0) sH[i][jc] -= f;
             } else if (g == 0) {
-                sG[j][i] = GE ? sH[i][j] + d : d;
+                sG[jc][i] = GE ? sH[i][jc] + dq : dq;
             }
         }
+        if (i + 1 < QP && w == ((i + 1) & 7)) reflect(i + 1);
     }
     __syncthreads();
     tcol(QP - 1);
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 32; s++)
-        if (s < ns) tail[cj + 16 * s] = a[s];
+    for (int q = 0; q < 4; q++) {
+        double *dst = tail + (size_t)(c0 + w + 8 * q) * b;
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+            if (g + 64 * s < nt) dst[g + 64 * s] = a[q][s];
+    }
     for (int e = t; e < QP * QP; e += QP_THREADS) {
         const int r = e & 31, c = e >> 5;
         if (GE || r <= c) head[(size_t)(c0 + c) * b + r] = sH[r][c];
