@@ -133,7 +133,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
 
 
 def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16),
-              tiers=(0, 1, 2, 4), solve_n=32768):
+              tiers=(0, 1, 2, 4), solve_n=32768, algos=("cholesky", "lu", "qr"), solves=True):
     """Survey configs 2 and 3 beside the headline (opt-in, --sweep):
     FP64 factorizations of Cholesky (S + N I), LU (G + N I) and QR (G) at
     b = 512 for each N (device time of one factorization after a warm-up),
@@ -141,7 +141,7 @@ def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), 
     import ctypes as C
     import numpy as np
     fac = {}
-    for algo in ("cholesky", "lu", "qr"):
+    for algo in algos:
         flops_f = {"cholesky": 1 / 3, "lu": 2 / 3, "qr": 4 / 3}[algo]
         for N in ns:
             A0, A = P.DeviceMatrix(N, b), P.DeviceMatrix(N, b)
@@ -162,6 +162,8 @@ def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), 
             if S is not None:
                 S.free()
     sol = {}
+    if not solves:
+        return {"factorizations_fp64_b512": fac}
     N = solve_n
     A = P.DeviceMatrix(N, b)
     L = P.lib()
@@ -276,6 +278,9 @@ def main():
     ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")
     ap.add_argument("--sweep", action="store_true",
                     help="only the survey's size / condition-number sweep (one JSON line with key 'sweep')")
+    ap.add_argument("--sweep-algos", default="cholesky,lu,qr", help="factorizations of the sweep")
+    ap.add_argument("--sweep-ns", default="8192,16384,24576,32768", help="matrix sizes of the sweep")
+    ap.add_argument("--sweep-solves", type=int, default=1, help="0: skip the sweep's Cholesky solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -284,7 +289,9 @@ def main():
     import palial_amd as P
     P.init(0)
     if args.sweep:
-        print(json.dumps({"sweep": run_sweep(P, b=args.b)}), flush=True)
+        sw = run_sweep(P, b=args.b, ns=[int(x) for x in args.sweep_ns.split(",")],
+                       algos=args.sweep_algos.split(","), solves=bool(args.sweep_solves))
+        print(json.dumps({"sweep": sw}), flush=True)
         P.finalize()
         return 0
     algos = args.algos.split(",")

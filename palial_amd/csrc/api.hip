@@ -1565,6 +1565,7 @@ int palial_hip_init(int device)
     HIPCHK(hipMalloc(&g.d_info, sizeof(int)));
     panel_init();
     panel2_init();
+    solve_init();
     mlp_init();
     {
         hipError_t e = hipGetLastError();   // attribute calls must not leave a sticky error

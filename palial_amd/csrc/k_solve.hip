@@ -36,7 +36,7 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 {
     constexpr int G = GV_THREADS / ROWS;      // column groups (op = N)
     extern __shared__ double sm[];
-    double *sv = sm;                          // b x NR input segment: sv[c + j*b]
+    double *sv = sm;                          // b x NR input segment (layout by op, below)
     double *sred = sm + (size_t)p.b * NR;     // G groups x ROWS rows x NR partial sums
     const int b = p.b;
     const int rb_per_tile = b / ROWS;
@@ -57,9 +57,12 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
     }
     const double *vi = p.vin + (size_t)vin * b;
     double *vo = p.vout + (size_t)vout * b;
+    // op = N reads the NR values of one row together: sv[i * NR + c]; op = T
+    // reads one right-hand side across the lanes: sv[i + c * b]
     for (int e = t; e < b * NR; e += GV_THREADS) {
         const int i = e % b, c = e / b;
-        sv[e] = (c < p.nrhs) ? vi[(size_t)i + (size_t)c * p.ld] : 0.0;   // padded right-hand sides are zero
+        const double x = (c < p.nrhs) ? vi[(size_t)i + (size_t)c * p.ld] : 0.0;   // padded right-hand sides are zero
+        sv[p.trans ? e : i * NR + c] = x;
     }
     __syncthreads();
     const int r0 = rb * ROWS;
@@ -72,8 +75,9 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 #pragma unroll 8
         for (int c = g; c < b; c += G) {
             const double a = (double)Tr[(size_t)c * b];
+            const double *svc = sv + c * NR;
 #pragma unroll
-            for (int j = 0; j < NR; j++) acc[j] += a * sv[c + j * b];
+            for (int j = 0; j < NR; j++) acc[j] += a * svc[j];
         }
 #pragma unroll
         for (int j = 0; j < NR; j++) sred[(g * ROWS + r) * NR + j] = acc[j];
@@ -103,23 +107,39 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 #pragma unroll
                 for (int j = 0; j < NR; j++) part[x][j] += a[x] * sv[c + j * b];
         }
+        // Sum the 4 NR values over the 64 lanes.  Butterfly steps on the high
+        // lane bits first (every lane keeps all values), then halving steps:
+        // at offset o a lane keeps one half of its values (the upper half when
+        // its bit o is set), sends the other half to lane ^ o and adds what it
+        // receives, so that in the end lane L holds the sum of value L mod 4 NR.
+        constexpr int NV = 4 * NR;
+        constexpr int NH = NV == 64 ? 6 : NV == 16 ? 4 : 2;    // halving steps: log2(NV)
+        double v[NV];
 #pragma unroll
         for (int x = 0; x < 4; x++)
 #pragma unroll
-            for (int j = 0; j < NR; j++) {
-                double v = part[x][j];
+            for (int j = 0; j < NR; j++) v[x * NR + j] = part[x][j];
 #pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-                part[x][j] = v;
+        for (int o = 32; o > (1 << (NH - 1)); o >>= 1)
+#pragma unroll
+            for (int i = 0; i < NV; i++) v[i] += __shfl_xor(v[i], o);
+        int n = NV;
+#pragma unroll
+        for (int o = 1 << (NH - 1); o >= 1; o >>= 1) {
+            const bool up = lane & o;
+            n >>= 1;
+#pragma unroll
+            for (int i = 0; i < NV / 2; i++) {
+                if (i < n) {
+                    const double send = up ? v[i] : v[i + n];
+                    const double keep = up ? v[i + n] : v[i];
+                    v[i] = keep + __shfl_xor(send, o);
+                }
             }
-        if (lane < 4 * NR && lane % NR < p.nrhs) {
+        }
+        if (lane < NV && lane % NR < p.nrhs) {
             const int x = lane / NR, j = lane % NR;
-            double s = 0.0;
-#pragma unroll
-            for (int xx = 0; xx < 4; xx++)
-#pragma unroll
-                for (int jj = 0; jj < NR; jj++)
-                    if (xx == x && jj == j) s = part[xx][jj];
+            const double s = v[0];
             double *o = vo + (size_t)(r0 + wv * 4 + x) + (size_t)j * p.ld;
             if (p.mode == GV_DIAG) *o = s;
             else *o -= s;
@@ -186,49 +206,67 @@ hipError_t launch_to_f32(const double *src, float *dst, int N, int b, bool upper
 // NORM additionally accumulates the row sums of |A| (for ||A||_inf) in the
 // same pass over A.
 //
-// General matrices: a workgroup covers 64 rows of one tile row and a chunk
-// of RES_CHUNK tile columns; slot = chunk.
+// General matrices: wave w of a workgroup owns the 64 rows of row block
+// 4 g + w (g = the workgroup's row group) and runs over a chunk of RES_CHUNK
+// tile columns; slot = chunk.  The x segment of each tile column is staged in
+// LDS once for the four waves, row c of it (the NR right-hand sides) stored
+// contiguously, so that a column of A costs one coalesced load per lane and
+// NR / 2 broadcast 16-byte LDS reads.  xt: x transposed to [row][NR] (NR > 1).
 constexpr int RES_CHUNK = 8;
+
+template <int NR>
+__global__ void residual_xt_kernel(const double *x, int N, int ld, int nrhs, double *xt)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)N * NR;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e / NR;
+        const int j = (int)(e % NR);
+        xt[e] = j < nrhs ? x[i + (size_t)j * ld] : 0.0;   // padded right-hand sides are zero
+    }
+}
 
 template <int NR, bool NORM>
 __global__ __launch_bounds__(256) void residual_partial_kernel(const double *A, int N, int bt, const double *x,
-                                                               int ld, int nrhs, double *part, double *partn)
+                                                               int nrhs, double *part, double *partn)
 {
-    __shared__ double sred[4][64 * NR];
-    __shared__ double snrm[4][64];
+    constexpr int SX = NR == 1 ? 1 : NR + 2;   // LDS row stride (16-byte aligned, 4-way write conflicts)
+    extern __shared__ double sx[];             // bt x SX
     const int MT = N / bt;
-    const int rows_per = bt / 64;
     const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
-    const int rowblk = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
-    const int m = rowblk / rows_per, r0 = (rowblk % rows_per) * 64;
-    const int t = threadIdx.x, rr = t % 64, q = t / 64;
+    const int rg = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int rowblk = rg * 4 + w;
+    const bool active = rowblk < N / 64;
+    const int rpt = bt / 64;
+    const int m = rowblk / rpt, r0 = (rowblk % rpt) * 64;
     double acc[NR], nacc = 0.0;
 #pragma unroll
     for (int j = 0; j < NR; j++) acc[j] = 0.0;
     const int n1 = min(MT, (chunk + 1) * RES_CHUNK);
     for (int n = chunk * RES_CHUNK; n < n1; n++) {
-        const double *T = A + tile_off(bt, MT, m, n) + r0 + rr;
-        const double *xs = x + (size_t)n * bt;
+        __syncthreads();   // every wave is done with the previous segment
+        const double *xs = x + (size_t)n * bt * NR;
+        for (int e = t; e < bt * NR; e += 256) sx[(e / NR) * SX + e % NR] = xs[e];
+        __syncthreads();
+        if (active) {
+            const double *T = A + tile_off(bt, MT, m, n) + r0 + lane;
 #pragma unroll 8
-        for (int c = q; c < bt; c += 4) {
-            const double a = T[(size_t)c * bt];
-            if (NORM) nacc += fabs(a);
+            for (int c = 0; c < bt; c++) {
+                const double a = T[(size_t)c * bt];
+                if (NORM) nacc += fabs(a);
+                const double *xc = sx + c * SX;
 #pragma unroll
-            for (int j = 0; j < NR; j++)
-                if (j < nrhs) acc[j] += a * xs[(size_t)c + (size_t)j * ld];
+                for (int j = 0; j < NR; j++) acc[j] += a * xc[j];
+            }
         }
     }
+    if (active) {
+        const size_t row = (size_t)m * bt + r0 + lane;
 #pragma unroll
-    for (int j = 0; j < NR; j++) sred[q][rr + j * 64] = acc[j];
-    if (NORM) snrm[q][rr] = nacc;
-    __syncthreads();
-    for (int e = t; e < 64 * nrhs; e += 256) {
-        const int i = e % 64, j = e / 64;
-        // part layout: [slot][j][row]
-        part[((size_t)chunk * NR + j) * N + (size_t)m * bt + r0 + i] = sred[0][e] + sred[1][e] + sred[2][e] + sred[3][e];
+        for (int j = 0; j < NR; j++)
+            if (j < nrhs) part[((size_t)chunk * NR + j) * N + row] = acc[j];
+        if (NORM) partn[(size_t)chunk * N + row] = nacc;
     }
-    if (NORM && t < 64)
-        partn[(size_t)chunk * N + (size_t)m * bt + r0 + t] = snrm[0][t] + snrm[1][t] + snrm[2][t] + snrm[3][t];
 }
 
 // Symmetric A given by its upper tiles (the Cholesky solves; one right-hand
@@ -347,28 +385,43 @@ hipError_t launch_residual(const double *A, int N, int b, const double *x, const
     }
     if (sym) return hipErrorInvalidValue;   // symmetric residual: one right-hand side
     const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
-    dim3 grid((unsigned)(MT * (b / 64) * nchunk)), block(256);
+    const int nrg = (N / 64 + 3) / 4;
+    dim3 grid((unsigned)(nrg * nchunk)), block(256);
     const int NRk = (nrhs == 1) ? 1 : 16;
     double *partn = part + (size_t)nchunk * NRk * N;
     if (nrhs == 1) {
-        if (rowsum) hipLaunchKernelGGL((residual_partial_kernel<1, true>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
-        else hipLaunchKernelGGL((residual_partial_kernel<1, false>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
+        const size_t lds = (size_t)b * 8;
+        if (rowsum) hipLaunchKernelGGL((residual_partial_kernel<1, true>), grid, block, lds, s, A, N, b, x, nrhs, part, partn);
+        else hipLaunchKernelGGL((residual_partial_kernel<1, false>), grid, block, lds, s, A, N, b, x, nrhs, part, partn);
         hipLaunchKernelGGL(residual_reduce_kernel<1>, dim3(1024), dim3(256), 0, s, part, partn, nchunk, rhs, r, rowsum,
                            N, ld, nrhs);
     } else {
-        if (rowsum) hipLaunchKernelGGL((residual_partial_kernel<16, true>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
-        else hipLaunchKernelGGL((residual_partial_kernel<16, false>), grid, block, 0, s, A, N, b, x, ld, nrhs, part, partn);
+        double *xt = partn + (size_t)nchunk * N;
+        const size_t lds = (size_t)b * 18 * 8;
+        hipLaunchKernelGGL(residual_xt_kernel<16>, dim3(512), dim3(256), 0, s, x, N, ld, nrhs, xt);
+        if (rowsum) hipLaunchKernelGGL((residual_partial_kernel<16, true>), grid, block, lds, s, A, N, b, xt, nrhs, part, partn);
+        else hipLaunchKernelGGL((residual_partial_kernel<16, false>), grid, block, lds, s, A, N, b, xt, nrhs, part, partn);
         hipLaunchKernelGGL(residual_reduce_kernel<16>, dim3(1024), dim3(256), 0, s, part, partn, nchunk, rhs, r,
                            rowsum, N, ld, nrhs);
     }
     return hipGetLastError();
 }
 
+void solve_init()
+{
+    // the 16-right-hand-side residual stages b x 18 doubles (74 KB at b = 512)
+    hipFuncSetAttribute((const void *)residual_partial_kernel<16, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        512 * 18 * 8);
+    hipFuncSetAttribute((const void *)residual_partial_kernel<16, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        512 * 18 * 8);
+}
+
+// device scratch (doubles) launch_residual needs in `part`
 size_t residual_scratch(int N, int b, int nrhs)
 {
     const int MT = N / b;
     const int nchunk = (MT + RES_CHUNK - 1) / RES_CHUNK;
-    const size_t chunked = (size_t)nchunk * ((nrhs == 1 ? 1 : 16) + 1) * N;
+    const size_t chunked = (size_t)nchunk * ((nrhs == 1 ? 1 : 16) + 1) * N + (nrhs == 1 ? 0 : (size_t)16 * N);
     const size_t sym = 2 * (size_t)MT * N;
     return chunked > sym ? chunked : sym;
 }
