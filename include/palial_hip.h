@@ -68,7 +68,10 @@ typedef struct palial_opts {
     int   max_iter;       /* solves: refinement steps (0 = 30) */
     double tol;           /* solves: target backward error (0 = 1e-14) */
     int   factor_tile;    /* solves: tile size of the internal factorization (0 = automatic) */
-    int   reserved[7];
+    int   no_fallback;    /* solves, mixed tiers: nonzero reports a breakdown or a refinement that
+                             does not converge as an error; zero (default) retries a broken-down
+                             Cholesky with a diagonal shift (then GMRES-IR) and finally solves in FP64 */
+    int   reserved[6];
 } palial_opts;
 
 /* per-call report; any pointer may be NULL */
@@ -85,7 +88,9 @@ typedef struct palial_report {
     long   ntasks;        /* tile tasks of the reference DAG this call covers */
     long   nlaunches;     /* kernel launches issued (or graph nodes replayed) */
     int    gmres_iterations; /* solves: GMRES inner iterations over all GMRES-IR steps */
-    int    reserved[7];
+    int    precision_used;   /* solves: tier of the factorization that produced the solution */
+    int    shifts;           /* solves: diagonal shifts applied after a low-precision breakdown */
+    int    reserved[5];
 } palial_report;
 
 /* ---------------------------------------------------------------- lifecycle */

@@ -42,7 +42,8 @@ class MATRIX_desc(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("device_matrix", C.c_int), ("precision", C.c_int), ("no_graph", C.c_int),
                 ("no_lookahead", C.c_int), ("stream", C.c_void_p), ("refine", C.c_int),
-                ("max_iter", C.c_int), ("tol", C.c_double), ("factor_tile", C.c_int), ("reserved", C.c_int * 7)]
+                ("max_iter", C.c_int), ("tol", C.c_double), ("factor_tile", C.c_int), ("no_fallback", C.c_int),
+                ("reserved", C.c_int * 6)]
 
 
 class Report(C.Structure):
@@ -50,7 +51,7 @@ class Report(C.Structure):
                 ("d2h_ms", C.c_double), ("gflop", C.c_double), ("tflops", C.c_double),
                 ("backward_error", C.c_double), ("info", C.c_int), ("iterations", C.c_int),
                 ("ntasks", C.c_long), ("nlaunches", C.c_long), ("gmres_iterations", C.c_int),
-                ("reserved", C.c_int * 7)]
+                ("precision_used", C.c_int), ("shifts", C.c_int), ("reserved", C.c_int * 5)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -139,9 +140,10 @@ def finalize():
 
 
 def _opts(device=False, no_graph=False, no_lookahead=False, precision=PREC_FP64, stream=None,
-          refine=0, max_iter=0, tol=0.0, factor_tile=0):
+          refine=0, max_iter=0, tol=0.0, factor_tile=0, no_fallback=False):
     o = Opts()
     o.factor_tile = factor_tile
+    o.no_fallback = int(bool(no_fallback))
     o.device_matrix = int(bool(device))
     o.no_graph = int(bool(no_graph))
     o.no_lookahead = int(bool(no_lookahead))
@@ -463,7 +465,7 @@ def factor_device(algo, dm, S=None, **kw):
 
 # ------------------------------------------------------------------ solve
 def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, max_iter=0, device=False,
-          check=True, factor_tile=0, refine=0):
+          check=True, factor_tile=0, refine=0, no_fallback=False):
     """Solve A X = B with iterative refinement (no reference counterpart).
 
     A: layout-L matrix (numpy flat array, torch tensor or DeviceMatrix), left
@@ -475,7 +477,7 @@ def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, m
     else:
         d, dev, keep = _desc_of(A, N, b)
     o = _opts(device=bool(dev), precision=precision, tol=tol, max_iter=max_iter, factor_tile=factor_tile,
-              refine=refine)
+              refine=refine, no_fallback=no_fallback)
     rep = Report()
     code = {"cholesky": 0, "lu": 1}[algo]
     if dev and not isinstance(B, np.ndarray):

@@ -1085,20 +1085,9 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     };
     Launcher L;
     HIPCHK(hipEventRecord(e0, s));
-    // factorization of a copy (A stays FP64 and intact for the residuals)
+    // factorization of a copy (A stays FP64 and intact for the residuals),
+    // with sigma |A(i,i)| added to its diagonal when sigma > 0
     double *Fw = fs ? (double *)(void *)F32 : g.d_F;      // the matrix the DAG factors
-    if (fs) {
-        // FP32 copy (Cholesky: upper tiles only, the ones the factorization reads)
-        if (b != bA) L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
-        L.chk(launch_to_f32(b == bA ? dA : g.d_F, F32, N, b, algo == 0, 0, s));
-    } else if (b == bA) {
-        L.chk(launch_copy(g.d_F, dA, nn * 8, s));
-    } else {
-        L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
-    }
-    hipMemsetAsync(g.d_info, 0, sizeof(int), s);
-    if (algo == 0) dag_cholesky(Fw, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision, fs);
-    else dag_lu(Fw, N, b, MT > 1, s, g.s_panel, L, o.precision, fs);
     // inverses of the diagonal tiles: one batched strip TRSM per factor with the
     // identity as right-hand side (Cholesky: U^-T; LU: U^-1 and L^-1)
     auto inverses = [&](int kind, double *out, const double *dbase) {
@@ -1108,23 +1097,74 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         t.batch = 1; t.ident = 1; t.dinv_stride = 64L * b; t.f32 = fs ? 1 : 0;
         L.chk(launch_trsm2(kind, t, dbase, s));
     };
-    if (algo == 0) {
-        inverses(TR_CHOL, Uinv, g.d_dinv);
-    } else {
-        inverses(TR_LUR, Uinv, g.d_dinv);
-        inverses(TR_LUL, Linv, g.d_dinv + (size_t)32 * b);
+    auto factor_once = [&](double sigma) {
+        if (fs) {
+            // FP32 copy (Cholesky: upper tiles only, the ones the factorization reads)
+            if (b != bA) L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
+            L.chk(launch_to_f32(b == bA ? dA : g.d_F, F32, N, b, algo == 0, 0, s));
+        } else if (b == bA) {
+            L.chk(launch_copy(g.d_F, dA, nn * 8, s));
+        } else {
+            L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
+        }
+        if (sigma > 0.0) L.chk(launch_add_diag(Fw, fs, N, b, dA, bA, sigma, s));
+        hipMemsetAsync(g.d_info, 0, sizeof(int), s);
+        if (algo == 0) dag_cholesky(Fw, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision, fs);
+        else dag_lu(Fw, N, b, MT > 1, s, g.s_panel, L, o.precision, fs);
+        if (algo == 0) {
+            inverses(TR_CHOL, Uinv, g.d_dinv);
+        } else {
+            inverses(TR_LUR, Uinv, g.d_dinv);
+            inverses(TR_LUL, Linv, g.d_dinv + (size_t)32 * b);
+        }
+        if (f32) {
+            if (!fs) L.chk(launch_to_f32(g.d_F, F32, N, b, algo == 0, 0, s));
+            L.chk(launch_to_f32(Uinv, Inv32, 0, b, false, (size_t)(algo == 0 ? 1 : 2) * MT * b * b, s));
+        }
+    };
+    // FP64 solve in place of this one (a mixed tier that broke down or did not
+    // converge); its report adds the time spent here
+    const bool fallback_ok = o.precision > 0 && !o.no_fallback;
+    auto fallback = [&](int shifts_done) -> int {
+        HIPCHK(hipEventRecord(e2, s));
+        HIPCHK(hipStreamSynchronize(s));
+        float spent = 0;
+        hipEventElapsedTime(&spent, e0, e2);
+        done(0);
+        palial_opts o2 = o;
+        o2.precision = 0;
+        o2.factor_tile = 0;
+        palial_report r2;
+        memset(&r2, 0, sizeof r2);
+        const int rc2 = run_solve(algo, A, B, nrhs, ldb, &o2, &r2);
+        r2.factor_ms += spent;
+        r2.shifts = shifts_done;
+        if (rep) *rep = r2;
+        return rc2;
+    };
+    // unit roundoff of the tier's update operands: the diagonal shift that
+    // keeps a low-precision Cholesky from breaking down (Higham and Pranesh,
+    // SIAM J. Sci. Comput. 43(1), 2021) starts at 4 u and grows 16x per retry
+    const double u_tier = o.precision == 1 ? 0x1p-24 : o.precision == 2 ? 0x1p-8 : o.precision == 4 ? 0x1p-16 : 0.0;
+    int shifts = 0;
+    double sigma = 0.0;
+    int info = 0;
+    factor_once(0.0);
+    HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    while (info > 0 && fallback_ok && algo == 0 && shifts < 3) {
+        sigma = sigma == 0.0 ? 4.0 * u_tier : 16.0 * sigma;
+        shifts++;
+        factor_once(sigma);
+        HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
     }
-    if (f32) {
-        if (!fs) L.chk(launch_to_f32(g.d_F, F32, N, b, algo == 0, 0, s));
-        L.chk(launch_to_f32(Uinv, Inv32, 0, b, false, (size_t)(algo == 0 ? 1 : 2) * MT * b * b, s));
-    }
+    if (info > 0 && fallback_ok) return fallback(shifts);
     const void *sF = f32 ? (const void *)F32 : (const void *)g.d_F;
     const void *sU = f32 ? (const void *)Inv32 : (const void *)Uinv;
     const void *sL = f32 ? (const void *)(Inv32 + (size_t)MT * b * b) : (const void *)Linv;
     HIPCHK(hipEventRecord(e1, s));
     if (L.err != hipSuccess) return done(fail(PALIAL_ERR_HIP, "factorization launch: %s", hipGetErrorString(L.err)));
-    int info = 0;
-    HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
     // ||b||_inf now; ||A||_inf comes with the first residual (same pass over A)
     L.chk(launch_colmax(Bd, N, N, nrhs, red + 1, s));
     // x0 = F^{-1} b
@@ -1136,8 +1176,9 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     double hred[64];
     double normA = 0.0, normB[16];
     double eta = 0.0, eta_prev = 0.0;
-    int it = 0, stall = 0, gmres_total = 0;
-    int mode = (o.refine == 1) ? 1 : 0;      // 0 LU-IR, 1 GMRES-IR
+    int it = 0, stall = 0, gmres_total = 0, gm_slow = 0;
+    // 0 LU-IR, 1 GMRES-IR (always after a shift: F then factors A + sigma |diag A|)
+    int mode = (o.refine == 1 || (shifts > 0 && o.refine != 2)) ? 1 : 0;
     // GMRES workspace: Krylov basis V (N x (GM+1)), two vectors, a zero
     // right-hand side, the Hessenberg column on the device
     constexpr int GM = 30;
@@ -1247,6 +1288,13 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
             if (!(ej <= eta)) eta = ej;      // NaN propagates
         }
         if (eta <= tol || it >= max_iter) break;
+        // with the FP64 fallback allowed, GMRES-IR that stalls (two steps in a
+        // row that gain less than 4x) or runs long (more than one restart
+        // cycle per right-hand side) would cost more than the FP64 solve
+        if (fallback_ok && mode == 1 && it > 0) {
+            gm_slow = (eta <= 0.25 * eta_prev) ? 0 : gm_slow + 1;
+            if (gm_slow >= 2 || gmres_total > GM * nrhs) break;
+        }
         // auto: LU-IR until it fails twice in a row to halve eta, then GMRES-IR
         if (o.refine == 0 && mode == 0 && it > 0 && !(eta <= 0.5 * eta_prev)) {
             if (++stall >= 2) mode = 1;
@@ -1268,6 +1316,12 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         }
         L.chk(launch_axpy(X, D, N, N, nrhs, s));
     }
+    // a mixed tier that did not converge hands over to the FP64 solve before
+    // anything is written to B (the fallback reads B as the right-hand side)
+    if (!(eta <= tol) && fallback_ok) {
+        prof_collect();
+        return fallback(shifts);
+    }
     HIPCHK(hipEventRecord(e2, s));
     HIPCHK(hipMemcpy2DAsync(B, (size_t)ldb * 8, X, (size_t)N * 8, (size_t)N * 8, nrhs,
                             o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
@@ -1287,6 +1341,8 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         rep->gmres_iterations = gmres_total;
         rep->info = info;
         rep->nlaunches = L.launches;
+        rep->precision_used = o.precision;
+        rep->shifts = shifts;
     }
     if (!(eta <= tol)) return done(fail(PALIAL_ERR_NOCONV, "refinement stopped at backward error %.3e after %d steps", eta, it));
     return done(0);

@@ -164,6 +164,8 @@ hipError_t launch_residual(const double *A, int N, int b, const double *x, const
                            int nrhs, double *part, bool sym, double *rowsum, hipStream_t s);
 size_t residual_scratch(int N, int b, int nrhs);   // doubles of `part` launch_residual needs
 void solve_init();   // LDS limits of the solve kernels (call once, outside capture)
+// F(i,i) += sigma |A(i,i)|: F the working copy (tile b, FP32 when f32), A the FP64 matrix (tile bA)
+hipError_t launch_add_diag(void *F, bool f32, int N, int b, const double *A, int bA, double sigma, hipStream_t s);
 hipError_t launch_colmax(const double *v, int N, int ld, int nrhs, double *out, hipStream_t s);
 hipError_t launch_axpy(double *x, const double *d, int N, int ld, int nrhs, hipStream_t s);
 // GMRES: out[i] = V(:,i).w (i < k); w = alpha*w - V(:,0:k) h

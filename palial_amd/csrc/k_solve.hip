@@ -407,6 +407,24 @@ hipError_t launch_residual(const double *A, int N, int b, const double *x, const
     return hipGetLastError();
 }
 
+// F(i, i) += sigma |A(i, i)| on the working copy F (layout L, tile b, FP64 or
+// FP32) from the FP64 matrix A (layout L, tile bA)
+__global__ void add_diag_kernel(void *F, int f32, int N, int b, const double *A, int bA, double sigma)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double d = A[tile_off(bA, N / bA, i / bA, i / bA) + (size_t)(i % bA) * (bA + 1)];
+        const size_t o = tile_off(b, N / b, i / b, i / b) + (size_t)(i % b) * (b + 1);
+        if (f32) ((float *)F)[o] = (float)((double)((float *)F)[o] + sigma * fabs(d));
+        else ((double *)F)[o] += sigma * fabs(d);
+    }
+}
+
+hipError_t launch_add_diag(void *F, bool f32, int N, int b, const double *A, int bA, double sigma, hipStream_t s)
+{
+    hipLaunchKernelGGL(add_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, F, f32 ? 1 : 0, N, b, A, bA, sigma);
+    return hipGetLastError();
+}
+
 void solve_init()
 {
     // the 16-right-hand-side residual stages b x 18 doubles (74 KB at b = 512)

@@ -139,3 +139,38 @@ def test_split_bf16_fewer_refinement_steps(hip, orc):
         assert rc == 0 and backward_error(A, X, B) <= 2e-14
         its[prec] = rep.iterations
     assert its[4] < its[2], its
+
+
+@pytest.mark.parametrize("prec,kappa", [(1, 1e9), (2, 1e7), (4, 1e9)])
+def test_breakdown_shift_or_fallback(hip, orc, prec, kappa):
+    """A low-precision Cholesky of an ill-conditioned SPD matrix meets a
+    non-positive pivot.  With no_fallback the solve reports it (rc = the
+    column, > 0); by default it refactors A + sigma |diag A| (GMRES-IR after
+    that) or solves in FP64, and the answer reaches backward error 1e-14."""
+    N, b = 1024, 256
+    A = spd_cond(N, kappa, 11)
+    a = orc.from_dense(A, N, b)
+    B = np.random.default_rng(4).standard_normal((N, 1))
+    X0, rep0, rc0 = hip.solve(a, B, algo="cholesky", precision=prec, b=b, check=False, no_fallback=True)
+    X, rep, rc = hip.solve(a, B, algo="cholesky", precision=prec, b=b, check=False)
+    info = (rc0, rc, {k: rep.as_dict()[k] for k in ("precision_used", "shifts", "iterations", "gmres_iterations",
+                                                    "backward_error")})
+    assert rc == 0 and backward_error(A, X, B) <= 1e-14, (info, backward_error(A, X, B))
+    if rc0 > 0:                                   # the plain factorization broke down
+        assert rep.shifts > 0 or rep.precision_used == 0, info
+    else:
+        assert rep.shifts == 0, info
+    assert rep.precision_used in (0, prec), info
+
+
+def test_lu_nonconvergence_falls_back(hip, orc):
+    """BF16 LU-IR on kappa = 1e9 does not converge in 5 steps: the default
+    solve finishes in FP64 (precision_used 0); no_fallback reports it."""
+    N, b = 1024, 256
+    A = spd_cond(N, 1e9, 12)
+    a = orc.from_dense(A, N, b)
+    B = np.random.default_rng(5).standard_normal((N, 1))
+    X0, rep0, rc0 = hip.solve(a, B, algo="lu", precision=2, b=b, max_iter=5, refine=2, check=False, no_fallback=True)
+    assert rc0 != 0
+    X, rep, rc = hip.solve(a, B, algo="lu", precision=2, b=b, max_iter=5, refine=2)
+    assert rc == 0 and rep.precision_used == 0 and backward_error(A, X, B) <= 1e-14
