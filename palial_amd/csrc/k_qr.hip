@@ -218,7 +218,7 @@ namespace {
 
 constexpr int QP = 32;
 constexpr int QP_THREADS = 512;    // thread (j, g): panel column j = t / 16, row group g = t % 16
-constexpr int QA_THREADS = 256;    // four waves, one 16 x 16 quarter of a 32 x 32 block each
+constexpr int QA_THREADS = 512;    // eight waves: a 16 x 16 quarter of a 32 x 32 block and one half of K each
 
 typedef double qd4 __attribute__((ext_vector_type(4)));
 
@@ -273,8 +273,7 @@ __device__ inline double wave_sum64(double v)
 // the double-buffered sv) while the other waves finish step i.  Column c of
 // the diagonal T block is formed in step c+1, 16 lanes per row of T.
 template <bool GE>
-__global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double *A2, double *T, double *tau_out,
-                                                              int b, int c0)
+__device__ void panel_body(double *A1, double *A2, double *T, double *tau_out, int b, int c0)
 {
     constexpr int SV = 10;              // sv stride per lane (80 B: b128 reads without bank conflicts)
     __shared__ double sH[QP][QP + 1];   // head rows: sH[h][c] = (head row h, panel column c)
@@ -410,6 +409,13 @@ __global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double
     }
 }
 
+template <bool GE>
+__global__ __launch_bounds__(QP_THREADS) void qr_panel_kernel(double *A1, double *A2, double *T, double *tau_out,
+                                                              int b, int c0)
+{
+    panel_body<GE>(A1, A2, T, tau_out, b, c0);
+}
+
 // acc += Y(0:n, 0:16)^T X(0:n, 0:16) for column-major Y, X (ld b), n % 16 == 0.
 // Lane l takes column l % 16 and the four consecutive rows 4 (l / 16) + s of
 // each 16-row group, so that both operands are read as 32-byte pieces.
@@ -448,18 +454,25 @@ __device__ inline qd4 gram16(const double *Y, const double *X, int n, int b, qd4
 
 // Role A (blocks [0, na)): chunk x = columns cc = c0 + 32 + 32 x:
 //   W = Yp^T X;  W = Tpp^T W;  X -= Yp W          (dlarfb / dtsqrt's gemv + ger)
+//   with look != 0, block 0 (chunk 0 = the columns of panel p+1) then factors
+//   panel p+1 (panel_body), while the other blocks finish this launch's work
 // Role B (blocks [na, na + nbk)): q < p: G(32q + j, c0 + i) = Y_q(:, j)^T Y_p(:, i)
 // Role C (the rest): row chunk q < pc of T column block pc (cq = 32 pc):
 //   T(q, pc) = -(sum_{r=q}^{pc-1} T(q, r) G(r, pc)) T(pc, pc)
+// No two blocks of a launch touch the same columns: panel p+1 and the chunks
+// are disjoint, role B reads only panels <= p, role C writes T column block pc.
 template <bool GE>
-__global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double *A2, double *T, double *G, int b,
-                                                              int p, int na, int nbk, int pc)
+__global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double *A2, double *T, double *G,
+                                                              double *tau, int b, int p, int na, int nbk, int pc,
+                                                              int look)
 {
     __shared__ double sW[QP][QP + 1];
     __shared__ double sW2[QP][QP + 1];
     __shared__ double sTp[QP][QP + 1];
     __shared__ double sY[QP][QP + 1];
-    const int t = threadIdx.x, l = t & 63, w = t >> 6, wi = w & 1, wj = w >> 1;
+    __shared__ qd4 sAcc[4][64];                    // K-half partial products of waves 4..7
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int blk = w & 3, wi = blk & 1, wj = blk >> 1, kh = w >> 2;
     const int c0 = QP * p;
     const int x = blockIdx.x;
     const int nt = GE ? b - c0 - QP : b;
@@ -470,8 +483,8 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
         const bool roleA = x < na;
         const int xc = roleA ? c0 + QP + QP * x : QP * (x - na);   // first column of X
         qd4 acc = {0.0, 0.0, 0.0, 0.0};
-        // head rows: Yp unit lower (GE) or identity (TS)
-        if (GE) {
+        // head rows: Yp unit lower (GE) or identity (TS); K half 0 only
+        if (GE && kh == 0) {
             const int c = l & 15, kk = l >> 4;
             const int ic = 16 * wi + c;                 // panel column of the A operand
             for (int r = 0; r < QP; r += 16) {
@@ -484,22 +497,37 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
                 }
             }
         }
-        acc = gram16(tail + (size_t)(c0 + 16 * wi) * ldb, tail + (size_t)(xc + 16 * wj) * ldb, nt, b, acc);
+        {
+            const int half = (nt >> 5) << 4;            // tail rows [0, half) | [half, nt)
+            const int r0 = kh ? half : 0, r1 = kh ? nt : half;
+            acc = gram16(tail + (size_t)(c0 + 16 * wi) * ldb + r0, tail + (size_t)(xc + 16 * wj) * ldb + r0, r1 - r0,
+                         b, acc);
+        }
+        if (kh) sAcc[blk][l] = acc;
+        __syncthreads();
+        if (!kh) {
+            const qd4 o = sAcc[blk][l];
+            acc[0] += o[0]; acc[1] += o[1]; acc[2] += o[2]; acc[3] += o[3];
+        }
         // D layout: lane l, value v -> (i = 16 wi + l / 16 + 4 v, j = 16 wj + l % 16)
         if (!roleA) {
+            if (!kh) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int i = 16 * wi + (l >> 4) + 4 * v, jj = 16 * wj + (l & 15);
-                G[(size_t)(xc + jj) + (size_t)(c0 + i) * ldb] = acc[v];
+                for (int v = 0; v < 4; v++) {
+                    const int i = 16 * wi + (l >> 4) + 4 * v, jj = 16 * wj + (l & 15);
+                    G[(size_t)(xc + jj) + (size_t)(c0 + i) * ldb] = acc[v];
+                }
             }
             return;
         }
+        if (!kh) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = 16 * wi + (l >> 4) + 4 * v, jj = 16 * wj + (l & 15);
-            double wv = acc[v];
-            if (!GE) wv += head[(size_t)(xc + jj) * ldb + i];   // identity head of Yp
-            sW[i][jj] = wv;
+            for (int v = 0; v < 4; v++) {
+                const int i = 16 * wi + (l >> 4) + 4 * v, jj = 16 * wj + (l & 15);
+                double wv = acc[v];
+                if (!GE) wv += head[(size_t)(xc + jj) * ldb + i];   // identity head of Yp
+                sW[i][jj] = wv;
+            }
         }
         for (int e = t; e < QP * QP; e += QA_THREADS) {
             const int r = e & 31, c = e >> 5;
@@ -524,7 +552,7 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
             head[(size_t)(xc + jj) * ldb + h] -= s;
         }
         // tail rows: X(r, j) -= sum_k Yp(r, k) W2(k, j), as D'[j][r] = sum_k W2(k, j) Yp(r, k).
-        // Wave w: columns j0 = 16 (w & 1), row blocks (w >> 1) + 2 m; the next
+        // Wave w: columns j0 = 16 (w & 1), row blocks (w >> 1) + 4 m; the next
         // block's operands are loaded before the current block's MFMAs.
         const int nrb = nt >> 4, j0 = 16 * (w & 1);
         double wa[8];
@@ -540,14 +568,14 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
 #pragma unroll
             for (int v = 0; v < 4; v++) xv[v] = xp[(size_t)4 * v * ldb + 16 * rb];
         }
-        for (; rb < nrb; rb += 2) {
+        for (; rb < nrb; rb += 4) {
             qd4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < 8; k++) d = qmfma(wa[k], yb[k], d);
             double xo[4];
 #pragma unroll
             for (int v = 0; v < 4; v++) xo[v] = xv[v] - d[v];
-            const int rn = rb + 2;
+            const int rn = rb + 4;
             if (rn < nrb) {
 #pragma unroll
                 for (int k = 0; k < 8; k++) yb[k] = yp[(size_t)4 * k * ldb + 16 * rn];
@@ -557,12 +585,17 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
 #pragma unroll
             for (int v = 0; v < 4; v++) xp[(size_t)4 * v * ldb + 16 * rb] = xo[v];
         }
+        if (x == 0 && look) {
+            // panel p+1 = the chunk this block just updated (visible after the barrier)
+            __syncthreads();
+            panel_body<GE>(A1, A2, T, tau, b, c0 + QP);
+        }
         return;
     }
-    // role C
+    // role C (waves 0..3 form the product, all threads finish it)
     const int q = x - na - nbk, cq = QP * pc;
-    qd4 acc = {0.0, 0.0, 0.0, 0.0};
-    {
+    if (w < 4) {
+        qd4 acc = {0.0, 0.0, 0.0, 0.0};
         const int c = l & 15, kk = l >> 4;
         const int ir = 16 * wi + c;                      // row inside chunk q
         const double *trow = T + (size_t)(QP * q + ir);
@@ -601,10 +634,10 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
                 }
             }
         }
-    }
-    // D layout: lane l, value v -> Z(16 wi + l / 16 + 4 v, 16 wj + l % 16)
+        // D layout: lane l, value v -> Z(16 wi + l / 16 + 4 v, 16 wj + l % 16)
 #pragma unroll
-    for (int v = 0; v < 4; v++) sW[16 * wi + (l >> 4) + 4 * v][16 * wj + (l & 15)] = acc[v];
+        for (int v = 0; v < 4; v++) sW[16 * wi + (l >> 4) + 4 * v][16 * wj + (l & 15)] = acc[v];
+    }
     for (int e = t; e < QP * QP; e += QA_THREADS) {
         const int r = e & 31, c = e >> 5;
         sTp[r][c] = r <= c ? T[(size_t)(cq + r) + (size_t)(cq + c) * ldb] : 0.0;
@@ -618,22 +651,24 @@ __global__ __launch_bounds__(QA_THREADS) void qr_apply_kernel(double *A1, double
     }
 }
 
+// Panel 0 on its own; then launch p (p < P) applies panel p, factors panel
+// p+1 in block 0, and forms G(:, p) and the T blocks of panel p-1; the last
+// launch forms the T blocks of panel P-1.
 hipError_t launch_blocked(bool ge, double *A1, double *A2, double *T, double *tau, double *G, int b, hipStream_t s)
 {
     const int P = b / QP;
+    if (ge) hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(1), dim3(QP_THREADS), 0, s, A1, A2, T, tau, b, 0);
+    else hipLaunchKernelGGL(qr_panel_kernel<false>, dim3(1), dim3(QP_THREADS), 0, s, A1, A2, T, tau, b, 0);
     for (int p = 0; p <= P; p++) {
-        if (p < P) {
-            if (ge) hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(1), dim3(QP_THREADS), 0, s, A1, A2, T, tau, b, QP * p);
-            else hipLaunchKernelGGL(qr_panel_kernel<false>, dim3(1), dim3(QP_THREADS), 0, s, A1, A2, T, tau, b, QP * p);
-        }
         const int na = p < P ? P - 1 - p : 0, nbk = p < P ? p : 0, nc = p > 1 ? p - 1 : 0;
+        const int look = p + 1 < P ? 1 : 0;      // na >= 1 exactly then: block 0 is chunk 0
         if (na + nbk + nc == 0) continue;
         if (ge)
-            hipLaunchKernelGGL(qr_apply_kernel<true>, dim3(na + nbk + nc), dim3(QA_THREADS), 0, s, A1, A2, T, G, b, p,
-                               na, nbk, p - 1);
+            hipLaunchKernelGGL(qr_apply_kernel<true>, dim3(na + nbk + nc), dim3(QA_THREADS), 0, s, A1, A2, T, G, tau,
+                               b, p, na, nbk, p - 1, look);
         else
-            hipLaunchKernelGGL(qr_apply_kernel<false>, dim3(na + nbk + nc), dim3(QA_THREADS), 0, s, A1, A2, T, G, b,
-                               p, na, nbk, p - 1);
+            hipLaunchKernelGGL(qr_apply_kernel<false>, dim3(na + nbk + nc), dim3(QA_THREADS), 0, s, A1, A2, T, G,
+                               tau, b, p, na, nbk, p - 1, look);
     }
     return hipGetLastError();
 }
