@@ -1301,10 +1301,10 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         if (eta <= tol || it >= max_iter) break;
         // with the FP64 fallback allowed, GMRES-IR that stalls (two steps in a
         // row that gain less than 4x) or runs long (more than one restart
-        // cycle per right-hand side) would cost more than the FP64 solve
+        // cycle in all) would cost more than the FP64 solve
         if (fallback_ok && mode == 1 && it > 0) {
             gm_slow = (eta <= 0.25 * eta_prev) ? 0 : gm_slow + 1;
-            if (gm_slow >= 2 || gmres_total > GM * nrhs) break;
+            if (gm_slow >= 2 || gmres_total > 2 * GM) break;
         }
         // auto: LU-IR until it fails twice in a row to halve eta, then GMRES-IR
         if (o.refine == 0 && mode == 0 && it > 0 && !(eta <= 0.5 * eta_prev)) {
@@ -1315,11 +1315,15 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         eta_prev = eta;
         if (mode == 1) {
             // GMRES-IR: A d = r by GMRES preconditioned on the left with F^-1
+            bool abandon = false;
             for (int j = 0; j < nrhs; j++) {
                 int inner = 0;
                 if ((rc = gmres(R + (size_t)j * N, D + (size_t)j * N, inner))) return done(rc);
                 gmres_total += inner;
+                // one GMRES(30) cycle costs about as much as the FP64 solve
+                if (fallback_ok && gmres_total > GM && j + 1 < nrhs) { abandon = true; break; }
             }
+            if (abandon) break;
         } else {
             // LU-IR: d = F^{-1} r
             L.chk(launch_copy(R2, R, nv * 8, s));
