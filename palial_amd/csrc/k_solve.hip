@@ -147,11 +147,98 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
     }
 }
 
+// The same products for FP32 tiles and 5..16 right-hand sides on
+// v_mfma_f32_16x16x4f32: FP32 products and sums, the input segment rounded to
+// FP32 (the mixed-tier correction solves, whose factor carries BF16/FP32-level
+// error anyway).  A wave owns 16 output rows and runs over all b columns:
+// lane l supplies row i = l % 16 of op(T) at the four consecutive k =
+// k0 + 4 (l / 16) + s of each 16-k group, and the matching rows of the input
+// segment from LDS; D (16 rows x 16 right-hand sides) lands at rows
+// 4 (l / 16) + v, right-hand side l % 16.
+typedef float gf4 __attribute__((ext_vector_type(4)));
+
+template <bool TRANS>
+__global__ __launch_bounds__(GV_THREADS) void tile_gemm16_kernel(GemvArgs p)
+{
+    constexpr int SX = 20;                       // LDS row stride in floats: conflict-free operand reads
+    extern __shared__ float sxf[];               // b x SX: sxf[c * SX + j] = v(c, j)
+    const int b = p.b;
+    const int rb_per_tile = b / 64;
+    const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const float *mat = (const float *)p.mat;
+    const float *inv = (const float *)p.inv;
+    const float *T;
+    int vin, vout;
+    {
+        const int MT = p.N / b, k = p.k;
+        switch (p.mode) {
+        case GV_DIAG: T = inv + (size_t)k * b * b; vin = k; vout = k; break;
+        case GV_CHOL_FWD: T = mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
+        case GV_LU_FWD: T = mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
+        default: T = mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
+        }
+    }
+    const double *vi = p.vin + (size_t)vin * b;
+    double *vo = p.vout + (size_t)vout * b;
+    for (int e = t; e < b * 16; e += GV_THREADS) {
+        const int c = e % b, j = e / b;
+        sxf[c * SX + j] = j < p.nrhs ? (float)vi[(size_t)c + (size_t)j * p.ld] : 0.0f;   // padded columns are zero
+    }
+    __syncthreads();
+    const int r0 = rb * 64 + w * 16, i = l & 15, kk = l >> 4;
+    gf4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    // op(T) values of 16-k group g for this lane (four loads in flight per group)
+    auto load = [&](int g, float *av) {
+        const int kb = 16 * g + 4 * kk;
+        if (TRANS) {
+            const float4 x = *(const float4 *)(T + (size_t)kb + (size_t)(r0 + i) * b);
+            av[0] = x.x; av[1] = x.y; av[2] = x.z; av[3] = x.w;
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++) av[s] = T[(size_t)(r0 + i) + (size_t)(kb + s) * b];
+        }
+    };
+    constexpr int D = 4;                         // groups loaded ahead
+    const int ng = b / 16;                       // a multiple of 4 (b % 64 == 0)
+    float ab[D][4];
+#pragma unroll
+    for (int u = 0; u < D; u++) load(u, ab[u]);
+    for (int g0 = 0; g0 < ng; g0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; u++) {
+            float av[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) av[s] = ab[u][s];
+            if (g0 + u + D < ng) load(g0 + u + D, ab[u]);
+            const int kb = 16 * (g0 + u) + 4 * kk;
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], sxf[(kb + s) * SX + i], acc, 0, 0, 0);
+        }
+    }
+    if (i < p.nrhs) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double *o = vo + (size_t)(r0 + 4 * kk + v) + (size_t)i * p.ld;
+            if (p.mode == GV_DIAG) *o = (double)acc[v];
+            else *o -= (double)acc[v];
+        }
+    }
+}
+
 hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
 {
     if (a.ntasks <= 0) return hipSuccess;
     if (a.b > 512 || a.b % 64 || a.nrhs > GV_MAXR || a.nrhs < 1) return hipErrorInvalidValue;
     // FP32 tiles read by columns (op = N): 32 rows per workgroup (128-byte column pieces)
+    if (a.f32 && a.nrhs > 4) {
+        dim3 g16((unsigned)(a.ntasks * (a.b / 64))), b16(GV_THREADS);
+        const size_t lds = (size_t)a.b * 20 * 4;
+        if (a.trans) hipLaunchKernelGGL(tile_gemm16_kernel<true>, g16, b16, lds, s, a);
+        else hipLaunchKernelGGL(tile_gemm16_kernel<false>, g16, b16, lds, s, a);
+        return hipGetLastError();
+    }
     const bool wide = a.f32 && !a.trans;
     dim3 grid((unsigned)(a.ntasks * (a.b / (wide ? 32 : GV_ROWS)))), block(GV_THREADS);
     auto lds = [&](int nr) { return ((size_t)a.b * nr + (size_t)GV_THREADS * nr) * 8; };
