@@ -200,7 +200,8 @@ def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), 
 def run_batched(P, count=10000, steps=3, seed=0):
     """Batched small factorizations: `count` matrices with n ~ UniformInt[16, 512]
     (seeded), even i Cholesky of S + n I, odd i LU of G + n I, one N(0,1)
-    right-hand side each, factored and solved in one launch per kind; FP64
+    right-hand side each, factored and solved in one persistent launch (replayed
+    from its captured hipGraph); FP64
     and FP32.  Inputs are regenerated before each timed run."""
     import numpy as np
     rng = np.random.default_rng(seed)

@@ -422,9 +422,11 @@ class Batch:
         vecs = [b[o:o + m] for o, m in zip(self.boff, self.n)]
         return mats, vecs
 
-    def factor(self, solve=True):
-        """Factor every matrix in place (and solve its right-hand side); returns the report."""
-        o = _opts(device=True)
+    def factor(self, solve=True, no_graph=False):
+        """Factor every matrix in place (and solve its right-hand side); returns the
+        report.  One persistent launch, replayed from a captured hipGraph unless
+        no_graph."""
+        o = _opts(device=True, no_graph=no_graph)
         rep = Report()
         _check(lib().palial_hip_batched(self.count, self.n.ctypes.data, self.off.ctypes.data,
                                         self.kind.ctypes.data, self.A, self.B if solve else None,

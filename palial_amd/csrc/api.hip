@@ -39,6 +39,13 @@ struct Graph {
     long nodes = 0;
 };
 
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int count = 0, f32 = 0, bmax = 0, grid = 0;
+    const void *A = nullptr, *B = nullptr, *idx = nullptr, *work = nullptr;
+};
+
 struct Ctx {
     bool init = false;
     int device = -1;
@@ -78,6 +85,7 @@ struct Ctx {
     size_t F_cap = 0;
     double *d_inv = nullptr;
     size_t inv_cap = 0;
+    BatchGraph bgraph;              // batched factorizations: the captured persistent launch
     void *d_bidx = nullptr;         // batched factorizations: per-matrix index arrays
     size_t bidx_cap = 0;            // bytes
     double *d_bwork = nullptr;      // batched factorizations: padded FP64 workspace
@@ -144,6 +152,9 @@ void drop_graphs()
         if (kv.second.graph) hipGraphDestroy(kv.second.graph);
     }
     g.graphs.clear();
+    if (g.bgraph.exec) hipGraphExecDestroy(g.bgraph.exec);
+    if (g.bgraph.graph) hipGraphDestroy(g.bgraph.graph);
+    g.bgraph = BatchGraph();
 }
 
 constexpr int QR_MAXS = 8;     // most panel-chain (and update) streams of the QR schedule
@@ -1793,10 +1804,10 @@ int grow_bytes(void **p, size_t *cap, size_t need)
 }
 
 struct BatchPlan {
-    std::vector<int> orderC, orderL;
+    std::vector<int> order;          // all matrices, largest first
     std::vector<long> woff;
     size_t work = 0, a_ext = 0, b_ext = 0;
-    int bmaxC = 32, bmaxL = 32;
+    int bmax = 32;
 };
 
 int batch_plan(int count, const int *n, const long *off, const int *kind, const long *boff, BatchPlan &P)
@@ -1813,13 +1824,11 @@ int batch_plan(int count, const int *n, const long *off, const int *kind, const 
         P.work += (size_t)b * b;
         P.a_ext = std::max(P.a_ext, (size_t)off[i] + (size_t)n[i] * n[i]);
         if (boff) P.b_ext = std::max(P.b_ext, (size_t)boff[i] + (size_t)n[i]);
-        (kind[i] == 0 ? P.orderC : P.orderL).push_back(i);
-        int &bm = kind[i] == 0 ? P.bmaxC : P.bmaxL;
-        bm = std::max(bm, b);
+        P.order.push_back(i);
+        P.bmax = std::max(P.bmax, b);
     }
     auto larger = [&](int a, int b) { return n[a] > n[b]; };
-    std::stable_sort(P.orderC.begin(), P.orderC.end(), larger);
-    std::stable_sort(P.orderL.begin(), P.orderL.end(), larger);
+    std::stable_sort(P.order.begin(), P.order.end(), larger);
     return 0;
 }
 
@@ -1838,13 +1847,14 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
     BatchPlan P;
     if ((rc = batch_plan(count, n, off, kind, B ? boff : nullptr, P))) return rc;
     const size_t es = prec == PALIAL_PREC_FP32 ? 4 : 8;
-    // device index arrays: n | off | woff | boff | orderC | orderL | info
-    const size_t ibytes = (size_t)count * (4 + 8 + 8 + 8 + 4 + 4 + 4) + 64;
+    // device index arrays: off | woff | boff | n | order | kind | info | queue
+    const size_t ibytes = (size_t)count * (8 + 8 + 8 + 4 + 4 + 4 + 4) + 64;
     if ((rc = grow_bytes(&g.d_bidx, &g.bidx_cap, ibytes))) return rc;
     if ((rc = grow(&g.d_bwork, &g.bwork_cap, P.work))) return rc;
     char *base = (char *)g.d_bidx;
     long *d_off = (long *)base, *d_woff = d_off + count, *d_boff = d_woff + count;
-    int *d_n = (int *)(d_boff + count), *d_oc = d_n + count, *d_ol = d_oc + P.orderC.size(), *d_info = d_ol + P.orderL.size();
+    int *d_n = (int *)(d_boff + count), *d_ord = d_n + count, *d_kind = d_ord + count, *d_info = d_kind + count;
+    int *d_queue = d_info + count;
     hipStream_t s = g.s_main;
     std::vector<long> hb(count, 0);
     if (B) for (int i = 0; i < count; i++) hb[i] = boff[i];
@@ -1852,8 +1862,8 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
     HIPCHK(hipMemcpyAsync(d_woff, P.woff.data(), (size_t)count * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d_boff, hb.data(), (size_t)count * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d_n, n, (size_t)count * 4, hipMemcpyHostToDevice, s));
-    if (!P.orderC.empty()) HIPCHK(hipMemcpyAsync(d_oc, P.orderC.data(), P.orderC.size() * 4, hipMemcpyHostToDevice, s));
-    if (!P.orderL.empty()) HIPCHK(hipMemcpyAsync(d_ol, P.orderL.data(), P.orderL.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_ord, P.order.data(), (size_t)count * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_kind, kind, (size_t)count * 4, hipMemcpyHostToDevice, s));
     void *dA = A, *dB = B;
     bool own = false;
     if (!o.device_matrix) {
@@ -1872,18 +1882,38 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
     memset(&a, 0, sizeof a);
     a.n = d_n; a.off = d_off; a.woff = d_woff; a.boff = d_boff; a.A = dA; a.B = dB;
     a.work = g.d_bwork; a.info = d_info; a.f32 = (prec == PALIAL_PREC_FP32);
+    a.count = count; a.order = d_ord; a.kind = d_kind; a.queue = d_queue;
+    const int grid = batch_grid(P.bmax, count);
+    if (grid < 1) return fail(PALIAL_ERR_HIP, "batched launch shape");
+    // one persistent launch for both kinds, captured once into a hipGraph
+    // (per argument set) and replayed; the queue counter is zeroed before it,
+    // outside the graph
+    HIPCHK(hipMemsetAsync(d_queue, 0, sizeof(int), s));
     HIPCHK(hipEventRecord(e0, s));
-    // the Cholesky and LU sub-batches run side by side on the two streams
-    hipEvent_t fork = g.events[0], join = g.events[1];
-    HIPCHK(hipEventRecord(fork, s));
-    HIPCHK(hipStreamWaitEvent(g.s_panel, fork, 0));
-    BatchArgs ac = a, al = a;
-    ac.count = (int)P.orderC.size(); ac.order = d_oc;
-    al.count = (int)P.orderL.size(); al.order = d_ol;
-    HIPCHK(launch_batch(ac, true, P.bmaxC, s));
-    HIPCHK(launch_batch(al, false, P.bmaxL, g.s_panel));
-    HIPCHK(hipEventRecord(join, g.s_panel));
-    HIPCHK(hipStreamWaitEvent(s, join, 0));
+    const bool use_graph = !o.no_graph && !g_prof.on;
+    if (use_graph) {
+        BatchGraph &bg = g.bgraph;
+        const bool same = bg.exec && bg.count == count && bg.A == dA && bg.B == dB && bg.idx == g.d_bidx &&
+                          bg.work == g.d_bwork && bg.f32 == a.f32 && bg.bmax == P.bmax && bg.grid == grid;
+        if (!same) {
+            if (bg.exec) hipGraphExecDestroy(bg.exec);
+            if (bg.graph) hipGraphDestroy(bg.graph);
+            bg = BatchGraph();
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = launch_batch(a, P.bmax, grid, s);
+            hipGraph_t gr = nullptr;
+            const hipError_t ce = hipStreamEndCapture(s, &gr);
+            if (le != hipSuccess || ce != hipSuccess || !gr)
+                return fail(PALIAL_ERR_HIP, "batched capture: %s / %s", hipGetErrorString(le), hipGetErrorString(ce));
+            bg.graph = gr;
+            HIPCHK(hipGraphInstantiate(&bg.exec, gr, nullptr, nullptr, 0));
+            bg.count = count; bg.A = dA; bg.B = dB; bg.idx = g.d_bidx; bg.work = g.d_bwork;
+            bg.f32 = a.f32; bg.bmax = P.bmax; bg.grid = grid;
+        }
+        HIPCHK(hipGraphLaunch(bg.exec, s));
+    } else {
+        HIPCHK(launch_batch(a, P.bmax, grid, s));
+    }
     HIPCHK(hipEventRecord(e1, s));
     if (info) HIPCHK(hipMemcpyAsync(info, d_info, (size_t)count * 4, hipMemcpyDeviceToHost, s));
     if (own) {
@@ -1907,7 +1937,7 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
         rep->gflop = fl / 1e9;
         rep->tflops = ms > 0 ? fl / (ms * 1e-3) / 1e12 : 0.0;
         rep->ntasks = count;
-        rep->nlaunches = 2;
+        rep->nlaunches = 1;
     }
     return 0;
 }

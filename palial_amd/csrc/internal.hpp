@@ -100,7 +100,7 @@ hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s);
 // of 32*32 doubles, column-major).
 bool panel2_ok(int b);
 void panel2_init();
-// Batched small factorizations (k_panel2.hip): one launch per kind; matrix
+// Batched small factorizations (k_panel2.hip): one persistent launch; matrix
 // i (order n[i] <= 512, column-major, ld n[i]) at A + off[i] is copied into
 // a 32-padded FP64 workspace (identity padding), factored there by the tile
 // kernels' code (kind 0 Cholesky, 1 LU without pivoting), optionally solved
@@ -108,7 +108,9 @@ void panel2_init();
 // when f32).  All index arrays are device arrays.
 struct BatchArgs {
     int count;
-    const int *order;    // matrices of this launch's kind, processing order (largest first)
+    const int *order;    // processing order of the matrices (largest first)
+    const int *kind;     // per matrix: 0 Cholesky, 1 LU
+    int *queue;          // work counter of the persistent launch (zero before each launch)
     const int *n;
     const long *off;
     const long *woff;    // workspace offsets (doubles)
@@ -119,7 +121,10 @@ struct BatchArgs {
     int *info;
     int f32;
 };
-hipError_t launch_batch(const BatchArgs &a, bool chol, int bmax, hipStream_t s);
+// one persistent launch of `grid` workgroups (batch_grid: what the CUs hold
+// at once for matrices of order <= bmax, at most count)
+int batch_grid(int bmax, int count);
+hipError_t launch_batch(const BatchArgs &a, int bmax, int grid, hipStream_t s);
 // seeded batch inputs: kind 0 -> S + n I (S symmetric), 1 -> G + n I; seed of matrix i: seed + i
 hipError_t launch_batch_generate(const int *n, const long *off, const int *kind, void *A, int count, int f32,
                                  uint64_t seed, hipStream_t s);

@@ -509,17 +509,13 @@ __device__ inline double tri32_bwd(double v, const double (&reg)[NB2], int lane)
     return v;
 }
 
-// One workgroup per matrix; workgroup g takes the g-th matrix of the order
-// list (largest first: the dispatcher hands out workgroups in index order,
-// which approximates longest-processing-time scheduling).  No counters, so
-// the launch can be captured and replayed.
-template <bool CHOL>
-__global__ __launch_bounds__(PF_THREADS) void batch_kernel(BatchArgs p)
+// Matrix i of a batch, by one workgroup: copy into the 32-padded FP64
+// workspace (identity padding), factor with the tile kernels' code (chol:
+// Cholesky, else LU without pivoting), optionally solve, copy back.
+__device__ void batch_one(const BatchArgs &p, int i, bool chol, double *smem)
 {
-    extern __shared__ double smem[];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     {
-        const int i = p.order[blockIdx.x];
         const int n = p.n[i], b = (n + NB2 - 1) / NB2 * NB2;
         double *W = p.work + p.woff[i];
         const size_t off = (size_t)p.off[i];
@@ -533,7 +529,6 @@ __global__ __launch_bounds__(PF_THREADS) void batch_kernel(BatchArgs p)
         }
         if (t == 0) p.info[i] = 0;
         __syncthreads();
-        constexpr bool chol = CHOL;   // the launch holds one kind only (order lists per kind)
         if (chol) potrf2_tile(W, b, p.info + i, 0, nullptr, smem);
         else getrf2_tile(W, b, nullptr, nullptr, smem);
         __syncthreads();
@@ -597,6 +592,28 @@ __global__ __launch_bounds__(PF_THREADS) void batch_kernel(BatchArgs p)
             if (p.f32) ((float *)p.A)[off + e] = (float)v;
             else ((double *)p.A)[off + e] = v;
         }
+    }
+}
+
+// Persistent batch: a fixed grid of workgroups (what the CUs hold at once)
+// takes matrices from p.queue[0] in the order list (largest first, both
+// kinds: dynamic longest-processing-time scheduling) until the list is
+// exhausted; every workgroup leaves through the same test.  The queue counter
+// is zeroed before each launch.
+__global__ __launch_bounds__(PF_THREADS) void batch_persist_kernel(BatchArgs p)
+{
+    extern __shared__ double smem[];
+    __shared__ int s_idx;
+    const int t = threadIdx.x;
+    for (;;) {
+        if (t == 0) s_idx = atomicAdd(p.queue, 1);
+        __syncthreads();
+        const int idx = s_idx;
+        __syncthreads();                       // s_idx is rewritten by the next round
+        if (idx >= p.count) break;
+        const int i = p.order[idx];
+        batch_one(p, i, p.kind[i] == 0, smem);
+        __syncthreads();
     }
 }
 
@@ -892,9 +909,7 @@ void panel2_init()
                         (int)panel_lds_bytes(512, 2));
     hipFuncSetAttribute((const void *)getrf2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)panel_lds_bytes(512, 3));
-    hipFuncSetAttribute((const void *)batch_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)panel_lds_bytes(512, 3));
-    hipFuncSetAttribute((const void *)batch_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipFuncSetAttribute((const void *)batch_persist_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)panel_lds_bytes(512, 3));
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_CHOL, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
     hipFuncSetAttribute((const void *)trsm2_kernel<TR_LUL, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_trsm);
@@ -921,13 +936,23 @@ hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_batch(const BatchArgs &a, bool chol, int bmax, hipStream_t s)
+int batch_grid(int bmax, int count)
+{
+    int per = 0, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_persist_kernel, PF_THREADS,
+                                                     panel_lds_bytes(bmax, 3)) != hipSuccess || per < 1)
+        per = 1;
+    const int g = per * prop.multiProcessorCount;
+    return count < g ? count : g;
+}
+
+hipError_t launch_batch(const BatchArgs &a, int bmax, int grid, hipStream_t s)
 {
     if (a.count <= 0) return hipSuccess;
-    if (bmax <= 0 || bmax > 512 || bmax % NB2) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)a.count), block(PF_THREADS);
-    if (chol) hipLaunchKernelGGL(batch_kernel<true>, grid, block, panel_lds_bytes(bmax, 3), s, a);
-    else hipLaunchKernelGGL(batch_kernel<false>, grid, block, panel_lds_bytes(bmax, 3), s, a);
+    if (bmax <= 0 || bmax > 512 || bmax % NB2 || grid < 1 || !a.queue || !a.kind) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_persist_kernel, dim3((unsigned)grid), dim3(PF_THREADS), panel_lds_bytes(bmax, 3), s, a);
     return hipGetLastError();
 }
 
