@@ -304,3 +304,33 @@ def resid_lu(A, F):
     L = np.tril(F, -1) + np.eye(n)
     U = np.triu(F)
     return np.linalg.norm(A - L @ U) / (n * EPS * np.linalg.norm(A))
+
+
+def resid_qr(A, F, S, N, b):
+    """Scaled residuals (||A - QR||_F / (n eps ||A||_F), ||I - Q^T Q||_F / (n eps))
+    of a flat-tree tiled QR (src/qr.h:18-190) from its output in layout L:
+    F holds R (upper part), the GEQRT reflectors V (unit lower, tile (k,k))
+    and the TS reflectors (tile (m,k), m > k); S the T factors.  The
+    factorization applied C <- (I - Y T Y^T)^T C per step (dlarfb 'L','T';
+    src/srcqr/dtsmqr.h), so Q = H_1 H_2 ... with H = I - Y T Y^T, formed here
+    by applying the H in reverse order to the identity."""
+    MT = N // b
+
+    def tile(M, m, n):
+        o = b * b * (m + MT * n)
+        return M[o:o + b * b].reshape(b, b).T
+
+    ops = []
+    for k in range(MT):
+        ops.append(([k], np.tril(tile(F, k, k), -1) + np.eye(b), np.triu(tile(S, k, k))))
+        for m in range(k + 1, MT):
+            ops.append(([k, m], np.vstack([np.eye(b), tile(F, m, k)]), np.triu(tile(S, m, k))))
+    Q = np.eye(N)
+    for rows, Y, T in reversed(ops):
+        idx = np.concatenate([np.arange(r * b, (r + 1) * b) for r in rows])
+        X = Q[idx, :]
+        Q[idx, :] = X - Y @ (T @ (Y.T @ X))
+    R = np.triu(to_dense(F, N, b))
+    r1 = np.linalg.norm(A - Q @ R) / (N * EPS * np.linalg.norm(A))
+    r2 = np.linalg.norm(np.eye(N) - Q.T @ Q) / (N * EPS)
+    return r1, r2

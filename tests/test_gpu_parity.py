@@ -185,6 +185,8 @@ def test_qr(hip, orc, N, b):
                 t = s[off:off + b * b].reshape(b, b).T   # column-major tile -> [r][c]
                 tr = sref[off:off + b * b].reshape(b, b).T
                 assert rel(np.triu(t), np.triu(tr)) < TOL, (m, k)
+    r1, r2 = orc.resid_qr(orc.to_dense(a, N, b), out, s, N, b)     # ||A - QR||, ||I - Q^T Q|| (scaled)
+    assert r1 < 10 and r2 < 10, (r1, r2)
 
 
 @pytest.mark.parametrize("N,b", [(1024, 128), (2048, 256)])

@@ -234,6 +234,11 @@ def test_qr_invariants(orc, backend, N, b):
     sg = np.sign(np.diag(R)) * np.sign(np.diag(Rref))
     assert rel(R, sg[:, None] * Rref) < 1e-12
     assert rel(R.T @ R, A.T @ A) < 1e-13
+    r1, r2 = orc.resid_qr(A, r, s, N, b)            # Q formed from the reflectors and T factors
+    assert r1 < 10 and r2 < 10
+    bad = s.copy()
+    bad[b * b * 1:b * b * 1 + b * b] *= 1.01       # a perturbed T factor (tile (1, 0)) breaks Q
+    assert max(orc.resid_qr(A, r, bad, N, b)) > 1e3
 
 
 def test_backends_agree(orc):
