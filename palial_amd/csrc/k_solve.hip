@@ -227,16 +227,94 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemm16_kernel(GemvArgs p)
     }
 }
 
+// FP64 tiles with 5..16 right-hand sides on v_mfma_f64_16x16x4f64 (FP64
+// products and sums): the same wave layout as tile_gemm16_kernel; D lands at
+// rows l / 16 + 4 v, right-hand side l % 16.
+template <bool TRANS>
+__global__ __launch_bounds__(GV_THREADS) void tile_dgemm16_kernel(GemvArgs p)
+{
+    constexpr int SX = 20;                       // LDS row stride in doubles
+    extern __shared__ double sxd[];              // b x SX: sxd[c * SX + j] = v(c, j)
+    const int b = p.b;
+    const int rb_per_tile = b / 64;
+    const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const double *mat = (const double *)p.mat;
+    const double *inv = (const double *)p.inv;
+    const double *T;
+    int vin, vout;
+    {
+        const int MT = p.N / b, k = p.k;
+        switch (p.mode) {
+        case GV_DIAG: T = inv + (size_t)k * b * b; vin = k; vout = k; break;
+        case GV_CHOL_FWD: T = mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
+        case GV_LU_FWD: T = mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
+        default: T = mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
+        }
+    }
+    const double *vi = p.vin + (size_t)vin * b;
+    double *vo = p.vout + (size_t)vout * b;
+    for (int e = t; e < b * 16; e += GV_THREADS) {
+        const int c = e % b, j = e / b;
+        sxd[c * SX + j] = j < p.nrhs ? vi[(size_t)c + (size_t)j * p.ld] : 0.0;   // padded columns are zero
+    }
+    __syncthreads();
+    const int r0 = rb * 64 + w * 16, i = l & 15, kk = l >> 4;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    auto load = [&](int g, double *av) {
+        const int kb = 16 * g + 4 * kk;
+        if (TRANS) {
+            const d4 x = *(const d4 *)(T + (size_t)kb + (size_t)(r0 + i) * b);
+            av[0] = x[0]; av[1] = x[1]; av[2] = x[2]; av[3] = x[3];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++) av[s] = T[(size_t)(r0 + i) + (size_t)(kb + s) * b];
+        }
+    };
+    constexpr int D = 4;                         // groups loaded ahead
+    const int ng = b / 16;
+    double ab[D][4];
+#pragma unroll
+    for (int u = 0; u < D; u++) load(u, ab[u]);
+    for (int g0 = 0; g0 < ng; g0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; u++) {
+            double av[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) av[s] = ab[u][s];
+            if (g0 + u + D < ng) load(g0 + u + D, ab[u]);
+            const int kb = 16 * (g0 + u) + 4 * kk;
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], sxd[(kb + s) * SX + i], acc, 0, 0, 0);
+        }
+    }
+    if (i < p.nrhs) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double *o = vo + (size_t)(r0 + kk + 4 * v) + (size_t)i * p.ld;
+            if (p.mode == GV_DIAG) *o = acc[v];
+            else *o -= acc[v];
+        }
+    }
+}
+
 hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
 {
     if (a.ntasks <= 0) return hipSuccess;
     if (a.b > 512 || a.b % 64 || a.nrhs > GV_MAXR || a.nrhs < 1) return hipErrorInvalidValue;
     // FP32 tiles read by columns (op = N): 32 rows per workgroup (128-byte column pieces)
-    if (a.f32 && a.nrhs > 4) {
+    if (a.nrhs > 4) {
         dim3 g16((unsigned)(a.ntasks * (a.b / 64))), b16(GV_THREADS);
-        const size_t lds = (size_t)a.b * 20 * 4;
-        if (a.trans) hipLaunchKernelGGL(tile_gemm16_kernel<true>, g16, b16, lds, s, a);
-        else hipLaunchKernelGGL(tile_gemm16_kernel<false>, g16, b16, lds, s, a);
+        if (a.f32) {
+            const size_t lds = (size_t)a.b * 20 * 4;
+            if (a.trans) hipLaunchKernelGGL(tile_gemm16_kernel<true>, g16, b16, lds, s, a);
+            else hipLaunchKernelGGL(tile_gemm16_kernel<false>, g16, b16, lds, s, a);
+        } else {
+            const size_t lds = (size_t)a.b * 20 * 8;
+            if (a.trans) hipLaunchKernelGGL(tile_dgemm16_kernel<true>, g16, b16, lds, s, a);
+            else hipLaunchKernelGGL(tile_dgemm16_kernel<false>, g16, b16, lds, s, a);
+        }
         return hipGetLastError();
     }
     const bool wide = a.f32 && !a.trans;
@@ -514,6 +592,11 @@ hipError_t launch_add_diag(void *F, bool f32, int N, int b, const double *A, int
 
 void solve_init()
 {
+    // the FP64 16-right-hand-side sweeps stage b x 20 doubles (80 KB at b = 512)
+    hipFuncSetAttribute((const void *)tile_dgemm16_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        512 * 20 * 8);
+    hipFuncSetAttribute((const void *)tile_dgemm16_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        512 * 20 * 8);
     // the 16-right-hand-side residual stages b x 18 doubles (74 KB at b = 512)
     hipFuncSetAttribute((const void *)residual_partial_kernel<16, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         512 * 18 * 8);
