@@ -203,14 +203,20 @@ int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_
  * multiple of 128, N above it, and the default schedule.
  * B (column-major, leading dimension ldb >= N) holds the right-hand sides on
  * entry and the solution on return.  Requires b % 64 == 0 and b <= 512.
- * Returns PALIAL_ERR_NOCONV (with the reached backward error in rep) when
- * opts->max_iter refinement steps do not reach the tolerance. */
+ * A mixed tier whose Cholesky breaks down is refactored with a diagonal
+ * shift (then GMRES-IR), and a tier that still fails is solved again in FP64
+ * (rep->precision_used, rep->shifts); with opts->no_fallback set it returns
+ * the breakdown column (> 0) or PALIAL_ERR_NOCONV (with the reached backward
+ * error in rep) when opts->max_iter steps do not reach the tolerance. */
 int palial_hip_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palial_opts *opts,
                      palial_report *rep);
 
 /* ---------------------------------------------------------------- batched */
-/* Many independent small factorizations in one launch per kind (no reference
- * counterpart; for callers with batches of small systems).  Matrix i has
+/* Many independent small factorizations in one persistent launch (no
+ * reference counterpart; for callers with batches of small systems): a
+ * workgroup per CU takes matrices, largest first, from a work counter; the
+ * launch is captured into a hipGraph once per argument set and replayed
+ * (opts->no_graph: launched directly).  Matrix i has
  * order n[i] (1..512), is column-major with leading dimension n[i] and
  * starts at element off[i] of A; kind[i] 0 = Cholesky (A = U^T U, U in the
  * upper triangle, strict lower part untouched, as palial_hip_dpotrf), 1 = LU
@@ -220,7 +226,7 @@ int palial_hip_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, cons
  * element boff[i]; it is overwritten by the solution.  info[i] (optional):
  * 0, or the first non-positive pivot (Cholesky, 1-based).  n, off, kind,
  * boff, info are host arrays; A and B follow opts->device_matrix.
- * rep->factor_ms is the device time of the launches. */
+ * rep->factor_ms is the device time of the launch. */
 int palial_hip_batched(int count, const int *n, const long *off, const int *kind, void *A, void *B,
                        const long *boff, int *info, int prec, const palial_opts *opts, palial_report *rep);
 
