@@ -14,6 +14,7 @@
 // reduction by shuffles) while the other waves build the T column.
 #include "internal.hpp"
 #include <float.h>
+#include <type_traits>
 
 namespace palial {
 
@@ -273,7 +274,7 @@ __device__ inline double wave_sum64(double v)
 // the double-buffered sv) while the other waves finish step i.  Column c of
 // the diagonal T block is formed in step c+1, 16 lanes per row of T.
 template <bool GE>
-__device__ void panel_body(double *A1, double *A2, double *T, double *tau_out, int b, int c0)
+__device__ __forceinline__ void panel_body(double *A1, double *A2, double *T, double *tau_out, int b, int c0)
 {
     constexpr int SV = 10;              // sv stride per lane (80 B: b128 reads without bank conflicts)
     __shared__ double sH[QP][QP + 1];   // head rows: sH[h][c] = (head row h, panel column c)
@@ -349,7 +350,6 @@ __device__ void panel_body(double *A1, double *A2, double *T, double *tau_out, i
     for (int i = 0; i < QP; i++) {
         __syncthreads();
         const double tau = sTau[i];
-        if (i > 0) tcol(i - 1);
         const double *vp = sv[i & 1] + g * SV;
         double v[8];
 #pragma unroll
@@ -358,28 +358,24 @@ __device__ void panel_body(double *A1, double *A2, double *T, double *tau_out, i
             v[s] = x.x;
             v[s + 1] = x.y;
         }
-        double d[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
+        const bool hrow = GE && g < QP && g > i;   // head row of reflector i below its 1
+        const double vh = hrow ? sH[g][i] : 0.0;
+        // the wave holding column i+1 (slot qn) updates that column first and
+        // builds reflector i+1 before its other columns: that is the path to
+        // the next barrier.  slot<q> keeps the register index static.
+        auto slot = [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int jc = w + 8 * q;
+            if (jc == i) return;
             double d0 = 0.0, d1 = 0.0;
 #pragma unroll
             for (int s = 0; s < 8; s += 2) {
                 d0 += v[s] * a[q][s];
                 d1 += v[s + 1] * a[q][s + 1];
             }
-            d[q] = d0 + d1;
-        }
-        const bool hrow = GE && g < QP && g > i;   // head row of reflector i below its 1
-        const double vh = hrow ? sH[g][i] : 0.0;
-        if (hrow) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) d[q] += vh * sH[g][w + 8 * q];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int jc = w + 8 * q;
-            if (jc == i) continue;
-            const double dq = wave_sum64(d[q]);
+            double dq = d0 + d1;
+            if (hrow) dq += vh * sH[g][jc];
+            dq = wave_sum64(dq);
             if (jc > i) {
                 const double f = tau * (sH[i][jc] + dq);
 #pragma unroll
@@ -389,8 +385,27 @@ __device__ void panel_body(double *A1, double *A2, double *T, double *tau_out, i
             } else if (g == 0) {
                 sG[jc][i] = GE ? sH[i][jc] + dq : dq;
             }
+        };
+        using S0 = std::integral_constant<int, 0>;
+        using S1 = std::integral_constant<int, 1>;
+        using S2 = std::integral_constant<int, 2>;
+        using S3 = std::integral_constant<int, 3>;
+        const bool owner = i + 1 < QP && w == ((i + 1) & 7);
+        const int qn = owner ? (i + 1) >> 3 : -1;
+        if (owner) {
+            switch (qn) {
+            case 0: slot(S0{}); break;
+            case 1: slot(S1{}); break;
+            case 2: slot(S2{}); break;
+            default: slot(S3{}); break;
+            }
+            reflect(i + 1);
         }
-        if (i + 1 < QP && w == ((i + 1) & 7)) reflect(i + 1);
+        if (qn != 0) slot(S0{});
+        if (qn != 1) slot(S1{});
+        if (qn != 2) slot(S2{});
+        if (qn != 3) slot(S3{});
+        if (i > 0) tcol(i - 1);
     }
     __syncthreads();
     tcol(QP - 1);
