@@ -152,25 +152,67 @@ hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N
 }
 
 // One b x b column-major tile <-> its (b/bs) x (b/bs) sub-tiles in layout L
-// (sub-tile (i,j) at bs*bs*(i + (b/bs)*j), column-major inside).
-template <typename TT>
-__global__ void tile_split_kernel(double *__restrict__ sub, TT *__restrict__ tile, int b, int bs, int to_sub)
+// (sub-tile (i,j) at bs*bs*(i + (b/bs)*j), column-major inside).  A thread
+// moves runs of 4 rows of one column (b and bs are multiples of 4, so a run
+// stays inside one sub-tile), 4 runs in flight; loads are clamped into the
+// tile so that only the stores are guarded.  The grid is small: the diagonal
+// chain that calls this may own only a few reserved CUs.
+template <typename TT> __device__ inline void ld4(const TT *p, double (&v)[4]);
+template <> __device__ inline void ld4<float>(const float *p, double (&v)[4])
 {
-    const int n = b * b, ms = b / bs;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const int r = e % b, c = e / b;
-        const size_t so = tile_off(bs, ms, r / bs, c / bs) + (size_t)(r % bs) + (size_t)(c % bs) * bs;
-        if (to_sub) sub[so] = (double)tile[e];
-        else tile[e] = (TT)sub[so];
+    const float4 f = *(const float4 *)p;
+    v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+}
+template <> __device__ inline void ld4<double>(const double *p, double (&v)[4])
+{
+    const double2 a = *(const double2 *)p, c = *(const double2 *)(p + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = c.x; v[3] = c.y;
+}
+template <typename TT> __device__ inline void st4(TT *p, const double (&v)[4]);
+template <> __device__ inline void st4<float>(float *p, const double (&v)[4])
+{
+    *(float4 *)p = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+}
+template <> __device__ inline void st4<double>(double *p, const double (&v)[4])
+{
+    *(double2 *)p = make_double2(v[0], v[1]);
+    *(double2 *)(p + 2) = make_double2(v[2], v[3]);
+}
+
+template <typename TT>
+__global__ __launch_bounds__(256) void tile_split_kernel(double *__restrict__ sub, TT *__restrict__ tile, int b, int bs,
+                                                         int to_sub)
+{
+    const int bq = b / 4, nq = bq * b, ms = b / bs;
+    const int stride = gridDim.x * blockDim.x;
+    for (int q0 = blockIdx.x * blockDim.x + threadIdx.x; q0 < nq; q0 += 4 * stride) {
+        double v[4][4];
+        size_t so[4], to[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = min(q0 + u * stride, nq - 1);
+            const int r = 4 * (q % bq), c = q / bq;
+            to[u] = (size_t)r + (size_t)c * b;
+            so[u] = tile_off(bs, ms, r / bs, c / bs) + (size_t)(r % bs) + (size_t)(c % bs) * bs;
+            if (to_sub) ld4(tile + to[u], v[u]);
+            else ld4(sub + so[u], v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (q0 + u * stride >= nq) continue;
+            if (to_sub) st4(sub + so[u], v[u]);
+            else st4(tile + to[u], v[u]);
+        }
     }
 }
 
 hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s, bool tile32)
 {
-    if (b <= 0 || bs <= 0 || b % bs != 0) return hipErrorInvalidValue;
-    // one element per thread: the copy is latency-bound at this size
-    int blocks = (b * b + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
+    if (b <= 0 || bs <= 0 || b % bs != 0 || b % 4 != 0 || bs % 4 != 0) return hipErrorInvalidValue;
+    // 4 runs of 4 elements per thread and pass, at most 64 workgroups
+    const long nq = (long)b * b / 4;
+    int blocks = (int)((nq + 1023) / 1024);
+    if (blocks > 64) blocks = 64;
     if (tile32)
         hipLaunchKernelGGL(tile_split_kernel<float>, dim3(blocks), dim3(256), 0, s, sub, (float *)(void *)tile, b, bs,
                            to_sub ? 1 : 0);
