@@ -53,7 +53,8 @@ typedef struct palial_matrix_desc {
 #define PALIAL_PREC_FP64  0
 #define PALIAL_PREC_FP32  1
 #define PALIAL_PREC_BF16  2
-#define PALIAL_PREC_FP8   3   /* reserved, not built */
+#define PALIAL_PREC_FP8   3   /* E4M3 panels, one power-of-two scale per panel tile (largest magnitude in
+                                 [64, 128)), FP32 accumulation; ~2^-4 relative per operand */
 #define PALIAL_PREC_BF16X3 4  /* split BF16: each panel element as hi + lo BF16 terms, three MFMA
                                  products (hi*hi + hi*lo + lo*hi), ~2^-16 relative per product */
 
@@ -199,8 +200,8 @@ int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_
  * FP32, BF16 or split-BF16 operands with FP32 accumulation; the mixed tiers
  * factor an FP32 copy with FP32 panel solves when the factor tile is a
  * multiple of 128); the diagonal-tile factorizations, the residual b - A x
- * and the solution update stay FP64.  PALIAL_PREC_BF16X3 needs a factor tile
- * multiple of 128, N above it, and the default schedule.
+ * and the solution update stay FP64.  PALIAL_PREC_FP8 and PALIAL_PREC_BF16X3
+ * need a factor tile multiple of 128, N above it, and the default schedule.
  * B (column-major, leading dimension ldb >= N) holds the right-hand sides on
  * entry and the solution on return.  Requires b % 64 == 0 and b <= 512.
  * A mixed tier whose Cholesky breaks down is refactored with a diagonal

@@ -472,7 +472,7 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
         L.chk(hipErrorInvalidValue);     // the FP32 working copy needs the packed split schedule
         return;
     }
-    const double esz = prec == 2 ? 2.0 : 4.0;   // packed bytes per element (split BF16: hi + lo)
+    const double esz = prec == 2 ? 2.0 : prec == 3 ? 1.0 : 4.0;   // packed bytes per element (split BF16: hi + lo)
     const double sbt = fs ? 4.0 * b * b : tile_bytes(b);      // bytes of one stored tile
     auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * psb : 0); };
     char *pk1 = (char *)g.d_pack + 2 * psb;                    // S1's packed tile (fs)
@@ -599,7 +599,7 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
         L.chk(hipErrorInvalidValue);
         return;
     }
-    const double esz = prec == 2 ? 2.0 : 4.0;   // packed bytes per element (split BF16: hi + lo)
+    const double esz = prec == 2 ? 2.0 : prec == 3 ? 1.0 : 4.0;   // packed bytes per element (split BF16: hi + lo)
     const double sbt = fs ? 4.0 * b * b : tile_bytes(b);
     auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * psb : 0); };
     char *pk1U = (char *)g.d_pack + 2 * psb, *pk1L = pk1U + (size_t)b * b * 4;   // S1's packed tiles (fs)
@@ -1055,7 +1055,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     if (bA % 64) return fail(PALIAL_ERR_ARG, "solve needs the matrix tile size %% 64 == 0 (b=%d)", bA);
     if (nrhs < 1 || nrhs > 16) return fail(PALIAL_ERR_ARG, "nrhs=%d outside 1..16", nrhs);
     if (!B || ldb < N) return fail(PALIAL_ERR_ARG, "bad right-hand side (ldb=%d)", ldb);
-    if (o.precision < 0 || o.precision > 4 || o.precision == 3)
+    if (o.precision < 0 || o.precision > 4)
         return fail(PALIAL_ERR_ARG, "precision %d not available", o.precision);
     const double tol = o.tol > 0 ? o.tol : 1e-14;
     const int max_iter = o.max_iter > 0 ? o.max_iter : 30;
@@ -1070,9 +1070,10 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     // itself runs on the FP32 copy (trailing updates move 8 instead of 16 bytes
     // per element; diagonal tiles are factored in FP64 copies)
     const bool fs = f32 && g.f32work && b % 128 == 0 && MT > 1 && g.dag == 2;
-    if (o.precision == 4 && !fs)
-        return fail(PALIAL_ERR_ARG, "split BF16 needs the FP32 working copy on the split chain schedule "
-                    "(factor tile %% 128 == 0, N > tile, PALIAL_DAG=2, PALIAL_F32WORK=1)");
+    if ((o.precision == 3 || o.precision == 4) && !fs)
+        return fail(PALIAL_ERR_ARG, "%s needs the FP32 working copy on the split chain schedule "
+                    "(factor tile %% 128 == 0, N > tile, PALIAL_DAG=2, PALIAL_F32WORK=1)",
+                    o.precision == 3 ? "FP8" : "split BF16");
     if ((!fs || b != bA) && (rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
     if (fs && (rc = grow(&g.d_sub, &g.sub_cap, (size_t)b * b))) return rc;
     if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
@@ -1167,7 +1168,8 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     // unit roundoff of the tier's update operands: the diagonal shift that
     // keeps a low-precision Cholesky from breaking down (Higham and Pranesh,
     // SIAM J. Sci. Comput. 43(1), 2021) starts at 4 u and grows 16x per retry
-    const double u_tier = o.precision == 1 ? 0x1p-24 : o.precision == 2 ? 0x1p-8 : o.precision == 4 ? 0x1p-16 : 0.0;
+    const double u_tier = o.precision == 1 ? 0x1p-24 : o.precision == 2 ? 0x1p-8 : o.precision == 3 ? 0x1p-4
+                        : o.precision == 4 ? 0x1p-16 : 0.0;
     int shifts = 0;
     double sigma = 0.0;
     int info = 0;
@@ -1756,7 +1758,7 @@ int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double
     if (k < 0 || T <= 0 || reps <= 0) return fail(PALIAL_ERR_ARG, "bad step %d or reps %d", k, reps);
     const bool w32 = (prec & 8) != 0;    // FP32 tiles (the first N*N*4 bytes of the buffer)
     prec &= 7;
-    if (prec < 0 || prec > 4 || prec == 3 || (w32 && prec == 0))
+    if (prec < 0 || prec > 4 || (w32 && prec == 0))
         return fail(PALIAL_ERR_ARG, "precision %d not available", prec);
     if (prec > 0 && (b % 128 != 0)) return fail(PALIAL_ERR_ARG, "packed updates need b %% 128 == 0 (b=%d)", b);
     if (prec > 0 && (rc = ensure_pack(b, MT))) return rc;

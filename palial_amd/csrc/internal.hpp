@@ -141,9 +141,11 @@ hipError_t launch_getrf2(double *tile, int b, double *dinvU, double *dinvL, hipS
 hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s);
 
 // low-precision trailing updates (k_gemm_lp.hip): prec 1 = FP32, 2 = BF16
-// Panel tiles of step k rounded to `prec` (1 FP32, 2 BF16), K-contiguous:
-// row-panel tiles (k, first..first+n-1) keep their layout, column-panel tiles
-// (first.., k) are transposed; tile j lands at dst + (j-k-1)*b*b elements.
+// Panel tiles of step k rounded to `prec` (1 FP32, 2 BF16, 3 FP8 with a
+// per-tile power-of-two scale, 4 split BF16), K-contiguous: row-panel tiles
+// (k, first..first+n-1) keep their layout, column-panel tiles (first.., k)
+// are transposed; tile j lands at dst + (j-k-1)*b*b elements (FP8: in a slot
+// of b*b + 512 bytes that also holds the scale; split BF16: 2*b*b elements).
 // src32: mat holds FP32 tiles (the FP32 working copy; same element offsets)
 hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n, bool colpanel, int prec,
                        void *dst, hipStream_t s, bool src32 = false);

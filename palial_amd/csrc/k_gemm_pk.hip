@@ -26,6 +26,7 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef long l2 __attribute__((ext_vector_type(2)));
 
 __device__ inline unsigned short to_bf16(double d)
 {
@@ -97,6 +98,77 @@ __device__ inline void epilogue(const TaskPtrs &tp, const f4 (&acc)[FM][FM], int
     }
 }
 
+// FP8 panels (precision 3, E4M3): packed tile t is a slot of fp8_slot(b)
+// bytes at t * fp8_slot(b): b*b bytes in the K-contiguous layout of the other
+// tiers, then the maxima |FP32(x)| of its 64 x 64 blocks (floats, 256 bytes),
+// then the tile's scale exponent e (int).  The stored values are
+// FP8(FP32(x) * 2^e) with e = 7 - ex, mx = f * 2^ex, f in [0.5, 1), mx the
+// tile's largest magnitude: scaled magnitudes stay below 128, inside the E4M3
+// range.  An update divides the product of two tiles by 2^(e_a + e_b), which
+// is exact.
+__host__ __device__ constexpr size_t fp8_slot(int b) { return (size_t)b * b + 512; }
+
+__device__ inline void ld4f(const float *p, float (&v)[4])
+{
+    const float4 f = *(const float4 *)p;
+    v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+}
+__device__ inline void ld4f(const double *p, float (&v)[4])
+{
+    const double2 a = *(const double2 *)p, c = *(const double2 *)(p + 2);
+    v[0] = (float)a.x; v[1] = (float)a.y; v[2] = (float)c.x; v[3] = (float)c.y;
+}
+
+// One workgroup per 64 x 64 block of one panel tile (the grid of pack_kernel):
+// max |FP32(x)| of the block into its tile's slot.  NaNs are skipped.
+template <typename ST>
+__global__ __launch_bounds__(256) void block_max_kernel(const ST *mat, int b, int MT, int k, int first, int colp,
+                                                        unsigned char *dst)
+{
+    const int nsb = b / 64, per = nsb * nsb;
+    const int tix = blockIdx.x / per, sbk = blockIdx.x % per;
+    const int r0 = (sbk % nsb) * 64, c0 = (sbk / nsb) * 64;
+    const int idx = first + tix;
+    const ST *src = colp ? mat + tile_off(b, MT, idx, k) : mat + tile_off(b, MT, k, idx);
+    const int t = threadIdx.x;
+    float m = 0.f;
+#pragma unroll
+    for (int pass = 0; pass < 4; pass++) {
+        const int e = t + pass * 256;                  // 16 runs of 4 rows per column, 64 columns
+        float v[4];
+        ld4f(src + (size_t)(r0 + (e % 16) * 4) + (size_t)(c0 + e / 16) * b, v);
+#pragma unroll
+        for (int q = 0; q < 4; q++) m = fmaxf(m, fabsf(v[q]));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    __shared__ float sm[4];
+    if ((t & 63) == 0) sm[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+        float *bm = (float *)(dst + (size_t)(idx - k - 1) * fp8_slot(b) + (size_t)b * b);
+        bm[sbk] = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+    }
+}
+
+// scale exponent of packed FP8 tile `slot` from its block maxima
+__device__ inline int fp8_exponent(const unsigned char *slot, int b)
+{
+    const float *bm = (const float *)(slot + (size_t)b * b);
+    const int per = (b / 64) * (b / 64);
+    float mx = 0.f;
+    for (int i = 0; i < per; i++) mx = fmaxf(mx, bm[i]);
+    int ex = 0;
+    if (mx > 0.f && mx <= 3.402823466e38f) frexpf(mx, &ex);
+    return mx > 0.f && mx <= 3.402823466e38f ? 7 - ex : 0;
+}
+
+__device__ inline int fp8x4(float a, float b, float c, float d)
+{
+    int v = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    return __builtin_amdgcn_cvt_pk_fp8_f32(c, d, v, true);
+}
+
 // One workgroup per 64 x 64 sub-block of one panel tile.
 template <int PREC, typename ST>
 __global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT, int k, int first, int colp,
@@ -108,9 +180,16 @@ __global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT,
     const int r0 = (sbk % nsb) * 64, c0 = (sbk / nsb) * 64;
     const int idx = first + tix;
     const ST *src = colp ? mat + tile_off(b, MT, idx, k) : mat + tile_off(b, MT, k, idx);
-    // PREC 4 (split BF16): tile t holds hi at t*2b^2 and lo at t*2b^2 + b^2 (BF16 elements)
-    const size_t dbase = (size_t)(idx - k - 1) * b * b * (PREC == 4 ? 2 : 1);
+    // PREC 4 (split BF16): tile t holds hi at t*2b^2 and lo at t*2b^2 + b^2 (BF16 elements);
+    // PREC 3 (FP8): dbase counts bytes (slot t at t*fp8_slot(b))
+    const size_t dbase = PREC == 3 ? (size_t)(idx - k - 1) * fp8_slot(b) : (size_t)(idx - k - 1) * b * b * (PREC == 4 ? 2 : 1);
     const int t = threadIdx.x;
+    int e8 = 0;
+    if constexpr (PREC == 3) {
+        unsigned char *slot = (unsigned char *)dst + dbase;
+        e8 = fp8_exponent(slot, b);
+        if (sbk == 0 && t == 0) *(int *)(slot + (size_t)b * b + 256) = e8;   // read by the update
+    }
     if (!colp) {
         // same layout: 4 consecutive rows of one column per thread and pass
 #pragma unroll
@@ -137,6 +216,9 @@ __global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT,
                 bf16_split((float)y.x, h.z, l.z); bf16_split((float)y.y, h.w, l.w);
                 *(ushort4 *)((unsigned short *)dst + dbase + o) = h;
                 *(ushort4 *)((unsigned short *)dst + dbase + (size_t)b * b + o) = l;
+            } else if (PREC == 3) {
+                *(int *)((unsigned char *)dst + dbase + o) =
+                    fp8x4(ldexpf((float)x.x, e8), ldexpf((float)x.y, e8), ldexpf((float)y.x, e8), ldexpf((float)y.y, e8));
             } else {
                 *(float4 *)((float *)dst + dbase + o) = make_float4((float)x.x, (float)x.y, (float)y.x, (float)y.y);
             }
@@ -163,6 +245,8 @@ __global__ __launch_bounds__(256) void pack_kernel(const ST *mat, int b, int MT,
                 bf16_split(sT[q][r], h, l);
                 ((unsigned short *)dst)[o] = h;
                 ((unsigned short *)dst)[o + (size_t)b * b] = l;
+            } else if (PREC == 3) {
+                ((unsigned char *)dst)[o] = (unsigned char)(__builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(sT[q][r], e8), 0.f, 0, false) & 0xff);
             } else {
                 ((float *)dst)[o] = sT[q][r];
             }
@@ -314,7 +398,12 @@ __device__ inline void wait_vm(int n)
 // each stage also holds the lo slices and acc += B_hi A_hi + B_lo A_hi + B_hi A_lo,
 // products good to ~2^-16 instead of 2^-8 at three times the MFMA work.
 // NST stages, NST-1 of them in flight; LDS = NST * (2 or 4) * 8 KB.
-template <int MINB, bool F32C, bool X3 = false, int NST = 4>
+// F8 (FP8 E4M3 panels in fp8_slot(b) slots): a slice row holds 64 K values
+// in the same 64 bytes; of a lane's 16-byte chunk (K values 16 (l/16) .. +15
+// of the slice) the low 8 bytes feed one v_mfma_f32_16x16x32_fp8_fp8 and the
+// high 8 bytes a second, so A and B pair the same K values.  alpha is scaled
+// by 2^-(e_a + e_b).
+template <int MINB, bool F32C, bool X3 = false, int NST = 4, bool F8 = false>
 __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 {
     const Store<F32C> S{p.mat};
@@ -337,27 +426,30 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
-    const size_t tstride = (size_t)b * b * (X3 ? 2 : 1);
-    const unsigned short *PA = (const unsigned short *)p.pa + (size_t)tp.ia * tstride;
-    const unsigned short *PB = (const unsigned short *)p.pb + (size_t)tp.ib * tstride;
-    const int NS = b / KB;
+    // byte addressing: a panel row is rs bytes, a slice 64 bytes of it
+    const size_t tstride = F8 ? fp8_slot(b) : (size_t)b * b * (X3 ? 4 : 2);
+    const char *PA = (const char *)p.pa + (size_t)tp.ia * tstride;
+    const char *PB = (const char *)p.pb + (size_t)tp.ib * tstride;
+    const size_t rs = F8 ? (size_t)b : 2 * (size_t)b;
+    const int NS = F8 ? b / 64 : b / KB;
 
     auto issue = [&](int s) {
-        const int st = s % NST, kb = s * KB;
+        const int st = s % NST;
+        const size_t kb = (size_t)s * 64;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int j = w * 2 + q;                    // 1 KB chunk of the slice
             const int row = 16 * j + (lane >> 2);
             const int c = (lane & 3) ^ ((row >> 2) & 3);
-            __builtin_amdgcn_global_load_lds((const void *)(PA + (size_t)(i0 + row) * b + kb + c * 8),
+            __builtin_amdgcn_global_load_lds((const void *)(PA + (size_t)(i0 + row) * rs + kb + c * 16),
                                              (lds_void_t *)(lds[st][0] + j * 1024), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds((const void *)(PB + (size_t)(j0 + row) * b + kb + c * 8),
+            __builtin_amdgcn_global_load_lds((const void *)(PB + (size_t)(j0 + row) * rs + kb + c * 16),
                                              (lds_void_t *)(lds[st][1] + j * 1024), 16, 0, 0);
             if (X3) {
-                const size_t lo = (size_t)b * b;
-                __builtin_amdgcn_global_load_lds((const void *)(PA + lo + (size_t)(i0 + row) * b + kb + c * 8),
+                const size_t lo = 2 * (size_t)b * b;    // bytes from hi to lo
+                __builtin_amdgcn_global_load_lds((const void *)(PA + lo + (size_t)(i0 + row) * rs + kb + c * 16),
                                                  (lds_void_t *)(lds[st][NOPS - 2] + j * 1024), 16, 0, 0);
-                __builtin_amdgcn_global_load_lds((const void *)(PB + lo + (size_t)(j0 + row) * b + kb + c * 8),
+                __builtin_amdgcn_global_load_lds((const void *)(PB + lo + (size_t)(j0 + row) * rs + kb + c * 16),
                                                  (lds_void_t *)(lds[st][NOPS - 1] + j * 1024), 16, 0, 0);
             }
         }
@@ -404,12 +496,28 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
                     acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], al[x], acc[x][c], 0, 0, 0);
                 }
         }
+        if constexpr (F8) {
 #pragma unroll
-        for (int x = 0; x < FM; x++)
+            for (int x = 0; x < FM; x++)
 #pragma unroll
-            for (int c = 0; c < FM; c++)
-                acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], av[x], acc[x][c], 0, 0, 0);
+                for (int c = 0; c < FM; c++) {
+                    const l2 a2 = __builtin_bit_cast(l2, av[x]), b2 = __builtin_bit_cast(l2, bv[c]);
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(b2[0], a2[0], acc[x][c], 0, 0, 0);
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(b2[1], a2[1], acc[x][c], 0, 0, 0);
+                }
+        } else {
+#pragma unroll
+            for (int x = 0; x < FM; x++)
+#pragma unroll
+                for (int c = 0; c < FM; c++)
+                    acc[x][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], av[x], acc[x][c], 0, 0, 0);
+        }
         asm volatile("" ::: "memory");
+    }
+    double alpha_d = p.alpha;
+    if constexpr (F8) {
+        const int ea = *(const int *)(PA + (size_t)b * b + 256), eb = *(const int *)(PB + (size_t)b * b + 256);
+        alpha_d = ldexp(p.alpha, -(ea + eb));
     }
 
     if constexpr (F32C) {
@@ -423,7 +531,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
         float *sc = (float *)(void *)&lds[0][0][0] + w * 64 * LDC;
         float *Cf = (float *)(void *)const_cast<double *>(p.mat) + (size_t)(tp.C - p.mat);
         const float *Df = tp.D ? (const float *)(const void *)p.mat + (size_t)(tp.D - p.mat) : nullptr;
-        const float alpha = (float)p.alpha, beta = (float)p.beta;
+        const float alpha = (float)alpha_d, beta = (float)p.beta;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int r0 = i0 + wm * WM + 32 * h + 4 * (lane & 7);      // first of this lane's 4 rows
@@ -476,7 +584,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 #pragma unroll
             for (int v = 0; v < 4; v++) dv[c][v] = (tp.D && live(x, c, v)) ? S.ld(tp.D, cidx(x, c, v)) : 0.0;
     };
-    const double alpha = p.alpha, beta = p.beta;
+    const double alpha = alpha_d, beta = p.beta;
     auto stC = [&](int x, const double (&dv)[FM][4]) {
 #pragma unroll
         for (int c = 0; c < FM; c++)
@@ -581,12 +689,20 @@ hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n,
                        void *dst, hipStream_t s, bool src32)
 {
     if (n <= 0) return hipSuccess;
-    if (b % 64 != 0 || first <= k || first + n > N / b || (prec != 1 && prec != 2 && prec != 4))
+    if (b % 64 != 0 || first <= k || first + n > N / b || (prec != 1 && prec != 2 && prec != 3 && prec != 4))
         return hipErrorInvalidValue;
     const int nsb = b / 64, cp = colpanel ? 1 : 0;
     dim3 grid((unsigned)(n * nsb * nsb)), block(256);
     const float *m32 = (const float *)(const void *)mat;
-    if (prec == 4 && src32) hipLaunchKernelGGL((pack_kernel<4, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
+    if (prec == 3) {
+        // block maxima first (same grid), then the scaled FP8 rounding
+        unsigned char *d8 = (unsigned char *)dst;
+        if (src32) hipLaunchKernelGGL((block_max_kernel<float>), grid, block, 0, s, m32, b, N / b, k, first, cp, d8);
+        else hipLaunchKernelGGL((block_max_kernel<double>), grid, block, 0, s, mat, b, N / b, k, first, cp, d8);
+        if (src32) hipLaunchKernelGGL((pack_kernel<3, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
+        else hipLaunchKernelGGL((pack_kernel<3, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
+    }
+    else if (prec == 4 && src32) hipLaunchKernelGGL((pack_kernel<4, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
     else if (prec == 4) hipLaunchKernelGGL((pack_kernel<4, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
     else if (prec == 2 && src32) hipLaunchKernelGGL((pack_kernel<2, float>), grid, block, 0, s, m32, b, N / b, k, first, cp, dst);
     else if (prec == 2) hipLaunchKernelGGL((pack_kernel<2, double>), grid, block, 0, s, mat, b, N / b, k, first, cp, dst);
@@ -604,7 +720,11 @@ hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
         return hipErrorInvalidValue;
     const long nbd = a.b / 128;
     dim3 grid((unsigned)((long)a.ntasks * nbd * nbd)), block(256);
-    if (prec == 4) {
+    if (prec == 3) {
+        if (a.f32c) hipLaunchKernelGGL((gemm_bf16d_kernel<2, true, false, 4, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((gemm_bf16d_kernel<2, false, false, 4, true>), grid, block, 0, s, a);
+    }
+    else if (prec == 4) {
         // two stages (64 KB LDS, two workgroups per CU)
         if (a.f32c) hipLaunchKernelGGL((gemm_bf16d_kernel<2, true, true, 2>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((gemm_bf16d_kernel<2, false, true, 2>), grid, block, 0, s, a);

@@ -2,7 +2,8 @@
 through the palial_hip_update test hook, in every precision tier.
 
 The model is numpy in FP64 on operands rounded exactly as the device rounds
-them (FP64 -> FP32 nearest, then BF16 nearest-even): the only remaining
+them (FP64 -> FP32 nearest, then BF16 nearest-even, or E4M3 nearest-even
+after a per-tile power-of-two scale): the only remaining
 difference is the FP32 accumulation order, bounded by b * 2^-24 * sum |a||b|.
 The FP64 tier is checked with the FP64 unit roundoff in place of 2^-24.  The Cholesky update
 touches only tiles i <= j (and the upper triangle of diagonal tiles), as
@@ -31,13 +32,43 @@ def split_bf16(x):
     return hi, lo
 
 
+def round_e4m3(y):
+    """E4M3 nearest-even of values below 128 in magnitude: 3 fraction bits,
+    spacing 2^-9 below the smallest normal 2^-6."""
+    a = np.abs(y)
+    e = np.floor(np.log2(np.where(a > 0, a, 1.0)))
+    q = 2.0 ** (np.maximum(e, -6.0) - 3.0)
+    return np.where(a > 0, np.round(y / q) * q, y)
+
+
+def round_fp8_tiles(P, b, axis):
+    """FP8 panels (precision 3): each b x b tile of P (tiles along `axis`) is
+    rounded to FP32, scaled by 2^e, e = 7 - frexp(max |tile|)[1], rounded to
+    E4M3 and scaled back (both scalings exact)."""
+    out = np.empty_like(P)
+    for t in range(P.shape[axis] // b):
+        sl = [slice(None), slice(None)]
+        sl[axis] = slice(t * b, (t + 1) * b)
+        f = round_fp32(P[tuple(sl)])
+        mx = float(np.abs(f).max())
+        e = 7 - np.frexp(mx)[1] if mx > 0 else 0
+        out[tuple(sl)] = round_e4m3(f * 2.0 ** e) * 2.0 ** -e
+    return out
+
+
 ROUND = {0: lambda x: x, 1: round_fp32, 2: round_bf16}
-TOL = {0: 4.0 * 2.0 ** -53, 1: 4.0 * 2.0 ** -24, 2: 4.0 * 2.0 ** -24, 4: 4.0 * 2.0 ** -24}
+TOL = {0: 4.0 * 2.0 ** -53, 1: 4.0 * 2.0 ** -24, 2: 4.0 * 2.0 ** -24, 3: 4.0 * 2.0 ** -24, 4: 4.0 * 2.0 ** -24}
 
 
 def model_update(A, s0, s1, algo, prec):
     """Step-k update with operands as the device rounds them; the split tier
-    sums hi*hi + hi*lo + lo*hi (lo*lo is not formed)."""
+    sums hi*hi + hi*lo + lo*hi (lo*lo is not formed); the FP8 tier scales
+    each panel tile by its own power of two."""
+    b = s1 - s0
+    if prec == 3:
+        Ur = round_fp8_tiles(A[s0:s1, s1:], b, axis=1)
+        Lr = Ur.T if algo == "cholesky" else round_fp8_tiles(A[s1:, s0:s1], b, axis=0)
+        return Lr @ Ur, np.abs(Lr) @ np.abs(Ur)
     if algo == "cholesky":
         P = A[s0:s1, s1:]
         if prec == 4:
@@ -55,7 +86,7 @@ def model_update(A, s0, s1, algo, prec):
 
 
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
-@pytest.mark.parametrize("prec", [0, 1, 2, 4])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("N,b,k", [(1024, 256, 0), (1024, 256, 1), (1536, 512, 0), (768, 128, 2)])
 def test_trailing_update(hip, orc, algo, prec, N, b, k):
     a = orc.gen("dense", N, b, seed=11 + k, shift=0.0)
@@ -91,7 +122,7 @@ def test_trailing_update(hip, orc, algo, prec, N, b, k):
 
 
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
-@pytest.mark.parametrize("prec", [1, 2, 4])
+@pytest.mark.parametrize("prec", [1, 2, 3, 4])
 @pytest.mark.parametrize("N,b,k", [(1024, 256, 0), (1536, 512, 1)])
 def test_trailing_update_fp32_storage(hip, orc, algo, prec, N, b, k):
     """The same update on FP32 tiles (the mixed-precision solves' working copy,
