@@ -7,7 +7,7 @@ Householder QR of seeded Gaussians (a geometric spectrum factorises over the
 Kronecker index, so A = A1 (x) A2 is generated on the device in O(N^2)).
 Each precision tier factors a device copy of A and refines in FP64 until the
 backward error reaches 1e-14.  The FP64 tier factors an FP64 copy; the mixed
-tiers factor an FP32 copy (trailing updates from BF16 or FP32 packed panels
+tiers factor an FP32 copy (trailing updates from FP32, BF16, FP8 or split BF16 packed panels
 with FP32 accumulation, panel TRSMs in FP32, diagonal tiles in FP64).  The
 timed region covers copy + factor + triangular solves + every refinement
 step, with A and b resident in HBM.
@@ -87,7 +87,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
     out = {}
     for algo in algos:
         for prec in tiers:
-            name = {0: "fp64", 1: "fp32", 2: "bf16", 4: "bf16x3"}[prec]
+            name = {0: "fp64", 1: "fp32", 2: "bf16", 3: "fp8", 4: "bf16x3"}[prec]
             best = None
             for ft in factor_tiles:
                 times, rep, rc = [], None, 0
@@ -105,6 +105,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
                 r = {"ms": round(t, 2), "factor_tile": ft, "factor_ms": round(rep.factor_ms, 2),
                      "refine_ms": round(rep.solve_ms, 2), "iterations": rep.iterations,
                      "backward_error": rep.backward_error, "converged": rc == 0,
+                     "precision_used": rep.precision_used,
                      "tflops": round(flops / (t * 1e-3) / 1e12, 3)}
                 log(f"[bench] solve {algo} {name} ft={ft} kappa={kappa:g}: {t:.1f} ms (factor {rep.factor_ms:.1f}, "
                     f"refine {rep.solve_ms:.1f}, {rep.iterations} it, eta={rep.backward_error:.2e}, rc={rc})")
@@ -118,7 +119,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
     prof = None
     if conv:
         key = min(conv, key=lambda k: conv[k]["ms"])
-        prec = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 4}[key.split("_")[1]]
+        prec = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3, "bf16x3": 4}[key.split("_")[1]]
         P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8))
         P.synchronize()
         P.profile(True)
@@ -133,7 +134,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
 
 
 def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16),
-              tiers=(0, 1, 2, 4), solve_n=32768, algos=("cholesky", "lu", "qr"), solves=True):
+              tiers=(0, 1, 2, 3, 4), solve_n=32768, algos=("cholesky", "lu", "qr"), solves=True):
     """Survey configs 2 and 3 beside the headline (opt-in, --sweep):
     FP64 factorizations of Cholesky (S + N I), LU (G + N I) and QR (G) at
     b = 512 for each N (device time of one factorization after a warm-up),
@@ -175,7 +176,7 @@ def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), 
         for nrhs in nrhss:
             rhs = np.random.default_rng(1).standard_normal((nrhs, N))      # column j = row j (column-major)
             for prec in tiers:
-                name = {0: "fp64", 1: "fp32", 2: "bf16", 4: "bf16x3"}[prec]
+                name = {0: "fp64", 1: "fp32", 2: "bf16", 3: "fp8", 4: "bf16x3"}[prec]
                 res = None
                 for i in range(2):
                     P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8 * nrhs))
@@ -187,7 +188,7 @@ def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), 
                     res = {"ms": round((t1 - t0) * 1e3, 2), "factor_ms": round(rep.factor_ms, 2),
                            "refine_ms": round(rep.solve_ms, 2), "iterations": rep.iterations,
                            "gmres_iterations": rep.gmres_iterations, "backward_error": rep.backward_error,
-                           "converged": rc == 0}
+                           "converged": rc == 0, "precision_used": rep.precision_used}
                 sol[f"kappa{kappa:g}_nrhs{nrhs}_{name}"] = res
                 log(f"[sweep] cholesky solve N={N} kappa={kappa:g} nrhs={nrhs} {name}: {res['ms']:.1f} ms, "
                     f"{res['iterations']} it (+{res['gmres_iterations']} GMRES), eta={res['backward_error']:.2e}, "
@@ -273,7 +274,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=32768)
     ap.add_argument("--kappa", type=float, default=1e2)
-    ap.add_argument("--tiers", default="0,1,2,4")
+    ap.add_argument("--tiers", default="0,1,2,3,4")
     ap.add_argument("--factor-tiles", default="256,512", help="internal factorization tile sizes tried per tier")
     ap.add_argument("--no-factor", action="store_true", help="skip the plain factorization runs")
     ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")
