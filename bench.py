@@ -292,7 +292,8 @@ def main():
     P.init(0)
     if args.sweep:
         sw = run_sweep(P, b=args.b, ns=[int(x) for x in args.sweep_ns.split(",")],
-                       algos=args.sweep_algos.split(","), solves=bool(args.sweep_solves))
+                       tiers=[int(x) for x in args.tiers.split(",")],
+                       algos=[a for a in args.sweep_algos.split(",") if a], solves=bool(args.sweep_solves))
         print(json.dumps({"sweep": sw}), flush=True)
         P.finalize()
         return 0
