@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 NCLASS = 7                 # kernel classes of palial_hip_profile_stats
-NFEAT = NCLASS + 3 + 4     # class one-hot, log2 b, log10 flops, log10 bytes, tier one-hot
+NFEAT = NCLASS + 3 + 5     # class one-hot, log2 b, log10 flops, log10 bytes, tier one-hot
 HID = 64
 CLASSES = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4, "pack": 5, "pkgemm": 6}
 
@@ -29,9 +29,9 @@ def featurize(records):
     X[:, NCLASS] = np.log2(np.maximum(r[:, 1], 1.0)) / 9.0
     X[:, NCLASS + 1] = np.log10(r[:, 3] + 1.0) / 13.0
     X[:, NCLASS + 2] = np.log10(r[:, 4] + 1.0) / 11.0
-    # tiers: precision 0 FP64, 1 FP32, 2 BF16, 4 split BF16 -> columns 0..3
+    # tiers: precision 0 FP64, 1 FP32, 2 BF16, 3 FP8, 4 split BF16 -> columns 0..4
     tier = r[:, 2].astype(int)
-    X[np.arange(len(r)), NCLASS + 3 + np.where(tier >= 4, 3, tier.clip(0, 2))] = 1.0
+    X[np.arange(len(r)), NCLASS + 3 + tier.clip(0, 4)] = 1.0
     y = np.log(np.maximum(r[:, 5], 1e-4)).astype(np.float32)
     return X, y
 
@@ -143,7 +143,7 @@ def launches(algo, N, b, prec=0, split=True, fs=None):
     if fs is None:
         fs = packed and split and MT > 1
     sbt = 4.0 * b * b if fs else tb             # bytes of one stored tile
-    esz = 2.0 if prec == 2 else 4.0           # packed bytes per element (split BF16: hi + lo)
+    esz = {2: 2.0, 3: 1.0}.get(prec, 4.0)     # packed bytes per element (split BF16: hi + lo)
     out = []
     chol = algo == "cholesky"
     npan = 1 if chol else 2                      # packed panels per step

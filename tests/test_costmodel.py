@@ -24,11 +24,15 @@ def test_reference_gradient_matches_finite_differences():
 
 
 def test_featurize():
-    recs = np.array([[0, 512, 0, 2.7e11, 6e9, 3.4], [6, 256, 2, 1e9, 2e8, 0.1]])
+    recs = np.array([[0, 512, 0, 2.7e11, 6e9, 3.4], [6, 256, 2, 1e9, 2e8, 0.1],
+                     [6, 512, 3, 1e9, 2e8, 0.1], [6, 512, 4, 1e9, 2e8, 0.1]])
     X, y = cm.featurize(recs)
-    assert X.shape == (2, cm.NFEAT) and X.dtype == np.float32
+    assert X.shape == (4, cm.NFEAT) and X.dtype == np.float32
     assert X[0, 0] == 1 and X[1, 6] == 1 and X[0, cm.NCLASS + 3] == 1 and X[1, cm.NCLASS + 5] == 1
-    assert np.allclose(y, np.log([3.4, 0.1]))
+    # one tier column each: FP8 and split BF16 are not folded into BF16
+    assert X[2, cm.NCLASS + 6] == 1 and X[3, cm.NCLASS + 7] == 1
+    assert (X[:, cm.NCLASS + 3:].sum(1) == 1).all()
+    assert np.allclose(y, np.log([3.4, 0.1, 0.1, 0.1]))
 
 
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
