@@ -2,9 +2,10 @@
 // from packed low-precision panels.
 //
 // The panel of step k (row tiles U_kj of Cholesky, L_ik and U_kj of LU) is
-// rounded once to BF16 or FP32 by pack_kernel and stored K-contiguous, so
-// the update GEMM streams 2 (BF16) or 4 (FP32) bytes per operand element
-// instead of re-reading and converting FP64 tiles in every workgroup:
+// rounded once to FP32, BF16, split BF16 or FP8 (E4M3, per-tile scale) by
+// pack_kernel and stored K-contiguous, so the update GEMM streams 4, 2, 2 + 2
+// or 1 bytes per operand element instead of re-reading and converting FP64
+// tiles in every workgroup:
 //   row-panel tile (k, j) : element (q, c) at q + c*b   (its own layout)
 //   col-panel tile (i, k) : element (r, q) at q + r*b   (transposed)
 // with q the K index.  The FP64 output tile receives C = alpha*acc + beta*D.
@@ -15,6 +16,7 @@
 //
 // Operand layouts (lane l of 64):
 //   bf16 16x16x32: A[row = l%16][k = 8*(l/16) .. +7], B[k = 8*(l/16) .. +7][col = l%16]
+//   fp8  16x16x32: the same with one byte per k
 //   f32  16x16x4 : A[row = l%16][k = l/16],           B[k = l/16][col = l%16]
 //   f32 result   : D[row = 4*(l/16) + v][col = l%16]
 #include "gemm_common.hpp"
