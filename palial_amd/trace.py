@@ -1,0 +1,140 @@
+"""Per-task trace lines in the reference tracer's format for the Cholesky
+factorization: one `name, ms` line per tile task, in task-creation order, ms
+with six decimals (palial_trace/log.h:43-50).
+
+Names are built as src/cholesky.h:14-173 builds them: the task kind followed
+by k, m and n printed back to back, where m and n are whatever the loop
+variables hold when the task is created (so potrf of step k >= 1 carries the
+m = MT, n = MT - 1 that the previous step's loops left behind, and syrk
+carries the n of the previous column's gemm loop).
+
+Times come from the library's per-launch records (profile(2) before a
+factorization or solve, then profile_records()).  A launch of the chain
+schedule covers one kind of tile task of one step; its time is split over its
+tasks in proportion to their flops (potrf b^3/3, trsm and syrk b^3, gemm
+2 b^3).  Packing launches have no reference task and are left out, so the
+lines add up to the launches' time without them.  Launches are matched to the
+schedule's enumeration (costmodel.launches) group by group: within one
+(class, flops, bytes) shape both lists run in step order."""
+import numpy as np
+
+from . import costmodel as cm
+
+
+def cholesky_tasks(MT):
+    """[(name, task)] in creation order; task = ("potrf", k), ("trsm", k, m),
+    ("syrk", k, m) or ("gemm", k, m, n) (gemm updates tile (n, m))."""
+    out = []
+    k = m = n = 0
+    for k in range(MT):
+        out.append((f"potrf{k}{m}{n}", ("potrf", k)))
+        m = k + 1
+        while m < MT:
+            out.append((f"trsm{k}{m}{n}", ("trsm", k, m)))
+            m += 1
+        m = k + 1
+        while m < MT:
+            out.append((f"syrk{k}{m}{n}", ("syrk", k, m)))
+            n = k + 1
+            while n < m:
+                out.append((f"gemm{k}{m}{n}", ("gemm", k, m, n)))
+                n += 1
+            m += 1
+    return out
+
+
+def _covered(k, stream, cls, MT, split):
+    """Reference tasks of one enumerated launch of the Cholesky chain schedule."""
+    if cls == cm.CLASSES["diag"]:
+        return [("potrf", k)]
+    if cls == cm.CLASSES["trsm"]:
+        return [("trsm", k, k + 1)] if stream == "chain" else [("trsm", k, m) for m in range(k + 2, MT)]
+    if cls == cm.CLASSES["pack"]:
+        return []
+    if stream == "chain":                              # tile (k+1, k+1)
+        return [("syrk", k, k + 1)]
+    if not split:                                      # the rest of step k's update in one launch
+        return [("syrk", k, m) for m in range(k + 2, MT)] + \
+               [("gemm", k, m, n) for m in range(k + 2, MT) for n in range(k + 1, m)]
+    if stream == "first":                              # tile row k+1 and tile (k+2, k+2)
+        return [("gemm", k, m, k + 1) for m in range(k + 2, MT)] + [("syrk", k, k + 2)]
+    return [("syrk", k, m) for m in range(k + 3, MT)] + \
+           [("gemm", k, m, n) for m in range(k + 3, MT) for n in range(k + 2, m)]
+
+
+def _flops(task, b):
+    b3 = float(b) ** 3
+    return {"potrf": b3 / 3, "trsm": b3, "syrk": b3, "gemm": 2 * b3}[task[0]]
+
+
+def cholesky_task_times(records, N, b, prec=0, split=True):
+    """{task: ms} from the per-launch records of one Cholesky factorization
+    (the records of exactly one factorization, in issue order)."""
+    MT = N // b
+    enum = cm.launches("cholesky", N, b, prec, split=split)
+    rec = np.asarray(records, dtype=np.float64).reshape(-1, 6)
+    key = lambda c, f, by: (int(c), int(round(f)), int(round(by)))
+    groups = {}
+    for i, (k, st, c, f, by) in enumerate(enum):
+        groups.setdefault(key(c, f, by), []).append(i)
+    times = {}
+    seen = {g: 0 for g in groups}
+    ms_of = [None] * len(enum)
+    for c, _, _, f, by, ms in rec:
+        g = key(c, f, by)
+        if g not in groups or seen[g] >= len(groups[g]):
+            raise ValueError(f"record {g} has no launch left in the Cholesky schedule of N={N}, b={b}")
+        ms_of[groups[g][seen[g]]] = ms
+        seen[g] += 1
+    if any(t is None for t in ms_of):
+        raise ValueError("records cover only part of the schedule: pass the records of one factorization")
+    for (k, st, c, f, by), ms in zip(enum, ms_of):
+        tasks = _covered(k, st, c, MT, split)
+        tot = sum(_flops(t, b) for t in tasks)
+        for t in tasks:
+            times[t] = times.get(t, 0.0) + ms * _flops(t, b) / tot
+    return times
+
+
+def cholesky_trace_lines(records, N, b, prec=0, split=True):
+    """The reference tracer's `name, ms` lines for one Cholesky factorization."""
+    times = cholesky_task_times(records, N, b, prec, split)
+    return [f"{name}, {times[task]:f}" for name, task in cholesky_tasks(N // b)]
+
+
+def main(argv=None):
+    """python -m palial_amd.trace -m N -b B: one profiled FP64 Cholesky of a
+    generated SPD matrix (S + N I) on the GPU, printed as the reference
+    tracer prints its run."""
+    import argparse
+    from . import init, DeviceMatrix, GEN_SYM, profile, profile_records, factor_device
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("-m", type=int, default=8192, help="matrix size N")
+    ap.add_argument("-b", type=int, default=512, help="tile size")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    if args.m % args.b:
+        ap.error("the tile size must divide N")
+    init(0)
+    A = DeviceMatrix(args.m, args.b)
+    A.generate(GEN_SYM, seed=args.seed, shift=float(args.m))
+    profile(2)
+    info, _ = factor_device("cholesky", A)
+    rec = profile_records()
+    profile(0)
+    A.free()
+    if info != 0:
+        raise SystemExit(f"factorization failed (info {info})")
+    try:
+        lines = cholesky_trace_lines(rec, args.m, args.b)
+    except ValueError:                                  # the schedule without the split update
+        lines = cholesky_trace_lines(rec, args.m, args.b, split=False)
+    print("\n".join(lines))
+    return 0
+
+
+if __name__ == "__main__":
+    from . import finalize
+    rc = main()
+    finalize()
+    raise SystemExit(rc)
