@@ -1,12 +1,16 @@
-"""Per-task trace lines in the reference tracer's format for the Cholesky
-factorization: one `name, ms` line per tile task, in task-creation order, ms
-with six decimals (palial_trace/log.h:43-50).
+"""Per-task trace lines in the reference tracer's format for the Cholesky and
+LU factorizations: one `name, ms` line per tile task, in task-creation order,
+ms with six decimals (palial_trace/log.h:43-50).
 
 Names are built as src/cholesky.h:14-173 builds them: the task kind followed
 by k, m and n printed back to back, where m and n are whatever the loop
 variables hold when the task is created (so potrf of step k >= 1 carries the
 m = MT, n = MT - 1 that the previous step's loops left behind, and syrk
-carries the n of the previous column's gemm loop).
+carries the n of the previous column's gemm loop).  LU names follow
+src/lu.h:13-173 the same way; its k = 0 getrf and L-panel trsm print loop
+variables the reference never initialises (src/lu.h:13), printed here as 0.
+The Cholesky names are pinned to the reference's recorded trace; the
+reference holds no recorded LU trace.
 
 Times come from the library's per-launch records (profile(2) before a
 factorization or solve, then profile_records()).  A launch of the chain
@@ -43,6 +47,49 @@ def cholesky_tasks(MT):
     return out
 
 
+def lu_tasks(MT):
+    """[(name, task)] in creation order; task = ("getrf", k), ("trsmL", k, m)
+    (tile (m, k)), ("trsmU", k, n) (tile (k, n)) or ("gemm", k, m, n) (tile (m, n))."""
+    out = []
+    k = m = n = 0
+    for k in range(MT):
+        out.append((f"getrf{k}{m}{n}", ("getrf", k)))
+        m = k + 1
+        while m < MT:
+            out.append((f"trsm{k}{m}{n}", ("trsmL", k, m)))
+            m += 1
+        n = k + 1
+        while n < MT:
+            out.append((f"trsm{k}{m}{n}", ("trsmU", k, n)))
+            m = k + 1
+            while m < MT:
+                out.append((f"gemm{k}{m}{n}", ("gemm", k, m, n)))
+                m += 1
+            n += 1
+    return out
+
+
+def _covered_lu(k, stream, cls, MT, split, idx):
+    """Reference tasks of one enumerated launch of the LU chain schedule; idx:
+    0 for the first launch of its (step, stream, class), 1 for the second (the
+    row panel's TRSM is issued before the column panel's)."""
+    if cls == cm.CLASSES["diag"]:
+        return [("getrf", k)]
+    if cls == cm.CLASSES["trsm"]:
+        tiles = [k + 1] if stream == "chain" else list(range(k + 2, MT))
+        return [("trsmU", k, n) for n in tiles] if idx == 0 else [("trsmL", k, m) for m in tiles]
+    if cls == cm.CLASSES["pack"]:
+        return []
+    if stream == "chain":                              # tile (k+1, k+1)
+        return [("gemm", k, k + 1, k + 1)]
+    if not split:
+        return [("gemm", k, m, n) for n in range(k + 1, MT) for m in range(k + 1, MT) if (m, n) != (k + 1, k + 1)]
+    if stream == "first":                              # row and column k+1, and tile (k+2, k+2)
+        return [("gemm", k, k + 1, n) for n in range(k + 2, MT)] + \
+               [("gemm", k, m, k + 1) for m in range(k + 2, MT)] + [("gemm", k, k + 2, k + 2)]
+    return [("gemm", k, m, n) for n in range(k + 2, MT) for m in range(k + 2, MT) if (m, n) != (k + 2, k + 2)]
+
+
 def _covered(k, stream, cls, MT, split):
     """Reference tasks of one enumerated launch of the Cholesky chain schedule."""
     if cls == cm.CLASSES["diag"]:
@@ -64,14 +111,15 @@ def _covered(k, stream, cls, MT, split):
 
 def _flops(task, b):
     b3 = float(b) ** 3
-    return {"potrf": b3 / 3, "trsm": b3, "syrk": b3, "gemm": 2 * b3}[task[0]]
+    return {"potrf": b3 / 3, "getrf": 2 * b3 / 3, "trsm": b3, "trsmL": b3, "trsmU": b3, "syrk": b3,
+            "gemm": 2 * b3}[task[0]]
 
 
-def cholesky_task_times(records, N, b, prec=0, split=True):
-    """{task: ms} from the per-launch records of one Cholesky factorization
-    (the records of exactly one factorization, in issue order)."""
+def task_times(algo, records, N, b, prec=0, split=True):
+    """{task: ms} from the per-launch records of one factorization (the
+    records of exactly one factorization, in issue order)."""
     MT = N // b
-    enum = cm.launches("cholesky", N, b, prec, split=split)
+    enum = cm.launches(algo, N, b, prec, split=split)
     rec = np.asarray(records, dtype=np.float64).reshape(-1, 6)
     key = lambda c, f, by: (int(c), int(round(f)), int(round(by)))
     groups = {}
@@ -83,32 +131,45 @@ def cholesky_task_times(records, N, b, prec=0, split=True):
     for c, _, _, f, by, ms in rec:
         g = key(c, f, by)
         if g not in groups or seen[g] >= len(groups[g]):
-            raise ValueError(f"record {g} has no launch left in the Cholesky schedule of N={N}, b={b}")
+            raise ValueError(f"record {g} has no launch left in the {algo} schedule of N={N}, b={b}")
         ms_of[groups[g][seen[g]]] = ms
         seen[g] += 1
     if any(t is None for t in ms_of):
         raise ValueError("records cover only part of the schedule: pass the records of one factorization")
+    occ = {}
     for (k, st, c, f, by), ms in zip(enum, ms_of):
-        tasks = _covered(k, st, c, MT, split)
+        idx = occ.get((k, st, c), 0)
+        occ[(k, st, c)] = idx + 1
+        tasks = _covered(k, st, c, MT, split) if algo == "cholesky" else _covered_lu(k, st, c, MT, split, idx)
         tot = sum(_flops(t, b) for t in tasks)
         for t in tasks:
             times[t] = times.get(t, 0.0) + ms * _flops(t, b) / tot
     return times
 
 
+def cholesky_task_times(records, N, b, prec=0, split=True):
+    return task_times("cholesky", records, N, b, prec, split)
+
+
+def trace_lines(algo, records, N, b, prec=0, split=True):
+    """The reference tracer's `name, ms` lines for one factorization."""
+    times = task_times(algo, records, N, b, prec, split)
+    tasks = cholesky_tasks(N // b) if algo == "cholesky" else lu_tasks(N // b)
+    return [f"{name}, {times[task]:f}" for name, task in tasks]
+
+
 def cholesky_trace_lines(records, N, b, prec=0, split=True):
-    """The reference tracer's `name, ms` lines for one Cholesky factorization."""
-    times = cholesky_task_times(records, N, b, prec, split)
-    return [f"{name}, {times[task]:f}" for name, task in cholesky_tasks(N // b)]
+    return trace_lines("cholesky", records, N, b, prec, split)
 
 
 def main(argv=None):
-    """python -m palial_amd.trace -m N -b B: one profiled FP64 Cholesky of a
-    generated SPD matrix (S + N I) on the GPU, printed as the reference
-    tracer prints its run."""
+    """python -m palial_amd.trace [-a cholesky|lu] -m N -b B: one profiled FP64
+    factorization of a generated matrix (S + N I, or G + N I for LU) on the
+    GPU, printed as the reference tracer prints its run."""
     import argparse
-    from . import init, DeviceMatrix, GEN_SYM, profile, profile_records, factor_device
+    from . import init, DeviceMatrix, GEN_SYM, GEN_DENSE, profile, profile_records, factor_device
     ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("-a", choices=("cholesky", "lu"), default="cholesky")
     ap.add_argument("-m", type=int, default=8192, help="matrix size N")
     ap.add_argument("-b", type=int, default=512, help="tile size")
     ap.add_argument("--seed", type=int, default=0)
@@ -117,18 +178,18 @@ def main(argv=None):
         ap.error("the tile size must divide N")
     init(0)
     A = DeviceMatrix(args.m, args.b)
-    A.generate(GEN_SYM, seed=args.seed, shift=float(args.m))
+    A.generate(GEN_SYM if args.a == "cholesky" else GEN_DENSE, seed=args.seed, shift=float(args.m))
     profile(2)
-    info, _ = factor_device("cholesky", A)
+    info, _ = factor_device(args.a, A)
     rec = profile_records()
     profile(0)
     A.free()
     if info != 0:
         raise SystemExit(f"factorization failed (info {info})")
     try:
-        lines = cholesky_trace_lines(rec, args.m, args.b)
+        lines = trace_lines(args.a, rec, args.m, args.b)
     except ValueError:                                  # the schedule without the split update
-        lines = cholesky_trace_lines(rec, args.m, args.b, split=False)
+        lines = trace_lines(args.a, rec, args.m, args.b, split=False)
     print("\n".join(lines))
     return 0
 

@@ -29,8 +29,27 @@ def test_trace_of_a_profiled_solve(hip, orc, prec):
     assert abs(vals.sum() - (rec[:, 5].sum() - packs)) <= 1e-6 * len(vals) + 1e-9
 
 
+def test_lu_trace_of_a_profiled_solve(hip, orc):
+    N, b = 1024, 256
+    a = orc.gen("dense", N, b, seed=1, shift=N)
+    B = np.random.default_rng(0).standard_normal((N, 1))
+    hip.profile(2)
+    _, rep, rc = hip.solve(a, B, algo="lu", precision=2, b=b, factor_tile=b)
+    rec = hip.profile_records()
+    hip.profile(0)
+    assert rc == 0
+    lines = trace.trace_lines("lu", rec, N, b, 2)
+    assert [l.split(", ")[0] for l in lines] == [n for n, _ in trace.lu_tasks(N // b)]
+    vals = np.array([float(l.split(", ")[1]) for l in lines])
+    packs = rec[rec[:, 0] == cm.CLASSES["pack"], 5].sum()
+    assert (vals >= 0).all() and abs(vals.sum() - (rec[:, 5].sum() - packs)) <= 1e-6 * len(vals) + 1e-9
+
+
 def test_trace_cli_prints_reference_lines(capsys):
     assert trace.main(["-m", "1024", "-b", "256"]) == 0
     out = capsys.readouterr().out.strip().splitlines()
     assert [l.split(", ")[0] for l in out] == [n for n, _ in trace.cholesky_tasks(4)]
     assert all(float(l.split(", ")[1]) >= 0 for l in out)
+    assert trace.main(["-a", "lu", "-m", "1024", "-b", "256"]) == 0
+    out = capsys.readouterr().out.strip().splitlines()
+    assert [l.split(", ")[0] for l in out] == [n for n, _ in trace.lu_tasks(4)]

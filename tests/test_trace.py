@@ -62,3 +62,40 @@ def test_partial_records_are_rejected():
         trace.cholesky_task_times(rec[:-1], N, b, 0)
     with pytest.raises(ValueError):
         trace.cholesky_task_times(np.vstack([rec, rec[:1]]), N, b, 0)
+
+
+def test_lu_names_follow_the_reference_loops():
+    """src/lu.h:13-173 by hand for 3 x 3 tiles: getrf, the column panel's trsm
+    over m, then per n the row panel's trsm and the gemm column; m and n keep
+    their last loop values (the reference leaves them uninitialised at k = 0)."""
+    exp = ["getrf000", "trsm010", "trsm020", "trsm031", "gemm011", "gemm021", "trsm032", "gemm012", "gemm022",
+           "getrf133", "trsm123", "trsm132", "gemm122", "getrf233"]
+    assert [n for n, _ in trace.lu_tasks(3)] == exp
+
+
+@pytest.mark.parametrize("MT", [1, 2, 3, 5, 8])
+@pytest.mark.parametrize("split", [True, False])
+def test_lu_launches_cover_every_task_once(MT, split):
+    b = 256
+    tasks = [t for _, t in trace.lu_tasks(MT)]
+    assert len(set(tasks)) == len(tasks)
+    covered, occ = [], {}
+    for (k, st, c, f, by) in cm.launches("lu", MT * b, b, 0, split=split):
+        idx = occ.get((k, st, c), 0)
+        occ[(k, st, c)] = idx + 1
+        got = trace._covered_lu(k, st, c, MT, split, idx)
+        covered += got
+        if got:
+            assert abs(sum(trace._flops(t, b) for t in got) - f) <= 1e-9 * f
+    assert sorted(covered) == sorted(tasks)
+
+
+def test_lu_times_add_up():
+    N, b = 2048, 256
+    L = cm.launches("lu", N, b, 2)
+    ms = np.random.default_rng(1).uniform(0.01, 1.0, len(L))
+    rec = np.array([(c, b, 2, f, by, t) for (_, _, c, f, by), t in zip(L, ms)])
+    times = trace.task_times("lu", rec, N, b, 2)
+    packs = sum(t for (_, _, c, _, _), t in zip(L, ms) if c == cm.CLASSES["pack"])
+    assert abs(sum(times.values()) - (ms.sum() - packs)) < 1e-9
+    assert len(trace.trace_lines("lu", rec, N, b, 2)) == len(trace.lu_tasks(N // b))
