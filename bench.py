@@ -135,7 +135,7 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
 
 def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16),
               tiers=(0, 1, 2, 3, 4), solve_n=32768, algos=("cholesky", "lu", "qr"), solves=True):
-    """Survey configs 2 and 3 beside the headline (opt-in, --sweep):
+    """The size and condition-number sweep beside the headline (opt-in, --sweep):
     FP64 factorizations of Cholesky (S + N I), LU (G + N I) and QR (G) at
     b = 512 for each N (device time of one factorization after a warm-up),
     and Cholesky solves at N = solve_n for each kappa, nrhs and tier."""
@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--no-factor", action="store_true", help="skip the plain factorization runs")
     ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")
     ap.add_argument("--sweep", action="store_true",
-                    help="only the survey's size / condition-number sweep (one JSON line with key 'sweep')")
+                    help="only the size / condition-number sweep (one JSON line with key 'sweep')")
     ap.add_argument("--sweep-algos", default="cholesky,lu,qr", help="factorizations of the sweep")
     ap.add_argument("--sweep-ns", default="8192,16384,24576,32768", help="matrix sizes of the sweep")
     ap.add_argument("--sweep-solves", type=int, default=1, help="0: skip the sweep's Cholesky solves")
