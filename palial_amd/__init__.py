@@ -472,7 +472,10 @@ def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, m
 
     A: layout-L matrix (numpy flat array, torch tensor or DeviceMatrix), left
     unmodified.  B: N x nrhs column-major right-hand sides (numpy, copied;
-    or a device pointer with device=True).  Returns (X, report, rc)."""
+    or a device pointer with device=True).  precision: PREC_FP64, PREC_FP32,
+    PREC_BF16, PREC_FP8 (E4M3 panels with a power-of-two scale per tile) or
+    PREC_BF16X3 (split BF16); the last two need a factor tile that is a
+    multiple of 128 and smaller than N.  Returns (X, report, rc)."""
     init()
     if isinstance(A, DeviceMatrix):
         d, dev = A.desc, True
