@@ -1,5 +1,5 @@
-"""Per-task trace lines in the reference tracer's format for the Cholesky and
-LU factorizations: one `name, ms` line per tile task, in task-creation order,
+"""Per-task trace lines in the reference tracer's format for the Cholesky, LU
+and QR factorizations: one `name, ms` line per tile task, in task-creation order,
 ms with six decimals (palial_trace/log.h:43-50).
 
 Names are built as src/cholesky.h:14-173 builds them: the task kind followed
@@ -7,10 +7,12 @@ by k, m and n printed back to back, where m and n are whatever the loop
 variables hold when the task is created (so potrf of step k >= 1 carries the
 m = MT, n = MT - 1 that the previous step's loops left behind, and syrk
 carries the n of the previous column's gemm loop).  LU names follow
-src/lu.h:13-173 the same way; its k = 0 getrf and L-panel trsm print loop
-variables the reference never initialises (src/lu.h:13), printed here as 0.
+src/lu.h:13-173 and QR names src/qr.h:17-185 the same way; their k = 0 tasks
+print loop variables the reference never initialises (src/lu.h:13,
+src/qr.h:17), printed here as 0.  A QR update's three GEMM phases are split
+evenly over its n > k tiles.
 The Cholesky names are pinned to the reference's recorded trace; the
-reference holds no recorded LU trace.
+reference holds no recorded LU or QR trace.
 
 Times come from the library's per-launch records (profile(2) before a
 factorization or solve, then profile_records()).  A launch of the chain
@@ -90,6 +92,87 @@ def _covered_lu(k, stream, cls, MT, split, idx):
     return [("gemm", k, m, n) for n in range(k + 2, MT) for m in range(k + 2, MT) if (m, n) != (k + 2, k + 2)]
 
 
+def qr_tasks(MT):
+    """[(name, task)] in creation order (src/qr.h:17-185): ("geqrt", k),
+    ("ormqr", k, n), ("tsqrt", k, m), ("tsmqr", k, m, n)."""
+    out = []
+    k = m = n = 0
+    for k in range(MT):
+        out.append((f"geqrt{k}{m}{n}", ("geqrt", k)))
+        n = k + 1
+        while n < MT:
+            out.append((f"ormqr{k}{m}{n}", ("ormqr", k, n)))
+            n += 1
+        m = k + 1
+        while m < MT:
+            out.append((f"tsqrt{k}{m}{n}", ("tsqrt", k, m)))
+            n = k + 1
+            while n < MT:
+                out.append((f"tsmqr{k}{m}{n}", ("tsmqr", k, m, n)))
+                n += 1
+            m += 1
+    return out
+
+
+def qr_launches(N, b):
+    """The records of one QR factorization in issue order (dag_qr,
+    palial_amd/csrc/api.hip): (group, class, flops, bytes); group is a task
+    for geqrt and tsqrt, ("ormqr", k) or ("tsmqr", k, m) for the three GEMM
+    phases of an update (W = V^T C, W = T^T W, C -= V W over every n > k)."""
+    MT = N // b
+    b3, tb = float(b) ** 3, 8.0 * b * b
+    D, Q, U = cm.CLASSES["diag"], cm.CLASSES["tsqrt"], cm.CLASSES["qrupd"]
+    out = []
+    for k in range(MT):
+        out.append((("geqrt", k), D, 4 * b3 / 3, 3 * tb))
+        T = MT - k - 1
+        if T == 0:
+            continue
+        out += [(("ormqr", k), U, T * b3, 3 * T * tb)] * 3
+        for m in range(k + 1, MT):
+            out.append((("tsqrt", k, m), Q, 2 * b3, 5 * tb))
+            out += [(("tsmqr", k, m), U, 2 * T * b3, 4 * T * tb), (("tsmqr", k, m), U, T * b3, 4 * T * tb),
+                    (("tsmqr", k, m), U, 2 * T * b3, 3 * T * tb)]
+    return out
+
+
+def qr_task_times(records, N, b):
+    """{task: ms} for one QR factorization: geqrt and tsqrt records are tasks;
+    an update's three phase records are split evenly over its n > k tiles."""
+    MT = N // b
+    enum = qr_launches(N, b)
+    ms_of = _match(records, [(c, f, by) for _, c, f, by in enum], f"qr schedule of N={N}, b={b}")
+    times = {}
+    for (grp, _, _, _), ms in zip(enum, ms_of):
+        if grp[0] in ("geqrt", "tsqrt"):
+            tasks = [grp]
+        else:
+            tasks = [grp[:2] + (n,) if grp[0] == "ormqr" else grp + (n,) for n in range(grp[1] + 1, MT)]
+        for t in tasks:
+            times[t] = times.get(t, 0.0) + ms / len(tasks)
+    return times
+
+
+def _match(records, enum, what):
+    """Recorded ms per enumerated (class, flops, bytes), matched group by group in order."""
+    rec = np.asarray(records, dtype=np.float64).reshape(-1, 6)
+    key = lambda c, f, by: (int(c), int(round(f)), int(round(by)))
+    groups = {}
+    for i, (c, f, by) in enumerate(enum):
+        groups.setdefault(key(c, f, by), []).append(i)
+    seen = {g: 0 for g in groups}
+    ms_of = [None] * len(enum)
+    for c, _, _, f, by, ms in rec:
+        g = key(c, f, by)
+        if g not in groups or seen[g] >= len(groups[g]):
+            raise ValueError(f"record {g} has no launch left in the {what}")
+        ms_of[groups[g][seen[g]]] = ms
+        seen[g] += 1
+    if any(t is None for t in ms_of):
+        raise ValueError("records cover only part of the schedule: pass the records of one factorization")
+    return ms_of
+
+
 def _covered(k, stream, cls, MT, split):
     """Reference tasks of one enumerated launch of the Cholesky chain schedule."""
     if cls == cm.CLASSES["diag"]:
@@ -118,24 +201,12 @@ def _flops(task, b):
 def task_times(algo, records, N, b, prec=0, split=True):
     """{task: ms} from the per-launch records of one factorization (the
     records of exactly one factorization, in issue order)."""
+    if algo == "qr":
+        return qr_task_times(records, N, b)
     MT = N // b
     enum = cm.launches(algo, N, b, prec, split=split)
-    rec = np.asarray(records, dtype=np.float64).reshape(-1, 6)
-    key = lambda c, f, by: (int(c), int(round(f)), int(round(by)))
-    groups = {}
-    for i, (k, st, c, f, by) in enumerate(enum):
-        groups.setdefault(key(c, f, by), []).append(i)
+    ms_of = _match(records, [(c, f, by) for _, _, c, f, by in enum], f"{algo} schedule of N={N}, b={b}")
     times = {}
-    seen = {g: 0 for g in groups}
-    ms_of = [None] * len(enum)
-    for c, _, _, f, by, ms in rec:
-        g = key(c, f, by)
-        if g not in groups or seen[g] >= len(groups[g]):
-            raise ValueError(f"record {g} has no launch left in the {algo} schedule of N={N}, b={b}")
-        ms_of[groups[g][seen[g]]] = ms
-        seen[g] += 1
-    if any(t is None for t in ms_of):
-        raise ValueError("records cover only part of the schedule: pass the records of one factorization")
     occ = {}
     for (k, st, c, f, by), ms in zip(enum, ms_of):
         idx = occ.get((k, st, c), 0)
@@ -154,7 +225,7 @@ def cholesky_task_times(records, N, b, prec=0, split=True):
 def trace_lines(algo, records, N, b, prec=0, split=True):
     """The reference tracer's `name, ms` lines for one factorization."""
     times = task_times(algo, records, N, b, prec, split)
-    tasks = cholesky_tasks(N // b) if algo == "cholesky" else lu_tasks(N // b)
+    tasks = {"cholesky": cholesky_tasks, "lu": lu_tasks, "qr": qr_tasks}[algo](N // b)
     return [f"{name}, {times[task]:f}" for name, task in tasks]
 
 
@@ -163,13 +234,13 @@ def cholesky_trace_lines(records, N, b, prec=0, split=True):
 
 
 def main(argv=None):
-    """python -m palial_amd.trace [-a cholesky|lu] -m N -b B: one profiled FP64
-    factorization of a generated matrix (S + N I, or G + N I for LU) on the
-    GPU, printed as the reference tracer prints its run."""
+    """python -m palial_amd.trace [-a cholesky|lu|qr] -m N -b B: one profiled
+    FP64 factorization of a generated matrix (S + N I; G + N I for LU; G for
+    QR) on the GPU, printed as the reference tracer prints its run."""
     import argparse
     from . import init, DeviceMatrix, GEN_SYM, GEN_DENSE, profile, profile_records, factor_device
     ap = argparse.ArgumentParser(description=main.__doc__)
-    ap.add_argument("-a", choices=("cholesky", "lu"), default="cholesky")
+    ap.add_argument("-a", choices=("cholesky", "lu", "qr"), default="cholesky")
     ap.add_argument("-m", type=int, default=8192, help="matrix size N")
     ap.add_argument("-b", type=int, default=512, help="tile size")
     ap.add_argument("--seed", type=int, default=0)
@@ -178,12 +249,16 @@ def main(argv=None):
         ap.error("the tile size must divide N")
     init(0)
     A = DeviceMatrix(args.m, args.b)
-    A.generate(GEN_SYM if args.a == "cholesky" else GEN_DENSE, seed=args.seed, shift=float(args.m))
+    A.generate(GEN_SYM if args.a == "cholesky" else GEN_DENSE, seed=args.seed,
+               shift=0.0 if args.a == "qr" else float(args.m))
+    S = DeviceMatrix(args.m, args.b) if args.a == "qr" else None
     profile(2)
-    info, _ = factor_device(args.a, A)
+    info, _ = factor_device(args.a, A, S=S)
     rec = profile_records()
     profile(0)
     A.free()
+    if S is not None:
+        S.free()
     if info != 0:
         raise SystemExit(f"factorization failed (info {info})")
     try:

@@ -45,6 +45,25 @@ def test_lu_trace_of_a_profiled_solve(hip, orc):
     assert (vals >= 0).all() and abs(vals.sum() - (rec[:, 5].sum() - packs)) <= 1e-6 * len(vals) + 1e-9
 
 
+def test_qr_trace_of_a_profiled_factorization(hip):
+    N, b = 1024, 256
+    A, S = hip.DeviceMatrix(N, b), hip.DeviceMatrix(N, b)
+    try:
+        A.generate(hip.GEN_DENSE, seed=3)
+        hip.profile(2)
+        info, _ = hip.factor_device("qr", A, S=S)
+        rec = hip.profile_records()
+        hip.profile(0)
+    finally:
+        A.free()
+        S.free()
+    assert info == 0
+    lines = trace.trace_lines("qr", rec, N, b)
+    assert [l.split(", ")[0] for l in lines] == [n for n, _ in trace.qr_tasks(N // b)]
+    vals = np.array([float(l.split(", ")[1]) for l in lines])
+    assert (vals >= 0).all() and abs(vals.sum() - rec[:, 5].sum()) <= 1e-6 * len(vals) + 1e-9
+
+
 def test_trace_cli_prints_reference_lines(capsys):
     assert trace.main(["-m", "1024", "-b", "256"]) == 0
     out = capsys.readouterr().out.strip().splitlines()
@@ -53,3 +72,6 @@ def test_trace_cli_prints_reference_lines(capsys):
     assert trace.main(["-a", "lu", "-m", "1024", "-b", "256"]) == 0
     out = capsys.readouterr().out.strip().splitlines()
     assert [l.split(", ")[0] for l in out] == [n for n, _ in trace.lu_tasks(4)]
+    assert trace.main(["-a", "qr", "-m", "1024", "-b", "256"]) == 0
+    out = capsys.readouterr().out.strip().splitlines()
+    assert [l.split(", ")[0] for l in out] == [n for n, _ in trace.qr_tasks(4)]

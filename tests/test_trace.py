@@ -99,3 +99,24 @@ def test_lu_times_add_up():
     packs = sum(t for (_, _, c, _, _), t in zip(L, ms) if c == cm.CLASSES["pack"])
     assert abs(sum(times.values()) - (ms.sum() - packs)) < 1e-9
     assert len(trace.trace_lines("lu", rec, N, b, 2)) == len(trace.lu_tasks(N // b))
+
+
+def test_qr_names_follow_the_reference_loops():
+    """src/qr.h:17-185 by hand for 3 x 3 tiles: geqrt, ormqr over n, then per
+    m tsqrt and the tsmqr row; m and n keep their last loop values."""
+    exp = ["geqrt000", "ormqr001", "ormqr002", "tsqrt013", "tsmqr011", "tsmqr012", "tsqrt023", "tsmqr021",
+           "tsmqr022", "geqrt133", "ormqr132", "tsqrt123", "tsmqr122", "geqrt233"]
+    assert [n for n, _ in trace.qr_tasks(3)] == exp
+
+
+@pytest.mark.parametrize("MT", [1, 2, 5])
+def test_qr_times_cover_every_task(MT):
+    N, b = MT * 256, 256
+    L = trace.qr_launches(N, b)
+    ms = np.random.default_rng(2).uniform(0.01, 1.0, len(L))
+    rec = np.array([(c, b, 0, f, by, t) for (_, c, f, by), t in zip(L, ms)])
+    times = trace.task_times("qr", rec, N, b)
+    assert set(times) == {t for _, t in trace.qr_tasks(MT)}
+    assert abs(sum(times.values()) - ms.sum()) < 1e-9
+    with pytest.raises(ValueError):
+        trace.task_times("qr", rec[:-1], N, b)
