@@ -275,6 +275,23 @@ int palial_hip_dtsmqr(int side, int trans, int M, int N, int K, int ib, double *
                       int lda1, double *A2, int lda2, const double *V, int ldv,
                       const double *T, int ldt, double *work, int ldw,
                       const palial_opts *opts);                                  /* PALIAL_dtsmqr, src/srcqr/dtsmqr.h:11 */
+/* No counterpart in the reference: palial_hip_dtsmqr for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, semantics and results
+ * (bitwise) as palial_hip_dtsmqr, but it allocates nothing: its two b x b
+ * workspace tiles live in a static device array of the library (4 MiB per
+ * device, held until the process ends), so M = N = K = ib <= 512 (larger
+ * tiles return PALIAL_ERR_ARG), and A1, A2, V, T must be device pointers
+ * (opts->device_matrix = 1; host tiles would need staging and return
+ * PALIAL_ERR_ARG).  With opts->stream set the launches are enqueued on that
+ * stream and the call returns without waiting; with opts->stream NULL it
+ * runs on the library stream and waits.  Every call on one device shares
+ * the same workspace: calls that overlap on one device (two streams, or two
+ * threads with streams of their own) race. */
+int palial_hip_dtsmqr_noalloc(int side, int trans, int M, int N, int K, int ib, double *A1,
+                              int lda1, double *A2, int lda2, const double *V, int ldv,
+                              const double *T, int ldt, double *work, int ldw,
+                              const palial_opts *opts);
 
 /* The BLAS tile tasks of the Cholesky and LU DAGs, as the reference calls them:
  *   PALIAL_TASK_CHOL_TRSM  C <- A^-T C, A upper          src/cholesky.h:73-84

@@ -93,6 +93,7 @@ def lib():
                                         C.POINTER(Opts)]
         L.palial_hip_dtsmqr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int,
                                         dp, C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int, C.POINTER(Opts)]
+        L.palial_hip_dtsmqr_noalloc.argtypes = L.palial_hip_dtsmqr.argtypes
         L.palial_hip_tile_task.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(Opts)]
         L.palial_hip_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.palial_hip_free.argtypes = [dp]

@@ -31,7 +31,20 @@
 
 using namespace palial;
 
+// Workspace of palial_hip_dtsmqr_noalloc: the two b x b tiles W = A1 + V^T A2
+// and T^T W of one TSMQR, b <= NOALLOC_MAXB, held by the code object itself
+// (4 MiB per device, for the life of the process) so that the entry point
+// allocates nothing.  Host code never passes `scratch` itself to a kernel:
+// palial_hip_init resolves its device address per device (g_scratch below).
+constexpr int NOALLOC_MAXB = 512;
+static __device__ __align__(256) double scratch[2 * (size_t)NOALLOC_MAXB * NOALLOC_MAXB];
+
 namespace {
+
+// device address of `scratch` on each device (hipGetSymbolAddress with that
+// device current, at palial_hip_init); kept until the process ends
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};
 
 struct Graph {
     hipGraph_t graph = nullptr;
@@ -1609,6 +1622,13 @@ int palial_hip_init(int device)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(PALIAL_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+    if (device >= MAX_DEVICES) return fail(PALIAL_ERR_NODEV, "device %d above the supported %d", device, MAX_DEVICES);
+    if (!g_scratch[device]) {
+        void *sp = nullptr;
+        HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        if (!sp) return fail(PALIAL_ERR_HIP, "no device address for the no-allocation scratch");
+        g_scratch[device] = (double *)sp;
+    }
     // PALIAL_RESERVE_CUS=n (default 8): the bulk stream of the chain schedule
     // runs on all CUs but the first n, which stay free for the diagonal chain.
     // PALIAL_DAG=1 selects the chain schedule with one update launch per
@@ -2304,6 +2324,43 @@ int palial_hip_dtsmqr(int side, int trans, int M, int N, int K, int ib, double *
     HIPCHK(launch_gemm(single(dT, w1, w2, nullptr, d1, n, 1.0, 0.0, MK_UPPER), true, false, g.s_main));
     HIPCHK(launch_gemm(single(dV, w2, d2, d2, nullptr, n, -1.0, 1.0, MK_NONE), false, false, g.s_main));
     return st.finish();
+}
+
+// palial_hip_dtsmqr with its two workspace tiles in the static `scratch`
+// array instead of two hipMalloc'ed tiles: the same three launches of the same
+// kernels, so the results are bitwise those of palial_hip_dtsmqr.  Both tiles
+// are written in full by the first two launches before they are read, so
+// nothing in the scratch has to start at zero.
+int palial_hip_dtsmqr_noalloc(int side, int trans, int M, int N, int K, int ib, double *A1, int lda1, double *A2,
+                              int lda2, const double *V, int ldv, const double *T, int ldt, double *work, int ldw,
+                              const palial_opts *opts)
+{
+    (void)work; (void)ldw;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if (M == 0 || N == 0 || K == 0 || ib == 0) return 0;
+    if (!(side == PALIAL_LEFT && trans != PALIAL_NOTRANS)) return fail(PALIAL_ERR_ARG, "only side=Left, trans=Trans is supported");
+    if (!(M == N && N == K && K == ib)) return fail(PALIAL_ERR_ARG, "only M = N = K = ib (every reference call) is supported");
+    const int n = M;
+    if ((rc = check_tile(n, lda1, A1)) || (rc = check_tile(n, lda2, A2)) || (rc = check_tile(n, ldv, V)) ||
+        (rc = check_tile(n, ldt, T)))
+        return rc;
+    if (!opts || !opts->device_matrix)
+        return fail(PALIAL_ERR_ARG, "palial_hip_dtsmqr_noalloc needs device tiles (opts->device_matrix = 1): "
+                    "staging host tiles would allocate");
+    if (n > NOALLOC_MAXB)
+        return fail(PALIAL_ERR_ARG, "tile size %d above %d, the most the static workspace of "
+                    "palial_hip_dtsmqr_noalloc holds", n, NOALLOC_MAXB);
+    double *const sp = g_scratch[g.device];
+    if (!sp) return fail(PALIAL_ERR_STATE, "no-allocation workspace not set up on device %d", g.device);
+    double *w1 = sp, *w2 = sp + (size_t)n * n;
+    const hipStream_t s = opts->stream ? (hipStream_t)opts->stream : g.s_main;
+    HIPCHK(launch_gemm(single(V, A2, w1, A1, nullptr, n, 1.0, 1.0, MK_NONE), true, false, s));
+    HIPCHK(launch_gemm(single(T, w1, w2, nullptr, A1, n, 1.0, 0.0, MK_UPPER), true, false, s));
+    HIPCHK(launch_gemm(single(V, w2, A2, A2, nullptr, n, -1.0, 1.0, MK_NONE), false, false, s));
+    if (!opts->stream) HIPCHK(hipStreamSynchronize(s));
+    return 0;
 }
 
 int palial_hip_tile_task(int task, int n, const double *A, const double *B, double *C, const palial_opts *opts)
