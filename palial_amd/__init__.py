@@ -15,6 +15,7 @@ dtsmqr and the BLAS tile tasks) take column-major n x n numpy tiles.
 There is no CPU fallback: every call goes to the HIP library, and importing
 the package fails loudly when ``libpalial_hip.so`` has not been built.
 """
+import atexit
 import ctypes as C
 import os
 
@@ -131,6 +132,9 @@ def init(device=0):
     if not _inited:
         _check(lib().palial_hip_init(device))
         _inited = True
+        # release graphs, streams and device buffers while the HIP runtime is
+        # still whole (before the shared libraries' own teardown at exit)
+        atexit.register(finalize)
 
 
 def finalize():
