@@ -158,11 +158,16 @@ int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double
  *   kind 2: reference populate.h generator (include/populate.h:35-66) made
  *           reproducible: diagonal 173*(1 + U01), everything else 0.5
  *   kind 3: identity (include/populate.h:48,58)
+ *   kind 4, 5: see PALIAL_GEN_GAUSS, PALIAL_GEN_GEOM_DIAG
  * U(seed,i,j) = (splitmix64(splitmix64(seed) + (i<<32 | j)) >> 11) * 2^-53 - 0.5 */
 #define PALIAL_GEN_SYM 0
 #define PALIAL_GEN_DENSE 1
 #define PALIAL_GEN_PALIAL 2
 #define PALIAL_GEN_IDENTITY 3
+#define PALIAL_GEN_GAUSS 4      /* G_ij ~ N(0,1): h = splitmix64(splitmix64(seed) + (i<<32 | j)),
+                                   u1 = ((h >> 11) + 1) 2^-53, u2 = (splitmix64(h) >> 11) 2^-53,
+                                   G_ij = sqrt(-2 ln u1) cos(2 pi u2) (shift unused) */
+#define PALIAL_GEN_GEOM_DIAG 5  /* diag(s), s_i = kappa^(-i/(N-1)) with kappa = shift >= 1 */
 int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift,
                         const palial_opts *opts);
 
@@ -189,6 +194,33 @@ int palial_hip_lu(MATRIX_desc A, int *ipiv, const palial_opts *opts, palial_repo
  * Flat-tree TS Householder QR: R and V in A, T factors in the upper triangles
  * of S(k,k) and S(m,k); all other S entries are not written. */
 int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_report *rep);
+
+/* Q of a finished palial_hip_qr (V in A, T factors in S) applied from the
+ * left to C, an N x ncols matrix in layout L with the same tile size
+ * (N/b row tiles, ncols/b column tiles; tile (m,n) at b*b*(m + (N/b)*n)):
+ * trans PALIAL_TRANS gives C <- Q^T C (the transformation qr() applies to
+ * its own trailing columns, PALIAL_dormqr / PALIAL_dtsmqr at src/qr.h:43-182),
+ * PALIAL_NOTRANS gives C <- Q C, which the reference's tile kernels do not
+ * compute correctly (src/srcqr/dormqr.h:19-29, src/srcqr/dtsmqr.h:19-29).
+ * side must be PALIAL_LEFT.  Q itself is formed by applying PALIAL_NOTRANS to
+ * the identity (palial_hip_generate kind PALIAL_GEN_IDENTITY).
+ * rep->solve_ms is the device time, rep->gflop 2 N^2 ncols. */
+int palial_hip_qr_apply(int side, int trans, MATRIX_desc A, MATRIX_desc S, double *C, int ncols,
+                        const palial_opts *opts, palial_report *rep);
+
+/* Least-squares solution of A X = B from a finished palial_hip_qr (no
+ * reference counterpart): X = R^-1 (Q^T B), which for the square nonsingular
+ * A the reference factors is the solution of A X = B.  B (N x nrhs,
+ * column-major, leading dimension ldb >= N, nrhs <= 16) holds the right-hand
+ * sides on entry and X on return.  Requires b % 64 == 0 and b <= 512.  A and S
+ * are not modified. */
+int palial_hip_qr_solve(MATRIX_desc A, MATRIX_desc S, double *B, int nrhs, int ldb, const palial_opts *opts,
+                        palial_report *rep);
+
+/* In-place A <- A^T (average = 0) or A <- (A + A^T)/2 (average = 1) of a
+ * layout-L matrix, b % 32 == 0 (no reference counterpart; builds the
+ * symmetric test matrices Q diag(s) Q^T). */
+int palial_hip_transpose(MATRIX_desc A, int average, const palial_opts *opts);
 
 /* ---------------------------------------------------------------- solve */
 /* No counterpart in the reference (which only factors): solve A X = B for

@@ -45,6 +45,10 @@ def lib():
             getattr(L, f).argtypes = [Desc, C.c_uint64, C.c_double]
         L.orc_gen_palial.argtypes = [Desc, C.c_uint64]
         L.orc_gen_identity.argtypes = [Desc]
+        L.orc_gen_gauss.argtypes = [Desc, C.c_uint64]
+        L.orc_gen_geom_diag.argtypes = [Desc, C.c_double]
+        L.orc_gauss.restype = C.c_double
+        L.orc_gauss.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_gen_special4x4.argtypes = [Desc]
         L.orc_use_openblas.argtypes = [C.c_char_p]
         L.orc_wtime.restype = C.c_double
@@ -138,6 +142,10 @@ def gen(kind, N, b, seed=0, shift=None):
         L.orc_gen_palial(d, seed)
     elif kind == "identity":
         L.orc_gen_identity(d)
+    elif kind == "gauss":
+        L.orc_gen_gauss(d, seed)
+    elif kind == "geomdiag":
+        L.orc_gen_geom_diag(d, float(shift))
     elif kind == "special4x4":
         assert N == 4 and b == 2
         L.orc_gen_special4x4(d)
@@ -306,14 +314,15 @@ def resid_lu(A, F):
     return np.linalg.norm(A - L @ U) / (n * EPS * np.linalg.norm(A))
 
 
-def resid_qr(A, F, S, N, b):
-    """Scaled residuals (||A - QR||_F / (n eps ||A||_F), ||I - Q^T Q||_F / (n eps))
-    of a flat-tree tiled QR (src/qr.h:18-190) from its output in layout L:
-    F holds R (upper part), the GEQRT reflectors V (unit lower, tile (k,k))
-    and the TS reflectors (tile (m,k), m > k); S the T factors.  The
-    factorization applied C <- (I - Y T Y^T)^T C per step (dlarfb 'L','T';
-    src/srcqr/dtsmqr.h), so Q = H_1 H_2 ... with H = I - Y T Y^T, formed here
-    by applying the H in reverse order to the identity."""
+def gauss(seed, i, j):
+    return lib().orc_gauss(seed, i, j)
+
+
+def qr_blocks(F, S, N, b):
+    """The block reflectors of a flat-tree tiled QR (src/qr.h:18-190) in the
+    order the factorization applied them: (rows, Y, T) with the transformation
+    C[rows] <- (I - Y T Y^T)^T C[rows]; Y = unit lower V of tile (k,k) for the
+    GEQRT block of step k, Y = [I; A(m,k)] for the TS block (k, m)."""
     MT = N // b
 
     def tile(M, m, n):
@@ -325,11 +334,34 @@ def resid_qr(A, F, S, N, b):
         ops.append(([k], np.tril(tile(F, k, k), -1) + np.eye(b), np.triu(tile(S, k, k))))
         for m in range(k + 1, MT):
             ops.append(([k, m], np.vstack([np.eye(b), tile(F, m, k)]), np.triu(tile(S, m, k))))
-    Q = np.eye(N)
-    for rows, Y, T in reversed(ops):
+    return ops
+
+
+def apply_q(F, S, N, b, Cd, trans):
+    """Q^T C (trans) or Q C of a dense N x n matrix Cd, Q = H_1 H_2 ... with
+    H = I - Y T Y^T the blocks of qr_blocks."""
+    X = np.array(Cd, dtype=np.float64, copy=True)
+    ops = qr_blocks(F, S, N, b)
+    for rows, Y, T in (ops if trans else reversed(ops)):
         idx = np.concatenate([np.arange(r * b, (r + 1) * b) for r in rows])
-        X = Q[idx, :]
-        Q[idx, :] = X - Y @ (T @ (Y.T @ X))
+        Z = X[idx, :]
+        X[idx, :] = Z - Y @ ((T.T if trans else T) @ (Y.T @ Z))
+    return X
+
+
+def form_q(F, S, N, b):
+    return apply_q(F, S, N, b, np.eye(N), trans=False)
+
+
+def resid_qr(A, F, S, N, b):
+    """Scaled residuals (||A - QR||_F / (n eps ||A||_F), ||I - Q^T Q||_F / (n eps))
+    of a flat-tree tiled QR (src/qr.h:18-190) from its output in layout L:
+    F holds R (upper part), the GEQRT reflectors V (unit lower, tile (k,k))
+    and the TS reflectors (tile (m,k), m > k); S the T factors.  The
+    factorization applied C <- (I - Y T Y^T)^T C per step (dlarfb 'L','T';
+    src/srcqr/dtsmqr.h), so Q = H_1 H_2 ... with H = I - Y T Y^T, formed here
+    by applying the H in reverse order to the identity."""
+    Q = form_q(F, S, N, b)
     R = np.triu(to_dense(F, N, b))
     r1 = np.linalg.norm(A - Q @ R) / (N * EPS * np.linalg.norm(A))
     r2 = np.linalg.norm(np.eye(N) - Q.T @ Q) / (N * EPS)

@@ -557,6 +557,33 @@ void orc_gen_dense_shift(orc_desc A, uint64_t seed, double shift)
             a[orc_elem_offset(N, b, i, j)] = v;
         }
 }
+/* Standard normal entries by Box-Muller from one counter per entry (the
+ * build's PALIAL_GEN_GAUSS): h = sm64(sm64(seed) + (i<<32 | j)),
+ * u1 = ((h >> 11) + 1) 2^-53 in (0,1], u2 = (sm64(h) >> 11) 2^-53,
+ * z = sqrt(-2 ln u1) cos(2 pi u2). */
+double orc_gauss(uint64_t seed, uint64_t i, uint64_t j)
+{
+    uint64_t h = sm64(sm64(seed) + ((i << 32) | (j & 0xffffffffull)));
+    double u1 = (double)((h >> 11) + 1) * 0x1.0p-53;
+    double u2 = (double)(sm64(h) >> 11) * 0x1.0p-53;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+}
+void orc_gen_gauss(orc_desc A, uint64_t seed)
+{
+    int N = A.matrix_size, b = A.tile_size;
+    double *a = A.matrix;
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) a[orc_elem_offset(N, b, i, j)] = orc_gauss(seed, i, j);
+}
+/* diag(s), s_i = kappa^(-i/(N-1)): a geometric spectrum from 1 to 1/kappa */
+void orc_gen_geom_diag(orc_desc A, double kappa)
+{
+    int N = A.matrix_size, b = A.tile_size;
+    double *a = A.matrix;
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++)
+            a[orc_elem_offset(N, b, i, j)] = (i != j) ? 0.0 : (N > 1 ? pow(kappa, -(double)i / (double)(N - 1)) : 1.0);
+}
 void orc_gen_palial(orc_desc A, uint64_t seed)
 {
     /* include/populate.h:51-66: diagonal 173*(1+r), r in [0,1]; everything else 0.5 */

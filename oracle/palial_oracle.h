@@ -49,6 +49,9 @@ double orc_uniform(uint64_t seed, uint64_t i, uint64_t j);  /* U[-0.5,0.5) */
 void orc_gen_sym_shift(orc_desc A, uint64_t seed, double shift);   /* S + shift*I */
 void orc_gen_dense_shift(orc_desc A, uint64_t seed, double shift); /* G + shift*I */
 void orc_gen_palial(orc_desc A, uint64_t seed);  /* D + 0.5*11^T, include/populate.h:35-66 */
+double orc_gauss(uint64_t seed, uint64_t i, uint64_t j);   /* Box-Muller N(0,1) entry */
+void orc_gen_gauss(orc_desc A, uint64_t seed);                /* G_ij ~ N(0,1) */
+void orc_gen_geom_diag(orc_desc A, double kappa);             /* diag(kappa^(-i/(N-1))) */
 void orc_gen_identity(orc_desc A);               /* include/populate.h:48,58 (IDENTITY) */
 void orc_gen_special4x4(orc_desc A);             /* include/populate.h:13-33 (N=4, b=2) */
 

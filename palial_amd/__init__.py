@@ -30,7 +30,7 @@ LEFT, RIGHT = 141, 142
 NOTRANS, TRANS = 111, 112
 TASK_CHOL_TRSM, TASK_CHOL_SYRK, TASK_CHOL_GEMM = 1, 2, 3
 TASK_LU_TRSM_R, TASK_LU_TRSM_L, TASK_LU_GEMM = 4, 5, 6
-GEN_SYM, GEN_DENSE, GEN_PALIAL, GEN_IDENTITY = 0, 1, 2, 3
+GEN_SYM, GEN_DENSE, GEN_PALIAL, GEN_IDENTITY, GEN_GAUSS, GEN_GEOM_DIAG = 0, 1, 2, 3, 4, 5
 
 
 class MATRIX_desc(C.Structure):
@@ -95,6 +95,11 @@ def lib():
         L.palial_hip_dtsmqr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int,
                                         dp, C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int, C.POINTER(Opts)]
         L.palial_hip_dtsmqr_noalloc.argtypes = L.palial_hip_dtsmqr.argtypes
+        L.palial_hip_qr_apply.argtypes = [C.c_int, C.c_int, MATRIX_desc, MATRIX_desc, dp, C.c_int, C.POINTER(Opts),
+                                          C.POINTER(Report)]
+        L.palial_hip_qr_solve.argtypes = [MATRIX_desc, MATRIX_desc, dp, C.c_int, C.c_int, C.POINTER(Opts),
+                                          C.POINTER(Report)]
+        L.palial_hip_transpose.argtypes = [MATRIX_desc, C.c_int, C.POINTER(Opts)]
         L.palial_hip_tile_task.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(Opts)]
         L.palial_hip_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.palial_hip_free.argtypes = [dp]
@@ -234,6 +239,48 @@ def qr(A, S, b=None, N=None, **kw):
     rep = Report()
     rc = _check(lib().palial_hip_qr(dA, dS, C.byref(o), C.byref(rep)))
     return rc, rep
+
+
+def qr_apply(A, S, Cm, trans=TRANS, b=None, N=None, ncols=None, side=LEFT):
+    """C <- Q^T C (trans=TRANS) or Q C (NOTRANS) with the Q of a finished qr(A, S).
+    C: N x ncols in layout L (flat numpy array; ncols defaults to N), or a
+    DeviceMatrix together with device A and S.  Returns the report."""
+    init()
+    if isinstance(A, DeviceMatrix):
+        dA, dS, cp, dev, nc = A.desc, S.desc, Cm.ptr, True, Cm.N
+    else:
+        dA, dev, _ = _desc_of(A, N, b)
+        dS, _, _ = _desc_of(S, N, b)
+        nc = ncols or (Cm.size // dA.matrix_size)
+        cp = Cm.ctypes.data
+    o = _opts(device=bool(dev))
+    rep = Report()
+    _check(lib().palial_hip_qr_apply(side, trans, dA, dS, cp, nc, C.byref(o), C.byref(rep)))
+    return rep
+
+
+def qr_solve(A, S, B, b=None, N=None):
+    """X = R^-1 Q^T B from a finished qr(A, S) (host layout-L arrays); B is
+    N x nrhs (nrhs <= 16).  Returns (X, report)."""
+    init()
+    dA, dev, _ = _desc_of(A, N, b)
+    dS, _, _ = _desc_of(S, N, b)
+    X = np.asfortranarray(np.array(B, dtype=np.float64, copy=True).reshape(dA.matrix_size, -1))
+    o = _opts(device=bool(dev))
+    rep = Report()
+    _check(lib().palial_hip_qr_solve(dA, dS, X.ctypes.data, X.shape[1], X.shape[0], C.byref(o), C.byref(rep)))
+    return X, rep
+
+
+def transpose(A, average=False, b=None, N=None):
+    """In-place A <- A^T, or A <- (A + A^T)/2 with average=True (layout L)."""
+    init()
+    if isinstance(A, DeviceMatrix):
+        d, dev = A.desc, True
+    else:
+        d, dev, _ = _desc_of(A, N, b)
+    _check(lib().palial_hip_transpose(d, int(bool(average)), C.byref(_opts(device=bool(dev)))))
+    return A
 
 
 # ------------------------------------------------------------------ tile kernels
@@ -502,6 +549,35 @@ def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, m
     if check and rc < 0:
         _check(rc)
     return X, rep, rc
+
+
+class CondGenerator:
+    """Device matrices A = Q diag(s) Q^T with Q the orthogonal factor of the
+    library's own tiled Householder QR of a seeded N x N standard normal
+    matrix (GEN_GAUSS) and s_i = kappa^(-i/(N-1)) (GEN_GEOM_DIAG), so that the
+    2-norm condition number is kappa.  The QR is computed once per generator
+    and reused for every kappa:
+        A <- diag(s);  A <- Q A;  A <- A^T;  A <- Q A;  A <- (A + A^T)/2."""
+
+    def __init__(self, N, b, seed=0):
+        self.N, self.b = N, b
+        self.G = DeviceMatrix(N, b)
+        self.S = DeviceMatrix(N, b)
+        self.G.generate(GEN_GAUSS, seed=seed)
+        factor_device("qr", self.G, self.S)
+
+    def fill(self, A, kappa):
+        """Write Q diag(s) Q^T for this kappa into DeviceMatrix A (same N, b)."""
+        A.generate(GEN_GEOM_DIAG, shift=float(kappa))
+        qr_apply(self.G, self.S, A, trans=NOTRANS)
+        transpose(A)
+        qr_apply(self.G, self.S, A, trans=NOTRANS)
+        transpose(A, average=True)
+        return A
+
+    def free(self):
+        self.G.free()
+        self.S.free()
 
 
 def kron_spd_factors(N, kappa, seed=0, n1=None):

@@ -109,6 +109,10 @@ struct Ctx {
     size_t F32_cap = 0;             // (in doubles: the buffer is reused as floats)
     double *d_vec = nullptr;
     size_t vec_cap = 0;
+    double *d_aw = nullptr;         // Q application: W and op(T) W tiles, one pair per column tile of C
+    size_t aw_cap = 0;
+    double *d_qs = nullptr;         // QR solve: right-hand-side tiles, vectors, R tile inverses
+    size_t qs_cap = 0;
     std::map<std::tuple<int, void *, void *, int, int, int>, Graph> graphs;
 };
 
@@ -1007,6 +1011,65 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1
     }
 }
 
+// Q of a tiled QR (V in dA, T factors in dS) applied from the left to the N x
+// NC*b layout-L matrix dC: C <- Q^T C (trans) applies the reflector blocks in
+// the order the factorization applied them to its own trailing columns
+// (src/qr.h:43-182: ORMQR(k), then TSMQR(k, m) for m = k+1..), C <- Q C the
+// same blocks untransposed in reverse order.  A TS block (k, m) of rows k and
+// m is I - Y T Y^T with Y = [I; V], V = A(m,k), T = S(m,k), so
+//   W = C(k,:) + V^T C(m,:);  W2 = op(T) W;  C(k,:) -= W2;  C(m,:) -= V W2,
+// and the diagonal block k is I - V T V^T with V the unit lower part of A(k,k).
+// (The reference's own NoTrans paths start at the wrong index,
+// src/srcqr/dtsmqr.h:19-29, src/srcqr/dormqr.h:19-29; this follows the
+// algebra.)  The column tiles of C are independent, so one launch covers all
+// of them; w1 / w2 hold NC tiles each.
+void qr_apply_dev(bool trans, double *dA, const double *dS, double *dC, int N, int b, int NC, double *w1, double *w2,
+                  hipStream_t s, Launcher &L)
+{
+    const int MT = N / b;
+    L.b = b;
+    L.prec = 0;
+    // few column tiles: 32x32 output blocks give each launch enough workgroups
+    const bool fine = NC <= 2 && b % 32 == 0;
+    auto gemm = [&](const GemmArgs &a, bool ta, double flops) {
+        L.run(KC_QRUPD, flops, 3.0 * NC * tile_bytes(b), s, [&] {
+            return fine ? launch_gemm_fine(a, ta, false, s) : launch_gemm(a, ta, false, s);
+        });
+    };
+    const double bc = b3(b) * NC;
+    auto orm = [&](int k) {
+        GemmArgs a = gargs(GM_OA_W, dA, N, b, k, NC);
+        a.smat = dS; a.cmat = dC; a.w1 = w1; a.w2 = w2;
+        a.alpha = 1.0; a.beta = 0.0; a.maskA = MK_UNIT_LOWER;
+        gemm(a, true, bc);
+        a.mode = GM_OA_TW; a.maskA = MK_UPPER;
+        gemm(a, trans, bc);
+        a.mode = GM_OA_VW; a.maskA = MK_UNIT_LOWER; a.alpha = -1.0; a.beta = 1.0;
+        gemm(a, false, bc);
+    };
+    auto tsm = [&](int k, int m) {
+        GemmArgs a = gargs(GM_QA_W, dA, N, b, k, NC);
+        a.m = m; a.smat = dS; a.cmat = dC; a.w1 = w1; a.w2 = w2;
+        a.alpha = 1.0; a.beta = 1.0; a.maskA = MK_NONE;
+        gemm(a, true, 2 * bc);
+        a.mode = GM_QA_TW; a.maskA = MK_UPPER; a.beta = 0.0;
+        gemm(a, trans, bc);
+        a.mode = GM_QA_VW; a.maskA = MK_NONE; a.alpha = -1.0; a.beta = 1.0;
+        gemm(a, false, 2 * bc);
+    };
+    if (trans) {
+        for (int k = 0; k < MT; k++) {
+            orm(k);
+            for (int m = k + 1; m < MT; m++) tsm(k, m);
+        }
+    } else {
+        for (int k = MT - 1; k >= 0; k--) {
+            for (int m = MT - 1; m > k; m--) tsm(k, m);
+            orm(k);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ solve
 int grow(double **p, size_t *cap, size_t need)
 {
@@ -1704,6 +1767,8 @@ int palial_hip_finalize(void)
     if (g.d_bidx) hipFree(g.d_bidx);
     if (g.d_bwork) hipFree(g.d_bwork);
     if (g.d_vec) hipFree(g.d_vec);
+    if (g.d_aw) hipFree(g.d_aw);
+    if (g.d_qs) hipFree(g.d_qs);
     if (g.d_info) hipFree(g.d_info);
     if (g.s_bulk && g.s_bulk != g.s_main) hipStreamDestroy(g.s_bulk);
     if (g.s_aux) hipStreamDestroy(g.s_aux);
@@ -2117,7 +2182,8 @@ int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift, co
     int rc;
     if ((rc = ensure_init())) return rc;
     if ((rc = check_desc(A))) return rc;
-    if (kind < 0 || kind > 3) return fail(PALIAL_ERR_ARG, "unknown generator kind %d", kind);
+    if (kind < 0 || kind > 5) return fail(PALIAL_ERR_ARG, "unknown generator kind %d", kind);
+    if (kind == 5 && !(shift >= 1.0)) return fail(PALIAL_ERR_ARG, "geometric spectrum needs kappa >= 1 (got %g)", shift);
     palial_opts o = o_in ? *o_in : default_opts();
     const int N = A.matrix_size, b = A.tile_size;
     const size_t bytes = (size_t)N * N * 8;
@@ -2197,6 +2263,196 @@ int palial_hip_qr(MATRIX_desc A, MATRIX_desc S, const palial_opts *opts, palial_
 {
     std::lock_guard<std::mutex> lk(g_mu);
     return run_algo(ALG_QR, A, &S, opts, rep);
+}
+
+// Host tiles of the Q-application entry points are staged through device
+// copies; device pointers are used in place.
+static int qr_operands(const MATRIX_desc &A, const MATRIX_desc &S, bool dev, std::vector<double *> &owned, double *&dA,
+                double *&dS, hipStream_t s)
+{
+    const size_t bytes = (size_t)A.matrix_size * A.matrix_size * 8;
+    dA = (double *)A.matrix;
+    dS = (double *)S.matrix;
+    if (dev) return 0;
+    dA = dS = nullptr;
+    if (hipMalloc(&dA, bytes) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "device copy of A");
+    owned.push_back(dA);
+    if (hipMalloc(&dS, bytes) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "device copy of S");
+    owned.push_back(dS);
+    HIPCHK(hipMemcpyAsync(dA, A.matrix, bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dS, S.matrix, bytes, hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+static int check_qr_pair(const MATRIX_desc &A, const MATRIX_desc &S)
+{
+    int rc;
+    if ((rc = check_desc(A)) || (rc = check_desc(S))) return rc;
+    if (S.matrix_size != A.matrix_size || S.tile_size != A.tile_size)
+        return fail(PALIAL_ERR_ARG, "S must have the same N and b as A");
+    return 0;
+}
+
+int palial_hip_qr_apply(int side, int trans, MATRIX_desc A, MATRIX_desc S, double *C, int ncols,
+                        const palial_opts *o_in, palial_report *rep)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_qr_pair(A, S))) return rc;
+    if (side != PALIAL_LEFT) return fail(PALIAL_ERR_ARG, "only side = Left (PALIAL_LEFT) is supported");
+    if (trans != PALIAL_TRANS && trans != PALIAL_NOTRANS) return fail(PALIAL_ERR_ARG, "trans %d", trans);
+    const int N = A.matrix_size, b = A.tile_size;
+    if (!C || ncols <= 0 || ncols % b)
+        return fail(PALIAL_ERR_ARG, "C needs a whole number of %d-column tiles (ncols=%d)", b, ncols);
+    const int NC = ncols / b;
+    palial_opts o = o_in ? *o_in : default_opts();
+    const bool dev = o.device_matrix != 0;
+    hipStream_t s = (dev && o.stream) ? (hipStream_t)o.stream : g.s_main;
+    std::vector<double *> owned;
+    auto release = [&](int code) {
+        hipStreamSynchronize(s);
+        for (double *p : owned) hipFree(p);
+        return code;
+    };
+    double *dA, *dS, *dC = C;
+    if ((rc = qr_operands(A, S, dev, owned, dA, dS, s))) return release(rc);
+    const size_t cbytes = (size_t)N * ncols * 8;
+    if (!dev) {
+        if (hipMalloc(&dC, cbytes) != hipSuccess) return release(fail(PALIAL_ERR_NOMEM, "device copy of C"));
+        owned.push_back(dC);
+        if (hipMemcpyAsync(dC, C, cbytes, hipMemcpyHostToDevice, s) != hipSuccess)
+            return release(fail(PALIAL_ERR_HIP, "copy of C"));
+    }
+    if ((rc = grow(&g.d_aw, &g.aw_cap, 2 * (size_t)NC * b * b))) return release(rc);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    Launcher L;
+    hipError_t er = hipEventRecord(e0, s);
+    qr_apply_dev(trans == PALIAL_TRANS, dA, dS, dC, N, b, NC, g.d_aw, g.d_aw + (size_t)NC * b * b, s, L);
+    if (er == hipSuccess) er = hipEventRecord(e1, s);
+    if (er == hipSuccess && !dev) er = hipMemcpyAsync(C, dC, cbytes, hipMemcpyDeviceToHost, s);
+    if (er == hipSuccess) er = hipStreamSynchronize(s);
+    float ms = 0;
+    if (er == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    prof_collect();
+    if (L.err != hipSuccess) return release(fail(PALIAL_ERR_HIP, "Q application launch: %s", hipGetErrorString(L.err)));
+    if (er != hipSuccess) return release(fail(PALIAL_ERR_HIP, "Q application: %s", hipGetErrorString(er)));
+    if (rep) {
+        memset(rep, 0, sizeof *rep);
+        const double fl = 2.0 * (double)N * N * ncols;   // dormqr with m = k = N, n = ncols: 4mnk - 2nk^2
+        rep->solve_ms = ms;
+        rep->gflop = fl / 1e9;
+        rep->tflops = ms > 0 ? fl / (ms * 1e-3) / 1e12 : 0.0;
+        rep->nlaunches = L.launches;
+    }
+    return release(0);
+}
+
+int palial_hip_qr_solve(MATRIX_desc A, MATRIX_desc S, double *B, int nrhs, int ldb, const palial_opts *o_in,
+                        palial_report *rep)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_qr_pair(A, S))) return rc;
+    const int N = A.matrix_size, b = A.tile_size, MT = N / b;
+    if (b % 64 || b > 512) return fail(PALIAL_ERR_ARG, "QR solve needs b %% 64 == 0 and b <= 512 (b=%d)", b);
+    if (nrhs < 1 || nrhs > 16) return fail(PALIAL_ERR_ARG, "nrhs=%d outside 1..16", nrhs);
+    if (!B || ldb < N) return fail(PALIAL_ERR_ARG, "bad right-hand side (ldb=%d)", ldb);
+    palial_opts o = o_in ? *o_in : default_opts();
+    const bool dev = o.device_matrix != 0;
+    hipStream_t s = (dev && o.stream) ? (hipStream_t)o.stream : g.s_main;
+    std::vector<double *> owned;
+    auto release = [&](int code) {
+        hipStreamSynchronize(s);
+        for (double *p : owned) hipFree(p);
+        return code;
+    };
+    double *dA, *dS;
+    if ((rc = qr_operands(A, S, dev, owned, dA, dS, s))) return release(rc);
+    // workspace: right-hand-side tiles (N x b), Y and X (N x nrhs), the R tile
+    // inverses (MT b^2), the 32x32 block inverses (MT 32 b)
+    const size_t nv = (size_t)N * nrhs, nt = (size_t)N * b;
+    if ((rc = grow(&g.d_qs, &g.qs_cap, nt + 2 * nv + (size_t)MT * b * b + (size_t)MT * 32 * b))) return release(rc);
+    if ((rc = grow(&g.d_aw, &g.aw_cap, 2 * (size_t)b * b))) return release(rc);
+    double *Ct = g.d_qs, *Y = Ct + nt, *X = Y + nv, *Rinv = X + nv, *dinv = Rinv + (size_t)MT * b * b;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    Launcher L;
+    hipError_t er = hipMemcpy2DAsync(Y, (size_t)N * 8, B, (size_t)ldb * 8, (size_t)N * 8, nrhs,
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+    if (er == hipSuccess) er = hipEventRecord(e0, s);
+    // Q^T B on one tile column
+    L.chk(launch_cols_to_tiles(Y, N, N, nrhs, Ct, b, s));
+    qr_apply_dev(true, dA, dS, Ct, N, b, 1, g.d_aw, g.d_aw + (size_t)b * b, s, L);
+    L.chk(launch_tiles_to_cols(Ct, N, b, Y, N, nrhs, s));
+    // R x = Q^T B by tile back substitution: R lives where LU keeps U (upper
+    // triangles of the diagonal tiles and the tiles right of them)
+    L.chk(launch_upper_block_inverses(dA, N, b, dinv, 32L * b, s));
+    {
+        TrsmArgs t;
+        memset(&t, 0, sizeof t);
+        t.mat = dA; t.single = Rinv; t.N = N; t.b = b; t.MT = MT; t.ntiles = MT;
+        t.batch = 1; t.ident = 1; t.dinv_stride = 32L * b;
+        L.chk(launch_trsm2(TR_LUR, t, dinv, s));
+    }
+    GemvArgs a;
+    memset(&a, 0, sizeof a);
+    a.mat = dA; a.N = N; a.b = b; a.ld = N; a.nrhs = nrhs; a.inv = Rinv;
+    for (int k = MT - 1; k >= 0; k--) {
+        a.k = k;
+        a.mode = GV_DIAG; a.trans = 0; a.vin = Y; a.vout = X; a.ntasks = 1;
+        L.chk(launch_tile_gemv(a, s));
+        if (k > 0) {
+            a.mode = GV_BWD; a.vin = X; a.vout = Y; a.ntasks = k;
+            L.chk(launch_tile_gemv(a, s));
+        }
+    }
+    if (er == hipSuccess) er = hipEventRecord(e1, s);
+    if (er == hipSuccess)
+        er = hipMemcpy2DAsync(B, (size_t)ldb * 8, X, (size_t)N * 8, (size_t)N * 8, nrhs,
+                              dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s);
+    if (er == hipSuccess) er = hipStreamSynchronize(s);
+    float ms = 0;
+    if (er == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    prof_collect();
+    if (L.err != hipSuccess) return release(fail(PALIAL_ERR_HIP, "QR solve launch: %s", hipGetErrorString(L.err)));
+    if (er != hipSuccess) return release(fail(PALIAL_ERR_HIP, "QR solve: %s", hipGetErrorString(er)));
+    if (rep) {
+        memset(rep, 0, sizeof *rep);
+        rep->solve_ms = ms;
+        rep->nlaunches = L.launches;
+    }
+    return release(0);
+}
+
+int palial_hip_transpose(MATRIX_desc A, int average, const palial_opts *o_in)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init()) || (rc = check_desc(A))) return rc;
+    const int N = A.matrix_size, b = A.tile_size;
+    if (b % 32) return fail(PALIAL_ERR_ARG, "transpose needs b %% 32 == 0 (b=%d)", b);
+    palial_opts o = o_in ? *o_in : default_opts();
+    hipStream_t s = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
+    const size_t bytes = (size_t)N * N * 8;
+    double *d = (double *)A.matrix;
+    if (!o.device_matrix) {
+        if (hipMalloc(&d, bytes) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "device copy");
+    }
+    hipError_t e = o.device_matrix ? hipSuccess : hipMemcpyAsync(d, A.matrix, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_transpose(d, N, b, average != 0, s);
+    if (e == hipSuccess && !o.device_matrix) e = hipMemcpyAsync(A.matrix, d, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    else hipStreamSynchronize(s);
+    if (!o.device_matrix) hipFree(d);
+    if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "transpose: %s", hipGetErrorString(e));
+    return 0;
 }
 
 // ---------------------------------------------------------------- tile kernels

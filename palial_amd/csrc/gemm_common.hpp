@@ -152,6 +152,51 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.D = t.C;
         break;
     }
+    case GM_QA_W: {
+        const int n = task;
+        t.A = M + tile_off(b, MT, p.m, k);        // V (transposed)
+        t.B = p.cmat + tile_off(b, MT, p.m, n);   // C2
+        t.C = p.w1 + (size_t)task * bb;           // W
+        t.D = p.cmat + tile_off(b, MT, k, n);     // + C1
+        break;
+    }
+    case GM_QA_TW: {
+        const int n = task;
+        t.A = p.smat + tile_off(b, MT, p.m, k);   // T (op by the launch), upper
+        t.B = p.w1 + (size_t)task * bb;
+        t.C = p.w2 + (size_t)task * bb;
+        t.E = p.cmat + tile_off(b, MT, k, n);     // C1 -= op(T) W
+        break;
+    }
+    case GM_QA_VW: {
+        const int n = task;
+        t.A = M + tile_off(b, MT, p.m, k);        // V
+        t.B = p.w2 + (size_t)task * bb;
+        t.C = p.cmat + tile_off(b, MT, p.m, n);
+        t.D = t.C;
+        break;
+    }
+    case GM_OA_W: {
+        const int n = task;
+        t.A = M + tile_off(b, MT, k, k);          // V (transposed, unit lower)
+        t.B = p.cmat + tile_off(b, MT, k, n);
+        t.C = p.w1 + (size_t)task * bb;
+        break;
+    }
+    case GM_OA_TW: {
+        t.A = p.smat + tile_off(b, MT, k, k);     // T (op by the launch), upper
+        t.B = p.w1 + (size_t)task * bb;
+        t.C = p.w2 + (size_t)task * bb;
+        break;
+    }
+    case GM_OA_VW: {
+        const int n = task;
+        t.A = M + tile_off(b, MT, k, k);          // V (unit lower)
+        t.B = p.w2 + (size_t)task * bb;
+        t.C = p.cmat + tile_off(b, MT, k, n);
+        t.D = t.C;
+        break;
+    }
     default:
         t.valid = false;
     }

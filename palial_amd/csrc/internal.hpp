@@ -34,6 +34,15 @@ enum GemmMode : int {
     GM_LU_REST = 12,    // step k, tiles (m, n) with m, n >= k+2
     GM_CHOL_FIRST = 13, // GM_CHOL_ROW followed by tile (k+2, k+2) (task T)
     GM_LU_FIRST = 14,   // GM_LU_PANEL followed by tile (k+2, k+2) (task 2T-1)
+    // Q of a finished factorization (V in mat, T in smat) applied from the
+    // left to another layout-L matrix cmat with the same row tiles; task n =
+    // column tile n of cmat (every column tile, not only n > k)
+    GM_QA_W = 15,       // TS block (k,m): W_n = V^T C(m,n) + C(k,n), V = A(m,k)
+    GM_QA_TW = 16,      // W2_n = op(T) W_n (T = S(m,k) upper);  C(k,n) -= W2_n
+    GM_QA_VW = 17,      // C(m,n) -= V W2_n
+    GM_OA_W = 18,       // diagonal block k: W_n = V^T C(k,n), V = A(k,k) unit lower
+    GM_OA_TW = 19,      // W2_n = op(T) W_n (T = S(k,k) upper)
+    GM_OA_VW = 20,      // C(k,n) -= V W2_n
 };
 
 enum MaskA : int { MK_NONE = 0, MK_UPPER = 1, MK_UNIT_LOWER = 2 };
@@ -62,6 +71,7 @@ struct GemmArgs {
     int maskA;
     int single_diag;     // GM_SINGLE: output is a SYRK tile (upper triangle only)
     int f32c;            // packed updates: C and D are FP32 tiles at the same element offsets from mat
+    double *cmat;        // GM_QA_* / GM_OA_*: the matrix Q is applied to (layout L, MT row tiles)
 };
 
 // host launchers (k_gemm.hip)
@@ -195,5 +205,15 @@ hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N
 hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s,
                              bool tile32 = false);   // tile32: the tile is FP32 (sub stays FP64)
 hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2, hipStream_t s);
+// in-place A <- A^T (avg false) or A <- (A + A^T) / 2 (avg true) of a layout-L matrix, b % 32 == 0
+hipError_t launch_transpose(double *mat, int N, int b, bool avg, hipStream_t s);
+// N x nrhs column-major block (leading dimension ld) <-> N x b layout-L block
+// of one tile column (columns nrhs..b-1 zero on the way in), nrhs <= b
+hipError_t launch_cols_to_tiles(const double *v, int N, int ld, int nrhs, double *tiles, int b, hipStream_t s);
+hipError_t launch_tiles_to_cols(const double *tiles, int N, int b, double *v, int ld, int nrhs, hipStream_t s);
+// inverses of the 32x32 diagonal blocks of the upper triangles of the
+// diagonal tiles of a layout-L matrix: tile k's block j at dinv + k*stride + j*1024
+// (column-major, ld 32), as the factorization kernels write them (b % 32 == 0)
+hipError_t launch_upper_block_inverses(const double *mat, int N, int b, double *dinv, long stride, hipStream_t s);
 
 }  // namespace palial

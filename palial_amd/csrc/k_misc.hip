@@ -25,6 +25,19 @@ __device__ inline double uniform(uint64_t seed, uint64_t i, uint64_t j)
     return (double)(x >> 11) * 0x1.0p-53 - 0.5;
 }
 
+// Standard normal entry (i,j) by the Box-Muller transform of two uniforms
+// drawn from the same counter: h = splitmix64(splitmix64(seed) + (i<<32 | j)),
+// u1 = ((h >> 11) + 1) 2^-53 in (0, 1], u2 = (splitmix64(h) >> 11) 2^-53,
+// z = sqrt(-2 ln u1) cos(2 pi u2).  Host code with another libm agrees to a
+// few ulp of log and cos, not bit for bit.
+__device__ inline double gaussian(uint64_t seed, uint64_t i, uint64_t j)
+{
+    const uint64_t h = sm64(sm64(seed) + ((i << 32) | (j & 0xffffffffull)));
+    const double u1 = (double)((h >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(sm64(h) >> 11) * 0x1.0p-53;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+}
+
 __global__ void generate_kernel(double *mat, int N, int b, int kind, uint64_t seed, double shift)
 {
     const size_t total = (size_t)N * N;
@@ -49,6 +62,12 @@ __global__ void generate_kernel(double *mat, int N, int b, int kind, uint64_t se
         case 2:
             v = 0.5;
             if (i == j) v = (uniform(seed, i, i) + 0.5) * 173.0 + 173.0;
+            break;
+        case 4:
+            v = gaussian(seed, i, j);
+            break;
+        case 5:   // diag(s), s_i = kappa^(-i/(N-1)) from 1 down to 1/kappa (kappa = shift)
+            v = (i != j) ? 0.0 : (N > 1 ? pow(shift, -(double)i / (double)(N - 1)) : 1.0);
             break;
         default:
             v = (i == j) ? 1.0 : 0.0;
@@ -81,6 +100,90 @@ hipError_t launch_batch_generate(const int *n, const long *off, const int *kind,
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(batch_generate_kernel, dim3((unsigned)count), dim3(256), 0, s, n, off, kind, A, f32, seed);
+    return hipGetLastError();
+}
+
+// In-place transpose (avg = 0: A <- A^T) or symmetrization (avg = 1:
+// A <- (A + A^T) / 2) of a layout-L matrix.  Workgroup w takes the pair of
+// 32x32 blocks (bi, bj) / (bj, bi), bi <= bj, of the global matrix: both are
+// read through LDS (32 rows of one column are 256 contiguous bytes of a tile
+// column), combined and written back; the diagonal blocks pair with
+// themselves.  b % 32 == 0.
+__global__ __launch_bounds__(256) void transpose_kernel(double *mat, int N, int b, int avg, long npairs)
+{
+    __shared__ double s1[32][33], s2[32][33];
+    const int MT = N / b, NBK = N / 32;
+    const int t = threadIdx.x;
+    for (long w = blockIdx.x; w < npairs; w += gridDim.x) {
+        // w -> (bi, bj), bi <= bj: w = bj (bj + 1) / 2 + bi
+        int bj = (int)((sqrt(8.0 * (double)w + 1.0) - 1.0) * 0.5);
+        while ((long)(bj + 1) * (bj + 2) / 2 <= w) bj++;
+        while ((long)bj * (bj + 1) / 2 > w) bj--;
+        const int bi = (int)(w - (long)bj * (bj + 1) / 2);
+        if (bj >= NBK) break;
+        auto addr = [&](int gi, int gj) -> size_t {   // global element (gi, gj) in layout L
+            return tile_off(b, MT, gi / b, gj / b) + (size_t)(gi % b) + (size_t)(gj % b) * b;
+        };
+        for (int e = t; e < 1024; e += 256) {
+            const int r = e % 32, c = e / 32;
+            s1[r][c] = mat[addr(bi * 32 + r, bj * 32 + c)];
+            s2[r][c] = mat[addr(bj * 32 + r, bi * 32 + c)];
+        }
+        __syncthreads();
+        for (int e = t; e < 1024; e += 256) {
+            const int r = e % 32, c = e / 32;
+            // new (bi,bj)(r,c) from old (bj,bi)(c,r) and vice versa
+            const double x = s1[r][c], y = s2[c][r];
+            const double u = avg ? 0.5 * (x + y) : y;
+            const double v = avg ? u : x;
+            mat[addr(bi * 32 + r, bj * 32 + c)] = u;
+            if (bi != bj) mat[addr(bj * 32 + c, bi * 32 + r)] = v;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_transpose(double *mat, int N, int b, bool avg, hipStream_t s)
+{
+    if (b % 32 || N % b) return hipErrorInvalidValue;
+    const long nbk = N / 32, npairs = nbk * (nbk + 1) / 2;
+    const long blocks = npairs < 65536 ? npairs : 65536;
+    hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mat, N, b, avg ? 1 : 0, npairs);
+    return hipGetLastError();
+}
+
+__global__ void cols_to_tiles_kernel(const double *v, int N, int ld, int nrhs, double *tiles, int b)
+{
+    // tiles: N x b in layout L with one tile column, i.e. tile m at m*b*b
+    const size_t total = (size_t)N * b;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t m = e / ((size_t)b * b), within = e % ((size_t)b * b);
+        const int r = (int)(within % b), c = (int)(within / b);
+        const size_t i = m * b + r;
+        tiles[e] = (c < nrhs) ? v[i + (size_t)c * ld] : 0.0;
+    }
+}
+
+__global__ void tiles_to_cols_kernel(const double *tiles, int N, int b, double *v, int ld, int nrhs)
+{
+    const size_t total = (size_t)N * nrhs;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e % N, c = e / N;
+        v[i + c * ld] = tiles[(i / b) * b * b + (i % b) + c * b];
+    }
+}
+
+hipError_t launch_cols_to_tiles(const double *v, int N, int ld, int nrhs, double *tiles, int b, hipStream_t s)
+{
+    if (nrhs < 1 || nrhs > b || ld < N || N % b) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cols_to_tiles_kernel, dim3(2048), dim3(256), 0, s, v, N, ld, nrhs, tiles, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiles_to_cols(const double *tiles, int N, int b, double *v, int ld, int nrhs, hipStream_t s)
+{
+    if (nrhs < 1 || nrhs > b || ld < N || N % b) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tiles_to_cols_kernel, dim3(2048), dim3(256), 0, s, tiles, N, b, v, ld, nrhs);
     return hipGetLastError();
 }
 

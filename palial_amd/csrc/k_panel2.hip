@@ -898,6 +898,38 @@ __global__ __launch_bounds__(P2_THREADS, 2) void trsm2f_kernel(TrsmArgs p, const
     }
 }
 
+// ---------------------------------------------------------------- R blocks
+// Inverses of the 32x32 diagonal blocks of the upper triangle of every
+// diagonal tile (the R of a tiled QR): workgroup (k, j) inverts block j of
+// tile (k,k) with the factorizations' recursive doubling and writes it where
+// they write theirs, so that the batched TRSM2 can invert whole tiles.
+__global__ __launch_bounds__(256) void upper_block_inv_kernel(const double *mat, int N, int b, double *dinv,
+                                                               long stride)
+{
+    __shared__ double sM[NB2 * LDD], sX[NB2 * LDD], sT[NB2 * NB2];
+    const int MT = N / b, nbk = b / NB2;
+    const int k = blockIdx.x / nbk, j = blockIdx.x % nbk;
+    const double *D = mat + tile_off(b, MT, k, k);
+    const int j0 = j * NB2;
+    for (int e = threadIdx.x; e < NB2 * NB2; e += 256) {
+        const int r = e % NB2, c = e / NB2;
+        sM[r + c * LDD] = (r <= c) ? D[(size_t)(j0 + r) + (size_t)(j0 + c) * b] : 0.0;
+    }
+    __syncthreads();
+    tri_inverse32<true>(sM, sX, sT, 256);
+    double *out = dinv + (size_t)k * stride + (size_t)j * NB2 * NB2;
+    for (int e = threadIdx.x; e < NB2 * NB2; e += 256) out[e] = sX[(e % NB2) + (e / NB2) * LDD];
+}
+
+hipError_t launch_upper_block_inverses(const double *mat, int N, int b, double *dinv, long stride, hipStream_t s)
+{
+    if (b % NB2 || N % b) return hipErrorInvalidValue;
+    const int MT = N / b;
+    hipLaunchKernelGGL(upper_block_inv_kernel, dim3((unsigned)(MT * (b / NB2))), dim3(256), 0, s, mat, N, b, dinv,
+                       stride);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- launchers
 bool panel2_ok(int b) { return b % NB2 == 0 && b >= NB2 && b <= 512; }
 

@@ -280,3 +280,60 @@ def test_solves(orc):
     G = orc.to_dense(g, N, b)
     x = orc.lu_solve(orc.lu(g, N, b), N, b, rhs)
     assert rel(x, np.linalg.solve(G, rhs)) < 1e-13
+
+
+# ---------------------------------------------------------------- Q application, config-3 inputs
+@pytest.mark.parametrize("N,b", [(64, 16), (96, 32)])
+def test_apply_q_invariants(orc, N, b):
+    """The oracle's Q application is pinned by the factorization it inverts:
+    Q^T A = R and Q R = A (a reversed order, a dropped block or T in place of
+    T^T breaks both), Q orthogonal, and Q^T (Q C) = C."""
+    a = orc.gen("dense", N, b, seed=3, shift=0.0)
+    F, S = orc.qr(a, N, b)
+    A = orc.to_dense(a, N, b)
+    R = np.triu(orc.to_dense(F, N, b))
+    assert np.abs(orc.apply_q(F, S, N, b, A, trans=True) - R).max() < 1e-13 * N
+    assert np.abs(orc.apply_q(F, S, N, b, R, trans=False) - A).max() < 1e-13 * N
+    Q = orc.form_q(F, S, N, b)
+    assert np.abs(Q.T @ Q - np.eye(N)).max() < 1e-14 * N
+    Cm = np.random.default_rng(0).standard_normal((N, 2 * b))
+    back = orc.apply_q(F, S, N, b, orc.apply_q(F, S, N, b, Cm, trans=False), trans=True)
+    assert np.abs(back - Cm).max() < 1e-13 * N
+
+
+def test_gauss_generator_is_standard_normal(orc):
+    """N(0,1) by Box-Muller: moments and a Kolmogorov-Smirnov test on 2^18
+    entries, plus the closed form of one entry recomputed from its counter."""
+    import math
+    from scipy import stats
+    N, b = 512, 128
+    g = orc.gen("gauss", N, b, seed=11)
+    n = g.size
+    assert abs(g.mean()) < 5 / math.sqrt(n)
+    assert abs(g.var() - 1.0) < 5 * math.sqrt(2.0 / n)
+    assert stats.kstest(g, "norm").pvalue > 1e-3
+
+    def sm64(z):
+        M = (1 << 64) - 1
+        z = (z + 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    i, j = 5, 300
+    h = sm64((sm64(11) + ((i << 32) | j)) & ((1 << 64) - 1))
+    u1 = ((h >> 11) + 1) * 2.0 ** -53
+    u2 = (sm64(h) >> 11) * 2.0 ** -53
+    z = math.sqrt(-2 * math.log(u1)) * math.cos(2 * math.pi * u2)
+    D = orc.to_dense(g, N, b)
+    assert D[i, j] == pytest.approx(z, rel=1e-15, abs=1e-15)
+    assert orc.gauss(11, i, j) == D[i, j]
+
+
+def test_geom_diag_spectrum(orc):
+    N, b, kappa = 256, 64, 1e8
+    d = np.diag(orc.to_dense(orc.gen("geomdiag", N, b, shift=kappa), N, b))
+    assert d[0] == 1.0 and d[-1] == pytest.approx(1 / kappa, rel=4e-16)
+    r = d[1:] / d[:-1]
+    assert np.allclose(r, kappa ** (-1.0 / (N - 1)), rtol=1e-13)
+    off = orc.to_dense(orc.gen("geomdiag", N, b, shift=kappa), N, b) - np.diag(d)
+    assert not off.any()
