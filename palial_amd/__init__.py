@@ -247,7 +247,11 @@ def qr_apply(A, S, Cm, trans=TRANS, b=None, N=None, ncols=None, side=LEFT):
     DeviceMatrix together with device A and S.  Returns the report."""
     init()
     if isinstance(A, DeviceMatrix):
-        dA, dS, cp, dev, nc = A.desc, S.desc, Cm.ptr, True, Cm.N
+        dA, dS, dev = A.desc, S.desc, True
+        if isinstance(Cm, DeviceMatrix):
+            cp, nc = Cm.ptr, Cm.N
+        else:                       # a device pointer to N x ncols in layout L
+            cp, nc = Cm, ncols
     else:
         dA, dev, _ = _desc_of(A, N, b)
         dS, _, _ = _desc_of(S, N, b)

@@ -18,6 +18,7 @@
 #include "../../include/palial_hip.h"
 
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <chrono>
 #include <cstdarg>
@@ -1384,10 +1385,14 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         eta = 0.0;
         for (int j = 0; j < nrhs; j++) {
             const double den = normA * hred[1 + 2 * nrhs + j] + normB[j];
-            const double ej = den > 0 ? hred[1 + nrhs + j] / den : 0.0;
+            const double rn = hred[1 + nrhs + j];
+            // den == 0 only for b = 0 and x = 0 (then r = 0); a NaN or infinite
+            // norm makes eta NaN
+            const double ej = (den > 0 && std::isfinite(den)) ? rn / den : (den == 0 && rn == 0 ? 0.0 : NAN);
             if (!(ej <= eta)) eta = ej;      // NaN propagates
         }
-        if (eta <= tol || it >= max_iter) break;
+        // a factor whose solution overflowed or went NaN cannot be refined
+        if (eta <= tol || it >= max_iter || !std::isfinite(eta)) break;
         // with the FP64 fallback allowed, GMRES-IR that stalls (two steps in a
         // row that gain less than 4x) or runs long (more than one restart
         // cycle in all) would cost more than the FP64 solve

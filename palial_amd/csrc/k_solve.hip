@@ -614,17 +614,22 @@ size_t residual_scratch(int N, int b, int nrhs)
     return chunked > sym ? chunked : sym;
 }
 
-// per-column max |v| of an N x nrhs block (one workgroup per column)
+// max that keeps a NaN (fmax drops it): a NaN anywhere in the column must
+// make its norm NaN, or a broken solution would read as a zero residual
+__device__ inline double max_nan(double m, double a) { return (a > m || a != a) ? a : m; }
+
+// per-column max |v| of an N x nrhs block (one workgroup per column); NaN if
+// the column holds one
 __global__ __launch_bounds__(256) void colmax_abs_kernel(const double *v, int N, int ld, double *out)
 {
     __shared__ double sred[256];
     const int j = blockIdx.x, t = threadIdx.x;
     double m = 0.0;
-    for (int i = t; i < N; i += 256) m = fmax(m, fabs(v[(size_t)i + (size_t)j * ld]));
+    for (int i = t; i < N; i += 256) m = max_nan(m, fabs(v[(size_t)i + (size_t)j * ld]));
     sred[t] = m;
     __syncthreads();
     for (int s = 128; s >= 1; s >>= 1) {
-        if (t < s) sred[t] = fmax(sred[t], sred[t + s]);
+        if (t < s) sred[t] = max_nan(sred[t], sred[t + s]);
         __syncthreads();
     }
     if (t == 0) out[j] = sred[0];
