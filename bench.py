@@ -2,19 +2,21 @@
 
 Headline workload (one "step"): solve A x = b to backward error 1e-14 with
 N = 32768, b = 512, one N(0,1) right-hand side, A = Q diag(s) Q^T with s
-geometric from 1 to 1/kappa (kappa = 1e2 by default) and Q = Q1 (x) Q2 from
-Householder QR of seeded Gaussians (a geometric spectrum factorises over the
-Kronecker index, so A = A1 (x) A2 is generated on the device in O(N^2)).
-Each precision tier factors a device copy of A and refines in FP64 until the
+geometric from 1 to 1/kappa (kappa = 1e2 by default) and Q the orthogonal
+factor of the library's own tiled Householder QR of a seeded N x N standard
+normal matrix (formed once, applied twice per kappa on the device).  Each
+precision tier factors a device copy of A and refines in FP64 until the
 backward error reaches 1e-14.  The FP64 tier factors an FP64 copy; the mixed
-tiers factor an FP32 copy (trailing updates from FP32, BF16, FP8 or split BF16 packed panels
-with FP32 accumulation, panel TRSMs in FP32, diagonal tiles in FP64).  The
-timed region covers copy + factor + triangular solves + every refinement
-step, with A and b resident in HBM.
+tiers factor an FP32 copy (trailing updates from FP32, BF16, FP8 or split
+BF16 packed panels with FP32 accumulation, panel TRSMs in FP32, diagonal
+tiles in FP64).  The timed region covers copy + factor + triangular solves +
+every refinement step, with A and b resident in HBM.
 
 Prints ONE JSON line.  ``value`` is the Cholesky solve's effective
 throughput N^3/3 / time-to-solution of the fastest tier that reaches 1e-14
 (TFLOP/s); the FP64 tier and the LU solves are reported beside it.
+``grid`` repeats every tier for kappa in {1e2, 1e5, 1e8} and 1 and 16
+right-hand sides (one warm-up and one timed solve per point).
 ``roofline`` describes the dominant kernel of the headline solve (the packed
 low-precision trailing update, HBM-bound on the FP32 C tiles) from live
 HIP-event timing of every launch in one profiled solve; ``roofline_fp64_gemm``
@@ -22,7 +24,8 @@ does the same for the FP64 update GEMM over one profiled plain FP64
 factorization (A = S + N*I).  ``traffic`` is the measured HBM bytes per
 launch from profiles/traffic.json.  ``cpu_baseline`` times the CPU oracle's
 OpenMP task DAG (OpenBLAS tile kernels, one BLAS thread per task) on a
-bounded sample.
+bounded sample, with the plan's smaller runs (N=2048 and 8192, all three
+algorithms, median of 3, scaled residuals of both paths) under ``runs``.
 """
 import argparse
 import json
@@ -42,64 +45,79 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_algo(P, algo, N, b, steps, warmup, seed=0):
+def run_algo(P, algo, N, b, steps, warmup, seed=0, profile=True):
+    """Plain FP64 factorization of config 2's input (Cholesky S + N I, LU
+    G + N I, QR G), copied from a pristine device matrix before each run."""
     kind = P.GEN_SYM if algo == "cholesky" else P.GEN_DENSE
     A0 = P.DeviceMatrix(N, b)
     A = P.DeviceMatrix(N, b)
-    A0.generate(kind, seed=seed, shift=float(N))
-    flops = N ** 3 / 3 if algo == "cholesky" else 2 * N ** 3 / 3
+    S = P.DeviceMatrix(N, b) if algo == "qr" else None
+    A0.generate(kind, seed=seed, shift=0.0 if algo == "qr" else float(N))
+    flops = {"cholesky": N ** 3 / 3, "lu": 2 * N ** 3 / 3, "qr": 4 * N ** 3 / 3}[algo]
     for i in range(warmup):
         A.copy_from(A0)
-        info, rep = P.factor_device(algo, A)
+        info, rep = P.factor_device(algo, A, S=S)
         log(f"[bench] warmup {algo} {i}: {rep.factor_ms:.1f} ms info={info}")
     P.synchronize()
     t0 = time.perf_counter()
     fac = []
     for i in range(steps):
         A.copy_from(A0)
-        info, rep = P.factor_device(algo, A)
+        info, rep = P.factor_device(algo, A, S=S)
         fac.append(rep.factor_ms)
     P.synchronize()
     wall = (time.perf_counter() - t0) / steps
     log(f"[bench] {algo} N={N} b={b}: {wall*1e3:.1f} ms/step, factor {min(fac):.1f}-{max(fac):.1f} ms")
-    # profiled pass: per-launch HIP events, single stream (no look-ahead), not timed
-    A.copy_from(A0)
-    P.profile(True)
-    P.factor_device(algo, A, no_lookahead=True)
-    stats = {c: P.profile_stats(c) for c in ("gemm", "diag", "trsm")}
-    P.profile(False)
+    stats = None
+    if profile:
+        # profiled pass: per-launch HIP events, single stream (no look-ahead), not timed
+        A.copy_from(A0)
+        P.profile(True)
+        P.factor_device(algo, A, S=S, no_lookahead=True)
+        stats = {c: P.profile_stats(c) for c in ("gemm", "diag", "trsm", "tsqrt", "qrupd")}
+        P.profile(False)
     A.free(); A0.free()
+    if S is not None:
+        S.free()
     return {"ms_per_step": wall * 1e3, "factor_ms": sorted(fac)[len(fac) // 2], "tflops": flops / wall / 1e12,
             "factor_tflops": flops / (sorted(fac)[len(fac) // 2] * 1e-3) / 1e12, "graph_nodes": rep.nlaunches,
             "kernels": stats, "info": info}
 
 
-def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
+TIER_NAMES = {0: "fp64", 1: "fp32", 2: "bf16", 3: "fp8", 4: "bf16x3"}
+
+
+def solve_once(P, A, dX, rhs, nrhs, algo, prec, ft, max_iter=60):
+    """Copy the right-hand sides in, then time one solve (host clock around the
+    synchronous call; A and b already in HBM)."""
+    L = P.lib()
+    N = A.N
+    P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8 * nrhs))
+    P.synchronize()
+    t0 = time.perf_counter()
+    _, rep, rc = P.solve(A, (dX, nrhs, N), algo=algo, precision=prec, device=True, max_iter=max_iter,
+                         check=False, factor_tile=ft)
+    return (time.perf_counter() - t0) * 1e3, rep, rc
+
+
+def run_solves(P, A, kappa, steps, warmup, tiers, algos, factor_tiles):
     import ctypes as C
     import numpy as np
-    A = P.DeviceMatrix(N, b)
-    A1, A2 = P.kron_spd_factors(N, kappa, seed=0)
-    A.generate_kron(A1, A2)
+    N = A.N
     rhs = np.random.default_rng(1).standard_normal(N)
     dX = C.c_void_p()
     P._check(P.lib().palial_hip_malloc(C.byref(dX), N * 8))
-    L = P.lib()
     out = {}
     for algo in algos:
         for prec in tiers:
-            name = {0: "fp64", 1: "fp32", 2: "bf16", 3: "fp8", 4: "bf16x3"}[prec]
+            name = TIER_NAMES[prec]
             best = None
             for ft in factor_tiles:
                 times, rep, rc = [], None, 0
                 for i in range(warmup + steps):
-                    P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8))
-                    P.synchronize()
-                    t0 = time.perf_counter()
-                    _, rep, rc = P.solve(A, (dX, 1, N), algo=algo, precision=prec, device=True, max_iter=40,
-                                         check=False, factor_tile=ft)
-                    t1 = time.perf_counter()
+                    t, rep, rc = solve_once(P, A, dX, rhs, 1, algo, prec, ft, max_iter=40)
                     if i >= warmup:
-                        times.append((t1 - t0) * 1e3)
+                        times.append(t)
                 t = sorted(times)[len(times) // 2]
                 flops = N ** 3 / 3 if algo == "cholesky" else 2 * N ** 3 / 3
                 r = {"ms": round(t, 2), "factor_tile": ft, "factor_ms": round(rep.factor_ms, 2),
@@ -119,28 +137,54 @@ def run_solves(P, N, b, kappa, steps, warmup, tiers, algos, factor_tiles):
     prof = None
     if conv:
         key = min(conv, key=lambda k: conv[k]["ms"])
-        prec = {"fp64": 0, "fp32": 1, "bf16": 2, "fp8": 3, "bf16x3": 4}[key.split("_")[1]]
-        P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8))
-        P.synchronize()
+        prec = {v: k for k, v in TIER_NAMES.items()}[key.split("_")[1]]
         P.profile(True)
-        P.solve(A, (dX, 1, N), algo=lead, precision=prec, device=True, max_iter=40, check=False,
-                factor_tile=conv[key]["factor_tile"])
+        solve_once(P, A, dX, rhs, 1, lead, prec, conv[key]["factor_tile"], max_iter=40)
         prof = {c: P.profile_stats(c) for c in ("gemm", "pkgemm", "diag", "trsm", "pack")}
         prof["tier"] = key
         P.profile(False)
     P.lib().palial_hip_free(dX)
-    A.free()
     return out, prof
 
 
-def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), nrhss=(1, 16),
-              tiers=(0, 1, 2, 3, 4), solve_n=32768, algos=("cholesky", "lu", "qr"), solves=True):
-    """The size and condition-number sweep beside the headline (opt-in, --sweep):
-    FP64 factorizations of Cholesky (S + N I), LU (G + N I) and QR (G) at
-    b = 512 for each N (device time of one factorization after a warm-up),
-    and Cholesky solves at N = solve_n for each kappa, nrhs and tier."""
+def run_grid(P, gen, A, kappas, nrhss, tiers, algos, ft):
+    """Every tier at each (kappa, nrhs): one warm-up and one timed solve."""
     import ctypes as C
     import numpy as np
+    N = A.N
+    dX = C.c_void_p()
+    P._check(P.lib().palial_hip_malloc(C.byref(dX), N * 8 * max(nrhss)))
+    grid = {}
+    for kappa in kappas:
+        gen.fill(A, kappa)
+        for nrhs in nrhss:
+            rhs = np.random.default_rng(1).standard_normal((nrhs, N))   # row j = column j (column-major)
+            for algo in algos:
+                fp64_ms = None
+                for prec in tiers:
+                    name = TIER_NAMES[prec]
+                    solve_once(P, A, dX, rhs, nrhs, algo, prec, ft)
+                    t, rep, rc = solve_once(P, A, dX, rhs, nrhs, algo, prec, ft)
+                    if prec == 0:
+                        fp64_ms = t
+                    r = {"ms": round(t, 2), "factor_ms": round(rep.factor_ms, 2), "refine_ms": round(rep.solve_ms, 2),
+                         "iterations": rep.iterations, "gmres_iterations": rep.gmres_iterations,
+                         "backward_error": rep.backward_error, "converged": rc == 0,
+                         "precision_used": TIER_NAMES.get(rep.precision_used, rep.precision_used),
+                         "shifts": rep.shifts,
+                         "speedup_vs_fp64": round(fp64_ms / t, 3) if fp64_ms else None}
+                    grid[f"{algo}_kappa{kappa:g}_nrhs{nrhs}_{name}"] = r
+                    log(f"[grid] {algo} kappa={kappa:g} nrhs={nrhs} {name}: {t:.1f} ms, {rep.iterations} it "
+                        f"(+{rep.gmres_iterations} GMRES), eta={rep.backward_error:.2e}, used "
+                        f"{r['precision_used']}, shifts {rep.shifts}, x{r['speedup_vs_fp64']}")
+    P.lib().palial_hip_free(dX)
+    return grid
+
+
+def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), algos=("cholesky", "lu", "qr")):
+    """The size sweep beside the headline (opt-in, --sweep): FP64
+    factorizations of Cholesky (S + N I), LU (G + N I) and QR (G) at b = 512
+    for each N (device time of one factorization after a warm-up)."""
     fac = {}
     for algo in algos:
         flops_f = {"cholesky": 1 / 3, "lu": 2 / 3, "qr": 4 / 3}[algo]
@@ -162,40 +206,7 @@ def run_sweep(P, b=512, ns=(8192, 16384, 24576, 32768), kappas=(1e2, 1e5, 1e8), 
             A0.free(); A.free()
             if S is not None:
                 S.free()
-    sol = {}
-    if not solves:
-        return {"factorizations_fp64_b512": fac}
-    N = solve_n
-    A = P.DeviceMatrix(N, b)
-    L = P.lib()
-    dX = C.c_void_p()
-    P._check(L.palial_hip_malloc(C.byref(dX), N * 8 * max(nrhss)))
-    for kappa in kappas:
-        A1, A2 = P.kron_spd_factors(N, kappa, seed=0)
-        A.generate_kron(A1, A2)
-        for nrhs in nrhss:
-            rhs = np.random.default_rng(1).standard_normal((nrhs, N))      # column j = row j (column-major)
-            for prec in tiers:
-                name = {0: "fp64", 1: "fp32", 2: "bf16", 3: "fp8", 4: "bf16x3"}[prec]
-                res = None
-                for i in range(2):
-                    P._check(L.palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8 * nrhs))
-                    P.synchronize()
-                    t0 = time.perf_counter()
-                    _, rep, rc = P.solve(A, (dX, nrhs, N), algo="cholesky", precision=prec, device=True,
-                                         max_iter=60, check=False, factor_tile=b)
-                    t1 = time.perf_counter()
-                    res = {"ms": round((t1 - t0) * 1e3, 2), "factor_ms": round(rep.factor_ms, 2),
-                           "refine_ms": round(rep.solve_ms, 2), "iterations": rep.iterations,
-                           "gmres_iterations": rep.gmres_iterations, "backward_error": rep.backward_error,
-                           "converged": rc == 0, "precision_used": rep.precision_used}
-                sol[f"kappa{kappa:g}_nrhs{nrhs}_{name}"] = res
-                log(f"[sweep] cholesky solve N={N} kappa={kappa:g} nrhs={nrhs} {name}: {res['ms']:.1f} ms, "
-                    f"{res['iterations']} it (+{res['gmres_iterations']} GMRES), eta={res['backward_error']:.2e}, "
-                    f"converged={res['converged']}")
-    L.palial_hip_free(dX)
-    A.free()
-    return {"factorizations_fp64_b512": fac, "cholesky_solves": sol}
+    return {"factorizations_fp64_b512": fac}
 
 
 def run_batched(P, count=10000, steps=3, seed=0):
@@ -241,15 +252,97 @@ def load_traffic():
         return {}
 
 
-def cpu_baseline(N=32768, b=512):
-    import oracle
-    ob = oracle.use_openblas()
-    oracle.openblas_threads(1)
+def cpu_info():
+    """(CPU model, cores in this process's affinity set, threads used): the
+    threads are every core the process may run on, capped by OMP_NUM_THREADS
+    when the environment sets it (the host's share of cores for one GPU)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count()
-    threads = max(1, min(16, cores))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(cores, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else cores
+    return model, cores, threads
+
+
+def scaled_residuals(P, oracle, algo, a, F, S, N, b, gpu_q):
+    """||A - F||_F / (n eps ||A||_F) of a factorization in layout L (QR: also
+    ||I - Q^T Q||_F / (n eps)); Q formed by the GPU Q application (gpu_q) or
+    by the oracle."""
+    import numpy as np
+    eps = np.finfo(np.float64).eps
+    A = oracle.to_dense(a, N, b)
+    D = oracle.to_dense(F, N, b)
+    nA = np.linalg.norm(A)
+    if algo == "cholesky":
+        U = np.triu(D)
+        return [float(np.linalg.norm(A - U.T @ U) / (N * eps * nA))]
+    if algo == "lu":
+        return [float(np.linalg.norm(A - (np.tril(D, -1) + np.eye(N)) @ np.triu(D)) / (N * eps * nA))]
+    if gpu_q:
+        Qt = oracle.gen("identity", N, b)
+        P.qr_apply(F, S, Qt, trans=P.NOTRANS, b=b, N=N)
+        Q = oracle.to_dense(Qt, N, b)
+    else:
+        Q = oracle.form_q(F, S, N, b)
+    return [float(np.linalg.norm(A - Q @ np.triu(D)) / (N * eps * nA)),
+            float(np.linalg.norm(np.eye(N) - Q.T @ Q) / (N * eps))]
+
+
+def cpu_baseline(P, N=32768, b=512, small=((2048, 256), (8192, 512)), reps=3):
+    """The reference algorithm on the host: the oracle's OpenMP task DAG
+    (src/cholesky.h, src/lu.h, src/qr.h order, correct tile dependences) with
+    OpenBLAS tile kernels and one BLAS thread per task.  value: one Cholesky at
+    the headline size.  runs: the smaller configurations for all three
+    algorithms, median of reps, with the scaled residuals of the CPU and GPU
+    factorizations of the same input."""
+    import numpy as np
+    import oracle
+    ob = oracle.use_openblas()
+    oracle.openblas_threads(1)
+    model, cores, threads = cpu_info()
+    runs = {}
+    flops_f = {"cholesky": 1 / 3, "lu": 2 / 3, "qr": 4 / 3}
+    for (n, bb) in small:
+        for algo in ("cholesky", "lu", "qr"):
+            a = oracle.gen("sym", n, bb, seed=0, shift=n) if algo == "cholesky" else \
+                oracle.gen("dense", n, bb, seed=0, shift=0.0 if algo == "qr" else float(n))
+            ts, F, S = [], None, None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                if algo == "cholesky":
+                    F, _ = oracle.cholesky(a, n, bb, omp_threads=threads)
+                elif algo == "lu":
+                    F = oracle.lu(a, n, bb, omp_threads=threads)
+                else:
+                    F, S = oracle.qr(a, n, bb, omp_threads=threads)
+                ts.append(time.perf_counter() - t0)
+            t = sorted(ts)[len(ts) // 2]
+            G = a.copy()
+            GS = np.zeros(n * n) if algo == "qr" else None
+            if algo == "cholesky":
+                _, grp = P.cholesky(G, b=bb)
+            elif algo == "lu":
+                _, grp, _ = P.lu(G, b=bb)
+            else:
+                _, grp = P.qr(G, GS, b=bb)
+            r = {"cpu_ms": round(t * 1e3, 2), "cpu_gflops": round(flops_f[algo] * n ** 3 / t / 1e9, 1),
+                 "gpu_factor_ms": round(grp.factor_ms, 3),
+                 "cpu_scaled_residuals": scaled_residuals(P, oracle, algo, a, F, S, n, bb, gpu_q=False),
+                 "gpu_scaled_residuals": scaled_residuals(P, oracle, algo, a, G, GS, n, bb, gpu_q=True),
+                 "max_rel_diff_gpu_cpu": float(np.abs(G - F).max() / np.abs(F).max())}
+            runs[f"{algo}_N{n}_b{bb}"] = r
+            log(f"[bench] cpu {algo} N={n} b={bb}: {t*1e3:.1f} ms ({r['cpu_gflops']} GFLOP/s), residuals cpu "
+                f"{r['cpu_scaled_residuals']} gpu {r['gpu_scaled_residuals']}, diff {r['max_rel_diff_gpu_cpu']:.1e}")
     a = oracle.gen("sym", N, b, seed=0, shift=N)
     t0 = time.perf_counter()
     oracle.cholesky(a, N, b, omp_threads=threads)
@@ -258,9 +351,12 @@ def cpu_baseline(N=32768, b=512):
     tf = N ** 3 / 3 / dt / 1e12
     log(f"[bench] cpu baseline cholesky N={N} b={b}: {dt:.2f} s, {tf*1e3:.1f} GFLOP/s on {threads} threads")
     return {"value": tf, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "cores_available": cores,
             "sample": f"oracle OpenMP task DAG (src/cholesky.h order, OpenBLAS tile kernels "
-                      f"{'bound' if ob else 'absent: naive C'}, 1 BLAS thread/task), Cholesky N={N} b={b}, "
-                      f"{dt:.2f} s"}
+                      f"{'bound' if ob else 'absent: naive C'}, 1 BLAS thread/task), Cholesky N={N} b={b} "
+                      f"(A = S + N I, seed 0), one run of {dt:.2f} s; runs: N=2048/b=256 and N=8192/b=512, "
+                      f"all three algorithms, median of {reps}",
+            "runs": runs}
 
 
 def main():
@@ -276,13 +372,16 @@ def main():
     ap.add_argument("--kappa", type=float, default=1e2)
     ap.add_argument("--tiers", default="0,1,2,3,4")
     ap.add_argument("--factor-tiles", default="256,512", help="internal factorization tile sizes tried per tier")
+    ap.add_argument("--grid-kappas", default="1e2,1e5,1e8", help="condition numbers of the grid ('' skips it)")
+    ap.add_argument("--grid-nrhs", default="1,16", help="right-hand-side counts of the grid")
     ap.add_argument("--no-factor", action="store_true", help="skip the plain factorization runs")
+    ap.add_argument("--factor-algos", default="cholesky,lu,qr", help="plain FP64 factorizations timed")
     ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")
+    ap.add_argument("--seed", type=int, default=0, help="seed of the Gaussian whose QR gives Q")
     ap.add_argument("--sweep", action="store_true",
-                    help="only the size / condition-number sweep (one JSON line with key 'sweep')")
+                    help="only the size sweep of the FP64 factorizations (one JSON line with key 'sweep')")
     ap.add_argument("--sweep-algos", default="cholesky,lu,qr", help="factorizations of the sweep")
     ap.add_argument("--sweep-ns", default="8192,16384,24576,32768", help="matrix sizes of the sweep")
-    ap.add_argument("--sweep-solves", type=int, default=1, help="0: skip the sweep's Cholesky solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -292,23 +391,34 @@ def main():
     P.init(0)
     if args.sweep:
         sw = run_sweep(P, b=args.b, ns=[int(x) for x in args.sweep_ns.split(",")],
-                       tiers=[int(x) for x in args.tiers.split(",")],
-                       algos=[a for a in args.sweep_algos.split(",") if a], solves=bool(args.sweep_solves))
+                       algos=[a for a in args.sweep_algos.split(",") if a])
         print(json.dumps({"sweep": sw}), flush=True)
         P.finalize()
         return 0
     algos = args.algos.split(",")
     tiers = [int(x) for x in args.tiers.split(",")]
     fts = [int(x) for x in args.factor_tiles.split(",")]
-    solves, sprof = run_solves(P, args.n, args.b, args.kappa, args.steps, args.warmup, tiers, algos, fts)
+    # config-3 input: Q from the library's own QR of a seeded Gaussian
+    t0 = time.perf_counter()
+    gen = P.CondGenerator(args.n, args.b, seed=args.seed)
+    A = P.DeviceMatrix(args.n, args.b)
+    gen.fill(A, args.kappa)
+    P.synchronize()
+    log(f"[bench] input: QR of the Gaussian + Q diag(s) Q^T in {time.perf_counter() - t0:.1f} s")
+    solves, sprof = run_solves(P, A, args.kappa, args.steps, args.warmup, tiers, algos, fts)
+    kappas = [float(x) for x in args.grid_kappas.split(",") if x]
+    grid = run_grid(P, gen, A, kappas, [int(x) for x in args.grid_nrhs.split(",")], tiers, algos,
+                    args.b) if kappas else None
+    A.free()
+    gen.free()
     res = {}
     if not args.no_factor:
-        for algo in algos:
-            res[algo] = run_algo(P, algo, args.n, args.b, args.steps, args.warmup)
+        for algo in [a for a in args.factor_algos.split(",") if a]:
+            res[algo] = run_algo(P, algo, args.n, args.b, args.steps, args.warmup, profile=(algo == "cholesky"))
     traffic = load_traffic()
-    head = res.get("cholesky") or next(iter(res.values()), None)
+    head = res.get("cholesky")
     roof64 = None
-    if head:
+    if head and head["kernels"]:
         g = head["kernels"]["gemm"]
         achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
         t64 = traffic.get("gemm_mfma_kernel")
@@ -332,12 +442,11 @@ def main():
                 "bytes_per_launch": k["bytes"] / max(1, k["launches"]),
                 "tflops": round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 1)}
     batched = run_batched(P, args.batched) if args.batched > 0 else None
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.b)
+    cpu = None if args.no_cpu else cpu_baseline(P, args.cpu_n, args.b)
     lead = algos[0]
     conv = {k: v for k, v in solves.items() if k.startswith(lead + "_") and v["converged"]}
     best_key = min(conv, key=lambda k: conv[k]["ms"]) if conv else f"{lead}_fp64"
     best = solves[best_key]
-    fp64 = solves.get(f"{lead}_fp64")
     tier = best_key.split("_")[1]
     for k, v in solves.items():
         ref = solves.get(k.split("_")[0] + "_fp64")
@@ -355,8 +464,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if tier == "fp64" else f"{tier}+f64",
-        "data": "synthetic (A = Q diag(s) Q^T, Q = Q1 (x) Q2 from QR of seeded Gaussians, geometric s; "
-                "b ~ N(0,1) seeded)",
+        "data": "synthetic: A = Q diag(s) Q^T, Q from the library's tiled Householder QR of a seeded N x N "
+                "N(0,1) matrix (Box-Muller, seed %d), s_i = kappa^(-i/(N-1)); b ~ N(0,1) seeded" % args.seed,
         "config": {"workload": f"{lead} solve to backward error 1e-14 (time-to-solution incl. factorization "
                                f"and FP64 refinement), N={args.n}, b={args.b}, kappa={args.kappa:g}, 1 RHS, "
                                f"tier {tier}", "N": args.n, "b": args.b, "kappa": args.kappa, "nrhs": 1,
@@ -366,6 +475,7 @@ def main():
         "speedup_vs_fp64": best["speedup_vs_fp64"],
         "backward_error": best["backward_error"],
         "solves": solves,
+        "grid": grid,
         "roofline": roof,
         "roofline_fp64_gemm": roof64,
         "solve_kernels": {c: v for c, v in (sprof or {}).items() if c != "tier"},
