@@ -2,7 +2,7 @@
  * palial_hip.h -- C ABI of libpalial_hip.so, the MI355X (gfx950) tiled dense
  * factorization library.  Plain C types only: a drop-in for the reference's
  * algorithm layer (src/cholesky.h, src/lu.h, src/qr.h) and tile kernels
- * (src/srclu/*.h, src/srcqr/*.h), with the reference's tile layout
+ * (the headers of src/srclu and src/srcqr), with the reference's tile layout
  * (include/descriptor.h, include/tile_address.h).
  *
  * Tile layout L: an N x N matrix (N % b == 0) is stored as (N/b)^2 contiguous
@@ -125,6 +125,32 @@ int palial_hip_profile(int enable);   /* 1: per-class totals, 2: also one record
 long palial_hip_profile_records(double *out, long cap);
 int palial_hip_profile_stats(int kclass, long *launches, double *total_ms, double *flops,
                              double *bytes);
+
+/* Per-task trace (the device-side counterpart of the reference's OMPT tracer,
+ * palial_trace/palial_trace.c:21-71, log.h:43-50).  palial_hip_trace(1): the
+ * following factorizations (plain, or inside a solve) launch stream-ordered
+ * and record the device time of every reference tile task: tasks inside a
+ * batched launch by device timestamps of their first and last workgroup,
+ * the diagonal-tile and QR panel tasks by HIP events.  When a loaded tracer
+ * defines the reference's hooks (include/common.h:9-11) they are called once
+ * per task after the factorization, in task-creation order, with host times
+ * (start of the call + device offsets).  palial_hip_trace_records copies the
+ * last factorization's tasks in creation order, 6 doubles each: kind, k, m, n
+ * (as palial_hip_task_name numbers them; unused -1), device ms (-1: not
+ * measured), start in ms from the start of the factorization; it returns the
+ * task count (negative count - 1 if the timestamp buffer overflowed). */
+int  palial_hip_trace(int enable);
+long palial_hip_trace_records(double *out, long cap);
+/* Reference task list (no GPU needed): algo 0 Cholesky, 1 LU, 2 QR with MT
+ * tiles per side, in the order the reference's loops create the tasks, with
+ * the names they give them (kind followed by k, m, n as the loop variables
+ * hold them then, src/cholesky.h:14-173, src/lu.h:13-173, src/qr.h:17-185).
+ * Writes the name of task `index` to buf (when buf is not NULL) and returns
+ * the task count.  Kinds: Cholesky 0 potrf(k), 1 trsm(k,m), 2 syrk(k,m),
+ * 3 gemm(k,m,n); LU 4 getrf(k), 5 trsm(k,m) of tile (m,k), 6 trsm(k,n) of
+ * tile (k,n), 7 gemm(k,m,n); QR 8 geqrt(k), 9 ormqr(k,n), 10 tsqrt(k,m),
+ * 11 tsmqr(k,m,n). */
+long palial_hip_task_name(int algo, int MT, long index, char *buf, int len);
 
 /* Test hook (no counterpart in the reference): enumerate the tile tasks of
  * one batched trailing-update launch (mode = internal GEMM mode, step k, TS

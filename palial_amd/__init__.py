@@ -119,6 +119,11 @@ def lib():
         L.palial_hip_mlp_forward.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.POINTER(Opts)]
         L.palial_hip_mlp_train.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_float, dp, dp,
                                            C.POINTER(Opts)]
+        L.palial_hip_trace.argtypes = [C.c_int]
+        L.palial_hip_trace_records.restype = C.c_long
+        L.palial_hip_trace_records.argtypes = [dp, C.c_long]
+        L.palial_hip_task_name.restype = C.c_long
+        L.palial_hip_task_name.argtypes = [C.c_int, C.c_int, C.c_long, C.c_char_p, C.c_int]
         L.palial_hip_profile_records.restype = C.c_long
         L.palial_hip_profile_records.argtypes = [dp, C.c_long]
         _lib = L
@@ -495,6 +500,36 @@ class Batch:
             if p:
                 lib().palial_hip_free(p)
         self.A = self.B = None
+
+
+def trace_enable(on=True):
+    """Per-task device timing of the following factorizations (palial_hip_trace)."""
+    init()
+    _check(lib().palial_hip_trace(int(bool(on))))
+
+
+def trace_records():
+    """The last traced factorization's tasks in creation order: (n, 6) array of
+    (kind, k, m, n, device ms, start ms); see palial_amd.trace."""
+    n = lib().palial_hip_trace_records(None, 0)
+    if n < 0:
+        raise PalialError(-3, "the task trace's timestamp buffer overflowed")
+    out = np.zeros((n, 6))
+    if n > 0:
+        lib().palial_hip_trace_records(out.ctypes.data, n)
+    return out
+
+
+def task_names(algo, MT):
+    """The library's reference task names in creation order (no GPU needed)."""
+    code = {"cholesky": 0, "lu": 1, "qr": 2}[algo]
+    n = lib().palial_hip_task_name(code, MT, 0, None, 0)
+    buf = C.create_string_buffer(32)
+    out = []
+    for i in range(n):
+        lib().palial_hip_task_name(code, MT, i, buf, 32)
+        out.append(buf.value.decode())
+    return out
 
 
 def profile_records():

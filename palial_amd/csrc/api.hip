@@ -15,6 +15,7 @@
 // replayed (graph replays do not keep stream CU masks, so a reserved schedule
 // is launched eagerly).
 #include "internal.hpp"
+#include "gemm_common.hpp"
 #include "../../include/palial_hip.h"
 
 #include <algorithm>
@@ -24,6 +25,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <sys/time.h>
 #include <map>
 #include <mutex>
 #include <string>
@@ -286,6 +288,40 @@ struct Prof {
 };
 Prof g_prof;
 
+// ------------------------------------------------------------------ per-task trace
+// palial_hip_trace(1): the next factorizations (plain or inside a solve)
+// record the device time of every reference tile task.  Batched launches on
+// the factorization's matrix get timestamp slots (trace_gemm / trace_trsm,
+// filled in by the kernels); the single-task launches of the diagonal chain
+// and the QR panels get a pair of HIP events.  trace_collect maps each launch
+// task back to its reference task with the kernels' own task decode.
+struct TraceLaunch {
+    bool gemm;
+    GemmArgs g;
+    int kind;
+    TrsmArgs t;
+};
+struct TraceSingle {
+    int cls;                  // KC_DIAG or KC_TSQRT
+    hipEvent_t a, e;
+};
+struct TaskTrace {
+    bool on = false;          // palial_hip_trace(1)
+    bool active = false;      // a traced factorization is being issued
+    const double *mat = nullptr;
+    int N = 0, b = 0, MT = 0, algo = -1;
+    unsigned long long *d_ts = nullptr;   // slot pairs; the last pair's start holds the t0 tick
+    size_t cap = 0;                       // slots
+    long next = 0;
+    bool overflow = false;
+    std::vector<TraceLaunch> launches;
+    std::vector<TraceSingle> singles;
+    hipEvent_t t0 = nullptr;
+    double host_t0 = 0.0;                 // gettimeofday at the start of the factorization (s)
+    std::vector<std::array<double, 6>> records;   // creation order: kind, k, m, n, ms, start ms
+};
+TaskTrace g_tr;
+
 // ------------------------------------------------------------------ DAGs
 struct Launcher {
     long launches = 0;
@@ -297,20 +333,36 @@ struct Launcher {
         launches++;
         if (e != hipSuccess && err == hipSuccess) err = e;
     }
+    void note(hipError_t e)      // a runtime call that is not a kernel launch
+    {
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    }
     // run one kernel launch of class cls with its algorithmic flops / bytes
     template <class F>
     void run(int cls, double flops, double bytes, hipStream_t s, F &&f)
     {
-        if (g_prof.on && !capturing) {
-            ProfRec r{cls, flops, bytes, b, prec, nullptr, nullptr};
-            hipEventCreate(&r.a);
-            hipEventCreate(&r.b_ev);
-            hipEventRecord(r.a, s);
-            chk(f());
-            hipEventRecord(r.b_ev, s);
+        const bool prof = g_prof.on && !capturing;
+        const bool tr = g_tr.active && !capturing && (cls == KC_DIAG || cls == KC_TSQRT);
+        ProfRec r{cls, flops, bytes, b, prec, nullptr, nullptr};
+        TraceSingle ts{cls, nullptr, nullptr};
+        if (prof) {
+            note(hipEventCreate(&r.a));
+            note(hipEventCreate(&r.b_ev));
+            note(hipEventRecord(r.a, s));
+        }
+        if (tr) {
+            note(hipEventCreate(&ts.a));
+            note(hipEventCreate(&ts.e));
+            note(hipEventRecord(ts.a, s));
+        }
+        chk(f());
+        if (prof) {
+            note(hipEventRecord(r.b_ev, s));
             g_prof.pending.push_back(r);
-        } else {
-            chk(f());
+        }
+        if (tr) {
+            note(hipEventRecord(ts.e, s));
+            g_tr.singles.push_back(ts);
         }
     }
 };
@@ -329,6 +381,269 @@ void prof_collect()
         hipEventDestroy(r.b_ev);
     }
     g_prof.pending.clear();
+}
+
+// ------------------------------------------------------------------ per-task trace
+// Reference tile tasks in creation order with the names the reference's task
+// loops give them: the kind, then k, m and n printed back to back, m and n
+// being whatever the loop variables hold when the task is created
+// (src/cholesky.h:14-173, src/lu.h:13-173, src/qr.h:17-185; the k = 0 tasks
+// of LU and QR print variables the reference leaves uninitialised, here 0).
+// Kinds: Cholesky 0 potrf(k), 1 trsm(k,m) [tile (k,m)], 2 syrk(k,m) [tile
+// (m,m)], 3 gemm(k,m,n) [tile (n,m)]; LU 4 getrf(k), 5 trsm(k,m) [tile
+// (m,k)], 6 trsm(k,n) [tile (k,n)], 7 gemm(k,m,n) [tile (m,n)]; QR 8
+// geqrt(k), 9 ormqr(k,n), 10 tsqrt(k,m), 11 tsmqr(k,m,n).  Unused indices -1.
+struct RefTask {
+    int kind, k, m, n;
+    char name[32];
+};
+enum TaskKind : int { TK_POTRF = 0, TK_CTRSM, TK_SYRK, TK_CGEMM, TK_GETRF, TK_TRSML, TK_TRSMU, TK_LGEMM,
+                      TK_GEQRT, TK_ORMQR, TK_TSQRT, TK_TSMQR };
+const char *const TK_NAME[] = {"potrf", "trsm", "syrk", "gemm", "getrf", "trsm", "trsm", "gemm",
+                               "geqrt", "ormqr", "tsqrt", "tsmqr"};
+
+void ref_tasks(int algo, int MT, std::vector<RefTask> &out)
+{
+    out.clear();
+    int k = 0, m = 0, n = 0;
+    auto add = [&](int kind, int kk, int mm, int nn) {
+        RefTask t{kind, kk, mm, nn, {0}};
+        snprintf(t.name, sizeof t.name, "%s%d%d%d", TK_NAME[kind], k, m, n);
+        out.push_back(t);
+    };
+    for (k = 0; k < MT; k++) {
+        if (algo == 0) {
+            add(TK_POTRF, k, -1, -1);
+            for (m = k + 1; m < MT; m++) add(TK_CTRSM, k, m, -1);
+            for (m = k + 1; m < MT; m++) {
+                add(TK_SYRK, k, m, -1);
+                for (n = k + 1; n < m; n++) add(TK_CGEMM, k, m, n);
+            }
+        } else if (algo == 1) {
+            add(TK_GETRF, k, -1, -1);
+            for (m = k + 1; m < MT; m++) add(TK_TRSML, k, m, -1);
+            for (n = k + 1; n < MT; n++) {
+                add(TK_TRSMU, k, -1, n);
+                for (m = k + 1; m < MT; m++) add(TK_LGEMM, k, m, n);
+            }
+        } else {
+            add(TK_GEQRT, k, -1, -1);
+            for (n = k + 1; n < MT; n++) add(TK_ORMQR, k, -1, n);
+            for (m = k + 1; m < MT; m++) {
+                add(TK_TSQRT, k, m, -1);
+                for (n = k + 1; n < MT; n++) add(TK_TSMQR, k, m, n);
+            }
+        }
+    }
+}
+
+// Start tracing the factorization of mat that is about to be issued on s.
+void trace_begin(double *mat, int N, int b, int algo, hipStream_t s, Launcher &L)
+{
+    if (!g_tr.on) return;
+    for (auto &x : g_tr.singles) {
+        hipEventDestroy(x.a);
+        hipEventDestroy(x.e);
+    }
+    g_tr.singles.clear();
+    g_tr.launches.clear();
+    g_tr.records.clear();
+    g_tr.mat = mat;
+    g_tr.N = N; g_tr.b = b; g_tr.MT = N / b; g_tr.algo = algo;
+    g_tr.next = 0;
+    g_tr.overflow = false;
+    const size_t MT = (size_t)(N / b);
+    // every reference task once per launch covering it (QR updates: 3 phases)
+    const size_t need = 3 * MT * MT * MT / 2 + 4 * MT * MT + 64;
+    if (g_tr.cap < need) {
+        if (g_tr.d_ts) hipFree(g_tr.d_ts);
+        g_tr.d_ts = nullptr;
+        g_tr.cap = 0;
+        if (hipMalloc(&g_tr.d_ts, (need + 1) * 16) != hipSuccess) {
+            L.note(hipErrorOutOfMemory);
+            return;
+        }
+        g_tr.cap = need;
+    }
+    if (!g_tr.t0) L.note(hipEventCreate(&g_tr.t0));
+    L.note(launch_trace_init(g_tr.d_ts, g_tr.cap, s));
+    L.note(hipEventRecord(g_tr.t0, s));
+    L.note(launch_clock_stamp(g_tr.d_ts + 2 * g_tr.cap, s));
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    g_tr.host_t0 = tv.tv_sec + 1e-6 * tv.tv_usec;
+    g_tr.active = true;
+}
+
+void trace_end() { g_tr.active = false; }
+
+}  // namespace
+
+// the reference's trace hooks (include/common.h:9-11), called only when a
+// tracer that defines them is loaded
+extern "C" {
+void palial_set_task_name(const char *name) __attribute__((weak));
+void palial_set_task_cpu_node(int cpu, int node, char *name) __attribute__((weak));
+void palial_get_task_time(struct timeval start, struct timeval end, char *name) __attribute__((weak));
+}
+
+namespace palial {
+
+void trace_gemm(GemmArgs &a)
+{
+    if (!g_tr.active) return;
+    const double *base = g_tr.mat;
+    const size_t nn = (size_t)g_tr.N * g_tr.N;
+    const bool mine = a.mode == GM_SINGLE ? (a.C >= base && a.C < base + nn) : a.mat == base;
+    if (!mine || a.ntasks <= 0) return;
+    if ((size_t)(g_tr.next + a.ntasks) > g_tr.cap) {
+        g_tr.overflow = true;
+        return;
+    }
+    a.ts = g_tr.d_ts;
+    a.ts_base = g_tr.next;
+    g_tr.next += a.ntasks;
+    TraceLaunch r;
+    memset(&r, 0, sizeof r);
+    r.gemm = true;
+    r.g = a;
+    g_tr.launches.push_back(r);
+}
+
+void trace_trsm(int kind, TrsmArgs &a)
+{
+    if (!g_tr.active || a.batch || a.mat != g_tr.mat || a.ntiles <= 0) return;
+    if ((size_t)(g_tr.next + a.ntiles) > g_tr.cap) {
+        g_tr.overflow = true;
+        return;
+    }
+    a.ts = g_tr.d_ts;
+    a.ts_base = g_tr.next;
+    g_tr.next += a.ntiles;
+    TraceLaunch r;
+    memset(&r, 0, sizeof r);
+    r.gemm = false;
+    r.kind = kind;
+    r.t = a;
+    g_tr.launches.push_back(r);
+}
+
+}  // namespace palial
+
+namespace {
+
+// After the traced factorization has completed: per reference task, the sum
+// of the device durations of the launch tasks that worked on it (a QR update
+// is three launches) and the earliest start, in ms from the start of the
+// factorization; -1 for a task nothing measured.  Calls the reference's trace
+// hooks when a tracer defines them.
+int trace_collect()
+{
+    if (!g_tr.on || g_tr.algo < 0) return 0;
+    const int MT = g_tr.MT, b = g_tr.b;
+    const size_t nslot = g_tr.cap + 1;
+    std::vector<unsigned long long> ts(2 * nslot);
+    HIPCHK(hipMemcpy(ts.data(), g_tr.d_ts, ts.size() * 8, hipMemcpyDeviceToHost));
+    const unsigned long long tick0 = ts[2 * g_tr.cap];
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g.device));
+    const double tick_ms = khz > 0 ? 1.0 / khz : 1e-5;          // 100 MHz when the query fails
+    std::map<std::array<int, 4>, std::array<double, 2>> acc;     // (kind,k,m,n) -> (ms, start)
+    auto add = [&](int kind, int k, int m, int n, double ms, double start) {
+        auto &v = acc.emplace(std::array<int, 4>{kind, k, m, n}, std::array<double, 2>{0.0, 1e300}).first->second;
+        v[0] += ms;
+        v[1] = std::min(v[1], start);
+    };
+    const double *base = g_tr.mat;
+    auto tile_of = [&](const double *p, int &r, int &c) {
+        const size_t off = (size_t)(p - base) / ((size_t)b * b);
+        r = (int)(off % MT);
+        c = (int)(off / MT);
+    };
+    for (const auto &L : g_tr.launches) {
+        const long base_slot = L.gemm ? L.g.ts_base : L.t.ts_base;
+        const int nt = L.gemm ? L.g.ntasks : L.t.ntiles;
+        for (int t = 0; t < nt; t++) {
+            const unsigned long long st = ts[2 * (base_slot + t)], en = ts[2 * (base_slot + t) + 1];
+            if (st == ~0ull || en < st) continue;
+            const double ms = (double)(en - st) * tick_ms, start = ((double)st - (double)tick0) * tick_ms;
+            if (!L.gemm) {
+                const int k = L.t.k, idx = L.t.first + t;
+                if (L.kind == TR_CHOL) add(TK_CTRSM, k, idx, -1, ms, start);
+                else if (L.kind == TR_LUR) add(TK_TRSML, k, idx, -1, ms, start);
+                else add(TK_TRSMU, k, -1, idx, ms, start);
+                continue;
+            }
+            const GemmArgs &a = L.g;
+            const int tg = a.task0 + t;
+            switch (a.mode) {
+            case GM_QR_W: case GM_QR_TW: case GM_QR_VW:
+                add(TK_TSMQR, a.k, a.m, a.k + 1 + tg, ms, start);
+                break;
+            case GM_ORM_W: case GM_ORM_TW: case GM_ORM_VW:
+                add(TK_ORMQR, a.k, -1, a.k + 1 + tg, ms, start);
+                break;
+            default: {
+                const TaskPtrs tp = decode(a, t);
+                int r, c, ar, ac;
+                tile_of(tp.C, r, c);
+                tile_of(tp.A, ar, ac);
+                if (g_tr.algo == 0) {
+                    const int k = a.mode == GM_SINGLE ? ar : a.k;    // A = tile (k, n)
+                    if (r == c) add(TK_SYRK, k, c, -1, ms, start);
+                    else add(TK_CGEMM, k, c, r, ms, start);
+                } else {
+                    const int k = a.mode == GM_SINGLE ? ac : a.k;    // A = tile (m, k)
+                    add(TK_LGEMM, k, r, c, ms, start);
+                }
+            }
+            }
+        }
+    }
+    // single-task launches in issue order: diagonal tasks k = 0.. (QR: GEQRT
+    // k), TSQRT (k, m) in (k, m) order
+    int kd = 0, kq = 0, mq = 1;
+    for (const auto &x : g_tr.singles) {
+        float ms = -1, st = 0;
+        if (hipEventElapsedTime(&ms, x.a, x.e) != hipSuccess) ms = -1;
+        if (hipEventElapsedTime(&st, g_tr.t0, x.a) != hipSuccess) st = 0;
+        if (x.cls == KC_DIAG) {
+            add(g_tr.algo == 0 ? TK_POTRF : g_tr.algo == 1 ? TK_GETRF : TK_GEQRT, kd, -1, -1, ms, st);
+            kd++;
+        } else {
+            if (mq >= MT) { kq++; mq = kq + 1; }
+            add(TK_TSQRT, kq, mq, -1, ms, st);
+            mq++;
+        }
+    }
+    std::vector<RefTask> tasks;
+    ref_tasks(g_tr.algo, MT, tasks);
+    g_tr.records.clear();
+    for (const auto &t : tasks) {
+        auto it = acc.find({t.kind, t.k, t.m, t.n});
+        const double ms = it == acc.end() ? -1.0 : it->second[0];
+        const double start = it == acc.end() ? -1.0 : it->second[1];
+        g_tr.records.push_back({(double)t.kind, (double)t.k, (double)t.m, (double)t.n, ms, start});
+    }
+    if (palial_set_task_name && palial_set_task_cpu_node && palial_get_task_time) {
+        for (size_t i = 0; i < tasks.size(); i++) {
+            const auto &r = g_tr.records[i];
+            if (r[4] < 0) continue;
+            char name[32];
+            snprintf(name, sizeof name, "%s", tasks[i].name);
+            auto tv = [](double sec) {
+                struct timeval v;
+                v.tv_sec = (time_t)sec;
+                v.tv_usec = (suseconds_t)((sec - (double)v.tv_sec) * 1e6);
+                return v;
+            };
+            const double s0 = g_tr.host_t0 + 1e-3 * r[5];
+            palial_set_task_name(name);
+            palial_set_task_cpu_node(g.device, 0, name);
+            palial_get_task_time(tv(s0), tv(s0 + 1e-3 * r[4]), name);
+        }
+    }
+    return 0;
 }
 
 // algorithmic work of the batched launches (leading terms, LAPACK counts)
@@ -1208,9 +1523,11 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
             L.chk(launch_relayout(g.d_F, b, dA, bA, N, s));
         }
         if (sigma > 0.0) L.chk(launch_add_diag(Fw, fs, N, b, dA, bA, sigma, s));
-        hipMemsetAsync(g.d_info, 0, sizeof(int), s);
+        L.note(hipMemsetAsync(g.d_info, 0, sizeof(int), s));
+        trace_begin(Fw, N, b, algo, s, L);
         if (algo == 0) dag_cholesky(Fw, N, b, MT > 1, s, g.s_panel, g.d_info, L, o.precision, fs);
         else dag_lu(Fw, N, b, MT > 1, s, g.s_panel, L, o.precision, fs);
+        trace_end();
         if (algo == 0) {
             inverses(TR_CHOL, Uinv, g.d_dinv);
         } else {
@@ -1253,12 +1570,14 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     factor_once(0.0);
     HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if ((rc = trace_collect())) return done(rc);
     while (info > 0 && fallback_ok && algo == 0 && shifts < 3) {
         sigma = sigma == 0.0 ? 4.0 * u_tier : 16.0 * sigma;
         shifts++;
         factor_once(sigma);
         HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        if ((rc = trace_collect())) return done(rc);
     }
     if (info > 0 && fallback_ok) return fallback(shifts);
     const void *sF = f32 ? (const void *)F32 : (const void *)g.d_F;
@@ -1477,7 +1796,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     // QR's two-stream schedule runs eagerly (a graph replay drops the bulk stream's CU mask)
     const bool qr_two = algo == ALG_QR && !o.no_lookahead && MT > 1;
     // the factorizations here are FP64: their chain schedule uses no CU mask (bulk_stream)
-    const bool use_graph = !o.no_graph && !g_prof.on && !qr_two;
+    const bool use_graph = !o.no_graph && !g_prof.on && !g_tr.on && !qr_two;
     if ((rc = ensure_events(events_needed(MT)))) return rc;
     if (algo == ALG_QR && (rc = ensure_ws(b, MT))) return rc;
     if ((rc = ensure_dinv(b, MT))) return rc;
@@ -1504,13 +1823,14 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
         if (hipMalloc(&dA, bytes) != hipSuccess) { cleanup(); return fail(PALIAL_ERR_NOMEM, "device alloc %zu B", bytes); }
         if (S && hipMalloc(&dS, bytes) != hipSuccess) { cleanup(); return fail(PALIAL_ERR_NOMEM, "device alloc %zu B", bytes); }
     }
-    hipEventRecord(e0, s0);
+    hipError_t he = hipEventRecord(e0, s0);
     if (!o.device_matrix) {
-        hipMemcpyAsync(dA, A.matrix, bytes, hipMemcpyHostToDevice, s0);
-        if (S) hipMemcpyAsync(dS, S->matrix, bytes, hipMemcpyHostToDevice, s0);
+        if (he == hipSuccess) he = hipMemcpyAsync(dA, A.matrix, bytes, hipMemcpyHostToDevice, s0);
+        if (S && he == hipSuccess) he = hipMemcpyAsync(dS, S->matrix, bytes, hipMemcpyHostToDevice, s0);
     }
-    hipMemsetAsync(g.d_info, 0, sizeof(int), s0);
-    hipEventRecord(e1, s0);
+    if (he == hipSuccess) he = hipMemsetAsync(g.d_info, 0, sizeof(int), s0);
+    if (he == hipSuccess) he = hipEventRecord(e1, s0);
+    if (he != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "staging before the factorization: %s", hipGetErrorString(he)); }
 
     if (qr_two && (rc = qr_streams_up())) {
         qr_streams_down();
@@ -1560,20 +1880,28 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
         }
         nodes = it->second.nodes;
     } else {
+        trace_begin(dA, N, b, algo, s0, L);
         issue(s0);
-        if (L.err != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "launch failed: %s", hipGetErrorString(L.err)); }
+        trace_end();
+        if (L.err != hipSuccess) {
+            hipStreamSynchronize(s0);
+            cleanup();
+            return fail(PALIAL_ERR_HIP, "launch failed: %s", hipGetErrorString(L.err));
+        }
         nodes = L.launches;
     }
-    hipEventRecord(e2, s0);
+    he = hipEventRecord(e2, s0);
     int info = 0;
-    hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s0);
+    if (he == hipSuccess) he = hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, s0);
     if (!o.device_matrix) {
-        hipMemcpyAsync(A.matrix, dA, bytes, hipMemcpyDeviceToHost, s0);
-        if (S) hipMemcpyAsync(S->matrix, dS, bytes, hipMemcpyDeviceToHost, s0);
+        if (he == hipSuccess) he = hipMemcpyAsync(A.matrix, dA, bytes, hipMemcpyDeviceToHost, s0);
+        if (S && he == hipSuccess) he = hipMemcpyAsync(S->matrix, dS, bytes, hipMemcpyDeviceToHost, s0);
     }
-    hipEventRecord(e3, s0);
+    if (he == hipSuccess) he = hipEventRecord(e3, s0);
     hipError_t e = hipStreamSynchronize(s0);
+    if (he != hipSuccess) e = he;
     if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "synchronize: %s", hipGetErrorString(e)); }
+    if ((rc = trace_collect())) { cleanup(); return rc; }
     float h2d = 0, fac = 0, d2h = 0;
     hipEventElapsedTime(&h2d, e0, e1);
     hipEventElapsedTime(&fac, e1, e2);
@@ -1773,6 +2101,13 @@ int palial_hip_finalize(void)
     if (g.d_bwork) hipFree(g.d_bwork);
     if (g.d_vec) hipFree(g.d_vec);
     if (g.d_aw) hipFree(g.d_aw);
+    for (auto &x : g_tr.singles) {
+        hipEventDestroy(x.a);
+        hipEventDestroy(x.e);
+    }
+    if (g_tr.t0) hipEventDestroy(g_tr.t0);
+    if (g_tr.d_ts) hipFree(g_tr.d_ts);
+    g_tr = TaskTrace();
     if (g.d_qs) hipFree(g.d_qs);
     if (g.d_info) hipFree(g.d_info);
     if (g.s_bulk && g.s_bulk != g.s_main) hipStreamDestroy(g.s_bulk);
@@ -1781,6 +2116,38 @@ int palial_hip_finalize(void)
     hipStreamDestroy(g.s_panel);
     g = Ctx();
     return 0;
+}
+
+int palial_hip_trace(int enable)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    g_tr.on = enable != 0;
+    g_tr.records.clear();
+    g_tr.algo = -1;
+    return 0;
+}
+
+long palial_hip_trace_records(double *out, long cap)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    const long n = (long)g_tr.records.size();
+    for (long i = 0; i < n && i < cap && out; i++)
+        for (int j = 0; j < 6; j++) out[6 * i + j] = g_tr.records[i][j];
+    return g_tr.overflow ? -n - 1 : n;
+}
+
+long palial_hip_task_name(int algo, int MT, long index, char *buf, int len)
+{
+    if (algo < 0 || algo > 2 || MT < 1 || MT > 4096) return fail(PALIAL_ERR_ARG, "task list of algo %d, MT %d", algo, MT);
+    std::vector<RefTask> tasks;
+    ref_tasks(algo, MT, tasks);
+    if (buf && len > 0) {
+        if (index < 0 || index >= (long)tasks.size()) return fail(PALIAL_ERR_ARG, "task %ld of %zu", index, tasks.size());
+        snprintf(buf, (size_t)len, "%s", tasks[(size_t)index].name);
+    }
+    return (long)tasks.size();
 }
 
 int palial_hip_profile(int enable)

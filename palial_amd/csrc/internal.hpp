@@ -72,7 +72,27 @@ struct GemmArgs {
     int single_diag;     // GM_SINGLE: output is a SYRK tile (upper triangle only)
     int f32c;            // packed updates: C and D are FP32 tiles at the same element offsets from mat
     double *cmat;        // GM_QA_* / GM_OA_*: the matrix Q is applied to (layout L, MT row tiles)
+    unsigned long long *ts;   // per-task tracing: timestamp slots (nullptr: not traced)
+    long ts_base;             // slot of task 0 of this launch
 };
+
+// ------------------------------------------------------------------ task trace
+// Device timestamps of the tile tasks of a batched launch (per-task tracing
+// only; ts == nullptr otherwise).  Slot s: ts[2s] = earliest start and
+// ts[2s+1] = latest end over the task's workgroups, in ticks of the GPU's
+// constant-rate wall clock.  The host fills starts with ~0 and ends with 0
+// before the factorization.
+__device__ inline void task_begin(unsigned long long *ts, long slot)
+{
+    if (ts && threadIdx.x == 0) atomicMin(ts + 2 * slot, (unsigned long long)wall_clock64());
+}
+// every thread of the workgroup calls it (it holds a barrier) after its last store
+__device__ inline void task_end(unsigned long long *ts, long slot)
+{
+    if (!ts) return;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(ts + 2 * slot + 1, (unsigned long long)wall_clock64());
+}
 
 // host launchers (k_gemm.hip)
 hipError_t launch_gemm(const GemmArgs &a, bool transA, bool transB, hipStream_t s);
@@ -102,8 +122,16 @@ struct TrsmArgs {
     int batch, ident;
     long dinv_stride;
     int f32;             // trsm2: the tiles of mat and diag are FP32 (single stays FP64)
+    unsigned long long *ts;   // per-task tracing: timestamp slots (nullptr: not traced)
+    long ts_base;             // slot of tile 0 of this launch
 };
 hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s);
+
+// Task trace, host side (api.hip): while a traced factorization is being
+// issued, register a batched launch on the factorization's matrix and give it
+// its timestamp slots (sets a.ts / a.ts_base); otherwise leave them as they are.
+void trace_gemm(GemmArgs &a);
+void trace_trsm(int kind, TrsmArgs &a);
 
 // MFMA panel kernels for b % 32 == 0, b <= 512 (k_panel2.hip).  dinv holds
 // the inverses of the 32x32 diagonal blocks of the factored tile (b/32 blocks
@@ -205,6 +233,9 @@ hipError_t launch_relayout(double *dst, int b2, const double *src, int b1, int N
 hipError_t launch_tile_split(double *sub, double *tile, int b, int bs, bool to_sub, hipStream_t s,
                              bool tile32 = false);   // tile32: the tile is FP32 (sub stays FP64)
 hipError_t launch_kron(double *mat, int N, int b, const double *A1, int n1, const double *A2, int n2, hipStream_t s);
+// task trace: fill nslots slot pairs (start ~0, end 0); write the device wall clock to *dst
+hipError_t launch_trace_init(unsigned long long *ts, size_t nslots, hipStream_t s);
+hipError_t launch_clock_stamp(unsigned long long *dst, hipStream_t s);
 // in-place A <- A^T (avg false) or A <- (A + A^T) / 2 (avg true) of a layout-L matrix, b % 32 == 0
 hipError_t launch_transpose(double *mat, int N, int b, bool avg, hipStream_t s);
 // N x nrhs column-major block (leading dimension ld) <-> N x b layout-L block

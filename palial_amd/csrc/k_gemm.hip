@@ -49,6 +49,7 @@ __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
     TaskPtrs tp = decode(p, task);
     if (!tp.valid) return;
     if (tp.diag && bi > bj) return;     // strictly below the diagonal of a SYRK tile
+    task_begin(p.ts, p.ts_base + task);
 
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = (int)threadIdx.x;
@@ -242,6 +243,7 @@ __global__ __launch_bounds__(2 * BM, MINB) void gemm_mfma_kernel(GemmArgs p)
                     tp.E[(size_t)i + (size_t)j * b] -= alpha * acc[x][c][v];
                 }
     }
+    task_end(p.ts, p.ts_base + task);
 }
 
 // Scalar fallback for tile sizes that are not a multiple of 32 (tiny tests,
@@ -357,16 +359,20 @@ static int gemm_variant()
     return v;
 }
 
-hipError_t launch_gemm_fine(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
+hipError_t launch_gemm_fine(const GemmArgs &a_in, bool ta, bool tb, hipStream_t s)
 {
-    if (a.ntasks <= 0) return hipSuccess;
-    if (a.b % 32 != 0) return launch_gemm(a, ta, tb, s);
+    if (a_in.ntasks <= 0) return hipSuccess;
+    if (a_in.b % 32 != 0) return launch_gemm(a_in, ta, tb, s);
+    GemmArgs a = a_in;
+    trace_gemm(a);
     return launch_mfma<32, 32, 1>(a, ta, tb, s);      // K slices of 32: half the barriers
 }
 
-hipError_t launch_gemm(const GemmArgs &a, bool ta, bool tb, hipStream_t s)
+hipError_t launch_gemm(const GemmArgs &a_in, bool ta, bool tb, hipStream_t s)
 {
-    if (a.ntasks <= 0) return hipSuccess;
+    if (a_in.ntasks <= 0) return hipSuccess;
+    GemmArgs a = a_in;
+    if (a.b % 32 == 0) trace_gemm(a);     // the scalar fallback below keeps no timestamps
     if (a.b % 128 == 0) {
         // 2 waves per SIMD (<= 256 registers per lane, LDS 2 x 73.7 KB per CU)
         switch (gemm_variant()) {

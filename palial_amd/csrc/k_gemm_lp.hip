@@ -95,6 +95,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_kernel(GemmArgs p)
     TaskPtrs tp = decode(p, task);
     if (!tp.valid) return;
     if (tp.diag && bi > bj) return;
+    task_begin(p.ts, p.ts_base + task);
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
@@ -180,6 +181,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_kernel(GemmArgs p)
     }
 
     lp_epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
+    task_end(p.ts, p.ts_base + task);
 }
 
 // ------------------------------------------------------------------ BF16
@@ -203,6 +205,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(GemmArgs p)
     TaskPtrs tp = decode(p, task);
     if (!tp.valid) return;
     if (tp.diag && bi > bj) return;
+    task_begin(p.ts, p.ts_base + task);
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
@@ -303,11 +306,14 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(GemmArgs p)
     }
 
     lp_epilogue<FM, WM>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta);
+    task_end(p.ts, p.ts_base + task);
 }
 
-hipError_t launch_gemm_lp(const GemmArgs &a, bool ta, bool tb, int prec, hipStream_t s)
+hipError_t launch_gemm_lp(const GemmArgs &a_in, bool ta, bool tb, int prec, hipStream_t s)
 {
-    if (a.ntasks <= 0) return hipSuccess;
+    if (a_in.ntasks <= 0) return hipSuccess;
+    GemmArgs a = a_in;
+    trace_gemm(a);
     if (a.b % 128 != 0 || a.maskA != MK_NONE) return hipErrorInvalidValue;
     const long nbd = a.b / 128;
     dim3 grid((unsigned)((long)a.ntasks * nbd * nbd)), block(256);

@@ -280,6 +280,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16p_kernel(GemmArgs p)
     const TaskPtrs tp = decode(p, task);
     if (!tp.valid) return;
     if (tp.diag && bi > bj) return;
+    task_begin(p.ts, p.ts_base + task);
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
@@ -372,6 +373,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16p_kernel(GemmArgs p)
     stC(1, d1);
     stC(2, d2);
     stC(3, d0);
+    task_end(p.ts, p.ts_base + task);
 }
 
 // BF16 update with asynchronous global -> LDS loads (global_load_lds_dwordx4,
@@ -425,6 +427,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
     const TaskPtrs tp = decode(p, task);
     if (!tp.valid) return;
     if (tp.diag && bi > bj) return;
+    task_begin(p.ts, p.ts_base + task);
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
@@ -570,6 +573,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
             }
             __syncthreads();
         }
+        task_end(p.ts, p.ts_base + task);
         return;
     }
     // acc[x][c][v] = C(i0 + wm*64 + 16x + l%16, j0 + wn*64 + 16c + 4(l/16) + v);
@@ -606,6 +610,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
     ldD(3, d1);
     stC(2, d0);
     stC(3, d1);
+    task_end(p.ts, p.ts_base + task);
 }
 
 // FP32 operands, v_mfma_f32_16x16x4_f32; same blocking with KB = 32.
@@ -628,6 +633,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f32p_kernel(GemmArgs p)
     const TaskPtrs tp = decode(p, task);
     if (!tp.valid) return;
     if (tp.diag && bi > bj) return;
+    task_begin(p.ts, p.ts_base + task);
     const int i0 = bi * BM, j0 = bj * BM;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
@@ -683,6 +689,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f32p_kernel(GemmArgs p)
     }
 
     epilogue<FM, WM, F32C>(tp, acc, i0, j0, wm, wn, lane, b, p.alpha, p.beta, p.mat);
+    task_end(p.ts, p.ts_base + task);
 }
 
 }  // namespace
@@ -713,9 +720,11 @@ hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n,
     return hipGetLastError();
 }
 
-hipError_t launch_gemm_packed(const GemmArgs &a, int prec, hipStream_t s)
+hipError_t launch_gemm_packed(const GemmArgs &a_in, int prec, hipStream_t s)
 {
-    if (a.ntasks <= 0) return hipSuccess;
+    if (a_in.ntasks <= 0) return hipSuccess;
+    GemmArgs a = a_in;
+    trace_gemm(a);
     if (a.b % 128 != 0 || a.maskA != MK_NONE || !a.pa || !a.pb) return hipErrorInvalidValue;
     if (a.mode != GM_CHOL_TRAIL && a.mode != GM_CHOL_REST && a.mode != GM_CHOL_ROW && a.mode != GM_LU_TRAIL &&
         a.mode != GM_LU_REST && a.mode != GM_LU_PANEL && a.mode != GM_CHOL_FIRST && a.mode != GM_LU_FIRST)

@@ -187,6 +187,32 @@ hipError_t launch_tiles_to_cols(const double *tiles, int N, int b, double *v, in
     return hipGetLastError();
 }
 
+// task-trace slots: starts ~0 (atomicMin targets), ends 0 (atomicMax targets)
+__global__ void trace_init_kernel(unsigned long long *ts, size_t nslots)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (size_t)gridDim.x * blockDim.x) {
+        ts[2 * i] = ~0ull;
+        ts[2 * i + 1] = 0ull;
+    }
+}
+
+__global__ void clock_stamp_kernel(unsigned long long *dst)
+{
+    if (threadIdx.x == 0) *dst = (unsigned long long)wall_clock64();
+}
+
+hipError_t launch_trace_init(unsigned long long *ts, size_t nslots, hipStream_t s)
+{
+    hipLaunchKernelGGL(trace_init_kernel, dim3(1024), dim3(256), 0, s, ts, nslots);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock_stamp(unsigned long long *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(clock_stamp_kernel, dim3(1), dim3(64), 0, s, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, double shift, hipStream_t s)
 {
     const size_t total = (size_t)N * N;

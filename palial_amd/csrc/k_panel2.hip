@@ -652,6 +652,7 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
     } else {
         X = p.single;
     }
+    task_begin(p.ts, p.ts_base + tix);
     auto ldx = [&](size_t i) -> double { return (F32 && X32) ? (double)X32[i] : X[i]; };
     auto stx = [&](size_t i, double v) {
         if (F32 && X32) X32[i] = (DT)v;
@@ -759,6 +760,7 @@ __global__ __launch_bounds__(P2_THREADS) void trsm2_kernel(TrsmArgs p, const dou
         if (KIND == TR_LUR) stx((size_t)(c0 + e % SW) + (size_t)(e / SW) * b, sX[xidx(e / SW, e % SW)]);
         else stx((size_t)(e % b) + (size_t)(c0 + e / b) * b, sX[xidx(e % b, e / b)]);
     }
+    task_end(p.ts, p.ts_base + tix);
 }
 
 // FP32 variant for the mixed-precision tiers' FP32 working copy: the strip
@@ -799,6 +801,7 @@ __global__ __launch_bounds__(P2_THREADS, 2) void trsm2f_kernel(TrsmArgs p, const
     } else {
         Xd = p.single;
     }
+    task_begin(p.ts, p.ts_base + tix);
     for (int e0 = t; e0 < b * SW; e0 += P2_THREADS * 8) {
         float v[8];
 #pragma unroll
@@ -896,6 +899,7 @@ __global__ __launch_bounds__(P2_THREADS, 2) void trsm2f_kernel(TrsmArgs p, const
         if (X32) X32[o] = v;
         else Xd[o] = (double)v;
     }
+    task_end(p.ts, p.ts_base + tix);
 }
 
 // ---------------------------------------------------------------- R blocks
@@ -988,9 +992,11 @@ hipError_t launch_batch(const BatchArgs &a, int bmax, int grid, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_trsm2(int kind, const TrsmArgs &a, const double *dinv, hipStream_t s)
+hipError_t launch_trsm2(int kind, const TrsmArgs &a_in, const double *dinv, hipStream_t s)
 {
-    if (a.ntiles <= 0) return hipSuccess;
+    if (a_in.ntiles <= 0) return hipSuccess;
+    TrsmArgs a = a_in;
+    trace_trsm(kind, a);
     dim3 grid((unsigned)(a.ntiles * (a.b / NB2))), block(P2_THREADS);
     const size_t lds = (size_t)a.b * NB2 * 8;
     if (a.f32) {

@@ -16,6 +16,9 @@
  *   -t        print "algorithm, N, B, factor_ms, TFLOP/s" after the run
  *   -v        check the result: Cholesky/LU solve one right-hand side with
  *             the factorization tier and print the backward error
+ *   -T        trace the factorization per task: a loaded tracer that defines
+ *             the reference's hooks (include/common.h:9-11) receives each
+ *             task's name and device time
  * Library errors go to stderr with exit status 1.
  */
 #include "palial_hip.h"
@@ -34,7 +37,7 @@ static int fail(int rc, const char *what)
 int main(int argc, char *argv[])
 {
     char algorithm[16] = "";
-    int N = 0, b = 0, timing = 0, verify = 0;
+    int N = 0, b = 0, timing = 0, verify = 0, trace = 0;
     unsigned long long seed = 0;
     int gen = PALIAL_GEN_PALIAL;
     static struct option longopts[] = {
@@ -49,7 +52,7 @@ int main(int argc, char *argv[])
         return 0;
     }
     int c;
-    while ((c = getopt_long(argc, argv, "a:m:b:s:g:tvh", longopts, NULL)) != -1) {
+    while ((c = getopt_long(argc, argv, "a:m:b:s:g:tvTh", longopts, NULL)) != -1) {
         switch (c) {
         case 'a': snprintf(algorithm, sizeof algorithm, "%s", optarg); break;
         case 'm': N = atoi(optarg); break;
@@ -62,6 +65,7 @@ int main(int argc, char *argv[])
             break;
         case 't': timing = 1; break;
         case 'v': verify = 1; break;
+        case 'T': trace = 1; break;
         case 'h': printf("HELP\n"); return 0;
         default:
             printf("Invalid arguments.\n");
@@ -80,6 +84,7 @@ int main(int argc, char *argv[])
 
     int rc = palial_hip_init(0);
     if (rc) return fail(rc, "init");
+    if (trace && (rc = palial_hip_trace(1))) return fail(rc, "trace");
     const size_t n2 = (size_t)N * N;
     double *a = NULL, *s = NULL;
     if (posix_memalign((void **)&a, 4096, n2 * sizeof(double))) {

@@ -9,22 +9,24 @@ m = MT, n = MT - 1 that the previous step's loops left behind, and syrk
 carries the n of the previous column's gemm loop).  LU names follow
 src/lu.h:13-173 and QR names src/qr.h:17-185 the same way; their k = 0 tasks
 print loop variables the reference never initialises (src/lu.h:13,
-src/qr.h:17), printed here as 0.  A QR update's three GEMM phases are split
-evenly over its n > k tiles.
-The Cholesky names are pinned to the reference's recorded trace; the
-reference holds no recorded LU or QR trace.
+src/qr.h:17), printed here as 0.  The Cholesky names are pinned to the
+reference's recorded trace; the reference holds no recorded LU or QR trace.
+The library builds the same list (palial_hip_task_name) for the reference's
+trace hooks.
 
-Times come from the library's per-launch records (profile(2) before a
-factorization or solve, then profile_records()).  A launch of the chain
-schedule covers one kind of tile task of one step; its time is split over its
-tasks in proportion to their flops (potrf b^3/3, trsm and syrk b^3, gemm
-2 b^3).  Packing launches have no reference task and are left out, so the
-lines add up to the launches' time without them.  Launches are matched to the
-schedule's enumeration (costmodel.launches) group by group: within one
-(class, flops, bytes) shape both lists run in step order."""
+Times are measured on the device per task (palial_hip_trace): a task inside a
+batched launch from the timestamps of its first and last workgroup, a
+diagonal-tile or QR panel task from HIP events around its launch; a QR
+update's time is the sum over its three launches (W = V^T C, W = T^T W,
+C -= V W).  `dag_edges` gives the task DAG the reference's tracer records from
+OpenMP dependences (palial_trace/palial_trace.c:131-202), derived here from
+the tiles each task reads and writes (the reference's LU names wrong tiles in
+two depend clauses, src/lu.h:62,104; the edges here are the correct ones)."""
 import numpy as np
 
-from . import costmodel as cm
+# library task kinds (palial_hip_task_name) <-> the tuples of this module
+KINDS = {0: "potrf", 1: "trsm", 2: "syrk", 3: "gemm", 4: "getrf", 5: "trsmL", 6: "trsmU", 7: "gemm",
+         8: "geqrt", 9: "ormqr", 10: "tsqrt", 11: "tsmqr"}
 
 
 def cholesky_tasks(MT):
@@ -71,27 +73,6 @@ def lu_tasks(MT):
     return out
 
 
-def _covered_lu(k, stream, cls, MT, split, idx):
-    """Reference tasks of one enumerated launch of the LU chain schedule; idx:
-    0 for the first launch of its (step, stream, class), 1 for the second (the
-    row panel's TRSM is issued before the column panel's)."""
-    if cls == cm.CLASSES["diag"]:
-        return [("getrf", k)]
-    if cls == cm.CLASSES["trsm"]:
-        tiles = [k + 1] if stream == "chain" else list(range(k + 2, MT))
-        return [("trsmU", k, n) for n in tiles] if idx == 0 else [("trsmL", k, m) for m in tiles]
-    if cls == cm.CLASSES["pack"]:
-        return []
-    if stream == "chain":                              # tile (k+1, k+1)
-        return [("gemm", k, k + 1, k + 1)]
-    if not split:
-        return [("gemm", k, m, n) for n in range(k + 1, MT) for m in range(k + 1, MT) if (m, n) != (k + 1, k + 1)]
-    if stream == "first":                              # row and column k+1, and tile (k+2, k+2)
-        return [("gemm", k, k + 1, n) for n in range(k + 2, MT)] + \
-               [("gemm", k, m, k + 1) for m in range(k + 2, MT)] + [("gemm", k, k + 2, k + 2)]
-    return [("gemm", k, m, n) for n in range(k + 2, MT) for m in range(k + 2, MT) if (m, n) != (k + 2, k + 2)]
-
-
 def qr_tasks(MT):
     """[(name, task)] in creation order (src/qr.h:17-185): ("geqrt", k),
     ("ormqr", k, n), ("tsqrt", k, m), ("tsmqr", k, m, n)."""
@@ -114,158 +95,160 @@ def qr_tasks(MT):
     return out
 
 
-def qr_launches(N, b):
-    """The records of one QR factorization in issue order (dag_qr,
-    palial_amd/csrc/api.hip): (group, class, flops, bytes); group is a task
-    for geqrt and tsqrt, ("ormqr", k) or ("tsmqr", k, m) for the three GEMM
-    phases of an update (W = V^T C, W = T^T W, C -= V W over every n > k)."""
-    MT = N // b
-    b3, tb = float(b) ** 3, 8.0 * b * b
-    D, Q, U = cm.CLASSES["diag"], cm.CLASSES["tsqrt"], cm.CLASSES["qrupd"]
+def tasks(algo, MT):
+    return {"cholesky": cholesky_tasks, "lu": lu_tasks, "qr": qr_tasks}[algo](MT)
+
+
+def _key(task):
+    """(kind code, k, m, n) of a task tuple, as the library numbers them."""
+    kind = task[0]
+    if kind in ("potrf", "getrf", "geqrt"):
+        return ({"potrf": 0, "getrf": 4, "geqrt": 8}[kind], task[1], -1, -1)
+    if kind in ("trsm", "syrk", "trsmL", "tsqrt"):
+        return ({"trsm": 1, "syrk": 2, "trsmL": 5, "tsqrt": 10}[kind], task[1], task[2], -1)
+    if kind in ("trsmU", "ormqr"):
+        return ({"trsmU": 6, "ormqr": 9}[kind], task[1], -1, task[2])
+    return (None, task[1], task[2], task[3])      # gemm / tsmqr: the code depends on the algorithm
+
+
+def trace_lines(algo, records, N, b):
+    """The reference tracer's `name, ms` lines from the library's per-task
+    records of one factorization (palial_amd.trace_records())."""
+    rec = np.asarray(records, dtype=np.float64).reshape(-1, 6)
+    tl = tasks(algo, N // b)
+    if len(rec) != len(tl):
+        raise ValueError(f"{len(rec)} records for {len(tl)} {algo} tasks of N={N}, b={b}")
+    gemm_code = {"cholesky": 3, "lu": 7, "qr": 11}[algo]
     out = []
-    for k in range(MT):
-        out.append((("geqrt", k), D, 4 * b3 / 3, 3 * tb))
-        T = MT - k - 1
-        if T == 0:
-            continue
-        out += [(("ormqr", k), U, T * b3, 3 * T * tb)] * 3
-        for m in range(k + 1, MT):
-            out.append((("tsqrt", k, m), Q, 2 * b3, 5 * tb))
-            out += [(("tsmqr", k, m), U, 2 * T * b3, 4 * T * tb), (("tsmqr", k, m), U, T * b3, 4 * T * tb),
-                    (("tsmqr", k, m), U, 2 * T * b3, 3 * T * tb)]
+    for (name, task), r in zip(tl, rec):
+        code, k, m, n = _key(task)
+        code = gemm_code if code is None else code
+        if (code, k, m, n) != tuple(int(x) for x in r[:4]):
+            raise ValueError(f"record {r[:4]} is not task {name}")
+        out.append(f"{name}, {r[4]:f}")
     return out
 
 
-def qr_task_times(records, N, b):
-    """{task: ms} for one QR factorization: geqrt and tsqrt records are tasks;
-    an update's three phase records are split evenly over its n > k tiles."""
-    MT = N // b
-    enum = qr_launches(N, b)
-    ms_of = _match(records, [(c, f, by) for _, c, f, by in enum], f"qr schedule of N={N}, b={b}")
-    times = {}
-    for (grp, _, _, _), ms in zip(enum, ms_of):
-        if grp[0] in ("geqrt", "tsqrt"):
-            tasks = [grp]
+def tiles_of(algo, task):
+    """(read tiles, written tiles) of a reference task, tiles as (row, col)."""
+    kind, k = task[0], task[1]
+    if algo == "cholesky":
+        if kind == "potrf":
+            return [], [(k, k)]
+        if kind == "trsm":
+            return [(k, k)], [(k, task[2])]
+        if kind == "syrk":
+            return [(k, task[2])], [(task[2], task[2])]
+        m, n = task[2], task[3]                       # A(n,m) -= A(k,n)^T A(k,m)
+        return [(k, n), (k, m)], [(n, m)]
+    if algo == "lu":
+        if kind == "getrf":
+            return [], [(k, k)]
+        if kind == "trsmL":
+            return [(k, k)], [(task[2], k)]
+        if kind == "trsmU":
+            return [(k, k)], [(k, task[2])]
+        m, n = task[2], task[3]                       # A(m,n) -= A(m,k) A(k,n)
+        return [(m, k), (k, n)], [(m, n)]
+    # QR: V / T tiles are read, R / A tiles written; S(m,k) carries T with A(m,k)
+    if kind == "geqrt":
+        return [], [(k, k)]
+    if kind == "ormqr":
+        return [(k, k)], [(k, task[2])]
+    if kind == "tsqrt":
+        return [], [(k, k), (task[2], k)]
+    m, n = task[2], task[3]                           # rows k and m of column n, V = A(m,k)
+    return [(m, k)], [(k, n), (m, n)]
+
+
+def dag_edges(algo, MT):
+    """Dependences of the task DAG in the sequential semantics: for every task
+    (creation index i) the earlier tasks it must follow -- the last writer of
+    each tile it reads or writes, and the readers of a tile since its last
+    write before it writes that tile.  Returns {i: sorted list of j < i}."""
+    tl = tasks(algo, MT)
+    last_w, readers = {}, {}
+    edges = {}
+    for i, (_, t) in enumerate(tl):
+        rd, wr = tiles_of(algo, t)
+        dep = set()
+        for x in rd + wr:
+            if x in last_w:
+                dep.add(last_w[x])
+        for x in wr:
+            dep.update(readers.get(x, ()))
+        # a QR task reading A(m,k) written by its own step's tsqrt is covered
+        # above; drop self-references
+        dep.discard(i)
+        edges[i] = sorted(dep)
+        for x in rd:
+            readers.setdefault(x, set()).add(i)
+        for x in wr:
+            last_w[x] = i
+            readers[x] = set()
+    return edges
+
+
+def measure(algo, N, b, seed=0, precision=0):
+    """One traced factorization on the GPU of the generated input (S + N I for
+    Cholesky, G + N I for LU, G for QR; precision > 0: a Cholesky/LU solve in
+    that tier, whose factorization is traced).  Returns the library's records."""
+    from . import (init, DeviceMatrix, GEN_SYM, GEN_DENSE, factor_device, solve, trace_enable,
+                   trace_records)
+    init(0)
+    A = DeviceMatrix(N, b)
+    S = DeviceMatrix(N, b) if algo == "qr" else None
+    try:
+        A.generate(GEN_SYM if algo == "cholesky" else GEN_DENSE, seed=seed,
+                   shift=0.0 if algo == "qr" else float(N))
+        trace_enable(True)
+        if precision == 0:
+            info, _ = factor_device(algo, A, S=S)
+            if info != 0:
+                raise RuntimeError(f"factorization failed (info {info})")
         else:
-            tasks = [grp[:2] + (n,) if grp[0] == "ormqr" else grp + (n,) for n in range(grp[1] + 1, MT)]
-        for t in tasks:
-            times[t] = times.get(t, 0.0) + ms / len(tasks)
-    return times
-
-
-def _match(records, enum, what):
-    """Recorded ms per enumerated (class, flops, bytes), matched group by group in order."""
-    rec = np.asarray(records, dtype=np.float64).reshape(-1, 6)
-    key = lambda c, f, by: (int(c), int(round(f)), int(round(by)))
-    groups = {}
-    for i, (c, f, by) in enumerate(enum):
-        groups.setdefault(key(c, f, by), []).append(i)
-    seen = {g: 0 for g in groups}
-    ms_of = [None] * len(enum)
-    for c, _, _, f, by, ms in rec:
-        g = key(c, f, by)
-        if g not in groups or seen[g] >= len(groups[g]):
-            raise ValueError(f"record {g} has no launch left in the {what}")
-        ms_of[groups[g][seen[g]]] = ms
-        seen[g] += 1
-    if any(t is None for t in ms_of):
-        raise ValueError("records cover only part of the schedule: pass the records of one factorization")
-    return ms_of
-
-
-def _covered(k, stream, cls, MT, split):
-    """Reference tasks of one enumerated launch of the Cholesky chain schedule."""
-    if cls == cm.CLASSES["diag"]:
-        return [("potrf", k)]
-    if cls == cm.CLASSES["trsm"]:
-        return [("trsm", k, k + 1)] if stream == "chain" else [("trsm", k, m) for m in range(k + 2, MT)]
-    if cls == cm.CLASSES["pack"]:
-        return []
-    if stream == "chain":                              # tile (k+1, k+1)
-        return [("syrk", k, k + 1)]
-    if not split:                                      # the rest of step k's update in one launch
-        return [("syrk", k, m) for m in range(k + 2, MT)] + \
-               [("gemm", k, m, n) for m in range(k + 2, MT) for n in range(k + 1, m)]
-    if stream == "first":                              # tile row k+1 and tile (k+2, k+2)
-        return [("gemm", k, m, k + 1) for m in range(k + 2, MT)] + [("syrk", k, k + 2)]
-    return [("syrk", k, m) for m in range(k + 3, MT)] + \
-           [("gemm", k, m, n) for m in range(k + 3, MT) for n in range(k + 2, m)]
-
-
-def _flops(task, b):
-    b3 = float(b) ** 3
-    return {"potrf": b3 / 3, "getrf": 2 * b3 / 3, "trsm": b3, "trsmL": b3, "trsmU": b3, "syrk": b3,
-            "gemm": 2 * b3}[task[0]]
-
-
-def task_times(algo, records, N, b, prec=0, split=True):
-    """{task: ms} from the per-launch records of one factorization (the
-    records of exactly one factorization, in issue order)."""
-    if algo == "qr":
-        return qr_task_times(records, N, b)
-    MT = N // b
-    enum = cm.launches(algo, N, b, prec, split=split)
-    ms_of = _match(records, [(c, f, by) for _, _, c, f, by in enum], f"{algo} schedule of N={N}, b={b}")
-    times = {}
-    occ = {}
-    for (k, st, c, f, by), ms in zip(enum, ms_of):
-        idx = occ.get((k, st, c), 0)
-        occ[(k, st, c)] = idx + 1
-        tasks = _covered(k, st, c, MT, split) if algo == "cholesky" else _covered_lu(k, st, c, MT, split, idx)
-        tot = sum(_flops(t, b) for t in tasks)
-        for t in tasks:
-            times[t] = times.get(t, 0.0) + ms * _flops(t, b) / tot
-    return times
-
-
-def cholesky_task_times(records, N, b, prec=0, split=True):
-    return task_times("cholesky", records, N, b, prec, split)
-
-
-def trace_lines(algo, records, N, b, prec=0, split=True):
-    """The reference tracer's `name, ms` lines for one factorization."""
-    times = task_times(algo, records, N, b, prec, split)
-    tasks = {"cholesky": cholesky_tasks, "lu": lu_tasks, "qr": qr_tasks}[algo](N // b)
-    return [f"{name}, {times[task]:f}" for name, task in tasks]
-
-
-def cholesky_trace_lines(records, N, b, prec=0, split=True):
-    return trace_lines("cholesky", records, N, b, prec, split)
+            import ctypes as C
+            from . import lib, _check
+            rhs = np.random.default_rng(seed).standard_normal(N)
+            dX = C.c_void_p()
+            _check(lib().palial_hip_malloc(C.byref(dX), N * 8))
+            _check(lib().palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * 8))
+            _, rep, rc = solve(A, (dX, 1, N), algo=algo, precision=precision, device=True, factor_tile=b,
+                               check=False)
+            lib().palial_hip_free(dX)
+            if rc != 0:
+                raise RuntimeError(f"solve failed ({rc})")
+        return trace_records()
+    finally:
+        trace_enable(False)
+        A.free()
+        if S is not None:
+            S.free()
 
 
 def main(argv=None):
-    """python -m palial_amd.trace [-a cholesky|lu|qr] -m N -b B: one profiled
-    FP64 factorization of a generated matrix (S + N I; G + N I for LU; G for
-    QR) on the GPU, printed as the reference tracer prints its run."""
+    """python -m palial_amd.trace [-a cholesky|lu|qr] -m N -b B [--edges]: one
+    traced FP64 factorization of a generated matrix (S + N I; G + N I for LU;
+    G for QR) on the GPU, printed as the reference tracer prints its run;
+    --edges prints each task's dependences instead."""
     import argparse
-    from . import init, DeviceMatrix, GEN_SYM, GEN_DENSE, profile, profile_records, factor_device
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("-a", choices=("cholesky", "lu", "qr"), default="cholesky")
     ap.add_argument("-m", type=int, default=8192, help="matrix size N")
     ap.add_argument("-b", type=int, default=512, help="tile size")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precision", type=int, default=0, help="tier of a traced solve (0: plain FP64 factorization)")
+    ap.add_argument("--edges", action="store_true", help="print the task DAG's dependences")
     args = ap.parse_args(argv)
     if args.m % args.b:
         ap.error("the tile size must divide N")
-    init(0)
-    A = DeviceMatrix(args.m, args.b)
-    A.generate(GEN_SYM if args.a == "cholesky" else GEN_DENSE, seed=args.seed,
-               shift=0.0 if args.a == "qr" else float(args.m))
-    S = DeviceMatrix(args.m, args.b) if args.a == "qr" else None
-    profile(2)
-    info, _ = factor_device(args.a, A, S=S)
-    rec = profile_records()
-    profile(0)
-    A.free()
-    if S is not None:
-        S.free()
-    if info != 0:
-        raise SystemExit(f"factorization failed (info {info})")
-    try:
-        lines = trace_lines(args.a, rec, args.m, args.b)
-    except ValueError:                                  # the schedule without the split update
-        lines = trace_lines(args.a, rec, args.m, args.b, split=False)
-    print("\n".join(lines))
+    if args.edges:
+        tl = tasks(args.a, args.m // args.b)
+        for i, deps in dag_edges(args.a, args.m // args.b).items():
+            print(f"{tl[i][0]}: " + " ".join(tl[j][0] for j in deps))
+        return 0
+    rec = measure(args.a, args.m, args.b, seed=args.seed, precision=args.precision)
+    print("\n".join(trace_lines(args.a, rec, args.m, args.b)))
     return 0
 
 
