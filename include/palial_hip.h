@@ -72,7 +72,14 @@ typedef struct palial_opts {
     int   no_fallback;    /* solves, mixed tiers: nonzero reports a breakdown or a refinement that
                              does not converge as an error; zero (default) retries a broken-down
                              Cholesky with a diagonal shift (then GMRES-IR) and finally solves in FP64 */
-    int   reserved[6];
+    int   schedule;       /* factorization calls, task priority / look-ahead of the tile DAG (0 = default):
+                             Cholesky, LU: 1 diagonal chain on its own stream with the trailing update
+                             split in two launches per step, 2 diagonal chain with one update launch,
+                             3 row look-ahead, 4 one stream in the reference's order;
+                             QR: 1..8 panel chains in flight (steps progressing together) */
+    int   batch_grid;     /* palial_hip_batched: workgroups of the persistent launch (0 = as many as
+                             the CUs hold at once) */
+    int   reserved[4];
 } palial_opts;
 
 /* per-call report; any pointer may be NULL */

@@ -44,7 +44,7 @@ class Opts(C.Structure):
     _fields_ = [("device_matrix", C.c_int), ("precision", C.c_int), ("no_graph", C.c_int),
                 ("no_lookahead", C.c_int), ("stream", C.c_void_p), ("refine", C.c_int),
                 ("max_iter", C.c_int), ("tol", C.c_double), ("factor_tile", C.c_int), ("no_fallback", C.c_int),
-                ("reserved", C.c_int * 6)]
+                ("schedule", C.c_int), ("batch_grid", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class Report(C.Structure):
@@ -155,8 +155,10 @@ def finalize():
 
 
 def _opts(device=False, no_graph=False, no_lookahead=False, precision=PREC_FP64, stream=None,
-          refine=0, max_iter=0, tol=0.0, factor_tile=0, no_fallback=False):
+          refine=0, max_iter=0, tol=0.0, factor_tile=0, no_fallback=False, schedule=0, batch_grid=0):
     o = Opts()
+    o.schedule = schedule
+    o.batch_grid = batch_grid
     o.factor_tile = factor_tile
     o.no_fallback = int(bool(no_fallback))
     o.device_matrix = int(bool(device))
@@ -483,11 +485,11 @@ class Batch:
         vecs = [b[o:o + m] for o, m in zip(self.boff, self.n)]
         return mats, vecs
 
-    def factor(self, solve=True, no_graph=False):
+    def factor(self, solve=True, no_graph=False, grid=0):
         """Factor every matrix in place (and solve its right-hand side); returns the
-        report.  One persistent launch, replayed from a captured hipGraph unless
-        no_graph."""
-        o = _opts(device=True, no_graph=no_graph)
+        report.  One persistent launch of `grid` workgroups (0: what the CUs
+        hold), replayed from a captured hipGraph unless no_graph."""
+        o = _opts(device=True, no_graph=no_graph, batch_grid=grid)
         rep = Report()
         _check(lib().palial_hip_batched(self.count, self.n.ctypes.data, self.off.ctypes.data,
                                         self.kind.ctypes.data, self.A, self.B if solve else None,

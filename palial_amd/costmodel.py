@@ -1,39 +1,138 @@
 """Task-time cost model: a 3-layer perceptron (64 hidden units, FP32) whose
 forward pass and training run in the library's own HIP kernels
-(palial_hip_mlp_forward / palial_hip_mlp_train), trained on per-launch device
-times recorded by palial_hip_profile(2).
+(palial_hip_mlp_forward / palial_hip_mlp_train), trained on the per-task
+device times that traced factorizations record (palial_hip_trace: the
+reference's `name, ms` task times, measured on the GPU).
 
-A launch is described by its kernel class, tile size, precision tier and
-algorithmic flops and bytes; the model predicts log(ms).  `launches()`
-enumerates the launches a factorization issues (the same batched launches
-the library records), split into the diagonal chain and the bulk stream, so
-a schedule's time can be predicted as the sum over steps of the slower of
-the two."""
+One sample is one reference tile task of one factorization.  Features
+(NFEAT = 16): the task's kernel type (6: diagonal factorization, TRSM, SYRK,
+GEMM, TSQRT, QR block-reflector update), log2 of the tile size, the precision
+tier (5), log2 N, the task's DAG depth k/MT, log2 of the number of tasks in
+its batched launch (its concurrency), and whether it lies on the critical
+path (the diagonal chain).  The model predicts log(ms); a task's time is
+taken to be the same under every schedule, which enters through the replay
+below.
+
+`predict_ms` turns per-task predictions into a factorization time by
+replaying the library's schedule for that configuration: the streams of the
+diagonal-chain schedule (or the QR panel chains and update streams) with
+their cross-stream waits, each batched launch lasting as long as its tasks'
+workgroups take on the CUs (sum over tasks of time x workgroups / resident
+workgroup slots, and at least its longest task), the makespan bounded below
+by the total work.  A per-algorithm scale is fitted on the measured
+configurations (`calibrate`)."""
 import ctypes as C
 import math
 
 import numpy as np
 
-NCLASS = 7                 # kernel classes of palial_hip_profile_stats
-NFEAT = NCLASS + 3 + 5     # class one-hot, log2 b, log10 flops, log10 bytes, tier one-hot
 HID = 64
-CLASSES = {"gemm": 0, "diag": 1, "trsm": 2, "tsqrt": 3, "qrupd": 4, "pack": 5, "pkgemm": 6}
+KTYPES = ("diag", "trsm", "syrk", "gemm", "tsqrt", "qrupd")
+NTYPE = len(KTYPES)
+NFEAT = NTYPE + 1 + 5 + 4          # 16: type one-hot, log2 b, tier one-hot, log2 N, depth, log2 concurrency, chain
+# library task kinds (palial_hip_task_name) -> kernel type
+KIND_TYPE = {0: 0, 1: 1, 2: 2, 3: 3, 4: 0, 5: 1, 6: 1, 7: 3, 8: 0, 9: 5, 10: 4, 11: 5}
+ALGO_CODE = {"cholesky": 0, "lu": 1, "qr": 2}
+NCU = 256                           # MI355X compute units
+LAUNCH_MS = 0.005                   # gap between dependent launches on one stream
 
 
-def featurize(records):
-    """records: (n, 6) array of (class, b, tier, flops, bytes, ms) -> (X float32 (n, NFEAT), y = log ms)."""
-    r = np.asarray(records, dtype=np.float64).reshape(-1, 6)
+def launch_tasks(algo, MT, kind, k, m, n):
+    """Tasks in the batched launch of the default schedule that covers each
+    task (arrays of kind, k, m, n; dag_cholesky_chain / dag_lu_chain /
+    dag_qr): 1 for the chain's one-tile launches, the rest of the panel for
+    the row TRSM, the tile row (and column) k+1 plus tile (k+2,k+2) for the
+    first update launch, every other trailing tile for the second, T = MT-k-1
+    for the QR updates."""
+    kind, k, m, n = (np.asarray(x, dtype=np.int64) for x in (kind, k, m, n))
+    T = MT - k - 1
+    out = np.ones(len(kind), dtype=np.int64)
+    q = (kind == 9) | (kind == 11)
+    out[q] = np.maximum(T[q], 1)
+    tr = (kind == 1) | (kind == 5)
+    out[tr] = np.where(m[tr] == k[tr] + 1, 1, np.maximum(T[tr] - 1, 1))
+    tu = kind == 6
+    out[tu] = np.where(n[tu] == k[tu] + 1, 1, np.maximum(T[tu] - 1, 1))
+    up = (kind == 2) | (kind == 3) | (kind == 7)
+    r, c = update_tile(algo, kind, m, n)
+    first = (r == k + 1) | (c == k + 1) | ((r == k + 2) & (c == k + 2))
+    chain = (r == k + 1) & (c == k + 1)
+    if algo == 0:
+        rest = np.maximum(T * (T + 1) // 2 - 1 - T, 1)
+        fcount = T
+    else:
+        rest = np.maximum((T - 1) ** 2 - 1, 1)
+        fcount = 2 * T - 1
+    val = np.where(chain, 1, np.where(first, fcount, rest))
+    out[up] = val[up]
+    return out
+
+
+def on_chain(algo, kind, k, m, n):
+    """1 for the tasks on the DAG's critical path, the diagonal chain: the
+    diagonal factorizations, the TRSMs of the tiles next to the diagonal and
+    the update of tile (k+1,k+1); QR: GEQRT and the TSQRT panel chain."""
+    kind, k, m, n = (np.asarray(x, dtype=np.int64) for x in (kind, k, m, n))
+    out = np.isin(kind, (0, 4, 8, 10))
+    idx = np.where(kind == 6, n, m)
+    out |= np.isin(kind, (1, 5, 6)) & (idx == k + 1)
+    r, c = update_tile(algo, kind, m, n)
+    out |= np.isin(kind, (2, 3, 7)) & (r == k + 1) & (c == k + 1)
+    return out.astype(np.float32)
+
+
+def update_tile(algo, kind, m, n):
+    """(row, column) tile an update task writes: Cholesky syrk(k,m) -> (m,m),
+    gemm(k,m,n) -> (n,m); LU gemm(k,m,n) -> (m,n)."""
+    kind, m, n = (np.asarray(x, dtype=np.int64) for x in (kind, m, n))
+    if algo == 0:
+        return np.where(kind == 2, m, n), m
+    return m, n
+
+
+def features(algo, N, b, tier, recs):
+    """Feature rows of the reference tasks given as (kind, k, m, n) rows."""
+    r = np.asarray(recs, dtype=np.float64).reshape(-1, 4)
+    MT = N // b
+    a = ALGO_CODE[algo]
     X = np.zeros((len(r), NFEAT), dtype=np.float32)
-    cls = r[:, 0].astype(int)
-    X[np.arange(len(r)), cls] = 1.0
-    X[:, NCLASS] = np.log2(np.maximum(r[:, 1], 1.0)) / 9.0
-    X[:, NCLASS + 1] = np.log10(r[:, 3] + 1.0) / 13.0
-    X[:, NCLASS + 2] = np.log10(r[:, 4] + 1.0) / 11.0
-    # tiers: precision 0 FP64, 1 FP32, 2 BF16, 3 FP8, 4 split BF16 -> columns 0..4
-    tier = r[:, 2].astype(int)
-    X[np.arange(len(r)), NCLASS + 3 + tier.clip(0, 4)] = 1.0
-    y = np.log(np.maximum(r[:, 5], 1e-4)).astype(np.float32)
-    return X, y
+    kinds = r[:, 0].astype(int)
+    types = np.array([KIND_TYPE[x] for x in range(12)])[kinds]
+    X[np.arange(len(r)), types] = 1.0
+    X[:, NTYPE] = math.log2(b) / 10.0
+    X[:, NTYPE + 1 + int(tier)] = 1.0
+    X[:, NTYPE + 6] = math.log2(N) / 16.0
+    X[:, NTYPE + 7] = r[:, 1] / max(MT, 1)
+    lt = launch_tasks(a, MT, kinds, r[:, 1].astype(int), r[:, 2].astype(int), r[:, 3].astype(int))
+    X[:, NTYPE + 8] = np.log2(1.0 + lt) / 12.0
+    X[:, NTYPE + 9] = on_chain(a, kinds, r[:, 1].astype(int), r[:, 2].astype(int), r[:, 3].astype(int))
+    return X
+
+
+def dataset(runs, max_per_run=4000, seed=0):
+    """(X, y = log ms) from traced runs: (algo, N, b, tier, schedule, records)
+    with records the library's (kind, k, m, n, ms, start) rows; at most
+    max_per_run tasks of each run (sampled per kind), unmeasured tasks
+    dropped."""
+    rng = np.random.default_rng(seed)
+    Xs, ys = [], []
+    for algo, N, b, tier, sched, rec in runs:
+        rec = np.asarray(rec, dtype=np.float64).reshape(-1, 6)
+        rec = rec[rec[:, 4] > 0]
+        if len(rec) > max_per_run:
+            keep = []
+            kinds = np.unique(rec[:, 0])
+            per = max(1, max_per_run // len(kinds))
+            for kd in kinds:
+                idx = np.flatnonzero(rec[:, 0] == kd)
+                keep.extend(rng.choice(idx, size=min(per, len(idx)), replace=False))
+            rec = rec[np.sort(keep)]
+        Xs.append(features(algo, N, b, tier, rec[:, :4]))
+        ys.append(np.log(rec[:, 4]).astype(np.float32))
+    X = np.concatenate(Xs)
+    y = np.concatenate(ys)
+    perm = rng.permutation(len(X))             # the trainer takes mini-batches in order
+    return X[perm], y[perm]
 
 
 def nparams(nfeat=NFEAT):
@@ -126,81 +225,174 @@ class MLP:
         return loss
 
 
-def launches(algo, N, b, prec=0, split=True, fs=None):
-    """Launches of one factorization in the diagonal-chain schedule:
-    list of (k, stream, class, flops, bytes) with stream 'chain', 'aux'
-    (row/column TRSMs and packing of step k), 'first' (update of tile row
-    k+1 and tile (k+2,k+2)) or 'rest' (the other trailing tiles).  Mirrors
-    dag_cholesky_chain / dag_lu_chain (palial_amd/csrc/api.hip) with the split
-    update (split=False: one update launch per step, labelled 'first'); the
-    mixed tiers use packed panels when b % 128 == 0.  fs: the solve's FP32
-    working copy (default: packed panels on the split schedule), with FP32
-    tiles in the byte counts and the chain's diagonal update on the packed
-    kernel."""
-    MT = N // b
-    b3, tb = float(b) ** 3, 8.0 * b * b
-    packed = prec > 0 and b % 128 == 0
-    if fs is None:
-        fs = packed and split and MT > 1
-    sbt = 4.0 * b * b if fs else tb             # bytes of one stored tile
-    esz = {2: 2.0, 3: 1.0}.get(prec, 4.0)     # packed bytes per element (split BF16: hi + lo)
+
+
+# ------------------------------------------------------------------ schedule replay
+def _slots(ktype, b, tier):
+    """Resident workgroups of a task's kernel on the whole GPU and workgroups
+    per task: 128x128 output blocks (two workgroups per CU) for the updates,
+    one strip workgroup per 32 columns for the TRSMs (one per CU in FP64, two
+    in FP32), one workgroup for a diagonal or panel factorization."""
+    if ktype in ("gemm", "syrk", "qrupd"):
+        blk = (b // 128) ** 2 if b % 128 == 0 else (b // 32) ** 2
+        per_cu = 2 if b % 128 == 0 else 4
+        if ktype == "syrk":
+            blk = max(1, (int(math.sqrt(blk)) * (int(math.sqrt(blk)) + 1)) // 2)
+        return NCU * per_cu, blk
+    if ktype == "trsm":
+        return NCU * (2 if tier > 0 else 1), max(1, b // 32)
+    return NCU, 1
+
+
+def _launch_ms(pred, ktype, b, tier):
+    """Duration of a batched launch from its tasks' predicted times."""
+    if len(pred) == 0:
+        return 0.0
+    slots, w = _slots(ktype, b, tier)
+    return max(float(np.max(pred)), float(np.sum(pred)) * w / slots)
+
+
+def task_keys(algo, MT):
+    """(kind, k, m, n) of every reference task in creation order, as an int array."""
+    from . import trace
+    code = {"cholesky": 3, "lu": 7, "qr": 11}[algo]
     out = []
-    chol = algo == "cholesky"
-    npan = 1 if chol else 2                      # packed panels per step
-    for k in range(MT):
-        T = MT - k - 1
-        if chol:
-            out.append((k, "chain", CLASSES["diag"], b3 / 3, 2 * tb))
-        else:
-            out.append((k, "chain", CLASSES["diag"], 2 * b3 / 3, 2 * tb))
-        if T > 0:
-            for _ in range(npan):
-                out.append((k, "chain", CLASSES["trsm"], b3, 3 * sbt))
-            if fs:
-                for _ in range(npan):
-                    out.append((k, "chain", CLASSES["pack"], 0.0, sbt + esz * b * b))
-                out.append((k, "chain", CLASSES["pkgemm"], b3 if chol else 2 * b3, 2 * sbt + npan * esz * b * b))
-            else:
-                out.append((k, "chain", CLASSES["gemm"], b3 if chol else 2 * b3, 3 * tb))
-        if T <= 1:
-            continue
-        for _ in range(npan):
-            out.append((k, "aux", CLASSES["trsm"], (T - 1) * b3, 3.0 * (T - 1) * sbt))
-        if packed:
-            for _ in range(npan):
-                out.append((k, "aux", CLASSES["pack"], 0.0, T * (sbt + esz * b * b)))
-        # (gemm tiles, syrk tiles, packed panel tiles read) of each update launch
-        if chol:
-            parts = [("first", T - 1.0, 1.0, T), ("rest", (T - 1) * (T - 2) / 2.0, T - 2.0, T - 1)] if split \
-                else [("first", T * (T - 1) / 2.0, T - 1.0, T)]
-        else:
-            parts = [("first", 2.0 * T - 1, 0.0, T), ("rest", (T - 1.0) ** 2 - 1, 0.0, T - 1)] if split \
-                else [("first", T * T - 1.0, 0.0, T)]
-        for st, ng, ns, np_ in parts:
-            if ng + ns <= 0:
-                continue
-            fl = 2 * b3 * ng + b3 * ns
-            if packed:
-                out.append((k, st, CLASSES["pkgemm"], fl, 2.0 * (ng + ns) * sbt + npan * np_ * esz * b * b))
-            else:
-                out.append((k, st, CLASSES["gemm"], fl, 3.0 * (ng + ns) * tb))
-    return out
+    for _, task in trace.tasks(algo, MT):
+        c, k, m, n = trace._key(task)
+        out.append((code if c is None else c, k, m, n))
+    return np.array(out, dtype=np.int64).reshape(-1, 4)
 
 
-def predict_factor_ms(model, algo, N, b, prec=0):
-    """Predicted factorization time of the split chain schedule: step k's
-    first update launch, then its remaining update beside step k+1's row
-    TRSMs and packing, against the chain's step (both bounded below by it)."""
-    L = launches(algo, N, b, prec)
-    recs = np.array([(c, b, prec, f, by, 1.0) for (_, _, c, f, by) in L])
-    X, _ = featurize(recs)
-    ms = np.exp(model.predict_log_ms(X).astype(np.float64))
+def _group(gid, ngroups, ms):
+    s = np.bincount(gid, weights=ms, minlength=ngroups)
+    mx = np.zeros(ngroups)
+    np.maximum.at(mx, gid, ms)
+    cnt = np.bincount(gid, minlength=ngroups)
+    return s, mx, cnt
+
+
+def replay(algo, N, b, tier, schedule, ms, keys=None):
+    """Makespan (ms) of one factorization from per-task times ms (creation
+    order), replaying the library's streams for the schedule: Cholesky / LU
+    1 diagonal chain + split update (also 0, the default), 2 diagonal chain +
+    one update launch, 3 row look-ahead (the panel stream updates tile row
+    k+1, factors tile (k+1,k+1) and solves panel k+1 while the main stream
+    updates the rest), 4 one stream; QR:
+    schedule = panel chains in flight (0: 8)."""
     MT = N // b
-    acc = {st: np.zeros(MT + 1) for st in ("chain", "aux", "first", "rest")}
-    for (k, st, _, _, _), t in zip(L, ms):
-        acc[st][k] += t
-    total = acc["aux"][0]
-    for k in range(MT):
-        bulk = acc["first"][k] + max(acc["rest"][k], acc["aux"][k + 1])
-        total += max(acc["chain"][k], bulk)
-    return float(total)
+    keys = task_keys(algo, MT) if keys is None else keys
+    ms = np.asarray(ms, dtype=np.float64)
+    kind, k, m, n = keys.T
+    if algo == "qr":
+        sl, w = _slots("qrupd", b, tier)
+        ns = schedule if schedule > 0 else 8
+        geq = np.zeros(MT)
+        geq[k[kind == 8]] = ms[kind == 8] + LAUNCH_MS
+        tsq = np.zeros((MT, MT))
+        sel = kind == 10
+        tsq[k[sel], m[sel]] = ms[sel] + LAUNCH_MS
+        sel = kind == 9
+        o_s, o_m, _ = _group(k[sel], MT, ms[sel])
+        sel = kind == 11
+        g_s, g_m, _ = _group(k[sel] * MT + m[sel], MT * MT, ms[sel])
+        dur = lambda s_, m_: 3 * LAUNCH_MS + max(m_, s_ * w / sl)     # three launches per update
+        chain_free = [0.0] * ns
+        upd_free = [0.0] * ns
+        tsm_prev = np.zeros(MT)
+        for kk in range(MT):
+            c = kk % ns
+            g_end = max(chain_free[c], tsm_prev[kk] if kk > 0 else 0.0) + geq[kk]
+            chain_free[c] = g_end
+            tsm_now = np.zeros(MT)
+            if kk + 1 < MT:
+                upd_free[c] = max(upd_free[c], g_end) + dur(o_s[kk], o_m[kk])
+                for mm in range(kk + 1, MT):
+                    q_end = max(chain_free[c], tsm_prev[mm] if kk > 0 else 0.0) + tsq[kk, mm]
+                    chain_free[c] = q_end
+                    gi = kk * MT + mm
+                    upd_free[c] = max(upd_free[c], q_end) + dur(g_s[gi], g_m[gi])
+                    tsm_now[mm] = upd_free[c]
+            tsm_prev = tsm_now
+        work = (np.sum(ms[kind == 9]) + np.sum(ms[kind == 11])) * w / sl
+        return max(max(chain_free), max(upd_free), work)
+    chol = algo == "cholesky"
+    dsel = kind == (0 if chol else 4)
+    diag = np.zeros(MT)
+    diag[k[dsel]] = ms[dsel] + LAUNCH_MS
+    slt, wt = _slots("trsm", b, tier)
+    slg, wg = _slots("gemm", b, tier)
+
+    def launches(mask, slots, w):
+        """Per-step duration of one batched launch holding the masked tasks
+        (with its dispatch gap; 0 for a step without such tasks)."""
+        s_, m_, cnt = _group(k[mask], MT, ms[mask])
+        return np.where(cnt > 0, np.maximum(m_, s_ * w / slots) + LAUNCH_MS, 0.0)
+
+    idx1 = np.where(kind == 6, n, m)
+    # TRSM launches: Cholesky one panel; LU the U row (kind 6) and the L column (kind 5)
+    tr_groups = [kind == 1] if chol else [kind == 6, kind == 5]
+    tr_chain = sum(launches(g & (idx1 == k + 1), slt, wt) for g in tr_groups)
+    tr_rest = sum(launches(g & (idx1 > k + 1), slt, wt) for g in tr_groups)
+    tr_all = sum(launches(g, slt, wt) for g in tr_groups)
+    up = (kind == 2) | (kind == 3) | (kind == 7)
+    r, c = update_tile(0 if chol else 1, kind, m, n)
+    s1 = up & (r == k + 1) & (c == k + 1)
+    row = up & ((r == k + 1) | (c == k + 1))                 # tile row (LU: and column) k+1, with (k+1,k+1)
+    first = up & ~s1 & (row | ((r == k + 2) & (c == k + 2)))
+    rest = up & ~s1 & ~first
+    cs = launches(s1, slg, wg)
+    work = float(np.sum(ms[up]) * wg / slg)
+    end = 0.0
+    if schedule == 4:
+        return float(np.sum(diag) + np.sum(tr_all) + np.sum(launches(up, slg, wg)))
+    if schedule == 3:
+        d_row = launches(row, slg, wg)
+        d_rest = launches(up & ~row, slg, wg)
+        s1_free = diag[0] + tr_all[0]
+        p_ready = s1_free
+        s0_free = 0.0
+        u_prev = 0.0
+        for kk in range(MT - 1):
+            s1_free = max(s1_free, u_prev) + d_row[kk] + diag[kk + 1] + tr_all[kk + 1]
+            s0_free = max(s0_free, p_ready) + d_rest[kk]
+            u_prev = s0_free
+            p_ready = s1_free
+            end = max(end, s1_free, s0_free)
+        return max(end, work)
+    split = schedule in (0, 1)
+    dfi = launches(first, slg, wg) if split else launches(first | rest, slg, wg)
+    dre = launches(rest, slg, wg) if split else np.zeros(MT)
+    p_end = diag[0]
+    f_prev = r_prev = 0.0
+    aux_free = bulk_free = 0.0
+    end = p_end
+    for kk in range(MT - 1):
+        t1_end = max(p_end, f_prev if split else r_prev) + tr_chain[kk]
+        p_next = t1_end + cs[kk] + diag[kk + 1]
+        if split:
+            aux_end = max(aux_free, f_prev, p_end) + tr_rest[kk]
+            aux_free = aux_end
+            f_end = max(bulk_free, aux_end, t1_end) + dfi[kk]
+            bulk_free = f_end + dre[kk]
+            f_prev, r_prev = f_end, bulk_free
+        else:
+            tb_end = max(bulk_free, p_end) + tr_rest[kk]
+            bulk_free = max(tb_end, t1_end) + dfi[kk]
+            f_prev = r_prev = bulk_free
+        p_end = p_next
+        end = max(end, p_end, bulk_free)
+    return max(end, work)
+
+
+def predict_ms(model, algo, N, b, tier, schedule):
+    """Predicted factorization time (before calibration) of one configuration."""
+    keys = task_keys(algo, N // b)
+    X = features(algo, N, b, tier, keys)
+    ms = np.exp(model.predict_log_ms(X).astype(np.float64))
+    return replay(algo, N, b, tier, schedule, ms, keys)
+
+
+def calibrate(pred, meas):
+    """Scale c minimising the squared log error of c * pred against meas."""
+    pred, meas = np.asarray(pred, dtype=np.float64), np.asarray(meas, dtype=np.float64)
+    return float(np.exp(np.mean(np.log(meas) - np.log(pred))))

@@ -1239,7 +1239,7 @@ void qr_streams_down()
 // TSMQR(k, m) and TSMQR(k+1, m') write rows {k, m} and {k+1, m'}, m' < m),
 // so up to ns steps progress together and the factors are the same bits as
 // in the reference's order.
-void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1, bool two, Launcher &L)
+void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1, bool two, Launcher &L, int ns)
 {
     L.b = b;
     L.prec = 0;
@@ -1292,7 +1292,7 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1
     }
     (void)s1;
     const hipStream_t *chain = g.s_qr, *bulk = g.s_qrb;
-    const int ns = g.qr_streams;
+    ns = std::max(1, std::min(ns, g.qr_streams));
     hipEvent_t *E = g.events.data();                 // E[(k & 1) MT + m]: TSMQR(k, m) done
     hipEvent_t evG = g.events[2 * MT], evQ = g.events[2 * MT + 1], fork = g.events[2 * MT + 2];
     hipEvent_t *join = g.events.data() + 2 * MT + 3;   // 2 ns events
@@ -1792,7 +1792,14 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     const size_t bytes = (size_t)N * N * 8;
     hipStream_t s0 = (o.device_matrix && o.stream) ? (hipStream_t)o.stream : g.s_main;
     hipStream_t s1 = g.s_panel;
-    const bool lookahead = !o.no_lookahead && MT > 1 && algo != ALG_QR;
+    // schedule of this call (palial_opts.schedule): Cholesky / LU 1 diagonal
+    // chain with the split update, 2 diagonal chain, 3 row look-ahead, 4 one
+    // stream; QR: panel chains in flight (1..8); 0: the library default
+    if (o.schedule < 0 || o.schedule > (algo == ALG_QR ? QR_MAXS : 4))
+        return fail(PALIAL_ERR_ARG, "schedule %d out of range", o.schedule);
+    const int dag = (algo != ALG_QR && o.schedule >= 1 && o.schedule <= 3) ? 3 - o.schedule : g.dag;
+    const int qr_chains = (algo == ALG_QR && o.schedule > 0) ? std::min(o.schedule, g.qr_streams) : g.qr_streams;
+    const bool lookahead = !o.no_lookahead && MT > 1 && algo != ALG_QR && !(algo != ALG_QR && o.schedule == 4);
     // QR's two-stream schedule runs eagerly (a graph replay drops the bulk stream's CU mask)
     const bool qr_two = algo == ALG_QR && !o.no_lookahead && MT > 1;
     // the factorizations here are FP64: their chain schedule uses no CU mask (bulk_stream)
@@ -1839,13 +1846,16 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     }
     Launcher L;
     auto issue = [&](hipStream_t st) {
+        const int dag_default = g.dag;
+        g.dag = dag;        // the dispatchers read the schedule from the context (calls are serialised)
         if (algo == ALG_CHOL) dag_cholesky(dA, N, b, lookahead, st, s1, g.d_info, L);
         else if (algo == ALG_LU) dag_lu(dA, N, b, lookahead, st, s1, L);
-        else dag_qr(dA, dS, N, b, st, s1, qr_two, L);
+        else dag_qr(dA, dS, N, b, st, s1, qr_two, L, qr_chains);
+        g.dag = dag_default;
     };
     long nodes = 0;
     if (use_graph) {
-        auto key = std::make_tuple(algo, (void *)dA, (void *)dS, N, b, (int)lookahead);
+        auto key = std::make_tuple(algo, (void *)dA, (void *)dS, N, b, (int)lookahead + 2 * dag);
         auto it = g.graphs.find(key);
         if (it == g.graphs.end() || it->second.exec == nullptr) {
             Graph G;
@@ -2342,7 +2352,9 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
     a.n = d_n; a.off = d_off; a.woff = d_woff; a.boff = d_boff; a.A = dA; a.B = dB;
     a.work = g.d_bwork; a.info = d_info; a.f32 = (prec == PALIAL_PREC_FP32);
     a.count = count; a.order = d_ord; a.kind = d_kind; a.queue = d_queue;
-    const int grid = batch_grid(P.bmax, count);
+    if (o.batch_grid < 0) return fail(PALIAL_ERR_ARG, "batch_grid %d", o.batch_grid);
+    // persistent workgroups: what the CUs hold at once, or the caller's count
+    const int grid = o.batch_grid > 0 ? std::min(o.batch_grid, count) : batch_grid(P.bmax, count);
     if (grid < 1) return fail(PALIAL_ERR_HIP, "batched launch shape");
     // one persistent launch for both kinds, captured once into a hipGraph
     // (per argument set) and replayed; the queue counter is zeroed before it,

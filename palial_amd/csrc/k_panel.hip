@@ -285,6 +285,7 @@ __global__ __launch_bounds__(TRSM_THREADS) void trsm_kernel(TrsmArgs p)
     auto Mat = [&](int i, int q) -> double {
         return (KIND == TR_LUL) ? D[(size_t)i + (size_t)q * b] : D[(size_t)q + (size_t)i * b];
     };
+    task_begin(p.ts, p.ts_base + tix);
 
     if (KIND == TR_LUR) {
         for (int e = t; e < b * SW; e += TRSM_THREADS) {
@@ -343,6 +344,7 @@ __global__ __launch_bounds__(TRSM_THREADS) void trsm_kernel(TrsmArgs p)
             X[(size_t)i + (size_t)(c0 + c) * b] = sx[i * SW + c];
         }
     }
+    task_end(p.ts, p.ts_base + tix);
 }
 
 // ---------------------------------------------------------------- tiny tiles
@@ -462,11 +464,13 @@ hipError_t launch_trsm(int kind, const TrsmArgs &a, hipStream_t s)
         return hipGetLastError();
     }
     if (a.b % 16 != 0 || (size_t)a.b * TRSM_SW * 8 > 150 * 1024) return hipErrorInvalidValue;
-    dim3 grid((unsigned)(a.ntiles * (a.b / TRSM_SW))), block(TRSM_THREADS);
+    TrsmArgs t = a;
+    trace_trsm(kind, t);
+    dim3 grid((unsigned)(t.ntiles * (t.b / TRSM_SW))), block(TRSM_THREADS);
     const size_t lds = (size_t)a.b * TRSM_SW * 8;
-    if (kind == TR_CHOL) hipLaunchKernelGGL(trsm_kernel<TR_CHOL>, grid, block, lds, s, a);
-    else if (kind == TR_LUL) hipLaunchKernelGGL(trsm_kernel<TR_LUL>, grid, block, lds, s, a);
-    else hipLaunchKernelGGL(trsm_kernel<TR_LUR>, grid, block, lds, s, a);
+    if (kind == TR_CHOL) hipLaunchKernelGGL(trsm_kernel<TR_CHOL>, grid, block, lds, s, t);
+    else if (kind == TR_LUL) hipLaunchKernelGGL(trsm_kernel<TR_LUL>, grid, block, lds, s, t);
+    else hipLaunchKernelGGL(trsm_kernel<TR_LUR>, grid, block, lds, s, t);
     return hipGetLastError();
 }
 

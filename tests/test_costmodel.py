@@ -23,35 +23,75 @@ def test_reference_gradient_matches_finite_differences():
         assert abs((lp - lm) / (2 * eps) - g[i]) <= 1e-6 * max(1.0, abs(g[i]))
 
 
-def test_featurize():
-    recs = np.array([[0, 512, 0, 2.7e11, 6e9, 3.4], [6, 256, 2, 1e9, 2e8, 0.1],
-                     [6, 512, 3, 1e9, 2e8, 0.1], [6, 512, 4, 1e9, 2e8, 0.1]])
-    X, y = cm.featurize(recs)
-    assert X.shape == (4, cm.NFEAT) and X.dtype == np.float32
-    assert X[0, 0] == 1 and X[1, 6] == 1 and X[0, cm.NCLASS + 3] == 1 and X[1, cm.NCLASS + 5] == 1
-    # one tier column each: FP8 and split BF16 are not folded into BF16
-    assert X[2, cm.NCLASS + 6] == 1 and X[3, cm.NCLASS + 7] == 1
-    assert (X[:, cm.NCLASS + 3:].sum(1) == 1).all()
-    assert np.allclose(y, np.log([3.4, 0.1, 0.1, 0.1]))
+def test_features():
+    keys = cm.task_keys("cholesky", 4)
+    X = cm.features("cholesky", 2048, 512, 2, keys)
+    assert X.shape == (len(keys), cm.NFEAT) and X.dtype == np.float32 and cm.NFEAT == 16
+    assert (X[:, :cm.NTYPE].sum(1) == 1).all()
+    assert (X[:, cm.NTYPE + 1:cm.NTYPE + 6].sum(1) == 1).all() and (X[:, cm.NTYPE + 3] == 1).all()
+    # potrf000 is a diagonal task of depth 0 in a one-task launch, on the critical path
+    assert X[0, 0] == 1 and X[0, cm.NTYPE + 7] == 0 and X[0, cm.NTYPE + 8] == 1 / 12 and X[0, cm.NTYPE + 9] == 1
+    # chain: potrf(k), trsm(k,k+1), syrk(k,k+1) -- 3 per step but the last step's potrf only
+    assert X[:, cm.NTYPE + 9].sum() == 3 * 3 + 1
 
 
-@pytest.mark.parametrize("algo", ["cholesky", "lu"])
-@pytest.mark.parametrize("prec", [0, 2])
-def test_launch_flops_sum_to_tiled_count(algo, prec):
+def test_launch_tasks_cover_the_default_schedule():
+    """The launch sizes of the split chain schedule: per step k the chain's
+    one-tile update of (k+1,k+1), the first update launch (tile row k+1 and
+    tile (k+2,k+2); LU also column k+1) and the rest: exactly `size` tasks
+    carry each size, and the three add up to the step's trailing tiles."""
+    MT = 7
+    for algo, a in (("cholesky", 0), ("lu", 1)):
+        keys = cm.task_keys(algo, MT)
+        lt = cm.launch_tasks(a, MT, *keys.T)
+        kind, k, m, n = keys.T
+        up = (kind == 2) | (kind == 3) | (kind == 7)
+        for kk in range(MT - 3):          # T >= 3: three distinct launch sizes
+            T = MT - kk - 1
+            sizes = lt[up & (k == kk)]
+            first = T if a == 0 else 2 * T - 1
+            rest = T * (T + 1) // 2 - 1 - T if a == 0 else (T - 1) ** 2 - 1
+            assert sorted(set(sizes.tolist())) == sorted({1, first, rest})
+            assert (sizes == 1).sum() == 1 and (sizes == first).sum() == first and (sizes == rest).sum() == rest
+        trs = (kind == 1) | (kind == 5) | (kind == 6)
+        assert set(lt[trs & (k == 0)].tolist()) == {1, MT - 2}
+
+
+def test_replay_closed_forms(monkeypatch):
+    monkeypatch.setattr(cm, "LAUNCH_MS", 0.0)     # dispatch gaps off: pure task times
     N, b = 4096, 512
     MT = N // b
-    L = cm.launches(algo, N, b, prec)
-    tot = sum(f for (_, _, _, f, _) in L)
-    b3 = float(b) ** 3
-    pairs = MT * (MT - 1) / 2
-    if algo == "cholesky":
-        # POTRF + TRSM + SYRK + GEMM of the reference DAG (src/cholesky.h:32-173)
-        exp = MT * b3 / 3 + pairs * b3 + pairs * b3 + MT * (MT - 1) * (MT - 2) / 6 * 2 * b3
-    else:
-        # GETRF + 2 TRSM per pair + GEMM over the trailing squares (src/lu.h)
-        exp = MT * 2 * b3 / 3 + 2 * pairs * b3 + sum((MT - k - 1) ** 2 for k in range(MT)) * 2 * b3
-    assert np.isclose(tot, exp, rtol=1e-12)
-    assert {st for (_, st, _, _, _) in L} == {"chain", "aux", "first", "rest"}
-    # one update launch per step: same flops
-    tot1 = sum(f for (_, _, _, f, _) in cm.launches(algo, N, b, prec, split=False))
-    assert np.isclose(tot1, exp, rtol=1e-12)
+    for algo in ("cholesky", "lu"):
+        keys = cm.task_keys(algo, MT)
+        diag = (keys[:, 0] == 0) | (keys[:, 0] == 4)
+        ms = np.where(diag, 2.0, 0.0)
+        for sched in (1, 2, 3, 4):      # only the diagonal tasks cost: MT of them in a row
+            assert cm.replay(algo, N, b, 0, sched, ms, keys) == pytest.approx(2.0 * MT)
+        rng = np.random.default_rng(1)
+        ms = rng.uniform(0.1, 1.0, len(keys))
+        one = cm.replay(algo, N, b, 0, 4, ms, keys)
+        for sched in (1, 2):
+            assert cm.replay(algo, N, b, 0, sched, ms, keys) <= one + 1e-9
+    keys = cm.task_keys("qr", MT)
+    ms = np.random.default_rng(2).uniform(0.1, 1.0, len(keys))
+    t = [cm.replay("qr", N, b, 0, s, ms, keys) for s in (1, 2, 4, 8)]
+    assert all(x >= y - 1e-9 for x, y in zip(t, t[1:]))
+    # one chain: every panel task in a row, so at least their sum
+    panel = (keys[:, 0] == 8) | (keys[:, 0] == 10)
+    assert t[0] >= ms[panel].sum() - 1e-9
+
+
+def test_calibrate():
+    assert cm.calibrate([1.0, 2.0, 4.0], [3.0, 6.0, 12.0]) == pytest.approx(3.0)
+
+
+def test_subset_design():
+    from palial_amd import autotune as at
+    c = at.factor_candidates("cholesky", 8192)
+    assert {b for b, _ in c} == {128, 256, 512, 1024}
+    sub = at.subset(c, 512, 1)
+    assert len(sub) == 4 + 3 and all(b == 512 or s == 1 for b, s in sub)
+    assert {b for b, _ in at.factor_candidates("cholesky", 24576)} == {128, 256, 384, 512, 768, 1024}
+    assert {b for b, _ in at.factor_candidates("qr", 8192)} == {128, 256, 512}
+    sc = at.solve_candidates(24576)
+    assert (384, 3) in sc and (384, 4) in sc and all(t < 3 or b % 128 == 0 for b, t in sc)
