@@ -216,12 +216,16 @@ def run_batched(P, count=10000, steps=3, seed=0):
     from its captured hipGraph); FP64
     and FP32.  Inputs are regenerated before each timed run."""
     import numpy as np
+    from palial_amd import autotune
     rng = np.random.default_rng(seed)
     n = rng.integers(16, 513, count).astype(np.int32)
     kind = (np.arange(count) % 2).astype(np.int32)
     rhs = rng.standard_normal(int(n.sum()))
     out = {}
     for prec, name in ((0, "fp64"), (1, "fp32")):
+        # the persistent launch's workgroup count from the autotuner (every candidate measured)
+        tuned = autotune.tune_batch(P, n, kind, precision=prec)
+        log(f"[bench] batched {name}: autotuned grid {tuned['grid']} from {tuned['measured_ms']}")
         bt = P.Batch(n, kind, precision=prec)
         times = []
         flat = rhs.astype(bt.dtype)
@@ -229,14 +233,14 @@ def run_batched(P, count=10000, steps=3, seed=0):
             bt.generate(seed)
             P._check(P.lib().palial_hip_memcpy_h2d(bt.B, flat.ctypes.data, flat.nbytes))
             P.synchronize()
-            rep = bt.factor(solve=True)
+            rep = bt.factor(solve=True, grid=tuned["grid"])
             if i > 0:
                 times.append(rep.factor_ms)
         fails = int((bt.info != 0).sum())
         bt.free()
         t = sorted(times)[len(times) // 2]
         out[name] = {"ms": round(t, 3), "gflop": round(rep.gflop, 2), "tflops": round(rep.gflop / t, 3),
-                     "failures": fails}
+                     "failures": fails, "grid": tuned["grid"], "grid_candidates_ms": tuned["measured_ms"]}
         log(f"[bench] batched {count} matrices {name}: {t:.2f} ms ({rep.gflop / t:.2f} TFLOP/s factor+solve)")
     return {"matrices": count, "n": "UniformInt[16,512] seed 0, even Cholesky / odd LU, 1 RHS", **out}
 
