@@ -168,12 +168,12 @@ int ensure_events(size_t n)
 void drop_graphs()
 {
     for (auto &kv : g.graphs) {
-        if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.graph) hipGraphDestroy(kv.second.graph);
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
     g.graphs.clear();
-    if (g.bgraph.exec) hipGraphExecDestroy(g.bgraph.exec);
-    if (g.bgraph.graph) hipGraphDestroy(g.bgraph.graph);
+    if (g.bgraph.exec) (void)hipGraphExecDestroy(g.bgraph.exec);
+    if (g.bgraph.graph) (void)hipGraphDestroy(g.bgraph.graph);
     g.bgraph = BatchGraph();
 }
 
@@ -185,8 +185,8 @@ int ensure_ws(int b, int MT)
     const size_t need = (size_t)QR_BULK * b * b * (size_t)(MT > 1 ? MT : 1);
     if (g.w_cap < need || g.tau_cap < (size_t)QR_CHAINS * b) drop_graphs();   // captured graphs hold the old pointers
     if (g.w_cap < need) {
-        if (g.d_w1) hipFree(g.d_w1);
-        if (g.d_w2) hipFree(g.d_w2);
+        if (g.d_w1) (void)hipFree(g.d_w1);
+        if (g.d_w2) (void)hipFree(g.d_w2);
         g.d_w1 = g.d_w2 = nullptr;
         g.w_cap = 0;
         if (hipMalloc(&g.d_w1, need * 8) != hipSuccess || hipMalloc(&g.d_w2, need * 8) != hipSuccess)
@@ -195,7 +195,7 @@ int ensure_ws(int b, int MT)
     }
     if (g.qg_cap < (size_t)QR_CHAINS * b * b) {
         drop_graphs();
-        if (g.d_qg) hipFree(g.d_qg);
+        if (g.d_qg) (void)hipFree(g.d_qg);
         g.d_qg = nullptr;
         g.qg_cap = 0;
         if (hipMalloc(&g.d_qg, (size_t)QR_CHAINS * b * b * 8) != hipSuccess)
@@ -203,7 +203,7 @@ int ensure_ws(int b, int MT)
         g.qg_cap = (size_t)QR_CHAINS * b * b;
     }
     if (g.tau_cap < (size_t)QR_CHAINS * b) {
-        if (g.d_tau) hipFree(g.d_tau);
+        if (g.d_tau) (void)hipFree(g.d_tau);
         g.d_tau = nullptr;
         g.tau_cap = 0;
         if (hipMalloc(&g.d_tau, (size_t)QR_CHAINS * b * 8) != hipSuccess)
@@ -220,7 +220,7 @@ int ensure_dinv(int b, int MT)
     const size_t need = (size_t)MT * 64 * b;
     if (g.dinv_cap < need) {
         drop_graphs();
-        if (g.d_dinv) hipFree(g.d_dinv);
+        if (g.d_dinv) (void)hipFree(g.d_dinv);
         g.d_dinv = nullptr;
         if (hipMalloc(&g.d_dinv, need * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "dinv alloc failed");
         g.dinv_cap = need;
@@ -228,7 +228,7 @@ int ensure_dinv(int b, int MT)
     const size_t sub = (size_t)b * b;
     if (g.subtile && b >= 256 && b % 128 == 0 && g.sub_cap < sub) {
         drop_graphs();
-        if (g.d_sub) hipFree(g.d_sub);
+        if (g.d_sub) (void)hipFree(g.d_sub);
         g.d_sub = nullptr;
         g.sub_cap = 0;
         if (hipMalloc(&g.d_sub, sub * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "sub-tile alloc failed");
@@ -249,7 +249,7 @@ int ensure_pack(int b, int MT)
     const size_t need = 2 * pack_set_bytes(b, MT) + 2 * (size_t)b * b * 4;
     if (g.pack_cap < need) {
         drop_graphs();
-        if (g.d_pack) hipFree(g.d_pack);
+        if (g.d_pack) (void)hipFree(g.d_pack);
         g.d_pack = nullptr;
         g.pack_cap = 0;
         if (hipMalloc(&g.d_pack, need) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "packed panel alloc failed");
@@ -371,14 +371,14 @@ void prof_collect()
 {
     for (auto &r : g_prof.pending) {
         float ms = 0;
-        hipEventElapsedTime(&ms, r.a, r.b_ev);
+        (void)hipEventElapsedTime(&ms, r.a, r.b_ev);
         g_prof.launches[r.cls]++;
         g_prof.ms[r.cls] += ms;
         g_prof.flops[r.cls] += r.flops;
         g_prof.bytes[r.cls] += r.bytes;
         if (g_prof.keep) g_prof.records.push_back({(double)r.cls, (double)r.b, (double)r.prec, r.flops, r.bytes, (double)ms});
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b_ev);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b_ev);
     }
     g_prof.pending.clear();
 }
@@ -442,8 +442,8 @@ void trace_begin(double *mat, int N, int b, int algo, hipStream_t s, Launcher &L
 {
     if (!g_tr.on) return;
     for (auto &x : g_tr.singles) {
-        hipEventDestroy(x.a);
-        hipEventDestroy(x.e);
+        (void)hipEventDestroy(x.a);
+        (void)hipEventDestroy(x.e);
     }
     g_tr.singles.clear();
     g_tr.launches.clear();
@@ -456,7 +456,7 @@ void trace_begin(double *mat, int N, int b, int algo, hipStream_t s, Launcher &L
     // every reference task once per launch covering it (QR updates: 3 phases)
     const size_t need = 3 * MT * MT * MT / 2 + 4 * MT * MT + 64;
     if (g_tr.cap < need) {
-        if (g_tr.d_ts) hipFree(g_tr.d_ts);
+        if (g_tr.d_ts) (void)hipFree(g_tr.d_ts);
         g_tr.d_ts = nullptr;
         g_tr.cap = 0;
         if (hipMalloc(&g_tr.d_ts, (need + 1) * 16) != hipSuccess) {
@@ -1220,9 +1220,9 @@ void qr_streams_down()
 {
     if (!g.qr_up) return;
     for (auto &q : g.s_qr)
-        if (q) { hipStreamDestroy(q); q = nullptr; }
+        if (q) { (void)hipStreamDestroy(q); q = nullptr; }
     for (auto &q : g.s_qrb)
-        if (q) { hipStreamDestroy(q); q = nullptr; }
+        if (q) { (void)hipStreamDestroy(q); q = nullptr; }
     g.qr_up = false;
 }
 
@@ -1391,7 +1391,7 @@ int grow(double **p, size_t *cap, size_t need)
 {
     if (*cap >= need) return 0;
     drop_graphs();
-    if (*p) hipFree(*p);
+    if (*p) (void)hipFree(*p);
     *p = nullptr;
     *cap = 0;
     if (hipMalloc(p, need * 8) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "solve workspace of %zu bytes", need * 8);
@@ -1493,9 +1493,9 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventCreate(&e2));
     auto done = [&](int code) {
-        hipStreamSynchronize(s);
-        if (ownA) hipFree(dA);
-        hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
+        (void)hipStreamSynchronize(s);
+        if (ownA) (void)hipFree(dA);
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
         return code;
     };
     Launcher L;
@@ -1546,7 +1546,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         HIPCHK(hipEventRecord(e2, s));
         HIPCHK(hipStreamSynchronize(s));
         float spent = 0;
-        hipEventElapsedTime(&spent, e0, e2);
+        (void)hipEventElapsedTime(&spent, e0, e2);
         done(0);
         palial_opts o2 = o;
         o2.precision = 0;
@@ -1755,8 +1755,8 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
                             o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     float fms = 0, sms = 0;
-    hipEventElapsedTime(&fms, e0, e1);
-    hipEventElapsedTime(&sms, e1, e2);
+    (void)hipEventElapsedTime(&fms, e0, e1);
+    (void)hipEventElapsedTime(&sms, e1, e2);
     prof_collect();
     if (rep) {
         const double n = N;
@@ -1817,10 +1817,10 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     HIPCHK(hipEventCreate(&e3));
     auto cleanup = [&]() {
         if (owned) {
-            if (dA) hipFree(dA);
-            if (dS) hipFree(dS);
+            if (dA) (void)hipFree(dA);
+            if (dS) (void)hipFree(dS);
         }
-        hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
     };
     if (o.device_matrix) {
         dA = (double *)A.matrix;
@@ -1873,20 +1873,24 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
             e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
             if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
             size_t n = 0;
-            hipGraphGetNodes(G.graph, nullptr, &n);
+            if (hipGraphGetNodes(G.graph, nullptr, &n) != hipSuccess) n = 0;
             G.nodes = (long)n;
             g.graphs[key] = G;
             it = g.graphs.find(key);
         }
         // order the replay after the copies on s0
         hipEvent_t gate = g.events[2 * MT + 3];
-        hipEventRecord(gate, s0);
-        hipStreamWaitEvent(g.s_main, gate, 0);
-        hipError_t e = hipGraphLaunch(it->second.exec, g.s_main);
-        if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "graph launch: %s", hipGetErrorString(e)); }
-        if (s0 != g.s_main) {
-            hipEventRecord(gate, g.s_main);
-            hipStreamWaitEvent(s0, gate, 0);
+        hipError_t e = hipEventRecord(gate, s0);
+        if (e == hipSuccess) e = hipStreamWaitEvent(g.s_main, gate, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(it->second.exec, g.s_main);
+        if (e == hipSuccess && s0 != g.s_main) {
+            e = hipEventRecord(gate, g.s_main);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s0, gate, 0);
+        }
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(s0);
+            cleanup();
+            return fail(PALIAL_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
         }
         nodes = it->second.nodes;
     } else {
@@ -1894,7 +1898,7 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
         issue(s0);
         trace_end();
         if (L.err != hipSuccess) {
-            hipStreamSynchronize(s0);
+            (void)hipStreamSynchronize(s0);
             cleanup();
             return fail(PALIAL_ERR_HIP, "launch failed: %s", hipGetErrorString(L.err));
         }
@@ -1913,9 +1917,9 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
     if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "synchronize: %s", hipGetErrorString(e)); }
     if ((rc = trace_collect())) { cleanup(); return rc; }
     float h2d = 0, fac = 0, d2h = 0;
-    hipEventElapsedTime(&h2d, e0, e1);
-    hipEventElapsedTime(&fac, e1, e2);
-    hipEventElapsedTime(&d2h, e2, e3);
+    (void)hipEventElapsedTime(&h2d, e0, e1);
+    (void)hipEventElapsedTime(&fac, e1, e2);
+    (void)hipEventElapsedTime(&d2h, e2, e3);
     prof_collect();
     if (rep) {
         const double n = N;
@@ -1944,7 +1948,7 @@ struct Stage {
     Stage(size_t by, bool d) : bytes(by), dev(d) {}
     ~Stage()
     {
-        for (double *p : owned) hipFree(p);
+        for (double *p : owned) (void)hipFree(p);
     }
     double *in(const double *h, bool copy_back)
     {
@@ -1952,7 +1956,7 @@ struct Stage {
         double *d = nullptr;
         if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
         owned.push_back(d);
-        hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g.s_main);
+        if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g.s_main) != hipSuccess) return nullptr;
         if (copy_back) back.push_back({(double *)h, d});
         return d;
     }
@@ -1965,7 +1969,7 @@ struct Stage {
     }
     int finish()
     {
-        for (auto &pr : back) hipMemcpyAsync(pr.first, pr.second, bytes, hipMemcpyDeviceToHost, g.s_main);
+        for (auto &pr : back) HIPCHK(hipMemcpyAsync(pr.first, pr.second, bytes, hipMemcpyDeviceToHost, g.s_main));
         HIPCHK(hipStreamSynchronize(g.s_main));
         return 0;
     }
@@ -2091,39 +2095,39 @@ int palial_hip_finalize(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g.init) return 0;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     drop_graphs();
     qr_streams_down();
-    for (auto e : g.events) hipEventDestroy(e);
+    for (auto e : g.events) (void)hipEventDestroy(e);
     g.events.clear();
-    if (g.d_w1) hipFree(g.d_w1);
-    if (g.d_w2) hipFree(g.d_w2);
-    if (g.d_qg) hipFree(g.d_qg);
-    if (g.d_tau) hipFree(g.d_tau);
-    if (g.d_dinv) hipFree(g.d_dinv);
-    if (g.d_sub) hipFree(g.d_sub);
-    if (g.d_pack) hipFree(g.d_pack);
-    if (g.d_F) hipFree(g.d_F);
-    if (g.d_inv) hipFree(g.d_inv);
-    if (g.d_F32) hipFree(g.d_F32);
-    if (g.d_gm) hipFree(g.d_gm);
-    if (g.d_bidx) hipFree(g.d_bidx);
-    if (g.d_bwork) hipFree(g.d_bwork);
-    if (g.d_vec) hipFree(g.d_vec);
-    if (g.d_aw) hipFree(g.d_aw);
+    if (g.d_w1) (void)hipFree(g.d_w1);
+    if (g.d_w2) (void)hipFree(g.d_w2);
+    if (g.d_qg) (void)hipFree(g.d_qg);
+    if (g.d_tau) (void)hipFree(g.d_tau);
+    if (g.d_dinv) (void)hipFree(g.d_dinv);
+    if (g.d_sub) (void)hipFree(g.d_sub);
+    if (g.d_pack) (void)hipFree(g.d_pack);
+    if (g.d_F) (void)hipFree(g.d_F);
+    if (g.d_inv) (void)hipFree(g.d_inv);
+    if (g.d_F32) (void)hipFree(g.d_F32);
+    if (g.d_gm) (void)hipFree(g.d_gm);
+    if (g.d_bidx) (void)hipFree(g.d_bidx);
+    if (g.d_bwork) (void)hipFree(g.d_bwork);
+    if (g.d_vec) (void)hipFree(g.d_vec);
+    if (g.d_aw) (void)hipFree(g.d_aw);
     for (auto &x : g_tr.singles) {
-        hipEventDestroy(x.a);
-        hipEventDestroy(x.e);
+        (void)hipEventDestroy(x.a);
+        (void)hipEventDestroy(x.e);
     }
-    if (g_tr.t0) hipEventDestroy(g_tr.t0);
-    if (g_tr.d_ts) hipFree(g_tr.d_ts);
+    if (g_tr.t0) (void)hipEventDestroy(g_tr.t0);
+    if (g_tr.d_ts) (void)hipFree(g_tr.d_ts);
     g_tr = TaskTrace();
-    if (g.d_qs) hipFree(g.d_qs);
-    if (g.d_info) hipFree(g.d_info);
-    if (g.s_bulk && g.s_bulk != g.s_main) hipStreamDestroy(g.s_bulk);
-    if (g.s_aux) hipStreamDestroy(g.s_aux);
-    hipStreamDestroy(g.s_main);
-    hipStreamDestroy(g.s_panel);
+    if (g.d_qs) (void)hipFree(g.d_qs);
+    if (g.d_info) (void)hipFree(g.d_info);
+    if (g.s_bulk && g.s_bulk != g.s_main) (void)hipStreamDestroy(g.s_bulk);
+    if (g.s_aux) (void)hipStreamDestroy(g.s_aux);
+    (void)hipStreamDestroy(g.s_main);
+    (void)hipStreamDestroy(g.s_panel);
     g = Ctx();
     return 0;
 }
@@ -2208,9 +2212,9 @@ int palial_hip_time_update(int algo, MATRIX_desc A, int k, int reps, double *ms)
     HIPCHK(hipEventRecord(e1, g.s_main));
     HIPCHK(hipEventSynchronize(e1));
     float t = 0;
-    hipEventElapsedTime(&t, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (ms) *ms = t / reps;
     return 0;
 }
@@ -2251,9 +2255,9 @@ int palial_hip_update(int algo, MATRIX_desc A, int k, int prec, int reps, double
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float t = 0;
-    hipEventElapsedTime(&t, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (ms) *ms = t / reps;
     return 0;
 }
@@ -2264,7 +2268,7 @@ namespace {
 int grow_bytes(void **p, size_t *cap, size_t need)
 {
     if (*cap >= need) return 0;
-    if (*p) hipFree(*p);
+    if (*p) (void)hipFree(*p);
     *p = nullptr;
     *cap = 0;
     if (hipMalloc(p, need) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "batch workspace of %zu bytes", need);
@@ -2339,7 +2343,7 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
         own = true;
         dA = dB = nullptr;
         if (hipMalloc(&dA, P.a_ext * es) != hipSuccess) return fail(PALIAL_ERR_NOMEM, "batch matrices");
-        if (B && hipMalloc(&dB, P.b_ext * es) != hipSuccess) { hipFree(dA); return fail(PALIAL_ERR_NOMEM, "batch vectors"); }
+        if (B && hipMalloc(&dB, P.b_ext * es) != hipSuccess) { (void)hipFree(dA); return fail(PALIAL_ERR_NOMEM, "batch vectors"); }
         HIPCHK(hipMemcpyAsync(dA, A, P.a_ext * es, hipMemcpyHostToDevice, s));
         if (B) HIPCHK(hipMemcpyAsync(dB, B, P.b_ext * es, hipMemcpyHostToDevice, s));
     }
@@ -2367,8 +2371,8 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
         const bool same = bg.exec && bg.count == count && bg.A == dA && bg.B == dB && bg.idx == g.d_bidx &&
                           bg.work == g.d_bwork && bg.f32 == a.f32 && bg.bmax == P.bmax && bg.grid == grid;
         if (!same) {
-            if (bg.exec) hipGraphExecDestroy(bg.exec);
-            if (bg.graph) hipGraphDestroy(bg.graph);
+            if (bg.exec) (void)hipGraphExecDestroy(bg.exec);
+            if (bg.graph) (void)hipGraphDestroy(bg.graph);
             bg = BatchGraph();
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const hipError_t le = launch_batch(a, P.bmax, grid, s);
@@ -2393,12 +2397,12 @@ int palial_hip_batched(int count, const int *n, const long *off, const int *kind
     }
     const hipError_t e = hipStreamSynchronize(s);
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (own) {
-        hipFree(dA);
-        if (dB) hipFree(dB);
+        (void)hipFree(dA);
+        if (dB) (void)hipFree(dB);
     }
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "batched factorization: %s", hipGetErrorString(e));
     if (rep) {
@@ -2438,7 +2442,7 @@ int palial_hip_batched_generate(int count, const int *n, const long *off, const 
     HIPCHK(launch_batch_generate(d_n, d_off, d_k, dA, count, prec == PALIAL_PREC_FP32, seed, s));
     if (!o.device_matrix) HIPCHK(hipMemcpyAsync(A, dA, P.a_ext * es, hipMemcpyDeviceToHost, s));
     const hipError_t e = hipStreamSynchronize(s);
-    if (!o.device_matrix) hipFree(dA);
+    if (!o.device_matrix) (void)hipFree(dA);
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "batch generate: %s", hipGetErrorString(e));
     return 0;
 }
@@ -2467,7 +2471,7 @@ int palial_hip_mlp_forward(int n, int nfeat, const float *X, const float *params
     HIPCHK(launch_mlp_forward(n, nfeat, dX, dP, dY, s));
     if (!o.device_matrix) HIPCHK(hipMemcpyAsync(y, dY, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     const hipError_t e = hipStreamSynchronize(s);
-    if (buf) hipFree(buf);
+    if (buf) (void)hipFree(buf);
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "cost-model forward: %s", hipGetErrorString(e));
     return 0;
 }
@@ -2491,7 +2495,7 @@ int palial_hip_mlp_train(int n, int nfeat, const float *X, const float *y, float
     hipStream_t s = g.s_main;
     const hipMemcpyKind in = o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipMemcpyKind out = o.device_matrix ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    auto done = [&](int code) { hipStreamSynchronize(s); hipFree(buf); return code; };
+    auto done = [&](int code) { (void)hipStreamSynchronize(s); (void)hipFree(buf); return code; };
     if (hipMemcpyAsync(bx, X, (size_t)n * nfeat * 4, in, s) != hipSuccess ||
         hipMemcpyAsync(by, y, (size_t)n * 4, in, s) != hipSuccess ||
         hipMemcpyAsync(bp, params, (size_t)np * 4, in, s) != hipSuccess ||
@@ -2499,11 +2503,13 @@ int palial_hip_mlp_train(int n, int nfeat, const float *X, const float *y, float
         return done(fail(PALIAL_ERR_HIP, "cost-model copies"));
     if (launch_mlp_train(n, nfeat, bx, by, bp, bm, bv, epochs, lr, bl, bg, s) != hipSuccess)
         return done(fail(PALIAL_ERR_HIP, "cost-model training launch"));
-    if (epochs > 0) hipMemcpyAsync(params, bp, (size_t)np * 4, out, s);
-    if (grad) hipMemcpyAsync(grad, bg, (size_t)np * 4, out, s);
-    if (loss) hipMemcpyAsync(loss, bl, (size_t)nl * 4, out, s);
-    const hipError_t e = hipStreamSynchronize(s);
-    hipFree(buf);
+    hipError_t e = hipSuccess;
+    if (epochs > 0) e = hipMemcpyAsync(params, bp, (size_t)np * 4, out, s);
+    if (grad && e == hipSuccess) e = hipMemcpyAsync(grad, bg, (size_t)np * 4, out, s);
+    if (loss && e == hipSuccess) e = hipMemcpyAsync(loss, bl, (size_t)nl * 4, out, s);
+    const hipError_t es = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = es;
+    (void)hipFree(buf);
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "cost-model training: %s", hipGetErrorString(e));
     return 0;
 }
@@ -2520,8 +2526,8 @@ int palial_hip_free(void *p)
     // graphs captured on this pointer are no longer valid
     for (auto it = g.graphs.begin(); it != g.graphs.end();) {
         if (std::get<1>(it->first) == p || std::get<2>(it->first) == p) {
-            if (it->second.exec) hipGraphExecDestroy(it->second.exec);
-            if (it->second.graph) hipGraphDestroy(it->second.graph);
+            if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
+            if (it->second.graph) (void)hipGraphDestroy(it->second.graph);
             it = g.graphs.erase(it);
         } else {
             ++it;
@@ -2579,7 +2585,7 @@ int palial_hip_generate(MATRIX_desc A, int kind, uint64_t seed, double shift, co
     hipError_t e = launch_generate(d, N, b, kind, seed, shift, s);
     if (e == hipSuccess && !o.device_matrix) e = hipMemcpyAsync(A.matrix, d, bytes, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (!o.device_matrix) hipFree(d);
+    if (!o.device_matrix) (void)hipFree(d);
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "generate: %s", hipGetErrorString(e));
     return 0;
 }
@@ -2604,9 +2610,9 @@ int palial_hip_generate_kron(MATRIX_desc A, const double *A1, int n1, const doub
     if (e == hipSuccess) e = launch_kron(d, N, b, d1, n1, d2, n2, g.s_main);
     if (e == hipSuccess && !o.device_matrix) e = hipMemcpyAsync(A.matrix, d, bytes, hipMemcpyDeviceToHost, g.s_main);
     if (e == hipSuccess) e = hipStreamSynchronize(g.s_main);
-    hipFree(d1);
-    hipFree(d2);
-    if (!o.device_matrix) hipFree(d);
+    (void)hipFree(d1);
+    (void)hipFree(d2);
+    if (!o.device_matrix) (void)hipFree(d);
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "generate_kron: %s", hipGetErrorString(e));
     return 0;
 }
@@ -2694,8 +2700,8 @@ int palial_hip_qr_apply(int side, int trans, MATRIX_desc A, MATRIX_desc S, doubl
     hipStream_t s = (dev && o.stream) ? (hipStream_t)o.stream : g.s_main;
     std::vector<double *> owned;
     auto release = [&](int code) {
-        hipStreamSynchronize(s);
-        for (double *p : owned) hipFree(p);
+        (void)hipStreamSynchronize(s);
+        for (double *p : owned) (void)hipFree(p);
         return code;
     };
     double *dA, *dS, *dC = C;
@@ -2718,9 +2724,9 @@ int palial_hip_qr_apply(int side, int trans, MATRIX_desc A, MATRIX_desc S, doubl
     if (er == hipSuccess && !dev) er = hipMemcpyAsync(C, dC, cbytes, hipMemcpyDeviceToHost, s);
     if (er == hipSuccess) er = hipStreamSynchronize(s);
     float ms = 0;
-    if (er == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (er == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     prof_collect();
     if (L.err != hipSuccess) return release(fail(PALIAL_ERR_HIP, "Q application launch: %s", hipGetErrorString(L.err)));
     if (er != hipSuccess) return release(fail(PALIAL_ERR_HIP, "Q application: %s", hipGetErrorString(er)));
@@ -2750,8 +2756,8 @@ int palial_hip_qr_solve(MATRIX_desc A, MATRIX_desc S, double *B, int nrhs, int l
     hipStream_t s = (dev && o.stream) ? (hipStream_t)o.stream : g.s_main;
     std::vector<double *> owned;
     auto release = [&](int code) {
-        hipStreamSynchronize(s);
-        for (double *p : owned) hipFree(p);
+        (void)hipStreamSynchronize(s);
+        for (double *p : owned) (void)hipFree(p);
         return code;
     };
     double *dA, *dS;
@@ -2801,9 +2807,9 @@ int palial_hip_qr_solve(MATRIX_desc A, MATRIX_desc S, double *B, int nrhs, int l
                               dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s);
     if (er == hipSuccess) er = hipStreamSynchronize(s);
     float ms = 0;
-    if (er == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (er == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     prof_collect();
     if (L.err != hipSuccess) return release(fail(PALIAL_ERR_HIP, "QR solve launch: %s", hipGetErrorString(L.err)));
     if (er != hipSuccess) return release(fail(PALIAL_ERR_HIP, "QR solve: %s", hipGetErrorString(er)));
@@ -2833,8 +2839,8 @@ int palial_hip_transpose(MATRIX_desc A, int average, const palial_opts *o_in)
     if (e == hipSuccess) e = launch_transpose(d, N, b, average != 0, s);
     if (e == hipSuccess && !o.device_matrix) e = hipMemcpyAsync(A.matrix, d, bytes, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    else hipStreamSynchronize(s);
-    if (!o.device_matrix) hipFree(d);
+    else (void)hipStreamSynchronize(s);
+    if (!o.device_matrix) (void)hipFree(d);
     if (e != hipSuccess) return fail(PALIAL_ERR_HIP, "transpose: %s", hipGetErrorString(e));
     return 0;
 }
@@ -2849,12 +2855,12 @@ int palial_hip_dpotrf(int n, double *A, int lda, const palial_opts *opts)
     Stage st((size_t)n * n * 8, opts && opts->device_matrix);
     double *d = st.in(A, true);
     if (!d) return fail(PALIAL_ERR_NOMEM, "tile alloc");
-    hipMemsetAsync(g.d_info, 0, sizeof(int), g.s_main);
+    HIPCHK(hipMemsetAsync(g.d_info, 0, sizeof(int), g.s_main));
     if ((rc = ensure_dinv(n, 1))) return rc;
     if (panel2_ok(n)) HIPCHK(launch_potrf2(d, n, g.d_info, 0, g.d_dinv, g.s_main));
     else HIPCHK(launch_potrf(d, n, g.d_info, 0, g.s_main));
     int info = 0;
-    hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, g.s_main);
+    HIPCHK(hipMemcpyAsync(&info, g.d_info, sizeof(int), hipMemcpyDeviceToHost, g.s_main));
     if ((rc = st.finish())) return rc;
     return info;
 }
@@ -2891,7 +2897,7 @@ int palial_hip_dgeqrt(int n, int nb, double *A, int lda, double *T, int ldt, dou
     double *dG = st.scratch((size_t)n * n * 8);
     if (!dG) return fail(PALIAL_ERR_NOMEM, "tile alloc");
     HIPCHK(launch_geqrt(dA, dT, dtau, n, g.s_main, dG));
-    if (!dev && tau) hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main);
+    if (!dev && tau) HIPCHK(hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main));
     return st.finish();
 }
 
@@ -2936,7 +2942,7 @@ int palial_hip_dtsqrt(int n, double *A1, int lda1, double *A2, int lda2, double 
     double *dG = st.scratch((size_t)n * n * 8);
     if (!dG) return fail(PALIAL_ERR_NOMEM, "tile alloc");
     HIPCHK(launch_tsqrt(d1, d2, dT, dtau, n, g.s_main, dG));
-    if (!dev && tau) hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main);
+    if (!dev && tau) HIPCHK(hipMemcpyAsync(tau, dtau, (size_t)n * 8, hipMemcpyDeviceToHost, g.s_main));
     return st.finish();
 }
 
