@@ -79,7 +79,12 @@ typedef struct palial_opts {
                              QR: 1..8 panel chains in flight (steps progressing together) */
     int   batch_grid;     /* palial_hip_batched: workgroups of the persistent launch (0 = as many as
                              the CUs hold at once) */
-    int   reserved[4];
+    int   sweep_block;    /* solves: tiles per step of the triangular correction sweeps, which apply
+                             precomputed inverses of the diagonal blocks (0 = the library default,
+                             PALIAL_SWEEP_BLOCK or 1; 1 = one tile; blocks of q > 1 tiles need q <= 8
+                             dividing N / tile with N / tile >= 2q, <= 4 right-hand sides and
+                             q * tile * (4 when nrhs > 1) <= 4096) */
+    int   reserved[3];
 } palial_opts;
 
 /* per-call report; any pointer may be NULL */
@@ -159,6 +164,14 @@ long palial_hip_trace_records(double *out, long cap);
  * 11 tsmqr(k,m,n). */
 long palial_hip_task_name(int algo, int MT, long index, char *buf, int len);
 
+/* Schedule settings that palial_hip_init reads from the environment, changed
+ * at run time (no counterpart in the reference; for tests and tuners):
+ * "pair" (PALIAL_PAIR: 1 pairs the trailing updates of consecutive steps of
+ * the BF16 Cholesky and LU solves), "sweep_block" (PALIAL_SWEEP_BLOCK, 1..8).  Returns
+ * the previous value, or PALIAL_ERR_ARG for an unknown name or a value out of
+ * range.  Calls are serialised with the library's other calls. */
+int  palial_hip_set_option(const char *name, int value);
+
 /* Test hook (no counterpart in the reference): enumerate the tile tasks of
  * one batched trailing-update launch (mode = internal GEMM mode, step k, TS
  * row m for the QR modes) without touching the GPU.  Writes 3 longs per task:
@@ -166,6 +179,32 @@ long palial_hip_task_name(int algo, int MT, long index, char *buf, int len);
  * -2 a workspace or T-factor tile).  Returns the task count, -1 if cap is too
  * small. */
 int palial_hip_task_tiles(int mode, int MT, int b, int k, int m, long *out, int cap);
+
+/* Test hooks of the blocked correction sweeps (no counterpart in the
+ * reference), host views of what the solve launches, without the GPU:
+ *  - palial_hip_sweep_plan: the tile_gemv launches of the two sweeps of a
+ *    solve (algo 0 Cholesky, 1 LU; MT tiles per side, q tiles per step) in
+ *    issue order, 7 ints each: mode (0 diagonal inverse, 1 Cholesky forward,
+ *    2 LU forward, 3 backward), step, transposed, tasks, inverse set (0 U-side,
+ *    1 L-side), input and output vector (0 r, 1 y, 2 x);
+ *  - palial_hip_gemv_tiles: for each task of one such launch: output tile, the
+ *    source range j0, j1, then q source-tile element offsets from the inverse
+ *    base (mode 0) or the matrix base (-1 outside [j0, j1)): 3 + q longs;
+ *  - palial_hip_gather_map: for each tile (K, i, j) of the q x q blocks
+ *    (K-major, then i, then j): kind (0 zero, 1 tile inverse, 2 factor tile),
+ *    transposed, source offset, destination offset: 4 longs;
+ *  - palial_hip_blkinv_plan: the batched GEMM launches that complete the
+ *    block inverses, 5 doubles each: mode, level, term, alpha, beta;
+ *  - palial_hip_blkinv_tiles: for each task of one such launch: buffer (0 the
+ *    inverses X, 1 the tiles of T, 2 the products W; -1 none) and element
+ *    offset of the A, B, C and D operands: 8 longs.
+ * Each returns the number of records (steps, tasks, tiles or launches), or
+ * PALIAL_ERR_ARG when cap is too small or an argument is out of range. */
+int palial_hip_sweep_plan(int algo, int MT, int q, int *out, int cap);
+int palial_hip_gemv_tiles(int mode, int trans, int MT, int b, int q, int k, int ntasks, long *out, long cap);
+int palial_hip_gather_map(int MT, int b, int q, int upper, int dtrans, long *out, long cap);
+int palial_hip_blkinv_plan(int q, double *out, int cap);
+int palial_hip_blkinv_tiles(int mode, int MT, int b, int q, int bd, int bt, long *out, long cap);
 
 /* Test hook (no counterpart in the reference): average device time of one
  * batched trailing-update launch of step k (algo 0 Cholesky, 1 LU) over reps

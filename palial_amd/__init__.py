@@ -44,7 +44,8 @@ class Opts(C.Structure):
     _fields_ = [("device_matrix", C.c_int), ("precision", C.c_int), ("no_graph", C.c_int),
                 ("no_lookahead", C.c_int), ("stream", C.c_void_p), ("refine", C.c_int),
                 ("max_iter", C.c_int), ("tol", C.c_double), ("factor_tile", C.c_int), ("no_fallback", C.c_int),
-                ("schedule", C.c_int), ("batch_grid", C.c_int), ("reserved", C.c_int * 4)]
+                ("schedule", C.c_int), ("batch_grid", C.c_int), ("sweep_block", C.c_int),
+                ("reserved", C.c_int * 3)]
 
 
 class Report(C.Structure):
@@ -78,6 +79,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         dp = C.c_void_p
         L.palial_hip_init.argtypes = [C.c_int]
+        L.palial_hip_set_option.argtypes = [C.c_char_p, C.c_int]
         L.palial_hip_strerror.restype = C.c_char_p
         L.palial_hip_strerror.argtypes = [C.c_int]
         L.palial_hip_last_error.restype = C.c_char_p
@@ -155,9 +157,11 @@ def finalize():
 
 
 def _opts(device=False, no_graph=False, no_lookahead=False, precision=PREC_FP64, stream=None,
-          refine=0, max_iter=0, tol=0.0, factor_tile=0, no_fallback=False, schedule=0, batch_grid=0):
+          refine=0, max_iter=0, tol=0.0, factor_tile=0, no_fallback=False, schedule=0, batch_grid=0,
+          sweep_block=0):
     o = Opts()
     o.schedule = schedule
+    o.sweep_block = sweep_block
     o.batch_grid = batch_grid
     o.factor_tile = factor_tile
     o.no_fallback = int(bool(no_fallback))
@@ -504,6 +508,15 @@ class Batch:
         self.A = self.B = None
 
 
+def set_option(name, value):
+    """Change a schedule setting that init reads from the environment ("pair":
+    PALIAL_PAIR, "sweep_block": PALIAL_SWEEP_BLOCK); returns the previous value."""
+    init()
+    rc = lib().palial_hip_set_option(name.encode(), int(value))
+    _check(rc)
+    return rc
+
+
 def trace_enable(on=True):
     """Per-task device timing of the following factorizations (palial_hip_trace)."""
     init()
@@ -560,7 +573,7 @@ def factor_device(algo, dm, S=None, **kw):
 
 # ------------------------------------------------------------------ solve
 def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, max_iter=0, device=False,
-          check=True, factor_tile=0, refine=0, no_fallback=False):
+          check=True, factor_tile=0, refine=0, no_fallback=False, sweep_block=0):
     """Solve A X = B with iterative refinement (no reference counterpart).
 
     A: layout-L matrix (numpy flat array, torch tensor or DeviceMatrix), left
@@ -568,14 +581,15 @@ def solve(A, B, algo="cholesky", precision=PREC_FP64, b=None, N=None, tol=0.0, m
     or a device pointer with device=True).  precision: PREC_FP64, PREC_FP32,
     PREC_BF16, PREC_FP8 (E4M3 panels with a power-of-two scale per tile) or
     PREC_BF16X3 (split BF16); the last two need a factor tile that is a
-    multiple of 128 and smaller than N.  Returns (X, report, rc)."""
+    multiple of 128 and smaller than N.  sweep_block: tiles per step of the
+    correction sweeps (0 automatic, 1 one tile).  Returns (X, report, rc)."""
     init()
     if isinstance(A, DeviceMatrix):
         d, dev = A.desc, True
     else:
         d, dev, keep = _desc_of(A, N, b)
     o = _opts(device=bool(dev), precision=precision, tol=tol, max_iter=max_iter, factor_tile=factor_tile,
-              refine=refine, no_fallback=no_fallback)
+              refine=refine, no_fallback=no_fallback, sweep_block=sweep_block)
     rep = Report()
     code = {"cholesky": 0, "lu": 1}[algo]
     if dev and not isinstance(B, np.ndarray):

@@ -71,6 +71,8 @@ struct Ctx {
                                     // the reserved CUs (s_main when nothing is reserved)
     hipStream_t s_aux = nullptr;    // chain schedule with the split update: the row/column
                                     // TRSMs and panel packing (CU mask of s_bulk)
+    hipStream_t s_chainm = nullptr; // PALIAL_CHAIN_CUS=1: the diagonal chain of the mixed tiers
+                                    // masked to the reserved CUs (nullptr: s_panel, all CUs)
     hipStream_t s_qr[8] = {};       // QR panel-chain streams (all CUs)
     hipStream_t s_qrb[8] = {};      // QR update streams: CU-masked away from the first qr_reserve CUs
     int qr_streams = 8;             // QR: panel chains / update streams in use (1..8)
@@ -94,13 +96,17 @@ struct Ctx {
     size_t sub_cap = 0;             // doubles
     int subtile = 1;                // factor diagonal tiles of b >= 256 as 128-sub-tiles
     int f32work = 1;                // mixed-tier solves factor an FP32 working copy
-    void *d_pack = nullptr;         // low-precision packed panels of two steps (U then L each)
+    void *d_pack = nullptr;         // low-precision packed panels of PACK_SETS steps (U then L each)
     size_t pack_cap = 0;            // bytes
     // solve workspace: factor copy F, diagonal-tile inverses, vectors
     double *d_F = nullptr;
     size_t F_cap = 0;
     double *d_inv = nullptr;
     size_t inv_cap = 0;
+    double *d_blk = nullptr;        // block inverses of the blocked sweeps: tiles of T and products
+    size_t blk_cap = 0;
+    int sweep_block = 1;            // default tiles per correction-sweep step
+    int pair = 0;                   // BF16 Cholesky / LU solves: paired trailing updates
     BatchGraph bgraph;              // batched factorizations: the captured persistent launch
     void *d_bidx = nullptr;         // batched factorizations: per-matrix index arrays
     size_t bidx_cap = 0;            // bytes
@@ -242,11 +248,14 @@ int ensure_dinv(int b, int MT)
 // set k % 2): row panel at 0, column panel at MT*b*b*4 bytes of a set, 4 bytes
 // per element covering FP32 and BF16
 size_t pack_set_bytes(int b, int MT) { return 2 * (size_t)MT * b * b * 4; }
+// step sets of packed panels: the split schedule packs step k into set k % 4
+// (paired Cholesky steps read two sets while the next two are packed)
+constexpr int PACK_SETS = 4;
 
 int ensure_pack(int b, int MT)
 {
-    // two step sets, then the one or two packed tiles of the chain's S1 (FP32 working copy)
-    const size_t need = 2 * pack_set_bytes(b, MT) + 2 * (size_t)b * b * 4;
+    // the step sets, then the one or two packed tiles of the chain's S1 (FP32 working copy)
+    const size_t need = PACK_SETS * pack_set_bytes(b, MT) + 2 * (size_t)b * b * 4;
     if (g.pack_cap < need) {
         drop_graphs();
         if (g.d_pack) (void)hipFree(g.d_pack);
@@ -800,15 +809,15 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
     const bool lp = prec > 0 && b % 128 == 0;
     const bool split = sa != nullptr;
     const size_t psb = pack_set_bytes(b, MT);
-    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * psb;
-    if (fs && !(pk && split && v2 && g.pack_cap >= 2 * psb + 2 * (size_t)b * b * 4)) {
+    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? PACK_SETS : 1) * psb;
+    if (fs && !(pk && split && v2 && g.pack_cap >= PACK_SETS * psb + 2 * (size_t)b * b * 4)) {
         L.chk(hipErrorInvalidValue);     // the FP32 working copy needs the packed split schedule
         return;
     }
     const double esz = prec == 2 ? 2.0 : prec == 3 ? 1.0 : 4.0;   // packed bytes per element (split BF16: hi + lo)
     const double sbt = fs ? 4.0 * b * b : tile_bytes(b);      // bytes of one stored tile
-    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * psb : 0); };
-    char *pk1 = (char *)g.d_pack + 2 * psb;                    // S1's packed tile (fs)
+    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k % PACK_SETS) * psb : 0); };
+    char *pk1 = (char *)g.d_pack + PACK_SETS * psb;            // S1's packed tile (fs)
     auto tile = [&](int m, int n) -> double * {
         return fs ? (double *)(void *)((float *)(void *)dA + tile_off(b, MT, m, n)) : dA + tile_off(b, MT, m, n);
     };
@@ -839,23 +848,34 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
     };
     // trailing update of step k over tasks [task0, count) of `mode`: ngemm GEMM
     // and nsyrk SYRK tiles, reading npanel tiles of the packed row k
+    // panel2: the packed row k+1 as well (GM_CHOL_REST2: twice the products per output tile)
     auto update = [&](int mode, int k, int task0, int ntasks, double ngemm, double nsyrk, double npanel, void *panel,
-                      hipStream_t s) {
+                      hipStream_t s, void *panel2 = nullptr) {
         const int T = MT - k - 1;
         GemmArgs a = gargs(mode, dA, N, b, k, T);
         a.task0 = task0;
         a.ntasks = ntasks;
         a.f32c = fs ? 1 : 0;
         if (a.ntasks <= 0) return;
-        const double fl = 2 * b3(b) * ngemm + b3(b) * nsyrk;
+        const double np = panel2 ? 2.0 : 1.0;
+        const double fl = np * (2 * b3(b) * ngemm + b3(b) * nsyrk);
         if (pk) {
             a.pa = a.pb = panel;
-            L.run(KC_PKGEMM, fl, 2.0 * (ngemm + nsyrk) * sbt + npanel * esz * b * b, s,
+            a.pa2 = a.pb2 = panel2;
+            L.run(KC_PKGEMM, fl, 2.0 * (ngemm + nsyrk) * sbt + np * npanel * esz * b * b, s,
                   [&] { return launch_gemm_packed(a, prec, s); });
         } else {
             gemm(a, fl, ngemm + nsyrk, s);
         }
     };
+    // Paired steps (BF16 on the FP32 working copy): an even step j with at
+    // least 4 trailing tiles leaves the tiles (n, m), n >= j+3 of its update
+    // to step j+1, which applies panels j and j+1 to them in one pass, so
+    // those C tiles are read and written once for two steps.  Step j updates
+    // rows j+1 and j+2 (GM_CHOL_FIRST, GM_CHOL_ROW2): the chain's next two
+    // steps never wait for a full trailing update, only every other one.
+    const bool pair = fs && prec == 2 && split && pk && g.pair && !g_tr.active;
+    auto lead = [&](int k) { return pair && k % 2 == 0 && MT - k - 1 >= 4; };
     hipEvent_t *ev = g.events.data();
     hipEvent_t *evP = ev, *evR = ev + MT + 1, *evT = ev + 2 * MT + 4, *evB = ev + 3 * MT + 6;
     hipEvent_t fork = ev[2 * MT + 2], evEnd = ev[4 * MT + 7];
@@ -904,7 +924,17 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
         } else {
             if (T > 1) update(GM_CHOL_FIRST, k, 1, T, T - 1, 1, T, pset(k), sb);
             L.chk(hipEventRecord(evR[k], sb));
-            if (T > 2) update(GM_CHOL_REST, k, 1, (T - 1) * T / 2 - 1, (double)(T - 1) * (T - 2) / 2, T - 2, T - 1, pset(k), sb);
+            if (lead(k)) {
+                // row k+2 and tile (k+3, k+3); the rest waits for step k+1
+                update(GM_CHOL_ROW2, k, 0, T - 1, T - 2, 1, T - 1, pset(k), sb);
+            } else if (k > 0 && lead(k - 1)) {
+                // steps k-1 and k on the tiles (n, m), k+2 <= n <= m, but (k+2, k+2)
+                const int R = T - 1;          // rows k+2 .. MT-1
+                update(GM_CHOL_REST2, k - 1, 1, R * (R + 1) / 2 - 1, (double)R * (R - 1) / 2, R - 1, R, pset(k - 1),
+                       sb, pset(k));
+            } else if (T > 2) {
+                update(GM_CHOL_REST, k, 1, (T - 1) * T / 2 - 1, (double)(T - 1) * (T - 2) / 2, T - 2, T - 1, pset(k), sb);
+            }
         }
     }
     L.chk(hipEventRecord(evEnd, sb));
@@ -927,15 +957,15 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
     const bool lp = prec > 0 && b % 128 == 0;
     const bool split = sa != nullptr;
     const size_t psb = pack_set_bytes(b, MT);
-    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? 2 : 1) * psb;
-    if (fs && !(pk && split && v2 && g.pack_cap >= 2 * psb + 2 * (size_t)b * b * 4)) {
+    const bool pk = lp && g.d_pack && g.pack_cap >= (split ? PACK_SETS : 1) * psb;
+    if (fs && !(pk && split && v2 && g.pack_cap >= PACK_SETS * psb + 2 * (size_t)b * b * 4)) {
         L.chk(hipErrorInvalidValue);
         return;
     }
     const double esz = prec == 2 ? 2.0 : prec == 3 ? 1.0 : 4.0;   // packed bytes per element (split BF16: hi + lo)
     const double sbt = fs ? 4.0 * b * b : tile_bytes(b);
-    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k & 1) * psb : 0); };
-    char *pk1U = (char *)g.d_pack + 2 * psb, *pk1L = pk1U + (size_t)b * b * 4;   // S1's packed tiles (fs)
+    auto pset = [&](int k) { return (char *)g.d_pack + (split ? (size_t)(k % PACK_SETS) * psb : 0); };
+    char *pk1U = (char *)g.d_pack + PACK_SETS * psb, *pk1L = pk1U + (size_t)b * b * 4;   // S1's packed tiles (fs)
     auto tile = [&](int m, int n) -> double * {
         return fs ? (double *)(void *)((float *)(void *)dA + tile_off(b, MT, m, n)) : dA + tile_off(b, MT, m, n);
     };
@@ -968,23 +998,30 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
     };
     // trailing update of step k over tasks [task0, task0 + ntasks) of `mode`,
     // reading npanel tiles of each packed panel
-    auto update = [&](int mode, int k, int task0, int ntasks, double npanel, char *pU, char *pL, hipStream_t s) {
+    // pU2 / pL2: the packed panels of step k+1 as well (GM_LU_REST2)
+    auto update = [&](int mode, int k, int task0, int ntasks, double npanel, char *pU, char *pL, hipStream_t s,
+                      char *pU2 = nullptr, char *pL2 = nullptr) {
         const int T = MT - k - 1;
         GemmArgs a = gargs(mode, dA, N, b, k, T);
         a.task0 = task0;
         a.ntasks = ntasks;
         a.f32c = fs ? 1 : 0;
         if (a.ntasks <= 0) return;
-        const double nt = a.ntasks;
+        const double nt = a.ntasks, np = pU2 ? 2.0 : 1.0;
         if (pk) {
             a.pa = pL;
             a.pb = pU;
-            L.run(KC_PKGEMM, 2 * b3(b) * nt, 2.0 * nt * sbt + 2.0 * npanel * esz * b * b, s,
+            a.pa2 = pL2;
+            a.pb2 = pU2;
+            L.run(KC_PKGEMM, np * 2 * b3(b) * nt, 2.0 * nt * sbt + np * 2.0 * npanel * esz * b * b, s,
                   [&] { return launch_gemm_packed(a, prec, s); });
         } else {
             gemm(a, nt, s);
         }
     };
+    // paired steps as in the Cholesky chain schedule
+    const bool pair = fs && prec == 2 && split && pk && g.pair && !g_tr.active;
+    auto lead = [&](int k) { return pair && k % 2 == 0 && MT - k - 1 >= 4; };
     auto psetU = [&](int k) { return pset(k); };
     auto psetL = [&](int k) { return pset(k) + (size_t)MT * b * b * 4; };
     hipEvent_t *ev = g.events.data();
@@ -1038,7 +1075,16 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
         } else {
             if (T > 1) update(GM_LU_FIRST, k, 1, 2 * T - 1, T, psetU(k), psetL(k), sb);
             L.chk(hipEventRecord(evR[k], sb));
-            if (T > 2) update(GM_LU_REST, k, 1, (T - 1) * (T - 1) - 1, T - 1, psetU(k), psetL(k), sb);
+            if (lead(k)) {
+                // column and row k+2 and tile (k+3, k+3); the rest waits for step k+1
+                update(GM_LU_ROW2, k, 0, 2 * T - 3, T - 1, psetU(k), psetL(k), sb);
+            } else if (k > 0 && lead(k - 1)) {
+                // steps k-1 and k on the tiles (m, n), m, n >= k+2, but (k+2, k+2)
+                const int R = T - 1;
+                update(GM_LU_REST2, k - 1, 1, R * R - 1, R, psetU(k - 1), psetL(k - 1), sb, psetU(k), psetL(k));
+            } else if (T > 2) {
+                update(GM_LU_REST, k, 1, (T - 1) * (T - 1) - 1, T - 1, psetU(k), psetL(k), sb);
+            }
         }
     }
     L.chk(hipEventRecord(evEnd, sb));
@@ -1053,6 +1099,12 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
 hipStream_t bulk_stream(int prec, hipStream_t s0)
 {
     return prec > 0 ? g.s_bulk : s0;
+}
+// Diagonal-chain stream: the mixed tiers may confine it to the reserved CUs,
+// where its workgroups never share a CU with the bulk update's.
+hipStream_t chain_stream(int prec, hipStream_t s1)
+{
+    return prec > 0 && g.s_chainm ? g.s_chainm : s1;
 }
 
 // Cholesky, src/cholesky.h:15-176.  Look-ahead: step k's update of tile row
@@ -1095,8 +1147,8 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
               [&] { return lp ? launch_gemm_lp(a, true, false, prec, s) : launch_gemm(a, true, false, s); });
     };
     if (lookahead && g.dag >= 1) {
-        dag_cholesky_chain(dA, N, b, s0, s1, bulk_stream(prec, s0), g.dag == 2 ? g.s_aux : nullptr, dinfo, L, prec,
-                           fs);
+        dag_cholesky_chain(dA, N, b, s0, chain_stream(prec, s1), bulk_stream(prec, s0), g.dag == 2 ? g.s_aux : nullptr,
+                           dinfo, L, prec, fs);
         return;
     }
     if (!lookahead) {
@@ -1165,7 +1217,8 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
               [&] { return lp ? launch_gemm_lp(a, false, false, prec, s) : launch_gemm(a, false, false, s); });
     };
     if (lookahead && g.dag >= 1) {
-        dag_lu_chain(dA, N, b, s0, s1, bulk_stream(prec, s0), g.dag == 2 ? g.s_aux : nullptr, L, prec, fs);
+        dag_lu_chain(dA, N, b, s0, chain_stream(prec, s1), bulk_stream(prec, s0), g.dag == 2 ? g.s_aux : nullptr, L,
+                     prec, fs);
         return;
     }
     if (!lookahead) {
@@ -1400,35 +1453,90 @@ int grow(double **p, size_t *cap, size_t need)
 }
 
 // Triangular sweeps with the factored tiles F and the inverted diagonal tiles
-// (Cholesky: Uinv holds U_kk^-T; LU: U_kk^-1 and L_kk^-1):
+// (q = 1; Cholesky: Uinv holds U_kk^-T; LU: U_kk^-1 and L_kk^-1) or blocks of
+// q tiles (q > 1; lower triangular: Cholesky (U^T)^-1, LU L^-1 and (U^T)^-1):
 //   Cholesky: U^T y = r, U x = y;  LU: L y = r, U x = y.
+// Each step solves with one inverse (a product) and updates the rest of the
+// vector from the step's q tile columns (forward) or rows (backward).
 // r is destroyed (used as the running right-hand side of the forward sweep).
 // f32: F, Uinv, Linv point at FP32 copies (mixed-precision correction solves)
+int sweep_plan(int algo, int MT, int q, SweepStep *out, int cap)
+{
+    const int KT = MT / q;
+    int n = 0;
+    auto put = [&](int mode, int k, int trans, int ntasks, int inv, int vin, int vout) {
+        if (n < cap && out) out[n] = SweepStep{mode, k, trans, ntasks, inv, vin, vout};
+        n++;
+    };
+    // forward: y_k = inv_k r_k, then r_i -= op(F) y_k below the step
+    for (int k = 0; k < KT; k++) {
+        put(GV_DIAG, k, 0, q, algo == 0 ? 0 : 1, 0, 1);
+        if (k + 1 < KT) put(algo == 0 ? GV_CHOL_FWD : GV_LU_FWD, k, algo == 0, MT - (k + 1) * q, 0, 1, 0);
+    }
+    // backward: x_k = inv_k y_k (Cholesky, and LU with blocks: (U^T)^-1 read transposed),
+    // then y_i -= F x_k above the step
+    for (int k = KT - 1; k >= 0; k--) {
+        put(GV_DIAG, k, (algo == 0 || q > 1) ? 1 : 0, q, 0, 1, 2);
+        if (k > 0) put(GV_BWD, k, 0, k * q, 0, 2, 1);
+    }
+    return n;
+}
+
 void sweeps(int algo, const void *F, const void *Uinv, const void *Linv, double *r, double *y, double *x,
-            int N, int b, int nrhs, bool f32, hipStream_t s, Launcher &L)
+            int N, int b, int nrhs, bool f32, hipStream_t s, Launcher &L, int q = 1)
 {
     const int MT = N / b;
+    std::vector<SweepStep> plan((size_t)sweep_plan(algo, MT, q, nullptr, 0));
+    sweep_plan(algo, MT, q, plan.data(), (int)plan.size());
+    double *vec[3] = {r, y, x};
     GemvArgs a;
     memset(&a, 0, sizeof a);
-    a.mat = F; a.N = N; a.b = b; a.ld = N; a.nrhs = nrhs; a.f32 = f32 ? 1 : 0;
-    for (int k = 0; k < MT; k++) {
-        a.k = k;
-        a.mode = GV_DIAG; a.inv = (algo == 0) ? Uinv : Linv; a.trans = 0; a.vin = r; a.vout = y; a.ntasks = 1;
+    a.mat = F; a.N = N; a.b = b; a.ld = N; a.nrhs = nrhs; a.f32 = f32 ? 1 : 0; a.q = q;
+    for (const SweepStep &st : plan) {
+        a.mode = st.mode; a.k = st.k; a.trans = st.trans; a.ntasks = st.ntasks;
+        a.inv = st.inv ? Linv : Uinv;
+        a.vin = vec[st.vin];
+        a.vout = vec[st.vout];
         L.chk(launch_tile_gemv(a, s));
-        if (k + 1 < MT) {
-            a.mode = (algo == 0) ? GV_CHOL_FWD : GV_LU_FWD; a.trans = (algo == 0); a.vin = y; a.vout = r;
-            a.ntasks = MT - k - 1;
-            L.chk(launch_tile_gemv(a, s));
-        }
     }
-    for (int k = MT - 1; k >= 0; k--) {
-        a.k = k;
-        a.mode = GV_DIAG; a.inv = Uinv; a.trans = (algo == 0); a.vin = y; a.vout = x; a.ntasks = 1;
-        L.chk(launch_tile_gemv(a, s));
-        if (k > 0) {
-            a.mode = GV_BWD; a.trans = 0; a.vin = x; a.vout = y; a.ntasks = k;
-            L.chk(launch_tile_gemv(a, s));
+}
+
+// Inverses of the q x q tile blocks on the diagonal of the lower triangular
+// T (upper: T = F^T from F's upper tiles; else F's unit lower tiles), into X
+// (block k's tile (i, j) at ((kq + i) q + j) b^2) from the tile inverses D
+// (dtrans: D holds the inverses of F's upper diagonal tiles, X takes their
+// transposes).  Level by level (i = j + d):
+//   X(i, j) = -X(i, i) sum_{m=j}^{i-1} T(i, m) X(m, j)
+// one batched GEMM launch per term and one for the product with X(i, i).
+// W1: q MT tiles (the tiles of T, same offsets as X); W2: (q-1) MT / q tiles.
+int blkinv_plan(int q, BlkInvLaunch *out, int cap)
+{
+    int n = 0;
+    for (int d = 1; d < q; d++) {
+        for (int t = 0; t < d; t++) {
+            if (n < cap && out) out[n] = BlkInvLaunch{GM_BI_W, d, t, 1.0, t > 0 ? 1.0 : 0.0};
+            n++;
         }
+        if (n < cap && out) out[n] = BlkInvLaunch{GM_BI_X, d, 0, -1.0, 0.0};
+        n++;
+    }
+    return n;
+}
+
+void block_inverses(const void *F, bool f32, int N, int b, int q, bool upper, bool dtrans, const double *D,
+                    double *X, double *W1, double *W2, hipStream_t s, Launcher &L)
+{
+    L.chk(launch_blk_gather(F, f32, N, b, q, upper, dtrans, D, X, W1, s));
+    std::vector<BlkInvLaunch> plan((size_t)blkinv_plan(q, nullptr, 0));
+    blkinv_plan(q, plan.data(), (int)plan.size());
+    GemmArgs a;
+    memset(&a, 0, sizeof a);
+    a.mat = X; a.w1 = W1; a.w2 = W2; a.N = N; a.b = b; a.MT = N / b; a.bq = q; a.maskA = MK_NONE;
+    const int KT = N / b / q;
+    for (const BlkInvLaunch &l : plan) {
+        a.mode = l.mode; a.bd = l.bd; a.bt = l.bt; a.alpha = l.alpha; a.beta = l.beta;
+        a.ntasks = KT * (q - l.bd);
+        L.chk(launch_gemm(a, false, false, s));
     }
 }
 
@@ -1466,10 +1574,23 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         return fail(PALIAL_ERR_ARG, "%s needs the FP32 working copy on the split chain schedule "
                     "(factor tile %% 128 == 0, N > tile, PALIAL_DAG=2, PALIAL_F32WORK=1)",
                     o.precision == 3 ? "FP8" : "split BF16");
+    // tiles per sweep step (the blocked steps' limits: launch_tile_gemv)
+    int q = o.sweep_block > 0 ? o.sweep_block : g.sweep_block;
+    if (q > 1 && (q > 8 || MT % q || MT < 2 * q || nrhs > 4 || (size_t)q * b * (nrhs == 1 ? 1 : 4) > 4096)) {
+        if (o.sweep_block > 1) return fail(PALIAL_ERR_ARG, "sweep block %d: needs q <= 8 dividing N/b=%d with "
+                                           "N/b >= 2q, nrhs <= 4 and q*b*(nrhs > 1 ? 4 : 1) <= 4096", q, MT);
+        q = 1;
+    }
+    const int nf = algo == 0 ? 1 : 2;                     // triangular factors swept
+    const size_t tinv = (size_t)nf * MT * b * b;          // tile inverses
+    const size_t binv = q > 1 ? (size_t)q * tinv : 0;     // block inverses
     if ((!fs || b != bA) && (rc = grow(&g.d_F, &g.F_cap, nn))) return rc;
     if (fs && (rc = grow(&g.d_sub, &g.sub_cap, (size_t)b * b))) return rc;
-    if ((rc = grow(&g.d_inv, &g.inv_cap, (size_t)(algo == 0 ? 1 : 2) * MT * b * b))) return rc;
-    if (f32 && (rc = grow(&g.d_F32, &g.F32_cap, (nn + (size_t)(algo == 0 ? 1 : 2) * MT * b * b) / 2 + 2))) return rc;
+    if ((rc = grow(&g.d_inv, &g.inv_cap, tinv + binv))) return rc;
+    // W1: q MT tiles (the tiles of T at the offsets of X); W2: (q - 1) MT / q tiles
+    if (q > 1 && (rc = grow(&g.d_blk, &g.blk_cap, ((size_t)q * MT + (size_t)(q - 1) * (MT / q)) * b * b))) return rc;
+    const size_t inv32 = q > 1 ? binv : tinv;             // inverses the FP32 sweeps read
+    if (f32 && (rc = grow(&g.d_F32, &g.F32_cap, (nn + inv32) / 2 + 2))) return rc;
     float *F32 = (float *)g.d_F32, *Inv32 = F32 + nn;
     const size_t npart = residual_scratch(N, bA, nrhs);
     if ((rc = grow(&g.d_vec, &g.vec_cap, 6 * nv + 2 * (size_t)N + 64 + npart))) return rc;
@@ -1477,6 +1598,9 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     double *rowsum = R2 + nv, *red = rowsum + N;     // red: [0]=||A||, [1..]=col maxima
     double *part = red + 64;
     double *Uinv = g.d_inv, *Linv = g.d_inv + (size_t)MT * b * b;
+    // block inverses (q > 1): (U^T)^-1, then (LU) L^-1
+    double *XU = g.d_inv + tinv, *XL = XU + (size_t)q * MT * b * b;
+    double *BW1 = g.d_blk, *BW2 = g.d_blk + (size_t)q * MT * b * b;
 
     // inputs on the device
     double *dA = (double *)A.matrix;
@@ -1534,9 +1658,13 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
             inverses(TR_LUR, Uinv, g.d_dinv);
             inverses(TR_LUL, Linv, g.d_dinv + (size_t)32 * b);
         }
+        if (q > 1) {
+            block_inverses(Fw, fs, N, b, q, true, algo == 1, Uinv, XU, BW1, BW2, s, L);
+            if (algo == 1) block_inverses(Fw, fs, N, b, q, false, false, Linv, XL, BW1, BW2, s, L);
+        }
         if (f32) {
             if (!fs) L.chk(launch_to_f32(g.d_F, F32, N, b, algo == 0, 0, s));
-            L.chk(launch_to_f32(Uinv, Inv32, 0, b, false, (size_t)(algo == 0 ? 1 : 2) * MT * b * b, s));
+            L.chk(launch_to_f32(q > 1 ? g.d_inv + tinv : Uinv, Inv32, 0, b, false, inv32, s));
         }
     };
     // FP64 solve in place of this one (a mixed tier that broke down or did not
@@ -1581,15 +1709,22 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
     }
     if (info > 0 && fallback_ok) return fallback(shifts);
     const void *sF = f32 ? (const void *)F32 : (const void *)g.d_F;
-    const void *sU = f32 ? (const void *)Inv32 : (const void *)Uinv;
-    const void *sL = f32 ? (const void *)(Inv32 + (size_t)MT * b * b) : (const void *)Linv;
+    const void *sU, *sL;
+    if (q > 1) {
+        // FP32 copies in the order of the FP64 blocks
+        sU = f32 ? (const void *)Inv32 : (const void *)XU;
+        sL = f32 ? (const void *)(Inv32 + (size_t)q * MT * b * b) : (const void *)XL;
+    } else {
+        sU = f32 ? (const void *)Inv32 : (const void *)Uinv;
+        sL = f32 ? (const void *)(Inv32 + (size_t)MT * b * b) : (const void *)Linv;
+    }
     HIPCHK(hipEventRecord(e1, s));
     if (L.err != hipSuccess) return done(fail(PALIAL_ERR_HIP, "factorization launch: %s", hipGetErrorString(L.err)));
     // ||b||_inf now; ||A||_inf comes with the first residual (same pass over A)
     L.chk(launch_colmax(Bd, N, N, nrhs, red + 1, s));
     // x0 = F^{-1} b
     L.chk(launch_copy(R, Bd, nv * 8, s));
-    sweeps(algo, sF, sU, sL, R, Y, X, N, b, nrhs, f32, s, L);
+    sweeps(algo, sF, sU, sL, R, Y, X, N, b, nrhs, f32, s, L, q);
     // the Cholesky residual reads only the upper tiles of A (the symmetric
     // matrix the factorization sees)
     const bool sym = (algo == 0 && nrhs == 1);
@@ -1628,7 +1763,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         int err;
         std::vector<double> H((size_t)(GM + 1) * GM, 0.0), cs(GM, 0.0), sn(GM, 0.0), gv(GM + 1, 0.0), h(GM + 1);
         L.chk(launch_copy(T1, rj, (size_t)N * 8, s));
-        sweeps(algo, sF, sU, sL, T1, Y, V, N, b, 1, f32, s, L);          // v0 = F^-1 r
+        sweeps(algo, sF, sU, sL, T1, Y, V, N, b, 1, f32, s, L, q);          // v0 = F^-1 r
         double beta;
         if ((err = norm2(V, beta))) return err;
         inner = 0;
@@ -1643,7 +1778,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
             double *vk = V + (size_t)k * N, *vn = V + (size_t)(k + 1) * N;
             // w = F^-1 A v_k  (the residual kernel with a zero right-hand side gives -A v_k)
             L.chk(launch_residual(dA, N, bA, vk, Z0, T1, N, 1, part, sym, nullptr, s));
-            sweeps(algo, sF, sU, sL, T1, Y, Wv, N, b, 1, f32, s, L);
+            sweeps(algo, sF, sU, sL, T1, Y, Wv, N, b, 1, f32, s, L, q);
             L.chk(launch_vh_update(V, (size_t)N, 0, hd, Wv, -1.0, N, s));
             // classical Gram-Schmidt, twice
             std::vector<double> h2(k + 1);
@@ -1740,7 +1875,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         } else {
             // LU-IR: d = F^{-1} r
             L.chk(launch_copy(R2, R, nv * 8, s));
-            sweeps(algo, sF, sU, sL, R2, Y, D, N, b, nrhs, f32, s, L);
+            sweeps(algo, sF, sU, sL, R2, Y, D, N, b, nrhs, f32, s, L, q);
         }
         L.chk(launch_axpy(X, D, N, N, nrhs, s));
     }
@@ -2054,6 +2189,13 @@ int palial_hip_init(int device)
     // PALIAL_F32WORK=0 keeps the FP64 working copy in the mixed-tier solves
     const char *fw = getenv("PALIAL_F32WORK");
     g.f32work = fw ? atoi(fw) : 1;
+    // PALIAL_PAIR=1: BF16 Cholesky and LU solves apply two steps' updates per pass over the trailing tiles
+    const char *pr = getenv("PALIAL_PAIR");
+    g.pair = pr ? atoi(pr) : 0;
+    // PALIAL_SWEEP_BLOCK=q: default tiles per step of the correction sweeps
+    const char *sb = getenv("PALIAL_SWEEP_BLOCK");
+    g.sweep_block = sb ? atoi(sb) : 1;
+    if (g.sweep_block < 1 || g.sweep_block > 8) return fail(PALIAL_ERR_ARG, "PALIAL_SWEEP_BLOCK=%s: expected 1..8", sb);
     HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
     if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
@@ -2062,6 +2204,12 @@ int palial_hip_init(int device)
         for (int cu = g.reserve; cu < ncu; cu++) mu[cu / 32] |= 1u << (cu % 32);
         HIPCHK(hipExtStreamCreateWithCUMask(&g.s_bulk, words, mu.data()));
         HIPCHK(hipExtStreamCreateWithCUMask(&g.s_aux, words, mu.data()));
+        const char *cm = getenv("PALIAL_CHAIN_CUS");
+        if (cm && atoi(cm) == 1) {
+            for (auto &x : mu) x = ~x;
+            if (ncu % 32) mu[words - 1] &= (1u << (ncu % 32)) - 1;
+            HIPCHK(hipExtStreamCreateWithCUMask(&g.s_chainm, words, mu.data()));
+        }
     } else {
         g.reserve = 0;
         g.s_bulk = g.s_main;
@@ -2109,6 +2257,7 @@ int palial_hip_finalize(void)
     if (g.d_pack) (void)hipFree(g.d_pack);
     if (g.d_F) (void)hipFree(g.d_F);
     if (g.d_inv) (void)hipFree(g.d_inv);
+    if (g.d_blk) (void)hipFree(g.d_blk);
     if (g.d_F32) (void)hipFree(g.d_F32);
     if (g.d_gm) (void)hipFree(g.d_gm);
     if (g.d_bidx) (void)hipFree(g.d_bidx);
@@ -2126,10 +2275,118 @@ int palial_hip_finalize(void)
     if (g.d_info) (void)hipFree(g.d_info);
     if (g.s_bulk && g.s_bulk != g.s_main) (void)hipStreamDestroy(g.s_bulk);
     if (g.s_aux) (void)hipStreamDestroy(g.s_aux);
+    if (g.s_chainm) (void)hipStreamDestroy(g.s_chainm);
     (void)hipStreamDestroy(g.s_main);
     (void)hipStreamDestroy(g.s_panel);
     g = Ctx();
     return 0;
+}
+
+int palial_hip_sweep_plan(int algo, int MT, int q, int *out, int cap)
+{
+    if (algo < 0 || algo > 1 || q < 1 || MT < 1 || MT % q) return PALIAL_ERR_ARG;
+    const int n = sweep_plan(algo, MT, q, nullptr, 0);
+    if (!out || cap < n) return PALIAL_ERR_ARG;
+    std::vector<SweepStep> plan((size_t)n);
+    sweep_plan(algo, MT, q, plan.data(), n);
+    for (int i = 0; i < n; i++) {
+        const SweepStep &st = plan[(size_t)i];
+        const int v[7] = {st.mode, st.k, st.trans, st.ntasks, st.inv, st.vin, st.vout};
+        for (int j = 0; j < 7; j++) out[7 * i + j] = v[j];
+    }
+    return n;
+}
+
+int palial_hip_gemv_tiles(int mode, int trans, int MT, int b, int q, int k, int ntasks, long *out, long cap)
+{
+    if (mode < GV_DIAG || mode > GV_BWD || q < 1 || b < 1 || MT < 1 || ntasks < 0 || !out ||
+        cap < (long)ntasks * (3 + q))
+        return PALIAL_ERR_ARG;
+    for (int t = 0; t < ntasks; t++) {
+        const GemvMap m = gemv_map(mode, trans, k, q, t);
+        long *o = out + (long)t * (3 + q);
+        o[0] = m.vout; o[1] = m.j0; o[2] = m.j1;
+        for (int j = 0; j < q; j++)
+            o[3 + j] = (j >= m.j0 && j < m.j1) ? (long)gemv_src(mode, trans, k, q, b, MT, m.vout, j) : -1;
+    }
+    return ntasks;
+}
+
+int palial_hip_gather_map(int MT, int b, int q, int upper, int dtrans, long *out, long cap)
+{
+    if (q < 2 || MT % q || b < 1 || !out || cap < 4L * MT * q) return PALIAL_ERR_ARG;
+    const int KT = MT / q;
+    long n = 0;
+    for (int K = 0; K < KT; K++)
+        for (int i = 0; i < q; i++)
+            for (int j = 0; j < q; j++, n++) {
+                const GatherTile g2 = gather_tile(MT, b, q, upper != 0, dtrans != 0, K, i, j);
+                out[4 * n] = g2.kind; out[4 * n + 1] = g2.tr ? 1 : 0;
+                out[4 * n + 2] = (long)g2.src; out[4 * n + 3] = (long)g2.dst;
+            }
+    return (int)n;
+}
+
+int palial_hip_blkinv_plan(int q, double *out, int cap)
+{
+    if (q < 1) return PALIAL_ERR_ARG;
+    const int n = blkinv_plan(q, nullptr, 0);
+    if (n > 0 && (!out || cap < n)) return PALIAL_ERR_ARG;
+    std::vector<BlkInvLaunch> plan((size_t)n);
+    blkinv_plan(q, plan.data(), n);
+    for (int i = 0; i < n; i++) {
+        const BlkInvLaunch &l = plan[(size_t)i];
+        out[5 * i] = l.mode; out[5 * i + 1] = l.bd; out[5 * i + 2] = l.bt;
+        out[5 * i + 3] = l.alpha; out[5 * i + 4] = l.beta;
+    }
+    return n;
+}
+
+int palial_hip_blkinv_tiles(int mode, int MT, int b, int q, int bd, int bt, long *out, long cap)
+{
+    if ((mode != GM_BI_W && mode != GM_BI_X) || q < 2 || MT % q || bd < 1 || bd >= q || bt < 0 || bt >= bd || b < 1)
+        return PALIAL_ERR_ARG;
+    const int ntasks = MT / q * (q - bd);
+    if (!out || cap < 8L * ntasks) return PALIAL_ERR_ARG;
+    GemmArgs a;
+    memset(&a, 0, sizeof a);
+    a.mode = mode; a.b = b; a.MT = MT; a.N = MT * b; a.bq = q; a.bd = bd; a.bt = bt; a.ntasks = ntasks;
+    // three fake bases far apart, so that each pointer maps back to one buffer
+    const size_t span = (size_t)1 << 40;
+    double *base[3] = {(double *)(span * 8), (double *)(span * 8 * 3), (double *)(span * 8 * 5)};
+    a.mat = base[0]; a.w1 = base[1]; a.w2 = base[2];
+    auto rel = [&](const double *p, long *o) {
+        o[0] = -1; o[1] = -1;
+        if (!p) return;
+        for (int bi = 0; bi < 3; bi++) {
+            const long d = (long)(p - base[bi]);
+            if (d >= 0 && d < (long)span) { o[0] = bi; o[1] = d; return; }
+        }
+        o[0] = -2;
+    };
+    for (int t = 0; t < ntasks; t++) {
+        const TaskPtrs tp = decode(a, t);
+        long *o = out + 8L * t;
+        rel(tp.A, o); rel(tp.B, o + 2); rel(tp.C, o + 4); rel(tp.D, o + 6);
+    }
+    return ntasks;
+}
+
+int palial_hip_set_option(const char *name, int value)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = ensure_init())) return rc;
+    if (!name) return fail(PALIAL_ERR_ARG, "option name is NULL");
+    int *slot = nullptr;
+    int lo = 0, hi = 0;
+    if (!strcmp(name, "pair")) { slot = &g.pair; lo = 0; hi = 1; }
+    else if (!strcmp(name, "sweep_block")) { slot = &g.sweep_block; lo = 1; hi = 8; }
+    if (!slot) return fail(PALIAL_ERR_ARG, "unknown option %s", name);
+    if (value < lo || value > hi) return fail(PALIAL_ERR_ARG, "option %s=%d outside %d..%d", name, value, lo, hi);
+    const int prev = *slot;
+    *slot = value;
+    return prev;
 }
 
 int palial_hip_trace(int enable)

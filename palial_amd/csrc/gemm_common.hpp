@@ -197,6 +197,72 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
         t.D = t.C;
         break;
     }
+    case GM_CHOL_ROW2: {
+        // tasks 0..T-3: tiles (k+2, k+3+task); task T-2: tile (k+3, k+3)
+        const bool last = task == p.T - 2;
+        const int n = last ? k + 3 : k + 2, m = last ? k + 3 : k + 3 + task;
+        t.A = M + tile_off(b, MT, k, n);
+        t.B = M + tile_off(b, MT, k, m);
+        t.C = M + tile_off(b, MT, n, m);
+        t.D = t.C;
+        t.diag = (n == m);
+        t.ia = n - k - 1;
+        t.ib = m - k - 1;
+        break;
+    }
+    case GM_CHOL_REST2: {
+        int n, m;
+        tri_decode(task, n, m);
+        n += k + 3; m += k + 3;
+        t.A = M + tile_off(b, MT, k, n);
+        t.B = M + tile_off(b, MT, k, m);
+        t.C = M + tile_off(b, MT, n, m);
+        t.D = t.C;
+        t.diag = (n == m);
+        t.ia = n - k - 1;        // panel k; panel k+1 holds the same tiles at ia - 1, ib - 1
+        t.ib = m - k - 1;
+        break;
+    }
+    case GM_LU_ROW2:
+    case GM_LU_REST2: {
+        int m, n;
+        if (p.mode == GM_LU_REST2) {
+            const int rows = p.T - 2;
+            m = k + 3 + task % rows;
+            n = k + 3 + task / rows;
+        } else if (task < p.T - 2) {
+            m = k + 3 + task; n = k + 2;                      // column k+2
+        } else if (task < 2 * p.T - 4) {
+            m = k + 2; n = k + 3 + (task - (p.T - 2));        // row k+2
+        } else {
+            m = k + 3; n = k + 3;
+        }
+        t.A = M + tile_off(b, MT, m, k);
+        t.B = M + tile_off(b, MT, k, n);
+        t.C = M + tile_off(b, MT, m, n);
+        t.D = t.C;
+        t.ia = m - k - 1;        // GM_LU_REST2: panel k+1 holds the same tiles at ia - 1, ib - 1
+        t.ib = n - k - 1;
+        break;
+    }
+    case GM_BI_W:
+    case GM_BI_X: {
+        const int q = p.bq, w = q - p.bd;
+        const int K = task / w, j = task % w, i = j + p.bd;
+        auto tl = [&](int r, int c) { return ((size_t)(K * q + r) * q + c) * bb; };
+        if (p.mode == GM_BI_W) {
+            const int m = j + p.bt;
+            t.A = p.w1 + tl(i, m);
+            t.B = M + tl(m, j);
+            t.C = p.w2 + (size_t)task * bb;
+            t.D = p.bt > 0 ? t.C : nullptr;
+        } else {
+            t.A = M + tl(i, i);
+            t.B = p.w2 + (size_t)task * bb;
+            t.C = M + tl(i, j);
+        }
+        break;
+    }
     default:
         t.valid = false;
     }

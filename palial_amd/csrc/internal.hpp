@@ -43,6 +43,21 @@ enum GemmMode : int {
     GM_OA_W = 18,       // diagonal block k: W_n = V^T C(k,n), V = A(k,k) unit lower
     GM_OA_TW = 19,      // W2_n = op(T) W_n (T = S(k,k) upper)
     GM_OA_VW = 20,      // C(k,n) -= V W2_n
+    // inverses of the bq x bq tile blocks on the diagonal of a lower
+    // triangular matrix T (blocked correction sweeps).  mat: the inverses,
+    // block K's tile (i, j) at ((K bq + i) bq + j) b^2, diagonal tiles already
+    // in place; w1: the tiles T(i, m), m < i, at the same offsets.  Level bd
+    // (i = j + bd), task (K, j) = (task / (bq - bd), task % (bq - bd)):
+    GM_BI_W = 21,       // term bt: W_task (+)= T(i, j + bt) X(j + bt, j)   (w2: W)
+    GM_BI_X = 22,       // X(i, j) = -X(i, i) W_task
+    // Cholesky steps k and k+1 paired (packed panels): the bulk update of
+    // step k is split into GM_CHOL_FIRST, GM_CHOL_ROW2 and the part that
+    // GM_CHOL_REST2 applies together with step k+1's (pa2 / pb2: panel k+1)
+    GM_CHOL_ROW2 = 23,  // step k: tiles (k+2, m), m >= k+3, then tile (k+3, k+3) (task T-2)
+    GM_CHOL_REST2 = 24, // steps k, k+1: tiles (n, m), k+3 <= n <= m, from panels k and k+1
+    // the same for the LU (GM_LU_FIRST, GM_LU_ROW2, GM_LU_REST2)
+    GM_LU_ROW2 = 25,    // step k: column k+2 (m >= k+3), row k+2 (n >= k+3), then tile (k+3, k+3)
+    GM_LU_REST2 = 26,   // steps k, k+1: tiles (m, n), m, n >= k+3, from panels k and k+1
 };
 
 enum MaskA : int { MK_NONE = 0, MK_UPPER = 1, MK_UNIT_LOWER = 2 };
@@ -67,6 +82,8 @@ struct GemmArgs {
     int task0;           // first task of the enumeration this launch covers
     const void *pa;      // packed low-precision panel operands (launch_gemm_packed):
     const void *pb;      // tile t of the panel at t*b*b elements, K-contiguous
+    const void *pa2;     // GM_CHOL_REST2: the packed panel of step k+1 (tile t-1 pairs with
+    const void *pb2;     // tile t of pa / pb)
     double alpha, beta;
     int maskA;
     int single_diag;     // GM_SINGLE: output is a SYRK tile (upper triangle only)
@@ -74,6 +91,7 @@ struct GemmArgs {
     double *cmat;        // GM_QA_* / GM_OA_*: the matrix Q is applied to (layout L, MT row tiles)
     unsigned long long *ts;   // per-task tracing: timestamp slots (nullptr: not traced)
     long ts_base;             // slot of task 0 of this launch
+    int bq, bd, bt;           // GM_BI_*: tiles per block, level, term
 };
 
 // ------------------------------------------------------------------ task trace
@@ -193,16 +211,109 @@ hipError_t launch_gemm_lp(const GemmArgs &a, bool transA, bool transB, int prec,
 
 // solve path (k_solve.hip)
 enum GemvMode : int { GV_DIAG = 0, GV_CHOL_FWD = 1, GV_LU_FWD = 2, GV_BWD = 3 };
+// A sweep step covers q consecutive tiles (q = 1 or 0: one tile): step k
+// solves with the inverse of diagonal block k (tiles kq .. kq+q-1; q = 1:
+// tile k at k*b*b; q > 1: lower triangular, tile (i, j) of block k at
+// ((kq + i) q + j) b^2, used transposed when trans) and the update launches
+// read the q tiles of the block's tile column (forward) or row (backward).
 struct GemvArgs {
     int mode;
     const void *mat;     // factored matrix (layout L), double or float (f32)
-    const void *inv;     // inverses of the diagonal tiles, tile k at k*b*b
+    const void *inv;     // inverses of the diagonal tiles or blocks
     const double *vin;   // input vector block (N x nrhs, leading dimension ld)
     double *vout;        // output vector block
     int N, b, k, ld, nrhs, ntasks, trans;
     int f32;             // mat / inv hold FP32 copies
+    int q;               // tiles per sweep step
 };
+// Tiles of the lower triangular blocks the blocked sweeps invert, as FP64:
+// block k (q tiles) of T = F^T from the upper tiles (trans: Cholesky U and
+// LU's U) or T = F from the lower tiles (LU's unit L).  Writes X(k, i, i) =
+// op(D_{kq+i}) (D: the tile inverses, tile t at t*b*b; op = transpose for
+// LU's U, whose tile inverses are upper), zero tiles X(k, i, j > i), and
+// W1(k, i, m) = T(kq+i, kq+m) for m < i, all at ((kq + i) q + j) b^2.
+hipError_t launch_blk_gather(const void *F, bool f32, int N, int b, int q, bool upper, bool dtrans,
+                             const double *D, double *X, double *W1, hipStream_t s);
+// What launch_blk_gather writes to tile (K, i, j) (the kernel and the host
+// test hook share it): kind 0 zeros into X, 1 tile src of D into X, 2 tile
+// src of F into W1; tr: transposed; dst: the offset in X or W1.
+struct GatherTile {
+    int kind;
+    bool tr;
+    size_t src, dst;
+};
+__host__ __device__ inline GatherTile gather_tile(int MT, int b, int q, bool upper, bool dtrans, int K, int i, int j)
+{
+    const size_t bb = (size_t)b * b;
+    GatherTile g;
+    g.dst = ((size_t)(K * q + i) * q + j) * bb;
+    g.src = 0;
+    g.tr = false;
+    if (j > i) {
+        g.kind = 0;
+    } else if (j == i) {
+        g.kind = 1;
+        g.src = (size_t)(K * q + i) * bb;
+        g.tr = dtrans;
+    } else {
+        g.kind = 2;
+        g.src = upper ? tile_off(b, MT, K * q + j, K * q + i) : tile_off(b, MT, K * q + i, K * q + j);
+        g.tr = upper;
+    }
+    return g;
+}
+// The batched GEMM launches of the block inverses of q-tile blocks, in issue
+// order: level bd (1..q-1), term bt (GM_BI_W, bt < bd) then GM_BI_X.
+struct BlkInvLaunch {
+    int mode, bd, bt;
+    double alpha, beta;
+};
+// (api.hip: blkinv_plan(q, out, cap) returns the launch count)
 hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s);
+
+// Tile map of one tile_gemv task (the kernel and the host test hook share it):
+// output tile vout and source tiles j0 <= j < j1 of step k (q tiles per step)
+struct GemvMap {
+    int vout, j0, j1;
+};
+__host__ __device__ inline GemvMap gemv_map(int mode, int trans, int k, int q, int task)
+{
+    const int s0 = k * q;
+    GemvMap m;
+    m.j0 = 0;
+    m.j1 = q;
+    switch (mode) {
+    case GV_DIAG:
+        m.vout = s0 + task;
+        if (trans) m.j0 = task;      // block inverse read transposed: its tiles (j, task), j >= task
+        else m.j1 = task + 1;        // lower block inverse: tiles (task, j), j <= task
+        break;
+    case GV_CHOL_FWD:
+    case GV_LU_FWD: m.vout = s0 + q + task; break;
+    default: m.vout = task; break;   // GV_BWD
+    }
+    return m;
+}
+// element offset of source tile j of a task: from the inverse base (GV_DIAG:
+// block k's tile (vout - kq, j), or (j, vout - kq) when trans) or from the
+// matrix base (U tile (kq + j, vout) for GV_CHOL_FWD; (vout, kq + j) else)
+__host__ __device__ inline size_t gemv_src(int mode, int trans, int k, int q, int b, int MT, int vout, int j)
+{
+    const int s0 = k * q, i = vout - s0;
+    const size_t bb = (size_t)b * b;
+    switch (mode) {
+    case GV_DIAG: return ((size_t)(s0 + (trans ? j : i)) * q + (trans ? i : j)) * bb;
+    case GV_CHOL_FWD: return tile_off(b, MT, s0 + j, vout);
+    default: return tile_off(b, MT, vout, s0 + j);
+    }
+}
+// The launches of the two triangular sweeps (host side), in issue order.
+// inv: 0 the U-side inverses (Cholesky (U^T)^-1 tiles or blocks; LU U^-1 tiles
+// or (U^T)^-1 blocks), 1 the L-side ones; vin / vout: 0 r, 1 y, 2 x.
+struct SweepStep {
+    int mode, k, trans, ntasks, inv, vin, vout;
+};
+// (api.hip: sweep_plan(algo, MT, q, out, cap) returns the step count)
 // FP32 copy of a layout-L matrix (N > 0; upper_only: tiles m <= n) or of n doubles (N == 0)
 hipError_t launch_to_f32(const double *src, float *dst, int N, int b, bool upper_only, size_t n, hipStream_t s);
 hipError_t launch_residual(const double *A, int N, int b, const double *x, const double *rhs, double *r, int ld,

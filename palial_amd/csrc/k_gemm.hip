@@ -331,6 +331,10 @@ long gemm_task_count(int mode, int T)
     case GM_LU_REST: return (long)(T - 1) * (T - 1);
     case GM_CHOL_FIRST: return T + 1;
     case GM_LU_FIRST: return 2L * T;
+    case GM_CHOL_ROW2: return T >= 3 ? T - 1 : 0;
+    case GM_CHOL_REST2: return T >= 3 ? (long)(T - 2) * (T - 1) / 2 : 0;
+    case GM_LU_ROW2: return T >= 3 ? 2L * T - 3 : 0;
+    case GM_LU_REST2: return T >= 3 ? (long)(T - 2) * (T - 2) : 0;
     default: return T;  // QR / ORMQR modes: one task per trailing column tile
     }
 }

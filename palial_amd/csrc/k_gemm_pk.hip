@@ -433,14 +433,21 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
     const int wm = w & 1, wn = w >> 1;
     // byte addressing: a panel row is rs bytes, a slice 64 bytes of it
     const size_t tstride = F8 ? fp8_slot(b) : (size_t)b * b * (X3 ? 4 : 2);
-    const char *PA = (const char *)p.pa + (size_t)tp.ia * tstride;
-    const char *PB = (const char *)p.pb + (size_t)tp.ib * tstride;
+    const char *PA_ = (const char *)p.pa + (size_t)tp.ia * tstride;
+    const char *PB_ = (const char *)p.pb + (size_t)tp.ib * tstride;
+    // paired steps (GM_CHOL_REST2, GM_LU_REST2): the K loop runs on through panel k+1, whose
+    // tiles sit one position lower
+    const char *PA2 = p.pa2 ? (const char *)p.pa2 + (size_t)(tp.ia - 1) * tstride : nullptr;
+    const char *PB2 = p.pb2 ? (const char *)p.pb2 + (size_t)(tp.ib - 1) * tstride : nullptr;
     const size_t rs = F8 ? (size_t)b : 2 * (size_t)b;
-    const int NS = F8 ? b / 64 : b / KB;
+    const int NS1 = F8 ? b / 64 : b / KB;               // slices of one panel
+    const int NS = PA2 ? 2 * NS1 : NS1;
 
     auto issue = [&](int s) {
         const int st = s % NST;
-        const size_t kb = (size_t)s * 64;
+        const bool second = s >= NS1;
+        const char *PA = second ? PA2 : PA_, *PB = second ? PB2 : PB_;
+        const size_t kb = (size_t)(second ? s - NS1 : s) * 64;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int j = w * 2 + q;                    // 1 KB chunk of the slice
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
     }
     double alpha_d = p.alpha;
     if constexpr (F8) {
-        const int ea = *(const int *)(PA + (size_t)b * b + 256), eb = *(const int *)(PB + (size_t)b * b + 256);
+        const int ea = *(const int *)(PA_ + (size_t)b * b + 256), eb = *(const int *)(PB_ + (size_t)b * b + 256);
         alpha_d = ldexp(p.alpha, -(ea + eb));
     }
 
@@ -727,7 +734,12 @@ hipError_t launch_gemm_packed(const GemmArgs &a_in, int prec, hipStream_t s)
     trace_gemm(a);
     if (a.b % 128 != 0 || a.maskA != MK_NONE || !a.pa || !a.pb) return hipErrorInvalidValue;
     if (a.mode != GM_CHOL_TRAIL && a.mode != GM_CHOL_REST && a.mode != GM_CHOL_ROW && a.mode != GM_LU_TRAIL &&
-        a.mode != GM_LU_REST && a.mode != GM_LU_PANEL && a.mode != GM_CHOL_FIRST && a.mode != GM_LU_FIRST)
+        a.mode != GM_LU_REST && a.mode != GM_LU_PANEL && a.mode != GM_CHOL_FIRST && a.mode != GM_LU_FIRST &&
+        a.mode != GM_CHOL_ROW2 && a.mode != GM_CHOL_REST2 && a.mode != GM_LU_ROW2 && a.mode != GM_LU_REST2)
+        return hipErrorInvalidValue;
+    // two panels per launch: BF16 and split BF16 only, and exactly for the *_REST2 modes
+    const bool two = a.mode == GM_CHOL_REST2 || a.mode == GM_LU_REST2;
+    if (two != (a.pa2 != nullptr && a.pb2 != nullptr) || (a.pa2 && prec != 2 && prec != 4))
         return hipErrorInvalidValue;
     const long nbd = a.b / 128;
     dim3 grid((unsigned)((long)a.ntasks * nbd * nbd)), block(256);
@@ -745,6 +757,7 @@ hipError_t launch_gemm_packed(const GemmArgs &a_in, int prec, hipStream_t s)
             const char *e = getenv("PALIAL_PK_VARIANT");
             return e ? atoi(e) : 0;
         }();
+        if (a.pa2 && var != 0) return hipErrorInvalidValue;  // two panels only in the default kernel
         if (a.f32c) {
             if (var != 0) return hipErrorInvalidValue;       // FP32 output only in the default kernel
             hipLaunchKernelGGL((gemm_bf16d_kernel<2, true>), grid, block, 0, s, a);

@@ -17,9 +17,10 @@ namespace palial {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------- tile GEMV
-// For each task t: out_t(0:b, 0:nrhs) = op(T_t) v  (GV_DIAG) or
-// out_t -= op(T_t) v (updates).  A workgroup (256 threads) owns ROWS output
-// rows of one tile; the input segment (b x nrhs) sits in LDS.
+// For each task t: out_t(0:b, 0:nrhs) = sum_j op(T_tj) v_j  (GV_DIAG) or
+// out_t -= sum_j op(T_tj) v_j (updates), over the source tiles j of the step
+// (gemv_map, gemv_src).  A workgroup (256 threads) owns ROWS output rows of
+// one tile; the input segment (the step's q tiles x nrhs) sits in LDS.
 //   op = N: out(r) = sum_c T(r, c) v(c): ROWS row lanes x 256/ROWS column
 //           groups; ROWS = 16 for FP64 and 32 for FP32 tiles, so that a
 //           column piece is one full 128-byte line
@@ -36,31 +37,29 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 {
     constexpr int G = GV_THREADS / ROWS;      // column groups (op = N)
     extern __shared__ double sm[];
-    double *sv = sm;                          // b x NR input segment (layout by op, below)
-    double *sred = sm + (size_t)p.b * NR;     // G groups x ROWS rows x NR partial sums
     const int b = p.b;
+    const int q = p.q > 1 ? p.q : 1;
+    const int seg = q * b;                    // input segment: the q tiles of the step
+    double *sv = sm;                          // seg x NR input segment (layout by op, below)
+    double *sred = sm + (size_t)seg * NR;     // G groups x ROWS rows x NR partial sums
     const int rb_per_tile = b / ROWS;
     const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
     const int t = threadIdx.x;
     const E *mat = (const E *)p.mat;
     const E *inv = (const E *)p.inv;
-    const E *T;
-    int vin, vout;
-    {
-        const int MT = p.N / b, k = p.k;
-        switch (p.mode) {
-        case GV_DIAG: T = inv + (size_t)k * b * b; vin = k; vout = k; break;
-        case GV_CHOL_FWD: T = mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
-        case GV_LU_FWD: T = mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
-        default: T = mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
-        }
-    }
-    const double *vi = p.vin + (size_t)vin * b;
+    const int MT = p.N / b, s0 = p.k * q;     // first tile of the step
+    // output tile and the source tiles j0 <= j < j1 of the step (T_j below)
+    const GemvMap mp = gemv_map(p.mode, p.trans, p.k, q, task);
+    const int vout = mp.vout, j0 = mp.j0, j1 = mp.j1;
+    auto tileT = [&](int j) -> const E * {
+        return (p.mode == GV_DIAG ? inv : mat) + gemv_src(p.mode, p.trans, p.k, q, b, MT, vout, j);
+    };
+    const double *vi = p.vin + (size_t)s0 * b;
     double *vo = p.vout + (size_t)vout * b;
     // op = N reads the NR values of one row together: sv[i * NR + c]; op = T
-    // reads one right-hand side across the lanes: sv[i + c * b]
-    for (int e = t; e < b * NR; e += GV_THREADS) {
-        const int i = e % b, c = e / b;
+    // reads one right-hand side across the lanes: sv[i + c * seg]
+    for (int e = t; e < seg * NR; e += GV_THREADS) {
+        const int i = e % seg, c = e / seg;
         const double x = (c < p.nrhs) ? vi[(size_t)i + (size_t)c * p.ld] : 0.0;   // padded right-hand sides are zero
         sv[p.trans ? e : i * NR + c] = x;
     }
@@ -71,13 +70,16 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
     for (int j = 0; j < NR; j++) acc[j] = 0.0;
     if (!p.trans) {
         const int r = t % ROWS, g = t / ROWS;             // g: column group, columns c = g, g+G, ...
-        const E *Tr = T + r0 + r;
+        for (int jt = j0; jt < j1; jt++) {
+            const E *Tr = tileT(jt) + r0 + r;
+            const double *svt = sv + (size_t)jt * b * NR;
 #pragma unroll 8
-        for (int c = g; c < b; c += G) {
-            const double a = (double)Tr[(size_t)c * b];
-            const double *svc = sv + c * NR;
+            for (int c = g; c < b; c += G) {
+                const double a = (double)Tr[(size_t)c * b];
+                const double *svc = svt + c * NR;
 #pragma unroll
-            for (int j = 0; j < NR; j++) acc[j] += a * svc[j];
+                for (int j = 0; j < NR; j++) acc[j] += a * svc[j];
+            }
         }
 #pragma unroll
         for (int j = 0; j < NR; j++) sred[(g * ROWS + r) * NR + j] = acc[j];
@@ -97,15 +99,19 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
         for (int x = 0; x < 4; x++)
 #pragma unroll
             for (int j = 0; j < NR; j++) part[x][j] = 0.0;
-        for (int c0 = 0; c0 < b; c0 += 64) {
-            const int c = c0 + lane;
-            double a[4];
+        for (int jt = j0; jt < j1; jt++) {
+            const E *T = tileT(jt);
+            const double *svt = sv + (size_t)jt * b;
+            for (int c0 = 0; c0 < b; c0 += 64) {
+                const int c = c0 + lane;
+                double a[4];
 #pragma unroll
-            for (int x = 0; x < 4; x++) a[x] = (double)T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
+                for (int x = 0; x < 4; x++) a[x] = (double)T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
 #pragma unroll
-            for (int x = 0; x < 4; x++)
+                for (int x = 0; x < 4; x++)
 #pragma unroll
-                for (int j = 0; j < NR; j++) part[x][j] += a[x] * sv[c + j * b];
+                    for (int j = 0; j < NR; j++) part[x][j] += a[x] * svt[c + j * seg];
+            }
         }
         // Sum the 4 NR values over the 64 lanes.  Butterfly steps on the high
         // lane bits first (every lane keeps all values), then halving steps:
@@ -303,6 +309,9 @@ hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
 {
     if (a.ntasks <= 0) return hipSuccess;
     if (a.b > 512 || a.b % 64 || a.nrhs > GV_MAXR || a.nrhs < 1) return hipErrorInvalidValue;
+    const int q = a.q > 1 ? a.q : 1;
+    // blocked steps: 1..4 right-hand sides, the segment within the default LDS limit
+    if (q > 1 && (a.nrhs > 4 || (size_t)q * a.b * (a.nrhs == 1 ? 1 : 4) > 4096)) return hipErrorInvalidValue;
     // FP32 tiles read by columns (op = N): 32 rows per workgroup (128-byte column pieces)
     if (a.nrhs > 4) {
         dim3 g16((unsigned)(a.ntasks * (a.b / 64))), b16(GV_THREADS);
@@ -319,7 +328,7 @@ hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
     }
     const bool wide = a.f32 && !a.trans;
     dim3 grid((unsigned)(a.ntasks * (a.b / (wide ? 32 : GV_ROWS)))), block(GV_THREADS);
-    auto lds = [&](int nr) { return ((size_t)a.b * nr + (size_t)GV_THREADS * nr) * 8; };
+    auto lds = [&](int nr) { return ((size_t)q * a.b * nr + (size_t)GV_THREADS * nr) * 8; };
     if (wide) {
         if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, float, 32>), grid, block, lds(1), s, a);
         else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, float, 32>), grid, block, lds(4), s, a);
@@ -362,6 +371,65 @@ hipError_t launch_to_f32(const double *src, float *dst, int N, int b, bool upper
     const size_t total = N ? (size_t)N * N : n;
     if (total % 4) return hipErrorInvalidValue;
     hipLaunchKernelGGL(to_f32_kernel, dim3(8192), dim3(256), 0, s, src, dst, N, b, upper_only ? 1 : 0, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- block inverse inputs
+// One workgroup per 64 x 64 sub-block of one tile (K, i, j) of the q x q tile
+// blocks: zeros above the block diagonal, op(D) on it, the tiles of T below
+// it.  Transposed copies go through LDS, so reads and writes both run down
+// columns.
+template <typename E>
+__global__ __launch_bounds__(256) void blk_gather_kernel(const E *F, int N, int b, int q, int upper, int dtrans,
+                                                         const double *D, double *X, double *W1)
+{
+    __shared__ double tl[64][65];
+    const int nsb = b / 64, per_tile = nsb * nsb;
+    const long id = blockIdx.x;
+    const int sub = (int)(id % per_tile);
+    long tt = id / per_tile;
+    const int j = (int)(tt % q);
+    tt /= q;
+    const int i = (int)(tt % q);
+    const int K = (int)(tt / q);
+    const int r0 = (sub % nsb) * 64, c0 = (sub / nsb) * 64;   // output sub-block origin
+    const int MT = N / b;
+    const GatherTile gt = gather_tile(MT, b, q, upper != 0, dtrans != 0, K, i, j);
+    const int t = threadIdx.x, r = t & 63, cw = t >> 6;
+    if (gt.kind == 0) {
+        for (int c = cw; c < 64; c += 4) X[gt.dst + (size_t)(r0 + r) + (size_t)(c0 + c) * b] = 0.0;
+        return;
+    }
+    const E *srcE = gt.kind == 2 ? F + gt.src : nullptr;
+    const double *srcD = gt.kind == 1 ? D + gt.src : nullptr;
+    double *dst = (gt.kind == 1 ? X : W1) + gt.dst;
+    const bool tr = gt.tr;
+    auto src = [&](int rr, int cc) -> double {
+        const size_t e = (size_t)rr + (size_t)cc * b;
+        return srcD ? srcD[e] : (double)srcE[e];
+    };
+    if (!tr) {
+        for (int c = cw; c < 64; c += 4) dst[(size_t)(r0 + r) + (size_t)(c0 + c) * b] = src(r0 + r, c0 + c);
+    } else {
+        // dst(r0 + r, c0 + c) = src(c0 + c, r0 + r): stage tl[c][r] = src(c0 + r, r0 + c)
+        for (int c = cw; c < 64; c += 4) tl[c][r] = src(c0 + r, r0 + c);
+        __syncthreads();
+        for (int c = cw; c < 64; c += 4) dst[(size_t)(r0 + r) + (size_t)(c0 + c) * b] = tl[r][c];
+    }
+}
+
+hipError_t launch_blk_gather(const void *F, bool f32, int N, int b, int q, bool upper, bool dtrans,
+                             const double *D, double *X, double *W1, hipStream_t s)
+{
+    if (q < 2 || b % 64 || N % b || (N / b) % q) return hipErrorInvalidValue;
+    const long KT = (long)(N / b) / q, nsb = b / 64;
+    const dim3 grid((unsigned)(KT * q * q * nsb * nsb)), block(256);
+    if (f32)
+        hipLaunchKernelGGL((blk_gather_kernel<float>), grid, block, 0, s, (const float *)F, N, b, q, (int)upper,
+                           (int)dtrans, D, X, W1);
+    else
+        hipLaunchKernelGGL((blk_gather_kernel<double>), grid, block, 0, s, (const double *)F, N, b, q, (int)upper,
+                           (int)dtrans, D, X, W1);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@ import pytest
 CHOL_TRAIL, LU_TRAIL, QR_W, QR_TW, QR_VW, ORM_W, ORM_TW, ORM_VW = 1, 2, 3, 4, 5, 6, 7, 8
 CHOL_ROW, CHOL_REST, LU_PANEL, LU_REST = 9, 10, 11, 12
 CHOL_FIRST, LU_FIRST = 13, 14
+CHOL_ROW2, CHOL_REST2, LU_ROW2, LU_REST2 = 23, 24, 25, 26
 
 
 def tasks(mode, MT, k, m=0, b=4):
@@ -95,3 +96,44 @@ def test_qr_step_tasks(MT):
         o = tasks(ORM_VW, MT, k)
         assert [t[2] for t in o] == [(k, n) for n in range(k + 1, MT)]
         assert all(t[0] == (k, k) for t in o)
+
+
+@pytest.mark.parametrize("MT", [5, 6, 9])
+def test_paired_update_tasks(MT):
+    """Paired steps j, j+1 (T = MT-j-1 >= 4): step j's update is the chain's
+    tile (j+1,j+1), FIRST (from task 1), ROW2 (row j+2 and tile (j+3,j+3)) and
+    the tiles of REST2 (from task 1); REST2's tiles are also exactly step j+1's
+    REST (from task 1), so one REST2 launch applies both steps there."""
+    for j in range(MT - 4):
+        full = tasks(CHOL_TRAIL, MT, j)
+        first = tasks(CHOL_FIRST, MT, j)[1:]
+        row2 = tasks(CHOL_ROW2, MT, j)
+        rest2 = tasks(CHOL_REST2, MT, j)
+        assert rest2[0][2] == (j + 3, j + 3)
+        rest2 = rest2[1:]
+        outs = [t[2] for t in first] + [t[2] for t in row2] + [t[2] for t in rest2]
+        assert len(outs) == len(set(outs)) and (j + 1, j + 1) not in outs
+        assert set(first) | set(row2) | set(rest2) | {t for t in full if t[2] == (j + 1, j + 1)} == set(full)
+        assert all(t[2][0] == j + 2 or t[2] == (j + 3, j + 3) for t in row2)
+        nxt = tasks(CHOL_REST, MT, j + 1)[1:]
+        assert [t[2] for t in nxt] == [t[2] for t in rest2]
+
+
+@pytest.mark.parametrize("MT", [5, 6, 9])
+def test_paired_lu_update_tasks(MT):
+    """LU pairs: FIRST (from task 1), ROW2 (column and row j+2, tile (j+3,j+3))
+    and REST2 (from task 1) with the chain's (j+1,j+1) are step j's update;
+    REST2's tiles are step j+1's REST (from task 1)."""
+    for j in range(MT - 4):
+        full = tasks(LU_TRAIL, MT, j)
+        first = tasks(LU_FIRST, MT, j)[1:]
+        row2 = tasks(LU_ROW2, MT, j)
+        rest2 = tasks(LU_REST2, MT, j)
+        assert rest2[0][2] == (j + 3, j + 3)
+        rest2 = rest2[1:]
+        outs = [t[2] for t in first] + [t[2] for t in row2] + [t[2] for t in rest2]
+        assert len(outs) == len(set(outs)) and (j + 1, j + 1) not in outs
+        assert set(first) | set(row2) | set(rest2) | {t for t in full if t[2] == (j + 1, j + 1)} == set(full)
+        assert all(j + 2 in t[2] or t[2] == (j + 3, j + 3) for t in row2)
+        nxt = tasks(LU_REST, MT, j + 1)[1:]
+        assert [t[2] for t in nxt] == [t[2] for t in rest2]
