@@ -498,6 +498,122 @@ void palial_get_task_time(struct timeval start, struct timeval end, char *name) 
 
 namespace palial {
 
+// ------------------------------------------------------------------ schedule recorder
+// Resource ids: matrix tile m + MT n; RES + packed panel tile (set, half,
+// index); 2 RES + diagonal-block inverses (2k: U side, 2k+1: L side).
+// Record of one operation (longs): type (0 launch, 1 event record, 2 event
+// wait), stream, event, kind (1 diagonal factor, 2 TRSM, 3 GEMM, 4 pack),
+// nr, nw, nx, the nr read ids, the nw written ids, then nx groups of 4:
+// GEMM (C tile, step, A operand id, B operand id), TRSM (tile, step, diagonal
+// tile, inverses), pack (tile, step, written id, -1).
+struct SchedRec {
+    bool on = false;
+    const char *mat = nullptr;    // matrix base and the element size of its tiles
+    int es = 8, b = 0, MT = 0;
+    const char *pack = nullptr;   // packed panel sets
+    size_t psb = 0;
+    const char *dinv = nullptr;
+    std::vector<long> out;
+};
+SchedRec g_rec;
+constexpr long REC_RES = 1L << 24;
+
+bool sched_rec_on() { return g_rec.on; }
+
+long rec_tile_elem(long eoff) { return eoff / ((long)g_rec.b * g_rec.b); }
+long rec_tile_ptr(const void *p) { return rec_tile_elem((long)((const char *)p - g_rec.mat) / g_rec.es); }
+long rec_pack_id(const void *p, int idx)
+{
+    const size_t off = (size_t)((const char *)p - g_rec.pack);
+    const size_t half_bytes = (size_t)g_rec.MT * g_rec.b * g_rec.b * 4;
+    const size_t set = off / g_rec.psb, rem = off % g_rec.psb;
+    const long half = (long)(rem / half_bytes), first = (long)((rem % half_bytes) / ((size_t)g_rec.b * g_rec.b * 4));
+    return REC_RES + (long)set * 2 * g_rec.MT + half * g_rec.MT + first + idx;
+}
+long rec_dinv_id(const void *p) { return 2 * REC_RES + (long)(((const char *)p - g_rec.dinv) / 8 / (32L * g_rec.b)); }
+
+void rec_put(int type, hipStream_t s, hipEvent_t e, int kind, const std::vector<long> &rd,
+             const std::vector<long> &wr, const std::vector<long> &x)
+{
+    std::vector<long> &o = g_rec.out;
+    o.push_back(type);
+    o.push_back((long)(intptr_t)s);
+    o.push_back((long)(intptr_t)e);
+    o.push_back(kind);
+    o.push_back((long)rd.size());
+    o.push_back((long)wr.size());
+    o.push_back((long)x.size() / 4);
+    o.insert(o.end(), rd.begin(), rd.end());
+    o.insert(o.end(), wr.begin(), wr.end());
+    o.insert(o.end(), x.begin(), x.end());
+}
+
+void rec_gemm(const GemmArgs &a, bool packed, hipStream_t s)
+{
+    std::vector<long> rd, wr, x;
+    const double *base = (const double *)(const void *)g_rec.mat;
+    for (int t = 0; t < a.ntasks; t++) {
+        const TaskPtrs tp = decode(a, t);
+        if (!tp.valid) continue;
+        const double *mb = a.mode == GM_SINGLE ? base : a.mat;
+        const long c = rec_tile_elem((long)(tp.C - mb));
+        rd.push_back(c);
+        wr.push_back(c);
+        const long ta = rec_tile_elem((long)(tp.A - mb)), tb = rec_tile_elem((long)(tp.B - mb));
+        // the step of a product: the smaller index of its A tile (Cholesky U(k, n),
+        // LU L(m, k)); mode launches carry it as a.k
+        const int kk = a.mode == GM_SINGLE ? (int)std::min(ta % g_rec.MT, ta / g_rec.MT) : a.k;
+        if (packed) {
+            const long pa = rec_pack_id(a.pa, tp.ia), pb = rec_pack_id(a.pb, tp.ib);
+            rd.push_back(pa);
+            rd.push_back(pb);
+            x.insert(x.end(), {c, kk, pa, pb});
+            if (a.pa2) {
+                const long pa2 = rec_pack_id(a.pa2, tp.ia - 1), pb2 = rec_pack_id(a.pb2, tp.ib - 1);
+                rd.push_back(pa2);
+                rd.push_back(pb2);
+                x.insert(x.end(), {c, (long)kk + 1, pa2, pb2});
+            }
+        } else {
+            rd.push_back(ta);
+            rd.push_back(tb);
+            x.insert(x.end(), {c, kk, ta, tb});
+        }
+    }
+    rec_put(0, s, nullptr, 3, rd, wr, x);
+}
+
+void rec_trsm(int kind, const TrsmArgs &t, const double *dinv, hipStream_t s)
+{
+    std::vector<long> rd, wr, x;
+    const long d = rec_tile_ptr(t.diag), di = rec_dinv_id(dinv);
+    rd.push_back(d);
+    rd.push_back(di);
+    for (int i = 0; i < t.ntiles; i++) {
+        const int idx = t.first + i;
+        const long tl = kind == TR_LUR ? (long)idx + (long)t.MT * t.k : (long)t.k + (long)t.MT * idx;
+        rd.push_back(tl);
+        wr.push_back(tl);
+        x.insert(x.end(), {tl, (long)t.k, d, di});
+    }
+    rec_put(0, s, nullptr, 2, rd, wr, x);
+}
+
+void rec_pack(const double *mat, int b, int k, int first, int n, bool colpanel, const void *dst, hipStream_t s)
+{
+    (void)mat;
+    (void)b;
+    std::vector<long> rd, wr, x;
+    for (int i = 0; i < n; i++) {
+        const long tl = colpanel ? (long)(first + i) + (long)g_rec.MT * k : (long)k + (long)g_rec.MT * (first + i);
+        const long pid = rec_pack_id(dst, i);
+        rd.push_back(tl);
+        wr.push_back(pid);
+        x.insert(x.end(), {tl, (long)k, pid, -1L});
+    }
+    rec_put(0, s, nullptr, 4, rd, wr, x);
+}
+
 void trace_gemm(GemmArgs &a)
 {
     if (!g_tr.active) return;
@@ -703,6 +819,11 @@ TrsmArgs targs(double *mat, int N, int b, int k)
 // is factored in FP64 in the copy and rounded back.
 hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *dinv, hipStream_t s, bool tile32 = false)
 {
+    if (sched_rec_on()) {
+        const long t = rec_tile_ptr(tile), d = rec_dinv_id(dinv);
+        rec_put(0, s, nullptr, 1, {t}, {t, d}, {t, (long)(t % g_rec.MT), d, -1L});
+        return hipSuccess;
+    }
     const bool sub = g.subtile && b >= 256 && b % 128 == 0;
     const bool copy = (sub || tile32) && dinv && g.d_sub && g.sub_cap >= (size_t)b * b;
     if (!copy) {
@@ -735,6 +856,11 @@ hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *din
 // diagonal tile.
 hipError_t getrf_tile(double *tile, int b, double *dU, double *dL, hipStream_t s, bool tile32 = false)
 {
+    if (sched_rec_on()) {
+        const long t = rec_tile_ptr(tile), u = rec_dinv_id(dU), l = rec_dinv_id(dL);
+        rec_put(0, s, nullptr, 1, {t}, {t, u, l}, {t, (long)(t % g_rec.MT), u, l});
+        return hipSuccess;
+    }
     const bool sub = g.subtile && b >= 256 && b % 128 == 0;
     const bool copy = (sub || tile32) && g.d_sub && g.sub_cap >= (size_t)b * b;
     if (!copy) {
@@ -757,6 +883,24 @@ hipError_t getrf_tile(double *tile, int b, double *dU, double *dL, hipStream_t s
             return e;
     }
     return launch_tile_split(g.d_sub, tile, b, bs, false, s, tile32);
+}
+
+// event record / wait of the chain schedules (recorded under the schedule recorder)
+hipError_t ev_record(hipEvent_t e, hipStream_t s)
+{
+    if (sched_rec_on()) {
+        rec_put(1, s, e, 0, {}, {}, {});
+        return hipSuccess;
+    }
+    return hipEventRecord(e, s);
+}
+hipError_t ev_wait(hipStream_t s, hipEvent_t e)
+{
+    if (sched_rec_on()) {
+        rec_put(2, s, e, 0, {}, {}, {});
+        return hipSuccess;
+    }
+    return hipStreamWaitEvent(s, e, 0);
 }
 
 // events of one factorization: [0, MT+1) panel k ready, [MT+1, 2MT+2) update
@@ -792,8 +936,11 @@ GemmArgs single_tile(const double *A, const double *B, double *C, int b, bool sy
 // packing of row k run on a third stream sa after F(k-1) and P(k), i.e.
 // beside E(k-1), so they leave the bulk stream's critical path; the chain
 // waits for F(k-1) only.  The bulk order F(k-1) E(k-1) F(k) E(k) keeps each
-// tile's updates in increasing k, and packed set k % 2 is rewritten by step
-// k+2's packing only after F(k+1), which follows E(k) on the bulk stream.
+// tile's updates in increasing k, and packed set k % 4 is rewritten by step
+// k+4's packing only after F(k+3), which follows every launch that reads
+// panel k on the bulk stream (paired steps read panels k and k+1 until E of
+// step k+1, so two sets would not do; palial_hip_schedule_ops records the
+// schedule for a host-side race check).
 //
 // fs (mixed tiers with packed panels): dA holds FP32 tiles (element offsets
 // of the FP64 layout).  Diagonal tiles are factored in FP64 copies, the TRSMs
@@ -880,18 +1027,18 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
     hipEvent_t *evP = ev, *evR = ev + MT + 1, *evT = ev + 2 * MT + 4, *evB = ev + 3 * MT + 6;
     hipEvent_t fork = ev[2 * MT + 2], evEnd = ev[4 * MT + 7];
     hipStream_t st = split ? sa : sb;    // stream of TB(k) and the packing
-    L.chk(hipEventRecord(fork, s0));
-    L.chk(hipStreamWaitEvent(sc, fork, 0));
-    if (sb != s0) L.chk(hipStreamWaitEvent(sb, fork, 0));
-    if (split) L.chk(hipStreamWaitEvent(sa, fork, 0));
+    L.chk(ev_record(fork, s0));
+    L.chk(ev_wait(sc, fork));
+    if (sb != s0) L.chk(ev_wait(sb, fork));
+    if (split) L.chk(ev_wait(sa, fork));
     potrf(0, sc);
-    L.chk(hipEventRecord(evP[0], sc));
+    L.chk(ev_record(evP[0], sc));
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
         if (T > 0) {
-            if (k > 0) L.chk(hipStreamWaitEvent(sc, evR[k - 1], 0));
+            if (k > 0) L.chk(ev_wait(sc, evR[k - 1]));
             trsm(k, k + 1, 1, sc);
-            L.chk(hipEventRecord(evT[k], sc));
+            L.chk(ev_record(evT[k], sc));
             if (fs) {
                 // GM_CHOL_ROW task 0 is tile (k+1,k+1) from packed tile 0 of row k
                 pack(k, 1, pk1, sc);
@@ -902,28 +1049,28 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
                      b3(b), 1, sc);
             }
             potrf(k + 1, sc);
-            L.chk(hipEventRecord(evP[k + 1], sc));
+            L.chk(ev_record(evP[k + 1], sc));
         }
-        if (split && k > 0) L.chk(hipStreamWaitEvent(sa, evR[k - 1], 0));
-        L.chk(hipStreamWaitEvent(st, evP[k], 0));
+        if (split && k > 0) L.chk(ev_wait(sa, evR[k - 1]));
+        L.chk(ev_wait(st, evP[k]));
         if (T > 1) trsm(k, k + 2, T - 1, st);
         if (T > 1 && pk) {
             // row k rounded once; the update streams the packed copy
-            L.chk(hipStreamWaitEvent(st, evT[k], 0));
+            L.chk(ev_wait(st, evT[k]));
             pack(k, T, pset(k), st);
         }
         if (split) {
-            L.chk(hipEventRecord(evB[k], sa));
-            L.chk(hipStreamWaitEvent(sb, evB[k], 0));
+            L.chk(ev_record(evB[k], sa));
+            L.chk(ev_wait(sb, evB[k]));
         }
-        if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
+        if (T > 0) L.chk(ev_wait(sb, evT[k]));
         // task 0 of each enumeration is tile (k+1,k+1) (the chain's) or (k+2,k+2) (F(k)'s)
         if (!split) {
             if (T > 1) update(GM_CHOL_TRAIL, k, 1, T * (T + 1) / 2 - 1, (double)T * (T - 1) / 2, T - 1, T, pset(k), sb);
-            L.chk(hipEventRecord(evR[k], sb));
+            L.chk(ev_record(evR[k], sb));
         } else {
             if (T > 1) update(GM_CHOL_FIRST, k, 1, T, T - 1, 1, T, pset(k), sb);
-            L.chk(hipEventRecord(evR[k], sb));
+            L.chk(ev_record(evR[k], sb));
             if (lead(k)) {
                 // row k+2 and tile (k+3, k+3); the rest waits for step k+1
                 update(GM_CHOL_ROW2, k, 0, T - 1, T - 2, 1, T - 1, pset(k), sb);
@@ -937,8 +1084,8 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
             }
         }
     }
-    L.chk(hipEventRecord(evEnd, sb));
-    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evEnd, 0));
+    L.chk(ev_record(evEnd, sb));
+    if (sb != s0) L.chk(ev_wait(s0, evEnd));
 }
 
 // Diagonal-chain schedule of the no-pivot LU (src/lu.h:14-173), same shape:
@@ -1028,19 +1175,19 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
     hipEvent_t *evP = ev, *evR = ev + MT + 1, *evT = ev + 2 * MT + 4, *evB = ev + 3 * MT + 6;
     hipEvent_t fork = ev[2 * MT + 2], evEnd = ev[4 * MT + 7];
     hipStream_t st = split ? sa : sb;
-    L.chk(hipEventRecord(fork, s0));
-    L.chk(hipStreamWaitEvent(sc, fork, 0));
-    if (sb != s0) L.chk(hipStreamWaitEvent(sb, fork, 0));
-    if (split) L.chk(hipStreamWaitEvent(sa, fork, 0));
+    L.chk(ev_record(fork, s0));
+    L.chk(ev_wait(sc, fork));
+    if (sb != s0) L.chk(ev_wait(sb, fork));
+    if (split) L.chk(ev_wait(sa, fork));
     getrf(0, sc);
-    L.chk(hipEventRecord(evP[0], sc));
+    L.chk(ev_record(evP[0], sc));
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
         if (T > 0) {
-            if (k > 0) L.chk(hipStreamWaitEvent(sc, evR[k - 1], 0));
+            if (k > 0) L.chk(ev_wait(sc, evR[k - 1]));
             trsm(TR_LUL, k, k + 1, 1, sc);
             trsm(TR_LUR, k, k + 1, 1, sc);
-            L.chk(hipEventRecord(evT[k], sc));
+            L.chk(ev_record(evT[k], sc));
             if (fs) {
                 // GM_LU_PANEL task 0 is tile (k+1,k+1) from packed tiles 0 of both panels
                 pack(k, 1, pk1U, pk1L, sc);
@@ -1051,30 +1198,30 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
                      1, sc);
             }
             getrf(k + 1, sc);
-            L.chk(hipEventRecord(evP[k + 1], sc));
+            L.chk(ev_record(evP[k + 1], sc));
         }
-        if (split && k > 0) L.chk(hipStreamWaitEvent(sa, evR[k - 1], 0));
-        L.chk(hipStreamWaitEvent(st, evP[k], 0));
+        if (split && k > 0) L.chk(ev_wait(sa, evR[k - 1]));
+        L.chk(ev_wait(st, evP[k]));
         if (T > 1) {
             trsm(TR_LUL, k, k + 2, T - 1, st);
             trsm(TR_LUR, k, k + 2, T - 1, st);
         }
         if (T > 1 && pk) {
-            L.chk(hipStreamWaitEvent(st, evT[k], 0));
+            L.chk(ev_wait(st, evT[k]));
             pack(k, T, psetU(k), psetL(k), st);
         }
         if (split) {
-            L.chk(hipEventRecord(evB[k], sa));
-            L.chk(hipStreamWaitEvent(sb, evB[k], 0));
+            L.chk(ev_record(evB[k], sa));
+            L.chk(ev_wait(sb, evB[k]));
         }
-        if (T > 0) L.chk(hipStreamWaitEvent(sb, evT[k], 0));
+        if (T > 0) L.chk(ev_wait(sb, evT[k]));
         // task 0 of each enumeration is tile (k+1,k+1) (the chain's) or (k+2,k+2) (F(k)'s)
         if (!split) {
             if (T > 1) update(GM_LU_TRAIL, k, 1, T * T - 1, T, psetU(k), psetL(k), sb);
-            L.chk(hipEventRecord(evR[k], sb));
+            L.chk(ev_record(evR[k], sb));
         } else {
             if (T > 1) update(GM_LU_FIRST, k, 1, 2 * T - 1, T, psetU(k), psetL(k), sb);
-            L.chk(hipEventRecord(evR[k], sb));
+            L.chk(ev_record(evR[k], sb));
             if (lead(k)) {
                 // column and row k+2 and tile (k+3, k+3); the rest waits for step k+1
                 update(GM_LU_ROW2, k, 0, 2 * T - 3, T - 1, psetU(k), psetL(k), sb);
@@ -1087,8 +1234,8 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
             }
         }
     }
-    L.chk(hipEventRecord(evEnd, sb));
-    if (sb != s0) L.chk(hipStreamWaitEvent(s0, evEnd, 0));
+    L.chk(ev_record(evEnd, sb));
+    if (sb != s0) L.chk(ev_wait(s0, evEnd));
 }
 
 // Bulk stream of the chain schedule.  The mixed tiers are bound by the
@@ -2370,6 +2517,61 @@ int palial_hip_blkinv_tiles(int mode, int MT, int b, int q, int bd, int bt, long
         rel(tp.A, o); rel(tp.B, o + 2); rel(tp.C, o + 4); rel(tp.D, o + 6);
     }
     return ntasks;
+}
+
+int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int pair, long *out, long cap)
+{
+    // the chain schedules of the solves: FP64 (prec 0) on the FP64 copy, the
+    // mixed tiers on the FP32 working copy with packed panels (b % 128 == 0)
+    if (algo < 0 || algo > 1 || MT < 2 || MT > 64 || prec < 0 || prec > 4 || !panel2_ok(b) ||
+        (prec > 0 && b % 128) || pair < 0 || pair > 1)
+        return PALIAL_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx live;
+    std::swap(g, live);                  // the schedule runs on fake handles and addresses only
+    const bool prof = g_prof.on, tr = g_tr.active;
+    g_prof.on = false;
+    g_tr.active = false;
+    const int N = MT * b;
+    const bool fs = prec > 0;
+    const size_t psb = pack_set_bytes(b, MT);
+    char *const fake = (char *)((size_t)1 << 44);          // never dereferenced
+    g.dag = 2;
+    g.subtile = 1;
+    g.f32work = 1;
+    g.pair = pair;
+    g.d_pack = fake + ((size_t)1 << 40);
+    g.pack_cap = PACK_SETS * psb + 2 * (size_t)b * b * 4;
+    g.d_dinv = (double *)(fake + ((size_t)2 << 40));
+    g.dinv_cap = (size_t)MT * 64 * b;
+    g.d_sub = (double *)(fake + ((size_t)3 << 40));
+    g.sub_cap = (size_t)b * b;
+    g.events.resize(events_needed(MT));
+    for (size_t i = 0; i < g.events.size(); i++) g.events[i] = (hipEvent_t)(intptr_t)(1000 + i);
+    g_rec = SchedRec();
+    g_rec.on = true;
+    g_rec.mat = fake;
+    g_rec.es = fs ? 4 : 8;
+    g_rec.b = b;
+    g_rec.MT = MT;
+    g_rec.pack = (const char *)g.d_pack;
+    g_rec.psb = psb;
+    g_rec.dinv = (const char *)g.d_dinv;
+    hipStream_t s0 = (hipStream_t)(intptr_t)1, sc = (hipStream_t)(intptr_t)2, sb = (hipStream_t)(intptr_t)3,
+                sa = (hipStream_t)(intptr_t)4;
+    Launcher L;
+    double *dA = (double *)(void *)fake;
+    if (algo == 0) dag_cholesky_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, (int *)(fake + 8), L, prec, fs);
+    else dag_lu_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, L, prec, fs);
+    std::vector<long> rec;
+    rec.swap(g_rec.out);
+    g_rec = SchedRec();
+    std::swap(g, live);
+    g_prof.on = prof;
+    g_tr.active = tr;
+    if (L.err != hipSuccess) return fail(PALIAL_ERR_ARG, "schedule refused its arguments");
+    if (out && cap >= (long)rec.size()) std::copy(rec.begin(), rec.end(), out);
+    return (int)rec.size();
 }
 
 int palial_hip_set_option(const char *name, int value)

@@ -358,4 +358,15 @@ hipError_t launch_tiles_to_cols(const double *tiles, int N, int b, double *v, in
 // (column-major, ld 32), as the factorization kernels write them (b % 32 == 0)
 hipError_t launch_upper_block_inverses(const double *mat, int N, int b, double *dinv, long stride, hipStream_t s);
 
+// ------------------------------------------------------------------ schedule recorder
+// Host test hook: while sched_rec_on() the launchers of the factorization
+// schedules record each launch (its stream and the tiles, packed panel tiles
+// and diagonal-block inverses it reads and writes) instead of launching it,
+// and the schedules record their event records and waits, so that the happens-
+// before order of a whole schedule can be checked without a GPU.
+bool sched_rec_on();
+void rec_gemm(const GemmArgs &a, bool packed, hipStream_t s);
+void rec_trsm(int kind, const TrsmArgs &t, const double *dinv, hipStream_t s);
+void rec_pack(const double *mat, int b, int k, int first, int n, bool colpanel, const void *dst, hipStream_t s);
+
 }  // namespace palial

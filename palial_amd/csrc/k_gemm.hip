@@ -366,6 +366,7 @@ static int gemm_variant()
 hipError_t launch_gemm_fine(const GemmArgs &a_in, bool ta, bool tb, hipStream_t s)
 {
     if (a_in.ntasks <= 0) return hipSuccess;
+    if (sched_rec_on()) { rec_gemm(a_in, false, s); return hipSuccess; }
     if (a_in.b % 32 != 0) return launch_gemm(a_in, ta, tb, s);
     GemmArgs a = a_in;
     trace_gemm(a);
@@ -375,6 +376,7 @@ hipError_t launch_gemm_fine(const GemmArgs &a_in, bool ta, bool tb, hipStream_t 
 hipError_t launch_gemm(const GemmArgs &a_in, bool ta, bool tb, hipStream_t s)
 {
     if (a_in.ntasks <= 0) return hipSuccess;
+    if (sched_rec_on()) { rec_gemm(a_in, false, s); return hipSuccess; }
     GemmArgs a = a_in;
     if (a.b % 32 == 0) trace_gemm(a);     // the scalar fallback below keeps no timestamps
     if (a.b % 128 == 0) {

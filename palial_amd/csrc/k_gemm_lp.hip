@@ -312,6 +312,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(GemmArgs p)
 hipError_t launch_gemm_lp(const GemmArgs &a_in, bool ta, bool tb, int prec, hipStream_t s)
 {
     if (a_in.ntasks <= 0) return hipSuccess;
+    if (sched_rec_on()) { rec_gemm(a_in, false, s); return hipSuccess; }
     GemmArgs a = a_in;
     trace_gemm(a);
     if (a.b % 128 != 0 || a.maskA != MK_NONE) return hipErrorInvalidValue;

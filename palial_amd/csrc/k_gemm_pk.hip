@@ -705,6 +705,7 @@ hipError_t launch_pack(const double *mat, int N, int b, int k, int first, int n,
                        void *dst, hipStream_t s, bool src32)
 {
     if (n <= 0) return hipSuccess;
+    if (sched_rec_on()) { rec_pack(mat, b, k, first, n, colpanel, dst, s); return hipSuccess; }
     if (b % 64 != 0 || first <= k || first + n > N / b || (prec != 1 && prec != 2 && prec != 3 && prec != 4))
         return hipErrorInvalidValue;
     const int nsb = b / 64, cp = colpanel ? 1 : 0;
@@ -731,6 +732,7 @@ hipError_t launch_gemm_packed(const GemmArgs &a_in, int prec, hipStream_t s)
 {
     if (a_in.ntasks <= 0) return hipSuccess;
     GemmArgs a = a_in;
+    if (sched_rec_on()) { rec_gemm(a, true, s); return hipSuccess; }
     trace_gemm(a);
     if (a.b % 128 != 0 || a.maskA != MK_NONE || !a.pa || !a.pb) return hipErrorInvalidValue;
     if (a.mode != GM_CHOL_TRAIL && a.mode != GM_CHOL_REST && a.mode != GM_CHOL_ROW && a.mode != GM_LU_TRAIL &&

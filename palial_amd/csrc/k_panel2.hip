@@ -995,6 +995,7 @@ hipError_t launch_batch(const BatchArgs &a, int bmax, int grid, hipStream_t s)
 hipError_t launch_trsm2(int kind, const TrsmArgs &a_in, const double *dinv, hipStream_t s)
 {
     if (a_in.ntiles <= 0) return hipSuccess;
+    if (sched_rec_on()) { rec_trsm(kind, a_in, dinv, s); return hipSuccess; }
     TrsmArgs a = a_in;
     trace_trsm(kind, a);
     dim3 grid((unsigned)(a.ntiles * (a.b / NB2))), block(P2_THREADS);
