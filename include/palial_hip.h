@@ -206,21 +206,26 @@ int palial_hip_gather_map(int MT, int b, int q, int upper, int dtrans, long *out
 int palial_hip_blkinv_plan(int q, double *out, int cap);
 int palial_hip_blkinv_tiles(int mode, int MT, int b, int q, int bd, int bt, long *out, long cap);
 
-/* Test hook (no counterpart in the reference): the diagonal-chain schedule
- * of a solve's factorization (algo 0 Cholesky, 1 LU; MT tiles of b per side;
- * prec 0 the FP64 copy, 1..4 the FP32 working copy with packed panels; pair:
- * paired BF16 updates), recorded on the host instead of launched.  Writes a
- * stream of longs, one record per launch, event record or event wait in issue
- * order: type (0 launch, 1 record, 2 wait), stream (1 caller, 2 diagonal
- * chain, 3 bulk, 4 row TRSMs and packing), event, kind (1 diagonal factor,
- * 2 TRSM, 3 GEMM, 4 pack), nr, nw, nx, nr resource ids read, nw written,
- * and nx groups of 4 (GEMM: output tile, step, A and B operand ids; TRSM:
- * tile, step, diagonal tile, inverses; pack: tile, step, packed id, -1;
- * diagonal factor: tile, step, inverses U, L or -1).  Resource ids: matrix
- * tile m + MT n; 2^24 + packed panel tile; 2^25 + diagonal-block inverses
- * (2k U side, 2k + 1 L side).  Returns the number of longs (written when cap
- * holds them), PALIAL_ERR_ARG for arguments the schedule does not take. */
-int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int pair, long *out, long cap);
+/* Test hook (no counterpart in the reference): a factorization schedule,
+ * recorded on the host instead of launched: the diagonal-chain schedule of a
+ * solve (algo 0 Cholesky, 1 LU; prec 0 the FP64 copy, 1..4 the FP32 working
+ * copy with packed panels; variant 1: paired BF16 updates) or the QR schedule
+ * (algo 2, prec 0, variant = panel chains in flight, 1..8); MT tiles of b per
+ * side.  Writes a stream of longs, one record per launch, event record or
+ * event wait in issue order: type (0 launch, 1 record, 2 wait), stream (1
+ * caller, 2 diagonal chain, 3 bulk, 4 row TRSMs and packing; QR: 10 + i panel
+ * chain i, 20 + i update stream i), event, kind (1 diagonal factor, 2 TRSM,
+ * 3 GEMM, 4 pack, 5 QR panel), mode (the internal GEMM mode; QR panel 0
+ * GEQRT, 1 TSQRT), k, m, nr, nw, nx, nr resource ids read, nw written, and nx
+ * groups of 4 (GEMM: output tile, step, A and B operand ids; TRSM: tile,
+ * step, diagonal tile, inverses; pack: tile, step, packed id, -1; diagonal
+ * factor: tile, step, inverses U, L or -1).  Resource ids: matrix tile
+ * m + MT n; 2^24 + packed panel tile; 2^25 + diagonal-block inverses (2k U
+ * side, 2k + 1 L side); 3 * 2^24 + T-factor tile, 4 * 2^24 and 5 * 2^24 + W
+ * and op(T) W workspace tiles, 6 * 2^24 + tau and 7 * 2^24 + Gram workspace of
+ * a panel chain.  Returns the number of longs (written when cap holds them),
+ * PALIAL_ERR_ARG for arguments the schedule does not take. */
+int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int variant, long *out, long cap);
 
 /* Test hook (no counterpart in the reference): average device time of one
  * batched trailing-update launch of step k (algo 0 Cholesky, 1 LU) over reps

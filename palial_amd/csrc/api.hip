@@ -500,12 +500,23 @@ namespace palial {
 
 // ------------------------------------------------------------------ schedule recorder
 // Resource ids: matrix tile m + MT n; RES + packed panel tile (set, half,
-// index); 2 RES + diagonal-block inverses (2k: U side, 2k+1: L side).
+// index); 2 RES + diagonal-block inverses (2k: U side, 2k+1: L side); then
+// the QR buffers by address region: 3 RES + T-factor tile, 4 RES / 5 RES +
+// W / op(T) W workspace tile (per update stream), 6 RES + tau and 7 RES + Gram
+// workspace of a panel chain.
 // Record of one operation (longs): type (0 launch, 1 event record, 2 event
-// wait), stream, event, kind (1 diagonal factor, 2 TRSM, 3 GEMM, 4 pack),
-// nr, nw, nx, the nr read ids, the nw written ids, then nx groups of 4:
-// GEMM (C tile, step, A operand id, B operand id), TRSM (tile, step, diagonal
-// tile, inverses), pack (tile, step, written id, -1).
+// wait), stream, event, kind (1 diagonal factor, 2 TRSM, 3 GEMM, 4 pack,
+// 5 QR panel), mode (GEMM mode; QR panel: 0 GEQRT, 1 TSQRT), k, m, nr, nw,
+// nx, the nr read ids, the nw written ids, then nx groups of 4: GEMM (C tile,
+// step, A operand id, B operand id), TRSM (tile, step, diagonal tile,
+// inverses), pack (tile, step, written id, -1), diagonal factor (tile, step,
+// inverses, inverses or -1).
+constexpr long REC_RES = 1L << 24;
+struct RecRegion {
+    const char *base;
+    size_t bytes, unit;   // region size and bytes per resource
+    long id0;
+};
 struct SchedRec {
     bool on = false;
     const char *mat = nullptr;    // matrix base and the element size of its tiles
@@ -513,10 +524,10 @@ struct SchedRec {
     const char *pack = nullptr;   // packed panel sets
     size_t psb = 0;
     const char *dinv = nullptr;
+    std::vector<RecRegion> regions;   // further address regions (QR buffers)
     std::vector<long> out;
 };
 SchedRec g_rec;
-constexpr long REC_RES = 1L << 24;
 
 bool sched_rec_on() { return g_rec.on; }
 
@@ -531,15 +542,26 @@ long rec_pack_id(const void *p, int idx)
     return REC_RES + (long)set * 2 * g_rec.MT + half * g_rec.MT + first + idx;
 }
 long rec_dinv_id(const void *p) { return 2 * REC_RES + (long)(((const char *)p - g_rec.dinv) / 8 / (32L * g_rec.b)); }
+// any recorded address: a matrix tile, or a resource of one of the regions
+long rec_id(const void *p)
+{
+    const char *c = (const char *)p;
+    for (const RecRegion &r : g_rec.regions)
+        if (c >= r.base && c < r.base + r.bytes) return r.id0 + (long)((size_t)(c - r.base) / r.unit);
+    return rec_tile_ptr(p);
+}
 
 void rec_put(int type, hipStream_t s, hipEvent_t e, int kind, const std::vector<long> &rd,
-             const std::vector<long> &wr, const std::vector<long> &x)
+             const std::vector<long> &wr, const std::vector<long> &x, int mode = 0, int k = -1, int m = -1)
 {
     std::vector<long> &o = g_rec.out;
     o.push_back(type);
     o.push_back((long)(intptr_t)s);
     o.push_back((long)(intptr_t)e);
     o.push_back(kind);
+    o.push_back(mode);
+    o.push_back(k);
+    o.push_back(m);
     o.push_back((long)rd.size());
     o.push_back((long)wr.size());
     o.push_back((long)x.size() / 4);
@@ -551,15 +573,29 @@ void rec_put(int type, hipStream_t s, hipEvent_t e, int kind, const std::vector<
 void rec_gemm(const GemmArgs &a, bool packed, hipStream_t s)
 {
     std::vector<long> rd, wr, x;
-    const double *base = (const double *)(const void *)g_rec.mat;
     for (int t = 0; t < a.ntasks; t++) {
         const TaskPtrs tp = decode(a, t);
         if (!tp.valid) continue;
-        const double *mb = a.mode == GM_SINGLE ? base : a.mat;
-        const long c = rec_tile_elem((long)(tp.C - mb));
-        rd.push_back(c);
+        // matrix operands as element offsets (the FP32 working copy is addressed
+        // through the FP64 view of its base), the rest by address region
+        auto id = [&](const double *ptr) -> long {
+            if (!ptr) return -1;
+            const bool in_mat = (const char *)ptr >= g_rec.mat && (const char *)ptr < g_rec.mat + ((size_t)1 << 39);
+            if (in_mat && a.mode != GM_SINGLE) return rec_tile_elem((long)(ptr - a.mat));
+            return rec_id(ptr);
+        };
+        const long c = id(tp.C);
+        if (tp.D) rd.push_back(id(tp.D));
         wr.push_back(c);
-        const long ta = rec_tile_elem((long)(tp.A - mb)), tb = rec_tile_elem((long)(tp.B - mb));
+        if (tp.E) {
+            rd.push_back(id(tp.E));
+            wr.push_back(id(tp.E));
+        }
+        long ta = id(tp.A);
+        const long tb = id(tp.B);
+        // the unit-lower V of a QR diagonal tile is a resource of its own: the
+        // TSQRTs of step k rewrite only the tile's upper R while ORMQR(k) reads V
+        if (a.maskA == MK_UNIT_LOWER && ta >= 0 && ta < REC_RES && ta % g_rec.MT == ta / g_rec.MT) ta += 8 * REC_RES;
         // the step of a product: the smaller index of its A tile (Cholesky U(k, n),
         // LU L(m, k)); mode launches carry it as a.k
         const int kk = a.mode == GM_SINGLE ? (int)std::min(ta % g_rec.MT, ta / g_rec.MT) : a.k;
@@ -580,7 +616,7 @@ void rec_gemm(const GemmArgs &a, bool packed, hipStream_t s)
             x.insert(x.end(), {c, kk, ta, tb});
         }
     }
-    rec_put(0, s, nullptr, 3, rd, wr, x);
+    rec_put(0, s, nullptr, 3, rd, wr, x, a.mode, a.k, a.m);
 }
 
 void rec_trsm(int kind, const TrsmArgs &t, const double *dinv, hipStream_t s)
@@ -596,7 +632,7 @@ void rec_trsm(int kind, const TrsmArgs &t, const double *dinv, hipStream_t s)
         wr.push_back(tl);
         x.insert(x.end(), {tl, (long)t.k, d, di});
     }
-    rec_put(0, s, nullptr, 2, rd, wr, x);
+    rec_put(0, s, nullptr, 2, rd, wr, x, kind, t.k, -1);
 }
 
 void rec_pack(const double *mat, int b, int k, int first, int n, bool colpanel, const void *dst, hipStream_t s)
@@ -611,7 +647,28 @@ void rec_pack(const double *mat, int b, int k, int first, int n, bool colpanel, 
         wr.push_back(pid);
         x.insert(x.end(), {tl, (long)k, pid, -1L});
     }
-    rec_put(0, s, nullptr, 4, rd, wr, x);
+    rec_put(0, s, nullptr, 4, rd, wr, x, colpanel ? 1 : 0, k, -1);
+}
+
+// QR panels: GEQRT (A2 == nullptr) of tile A1, or TSQRT of [A1; A2]; T
+// factor, tau and Gram workspace written
+void rec_qr_panel(const double *A1, const double *A2, const double *T, const double *tau, const double *G,
+                  hipStream_t s)
+{
+    std::vector<long> rd, wr;
+    const long a1 = rec_id(A1);
+    rd.push_back(a1);
+    wr.push_back(a1);
+    if (!A2) wr.push_back(a1 + 8 * REC_RES);    // GEQRT writes V (below the diagonal) as well
+    if (A2) {
+        rd.push_back(rec_id(A2));
+        wr.push_back(rec_id(A2));
+    }
+    wr.push_back(rec_id(T));
+    wr.push_back(rec_id(tau));
+    if (G) wr.push_back(rec_id(G));
+    const long t2 = A2 ? rec_id(A2) : a1;
+    rec_put(0, s, nullptr, 5, rd, wr, {}, A2 ? 1 : 0, (int)(a1 / g_rec.MT), (int)(t2 % g_rec.MT));
 }
 
 void trace_gemm(GemmArgs &a)
@@ -821,7 +878,7 @@ hipError_t potrf_tile(double *tile, int b, int *info, int info_base, double *din
 {
     if (sched_rec_on()) {
         const long t = rec_tile_ptr(tile), d = rec_dinv_id(dinv);
-        rec_put(0, s, nullptr, 1, {t}, {t, d}, {t, (long)(t % g_rec.MT), d, -1L});
+        rec_put(0, s, nullptr, 1, {t}, {t, d}, {t, (long)(t % g_rec.MT), d, -1L}, 0, (int)(t % g_rec.MT), -1);
         return hipSuccess;
     }
     const bool sub = g.subtile && b >= 256 && b % 128 == 0;
@@ -858,7 +915,7 @@ hipError_t getrf_tile(double *tile, int b, double *dU, double *dL, hipStream_t s
 {
     if (sched_rec_on()) {
         const long t = rec_tile_ptr(tile), u = rec_dinv_id(dU), l = rec_dinv_id(dL);
-        rec_put(0, s, nullptr, 1, {t}, {t, u, l}, {t, (long)(t % g_rec.MT), u, l});
+        rec_put(0, s, nullptr, 1, {t}, {t, u, l}, {t, (long)(t % g_rec.MT), u, l}, 1, (int)(t % g_rec.MT), -1);
         return hipSuccess;
     }
     const bool sub = g.subtile && b >= 256 && b % 128 == 0;
@@ -1496,34 +1553,34 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1
     hipEvent_t *E = g.events.data();                 // E[(k & 1) MT + m]: TSMQR(k, m) done
     hipEvent_t evG = g.events[2 * MT], evQ = g.events[2 * MT + 1], fork = g.events[2 * MT + 2];
     hipEvent_t *join = g.events.data() + 2 * MT + 3;   // 2 ns events
-    L.chk(hipEventRecord(fork, s0));
+    L.chk(ev_record(fork, s0));
     for (int i = 0; i < ns; i++) {
-        L.chk(hipStreamWaitEvent(chain[i], fork, 0));
-        L.chk(hipStreamWaitEvent(bulk[i], fork, 0));
+        L.chk(ev_wait(chain[i], fork));
+        L.chk(ev_wait(bulk[i], fork));
     }
     for (int k = 0; k < MT; k++) {
         const int c = k % ns, u = k % ns;
         hipStream_t sc = chain[c], sb = bulk[u];
-        if (k > 0) L.chk(hipStreamWaitEvent(sc, E[((k - 1) & 1) * MT + k], 0));
+        if (k > 0) L.chk(ev_wait(sc, E[((k - 1) & 1) * MT + k]));
         geqrt(k, sc, c);
         if (k + 1 == MT) break;
-        L.chk(hipEventRecord(evG, sc));
-        L.chk(hipStreamWaitEvent(sb, evG, 0));
+        L.chk(ev_record(evG, sc));
+        L.chk(ev_wait(sb, evG));
         ormqr(k, sb, u);
         for (int m = k + 1; m < MT; m++) {
-            if (k > 0) L.chk(hipStreamWaitEvent(sc, E[((k - 1) & 1) * MT + m], 0));
+            if (k > 0) L.chk(ev_wait(sc, E[((k - 1) & 1) * MT + m]));
             tsqrt(k, m, sc, c);
-            L.chk(hipEventRecord(evQ, sc));
-            L.chk(hipStreamWaitEvent(sb, evQ, 0));
+            L.chk(ev_record(evQ, sc));
+            L.chk(ev_wait(sb, evQ));
             tsmqr(k, m, sb, u);
-            L.chk(hipEventRecord(E[(k & 1) * MT + m], sb));
+            L.chk(ev_record(E[(k & 1) * MT + m], sb));
         }
     }
     for (int i = 0; i < ns; i++) {
-        L.chk(hipEventRecord(join[2 * i], chain[i]));
-        L.chk(hipStreamWaitEvent(s0, join[2 * i], 0));
-        L.chk(hipEventRecord(join[2 * i + 1], bulk[i]));
-        L.chk(hipStreamWaitEvent(s0, join[2 * i + 1], 0));
+        L.chk(ev_record(join[2 * i], chain[i]));
+        L.chk(ev_wait(s0, join[2 * i]));
+        L.chk(ev_record(join[2 * i + 1], bulk[i]));
+        L.chk(ev_wait(s0, join[2 * i + 1]));
     }
 }
 
@@ -2519,12 +2576,15 @@ int palial_hip_blkinv_tiles(int mode, int MT, int b, int q, int bd, int bt, long
     return ntasks;
 }
 
-int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int pair, long *out, long cap)
+int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int variant, long *out, long cap)
 {
     // the chain schedules of the solves: FP64 (prec 0) on the FP64 copy, the
-    // mixed tiers on the FP32 working copy with packed panels (b % 128 == 0)
-    if (algo < 0 || algo > 1 || MT < 2 || MT > 64 || prec < 0 || prec > 4 || !panel2_ok(b) ||
-        (prec > 0 && b % 128) || pair < 0 || pair > 1)
+    // mixed tiers on the FP32 working copy with packed panels (b % 128 == 0);
+    // QR (algo 2, FP64) with `variant` panel chains in flight
+    const int pair = algo < 2 ? variant : 0;
+    if (algo < 0 || algo > 2 || MT < 2 || MT > 64 || prec < 0 || prec > 4 || !panel2_ok(b) ||
+        (prec > 0 && b % 128) || (algo < 2 && (variant < 0 || variant > 1)) ||
+        (algo == 2 && (prec != 0 || variant < 1 || variant > QR_MAXS)))
         return PALIAL_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_mu);
     Ctx live;
@@ -2561,8 +2621,31 @@ int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int pair, long *o
                 sa = (hipStream_t)(intptr_t)4;
     Launcher L;
     double *dA = (double *)(void *)fake;
-    if (algo == 0) dag_cholesky_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, (int *)(fake + 8), L, prec, fs);
-    else dag_lu_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, L, prec, fs);
+    if (algo == 2) {
+        // QR buffers: T factors, the W / op(T) W tiles of every update stream, tau and
+        // Gram workspace of every panel chain, each a region of its own
+        const size_t bb8 = (size_t)b * b * 8;
+        double *dS = (double *)(fake + ((size_t)4 << 40));
+        g.d_w1 = (double *)(fake + ((size_t)5 << 40));
+        g.d_w2 = (double *)(fake + ((size_t)6 << 40));
+        g.d_tau = (double *)(fake + ((size_t)7 << 40));
+        g.d_qg = (double *)(fake + ((size_t)8 << 40));
+        g_rec.regions = {{(const char *)dS, (size_t)MT * MT * bb8, bb8, 3 * REC_RES},
+                         {(const char *)g.d_w1, (size_t)QR_MAXS * MT * bb8, bb8, 4 * REC_RES},
+                         {(const char *)g.d_w2, (size_t)QR_MAXS * MT * bb8, bb8, 5 * REC_RES},
+                         {(const char *)g.d_tau, (size_t)QR_MAXS * b * 8, (size_t)b * 8, 6 * REC_RES},
+                         {(const char *)g.d_qg, (size_t)QR_MAXS * bb8, bb8, 7 * REC_RES}};
+        for (int i = 0; i < QR_MAXS; i++) {
+            g.s_qr[i] = (hipStream_t)(intptr_t)(10 + i);
+            g.s_qrb[i] = (hipStream_t)(intptr_t)(20 + i);
+        }
+        g.qr_streams = QR_MAXS;
+        dag_qr(dA, dS, N, b, s0, sc, true, L, variant);
+    } else if (algo == 0) {
+        dag_cholesky_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, (int *)(fake + 8), L, prec, fs);
+    } else {
+        dag_lu_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, L, prec, fs);
+    }
     std::vector<long> rec;
     rec.swap(g_rec.out);
     g_rec = SchedRec();

@@ -368,5 +368,7 @@ bool sched_rec_on();
 void rec_gemm(const GemmArgs &a, bool packed, hipStream_t s);
 void rec_trsm(int kind, const TrsmArgs &t, const double *dinv, hipStream_t s);
 void rec_pack(const double *mat, int b, int k, int first, int n, bool colpanel, const void *dst, hipStream_t s);
+void rec_qr_panel(const double *A1, const double *A2, const double *T, const double *tau, const double *G,
+                  hipStream_t s);
 
 }  // namespace palial

@@ -694,6 +694,7 @@ bool blocked_ok(int b, const double *G) { return G && b % QP == 0 && b >= QP && 
 
 hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s, double *G)
 {
+    if (sched_rec_on()) { rec_qr_panel(A, nullptr, T, tau, G, s); return hipSuccess; }
     if (blocked_ok(b, G)) return launch_blocked(true, A, nullptr, T, tau, G, b, s);
     hipLaunchKernelGGL(geqrt_kernel, dim3(1), dim3(QR_THREADS), (size_t)2 * b * 8, s, A, T, tau, b);
     return hipGetLastError();
@@ -701,6 +702,7 @@ hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s,
 
 hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s, double *G)
 {
+    if (sched_rec_on()) { rec_qr_panel(A1, A2, T, tau, G, s); return hipSuccess; }
     if (blocked_ok(b, G)) return launch_blocked(false, A1, A2, T, tau, G, b, s);
     hipLaunchKernelGGL(tsqrt_kernel, dim3(1), dim3(QR_THREADS), (size_t)b * 8, s, A1, A2, T, tau, b);
     return hipGetLastError();

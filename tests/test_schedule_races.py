@@ -15,7 +15,9 @@ and this test checks it as a race detector would:
   were solved and packed for that step, and every tile ends solved.
 
 FP64, FP32, BF16 (step by step and with paired updates), FP8 and split BF16
-schedules of Cholesky and LU are covered."""
+schedules of Cholesky and LU are covered, and the QR schedule with 1 to 8
+panel chains in flight (src/qr.h:15-192), whose per-tile write order must
+be the reference's and whose reads must see the reference's last writer."""
 import ctypes as C
 
 import numpy as np
@@ -37,15 +39,15 @@ def ops(algo, MT, b, prec, pair):
     v = np.array(buf[:], dtype=np.int64)
     out, i = [], 0
     while i < n:
-        typ, st, ev, kind, nr, nw, nx = (int(x) for x in v[i:i + 7])
-        i += 7
+        typ, st, ev, kind, mode, k, m, nr, nw, nx = (int(x) for x in v[i:i + 10])
+        i += 10
         rd = set(v[i:i + nr].tolist())
         i += nr
         wr = set(v[i:i + nw].tolist())
         i += nw
         x = v[i:i + 4 * nx].reshape(nx, 4).tolist()
         i += 4 * nx
-        out.append(dict(type=typ, stream=st, ev=ev, kind=kind, rd=rd, wr=wr, x=x))
+        out.append(dict(type=typ, stream=st, ev=ev, kind=kind, mode=mode, k=k, m=m, rd=rd, wr=wr, x=x))
     return out
 
 
@@ -165,3 +167,72 @@ def test_schedule_hook_rejects():
     L.palial_hip_schedule_ops.argtypes = [C.c_int] * 5 + [C.POINTER(C.c_long), C.c_long]
     assert L.palial_hip_schedule_ops(2, 8, 128, 0, 0, None, 0) < 0
     assert L.palial_hip_schedule_ops(0, 8, 64, 2, 0, None, 0) < 0       # packed panels need b % 128 == 0
+
+
+QR_W, QR_TW, QR_VW, ORM_W, ORM_TW, ORM_VW = 3, 4, 5, 6, 7, 8
+QRPANEL = 5
+WORKSPACE = 4 * RES     # W, op(T) W, tau and Gram workspace ids start here (S tiles below)
+
+
+def qr_label(o):
+    if o["kind"] == QRPANEL:
+        return ("G", o["k"]) if o["mode"] == 0 else ("Q", o["k"], o["m"])
+    if o["mode"] in (ORM_W, ORM_TW, ORM_VW):
+        return ("O", o["k"])
+    assert o["mode"] in (QR_W, QR_TW, QR_VW)
+    return ("T", o["k"], o["m"])
+
+
+def qr_reference(MT):
+    """The reference's loop order (src/qr.h:17-185) with the resources each
+    task writes and reads (tiles m + MT n, V of diagonal tile k at 8 RES + ...,
+    T-factor tiles at 3 RES + ...)."""
+    def t(m, n):
+        return m + MT * n
+    def V(k):
+        return 8 * RES + t(k, k)
+    def S(m, n):
+        return 3 * RES + t(m, n)
+    ops = []
+    for k in range(MT):
+        ops.append((("G", k), {t(k, k), V(k), S(k, k)}, {t(k, k)}))
+        if k + 1 == MT:
+            break
+        ops.append((("O", k), {t(k, n) for n in range(k + 1, MT)},
+                    {V(k), S(k, k)} | {t(k, n) for n in range(k + 1, MT)}))
+        for m in range(k + 1, MT):
+            ops.append((("Q", k, m), {t(k, k), t(m, k), S(m, k)}, {t(k, k), t(m, k)}))
+            ops.append((("T", k, m), {t(k, n) for n in range(k + 1, MT)} | {t(m, n) for n in range(k + 1, MT)},
+                        {t(m, k), S(m, k)} | {t(k, n) for n in range(k + 1, MT)} | {t(m, n) for n in range(k + 1, MT)}))
+    return ops
+
+
+def writer_history(ops_):
+    """Per resource: the labels of its writers in order; per (label, resource
+    read): the label of the resource's last writer before it (None: input)."""
+    hist, seen = {}, {}
+    for lab, wr, rd in ops_:
+        for r in rd:
+            if r < WORKSPACE or r >= 8 * RES:
+                seen.setdefault((lab, r), hist.get(r, [None])[-1])
+        for r in wr:
+            if r < WORKSPACE or r >= 8 * RES:
+                h = hist.setdefault(r, [])
+                if not h or h[-1] != lab:
+                    h.append(lab)
+    return hist, seen
+
+
+@pytest.mark.parametrize("MT", [3, 5, 8])
+@pytest.mark.parametrize("ns", [1, 2, 4, 8])
+def test_qr_schedule_race_free_and_in_reference_order(MT, ns):
+    seq = ops(2, MT, 128, 0, ns)
+    launches = happens_before(seq)
+    streams = {o["stream"] for o, _, _, _ in launches}
+    assert streams <= {10 + i for i in range(ns)} | {20 + i for i in range(ns)}
+    assert check_races(launches) > 0
+    got = [(qr_label(o), o["wr"], o["rd"]) for o, _, _, _ in launches]
+    hist_g, seen_g = writer_history(got)
+    hist_r, seen_r = writer_history(qr_reference(MT))
+    assert hist_g == hist_r
+    assert seen_g == seen_r
