@@ -208,8 +208,11 @@ int palial_hip_blkinv_tiles(int mode, int MT, int b, int q, int bd, int bt, long
 
 /* Test hook (no counterpart in the reference): a factorization schedule,
  * recorded on the host instead of launched: the diagonal-chain schedule of a
- * solve (algo 0 Cholesky, 1 LU; prec 0 the FP64 copy, 1..4 the FP32 working
- * copy with packed panels; variant 1: paired BF16 updates) or the QR schedule
+ * solve or factorization (algo 0 Cholesky, 1 LU; prec 0 the FP64 copy, 1..4
+ * the FP32 working copy with packed panels; variant = pair + 2 * schedule,
+ * pair 1: paired BF16 updates, schedule 0 diagonal chain with the split
+ * update, 1 chain with one update launch, 2 row look-ahead, 3 one stream, the
+ * mixed tiers taking schedule 0 only) or the QR schedule
  * (algo 2, prec 0, variant = panel chains in flight, 1..8); MT tiles of b per
  * side.  Writes a stream of longs, one record per launch, event record or
  * event wait in issue order: type (0 launch, 1 record, 2 wait), stream (1

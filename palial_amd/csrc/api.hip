@@ -1366,23 +1366,23 @@ void dag_cholesky(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipS
     hipEvent_t *evP = g.events.data();          // panel k ready
     hipEvent_t *evU = g.events.data() + MT + 1; // rest-update k done
     hipEvent_t fork = g.events[2 * MT + 2];
-    L.chk(hipEventRecord(fork, s0));
-    L.chk(hipStreamWaitEvent(s1, fork, 0));
+    L.chk(ev_record(fork, s0));
+    L.chk(ev_wait(s1, fork));
     potrf(0, s1);
     trsm(0, s1);
-    L.chk(hipEventRecord(evP[0], s1));
+    L.chk(ev_record(evP[0], s1));
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
         if (T > 0) {
-            if (k > 0) L.chk(hipStreamWaitEvent(s1, evU[k - 1], 0));
+            if (k > 0) L.chk(ev_wait(s1, evU[k - 1]));
             update(GM_CHOL_ROW, k, s1);
             potrf(k + 1, s1);
             trsm(k + 1, s1);
-            L.chk(hipEventRecord(evP[k + 1], s1));
+            L.chk(ev_record(evP[k + 1], s1));
         }
-        L.chk(hipStreamWaitEvent(s0, evP[k], 0));
+        L.chk(ev_wait(s0, evP[k]));
         if (T > 1) update(GM_CHOL_REST, k, s0);
-        L.chk(hipEventRecord(evU[k], s0));
+        L.chk(ev_record(evU[k], s0));
     }
 }
 
@@ -1435,21 +1435,21 @@ void dag_lu(double *dA, int N, int b, bool lookahead, hipStream_t s0, hipStream_
     hipEvent_t *evP = g.events.data();
     hipEvent_t *evU = g.events.data() + MT + 1;
     hipEvent_t fork = g.events[2 * MT + 2];
-    L.chk(hipEventRecord(fork, s0));
-    L.chk(hipStreamWaitEvent(s1, fork, 0));
+    L.chk(ev_record(fork, s0));
+    L.chk(ev_wait(s1, fork));
     panel(0, s1);
-    L.chk(hipEventRecord(evP[0], s1));
+    L.chk(ev_record(evP[0], s1));
     for (int k = 0; k < MT; k++) {
         const int T = MT - k - 1;
         if (T > 0) {
-            if (k > 0) L.chk(hipStreamWaitEvent(s1, evU[k - 1], 0));
+            if (k > 0) L.chk(ev_wait(s1, evU[k - 1]));
             update(GM_LU_PANEL, k, s1);
             panel(k + 1, s1);
-            L.chk(hipEventRecord(evP[k + 1], s1));
+            L.chk(ev_record(evP[k + 1], s1));
         }
-        L.chk(hipStreamWaitEvent(s0, evP[k], 0));
+        L.chk(ev_wait(s0, evP[k]));
         if (T > 1) update(GM_LU_REST, k, s0);
-        L.chk(hipEventRecord(evU[k], s0));
+        L.chk(ev_record(evU[k], s0));
     }
 }
 
@@ -2581,9 +2581,12 @@ int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int variant, long
     // the chain schedules of the solves: FP64 (prec 0) on the FP64 copy, the
     // mixed tiers on the FP32 working copy with packed panels (b % 128 == 0);
     // QR (algo 2, FP64) with `variant` panel chains in flight
-    const int pair = algo < 2 ? variant : 0;
+    // Cholesky / LU: variant = pair + 2 * schedule (0 diagonal chain with the split
+    // update, 1 chain with one update launch, 2 row look-ahead, 3 one stream;
+    // the mixed tiers run the split chain only)
+    const int pair = algo < 2 ? variant & 1 : 0, sched = algo < 2 ? variant >> 1 : 0;
     if (algo < 0 || algo > 2 || MT < 2 || MT > 64 || prec < 0 || prec > 4 || !panel2_ok(b) ||
-        (prec > 0 && b % 128) || (algo < 2 && (variant < 0 || variant > 1)) ||
+        (prec > 0 && b % 128) || (algo < 2 && (variant < 0 || sched > 3 || (prec > 0 && sched != 0))) ||
         (algo == 2 && (prec != 0 || variant < 1 || variant > QR_MAXS)))
         return PALIAL_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_mu);
@@ -2641,10 +2644,15 @@ int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int variant, long
         }
         g.qr_streams = QR_MAXS;
         dag_qr(dA, dS, N, b, s0, sc, true, L, variant);
-    } else if (algo == 0) {
-        dag_cholesky_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, (int *)(fake + 8), L, prec, fs);
     } else {
-        dag_lu_chain(dA, N, b, s0, sc, fs ? sb : s0, sa, L, prec, fs);
+        // the streams the schedules take from the context: bulk and aux (CU-masked), no chain mask
+        g.s_bulk = sb;
+        g.s_aux = sa;
+        g.s_chainm = nullptr;
+        g.dag = sched == 0 ? 2 : sched == 1 ? 1 : 0;
+        const bool lookahead = sched != 3;
+        if (algo == 0) dag_cholesky(dA, N, b, lookahead, s0, sc, (int *)(fake + 8), L, prec, fs);
+        else dag_lu(dA, N, b, lookahead, s0, sc, L, prec, fs);
     }
     std::vector<long> rec;
     rec.swap(g_rec.out);

@@ -135,17 +135,20 @@ def check_semantics(seq, algo, MT):
         assert m + MT * n in solved, (m, n)
 
 
-CASES = [(algo, MT, b, prec, pair) for algo in (0, 1) for (MT, b) in ((2, 128), (5, 128), (8, 128), (9, 256))
-         for prec, pair in ((0, 0), (1, 0), (2, 0), (2, 1), (3, 0), (4, 0))]
+CASES = [(algo, MT, b, prec, variant) for algo in (0, 1) for (MT, b) in ((2, 128), (5, 128), (8, 128), (9, 256))
+         for prec, variant in ((0, 0), (0, 2), (0, 4), (0, 6), (1, 0), (2, 0), (2, 1), (3, 0), (4, 0))]
 
 
-@pytest.mark.parametrize("algo,MT,b,prec,pair", CASES)
-def test_chain_schedule_race_free_and_complete(algo, MT, b, prec, pair):
-    seq = ops(algo, MT, b, prec, pair)
+@pytest.mark.parametrize("algo,MT,b,prec,variant", CASES)
+def test_chain_schedule_race_free_and_complete(algo, MT, b, prec, variant):
+    """variant = paired updates + 2 * schedule: 0 diagonal chain with the split
+    update (every tier), 1 chain with one update launch, 2 row look-ahead,
+    3 one stream (FP64)."""
+    seq = ops(algo, MT, b, prec, variant)
     launches = happens_before(seq)
     assert launches and {o["stream"] for o, _, _, _ in launches} <= {1, 2, 3, 4}
     nconf = check_races(launches)
-    assert nconf > 0 or MT < 3         # two tiles: everything runs on the chain
+    assert nconf > 0 or MT < 3 or variant == 6    # two tiles, or one stream: nothing to order
     check_semantics(seq, algo, MT)
 
 
@@ -167,6 +170,8 @@ def test_schedule_hook_rejects():
     L.palial_hip_schedule_ops.argtypes = [C.c_int] * 5 + [C.POINTER(C.c_long), C.c_long]
     assert L.palial_hip_schedule_ops(2, 8, 128, 0, 0, None, 0) < 0
     assert L.palial_hip_schedule_ops(0, 8, 64, 2, 0, None, 0) < 0       # packed panels need b % 128 == 0
+    assert L.palial_hip_schedule_ops(0, 8, 128, 2, 2, None, 0) < 0      # mixed tiers: the split chain only
+    assert L.palial_hip_schedule_ops(2, 8, 128, 0, 9, None, 0) < 0      # at most 8 QR chains
 
 
 QR_W, QR_TW, QR_VW, ORM_W, ORM_TW, ORM_VW = 3, 4, 5, 6, 7, 8
