@@ -474,7 +474,9 @@ def main():
                                f"and FP64 refinement), N={args.n}, b={args.b}, kappa={args.kappa:g}, 1 RHS, "
                                f"tier {tier}", "N": args.n, "b": args.b, "kappa": args.kappa, "nrhs": 1,
                    "parallelism": "single GPU, 3 HIP streams (diagonal chain, trailing update, "
-                                  "row TRSM and packing)"},
+                                  "row TRSM and packing)",
+                   "sweep_block": int(os.environ.get("PALIAL_SWEEP_BLOCK", "1")),
+                   "paired_updates": os.environ.get("PALIAL_PAIR", "0") == "1"},
         "pct_fp64_peak": best["pct_fp64_peak"],
         "speedup_vs_fp64": best["speedup_vs_fp64"],
         "backward_error": best["backward_error"],
