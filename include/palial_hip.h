@@ -83,7 +83,8 @@ typedef struct palial_opts {
                              precomputed inverses of the diagonal blocks (0 = the library default,
                              PALIAL_SWEEP_BLOCK or 1; 1 = one tile; blocks of q > 1 tiles need q <= 8
                              dividing N / tile with N / tile >= 2q, <= 4 right-hand sides and
-                             q * tile * (4 when nrhs > 1) <= 4096) */
+                             q * tile * (4 when nrhs > 1) <= 4096; a mixed tier's FP64 fallback
+                             uses the default) */
     int   reserved[3];
 } palial_opts;
 

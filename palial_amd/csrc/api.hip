@@ -1883,6 +1883,7 @@ int run_solve(int algo, MATRIX_desc A, double *B, int nrhs, int ldb, const palia
         palial_opts o2 = o;
         o2.precision = 0;
         o2.factor_tile = 0;
+        o2.sweep_block = 0;     // the FP64 solve picks its own tile, so its own sweep steps
         palial_report r2;
         memset(&r2, 0, sizeof r2);
         const int rc2 = run_solve(algo, A, B, nrhs, ldb, &o2, &r2);
