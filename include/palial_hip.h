@@ -173,6 +173,19 @@ long palial_hip_task_name(int algo, int MT, long index, char *buf, int len);
  * range.  Calls are serialised with the library's other calls. */
 int  palial_hip_set_option(const char *name, int value);
 
+/* CU sets per task class (no counterpart in the reference; the user-space GPU
+ * analogue of palial_power/server.c:69-84, which caps a core's clock by the
+ * type of the task it runs).  palial_hip_init reads PALIAL_CU_POLICY, e.g.
+ * "bulk=8-255,aux=8-127,chain=0-7": entries name=ranges separated by ',',
+ * name bulk (trailing updates), aux (row TRSMs and packing) or chain (the
+ * diagonal chain), ranges "lo-hi" or "cu" joined by '+'.  The mixed tiers'
+ * streams take these CU masks in place of PALIAL_RESERVE_CUS and
+ * PALIAL_CHAIN_CUS; unnamed classes get all CUs (bulk), the bulk's set (aux)
+ * or no mask (chain).  This function parses a policy for ncu CUs into three
+ * masks of `words` 32-bit words (bulk, aux, chain) and returns 1 when the
+ * chain is masked, 0 when not, PALIAL_ERR_ARG for a malformed policy. */
+int  palial_hip_cu_policy(const char *policy, int ncu, unsigned *out, int words);
+
 /* Test hook (no counterpart in the reference): enumerate the tile tasks of
  * one batched trailing-update launch (mode = internal GEMM mode, step k, TS
  * row m for the QR modes) without touching the GPU.  Writes 3 longs per task:

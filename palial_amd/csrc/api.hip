@@ -671,6 +671,57 @@ void rec_qr_panel(const double *A1, const double *A2, const double *T, const dou
     rec_put(0, s, nullptr, 5, rd, wr, {}, A2 ? 1 : 0, (int)(a1 / g_rec.MT), (int)(t2 % g_rec.MT));
 }
 
+// PALIAL_CU_POLICY: CU sets of the mixed tiers' task classes, entries
+// "name=ranges" separated by ',' with name bulk (trailing updates), aux (row
+// TRSMs and packing) or chain (diagonal chain), ranges "lo-hi" or "cu" joined
+// by '+'.  Unnamed classes: bulk all CUs, aux the bulk's set, chain unmasked
+// (given[2] false).  Returns 0, or -1 for a malformed policy.
+int parse_cu_policy(const char *pol, int ncu, std::vector<uint32_t> (&m)[3], bool (&given)[3])
+{
+    const int words = (ncu + 31) / 32;
+    static const char *names[3] = {"bulk", "aux", "chain"};
+    for (int c = 0; c < 3; c++) {
+        m[c].assign((size_t)words, 0u);
+        given[c] = false;
+    }
+    if (!pol || !*pol || ncu <= 0) return -1;
+    const char *p = pol;
+    while (*p) {
+        int c = -1;
+        for (int i = 0; i < 3; i++) {
+            const size_t n = strlen(names[i]);
+            if (!strncmp(p, names[i], n) && p[n] == '=') { c = i; p += n + 1; break; }
+        }
+        if (c < 0 || given[c]) return -1;
+        given[c] = true;
+        for (;;) {
+            char *e;
+            const long lo = strtol(p, &e, 10);
+            if (e == p) return -1;
+            long hi = lo;
+            p = e;
+            if (*p == '-') {
+                hi = strtol(p + 1, &e, 10);
+                if (e == p + 1) return -1;
+                p = e;
+            }
+            if (lo < 0 || hi < lo || hi >= ncu) return -1;
+            for (long cu = lo; cu <= hi; cu++) m[c][(size_t)cu / 32] |= 1u << (cu % 32);
+            if (*p != '+') break;
+            p++;
+        }
+        if (*p == ',') {
+            if (!*++p) return -1;      // a trailing separator
+        } else if (*p) {
+            return -1;
+        }
+    }
+    if (!given[0])
+        for (int cu = 0; cu < ncu; cu++) m[0][(size_t)cu / 32] |= 1u << (cu % 32);
+    if (!given[1]) m[1] = m[0];
+    return 0;
+}
+
 void trace_gemm(GemmArgs &a)
 {
     if (!g_tr.active) return;
@@ -2403,7 +2454,20 @@ int palial_hip_init(int device)
     if (g.sweep_block < 1 || g.sweep_block > 8) return fail(PALIAL_ERR_ARG, "PALIAL_SWEEP_BLOCK=%s: expected 1..8", sb);
     HIPCHK(hipStreamCreateWithFlags(&g.s_main, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&g.s_panel, hipStreamNonBlocking));
-    if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
+    // PALIAL_CU_POLICY (parse_cu_policy) replaces PALIAL_RESERVE_CUS / PALIAL_CHAIN_CUS
+    const char *pol = getenv("PALIAL_CU_POLICY");
+    if (pol) {
+        const int ncu = prop.multiProcessorCount, words = (ncu + 31) / 32;
+        std::vector<uint32_t> m[3];
+        bool given[3];
+        if (parse_cu_policy(pol, ncu, m, given))
+            return fail(PALIAL_ERR_ARG, "PALIAL_CU_POLICY=%s: expected entries bulk|aux|chain=lo-hi[+lo-hi...] "
+                        "separated by ',' with CUs below %d", pol, ncu);
+        g.reserve = 1;
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_bulk, words, m[0].data()));
+        HIPCHK(hipExtStreamCreateWithCUMask(&g.s_aux, words, m[1].data()));
+        if (given[2]) HIPCHK(hipExtStreamCreateWithCUMask(&g.s_chainm, words, m[2].data()));
+    } else if (g.reserve > 0 && g.reserve < prop.multiProcessorCount / 2) {
         const int ncu = prop.multiProcessorCount, words = (ncu + 31) / 32;
         std::vector<uint32_t> mu(words, 0);
         for (int cu = g.reserve; cu < ncu; cu++) mu[cu / 32] |= 1u << (cu % 32);
@@ -2664,6 +2728,17 @@ int palial_hip_schedule_ops(int algo, int MT, int b, int prec, int variant, long
     if (L.err != hipSuccess) return fail(PALIAL_ERR_ARG, "schedule refused its arguments");
     if (out && cap >= (long)rec.size()) std::copy(rec.begin(), rec.end(), out);
     return (int)rec.size();
+}
+
+int palial_hip_cu_policy(const char *policy, int ncu, unsigned *out, int words)
+{
+    if (ncu <= 0 || words < (ncu + 31) / 32 || !out) return PALIAL_ERR_ARG;
+    std::vector<uint32_t> m[3];
+    bool given[3];
+    if (parse_cu_policy(policy, ncu, m, given)) return PALIAL_ERR_ARG;
+    for (int c = 0; c < 3; c++)
+        for (int w = 0; w < words; w++) out[c * words + w] = (c < 2 || given[2]) && w < (int)m[c].size() ? m[c][(size_t)w] : 0u;
+    return given[2] ? 1 : 0;
 }
 
 int palial_hip_set_option(const char *name, int value)
