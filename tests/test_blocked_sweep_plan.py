@@ -187,3 +187,56 @@ def test_plan_shapes():
     buf = (C.c_int * 64)()
     assert L.palial_hip_sweep_plan(0, 6, 4, buf, 8) < 0        # q must divide MT
     assert L.palial_hip_sweep_plan(0, 8, 2, buf, 3) < 0        # cap too small
+
+
+@pytest.mark.parametrize("algo", [0, 1])
+def test_blocked_sweeps_fp32_accuracy(algo):
+    """The mixed tiers sweep FP32 copies of the factor and of the inverses
+    (built in FP64, then rounded), with FP64 sums.  Executing the library's
+    plans that way on an ill-conditioned factor (Cholesky of an SPD matrix with
+    kappa 1e8; LU with a non-normal unit L of condition ~1e6 and an ill-scaled
+    U), blocks of q tiles are as accurate as one tile per step: the rounding
+    of a block inverse costs no more than the roundings of the tile steps it
+    replaces."""
+    MT, b = 16, 16
+    N = MT * b
+    r = np.random.default_rng(31)
+    if algo == 0:
+        Q, _ = np.linalg.qr(r.standard_normal((N, N)))
+        A = (Q * np.geomspace(1.0, 1e-8, N)) @ Q.T
+        A = (A + A.T) / 2
+        U = np.linalg.cholesky(A).T
+        Lm = None
+    else:
+        Lm = np.eye(N) + 0.1 * np.tril(r.standard_normal((N, N)), -1)
+        U = np.diag(np.geomspace(1.0, 1e-6, N)[r.permutation(N)]) @ (np.eye(N) + np.triu(r.standard_normal((N, N)) * 0.05, 1))
+        A = Lm @ U
+    F = np.full(N * N, np.nan)
+    for m in range(MT):
+        for n in range(MT):
+            if m < n:
+                put(F, tile_off(b, MT, m, n), b, U[m * b:(m + 1) * b, n * b:(n + 1) * b])
+            elif m > n and algo == 1:
+                put(F, tile_off(b, MT, m, n), b, Lm[m * b:(m + 1) * b, n * b:(n + 1) * b])
+    sl = [slice(k * b, (k + 1) * b) for k in range(MT)]
+    if algo == 0:
+        DU = np.concatenate([np.linalg.inv(U[s, s]).T.ravel(order="F") for s in sl])
+        DL = None
+    else:
+        DU = np.concatenate([np.linalg.inv(U[s, s]).ravel(order="F") for s in sl])
+        DL = np.concatenate([np.linalg.inv(Lm[s, s]).ravel(order="F") for s in sl])
+    f32 = lambda v: None if v is None else v.astype(np.float32).astype(np.float64)
+    rhs = r.standard_normal(N)
+    exact = np.linalg.solve(A, rhs)
+    errs = {}
+    for q in (1, 2, 4):
+        if q == 1:
+            invs = [f32(DU), f32(DL)]
+        else:
+            XU = block_inverse(F, MT, b, q, 1, algo, DU)
+            XL = block_inverse(F, MT, b, q, 0, 0, DL) if algo == 1 else None
+            invs = [f32(XU), f32(XL)]
+        x = run_sweeps(algo, MT, b, q, f32(F), invs, rhs)
+        errs[q] = np.linalg.norm(x - exact) / np.linalg.norm(exact)
+    assert errs[1] < 1e-3, errs                  # the FP32 sweep itself is meaningful here
+    assert errs[2] <= 1.5 * errs[1] and errs[4] <= 1.5 * errs[1], errs
