@@ -244,12 +244,11 @@ int ensure_dinv(int b, int MT)
 }
 
 
-// packed low-precision panel operands of two consecutive steps (step k uses
-// set k % 2): row panel at 0, column panel at MT*b*b*4 bytes of a set, 4 bytes
-// per element covering FP32 and BF16
+// packed low-precision panel operands of one step (a set): row panel at 0,
+// column panel at MT*b*b*4 bytes, 4 bytes per element covering FP32, BF16,
+// split BF16 and FP8.  The split schedule packs step k into set k % PACK_SETS
+// (paired steps read two sets while the next two are packed).
 size_t pack_set_bytes(int b, int MT) { return 2 * (size_t)MT * b * b * 4; }
-// step sets of packed panels: the split schedule packs step k into set k % 4
-// (paired Cholesky steps read two sets while the next two are packed)
 constexpr int PACK_SETS = 4;
 
 int ensure_pack(int b, int MT)
