@@ -26,7 +26,10 @@ Procedure:
      by more than `max_error`;
   4. judge=True also measures every candidate, to report what the model's
      pick costs against the exhaustive best, the prediction error over all
-     candidates and the measurement time the model saved.
+     candidates and the measurement time the model saved;
+  5. solves: for the pick, the sweep block (tiles per correction-sweep step,
+     1, 2 or 4 as the tile count allows) and, for BF16, paired trailing
+     updates on or off, chosen by measurement.
 `tune_batch` picks the persistent launch's workgroup count for a batch of
 small factorizations by measuring each candidate (exhaustive)."""
 import argparse
@@ -126,19 +129,19 @@ class SolveBench:
         self.dX = C.c_void_p()
         P._check(P.lib().palial_hip_malloc(C.byref(self.dX), N * 8))
 
-    def _solve(self, b, tier):
+    def _solve(self, b, tier, sweep_block=0):
         P, N = self.P, self.A.N
         P._check(P.lib().palial_hip_memcpy_h2d(self.dX, self.rhs.ctypes.data, N * 8))
         P.synchronize()
         t0 = time.perf_counter()
         _, rep, rc = P.solve(self.A, (self.dX, 1, N), algo=self.algo, precision=tier, device=True, max_iter=60,
-                             check=False, factor_tile=b)
+                             check=False, factor_tile=b, sweep_block=sweep_block)
         return (time.perf_counter() - t0) * 1e3, rep, rc
 
-    def run(self, b, tier, reps=3, trace=False):
+    def run(self, b, tier, reps=3, trace=False, sweep_block=0):
         ts, rep, rc = [], None, 0
         for _ in range(reps):
-            t, rep, rc = self._solve(b, tier)
+            t, rep, rc = self._solve(b, tier, sweep_block)
             ts.append(t)
         rec = None
         if trace:
@@ -235,9 +238,37 @@ def tune_factor(P, algo, N, tiles=TILES, epochs=300, seed=0, max_error=0.25, jud
         fb.free()
 
 
-def tune_solve(P, A, algo, tiles=SOLVE_TILES, tiers=TIERS, epochs=300, seed=0, max_error=0.25, judge=False):
+def sweep_blocks(N, b, nrhs=1):
+    """Tiles per correction-sweep step a solve with factor tile b can take."""
+    MT = N // b
+    return [q for q in (1, 2, 4) if MT % q == 0 and MT >= 2 * q and q * b * (1 if nrhs == 1 else 4) <= 4096]
+
+
+def tune_knobs(P, sb, b, tier, N, reps=3):
+    """For the picked (b, tier): every sweep block (and, for BF16, paired
+    updates on and off), measured; returns ((sweep_block, pair), {choice: ms}).
+    The library's pairing setting is restored afterwards."""
+    pairs = [0, 1] if tier == 2 else [0]
+    res = {}
+    prev = P.set_option("pair", 0)
+    try:
+        for pr in pairs:
+            P.set_option("pair", pr)
+            for q in sweep_blocks(N, b):
+                ms, _, inf = sb.run(b, tier, reps=reps, sweep_block=q)
+                if inf["converged"] and inf["tier_used"] == tier:
+                    res[(q, pr)] = ms
+    finally:
+        P.set_option("pair", prev)
+    return (min(res, key=res.get) if res else (1, 0)), res
+
+
+def tune_solve(P, A, algo, tiles=SOLVE_TILES, tiers=TIERS, epochs=300, seed=0, max_error=0.25, judge=False,
+               knobs=True):
     """Best (factor tile, tier) of a solve of device matrix A: predicted time =
-    model factorization time + the refinement time measured for the tier."""
+    model factorization time + the refinement time measured for the tier.
+    knobs: then the sweep block and (BF16) paired updates for that pick, by
+    measurement."""
     N = A.N
     cands = solve_candidates(N, tiles, tiers)
     sub = subset(cands, 512, 0)
@@ -288,6 +319,10 @@ def tune_solve(P, A, algo, tiles=SOLVE_TILES, tiers=TIERS, epochs=300, seed=0, m
         out = _report(cands, pred, measured, t_meas, pick, fallback, t_judge, exhaustive)
         out.update({"algo": algo, "N": N, "train_samples": nsamp, "train_loss": [float(loss[0]), float(loss[-1])],
                     "scale": scale, "second": "tier", "refine_ms": refine})
+        if knobs:
+            (q, pr), km = tune_knobs(P, sb, pick[0], pick[1], N)
+            out["knobs"] = {"sweep_block": q, "pair": pr,
+                            "measured_ms": {f"q{kq}_pair{kp}": round(v, 3) for (kq, kp), v in km.items()}}
         return out
     finally:
         sb.free()

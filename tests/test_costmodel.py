@@ -95,3 +95,42 @@ def test_subset_design():
     assert {b for b, _ in at.factor_candidates("qr", 8192)} == {128, 256, 512}
     sc = at.solve_candidates(24576)
     assert (384, 3) in sc and (384, 4) in sc and all(t < 3 or b % 128 == 0 for b, t in sc)
+
+
+def test_tune_knobs_candidates_and_restore():
+    """The solve tuner's last stage: every admissible sweep block, paired
+    updates tried for BF16 only, non-converged or fallen-back runs ignored,
+    and the library's pairing setting restored."""
+    from palial_amd import autotune
+
+    assert autotune.sweep_blocks(8192, 512) == [1, 2, 4]
+    assert autotune.sweep_blocks(1536, 512) == [1]          # 3 tiles: no block fits twice
+    assert autotune.sweep_blocks(32768, 512, nrhs=2) == [1, 2]
+
+    class FakeP:
+        def __init__(self):
+            self.pair = 1
+            self.calls = []
+
+        def set_option(self, name, v):
+            assert name == "pair"
+            prev, self.pair = self.pair, v
+            return prev
+
+    class FakeBench:
+        def __init__(self, P):
+            self.P = P
+
+        def run(self, b, tier, reps=3, sweep_block=0):
+            self.P.calls.append((sweep_block, self.P.pair))
+            ms = 100.0 - 10 * sweep_block - 5 * self.P.pair
+            return ms, None, {"converged": not (sweep_block == 4 and self.P.pair == 1), "tier_used": tier}
+
+    P = FakeP()
+    best, res = autotune.tune_knobs(P, FakeBench(P), 512, 2, 8192)
+    assert sorted(P.calls) == sorted((q, pr) for q in (1, 2, 4) for pr in (0, 1))
+    assert (4, 1) not in res and best == (4, 0)
+    assert P.pair == 1
+    P = FakeP()
+    best, res = autotune.tune_knobs(P, FakeBench(P), 512, 0, 8192)
+    assert {pr for _, pr in P.calls} == {0} and best == (4, 0)
