@@ -65,3 +65,22 @@ def test_sweep_block_refused(hip, orc, q, nrhs):
     B = np.ones((N, nrhs))
     _, _, rc = hip.solve(a, B, algo="cholesky", precision=0, b=b, sweep_block=q, check=False)
     assert rc == -1                            # PALIAL_ERR_ARG
+
+
+def test_tuner_knobs(hip):
+    """The solve tuner's last stage on the GPU: each admissible sweep block
+    and pairing choice converges in the picked tier, and the best is taken."""
+    from palial_amd import autotune
+    N, b = 2048, 256
+    gen = hip.CondGenerator(N, b, seed=0)
+    A = hip.DeviceMatrix(N, b)
+    sb = autotune.SolveBench(hip, A, "cholesky")
+    try:
+        gen.fill(A, 1e2)
+        best, res = autotune.tune_knobs(hip, sb, b, 2, N, reps=2)
+    finally:
+        sb.free()
+        A.free()
+        gen.free()
+    assert set(res) == {(q, p) for q in (1, 2, 4) for p in (0, 1)}, res
+    assert best == min(res, key=res.get)
