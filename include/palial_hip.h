@@ -168,7 +168,8 @@ long palial_hip_task_name(int algo, int MT, long index, char *buf, int len);
 /* Schedule settings that palial_hip_init reads from the environment, changed
  * at run time (no counterpart in the reference; for tests and tuners):
  * "pair" (PALIAL_PAIR: 1 pairs the trailing updates of consecutive steps of
- * the BF16 Cholesky and LU solves), "sweep_block" (PALIAL_SWEEP_BLOCK, 1..8).  Returns
+ * the BF16 and split-BF16 Cholesky and LU solves), "sweep_block"
+ * (PALIAL_SWEEP_BLOCK, 1..8).  Returns
  * the previous value, or PALIAL_ERR_ARG for an unknown name or a value out of
  * range.  Calls are serialised with the library's other calls. */
 int  palial_hip_set_option(const char *name, int value);
@@ -224,7 +225,7 @@ int palial_hip_blkinv_tiles(int mode, int MT, int b, int q, int bd, int bt, long
  * recorded on the host instead of launched: the diagonal-chain schedule of a
  * solve or factorization (algo 0 Cholesky, 1 LU; prec 0 the FP64 copy, 1..4
  * the FP32 working copy with packed panels; variant = pair + 2 * schedule,
- * pair 1: paired BF16 updates, schedule 0 diagonal chain with the split
+ * pair 1: paired BF16 / split-BF16 updates, schedule 0 diagonal chain with the split
  * update, 1 chain with one update launch, 2 row look-ahead, 3 one stream, the
  * mixed tiers taking schedule 0 only) or the QR schedule
  * (algo 2, prec 0, variant = panel chains in flight, 1..8); MT tiles of b per

@@ -106,7 +106,7 @@ struct Ctx {
     double *d_blk = nullptr;        // block inverses of the blocked sweeps: tiles of T and products
     size_t blk_cap = 0;
     int sweep_block = 1;            // default tiles per correction-sweep step
-    int pair = 0;                   // BF16 Cholesky / LU solves: paired trailing updates
+    int pair = 0;                   // BF16 / split-BF16 Cholesky and LU solves: paired trailing updates
     BatchGraph bgraph;              // batched factorizations: the captured persistent launch
     void *d_bidx = nullptr;         // batched factorizations: per-matrix index arrays
     size_t bidx_cap = 0;            // bytes
@@ -1122,13 +1122,13 @@ void dag_cholesky_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc
             gemm(a, fl, ngemm + nsyrk, s);
         }
     };
-    // Paired steps (BF16 on the FP32 working copy): an even step j with at
+    // Paired steps (BF16 and split BF16 on the FP32 working copy): an even step j with at
     // least 4 trailing tiles leaves the tiles (n, m), n >= j+3 of its update
     // to step j+1, which applies panels j and j+1 to them in one pass, so
     // those C tiles are read and written once for two steps.  Step j updates
     // rows j+1 and j+2 (GM_CHOL_FIRST, GM_CHOL_ROW2): the chain's next two
     // steps never wait for a full trailing update, only every other one.
-    const bool pair = fs && prec == 2 && split && pk && g.pair && !g_tr.active;
+    const bool pair = fs && (prec == 2 || prec == 4) && split && pk && g.pair && !g_tr.active;
     auto lead = [&](int k) { return pair && k % 2 == 0 && MT - k - 1 >= 4; };
     hipEvent_t *ev = g.events.data();
     hipEvent_t *evP = ev, *evR = ev + MT + 1, *evT = ev + 2 * MT + 4, *evB = ev + 3 * MT + 6;
@@ -1274,7 +1274,7 @@ void dag_lu_chain(double *dA, int N, int b, hipStream_t s0, hipStream_t sc, hipS
         }
     };
     // paired steps as in the Cholesky chain schedule
-    const bool pair = fs && prec == 2 && split && pk && g.pair && !g_tr.active;
+    const bool pair = fs && (prec == 2 || prec == 4) && split && pk && g.pair && !g_tr.active;
     auto lead = [&](int k) { return pair && k % 2 == 0 && MT - k - 1 >= 4; };
     auto psetU = [&](int k) { return pset(k); };
     auto psetL = [&](int k) { return pset(k) + (size_t)MT * b * b * 4; };
@@ -2444,7 +2444,7 @@ int palial_hip_init(int device)
     // PALIAL_F32WORK=0 keeps the FP64 working copy in the mixed-tier solves
     const char *fw = getenv("PALIAL_F32WORK");
     g.f32work = fw ? atoi(fw) : 1;
-    // PALIAL_PAIR=1: BF16 Cholesky and LU solves apply two steps' updates per pass over the trailing tiles
+    // PALIAL_PAIR=1: BF16 and split-BF16 solves apply two steps' updates per pass over the trailing tiles
     const char *pr = getenv("PALIAL_PAIR");
     g.pair = pr ? atoi(pr) : 0;
     // PALIAL_SWEEP_BLOCK=q: default tiles per step of the correction sweeps

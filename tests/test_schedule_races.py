@@ -136,7 +136,7 @@ def check_semantics(seq, algo, MT):
 
 
 CASES = [(algo, MT, b, prec, variant) for algo in (0, 1) for (MT, b) in ((2, 128), (5, 128), (8, 128), (9, 256))
-         for prec, variant in ((0, 0), (0, 2), (0, 4), (0, 6), (1, 0), (2, 0), (2, 1), (3, 0), (4, 0))]
+         for prec, variant in ((0, 0), (0, 2), (0, 4), (0, 6), (1, 0), (2, 0), (2, 1), (3, 0), (4, 0), (4, 1))]
 
 
 @pytest.mark.parametrize("algo,MT,b,prec,variant", CASES)

@@ -1,4 +1,4 @@
-"""Paired trailing updates of the BF16 Cholesky and LU solves (no reference
+"""Paired trailing updates of the BF16 and split-BF16 Cholesky and LU solves (no reference
 counterpart): steps j and j+1 applied to the tiles n >= j+3 in one pass must
 give a factor as good as the step-by-step updates, seen through the
 refinement: same tier, about the same number of refinement steps, backward
@@ -15,9 +15,10 @@ def backward_error(A, X, B):
                for j in range(B.shape[1]))
 
 
+@pytest.mark.parametrize("prec", [2, 4])
 @pytest.mark.parametrize("algo", ["cholesky", "lu"])
 @pytest.mark.parametrize("N,b,kappa", [(1280, 128, 0), (2048, 256, 0), (1536, 128, 1e3)])
-def test_paired_updates_match(hip, orc, algo, N, b, kappa):
+def test_paired_updates_match(hip, orc, algo, N, b, kappa, prec):
     if kappa:
         r = np.random.default_rng(21)
         Q, _ = np.linalg.qr(r.standard_normal((N, N)))
@@ -33,9 +34,9 @@ def test_paired_updates_match(hip, orc, algo, N, b, kappa):
     try:
         for pair in (0, 1):
             hip.set_option("pair", pair)
-            X, rep, rc = hip.solve(a, B, algo=algo, precision=2, b=b, factor_tile=b)
+            X, rep, rc = hip.solve(a, B, algo=algo, precision=prec, b=b, factor_tile=b)
             assert rc == 0, (pair, rc, rep.as_dict())
-            assert rep.precision_used == 2 and rep.shifts == 0, (pair, rep.as_dict())
+            assert rep.precision_used == prec and rep.shifts == 0, (pair, rep.as_dict())
             assert backward_error(A, X, B) <= 2e-14, pair
             out[pair] = (X, rep.iterations)
     finally:
