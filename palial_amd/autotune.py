@@ -28,8 +28,8 @@ Procedure:
      pick costs against the exhaustive best, the prediction error over all
      candidates and the measurement time the model saved;
   5. solves: for the pick, the sweep block (tiles per correction-sweep step,
-     1, 2 or 4 as the tile count allows) and, for BF16, paired trailing
-     updates on or off, chosen by measurement.
+     1, 2 or 4 as the tile count allows) and, for BF16 and split BF16, paired
+     trailing updates on or off, chosen by measurement.
 `tune_batch` picks the persistent launch's workgroup count for a batch of
 small factorizations by measuring each candidate (exhaustive)."""
 import argparse
@@ -245,10 +245,10 @@ def sweep_blocks(N, b, nrhs=1):
 
 
 def tune_knobs(P, sb, b, tier, N, reps=3):
-    """For the picked (b, tier): every sweep block (and, for BF16, paired
-    updates on and off), measured; returns ((sweep_block, pair), {choice: ms}).
-    The library's pairing setting is restored afterwards."""
-    pairs = [0, 1] if tier == 2 else [0]
+    """For the picked (b, tier): every sweep block (and, for BF16 and split
+    BF16, paired updates on and off), measured; returns ((sweep_block, pair),
+    {choice: ms}).  The library's pairing setting is restored afterwards."""
+    pairs = [0, 1] if tier in (2, 4) else [0]
     res = {}
     prev = P.set_option("pair", 0)
     try:
@@ -267,8 +267,8 @@ def tune_solve(P, A, algo, tiles=SOLVE_TILES, tiers=TIERS, epochs=300, seed=0, m
                knobs=True):
     """Best (factor tile, tier) of a solve of device matrix A: predicted time =
     model factorization time + the refinement time measured for the tier.
-    knobs: then the sweep block and (BF16) paired updates for that pick, by
-    measurement."""
+    knobs: then the sweep block and (BF16, split BF16) paired updates for that
+    pick, by measurement."""
     N = A.N
     cands = solve_candidates(N, tiles, tiers)
     sub = subset(cands, 512, 0)

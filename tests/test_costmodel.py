@@ -134,3 +134,6 @@ def test_tune_knobs_candidates_and_restore():
     P = FakeP()
     best, res = autotune.tune_knobs(P, FakeBench(P), 512, 0, 8192)
     assert {pr for _, pr in P.calls} == {0} and best == (4, 0)
+    P = FakeP()
+    autotune.tune_knobs(P, FakeBench(P), 512, 4, 8192)
+    assert {pr for _, pr in P.calls} == {0, 1}
