@@ -32,13 +32,15 @@ constexpr int GV_MAXR = 16;          // right-hand sides
 
 // E: element type of the tiles (double, or float for the FP32 copies the
 // mixed-precision correction solves stream); sums are accumulated in FP64.
-template <int NR, typename E, int ROWS>
+// QB: steps of p.q tiles; without it the step is one tile (q = 1 known to the
+// compiler, no loop over source tiles).
+template <int NR, typename E, int ROWS, bool QB>
 __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
 {
     constexpr int G = GV_THREADS / ROWS;      // column groups (op = N)
     extern __shared__ double sm[];
     const int b = p.b;
-    const int q = p.q > 1 ? p.q : 1;
+    const int q = QB && p.q > 1 ? p.q : 1;
     const int seg = q * b;                    // input segment: the q tiles of the step
     double *sv = sm;                          // seg x NR input segment (layout by op, below)
     double *sred = sm + (size_t)seg * NR;     // G groups x ROWS rows x NR partial sums
@@ -329,19 +331,23 @@ hipError_t launch_tile_gemv(const GemvArgs &a, hipStream_t s)
     const bool wide = a.f32 && !a.trans;
     dim3 grid((unsigned)(a.ntasks * (a.b / (wide ? 32 : GV_ROWS)))), block(GV_THREADS);
     auto lds = [&](int nr) { return ((size_t)q * a.b * nr + (size_t)GV_THREADS * nr) * 8; };
+    // 1..4 right-hand sides here (5..16 took the MFMA kernels above)
+#define GV_LAUNCH(NR, E, ROWS)                                                                           \
+    do {                                                                                                 \
+        if (q > 1) hipLaunchKernelGGL((tile_gemv_kernel<NR, E, ROWS, true>), grid, block, lds(NR), s, a);  \
+        else hipLaunchKernelGGL((tile_gemv_kernel<NR, E, ROWS, false>), grid, block, lds(NR), s, a);       \
+    } while (0)
     if (wide) {
-        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, float, 32>), grid, block, lds(1), s, a);
-        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, float, 32>), grid, block, lds(4), s, a);
-        else hipLaunchKernelGGL((tile_gemv_kernel<16, float, 32>), grid, block, lds(16), s, a);
+        if (a.nrhs == 1) GV_LAUNCH(1, float, 32);
+        else GV_LAUNCH(4, float, 32);
     } else if (a.f32) {
-        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, float, 16>), grid, block, lds(1), s, a);
-        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, float, 16>), grid, block, lds(4), s, a);
-        else hipLaunchKernelGGL((tile_gemv_kernel<16, float, 16>), grid, block, lds(16), s, a);
+        if (a.nrhs == 1) GV_LAUNCH(1, float, 16);
+        else GV_LAUNCH(4, float, 16);
     } else {
-        if (a.nrhs == 1) hipLaunchKernelGGL((tile_gemv_kernel<1, double, 16>), grid, block, lds(1), s, a);
-        else if (a.nrhs <= 4) hipLaunchKernelGGL((tile_gemv_kernel<4, double, 16>), grid, block, lds(4), s, a);
-        else hipLaunchKernelGGL((tile_gemv_kernel<16, double, 16>), grid, block, lds(16), s, a);
+        if (a.nrhs == 1) GV_LAUNCH(1, double, 16);
+        else GV_LAUNCH(4, double, 16);
     }
+#undef GV_LAUNCH
     return hipGetLastError();
 }
 
