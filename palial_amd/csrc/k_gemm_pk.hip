@@ -407,7 +407,9 @@ __device__ inline void wait_vm(int n)
 // of the slice) the low 8 bytes feed one v_mfma_f32_16x16x32_fp8_fp8 and the
 // high 8 bytes a second, so A and B pair the same K values.  alpha is scaled
 // by 2^-(e_a + e_b).
-template <int MINB, bool F32C, bool X3 = false, int NST = 4, bool F8 = false>
+// TWO: paired steps (pa2 / pb2 set); without it the K loop covers one panel and
+// the panel-switch arithmetic is compiled out.
+template <int MINB, bool F32C, bool X3 = false, int NST = 4, bool F8 = false, bool TWO = false>
 __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
 {
     const Store<F32C> S{p.mat};
@@ -437,15 +439,15 @@ __global__ __launch_bounds__(256, MINB) void gemm_bf16d_kernel(GemmArgs p)
     const char *PB_ = (const char *)p.pb + (size_t)tp.ib * tstride;
     // paired steps (GM_CHOL_REST2, GM_LU_REST2): the K loop runs on through panel k+1, whose
     // tiles sit one position lower
-    const char *PA2 = p.pa2 ? (const char *)p.pa2 + (size_t)(tp.ia - 1) * tstride : nullptr;
-    const char *PB2 = p.pb2 ? (const char *)p.pb2 + (size_t)(tp.ib - 1) * tstride : nullptr;
+    const char *PA2 = TWO ? (const char *)p.pa2 + (size_t)(tp.ia - 1) * tstride : nullptr;
+    const char *PB2 = TWO ? (const char *)p.pb2 + (size_t)(tp.ib - 1) * tstride : nullptr;
     const size_t rs = F8 ? (size_t)b : 2 * (size_t)b;
     const int NS1 = F8 ? b / 64 : b / KB;               // slices of one panel
-    const int NS = PA2 ? 2 * NS1 : NS1;
+    const int NS = TWO ? 2 * NS1 : NS1;
 
     auto issue = [&](int s) {
         const int st = s % NST;
-        const bool second = s >= NS1;
+        const bool second = TWO && s >= NS1;
         const char *PA = second ? PA2 : PA_, *PB = second ? PB2 : PB_;
         const size_t kb = (size_t)(second ? s - NS1 : s) * 64;
 #pragma unroll
@@ -751,7 +753,11 @@ hipError_t launch_gemm_packed(const GemmArgs &a_in, int prec, hipStream_t s)
     }
     else if (prec == 4) {
         // two stages (64 KB LDS, two workgroups per CU)
-        if (a.f32c) hipLaunchKernelGGL((gemm_bf16d_kernel<2, true, true, 2>), grid, block, 0, s, a);
+        if (two) {
+            if (!a.f32c) return hipErrorInvalidValue;        // paired steps run on the FP32 working copy
+            hipLaunchKernelGGL((gemm_bf16d_kernel<2, true, true, 2, false, true>), grid, block, 0, s, a);
+        }
+        else if (a.f32c) hipLaunchKernelGGL((gemm_bf16d_kernel<2, true, true, 2>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((gemm_bf16d_kernel<2, false, true, 2>), grid, block, 0, s, a);
     }
     else if (prec == 2) {
@@ -759,8 +765,12 @@ hipError_t launch_gemm_packed(const GemmArgs &a_in, int prec, hipStream_t s)
             const char *e = getenv("PALIAL_PK_VARIANT");
             return e ? atoi(e) : 0;
         }();
-        if (a.pa2 && var != 0) return hipErrorInvalidValue;  // two panels only in the default kernel
-        if (a.f32c) {
+        if (two) {
+            // paired steps: the default kernel on the FP32 working copy
+            if (var != 0 || !a.f32c) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((gemm_bf16d_kernel<2, true, false, 4, false, true>), grid, block, 0, s, a);
+        }
+        else if (a.f32c) {
             if (var != 0) return hipErrorInvalidValue;       // FP32 output only in the default kernel
             hipLaunchKernelGGL((gemm_bf16d_kernel<2, true>), grid, block, 0, s, a);
         }
