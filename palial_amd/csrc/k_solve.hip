@@ -32,36 +32,38 @@ constexpr int GV_MAXR = 16;          // right-hand sides
 
 // E: element type of the tiles (double, or float for the FP32 copies the
 // mixed-precision correction solves stream); sums are accumulated in FP64.
-// QB: steps of p.q tiles; without it the step is one tile (q = 1 known to the
-// compiler, no loop over source tiles).
-template <int NR, typename E, int ROWS, bool QB>
-__global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
+// One-tile steps (q = 1, the default): the source tile is found once, before
+// the loops.
+template <int NR, typename E, int ROWS>
+__device__ inline void tile_gemv_one(const GemvArgs &p)
 {
     constexpr int G = GV_THREADS / ROWS;      // column groups (op = N)
     extern __shared__ double sm[];
+    double *sv = sm;                          // b x NR input segment (layout by op, below)
+    double *sred = sm + (size_t)p.b * NR;     // G groups x ROWS rows x NR partial sums
     const int b = p.b;
-    const int q = QB && p.q > 1 ? p.q : 1;
-    const int seg = q * b;                    // input segment: the q tiles of the step
-    double *sv = sm;                          // seg x NR input segment (layout by op, below)
-    double *sred = sm + (size_t)seg * NR;     // G groups x ROWS rows x NR partial sums
     const int rb_per_tile = b / ROWS;
     const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
     const int t = threadIdx.x;
     const E *mat = (const E *)p.mat;
     const E *inv = (const E *)p.inv;
-    const int MT = p.N / b, s0 = p.k * q;     // first tile of the step
-    // output tile and the source tiles j0 <= j < j1 of the step (T_j below)
-    const GemvMap mp = gemv_map(p.mode, p.trans, p.k, q, task);
-    const int vout = mp.vout, j0 = mp.j0, j1 = mp.j1;
-    auto tileT = [&](int j) -> const E * {
-        return (p.mode == GV_DIAG ? inv : mat) + gemv_src(p.mode, p.trans, p.k, q, b, MT, vout, j);
-    };
-    const double *vi = p.vin + (size_t)s0 * b;
+    const E *T;
+    int vin, vout;
+    {
+        const int MT = p.N / b, k = p.k;
+        switch (p.mode) {
+        case GV_DIAG: T = inv + (size_t)k * b * b; vin = k; vout = k; break;
+        case GV_CHOL_FWD: T = mat + tile_off(b, MT, k, k + 1 + task); vin = k; vout = k + 1 + task; break;
+        case GV_LU_FWD: T = mat + tile_off(b, MT, k + 1 + task, k); vin = k; vout = k + 1 + task; break;
+        default: T = mat + tile_off(b, MT, task, k); vin = k; vout = task; break;
+        }
+    }
+    const double *vi = p.vin + (size_t)vin * b;
     double *vo = p.vout + (size_t)vout * b;
     // op = N reads the NR values of one row together: sv[i * NR + c]; op = T
-    // reads one right-hand side across the lanes: sv[i + c * seg]
-    for (int e = t; e < seg * NR; e += GV_THREADS) {
-        const int i = e % seg, c = e / seg;
+    // reads one right-hand side across the lanes: sv[i + c * b]
+    for (int e = t; e < b * NR; e += GV_THREADS) {
+        const int i = e % b, c = e / b;
         const double x = (c < p.nrhs) ? vi[(size_t)i + (size_t)c * p.ld] : 0.0;   // padded right-hand sides are zero
         sv[p.trans ? e : i * NR + c] = x;
     }
@@ -72,16 +74,13 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
     for (int j = 0; j < NR; j++) acc[j] = 0.0;
     if (!p.trans) {
         const int r = t % ROWS, g = t / ROWS;             // g: column group, columns c = g, g+G, ...
-        for (int jt = j0; jt < j1; jt++) {
-            const E *Tr = tileT(jt) + r0 + r;
-            const double *svt = sv + (size_t)jt * b * NR;
+        const E *Tr = T + r0 + r;
 #pragma unroll 8
-            for (int c = g; c < b; c += G) {
-                const double a = (double)Tr[(size_t)c * b];
-                const double *svc = svt + c * NR;
+        for (int c = g; c < b; c += G) {
+            const double a = (double)Tr[(size_t)c * b];
+            const double *svc = sv + c * NR;
 #pragma unroll
-                for (int j = 0; j < NR; j++) acc[j] += a * svc[j];
-            }
+            for (int j = 0; j < NR; j++) acc[j] += a * svc[j];
         }
 #pragma unroll
         for (int j = 0; j < NR; j++) sred[(g * ROWS + r) * NR + j] = acc[j];
@@ -101,19 +100,15 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
         for (int x = 0; x < 4; x++)
 #pragma unroll
             for (int j = 0; j < NR; j++) part[x][j] = 0.0;
-        for (int jt = j0; jt < j1; jt++) {
-            const E *T = tileT(jt);
-            const double *svt = sv + (size_t)jt * b;
-            for (int c0 = 0; c0 < b; c0 += 64) {
-                const int c = c0 + lane;
-                double a[4];
+        for (int c0 = 0; c0 < b; c0 += 64) {
+            const int c = c0 + lane;
+            double a[4];
 #pragma unroll
-                for (int x = 0; x < 4; x++) a[x] = (double)T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
+            for (int x = 0; x < 4; x++) a[x] = (double)T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
 #pragma unroll
-                for (int x = 0; x < 4; x++)
+            for (int x = 0; x < 4; x++)
 #pragma unroll
-                    for (int j = 0; j < NR; j++) part[x][j] += a[x] * svt[c + j * seg];
-            }
+                for (int j = 0; j < NR; j++) part[x][j] += a[x] * sv[c + j * b];
         }
         // Sum the 4 NR values over the 64 lanes.  Butterfly steps on the high
         // lane bits first (every lane keeps all values), then halving steps:
@@ -151,6 +146,136 @@ __global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
             double *o = vo + (size_t)(r0 + wv * 4 + x) + (size_t)j * p.ld;
             if (p.mode == GV_DIAG) *o = s;
             else *o -= s;
+        }
+    }
+}
+
+// QB: steps of p.q tiles (blocked sweeps), the sum running over the step's
+// source tiles (gemv_map, gemv_src); without it, tile_gemv_one.
+template <int NR, typename E, int ROWS, bool QB>
+__global__ __launch_bounds__(GV_THREADS) void tile_gemv_kernel(GemvArgs p)
+{
+    if constexpr (!QB) {
+        tile_gemv_one<NR, E, ROWS>(p);
+        return;
+    } else {
+        constexpr int G = GV_THREADS / ROWS;      // column groups (op = N)
+        extern __shared__ double sm[];
+        const int b = p.b;
+        const int q = QB && p.q > 1 ? p.q : 1;
+        const int seg = q * b;                    // input segment: the q tiles of the step
+        double *sv = sm;                          // seg x NR input segment (layout by op, below)
+        double *sred = sm + (size_t)seg * NR;     // G groups x ROWS rows x NR partial sums
+        const int rb_per_tile = b / ROWS;
+        const int task = blockIdx.x / rb_per_tile, rb = blockIdx.x % rb_per_tile;
+        const int t = threadIdx.x;
+        const E *mat = (const E *)p.mat;
+        const E *inv = (const E *)p.inv;
+        const int MT = p.N / b, s0 = p.k * q;     // first tile of the step
+        // output tile and the source tiles j0 <= j < j1 of the step (T_j below)
+        const GemvMap mp = gemv_map(p.mode, p.trans, p.k, q, task);
+        // one-tile steps read exactly source tile 0 (a diagonal step has one task)
+        const int vout = mp.vout, j0 = QB ? mp.j0 : 0, j1 = QB ? mp.j1 : 1;
+        auto tileT = [&](int j) -> const E * {
+            return (p.mode == GV_DIAG ? inv : mat) + gemv_src(p.mode, p.trans, p.k, q, b, MT, vout, j);
+        };
+        const E *const T0 = QB ? nullptr : tileT(0);    // one-tile steps: the source tile, once
+        const double *vi = p.vin + (size_t)s0 * b;
+        double *vo = p.vout + (size_t)vout * b;
+        // op = N reads the NR values of one row together: sv[i * NR + c]; op = T
+        // reads one right-hand side across the lanes: sv[i + c * seg]
+        for (int e = t; e < seg * NR; e += GV_THREADS) {
+            const int i = e % seg, c = e / seg;
+            const double x = (c < p.nrhs) ? vi[(size_t)i + (size_t)c * p.ld] : 0.0;   // padded right-hand sides are zero
+            sv[p.trans ? e : i * NR + c] = x;
+        }
+        __syncthreads();
+        const int r0 = rb * ROWS;
+        double acc[NR];
+    #pragma unroll
+        for (int j = 0; j < NR; j++) acc[j] = 0.0;
+        if (!p.trans) {
+            const int r = t % ROWS, g = t / ROWS;             // g: column group, columns c = g, g+G, ...
+            for (int jt = j0; jt < j1; jt++) {
+                const E *Tr = (QB ? tileT(jt) : T0) + r0 + r;
+                const double *svt = sv + (size_t)jt * b * NR;
+    #pragma unroll 8
+                for (int c = g; c < b; c += G) {
+                    const double a = (double)Tr[(size_t)c * b];
+                    const double *svc = svt + c * NR;
+    #pragma unroll
+                    for (int j = 0; j < NR; j++) acc[j] += a * svc[j];
+                }
+            }
+    #pragma unroll
+            for (int j = 0; j < NR; j++) sred[(g * ROWS + r) * NR + j] = acc[j];
+            __syncthreads();
+            for (int e = t; e < ROWS * p.nrhs; e += GV_THREADS) {   // ROWS * NR may exceed the threads
+                const int rr = e % ROWS, j = e / ROWS;
+                double s = 0.0;
+                for (int q = 0; q < G; q++) s += sred[(q * ROWS + rr) * NR + j];
+                double *o = vo + (size_t)(r0 + rr) + (size_t)j * p.ld;
+                if (p.mode == GV_DIAG) *o = s;
+                else *o -= s;
+            }
+        } else if constexpr (ROWS == 16) {           // op = T: 4 waves x 4 rows (launched with ROWS = 16 only)
+            const int lane = t & 63, wv = t >> 6;
+            double part[4][NR];
+    #pragma unroll
+            for (int x = 0; x < 4; x++)
+    #pragma unroll
+                for (int j = 0; j < NR; j++) part[x][j] = 0.0;
+            for (int jt = j0; jt < j1; jt++) {
+                const E *T = QB ? tileT(jt) : T0;
+                const double *svt = sv + (size_t)jt * b;
+                for (int c0 = 0; c0 < b; c0 += 64) {
+                    const int c = c0 + lane;
+                    double a[4];
+    #pragma unroll
+                    for (int x = 0; x < 4; x++) a[x] = (double)T[(size_t)c + (size_t)(r0 + wv * 4 + x) * b];
+    #pragma unroll
+                    for (int x = 0; x < 4; x++)
+    #pragma unroll
+                        for (int j = 0; j < NR; j++) part[x][j] += a[x] * svt[c + j * seg];
+                }
+            }
+            // Sum the 4 NR values over the 64 lanes.  Butterfly steps on the high
+            // lane bits first (every lane keeps all values), then halving steps:
+            // at offset o a lane keeps one half of its values (the upper half when
+            // its bit o is set), sends the other half to lane ^ o and adds what it
+            // receives, so that in the end lane L holds the sum of value L mod 4 NR.
+            constexpr int NV = 4 * NR;
+            constexpr int NH = NV == 64 ? 6 : NV == 16 ? 4 : 2;    // halving steps: log2(NV)
+            double v[NV];
+    #pragma unroll
+            for (int x = 0; x < 4; x++)
+    #pragma unroll
+                for (int j = 0; j < NR; j++) v[x * NR + j] = part[x][j];
+    #pragma unroll
+            for (int o = 32; o > (1 << (NH - 1)); o >>= 1)
+    #pragma unroll
+                for (int i = 0; i < NV; i++) v[i] += __shfl_xor(v[i], o);
+            int n = NV;
+    #pragma unroll
+            for (int o = 1 << (NH - 1); o >= 1; o >>= 1) {
+                const bool up = lane & o;
+                n >>= 1;
+    #pragma unroll
+                for (int i = 0; i < NV / 2; i++) {
+                    if (i < n) {
+                        const double send = up ? v[i] : v[i + n];
+                        const double keep = up ? v[i + n] : v[i];
+                        v[i] = keep + __shfl_xor(send, o);
+                    }
+                }
+            }
+            if (lane < NV && lane % NR < p.nrhs) {
+                const int x = lane / NR, j = lane % NR;
+                const double s = v[0];
+                double *o = vo + (size_t)(r0 + wv * 4 + x) + (size_t)j * p.ld;
+                if (p.mode == GV_DIAG) *o = s;
+                else *o -= s;
+            }
         }
     }
 }
