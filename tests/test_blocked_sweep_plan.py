@@ -240,3 +240,26 @@ def test_blocked_sweeps_fp32_accuracy(algo):
         errs[q] = np.linalg.norm(x - exact) / np.linalg.norm(exact)
     assert errs[1] < 1e-3, errs                  # the FP32 sweep itself is meaningful here
     assert errs[2] <= 1.5 * errs[1] and errs[4] <= 1.5 * errs[1], errs
+
+
+def test_random_blocked_sweeps():
+    """Random tile counts, tile sizes and step widths (every q that divides
+    the tile count twice over): the library's plans still solve exactly."""
+    from hypothesis import assume, given, settings, strategies as st
+
+    @settings(max_examples=30, deadline=None, derandomize=True)
+    @given(algo=st.sampled_from([0, 1]), MT=st.integers(2, 12), q=st.integers(1, 6), b=st.sampled_from([4, 8]))
+    def check(algo, MT, q, b):
+        assume(MT % q == 0 and MT >= 2 * q)
+        A, F, U, Lm, DU, DL = factor(algo, MT, b, seed=MT * 31 + q * 7 + b + algo)
+        if q == 1:
+            invs = [DU, DL]
+        else:
+            invs = [block_inverse(F, MT, b, q, 1, algo, DU),
+                    block_inverse(F, MT, b, q, 0, 0, DL) if algo == 1 else None]
+        rhs = np.random.default_rng(MT + q).standard_normal(MT * b)
+        x = run_sweeps(algo, MT, b, q, F, invs, rhs)
+        ref = np.linalg.solve(A, rhs)
+        assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max()
+
+    check()

@@ -241,3 +241,35 @@ def test_qr_schedule_race_free_and_in_reference_order(MT, ns):
     hist_r, seen_r = writer_history(qr_reference(MT))
     assert hist_g == hist_r
     assert seen_g == seen_r
+
+
+def test_random_schedules_race_free():
+    """Random tile counts and tile sizes across every schedule and tier: the
+    same race and completeness checks (hypothesis-driven, bounded)."""
+    from hypothesis import given, settings, strategies as st
+
+    @settings(max_examples=40, deadline=None, derandomize=True)
+    @given(algo=st.sampled_from([0, 1]), MT=st.integers(2, 11), b=st.sampled_from([128, 256, 512]),
+           pv=st.sampled_from([(0, 0), (0, 2), (0, 4), (0, 6), (1, 0), (2, 0), (2, 1), (3, 0), (4, 0), (4, 1)]))
+    def check(algo, MT, b, pv):
+        prec, variant = pv
+        seq = ops(algo, MT, b, prec, variant)
+        check_races(happens_before(seq))
+        check_semantics(seq, algo, MT)
+
+    check()
+
+
+def test_random_qr_schedules():
+    from hypothesis import given, settings, strategies as st
+
+    @settings(max_examples=20, deadline=None, derandomize=True)
+    @given(MT=st.integers(2, 10), ns=st.integers(1, 8))
+    def check(MT, ns):
+        seq = ops(2, MT, 128, 0, ns)
+        launches = happens_before(seq)
+        check_races(launches)
+        got = [(qr_label(o), o["wr"], o["rd"]) for o, _, _, _ in launches]
+        assert writer_history(got) == writer_history(qr_reference(MT))
+
+    check()
