@@ -26,6 +26,9 @@ launch from profiles/traffic.json.  ``cpu_baseline`` times the CPU oracle's
 OpenMP task DAG (OpenBLAS tile kernels, one BLAS thread per task) on a
 bounded sample, with the plan's smaller runs (N=2048 and 8192, all three
 algorithms, median of 3, scaled residuals of both paths) under ``runs``.
+``variants`` (only with ``--variants q:pair,...``) repeats the BF16 Cholesky
+solve with other sweep blocks and with paired trailing updates, in a child
+process with a time limit of its own.
 """
 import argparse
 import json
@@ -363,6 +366,29 @@ def cpu_baseline(P, N=32768, b=512, small=((2048, 256), (8192, 512)), reps=3):
             "runs": runs}
 
 
+def run_variants(args, spec):
+    """The headline BF16 Cholesky solve with other sweep blocks and with paired
+    updates (`q:pair` items), in a child process with a time limit, so that the
+    bench reports them without depending on them; the child prints one JSON
+    line per variant (palial_amd.profile_solve --variants)."""
+    import subprocess
+    cmd = [sys.executable, "-m", "palial_amd.profile_solve", "--algo", "cholesky", "--precision", "2",
+           "--n", str(args.n), "--tile", str(args.b), "--kappa", str(args.kappa), "--seed", str(args.seed),
+           "--solves", "3", "--variants", spec]
+    try:
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=150,
+                           cwd=os.path.dirname(os.path.abspath(__file__)))
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after 150 s"}
+    rows = [json.loads(l) for l in p.stdout.splitlines() if l.startswith("{")]
+    for r in rows:
+        log(f"[bench] variant q={r['sweep_block']} pair={r['pair']}: {r['ms_median']:.1f} ms, rc {r['rc']}")
+    out = {f"q{r['sweep_block']}_pair{r['pair']}": r for r in rows}
+    if p.returncode != 0:
+        out["error"] = (p.stderr.strip().splitlines() or ["exit %d" % p.returncode])[-1]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +408,9 @@ def main():
     ap.add_argument("--factor-algos", default="cholesky,lu,qr", help="plain FP64 factorizations timed")
     ap.add_argument("--batched", type=int, default=10000, help="matrices in the batched run (0: skip)")
     ap.add_argument("--seed", type=int, default=0, help="seed of the Gaussian whose QR gives Q")
+    ap.add_argument("--variants", default="",
+                    help="sweep block:paired updates of extra BF16 Cholesky solves reported under "
+                         "'variants', e.g. 1:0,2:0,4:0,1:1 (default: none)")
     ap.add_argument("--sweep", action="store_true",
                     help="only the size sweep of the FP64 factorizations (one JSON line with key 'sweep')")
     ap.add_argument("--sweep-algos", default="cholesky,lu,qr", help="factorizations of the sweep")
@@ -446,6 +475,7 @@ def main():
                 "bytes_per_launch": k["bytes"] / max(1, k["launches"]),
                 "tflops": round(k["flops"] / (k["ms"] * 1e-3) / 1e12, 1)}
     batched = run_batched(P, args.batched) if args.batched > 0 else None
+    variants = run_variants(args, args.variants) if args.variants and "cholesky" in algos else None
     cpu = None if args.no_cpu else cpu_baseline(P, args.cpu_n, args.b)
     lead = algos[0]
     conv = {k: v for k, v in solves.items() if k.startswith(lead + "_") and v["converged"]}
@@ -486,6 +516,7 @@ def main():
         "roofline_fp64_gemm": roof64,
         "solve_kernels": {c: v for c, v in (sprof or {}).items() if c != "tier"},
         "batched": batched,
+        "variants": variants,
         "cpu_baseline": cpu,
         "results": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in res.items()},

@@ -42,6 +42,43 @@ def factorizations(a):
     return 0
 
 
+def variants(a, A, dX, rhs):
+    """Each variant q:pair -- solves with sweep block q and paired updates
+    on or off -- timed on the host around the synchronous call (A and b in
+    HBM), the first solve untimed; one JSON line per variant."""
+    import json
+    import time
+    from . import set_option, synchronize
+    N, b = a.n, a.tile
+    prev = set_option("pair", 0)
+    rc_all = 0
+    try:
+        for item in a.variants.split(","):
+            q, pr = (int(x) for x in item.split(":"))
+            set_option("pair", pr)
+            rows = []
+            for i in range(a.solves + 1):
+                _check(lib().palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * a.nrhs * 8))
+                synchronize()
+                t0 = time.perf_counter()
+                _, rep, rc = solve(A, (dX, a.nrhs, N), algo=a.algo, precision=a.precision, device=True,
+                                   factor_tile=b, check=False, sweep_block=q)
+                ms = (time.perf_counter() - t0) * 1e3
+                if i:
+                    rows.append((ms, rep.factor_ms, rep.solve_ms, rep.iterations, rep.backward_error, rc,
+                                 rep.precision_used))
+            ms = sorted(r[0] for r in rows)
+            out = {"sweep_block": q, "pair": pr, "ms_median": round(ms[len(ms) // 2], 3),
+                   "factor_ms": round(rows[-1][1], 3), "solve_ms": round(rows[-1][2], 3),
+                   "iterations": rows[-1][3], "backward_error": rows[-1][4], "rc": rows[-1][5],
+                   "precision_used": rows[-1][6]}
+            rc_all = rc_all or rows[-1][5]
+            print(json.dumps(out), flush=True)
+    finally:
+        set_option("pair", prev)
+    return 0 if rc_all == 0 else 1
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--algo", choices=["cholesky", "lu"], default="cholesky")
@@ -55,6 +92,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--factor", choices=["cholesky", "lu", "qr"], default=None,
                     help="time FP64 factorizations of the library's seeded inputs instead of solves")
+    ap.add_argument("--variants", default=None,
+                    help="q:pair,... -- for each sweep block q and paired-update setting, the solves "
+                         "(the first untimed), one JSON line each")
     a = ap.parse_args(argv)
     init(0)
     if a.factor:
@@ -68,6 +108,13 @@ def main(argv=None):
     dX = C.c_void_p()
     _check(lib().palial_hip_malloc(C.byref(dX), N * a.nrhs * 8))
     rc_all = 0
+    if a.variants:
+        try:
+            return variants(a, A, dX, rhs)
+        finally:
+            lib().palial_hip_free(dX)
+            A.free()
+            finalize()
     try:
         for i in range(a.solves):
             _check(lib().palial_hip_memcpy_h2d(dX, rhs.ctypes.data, N * a.nrhs * 8))
