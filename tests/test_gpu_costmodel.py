@@ -67,7 +67,7 @@ def test_autotune_solve(hip):
     A = hip.DeviceMatrix(N, 512)
     try:
         gen.fill(A, 1e2)
-        res = autotune.tune_solve(hip, A, "cholesky", judge=True, knobs=False)   # knobs: test_zx_sweep_block
+        res = autotune.tune_solve(hip, A, "cholesky", judge=True, knobs=False)   # knobs: wip_gpu_sweep_block.py
     finally:
         A.free()
         gen.free()
