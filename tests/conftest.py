@@ -25,5 +25,11 @@ def orc():
 @pytest.fixture(scope="session")
 def hip():
     import palial_amd
+    # PALIAL_HIPEMU=plain|asan: the GPU tests run on the CPU build of the same
+    # sources (tests/emu); the asan variant needs the ASan runtime preloaded
+    variant = os.environ.get("PALIAL_HIPEMU")
+    if variant:
+        from emu import build as emu_build
+        palial_amd.LIB_PATH = emu_build.build(variant)
     palial_amd.init(0)
     return palial_amd
