@@ -670,6 +670,13 @@ void rec_qr_panel(const double *A1, const double *A2, const double *T, const dou
     rec_put(0, s, nullptr, 5, rd, wr, {}, A2 ? 1 : 0, (int)(a1 / g_rec.MT), (int)(t2 % g_rec.MT));
 }
 
+// V of diagonal tile A (its own resource, as in rec_gemm) copied to workspace V
+void rec_unit_lower_copy(const double *V, const double *A, hipStream_t s)
+{
+    const long a = rec_id(A);
+    rec_put(0, s, nullptr, 6, {a + 8 * REC_RES}, {rec_id(V)}, {}, 0, (int)(a % g_rec.MT), -1);
+}
+
 // PALIAL_CU_POLICY: CU sets of the mixed tiers' task classes, entries
 // "name=ranges" separated by ',' with name bulk (trailing updates), aux (row
 // TRSMs and packing) or chain (diagonal chain), ranges "lo-hi" or "cu" joined
@@ -1560,13 +1567,19 @@ void dag_qr(double *dA, double *dS, int N, int b, hipStream_t s0, hipStream_t s1
     };
     auto ormqr = [&](int k, hipStream_t s, int u) {   // PALIAL_dormqr on A(k,n), n > k: C -= V T^T V^T C
         const int T = MT - k - 1;
+        // V(k) as a plain tile first: the products then never load A(k,k)'s
+        // upper R, which step k's TSQRTs rewrite meanwhile on the chain stream.
+        // It takes the stream's last W tile (ORMQR and TSMQR use T <= MT - 1).
+        double *vk = g.d_w1 + u * wstride + (size_t)(MT - 1) * b * b;
+        L.run(KC_PACK, 0, 2 * tile_bytes(b), s,
+              [&] { return launch_unit_lower_copy(vk, dA + tile_off(b, MT, k, k), b, s); });
         GemmArgs a = gargs(GM_ORM_W, dA, N, b, k, T);
-        a.smat = dS; a.w1 = g.d_w1 + u * wstride; a.w2 = g.d_w2 + u * wstride;
-        a.alpha = 1.0; a.beta = 0.0; a.maskA = MK_UNIT_LOWER;
+        a.A = vk; a.smat = dS; a.w1 = g.d_w1 + u * wstride; a.w2 = g.d_w2 + u * wstride;
+        a.alpha = 1.0; a.beta = 0.0; a.maskA = MK_NONE;
         L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, true, false, s); });
         a.mode = GM_ORM_TW; a.maskA = MK_UPPER;
         L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, true, false, s); });
-        a.mode = GM_ORM_VW; a.maskA = MK_UNIT_LOWER; a.alpha = -1.0; a.beta = 1.0;
+        a.mode = GM_ORM_VW; a.maskA = MK_NONE; a.alpha = -1.0; a.beta = 1.0;
         L.run(KC_QRUPD, T * b3(b), 3.0 * T * tile_bytes(b), s, [&] { return launch_gemm(a, false, false, s); });
     };
     auto tsqrt = [&](int k, int m, hipStream_t s, int c) {

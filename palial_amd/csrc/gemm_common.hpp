@@ -133,7 +133,7 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
     }
     case GM_ORM_W: {
         int n = k + 1 + task;
-        t.A = M + tile_off(b, MT, k, k);         // V (transposed, unit lower)
+        t.A = p.A ? p.A : M + tile_off(b, MT, k, k);   // V (transposed, unit lower)
         t.B = M + tile_off(b, MT, k, n);
         t.C = p.w1 + (size_t)task * bb;
         break;
@@ -146,7 +146,7 @@ __host__ __device__ inline TaskPtrs decode(const GemmArgs &p, int task)
     }
     case GM_ORM_VW: {
         int n = k + 1 + task;
-        t.A = M + tile_off(b, MT, k, k);         // V (unit lower)
+        t.A = p.A ? p.A : M + tile_off(b, MT, k, k);   // V (unit lower)
         t.B = p.w2 + (size_t)task * bb;
         t.C = M + tile_off(b, MT, k, n);
         t.D = t.C;

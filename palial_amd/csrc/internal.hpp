@@ -64,7 +64,8 @@ enum MaskA : int { MK_NONE = 0, MK_UPPER = 1, MK_UNIT_LOWER = 2 };
 
 struct GemmArgs {
     int mode;            // GemmMode
-    // GM_SINGLE operands
+    // GM_SINGLE operands; GM_ORM_W / GM_ORM_VW: A, when set, is V(k) as a plain
+    // unit-lower tile (launch_unit_lower_copy) read instead of A(k,k) masked
     const double *A;
     const double *B;
     double *C;
@@ -333,6 +334,10 @@ hipError_t launch_vh_update(const double *V, size_t ldv, int k, const double *h,
 // G != nullptr and b % 32 == 0, 32 <= b <= 512 the blocked kernels run
 hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s, double *G = nullptr);
 hipError_t launch_tsqrt(double *A1, double *A2, double *T, double *tau, int b, hipStream_t s, double *G = nullptr);
+// V <- the unit-lower V of a QR diagonal tile A (its strictly lower part, ones
+// on the diagonal, zeros above) as a plain b x b tile; reads only A's strictly
+// lower part, so it may run beside the TSQRTs that rewrite A's upper R
+hipError_t launch_unit_lower_copy(double *V, const double *A, int b, hipStream_t s);
 
 // misc
 hipError_t launch_generate(double *mat, int N, int b, int kind, uint64_t seed, double shift,
@@ -370,5 +375,6 @@ void rec_trsm(int kind, const TrsmArgs &t, const double *dinv, hipStream_t s);
 void rec_pack(const double *mat, int b, int k, int first, int n, bool colpanel, const void *dst, hipStream_t s);
 void rec_qr_panel(const double *A1, const double *A2, const double *T, const double *tau, const double *G,
                   hipStream_t s);
+void rec_unit_lower_copy(const double *V, const double *A, hipStream_t s);
 
 }  // namespace palial

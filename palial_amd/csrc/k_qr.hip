@@ -692,6 +692,28 @@ bool blocked_ok(int b, const double *G) { return G && b % QP == 0 && b >= QP && 
 
 }  // namespace
 
+// V of a QR diagonal tile as a plain operand: ones on the diagonal, zeros
+// above it, A's Householder vectors below.  Only the strictly lower part of A
+// is read.
+__global__ __launch_bounds__(256) void unit_lower_copy_kernel(double *__restrict__ V, const double *__restrict__ A,
+                                                              int b)
+{
+    const size_t bb = (size_t)b * b;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < bb; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % (size_t)b), j = (int)(e / (size_t)b);
+        V[e] = i > j ? A[e] : (i == j ? 1.0 : 0.0);
+    }
+}
+
+hipError_t launch_unit_lower_copy(double *V, const double *A, int b, hipStream_t s)
+{
+    if (sched_rec_on()) { rec_unit_lower_copy(V, A, s); return hipSuccess; }
+    size_t blocks = ((size_t)b * b + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(unit_lower_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, V, A, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_geqrt(double *A, double *T, double *tau, int b, hipStream_t s, double *G)
 {
     if (sched_rec_on()) { rec_qr_panel(A, nullptr, T, tau, G, s); return hipSuccess; }

@@ -175,13 +175,15 @@ def test_schedule_hook_rejects():
 
 
 QR_W, QR_TW, QR_VW, ORM_W, ORM_TW, ORM_VW = 3, 4, 5, 6, 7, 8
-QRPANEL = 5
+QRPANEL, VCOPY = 5, 6
 WORKSPACE = 4 * RES     # W, op(T) W, tau and Gram workspace ids start here (S tiles below)
 
 
 def qr_label(o):
     if o["kind"] == QRPANEL:
         return ("G", o["k"]) if o["mode"] == 0 else ("Q", o["k"], o["m"])
+    if o["kind"] == VCOPY:        # ORMQR(k)'s copy of V(k), read by its products
+        return ("O", o["k"])
     if o["mode"] in (ORM_W, ORM_TW, ORM_VW):
         return ("O", o["k"])
     assert o["mode"] in (QR_W, QR_TW, QR_VW)
