@@ -7,10 +7,13 @@ is mechanical and made on the fly: a dynamic LDS declaration
 `extern __shared__ T name[];` becomes `T *name = (T *)emu::g_dyn_lds;`, the
 launch's LDS block.  api.hip's translation unit also gets the test entry
 points of tests/emu/api_hooks.inc, which call the library's internal
-routines.  Output: tests/emu/_build/{plain,asan}/libpalial_emu.so, rebuilt
-only when a source or a flag changes.
+routines.  Output: tests/emu/_build/{plain,asan,race}/libpalial_emu.so,
+rebuilt only when a source or a flag changes.  The race variant compiles the
+library's sources with the thread-sanitizer instrumentation and the emulator
+runtime without it: the runtime's own callbacks check every device-memory
+access against the order of the stream operations (hipemu.cpp).
 
-    python tests/emu/build.py [--asan]
+    python tests/emu/build.py [--asan | --race]
 """
 import concurrent.futures as cf
 import hashlib
@@ -35,7 +38,13 @@ HOOKS = {"api.hip": "api_hooks.inc"}
 VARIANTS = {
     "plain": ["-O2"],
     "asan": ["-O1", "-fsanitize=address", "-fno-omit-frame-pointer"],
+    # stream race checking: the library's sources carry the thread-sanitizer
+    # instrumentation, whose callbacks the emulator runtime defines itself
+    # (no sanitizer runtime is linked)
+    "race": ["-O1", "-fsanitize=thread", "-DHIPEMU_RACE"],
 }
+# flags of the emulator runtime itself, where they differ from the sources'
+RUNTIME = {"race": ["-O1", "-DHIPEMU_RACE"]}
 
 
 def transform(text):
@@ -48,7 +57,7 @@ def _sources():
 
 def _digest(variant):
     h = hashlib.sha256()
-    h.update(" ".join([CXX, *FLAGS, *VARIANTS[variant]]).encode())
+    h.update(" ".join([CXX, *FLAGS, *VARIANTS[variant], "|", *RUNTIME.get(variant, [])]).encode())
     names = [os.path.join(CSRC, f) for f in _sources()]
     names += [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith((".hpp", ".h"))]
     names += [os.path.join(HERE, h) for h in sorted(HOOKS.values())]
@@ -71,7 +80,7 @@ def _compile(args):
 
 
 def build(variant="plain", verbose=False):
-    """Return the path of libpalial_emu.so for `variant` ("plain" or "asan")."""
+    """Return the path of libpalial_emu.so for `variant` ("plain", "asan" or "race")."""
     d = os.path.join(OUT, variant)
     lib = os.path.join(d, "libpalial_emu.so")
     stamp = os.path.join(d, "stamp")
@@ -90,7 +99,8 @@ def build(variant="plain", verbose=False):
             if f in HOOKS:
                 fh.write(f'\n#include "{os.path.join(HERE, HOOKS[f])}"\n')
         jobs.append((gen, os.path.join(d, f[:-4] + ".o"), flags))
-    jobs.append((os.path.join(HERE, "hipemu.cpp"), os.path.join(d, "hipemu.o"), flags))
+    jobs.append((os.path.join(HERE, "hipemu.cpp"), os.path.join(d, "hipemu.o"),
+                 [*FLAGS, *RUNTIME[variant]] if variant in RUNTIME else flags))
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, jobs))
     tmp = lib + ".tmp"
@@ -115,4 +125,4 @@ def asan_runtime():
 
 
 if __name__ == "__main__":
-    build("asan" if "--asan" in sys.argv else "plain", verbose=True)
+    build("asan" if "--asan" in sys.argv else "race" if "--race" in sys.argv else "plain", verbose=True)

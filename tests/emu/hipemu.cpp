@@ -3,6 +3,7 @@
 #include "hipemu.hpp"
 
 #include <chrono>
+#include <cxxabi.h>
 #include <cstdarg>
 #include <dlfcn.h>
 #include <map>
@@ -64,11 +65,65 @@ emu_trampoline:
 extern "C" void emu_switch(void **save_sp, void *new_sp);
 extern "C" void emu_trampoline();
 
-struct emu_stream { int id; };
-struct emu_event { double t_ms; };
-struct emu_graph { std::vector<std::function<void()>> ops; };
+// A stream's and an event's vector clock (race-checking build): entry i is
+// the number of operations of stream slot i that happen before what the
+// stream issues next / what the event marks.
+struct emu_stream {
+    int id;
+    bool blocking = true;             // synchronises with the null stream (hipStreamNonBlocking not set)
+    std::vector<uint32_t> vc;
+};
+struct emu_event {
+    double t_ms = 0.0;
+    std::vector<uint32_t> vc;
+};
+// a recorded stream operation: what it is (a kernel's address, or a copy or
+// fill) and the code that performs it
+struct emu_op {
+    const void *kfn;
+    const char *what;
+    std::function<void()> run;
+};
+struct emu_graph { std::vector<emu_op> ops; };
 
 namespace emu {
+
+// ------------------------------------------------------------------ race checker interface
+// Race-checking build (HIPEMU_RACE): every stream operation is an "op" with
+// a vector clock; the library's loads and stores reach the checker through
+// the compiler's thread-sanitizer instrumentation (callbacks at the end of
+// this file), and accesses to device memory are checked against the
+// happens-before order of the ops.  Elsewhere these are no-ops.
+namespace race {
+#ifdef HIPEMU_RACE
+void begin_op(hipStream_t s, const void *kfn, const char *what);
+void end_op();
+void set_workgroup(unsigned wg);
+void access(const void *p, size_t n, int kind);
+void *dev_alloc(size_t bytes);
+bool dev_free(void *p);
+void record(hipEvent_t e, hipStream_t s);
+bool wait(hipStream_t s, hipEvent_t e);
+void host_sync_stream(hipStream_t s);
+void host_sync_event(hipEvent_t e);
+void host_sync_device();
+void stream_created(hipStream_t s, bool blocking);
+void stream_destroyed(hipStream_t s);
+#else
+inline void begin_op(hipStream_t, const void *, const char *) {}
+inline void end_op() {}
+inline void set_workgroup(unsigned) {}
+inline void access(const void *, size_t, int) {}
+inline void record(hipEvent_t, hipStream_t) {}
+inline bool wait(hipStream_t, hipEvent_t) { return true; }
+inline void host_sync_stream(hipStream_t) {}
+inline void host_sync_event(hipEvent_t) {}
+inline void host_sync_device() {}
+inline void stream_created(hipStream_t, bool) {}
+inline void stream_destroyed(hipStream_t) {}
+#endif
+enum { READ = 0, WRITE = 1, ATOMIC = 2 };
+}  // namespace race
 
 Lane *g_lane = nullptr;
 dim3 g_block, g_grid, g_bdim;
@@ -256,6 +311,7 @@ void run_grid(const void *kfn, dim3 g, dim3 b, size_t shm, const std::function<v
         for (unsigned y = 0; y < g.y; y++)
             for (unsigned x = 0; x < g.x; x++) {
                 g_block = dim3(x, y, z);
+                race::set_workgroup(x + g.x * (y + g.y * z));
                 // fresh dynamic LDS per workgroup, exactly the launch's size,
                 // filled with a NaN pattern (LDS is not zeroed on the device)
                 void *lds = malloc(shm ? shm : 1);
@@ -286,10 +342,15 @@ double now_ms()
 }
 
 // run a stream operation now, or record it while a capture is open
-hipError_t stream_op(std::function<void()> op)
+hipError_t stream_op(hipStream_t s, const void *kfn, const char *what, std::function<void()> op)
 {
-    if (g_capture) g_capture->ops.push_back(std::move(op));
-    else op();
+    if (g_capture) {
+        g_capture->ops.push_back({kfn, what, std::move(op)});
+        return hipSuccess;
+    }
+    race::begin_op(s, kfn, what);
+    op();
+    race::end_op();
     return hipSuccess;
 }
 
@@ -329,13 +390,10 @@ unsigned char *slot(int lane)
     return g_slots[w][lane & 63];
 }
 
-void submit_launch(hipStream_t, const void *kfn, dim3 g, dim3 b, size_t shm, std::function<void()> lane_fn)
+void submit_launch(hipStream_t s, const void *kfn, const char *name, dim3 g, dim3 b, size_t shm,
+                   std::function<void()> lane_fn)
 {
-    if (g_capture) {
-        g_capture->ops.push_back([kfn, g, b, shm, lane_fn]() { exec_launch(kfn, g, b, shm, lane_fn); });
-        return;
-    }
-    exec_launch(kfn, g, b, shm, lane_fn);
+    stream_op(s, kfn, name, [kfn, g, b, shm, lane_fn]() { exec_launch(kfn, g, b, shm, lane_fn); });
 }
 
 // ------------------------------------------------------------------ MFMA
@@ -471,6 +529,7 @@ v4f mfma_f32_16x16x32_fp8(long a, long b, v4f c)
 
 void load_lds(void *dst, const void *src, size_t n)
 {
+    race::access(src, n, race::READ);
 #ifdef EMU_ASAN
     const bool bad_src = __asan_region_is_poisoned(const_cast<void *>(src), n) != nullptr;
     const bool bad_dst = __asan_region_is_poisoned(dst, n) != nullptr;
@@ -534,7 +593,11 @@ hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int)
     else *v = 0;
     return hipSuccess;
 }
-hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipDeviceSynchronize()
+{
+    race::host_sync_device();
+    return hipSuccess;
+}
 hipError_t hipGetLastError()
 {
     const hipError_t e = g_last_err;
@@ -558,69 +621,102 @@ hipError_t hipMalloc(void **p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) return hipSuccess;
+#ifdef HIPEMU_RACE
+    *p = race::dev_alloc(bytes);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+#else
     if (posix_memalign(p, 256, bytes) != 0) {
         *p = nullptr;
         return hipErrorOutOfMemory;
     }
     return hipSuccess;
+#endif
 }
 hipError_t hipFree(void *p)
 {
+#ifdef HIPEMU_RACE
+    if (p && !race::dev_free(p)) return hipErrorInvalidValue;
+#else
     free(p);
+#endif
     return hipSuccess;
 }
-hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind)
+// the synchronous copies and fills run on the null stream and return when done
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k)
 {
-    if (n) memmove(d, s, n);
-    return hipSuccess;
+    const hipError_t e = hipMemcpyAsync(d, s, n, k, nullptr);
+    race::host_sync_stream(nullptr);
+    return e;
 }
-hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t)
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t st)
 {
-    return stream_op([=]() { if (n) memmove(d, s, n); });
+    return stream_op(st, nullptr, "copy", [=]() {
+        if (!n) return;
+        race::access(s, n, race::READ);
+        race::access(d, n, race::WRITE);
+        memmove(d, s, n);
+    });
 }
 hipError_t hipMemcpy2DAsync(void *d, size_t dp, const void *s, size_t sp, size_t w, size_t h, hipMemcpyKind,
-                            hipStream_t)
+                            hipStream_t st)
 {
-    return stream_op([=]() {
-        for (size_t r = 0; r < h; r++) memmove((char *)d + r * dp, (const char *)s + r * sp, w);
+    return stream_op(st, nullptr, "2-D copy", [=]() {
+        for (size_t r = 0; r < h; r++) {
+            race::access((const char *)s + r * sp, w, race::READ);
+            race::access((char *)d + r * dp, w, race::WRITE);
+            memmove((char *)d + r * dp, (const char *)s + r * sp, w);
+        }
     });
 }
 hipError_t hipMemset(void *p, int v, size_t n)
 {
-    if (n) memset(p, v, n);
-    return hipSuccess;
+    const hipError_t e = hipMemsetAsync(p, v, n, nullptr);
+    race::host_sync_stream(nullptr);
+    return e;
 }
-hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t)
+hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t st)
 {
-    return stream_op([=]() { if (n) memset(p, v, n); });
+    return stream_op(st, nullptr, "fill", [=]() {
+        if (!n) return;
+        race::access(p, n, race::WRITE);
+        memset(p, v, n);
+    });
 }
 
-hipError_t hipStreamCreate(hipStream_t *s)
+hipError_t hipStreamCreate(hipStream_t *s) { return hipStreamCreateWithFlags(s, hipStreamDefault); }
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags)
 {
     static int next = 1;
     *s = new emu_stream{next++};
+    race::stream_created(*s, !(flags & hipStreamNonBlocking));
     return hipSuccess;
 }
-hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
 hipError_t hipExtStreamCreateWithCUMask(hipStream_t *s, uint32_t words, const uint32_t *mask)
 {
     if (words == 0 || !mask) return hipErrorInvalidValue;
-    return hipStreamCreate(s);
+    return hipStreamCreateWithFlags(s, hipStreamDefault);
 }
 hipError_t hipStreamDestroy(hipStream_t s)
 {
+    race::stream_destroyed(s);
     delete s;
     return hipSuccess;
 }
 hipError_t hipStreamSynchronize(hipStream_t s)
 {
     if (g_capture && s == g_capture_stream) return hipErrorStreamCaptureUnsupported;
+    race::host_sync_stream(s);
     return hipSuccess;
 }
-hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t e, unsigned) { return e ? hipSuccess : hipErrorInvalidValue; }
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned)
+{
+    if (!e) return hipErrorInvalidValue;
+    if (!g_capture) race::wait(s, e);   // a captured graph replays its ops in capture order
+    return hipSuccess;
+}
 hipError_t hipEventCreate(hipEvent_t *e)
 {
-    *e = new emu_event{0.0};
+    *e = new emu_event();
     return hipSuccess;
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
@@ -629,14 +725,20 @@ hipError_t hipEventDestroy(hipEvent_t e)
     delete e;
     return hipSuccess;
 }
-hipError_t hipEventRecord(hipEvent_t e, hipStream_t)
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s)
 {
     if (!e) return hipErrorInvalidValue;
     if (g_capture) return hipSuccess;
     e->t_ms = now_ms();
+    race::record(e, s);
     return hipSuccess;
 }
-hipError_t hipEventSynchronize(hipEvent_t e) { return e ? hipSuccess : hipErrorInvalidValue; }
+hipError_t hipEventSynchronize(hipEvent_t e)
+{
+    if (!e) return hipErrorInvalidValue;
+    race::host_sync_event(e);
+    return hipSuccess;
+}
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b)
 {
     if (!a || !b) return hipErrorInvalidValue;
@@ -663,10 +765,11 @@ hipError_t hipGraphInstantiate(hipGraphExec_t *x, hipGraph_t g, hipGraphNode_t *
     *x = new emu_graph(*g);
     return hipSuccess;
 }
-hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t)
+hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s)
 {
     if (!x) return hipErrorInvalidValue;
-    for (auto &op : x->ops) op();
+    // the recorded ops one after the other on the launch stream
+    for (auto &op : x->ops) stream_op(s, op.kfn, op.what, op.run);
     return hipSuccess;
 }
 hipError_t hipGraphExecDestroy(hipGraphExec_t x)
@@ -703,3 +806,393 @@ extern "C" {
 // launches and blocks run so far (a test can check that work was done)
 long hipemu_ticks(void) { return (long)g_tick; }
 }
+
+// ------------------------------------------------------------------ race checker
+// Built with HIPEMU_RACE: the library's sources are compiled with
+// -fsanitize=thread, which makes the compiler call __tsan_read4(p),
+// __tsan_write8(p), __tsan_memcpy(...) and so on around every load, store and
+// memory intrinsic; this file (compiled without it) defines those callbacks
+// itself -- no sanitizer runtime is linked -- and checks each access to device
+// memory against the happens-before order of the stream operations:
+//   * every copy, fill and kernel launch is an op on its stream; an op issued
+//     on stream s happens after the ops s has seen: its own earlier ops, the
+//     ops behind every event it waited for (at the time of the wait), the
+//     host's synchronisations so far, and for a blocking stream the null
+//     stream's ops (and the other way round);
+//   * device memory is a reserved address range with a shadow record per
+//     4-byte granule: the op and workgroup that last wrote it and up to two
+//     ops that read it since;
+//   * a read of a granule whose writer op does not happen before the reader,
+//     and a write of a granule whose writer or readers do not happen before
+//     the writer, is a race between streams; within one launch, a granule
+//     written by one workgroup and read or written by another is a race
+//     between workgroups (atomics excepted);
+//   * host code that loads or stores device memory is reported too (on the
+//     device it would fault).
+// What it does not see: accesses wider than 16 bytes in one instruction (the
+// instrumentation skips them), LDS, and ops inside a captured graph (replayed
+// in capture order on one stream; the same schedule issued eagerly is checked).
+#ifdef HIPEMU_RACE
+namespace emu {
+namespace race {
+namespace {
+
+constexpr uintptr_t DEV_SPAN = (uintptr_t)1 << 40;  // address space reserved for device memory
+constexpr uintptr_t ALIGN = 1 << 16;
+constexpr uint32_t ATOMIC_WG = 0xffffffffu;          // granule last written by an atomic
+constexpr uint32_t MANY_WG = 0xfffffffeu;            // read by several workgroups of one op
+
+struct Gr { uint32_t w, wwg, r[2], rwg[2]; };        // 24 bytes per 4-byte granule
+
+uintptr_t g_base = 0;
+Gr *g_shadow = nullptr;
+uintptr_t g_top = 0;
+std::map<uintptr_t, size_t> g_blocks;                 // device block -> reserved bytes
+std::map<uintptr_t, size_t> g_req;                    // device block -> requested bytes
+
+struct Op {
+    int slot;
+    uint32_t seq;
+    std::vector<uint32_t> vc;
+    const void *kfn;
+    const char *what;
+    int sid;
+};
+std::vector<Op> g_ops;                                // op id i + 1
+uint32_t g_cur = 0;                                   // op running now (0: host code)
+uint32_t g_wg = 0;
+int g_nslots = 1;                                     // slot 0: the null stream
+emu_stream g_null{0};
+std::vector<uint32_t> g_host;                         // what the host has waited for
+std::vector<emu_stream *> g_streams;
+int g_drop_wait = 0;                                  // planted mistake: skip the n-th wait from now
+
+long g_count = 0;
+std::string g_report;
+std::map<std::tuple<uint32_t, uint32_t, int>, int> g_seen;
+
+__attribute__((constructor)) void reserve()
+{
+    void *p = mmap(nullptr, DEV_SPAN, PROT_NONE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+    void *q = mmap(nullptr, DEV_SPAN / 4 * sizeof(Gr), PROT_NONE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+    if (p == MAP_FAILED || q == MAP_FAILED) fatal("race checker: address space reservation failed");
+    g_base = ((uintptr_t)p + ALIGN - 1) & ~(ALIGN - 1);
+    g_shadow = (Gr *)q;
+}
+
+void join(std::vector<uint32_t> &a, const std::vector<uint32_t> &b)
+{
+    if (a.size() < b.size()) a.resize(b.size(), 0);
+    for (size_t i = 0; i < b.size(); i++) a[i] = std::max(a[i], b[i]);
+}
+
+int slot_of(hipStream_t s) { return s ? s->id : 0; }
+emu_stream *st(hipStream_t s) { return s ? s : &g_null; }
+
+// does op a (1-based) happen before op b?
+bool hb(uint32_t a, uint32_t b)
+{
+    const Op &A = g_ops[a - 1], &B = g_ops[b - 1];
+    return (size_t)A.slot < B.vc.size() && B.vc[A.slot] >= A.seq;
+}
+
+std::string op_name(uint32_t id)
+{
+    const Op &o = g_ops[id - 1];
+    char buf[64];
+    snprintf(buf, sizeof buf, " (op %u, stream %d)", id, o.sid);
+    // a kernel by its symbol (the expression at its launch when the symbol is
+    // local to its file), a copy or fill by what it is
+    std::string k = o.kfn ? kname(o.kfn) : std::string();
+    int st = 0;
+    if (k == "?") k = o.what;
+    else if (char *d = abi::__cxa_demangle(k.c_str(), nullptr, nullptr, &st)) {
+        k = d;
+        free(d);
+    }
+    if (k.empty()) k = o.what;
+    return k + buf;
+}
+
+std::string where(uintptr_t a)
+{
+    auto it = g_req.upper_bound(a);
+    if (it == g_req.begin()) return "?";
+    --it;
+    char buf[96];
+    snprintf(buf, sizeof buf, "byte %zu of a %zu-byte device block", (size_t)(a - it->first), it->second);
+    return buf;
+}
+
+void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1, uint32_t wg2)
+{
+    static const char *const names[] = {"read after an unordered write", "write after an unordered write",
+                                        "write after an unordered read", "host access to device memory",
+                                        "read of another workgroup's write in one launch",
+                                        "write over another workgroup's write in one launch",
+                                        "write over another workgroup's read in one launch"};
+    g_count++;
+    auto key = std::make_tuple(first, second, kind);
+    if (g_seen.count(key) || g_seen.size() >= 64) return;
+    g_seen[key] = 1;
+    std::string m = std::string(names[kind]) + " at " + where(a) + ": ";
+    if (kind == 3) {
+        m += "host code";
+    } else if (kind >= 4) {
+        char buf[64];
+        snprintf(buf, sizeof buf, " workgroups %u and %u", wg1, wg2);
+        m += op_name(second) + buf;
+    } else {
+        m += op_name(first) + " and " + op_name(second);
+    }
+    g_report += m + "\n";
+}
+
+void check(uintptr_t a, size_t n, int kind)
+{
+    if (a - g_base >= DEV_SPAN || n == 0) return;
+    if (!g_cur) {
+        report(3, 0, 0, a, 0, 0);
+        return;
+    }
+    const uint32_t me = g_cur, wg = kind == ATOMIC ? ATOMIC_WG : g_wg;
+    const size_t i0 = (a - g_base) / 4, i1 = (a + n - 1 - g_base) / 4;
+    for (size_t i = i0; i <= i1; i++) {
+        Gr &g = g_shadow[i];
+        const uintptr_t ga = g_base + 4 * i;
+        if (kind == READ) {
+            if (g.w && g.w != me && !hb(g.w, me)) report(0, g.w, me, ga, 0, 0);
+            else if (g.w == me && g.wwg != wg && g.wwg != ATOMIC_WG) report(4, me, me, ga, g.wwg, wg);
+            int k = -1;
+            for (int j = 0; j < 2; j++)
+                if (g.r[j] == me) k = j;
+            if (k >= 0) {
+                if (g.rwg[k] != wg) g.rwg[k] = MANY_WG;
+                continue;
+            }
+            // readers that happen before this one are covered by it
+            for (int j = 0; j < 2; j++)
+                if (g.r[j] && hb(g.r[j], me)) g.r[j] = 0;
+            int f = g.r[0] == 0 ? 0 : g.r[1] == 0 ? 1 : -1;
+            if (f < 0) {     // keep the newer of the two
+                g.r[0] = g.r[1];
+                g.rwg[0] = g.rwg[1];
+                f = 1;
+            }
+            g.r[f] = me;
+            g.rwg[f] = wg;
+        } else {
+            if (g.w && g.w != me && !hb(g.w, me)) report(1, g.w, me, ga, 0, 0);
+            else if (g.w == me && g.wwg != wg && g.wwg != ATOMIC_WG && wg != ATOMIC_WG)
+                report(5, me, me, ga, g.wwg, wg);
+            for (int j = 0; j < 2; j++) {
+                if (!g.r[j]) continue;
+                if (g.r[j] != me && !hb(g.r[j], me)) report(2, g.r[j], me, ga, 0, 0);
+                else if (g.r[j] == me && g.rwg[j] != wg && wg != ATOMIC_WG) report(6, me, me, ga, g.rwg[j], wg);
+            }
+            g.w = me;
+            g.wwg = wg;
+            g.r[0] = g.r[1] = 0;
+        }
+    }
+}
+
+}  // namespace
+
+void *dev_alloc(size_t bytes)
+{
+    const size_t span = (bytes + ALIGN - 1) & ~(ALIGN - 1);
+    if (g_top + span + ALIGN > DEV_SPAN) return nullptr;
+    const uintptr_t p = g_base + g_top;
+    Gr *sh = g_shadow + (p - g_base) / 4;
+    if (mprotect((void *)p, span, PROT_READ | PROT_WRITE) != 0 ||
+        mprotect((void *)sh, span / 4 * sizeof(Gr), PROT_READ | PROT_WRITE) != 0)
+        return nullptr;
+    g_top += span + ALIGN;                            // an inaccessible gap after each block
+    g_blocks[p] = span;
+    g_req[p] = bytes;
+    return (void *)p;
+}
+
+bool dev_free(void *vp)
+{
+    const uintptr_t p = (uintptr_t)vp;
+    auto it = g_blocks.find(p);
+    if (it == g_blocks.end()) return false;
+    const size_t span = it->second;
+    Gr *sh = g_shadow + (p - g_base) / 4;
+    // addresses are never reused: a later access to the block faults
+    madvise((void *)p, span, MADV_DONTNEED);
+    mprotect((void *)p, span, PROT_NONE);
+    madvise((void *)sh, span / 4 * sizeof(Gr), MADV_DONTNEED);
+    mprotect((void *)sh, span / 4 * sizeof(Gr), PROT_NONE);
+    g_blocks.erase(it);
+    g_req.erase(p);
+    return true;
+}
+
+void stream_created(hipStream_t s, bool blocking)
+{
+    s->blocking = blocking;
+    g_nslots = std::max(g_nslots, s->id + 1);
+    s->vc = g_host;                                   // work the host waited for precedes it
+    g_streams.push_back(s);
+}
+
+void stream_destroyed(hipStream_t s)
+{
+    g_streams.erase(std::remove(g_streams.begin(), g_streams.end(), s), g_streams.end());
+}
+
+void begin_op(hipStream_t s, const void *kfn, const char *what)
+{
+    emu_stream *t = st(s);
+    join(t->vc, g_host);
+    if (!s) {
+        for (emu_stream *o : g_streams)
+            if (o->blocking) join(t->vc, o->vc);
+    } else if (s->blocking) {
+        join(t->vc, g_null.vc);
+    }
+    const int sl = slot_of(s);
+    if ((int)t->vc.size() <= sl) t->vc.resize(sl + 1, 0);
+    t->vc[sl]++;
+    g_ops.push_back(Op{sl, t->vc[sl], t->vc, kfn, what, s ? s->id : 0});
+    g_cur = (uint32_t)g_ops.size();
+    g_wg = 0;
+}
+
+void end_op() { g_cur = 0; }
+
+void set_workgroup(unsigned wg) { g_wg = wg; }
+
+void access(const void *p, size_t n, int kind) { check((uintptr_t)p, n, kind); }
+
+void record(hipEvent_t e, hipStream_t s)
+{
+    emu_stream *t = st(s);
+    join(t->vc, g_host);
+    e->vc = t->vc;
+}
+
+bool wait(hipStream_t s, hipEvent_t e)
+{
+    if (g_drop_wait > 0 && --g_drop_wait == 0) return false;
+    emu_stream *t = st(s);
+    join(t->vc, g_host);
+    join(t->vc, e->vc);
+    return true;
+}
+
+void host_sync_stream(hipStream_t s)
+{
+    join(g_host, st(s)->vc);
+}
+
+void host_sync_event(hipEvent_t e) { join(g_host, e->vc); }
+
+void host_sync_device()
+{
+    join(g_host, g_null.vc);
+    for (emu_stream *o : g_streams) join(g_host, o->vc);
+}
+
+}  // namespace race
+
+void race_atomic(const void *p, size_t n) { race::check((uintptr_t)p, n, race::ATOMIC); }
+
+}  // namespace emu
+
+// the thread-sanitizer instrumentation's callbacks
+extern "C" {
+void __tsan_init() {}
+void __tsan_func_entry(void *) {}
+void __tsan_func_exit() {}
+void __tsan_vptr_update(void **, void *) {}
+void __tsan_vptr_read(void **) {}
+#define EMU_TSAN_RW(N)                                                                            \
+    void __tsan_read##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::READ); }          \
+    void __tsan_write##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::WRITE); }        \
+    void __tsan_unaligned_read##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::READ); } \
+    void __tsan_unaligned_write##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::WRITE); } \
+    void __tsan_volatile_read##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::READ); }  \
+    void __tsan_volatile_write##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::WRITE); }
+EMU_TSAN_RW(1)
+EMU_TSAN_RW(2)
+EMU_TSAN_RW(4)
+EMU_TSAN_RW(8)
+EMU_TSAN_RW(16)
+#undef EMU_TSAN_RW
+void __tsan_read_range(void *p, unsigned long n) { emu::race::check((uintptr_t)p, n, emu::race::READ); }
+void __tsan_write_range(void *p, unsigned long n) { emu::race::check((uintptr_t)p, n, emu::race::WRITE); }
+void *__tsan_memcpy(void *d, const void *s, size_t n)
+{
+    emu::race::check((uintptr_t)s, n, emu::race::READ);
+    emu::race::check((uintptr_t)d, n, emu::race::WRITE);
+    return memcpy(d, s, n);
+}
+void *__tsan_memmove(void *d, const void *s, size_t n)
+{
+    emu::race::check((uintptr_t)s, n, emu::race::READ);
+    emu::race::check((uintptr_t)d, n, emu::race::WRITE);
+    return memmove(d, s, n);
+}
+void *__tsan_memset(void *d, int v, size_t n)
+{
+    emu::race::check((uintptr_t)d, n, emu::race::WRITE);
+    return memset(d, v, n);
+}
+// host-side atomics of the library (reference counts, locks): performed as such
+#define EMU_TSAN_ATOMIC(B, T)                                                                              \
+    T __tsan_atomic##B##_load(const volatile T *a, int) { return __atomic_load_n(a, __ATOMIC_SEQ_CST); }   \
+    void __tsan_atomic##B##_store(volatile T *a, T v, int) { __atomic_store_n(a, v, __ATOMIC_SEQ_CST); }   \
+    T __tsan_atomic##B##_exchange(volatile T *a, T v, int) { return __atomic_exchange_n(a, v, __ATOMIC_SEQ_CST); } \
+    T __tsan_atomic##B##_fetch_add(volatile T *a, T v, int) { return __atomic_fetch_add(a, v, __ATOMIC_SEQ_CST); } \
+    T __tsan_atomic##B##_fetch_sub(volatile T *a, T v, int) { return __atomic_fetch_sub(a, v, __ATOMIC_SEQ_CST); } \
+    T __tsan_atomic##B##_fetch_and(volatile T *a, T v, int) { return __atomic_fetch_and(a, v, __ATOMIC_SEQ_CST); } \
+    T __tsan_atomic##B##_fetch_or(volatile T *a, T v, int) { return __atomic_fetch_or(a, v, __ATOMIC_SEQ_CST); }   \
+    T __tsan_atomic##B##_fetch_xor(volatile T *a, T v, int) { return __atomic_fetch_xor(a, v, __ATOMIC_SEQ_CST); } \
+    int __tsan_atomic##B##_compare_exchange_strong(volatile T *a, T *c, T v, int, int)                    \
+    {                                                                                                      \
+        return __atomic_compare_exchange_n(a, c, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);            \
+    }                                                                                                      \
+    int __tsan_atomic##B##_compare_exchange_weak(volatile T *a, T *c, T v, int, int)                      \
+    {                                                                                                      \
+        return __atomic_compare_exchange_n(a, c, v, true, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);             \
+    }                                                                                                      \
+    T __tsan_atomic##B##_compare_exchange_val(volatile T *a, T c, T v, int, int)                          \
+    {                                                                                                      \
+        __atomic_compare_exchange_n(a, &c, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);                  \
+        return c;                                                                                          \
+    }
+EMU_TSAN_ATOMIC(8, unsigned char)
+EMU_TSAN_ATOMIC(16, unsigned short)
+EMU_TSAN_ATOMIC(32, unsigned int)
+EMU_TSAN_ATOMIC(64, unsigned long long)
+#undef EMU_TSAN_ATOMIC
+void __tsan_atomic_thread_fence(int) { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+void __tsan_atomic_signal_fence(int) { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+// test hooks: races found so far, their description, and a planted mistake
+long hipemu_races(void) { return emu::race::g_count; }
+long hipemu_race_report(char *buf, long cap)
+{
+    const std::string &r = emu::race::g_report;
+    if (buf && cap > 0) {
+        const size_t n = std::min((size_t)cap - 1, r.size());
+        memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
+    return (long)r.size();
+}
+void hipemu_race_reset(void)
+{
+    emu::race::g_count = 0;
+    emu::race::g_report.clear();
+    emu::race::g_seen.clear();
+}
+// the n-th hipStreamWaitEvent from now is skipped (0: none)
+void hipemu_drop_wait(int n) { emu::race::g_drop_wait = n; }
+long hipemu_ops(void) { return (long)emu::race::g_ops.size(); }
+}
+#endif

@@ -23,7 +23,10 @@
 // records the launches and copies issued until hipStreamEndCapture, and
 // hipGraphLaunch replays them.  Dynamic LDS is a heap block of exactly the
 // launch's size and hipMalloc returns blocks of exactly the requested size,
-// so an AddressSanitizer build reports any access outside them.
+// so an AddressSanitizer build reports any access outside them.  The
+// race-checking build (HIPEMU_RACE) instead checks every access to device
+// memory against the happens-before order of the stream operations, which
+// issue order is only one instance of (hipemu.cpp, end of file).
 #pragma once
 
 #include <algorithm>
@@ -215,7 +218,9 @@ unsigned long long clock_tick();
 inline int lane_id() { return g_lane->lin & 63; }
 
 // submit one launch (run now, or record while a capture is open)
-void submit_launch(hipStream_t s, const void *kfn, dim3 g, dim3 b, size_t shm, std::function<void()> lane_fn);
+// (name: the kernel expression as written at the launch, for reports)
+void submit_launch(hipStream_t s, const void *kfn, const char *name, dim3 g, dim3 b, size_t shm,
+                   std::function<void()> lane_fn);
 
 // one value per lane in, the value of lane src out
 template <class T>
@@ -248,13 +253,13 @@ int update_dpp(int old, int src, int ctrl);
 }  // namespace emu
 
 template <class... P, class... A>
-inline void emu_launch(void (*k)(P...), dim3 g, dim3 b, size_t shm, hipStream_t s, A &&...a)
+inline void emu_launch(const char *name, void (*k)(P...), dim3 g, dim3 b, size_t shm, hipStream_t s, A &&...a)
 {
     auto args = std::make_shared<std::tuple<std::decay_t<P>...>>(static_cast<std::decay_t<P>>(a)...);
-    emu::submit_launch(s, (const void *)k, g, b, shm, [k, args]() { std::apply(k, *args); });
+    emu::submit_launch(s, (const void *)k, name, g, b, shm, [k, args]() { std::apply(k, *args); });
 }
 #define hipLaunchKernelGGL(K, G, B, S, STR, ...) \
-    emu_launch((K), dim3(G), dim3(B), (size_t)(S), (hipStream_t)(STR), ##__VA_ARGS__)
+    emu_launch(#K, (K), dim3(G), dim3(B), (size_t)(S), (hipStream_t)(STR), ##__VA_ARGS__)
 
 #define threadIdx (emu::g_lane->tid)
 #define blockIdx (emu::g_block)
@@ -310,17 +315,33 @@ inline double __hiloint2double(int hi, int lo)
 inline unsigned long long wall_clock64() { return emu::clock_tick(); }
 inline unsigned long long clock64() { return emu::clock_tick(); }
 
-// one OS thread runs every lane, so plain read-modify-writes are atomic
+// one OS thread runs every lane, so plain read-modify-writes are atomic.  In
+// the race-checking build (HIPEMU_RACE) each one is reported to the checker
+// as one atomic access and its own loads and stores are left uninstrumented.
+namespace emu {
+#ifdef HIPEMU_RACE
+void race_atomic(const void *p, size_t n);
+#else
+inline void race_atomic(const void *, size_t) {}
+#endif
+}  // namespace emu
+#define EMU_ATOMIC __attribute__((no_sanitize("thread")))
 template <class T>
-inline T atomicAdd(T *p, T v) { T o = *p; *p = o + v; return o; }
+EMU_ATOMIC inline T atomicAdd(T *p, T v) { emu::race_atomic(p, sizeof(T)); T o = *p; *p = o + v; return o; }
 template <class T>
-inline T atomicCAS(T *p, T cmp, T v) { T o = *p; if (o == cmp) *p = v; return o; }
+EMU_ATOMIC inline T atomicCAS(T *p, T cmp, T v)
+{
+    emu::race_atomic(p, sizeof(T));
+    T o = *p;
+    if (o == cmp) *p = v;
+    return o;
+}
 template <class T>
-inline T atomicMin(T *p, T v) { T o = *p; if (v < o) *p = v; return o; }
+EMU_ATOMIC inline T atomicMin(T *p, T v) { emu::race_atomic(p, sizeof(T)); T o = *p; if (v < o) *p = v; return o; }
 template <class T>
-inline T atomicMax(T *p, T v) { T o = *p; if (v > o) *p = v; return o; }
+EMU_ATOMIC inline T atomicMax(T *p, T v) { emu::race_atomic(p, sizeof(T)); T o = *p; if (v > o) *p = v; return o; }
 template <class T>
-inline T atomicExch(T *p, T v) { T o = *p; *p = v; return o; }
+EMU_ATOMIC inline T atomicExch(T *p, T v) { emu::race_atomic(p, sizeof(T)); T o = *p; *p = v; return o; }
 
 #define EMU_MINMAX(T)                                  \
     inline T min(T a, T b) { return b < a ? b : a; } \

@@ -1,0 +1,135 @@
+"""Stream and workgroup race check of the library's schedules on the CPU.
+
+The race variant of the CPU build (tests/emu, PALIAL_HIPEMU=race) compiles
+every palial_amd/csrc/*.hip file with the compiler's thread-sanitizer
+instrumentation and gives the instrumentation callbacks to the emulator
+runtime, so that each load and store the real kernels make is checked against
+the happens-before order of the stream operations: stream order, event
+record -> wait, host synchronisation and the null stream.  A read or write of
+device memory that conflicts with an unordered access of another operation,
+or with another workgroup's access in the same launch, fails the test
+(tests/conftest.py).  This covers what the issue-order emulation cannot: a
+missing wait, a workspace shared by two streams, a tile read by one stream
+while another rewrites it -- for every schedule the GPU test files exercise,
+including paired trailing updates and blocked correction sweeps.
+
+Default: a subset of GPU tests (one per kind of schedule), run in parallel
+processes, plus planted mistakes that the check must report.
+PALIAL_EMU_FULL=1 runs whole GPU test files instead.
+"""
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+FULL = os.environ.get("PALIAL_EMU_FULL") == "1"
+
+# group: (GPU test node ids of the default subset, (targets, -k) of the full run)
+GROUPS = {
+    "factorizations": (["tests/test_gpu_parity.py::test_graph_and_streams_bitwise[cholesky]"],
+                       (["tests/test_gpu_parity.py"], "not 2048 and not test_qr")),
+    "qr": (["tests/test_gpu_parity.py::test_qr[512-128]", "tests/test_gpu_parity.py::test_qr_tiles[64]"],
+           (["tests/test_gpu_parity.py", "tests/test_gpu_qr_apply.py"],
+            "(test_qr or qr_apply or form_q or qr_solve) and not 2048")),
+    "solves": (["tests/test_gpu_solve.py::test_solve_diag_dominant[512-128-1-2-cholesky]",
+                "tests/test_gpu_solve.py::test_solve_diag_dominant[512-128-1-4-lu]",
+                "tests/test_gpu_solve.py::test_solve_diag_dominant[512-128-1-3-lu]"],
+               (["tests/test_gpu_solve.py"], "not 1024")),
+    "paired_updates": (["tests/wip_gpu_pair.py::test_paired_updates_match[1280-128-0-cholesky-2]"],
+                       (["tests/wip_gpu_pair.py"], "1280 or rejects")),
+    "blocked_sweeps": (["tests/wip_gpu_sweep_block.py::test_sweep_blocks_agree[1-2-cholesky]"],
+                       (["tests/wip_gpu_sweep_block.py"], "not tuner")),
+    "batched_and_tiles": (["tests/test_gpu_batched.py::test_batched_single_kind"],
+                          (["tests/test_gpu_batched.py", "tests/test_gpu_update.py", "tests/test_zz_no_alloc.py"],
+                           None)),
+}
+
+
+@pytest.fixture(scope="module")
+def race_lib():
+    sys.path.insert(0, HERE)
+    from emu import build as emu_build
+    return emu_build.build("race")
+
+
+def _env():
+    return dict(os.environ, PALIAL_HIPEMU="race")
+
+
+def _start(targets, kexpr=None):
+    cmd = [sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "gpu", *targets]
+    if kexpr:
+        cmd += ["-k", kexpr]
+    return subprocess.Popen(cmd, cwd=ROOT, env=_env(), stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+
+
+def test_schedules_race_free(race_lib):
+    """Every group's GPU tests pass on the race-checking build: their results
+    meet the GPU bars and no access races (the groups run side by side)."""
+    procs = {}
+    for name, (subset, (targets, kexpr)) in GROUPS.items():
+        procs[name] = _start(targets, kexpr) if FULL else _start(subset)
+    bad = {}
+    for name, p in procs.items():
+        out, err = p.communicate(timeout=4 * 3600 if FULL else 1200)
+        if p.returncode != 0 or " passed" not in out or "failed" in out or "error" in out:
+            bad[name] = out[-3000:] + err[-1000:]
+    assert not bad, bad
+
+
+_PLANT = r"""
+import ctypes as C, sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import palial_amd as P
+import oracle
+P.LIB_PATH = sys.argv[2]
+P.init(0)
+L = P.lib()
+L.hipemu_races.restype = C.c_long
+N, b = 512, 128
+rhs = np.random.default_rng(1).standard_normal((N, 1))
+def run(algo, drop, prec=None):
+    a = oracle.gen("sym" if algo == "cholesky" else "dense", N, b, seed=3, shift=0.0 if algo == "qr" else N)
+    L.hipemu_race_reset()
+    L.hipemu_drop_wait(drop)
+    if prec is not None:
+        P.solve(a, rhs, algo=algo, precision=prec, b=b, factor_tile=b, check=False)
+    elif algo == "qr":
+        P.qr(a, np.zeros(N * N), b=b, no_graph=True)
+    else:
+        getattr(P, algo)(a, b=b, no_graph=True)
+    L.hipemu_drop_wait(0)
+    return L.hipemu_races()
+for algo, prec, drops in (("cholesky", None, (3, 5)), ("lu", None, (3,)), ("qr", None, (17, 18)),
+                          ("cholesky", 2, (6,))):
+    print(algo, prec, run(algo, 0, prec), *[run(algo, d, prec) for d in drops], flush=True)
+assert L.hipemu_plant_wg_race() == 0
+buf = C.create_string_buffer(1 << 14)
+L.hipemu_race_report(buf, len(buf))
+print("WG", L.hipemu_races(), buf.value.decode().replace("\n", " | "))
+"""
+
+
+def test_checker_reports_planted_races(race_lib):
+    """The check has teeth.  With one event wait of a schedule left out
+    (hipemu_drop_wait: the harness's planted mistake) the FP64 Cholesky, LU
+    and QR factorizations and the BF16 Cholesky solve report racing accesses,
+    and the same runs with every wait in place report none; a launch in which
+    two workgroups store to one element, or one reads another's store, is
+    reported as a race between workgroups."""
+    r = subprocess.run([sys.executable, "-c", _PLANT, ROOT, race_lib], cwd=ROOT, env=_env(), capture_output=True,
+                       text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    lines = [l.split() for l in r.stdout.splitlines()]
+    runs = [l for l in lines if l[0] != "WG"]
+    assert len(runs) == 4, r.stdout
+    for l in runs:
+        clean, dropped = int(l[2]), [int(x) for x in l[3:]]
+        assert clean == 0 and all(d > 0 for d in dropped), l
+    wg = r.stdout[r.stdout.index("WG "):]
+    assert "write over another workgroup's write in one launch" in wg, wg
+    assert "read of another workgroup's write in one launch" in wg, wg
