@@ -99,6 +99,7 @@ namespace race {
 void begin_op(hipStream_t s, const void *kfn, const char *what);
 void end_op();
 void set_workgroup(unsigned wg);
+void barrier_opened();
 void access(const void *p, size_t n, int kind);
 void *dev_alloc(size_t bytes);
 bool dev_free(void *p);
@@ -113,6 +114,7 @@ void stream_destroyed(hipStream_t s);
 inline void begin_op(hipStream_t, const void *, const char *) {}
 inline void end_op() {}
 inline void set_workgroup(unsigned) {}
+inline void barrier_opened() {}
 inline void access(const void *, size_t, int) {}
 inline void record(hipEvent_t, hipStream_t) {}
 inline bool wait(hipStream_t, hipEvent_t) { return true; }
@@ -128,6 +130,7 @@ enum { READ = 0, WRITE = 1, ATOMIC = 2 };
 Lane *g_lane = nullptr;
 dim3 g_block, g_grid, g_bdim;
 void *g_dyn_lds = nullptr;
+size_t g_dyn_bytes = 0;                  // size of this launch's dynamic LDS block
 
 namespace {
 
@@ -286,6 +289,7 @@ void run_block()
         if (bar == live) {
             for (int i = 0; i < g_nt; i++)
                 if (g_fib[i].state == BAR) g_fib[i].state = RUN;
+            race::barrier_opened();
             continue;
         }
         fatal("deadlock in %s block (%u,%u,%u): %d live lanes, %d at the barrier", kname(g_kfn).c_str(), g_block.x,
@@ -317,6 +321,7 @@ void run_grid(const void *kfn, dim3 g, dim3 b, size_t shm, const std::function<v
                 void *lds = malloc(shm ? shm : 1);
                 memset(lds, 0xff, shm ? shm : 1);
                 g_dyn_lds = lds;
+                g_dyn_bytes = shm;
                 run_block();
                 free(lds);
                 g_dyn_lds = nullptr;
@@ -530,6 +535,7 @@ v4f mfma_f32_16x16x32_fp8(long a, long b, v4f c)
 void load_lds(void *dst, const void *src, size_t n)
 {
     race::access(src, n, race::READ);
+    race::access(dst, n, race::WRITE);       // the lane's store to LDS
 #ifdef EMU_ASAN
     const bool bad_src = __asan_region_is_poisoned(const_cast<void *>(src), n) != nullptr;
     const bool bad_dst = __asan_region_is_poisoned(dst, n) != nullptr;
@@ -828,10 +834,15 @@ long hipemu_ticks(void) { return (long)g_tick; }
 //     written by one workgroup and read or written by another is a race
 //     between workgroups (atomics excepted);
 //   * host code that loads or stores device memory is reported too (on the
-//     device it would fault).
+//     device it would fault);
+//   * LDS (the __shared__ arrays, gathered in section emu_lds, and a launch's
+//     dynamic block): an access that conflicts with another wave's access
+//     in the same barrier interval is a race, and a read of a granule this
+//     workgroup has not written reads what an earlier workgroup left.
 // What it does not see: accesses wider than 16 bytes in one instruction (the
-// instrumentation skips them), LDS, and ops inside a captured graph (replayed
-// in capture order on one stream; the same schedule issued eagerly is checked).
+// instrumentation skips them), conflicts between lanes of one wave (they run
+// in lockstep on the device), and ops inside a captured graph (replayed in
+// capture order on one stream; the same schedule issued eagerly is checked).
 #ifdef HIPEMU_RACE
 namespace emu {
 namespace race {
@@ -870,6 +881,17 @@ int g_drop_wait = 0;                                  // planted mistake: skip t
 long g_count = 0;
 std::string g_report;
 std::map<std::tuple<uint32_t, uint32_t, int>, int> g_seen;
+
+// LDS: per 4-byte granule, the workgroup run ("generation") and barrier
+// interval of the last write and its wave, and the waves that read it in the
+// last interval it was read in
+struct Ls {
+    uint32_t wgen, wep, rgen, rep;
+    uint16_t wwave, rmask;
+};
+uint32_t g_gen = 0;                                   // workgroups started so far
+uint32_t g_epoch = 0;                                 // barriers opened in this workgroup
+std::vector<Ls> g_lds_static, g_lds_dyn;
 
 __attribute__((constructor)) void reserve()
 {
@@ -930,13 +952,21 @@ void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1
                                         "write after an unordered read", "host access to device memory",
                                         "read of another workgroup's write in one launch",
                                         "write over another workgroup's write in one launch",
-                                        "write over another workgroup's read in one launch"};
+                                        "write over another workgroup's read in one launch",
+                                        "LDS read of another wave's write with no barrier between",
+                                        "LDS write over another wave's write with no barrier between",
+                                        "LDS write over another wave's read with no barrier between",
+                                        "LDS read before the workgroup wrote it"};
     g_count++;
     auto key = std::make_tuple(first, second, kind);
     if (g_seen.count(key) || g_seen.size() >= 64) return;
     g_seen[key] = 1;
-    std::string m = std::string(names[kind]) + " at " + where(a) + ": ";
-    if (kind == 3) {
+    std::string m = std::string(names[kind]) + " at " + (kind >= 7 ? std::string("an LDS byte") : where(a)) + ": ";
+    if (kind >= 7) {
+        char buf[96];
+        snprintf(buf, sizeof buf, " workgroup %u, waves %u and %u", g_wg, wg1, wg2);
+        m += op_name(second) + buf;
+    } else if (kind == 3) {
         m += "host code";
     } else if (kind >= 4) {
         char buf[64];
@@ -948,9 +978,64 @@ void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1
     g_report += m + "\n";
 }
 
+extern "C" {
+extern char __start_emu_lds[] __attribute__((weak, visibility("hidden")));
+extern char __stop_emu_lds[] __attribute__((weak, visibility("hidden")));
+}
+
+// an access of the running lane to LDS (a __shared__ array or the launch's
+// dynamic block); anything else is not LDS
+void lds_check(uintptr_t a, size_t n, int kind)
+{
+    Ls *sh;
+    const uintptr_t s0 = (uintptr_t)__start_emu_lds, s1 = (uintptr_t)__stop_emu_lds;
+    const uintptr_t d0 = (uintptr_t)g_dyn_lds;
+    if (s0 && a >= s0 && a < s1) {
+        if (g_lds_static.empty()) g_lds_static.assign((s1 - s0) / 4 + 1, Ls{});
+        sh = g_lds_static.data() + (a - s0) / 4;
+        n = std::min<size_t>(n, s1 - a);
+    } else if (d0 && a >= d0 && a < d0 + g_dyn_bytes) {
+        if (g_lds_dyn.size() < g_dyn_bytes / 4 + 1) g_lds_dyn.assign(160 * 1024 / 4 + 1, Ls{});
+        sh = g_lds_dyn.data() + (a - d0) / 4;
+        n = std::min<size_t>(n, d0 + g_dyn_bytes - a);
+    } else {
+        return;
+    }
+    const uint16_t w = (uint16_t)(g_lane->lin >> 6);
+    const size_t cnt = (a + n - 1) / 4 - a / 4 + 1;
+    for (size_t i = 0; i < cnt; i++) {
+        Ls &l = sh[i];
+        const uintptr_t ga = a + 4 * i;
+        if (kind == READ) {
+            if (l.wgen != g_gen) report(10, 0, g_cur, ga, w, w);
+            else if (l.wep == g_epoch && l.wwave != w) report(7, 0, g_cur, ga, l.wwave, w);
+            if (l.rgen == g_gen && l.rep == g_epoch) {
+                l.rmask |= (uint16_t)(1u << w);
+            } else {
+                l.rgen = g_gen;
+                l.rep = g_epoch;
+                l.rmask = (uint16_t)(1u << w);
+            }
+        } else {
+            if (l.wgen == g_gen && l.wep == g_epoch && l.wwave != w) report(8, 0, g_cur, ga, l.wwave, w);
+            if (l.rgen == g_gen && l.rep == g_epoch && (l.rmask & ~(1u << w))) {
+                unsigned o = l.rmask & ~(1u << w);
+                report(9, 0, g_cur, ga, (unsigned)__builtin_ctz(o), w);
+            }
+            l.wgen = g_gen;
+            l.wep = g_epoch;
+            l.wwave = w;
+        }
+    }
+}
+
 void check(uintptr_t a, size_t n, int kind)
 {
-    if (a - g_base >= DEV_SPAN || n == 0) return;
+    if (n == 0) return;
+    if (a - g_base >= DEV_SPAN) {
+        if (g_cur && g_lane) lds_check(a, n, kind);
+        return;
+    }
     if (!g_cur) {
         report(3, 0, 0, a, 0, 0);
         return;
@@ -1064,7 +1149,14 @@ void begin_op(hipStream_t s, const void *kfn, const char *what)
 
 void end_op() { g_cur = 0; }
 
-void set_workgroup(unsigned wg) { g_wg = wg; }
+void set_workgroup(unsigned wg)
+{
+    g_wg = wg;
+    g_gen++;
+    g_epoch = 0;
+}
+
+void barrier_opened() { g_epoch++; }
 
 void access(const void *p, size_t n, int kind) { check((uintptr_t)p, n, kind); }
 

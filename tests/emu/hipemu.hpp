@@ -46,7 +46,12 @@
 #define __global__
 #define __device__
 #define __host__
+#ifdef HIPEMU_RACE
+// LDS arrays are gathered in one section, so that the race checker knows them
+#define __shared__ static __attribute__((section("emu_lds")))
+#else
 #define __shared__ static
+#endif
 #define __constant__ static
 #define __launch_bounds__(...)
 #define __align__(n) __attribute__((aligned(n)))

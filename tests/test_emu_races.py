@@ -8,7 +8,9 @@ the happens-before order of the stream operations: stream order, event
 record -> wait, host synchronisation and the null stream.  A read or write of
 device memory that conflicts with an unordered access of another operation,
 or with another workgroup's access in the same launch, fails the test
-(tests/conftest.py).  This covers what the issue-order emulation cannot: a
+(tests/conftest.py); so does an LDS access that conflicts with another wave's
+access in the same barrier interval, or reads LDS the workgroup has not
+written.  This covers what the issue-order emulation cannot: a
 missing wait, a workspace shared by two streams, a tile read by one stream
 while another rewrites it -- for every schedule the GPU test files exercise,
 including paired trailing updates and blocked correction sweeps.
@@ -107,7 +109,9 @@ def run(algo, drop, prec=None):
 for algo, prec, drops in (("cholesky", None, (3, 5)), ("lu", None, (3,)), ("qr", None, (17, 18)),
                           ("cholesky", 2, (6,))):
     print(algo, prec, run(algo, 0, prec), *[run(algo, d, prec) for d in drops], flush=True)
+L.hipemu_race_reset()
 assert L.hipemu_plant_wg_race() == 0
+assert L.hipemu_plant_lds_race() == 0
 buf = C.create_string_buffer(1 << 14)
 L.hipemu_race_report(buf, len(buf))
 print("WG", L.hipemu_races(), buf.value.decode().replace("\n", " | "))
@@ -120,7 +124,9 @@ def test_checker_reports_planted_races(race_lib):
     and QR factorizations and the BF16 Cholesky solve report racing accesses,
     and the same runs with every wait in place report none; a launch in which
     two workgroups store to one element, or one reads another's store, is
-    reported as a race between workgroups."""
+    reported as a race between workgroups; a wave that reads what another
+    wave stored to LDS with no barrier between, and a workgroup that reads
+    LDS it never stored to, are reported too."""
     r = subprocess.run([sys.executable, "-c", _PLANT, ROOT, race_lib], cwd=ROOT, env=_env(), capture_output=True,
                        text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -133,3 +139,5 @@ def test_checker_reports_planted_races(race_lib):
     wg = r.stdout[r.stdout.index("WG "):]
     assert "write over another workgroup's write in one launch" in wg, wg
     assert "read of another workgroup's write in one launch" in wg, wg
+    assert "LDS write over another wave's read with no barrier between" in wg, wg
+    assert "LDS read before the workgroup wrote it" in wg, wg
