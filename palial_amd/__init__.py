@@ -186,14 +186,18 @@ def _desc_of(A, N=None, b=None):
         return A, None, A
     if _is_torch(A):
         import torch
-        assert A.dtype == torch.float64 and A.is_contiguous()
+        if A.dtype != torch.float64 or not A.is_contiguous():
+            raise TypeError("the matrix must be a contiguous float64 tensor in tile layout L")
         N = N or int(round(A.numel() ** 0.5))
-        assert N * N == A.numel() and b
+        if N * N != A.numel() or not b:
+            raise ValueError(f"a tensor of {A.numel()} elements is not N x N = {N} x {N}, or no tile size given")
         return MATRIX_desc(b, N * N, b * b, N, A.data_ptr()), A.is_cuda, A
     a = A
-    assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("the matrix must be a C-contiguous float64 numpy array in tile layout L")
     N = N or int(round(a.size ** 0.5))
-    assert N * N == a.size and b
+    if N * N != a.size or not b:
+        raise ValueError(f"an array of {a.size} elements is not N x N = {N} x {N}, or no tile size given")
     return MATRIX_desc(b, N * N, b * b, N, a.ctypes.data), False, a
 
 

@@ -54,3 +54,19 @@ def test_driver_built():
     import palial_amd
     assert os.path.exists(palial_amd.DRIVER_PATH) or not os.path.exists(
         os.path.join(ROOT, "palial_amd", "driver", "palial.c"))
+
+
+def test_matrix_arguments_are_checked():
+    # the Python mirror refuses what the C ABI could not describe, before any call
+    import numpy as np
+    import palial_amd
+    with pytest.raises(TypeError):
+        palial_amd._desc_of(np.zeros(16, dtype=np.float32), b=2)
+    with pytest.raises(TypeError):
+        palial_amd._desc_of(np.zeros((8, 8))[:, ::2].copy(order="F")[:, :4], b=2)
+    with pytest.raises(ValueError):
+        palial_amd._desc_of(np.zeros(15), b=2)
+    with pytest.raises(ValueError):
+        palial_amd._desc_of(np.zeros(16))
+    d, dev, _ = palial_amd._desc_of(np.zeros(16), b=2)
+    assert (d.matrix_size, d.tile_size, d.matrix_nelements, d.tile_nelements, dev) == (4, 2, 16, 4, False)
