@@ -100,6 +100,8 @@ void begin_op(hipStream_t s, const void *kfn, const char *what);
 void end_op();
 void set_workgroup(unsigned wg);
 void barrier_opened();
+void wave_synced(int wave);
+void dma_issued(const void *lds, size_t n);
 void access(const void *p, size_t n, int kind);
 void *dev_alloc(size_t bytes);
 bool dev_free(void *p);
@@ -115,6 +117,8 @@ inline void begin_op(hipStream_t, const void *, const char *) {}
 inline void end_op() {}
 inline void set_workgroup(unsigned) {}
 inline void barrier_opened() {}
+inline void wave_synced(int) {}
+inline void dma_issued(const void *, size_t) {}
 inline void access(const void *, size_t, int) {}
 inline void record(hipEvent_t, hipStream_t) {}
 inline bool wait(hipStream_t, hipEvent_t) { return true; }
@@ -276,6 +280,7 @@ void run_block()
             if (waiting > 0) {
                 for (int i = w * 64; i < std::min(g_nt, w * 64 + 64); i++)
                     if (g_fib[i].state == WAVE) g_fib[i].state = RUN;
+                race::wave_synced(w);
                 released = true;
             }
         }
@@ -536,6 +541,7 @@ void load_lds(void *dst, const void *src, size_t n)
 {
     race::access(src, n, race::READ);
     race::access(dst, n, race::WRITE);       // the lane's store to LDS
+    race::dma_issued(dst, n);
 #ifdef EMU_ASAN
     const bool bad_src = __asan_region_is_poisoned(const_cast<void *>(src), n) != nullptr;
     const bool bad_dst = __asan_region_is_poisoned(dst, n) != nullptr;
@@ -839,9 +845,11 @@ long hipemu_ticks(void) { return (long)g_tick; }
 //     dynamic block): an access that conflicts with another wave's access
 //     in the same barrier interval is a race, and a read of a granule this
 //     workgroup has not written reads what an earlier workgroup left.
+//   * lanes of one wave that pass a value through memory with no wave
+//     operation between, in an order whose outcome differs between the
+//     lockstep device and the emulator's lane-by-lane execution (lane_check).
 // What it does not see: accesses wider than 16 bytes in one instruction (the
-// instrumentation skips them), conflicts between lanes of one wave (they run
-// in lockstep on the device), and ops inside a captured graph (replayed in
+// instrumentation skips them), and ops inside a captured graph (replayed in
 // capture order on one stream; the same schedule issued eagerly is checked).
 #ifdef HIPEMU_RACE
 namespace emu {
@@ -892,6 +900,58 @@ struct Ls {
 uint32_t g_gen = 0;                                   // workgroups started so far
 uint32_t g_epoch = 0;                                 // barriers opened in this workgroup
 std::vector<Ls> g_lds_static, g_lds_dyn;
+
+// Lanes of one wave between two of its synchronisation points (a wave
+// operation or a barrier): the emulator runs them one after the other (lane
+// 63 first), the device in lockstep, instruction by instruction.  When one
+// lane writes a location another lane of the wave reads in that interval,
+// the two agree if the read instruction comes first in program order and the
+// reader runs first here, or the write comes first and the writer runs
+// first.  The lanes' own sequences tell the order (control flow uniform
+// across the wave): a writer that read the location before writing it ran
+// the read instruction first.  So it is reported when
+//   * a lane reads what another lane wrote after reading it itself (the read
+//     instruction is the earlier one, and the device's reader sees the old
+//     value), or
+//   * a lane writes what another lane read without having read it itself
+//     (the write is the earlier instruction, and the device's reader sees
+//     the new value),
+// and when two lanes write one location.  Per wave, a table of the granules
+// touched in its current interval (tag), with the writing lane, whether it
+// had read the granule first, and the reading lanes.
+struct LaneRec {                                      // one 4-byte granule, per byte
+    uintptr_t key;
+    uint64_t tag;
+    uint64_t rmask[4];
+    int8_t wlane[4];
+    bool wread[4];
+};
+constexpr int LANE_TAB = 1 << 15;
+constexpr int MAX_WAVES = 16;
+std::vector<LaneRec> g_lane_tab;                      // MAX_WAVES tables of LANE_TAB records
+uint64_t g_wave_tag[MAX_WAVES];
+uint64_t g_tag_next = 1;
+long g_lane_overflow = 0;                             // granules not tracked (table full)
+
+void new_interval(int w) { g_wave_tag[w] = g_tag_next++; }
+
+// global_load_lds stores in flight: per lane, its DMA instructions in issue
+// order (LDS address, bytes); per LDS granule, whether a load to it is in
+// flight.  s_waitcnt vmcnt(n) of a lane retires its oldest loads until n
+// remain (other vector memory instructions are not counted, so a load may
+// be held in flight longer than on the device, never shorter); a barrier
+// retires nothing.
+std::vector<std::vector<std::pair<uintptr_t, size_t>>> g_dma(MAXT);
+std::map<uintptr_t, int> g_dma_live;                  // granule -> loads in flight to it
+size_t g_dma_lanes = 0;                               // lanes with loads in flight
+
+void dma_reset()
+{
+    if (!g_dma_lanes) return;
+    for (auto &q : g_dma) q.clear();
+    g_dma_live.clear();
+    g_dma_lanes = 0;
+}
 
 __attribute__((constructor)) void reserve()
 {
@@ -956,13 +1016,27 @@ void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1
                                         "LDS read of another wave's write with no barrier between",
                                         "LDS write over another wave's write with no barrier between",
                                         "LDS write over another wave's read with no barrier between",
-                                        "LDS read before the workgroup wrote it"};
+                                        "LDS read before the workgroup wrote it",
+                                        "read of what another lane of the wave wrote, with no wave operation between",
+                                        "write over what another lane of the wave read, with no wave operation between",
+                                        "two lanes of one wave write one location, with no wave operation between",
+                                        "LDS read of a global_load_lds destination before its s_waitcnt vmcnt"};
     g_count++;
     auto key = std::make_tuple(first, second, kind);
     if (g_seen.count(key) || g_seen.size() >= 64) return;
     g_seen[key] = 1;
-    std::string m = std::string(names[kind]) + " at " + (kind >= 7 ? std::string("an LDS byte") : where(a)) + ": ";
-    if (kind >= 7) {
+    const bool dev = a - g_base < DEV_SPAN;
+    std::string m = std::string(names[kind]) + " at " +
+                    (kind >= 7 && !dev ? std::string("an LDS byte") : where(a)) + ": ";
+    if (kind == 14) {
+        char buf[96];
+        snprintf(buf, sizeof buf, " workgroup %u, lane %d of the reading wave", g_wg, g_lane->lin);
+        m += op_name(second) + buf;
+    } else if (kind >= 11) {
+        char buf[96];
+        snprintf(buf, sizeof buf, " workgroup %u, wave %d, lanes %u and %u", g_wg, g_lane->lin >> 6, wg1, wg2);
+        m += op_name(second) + buf;
+    } else if (kind >= 7) {
         char buf[96];
         snprintf(buf, sizeof buf, " workgroup %u, waves %u and %u", g_wg, wg1, wg2);
         m += op_name(second) + buf;
@@ -981,6 +1055,62 @@ void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1
 extern "C" {
 extern char __start_emu_lds[] __attribute__((weak, visibility("hidden")));
 extern char __stop_emu_lds[] __attribute__((weak, visibility("hidden")));
+}
+
+// bytes [b0, b1] of the 4-byte granule at ga accessed by the running lane of a kernel
+void lane_check(uintptr_t ga, int b0, int b1, int kind)
+{
+    const int lane = g_lane->lin & 63, w = g_lane->lin >> 6;
+    if (w >= MAX_WAVES) return;
+    if (g_lane_tab.empty()) g_lane_tab.assign((size_t)MAX_WAVES * LANE_TAB, LaneRec{});
+    LaneRec *tab = g_lane_tab.data() + (size_t)w * LANE_TAB;
+    const uint64_t tag = g_wave_tag[w];
+    size_t h = (size_t)((ga >> 2) * 0x9E3779B97F4A7C15ull >> 49) & (LANE_TAB - 1);
+    LaneRec *r = nullptr;
+    for (int probe = 0; probe < 64; probe++, h = (h + 1) & (LANE_TAB - 1)) {
+        LaneRec &c = tab[h];
+        if (c.tag != tag) {                           // free in this interval
+            c.key = ga;
+            c.tag = tag;
+            for (int k = 0; k < 4; k++) {
+                c.rmask[k] = 0;
+                c.wlane[k] = -1;
+                c.wread[k] = false;
+            }
+            r = &c;
+            break;
+        }
+        if (c.key == ga) {
+            r = &c;
+            break;
+        }
+    }
+    if (!r) {
+        g_lane_overflow++;
+        return;
+    }
+    const uint64_t me = 1ull << lane;
+    for (int k = b0; k <= b1; k++) {
+        if (kind == READ) {
+            if (r->wlane[k] >= 0 && r->wlane[k] != lane && r->wread[k])
+                report(11, 0, g_cur, ga + k, (unsigned)r->wlane[k], (unsigned)lane);
+            r->rmask[k] |= me;
+        } else {
+            if ((r->rmask[k] & ~me) && !(r->rmask[k] & me))
+                report(12, 0, g_cur, ga + k, (unsigned)__builtin_ctzll(r->rmask[k] & ~me), (unsigned)lane);
+            if (r->wlane[k] >= 0 && r->wlane[k] != lane && kind != ATOMIC)
+                report(13, 0, g_cur, ga + k, (unsigned)r->wlane[k], (unsigned)lane);
+            r->wlane[k] = (int8_t)lane;
+            r->wread[k] = (r->rmask[k] & me) != 0;
+        }
+    }
+}
+
+// the bytes of [a, a + n) inside the granule at ga
+inline void lane_bytes(uintptr_t a, size_t n, uintptr_t ga, int &b0, int &b1)
+{
+    b0 = a > ga ? (int)(a - ga) : 0;
+    b1 = a + n < ga + 4 ? (int)(a + n - 1 - ga) : 3;
 }
 
 // an access of the running lane to LDS (a __shared__ array or the launch's
@@ -1005,8 +1135,12 @@ void lds_check(uintptr_t a, size_t n, int kind)
     const size_t cnt = (a + n - 1) / 4 - a / 4 + 1;
     for (size_t i = 0; i < cnt; i++) {
         Ls &l = sh[i];
-        const uintptr_t ga = a + 4 * i;
+        const uintptr_t ga = (a & ~(uintptr_t)3) + 4 * i;
+        int b0, b1;
+        lane_bytes(a, n, ga, b0, b1);
+        lane_check(ga, b0, b1, kind);
         if (kind == READ) {
+            if (!g_dma_live.empty() && g_dma_live.count(ga)) report(14, 0, g_cur, ga, 0, 0);
             if (l.wgen != g_gen) report(10, 0, g_cur, ga, w, w);
             else if (l.wep == g_epoch && l.wwave != w) report(7, 0, g_cur, ga, l.wwave, w);
             if (l.rgen == g_gen && l.rep == g_epoch) {
@@ -1045,6 +1179,11 @@ void check(uintptr_t a, size_t n, int kind)
     for (size_t i = i0; i <= i1; i++) {
         Gr &g = g_shadow[i];
         const uintptr_t ga = g_base + 4 * i;
+        if (g_lane) {
+            int b0, b1;
+            lane_bytes(a, n, ga, b0, b1);
+            lane_check(ga, b0, b1, kind);
+        }
         if (kind == READ) {
             if (g.w && g.w != me && !hb(g.w, me)) report(0, g.w, me, ga, 0, 0);
             else if (g.w == me && g.wwg != wg && g.wwg != ATOMIC_WG) report(4, me, me, ga, g.wwg, wg);
@@ -1154,9 +1293,31 @@ void set_workgroup(unsigned wg)
     g_wg = wg;
     g_gen++;
     g_epoch = 0;
+    dma_reset();
+    for (int w = 0; w < MAX_WAVES; w++) new_interval(w);
 }
 
-void barrier_opened() { g_epoch++; }
+void barrier_opened()
+{
+    g_epoch++;
+    for (int w = 0; w < MAX_WAVES; w++) new_interval(w);
+}
+
+void wave_synced(int wave)
+{
+    if (wave < MAX_WAVES) new_interval(wave);
+}
+
+void dma_issued(const void *lds, size_t n)
+{
+    if (!g_lane || !n) return;
+    auto &q = g_dma[g_lane->lin];
+    if (q.empty()) g_dma_lanes++;
+    q.push_back({(uintptr_t)lds, n});
+    const uintptr_t a = (uintptr_t)lds & ~(uintptr_t)3;
+    for (uintptr_t ga = a; ga < (uintptr_t)lds + n; ga += 4) g_dma_live[ga]++;
+}
+
 
 void access(const void *p, size_t n, int kind) { check((uintptr_t)p, n, kind); }
 
@@ -1192,6 +1353,26 @@ void host_sync_device()
 }  // namespace race
 
 void race_atomic(const void *p, size_t n) { race::check((uintptr_t)p, n, race::ATOMIC); }
+
+// s_waitcnt of the running lane: vmcnt is bits [3:0] and [15:14] of the
+// immediate (gfx9 encoding)
+void waitcnt(int imm)
+{
+    if (!g_lane) return;
+    const size_t vm = (size_t)((imm & 0xF) | (((imm >> 14) & 3) << 4));
+    auto &q = race::g_dma[g_lane->lin];
+    if (q.size() <= vm) return;
+    const size_t drop = q.size() - vm;
+    for (size_t i = 0; i < drop; i++) {
+        const uintptr_t a = q[i].first & ~(uintptr_t)3;
+        for (uintptr_t ga = a; ga < q[i].first + q[i].second; ga += 4) {
+            auto it = race::g_dma_live.find(ga);
+            if (it != race::g_dma_live.end() && --it->second == 0) race::g_dma_live.erase(it);
+        }
+    }
+    q.erase(q.begin(), q.begin() + (long)drop);
+    if (q.empty()) race::g_dma_lanes--;
+}
 
 }  // namespace emu
 
@@ -1286,5 +1467,7 @@ void hipemu_race_reset(void)
 // the n-th hipStreamWaitEvent from now is skipped (0: none)
 void hipemu_drop_wait(int n) { emu::race::g_drop_wait = n; }
 long hipemu_ops(void) { return (long)emu::race::g_ops.size(); }
+// granules the lane-order check could not track (a wave's table was full)
+long hipemu_lane_overflow(void) { return emu::race::g_lane_overflow; }
 }
 #endif

@@ -13,7 +13,8 @@
 //     lanes of the 64-lane wave deposit their operands, then each computes its
 //     own result from the deposits;
 //   * global_load_lds: the lane's 16 bytes land at the LDS address plus 16x
-//     the lane index, at once (so a missing s_waitcnt is not detected);
+//     the lane index, at once (the race-checking build tracks them as in
+//     flight until the lane's s_waitcnt vmcnt retires them);
 //   * MFMA: the CDNA3/CDNA4 operand layouts -- lane l supplies row l % 16 of
 //     A (column l % 16 of B) at K block l / 16 and holds column l % 16 of D,
 //     rows 4 (l / 16) + v for FP32 accumulators, l / 16 + 4 v for FP64;
@@ -26,7 +27,10 @@
 // so an AddressSanitizer build reports any access outside them.  The
 // race-checking build (HIPEMU_RACE) instead checks every access to device
 // memory against the happens-before order of the stream operations, which
-// issue order is only one instance of (hipemu.cpp, end of file).
+// issue order is only one instance of, and reports LDS races between waves
+// and lane-order dependences inside a wave -- the one place where the
+// lane-by-lane execution above could compute what the device does not
+// (hipemu.cpp, end of file).
 #pragma once
 
 #include <algorithm>
@@ -274,7 +278,12 @@ inline void emu_launch(const char *name, void (*k)(P...), dim3 g, dim3 b, size_t
 // ------------------------------------------------------------------ device functions
 inline void __syncthreads() { emu::barrier(); }
 #define __builtin_amdgcn_s_barrier() emu::barrier()
+#ifdef HIPEMU_RACE
+namespace emu { void waitcnt(int imm); }
+#define __builtin_amdgcn_s_waitcnt(n) emu::waitcnt(n)
+#else
 #define __builtin_amdgcn_s_waitcnt(n) ((void)(n))
+#endif
 #define __builtin_amdgcn_rsq(x) (1.0 / std::sqrt((double)(x)))
 
 template <class T>

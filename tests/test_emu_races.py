@@ -9,8 +9,11 @@ record -> wait, host synchronisation and the null stream.  A read or write of
 device memory that conflicts with an unordered access of another operation,
 or with another workgroup's access in the same launch, fails the test
 (tests/conftest.py); so does an LDS access that conflicts with another wave's
-access in the same barrier interval, or reads LDS the workgroup has not
-written.  This covers what the issue-order emulation cannot: a
+access in the same barrier interval, a read of LDS the workgroup has not
+written, a value passed between lanes of one wave with no wave operation
+between whose outcome depends on the emulator's lane order (the device runs
+the wave in lockstep), and a read of LDS that a global_load_lds still in
+flight is filling (s_waitcnt vmcnt not yet reached).  This covers what the issue-order emulation cannot: a
 missing wait, a workspace shared by two streams, a tile read by one stream
 while another rewrites it -- for every schedule the GPU test files exercise,
 including paired trailing updates and blocked correction sweeps.
@@ -47,7 +50,17 @@ GROUPS = {
     "batched_and_tiles": (["tests/test_gpu_batched.py::test_batched_single_kind"],
                           (["tests/test_gpu_batched.py", "tests/test_gpu_update.py", "tests/test_zz_no_alloc.py"],
                            None)),
+    # the same schedules on other streams: the diagonal chain on its own CUs
+    # with paired updates, and per-task-class CU policies
+    "chain_cus": (["tests/test_gpu_solve.py::test_solve_diag_dominant[512-128-1-2-cholesky]",
+                   "tests/wip_gpu_pair.py::test_paired_updates_match[1280-128-0-lu-2]"],
+                  (["tests/test_gpu_solve.py", "tests/wip_gpu_pair.py"], "512-128 or 1280")),
+    "cu_policy": (["tests/test_gpu_solve.py::test_solve_diag_dominant[512-128-1-2-lu]"],
+                  (["tests/test_gpu_solve.py"], "512-128")),
 }
+# environment of a group's process (the options are read when the library starts)
+GROUP_ENV = {"chain_cus": {"PALIAL_CHAIN_CUS": "1"},
+             "cu_policy": {"PALIAL_CU_POLICY": "bulk=8-255,aux=8-71,chain=0-7"}}
 
 
 @pytest.fixture(scope="module")
@@ -57,28 +70,38 @@ def race_lib():
     return emu_build.build("race")
 
 
-def _env():
-    return dict(os.environ, PALIAL_HIPEMU="race")
+def _env(extra=None):
+    return dict(os.environ, PALIAL_HIPEMU="race", **(extra or {}))
 
 
-def _start(targets, kexpr=None):
+def _start(targets, kexpr=None, env=None):
     cmd = [sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "gpu", *targets]
     if kexpr:
         cmd += ["-k", kexpr]
-    return subprocess.Popen(cmd, cwd=ROOT, env=_env(), stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    return subprocess.Popen(cmd, cwd=ROOT, env=_env(env), stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                            text=True)
 
 
 def test_schedules_race_free(race_lib):
     """Every group's GPU tests pass on the race-checking build: their results
-    meet the GPU bars and no access races (the groups run side by side)."""
+    meet the GPU bars and no access races.  Beside them, tests/emu/fuzz.py
+    runs seeded random factorizations and solves (every algorithm, schedule,
+    precision tier, sweep block and pairing setting, tile sizes 32 to 256)
+    with the same checks.  Everything runs side by side, one process each."""
     procs = {}
     for name, (subset, (targets, kexpr)) in GROUPS.items():
-        procs[name] = _start(targets, kexpr) if FULL else _start(subset)
+        env = GROUP_ENV.get(name)
+        procs[name] = _start(targets, kexpr, env) if FULL else _start(subset, env=env)
+    fuzz = subprocess.Popen([sys.executable, os.path.join(HERE, "emu", "fuzz.py"), "7", "150" if FULL else "12"],
+                            cwd=ROOT, env=_env(), stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     bad = {}
     for name, p in procs.items():
         out, err = p.communicate(timeout=4 * 3600 if FULL else 1200)
         if p.returncode != 0 or " passed" not in out or "failed" in out or "error" in out:
             bad[name] = out[-3000:] + err[-1000:]
+    out, err = fuzz.communicate(timeout=4 * 3600 if FULL else 1200)
+    if fuzz.returncode != 0 or "fails 0" not in out:
+        bad["fuzz"] = "\n".join(l for l in out.splitlines() if not l.startswith("ok")) + err[-2000:]
     assert not bad, bad
 
 
@@ -125,8 +148,10 @@ def test_checker_reports_planted_races(race_lib):
     and the same runs with every wait in place report none; a launch in which
     two workgroups store to one element, or one reads another's store, is
     reported as a race between workgroups; a wave that reads what another
-    wave stored to LDS with no barrier between, and a workgroup that reads
-    LDS it never stored to, are reported too."""
+    wave stored to LDS with no barrier between, a workgroup that reads LDS it
+    never stored to, lanes of one wave that exchange values with no wave
+    operation between in an order the emulator does not reproduce, and a read
+    of a global_load_lds destination before its s_waitcnt, are reported too."""
     r = subprocess.run([sys.executable, "-c", _PLANT, ROOT, race_lib], cwd=ROOT, env=_env(), capture_output=True,
                        text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -141,3 +166,6 @@ def test_checker_reports_planted_races(race_lib):
     assert "read of another workgroup's write in one launch" in wg, wg
     assert "LDS write over another wave's read with no barrier between" in wg, wg
     assert "LDS read before the workgroup wrote it" in wg, wg
+    assert "read of what another lane of the wave wrote, with no wave operation between" in wg, wg
+    assert "write over what another lane of the wave read, with no wave operation between" in wg, wg
+    assert "LDS read of a global_load_lds destination before its s_waitcnt vmcnt" in wg, wg
