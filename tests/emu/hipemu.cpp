@@ -1354,6 +1354,23 @@ void host_sync_device()
 
 void race_atomic(const void *p, size_t n) { race::check((uintptr_t)p, n, race::ATOMIC); }
 
+namespace race {
+// a kernel's access of n >= 4 bytes, taken as aligned by the compiler, to
+// device memory or LDS: the address must be a multiple of n.  Such an access
+// cannot run here either (an aligned vector instruction faults), so the run
+// ends with the report.
+void aligned(uintptr_t a, size_t n)
+{
+    if (n < 4 || !g_cur || !g_lane || (a & (n - 1)) == 0) return;
+    const uintptr_t s0 = (uintptr_t)__start_emu_lds, s1 = (uintptr_t)__stop_emu_lds;
+    const uintptr_t d0 = (uintptr_t)g_dyn_lds;
+    const bool lds = (s0 && a >= s0 && a < s1) || (d0 && a >= d0 && a < d0 + g_dyn_bytes);
+    if (a - g_base >= DEV_SPAN && !lds) return;
+    fatal("vector access at an address not aligned to its size: %zu bytes at %s %p in %s, workgroup %u, lane %d",
+          n, lds ? "LDS" : "device", (void *)a, op_name(g_cur).c_str(), g_wg, g_lane->lin);
+}
+}  // namespace race
+
 // s_waitcnt of the running lane: vmcnt is bits [3:0] and [15:14] of the
 // immediate (gfx9 encoding)
 void waitcnt(int imm)
@@ -1383,9 +1400,20 @@ void __tsan_func_entry(void *) {}
 void __tsan_func_exit() {}
 void __tsan_vptr_update(void **, void *) {}
 void __tsan_vptr_read(void **) {}
+// the aligned forms: the compiler took the address as aligned to the access
+// size (at least 8 bytes for the 16-byte forms), so an address that is not
+// is a misaligned vector access on the device
 #define EMU_TSAN_RW(N)                                                                            \
-    void __tsan_read##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::READ); }          \
-    void __tsan_write##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::WRITE); }        \
+    void __tsan_read##N(void *p)                                                                  \
+    {                                                                                             \
+        emu::race::aligned((uintptr_t)p, N);                                                      \
+        emu::race::check((uintptr_t)p, N, emu::race::READ);                                       \
+    }                                                                                             \
+    void __tsan_write##N(void *p)                                                                 \
+    {                                                                                             \
+        emu::race::aligned((uintptr_t)p, N);                                                      \
+        emu::race::check((uintptr_t)p, N, emu::race::WRITE);                                      \
+    }                                                                                             \
     void __tsan_unaligned_read##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::READ); } \
     void __tsan_unaligned_write##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::WRITE); } \
     void __tsan_volatile_read##N(void *p) { emu::race::check((uintptr_t)p, N, emu::race::READ); }  \

@@ -66,12 +66,15 @@ struct dim3 {
     constexpr dim3(unsigned x_ = 1, unsigned y_ = 1, unsigned z_ = 1) : x(x_), y(y_), z(z_) {}
 };
 
-struct alignas(8) float2 { float x, y; };
-struct alignas(16) float4 { float x, y, z, w; };
-struct alignas(16) double2 { double x, y; };
-struct alignas(8) int2 { int x, y; };
-struct alignas(16) int4 { int x, y, z, w; };
-struct alignas(16) uint4 { unsigned x, y, z, w; };
+// HIP's vector types as clang vector types, as on the device: one access of
+// the whole vector (and the type's natural alignment), .x .y .z .w members
+typedef float float2 __attribute__((ext_vector_type(2)));
+typedef float float4 __attribute__((ext_vector_type(4)));
+typedef double double2 __attribute__((ext_vector_type(2)));
+typedef int int2 __attribute__((ext_vector_type(2)));
+typedef int int4 __attribute__((ext_vector_type(4)));
+typedef unsigned uint4 __attribute__((ext_vector_type(4)));
+// (ushort4 stays a structure: the library binds references to its members)
 struct alignas(8) ushort4 { unsigned short x, y, z, w; };
 inline float2 make_float2(float x, float y) { return {x, y}; }
 inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }

@@ -12,8 +12,9 @@ or with another workgroup's access in the same launch, fails the test
 access in the same barrier interval, a read of LDS the workgroup has not
 written, a value passed between lanes of one wave with no wave operation
 between whose outcome depends on the emulator's lane order (the device runs
-the wave in lockstep), and a read of LDS that a global_load_lds still in
-flight is filling (s_waitcnt vmcnt not yet reached).  This covers what the issue-order emulation cannot: a
+the wave in lockstep), a read of LDS that a global_load_lds still in flight
+is filling (s_waitcnt vmcnt not yet reached), and a vector access whose
+address is not a multiple of its size.  This covers what the issue-order emulation cannot: a
 missing wait, a workspace shared by two streams, a tile read by one stream
 while another rewrites it -- for every schedule the GPU test files exercise,
 including paired trailing updates and blocked correction sweeps.
@@ -150,8 +151,9 @@ def test_checker_reports_planted_races(race_lib):
     reported as a race between workgroups; a wave that reads what another
     wave stored to LDS with no barrier between, a workgroup that reads LDS it
     never stored to, lanes of one wave that exchange values with no wave
-    operation between in an order the emulator does not reproduce, and a read
-    of a global_load_lds destination before its s_waitcnt, are reported too."""
+    operation between in an order the emulator does not reproduce, a read of
+    a global_load_lds destination before its s_waitcnt, and a 16-byte LDS load
+    at an address that is a multiple of 8 only, are reported too."""
     r = subprocess.run([sys.executable, "-c", _PLANT, ROOT, race_lib], cwd=ROOT, env=_env(), capture_output=True,
                        text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -169,3 +171,9 @@ def test_checker_reports_planted_races(race_lib):
     assert "read of what another lane of the wave wrote, with no wave operation between" in wg, wg
     assert "write over what another lane of the wave read, with no wave operation between" in wg, wg
     assert "LDS read of a global_load_lds destination before its s_waitcnt vmcnt" in wg, wg
+    # a misaligned vector access ends the run with its report
+    code = "import ctypes as C, sys\nprint(C.CDLL(sys.argv[1]).hipemu_plant_misaligned())\n"
+    r = subprocess.run([sys.executable, "-c", code, race_lib], cwd=ROOT, env=_env(), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode != 0 and "vector access at an address not aligned to its size: 16 bytes at LDS" in r.stderr, \
+        r.stderr[-2000:]
