@@ -171,13 +171,21 @@ int ensure_events(size_t n)
     return 0;
 }
 
-void drop_graphs()
+// captured factorizations kept at most (each holds its schedule's launches)
+constexpr size_t MAX_GRAPHS = 16;
+
+void drop_factor_graphs()
 {
     for (auto &kv : g.graphs) {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
     g.graphs.clear();
+}
+
+void drop_graphs()
+{
+    drop_factor_graphs();
     if (g.bgraph.exec) (void)hipGraphExecDestroy(g.bgraph.exec);
     if (g.bgraph.graph) (void)hipGraphDestroy(g.bgraph.graph);
     g.bgraph = BatchGraph();
@@ -2269,15 +2277,23 @@ int run_algo(int algo, MATRIX_desc A, MATRIX_desc *S, const palial_opts *o_in, p
             issue(cs);
             e = hipStreamEndCapture(cs, &G.graph);
             if (e != hipSuccess || L.err != hipSuccess) {
+                if (G.graph) (void)hipGraphDestroy(G.graph);
                 cleanup();
                 return fail(PALIAL_ERR_HIP, "graph capture failed: %s / %s", hipGetErrorString(e),
                             hipGetErrorString(L.err));
             }
             e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) { cleanup(); return fail(PALIAL_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(G.graph);
+                cleanup();
+                return fail(PALIAL_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+            }
             size_t n = 0;
             if (hipGraphGetNodes(G.graph, nullptr, &n) != hipSuccess) n = 0;
             G.nodes = (long)n;
+            // the key holds the device address: calls on host buffers stage A
+            // in a fresh allocation each time, so the cache is bounded
+            if (g.graphs.size() >= MAX_GRAPHS) drop_factor_graphs();
             g.graphs[key] = G;
             it = g.graphs.find(key);
         }

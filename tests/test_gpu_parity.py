@@ -264,3 +264,17 @@ def test_schedules_bitwise(hip, orc, algo):
     for group in sols.values():
         for i in range(1, len(group)):
             assert np.array_equal(group[i].view(np.uint64), group[0].view(np.uint64)), i
+
+
+def test_graph_cache_bounded(hip, orc):
+    """More captured factorizations than the library keeps (16): the cache
+    is emptied and refilled, and every replay still gives the oracle's
+    factor (host buffers, so each call captures for its own shape)."""
+    shapes = [(n * b, b) for b in (32, 64) for n in range(1, 11)]
+    for N, b in shapes:
+        a = orc.gen("sym", N, b, seed=N + b, shift=N)
+        for _ in range(2):                      # capture, then replay
+            x = a.copy()
+            info, _ = hip.cholesky(x, b=b)
+            ref, _ = orc.cholesky(a, N, b)
+            assert info == 0 and rel(x, ref) < TOL, (N, b)
