@@ -876,6 +876,10 @@ struct Op {
     const void *kfn;
     const char *what;
     int sid;
+    // device-memory traffic of the op: bytes its accesses asked for (reads,
+    // writes), and its footprint: the distinct 4-byte granules it read
+    // (first access a read) and wrote without reading them first
+    uint64_t rd = 0, wr = 0, fp_rd = 0, fp_wr = 0;
 };
 std::vector<Op> g_ops;                                // op id i + 1
 uint32_t g_cur = 0;                                   // op running now (0: host code)
@@ -1175,10 +1179,13 @@ void check(uintptr_t a, size_t n, int kind)
         return;
     }
     const uint32_t me = g_cur, wg = kind == ATOMIC ? ATOMIC_WG : g_wg;
+    Op &op = g_ops[me - 1];
+    (kind == READ ? op.rd : op.wr) += n;
     const size_t i0 = (a - g_base) / 4, i1 = (a + n - 1 - g_base) / 4;
     for (size_t i = i0; i <= i1; i++) {
         Gr &g = g_shadow[i];
         const uintptr_t ga = g_base + 4 * i;
+        if (g.w != me && g.r[0] != me && g.r[1] != me) (kind == READ ? op.fp_rd : op.fp_wr) += 4;
         if (g_lane) {
             int b0, b1;
             lane_bytes(a, n, ga, b0, b1);
@@ -1495,6 +1502,19 @@ void hipemu_race_reset(void)
 // the n-th hipStreamWaitEvent from now is skipped (0: none)
 void hipemu_drop_wait(int n) { emu::race::g_drop_wait = n; }
 long hipemu_ops(void) { return (long)emu::race::g_ops.size(); }
+// op i (1-based): its name and device-memory traffic (bytes read, written;
+// footprint read, written), -1 for no such op
+long hipemu_op_stats(long i, double *out, char *name, long cap)
+{
+    if (i < 1 || i > (long)emu::race::g_ops.size()) return -1;
+    const auto &o = emu::race::g_ops[i - 1];
+    out[0] = (double)o.rd;
+    out[1] = (double)o.wr;
+    out[2] = (double)o.fp_rd;
+    out[3] = (double)o.fp_wr;
+    if (name && cap > 0) snprintf(name, (size_t)cap, "%s", emu::race::op_name((uint32_t)i).c_str());
+    return 0;
+}
 // granules the lane-order check could not track (a wave's table was full)
 long hipemu_lane_overflow(void) { return emu::race::g_lane_overflow; }
 }

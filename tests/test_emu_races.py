@@ -177,3 +177,66 @@ def test_checker_reports_planted_races(race_lib):
                        timeout=300)
     assert r.returncode != 0 and "vector access at an address not aligned to its size: 16 bytes at LDS" in r.stderr, \
         r.stderr[-2000:]
+
+
+_TRAFFIC = r"""
+import ctypes as C, json, sys
+sys.path.insert(0, sys.argv[1])
+import palial_amd as P
+P.LIB_PATH = sys.argv[2]
+P.init(0)
+L = P.lib()
+L.hipemu_ops.restype = C.c_long
+L.palial_hip_update.argtypes = [C.c_int, P.MATRIX_desc, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+N, b = 1024, 256
+A = P.DeviceMatrix(N, b)
+A.generate(P.GEN_SYM, seed=1, shift=float(N))
+for algo in (0, 1):
+    for prec in (1, 2, 3, 4):
+        n0 = L.hipemu_ops()
+        assert L.palial_hip_update(algo, A.desc, 0, prec | 8, 1, C.byref(C.c_double())) == 0
+        out, name = (C.c_double * 4)(), C.create_string_buffer(256)
+        for i in range(n0 + 1, L.hipemu_ops() + 1):
+            L.hipemu_op_stats(i, out, name, 256)
+            if name.value.startswith(b"(gemm_"):
+                print(json.dumps({"algo": algo, "prec": prec, "kernel": name.value.decode(),
+                                  "rd": out[0], "wr": out[1], "fp_rd": out[2], "fp_wr": out[3]}))
+"""
+
+
+def test_packed_update_moves_the_algorithmic_bytes(race_lib):
+    """The byte count that the roofline of the packed trailing update uses
+    (each FP32 C tile read and written once, each packed panel tile read
+    once) is what the kernel's own loads and stores touch.  One update of
+    step 0 at N=1024, b=256 (3 trailing tile rows) per tier and algorithm,
+    traffic counted by the race build:
+      * every C element read is written exactly once, and nothing is
+        written that was not read first;
+      * the rest of the read footprint is the packed panels: T tiles of
+        b^2 elements at 4 (FP32), 2 (BF16), 1 (FP8, plus one 4-byte scale
+        exponent per tile) or 4 (split BF16: hi and lo) bytes, twice for LU
+        (row and column panel);
+      * Cholesky writes the whole GEMM tiles and, in its diagonal (SYRK)
+        tiles, at most their upper 128-blocks -- never more than the whole
+        tiles the roofline counts."""
+    import json
+    r = subprocess.run([sys.executable, "-c", _TRAFFIC, ROOT, race_lib], cwd=ROOT, env=_env(), capture_output=True,
+                       text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    rows = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    assert len(rows) == 8, r.stdout
+    b, T = 256, 3
+    tile = 4 * b * b                               # an FP32 C tile
+    esz = {1: 4, 2: 2, 3: 1, 4: 4}
+    for x in rows:
+        panels = (1 if x["algo"] == 0 else 2) * T * (esz[x["prec"]] * b * b + (4 if x["prec"] == 3 else 0))
+        assert x["fp_wr"] == 0, x
+        assert x["fp_rd"] - panels == x["wr"], x
+        if x["algo"] == 1:
+            assert x["wr"] == T * T * tile, x
+        else:
+            gemm, syrk = T * (T - 1) // 2, T
+            nb = b // 128
+            upper = gemm * tile + syrk * tile * (nb * (nb + 1) // 2) // (nb * nb)
+            assert gemm * tile + syrk * tile // 2 <= x["wr"] <= upper, x
+        assert x["rd"] > x["fp_rd"]                   # panel slices are read again by other blocks
