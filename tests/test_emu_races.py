@@ -240,3 +240,44 @@ def test_packed_update_moves_the_algorithmic_bytes(race_lib):
             upper = gemm * tile + syrk * tile * (nb * (nb + 1) // 2) // (nb * nb)
             assert gemm * tile + syrk * tile // 2 <= x["wr"] <= upper, x
         assert x["rd"] > x["fp_rd"]                   # panel slices are read again by other blocks
+
+
+_SWEEPS = r"""
+import ctypes as C, json, sys
+L = C.CDLL(sys.argv[1])
+L.hipemu_ops.restype = C.c_long
+for algo in (0, 1):
+    for f32 in (0, 1):
+        for q, nrhs in ((1, 1), (2, 1), (1, 3)):
+            n0 = L.hipemu_ops()
+            assert L.hipemu_sweep_check(algo, 1024, 128, q, nrhs, f32, 0) == 0
+            tot = [0.0, 0.0]
+            out, name = (C.c_double * 4)(), C.create_string_buffer(256)
+            for i in range(n0 + 1, L.hipemu_ops() + 1):
+                L.hipemu_op_stats(i, out, name, 256)
+                tot[0] += out[0]
+                tot[1] += out[2]
+            print(json.dumps({"algo": algo, "f32": f32, "q": q, "nrhs": nrhs, "rd": tot[0], "fp_rd": tot[1]}))
+"""
+
+
+def test_correction_sweeps_read_the_factor_once(race_lib):
+    """The bandwidth floor of a refinement step's two triangular sweeps is
+    one read of each factor triangle (tile inverses in place of the
+    diagonal tiles).  On the CPU build, the sweep launches of one correction
+    (N=1024, b=128; Cholesky and LU; FP64 and FP32 factors; one-tile and
+    two-tile steps; 1 and 3 right-hand sides) touch those bytes plus the
+    vectors, and ask for little more: the factor is not read twice."""
+    import json
+    r = subprocess.run([sys.executable, "-c", _SWEEPS, race_lib], cwd=ROOT, env=_env(), capture_output=True,
+                       text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    rows = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    assert len(rows) == 12, r.stdout
+    N, b = 1024, 128
+    MT = N // b
+    for x in rows:
+        tri = MT * (MT + 1) // 2 * b * b * (4 if x["f32"] else 8)    # one triangle of tiles
+        vec = 3 * N * x["nrhs"] * 8                                     # r, y and x
+        assert 2 * tri <= x["fp_rd"] <= 2 * tri + 4 * vec, x
+        assert x["rd"] <= 1.3 * x["fp_rd"], x
