@@ -281,3 +281,48 @@ def test_correction_sweeps_read_the_factor_once(race_lib):
         vec = 3 * N * x["nrhs"] * 8                                     # r, y and x
         assert 2 * tri <= x["fp_rd"] <= 2 * tri + 4 * vec, x
         assert x["rd"] <= 1.3 * x["fp_rd"], x
+
+
+_RESIDUAL = r"""
+import ctypes as C, json, sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import palial_amd as P
+import oracle
+P.LIB_PATH = sys.argv[2]
+P.init(0)
+L = P.lib()
+L.hipemu_ops.restype = C.c_long
+N, b = 1024, 128
+for algo in ("cholesky", "lu"):
+    a = oracle.gen("sym" if algo == "cholesky" else "dense", N, b, seed=1, shift=N)
+    n0 = L.hipemu_ops()
+    X, rep, rc = P.solve(a, np.ones((N, 1)), algo=algo, b=b, factor_tile=b)
+    out, name = (C.c_double * 4)(), C.create_string_buffer(256)
+    for i in range(n0 + 1, L.hipemu_ops() + 1):
+        L.hipemu_op_stats(i, out, name, 256)
+        if "residual_sym_kernel" in name.value.decode() or "residual_partial_kernel" in name.value.decode():
+            print(json.dumps({"algo": algo, "rd": out[0], "fp_rd": out[2]}))
+"""
+
+
+def test_residual_reads_the_matrix_once(race_lib):
+    """The FP64 residual of a refinement step reads A once: the upper tiles
+    for Cholesky (the symmetric matrix the factorization sees, 4 N^2 bytes),
+    every tile for LU (8 N^2 bytes), plus the solution vector -- the HBM
+    figure the refinement's time model uses.  Counted on the CPU build for
+    the first residual of an FP64 solve at N=1024, b=128."""
+    import json
+    r = subprocess.run([sys.executable, "-c", _RESIDUAL, ROOT, race_lib], cwd=ROOT, env=_env(), capture_output=True,
+                       text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    rows = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    N, b = 1024, 128
+    MT = N // b
+    seen = set()
+    for x in rows:
+        mat = (MT * (MT + 1) // 2 * b * b if x["algo"] == "cholesky" else N * N) * 8
+        assert mat <= x["fp_rd"] <= mat + 4 * N * 8, x
+        assert x["rd"] <= 1.1 * x["fp_rd"], x
+        seen.add(x["algo"])
+    assert seen == {"cholesky", "lu"}, r.stdout
