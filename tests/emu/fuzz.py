@@ -1,11 +1,13 @@
 """Random configurations of the library's API on the CPU build of its HIP
-sources (tests/emu): factorizations of every algorithm and schedule, and
-solves of every precision tier, sweep block and paired-update setting, on
+sources (tests/emu): factorizations of every algorithm and schedule, solves
+of every precision tier, sweep block and paired-update setting, batched
+factorizations, and QR least-squares solves and Q formation, on
 small matrices with tile sizes from 32 to 256 (solves: multiples of 64, as
-palial_hip_solve requires).  Each result is checked -- factors
-against the oracle to 1e-10, solves to backward error 2e-14 recomputed with
-numpy, or a refusal with PALIAL_ERR_ARG -- and, on the race-checking variant,
-every access against the stream order.
+palial_hip_solve requires).  Each result is checked -- factors against the
+oracle to 1e-10, solves to backward error 2e-14 recomputed with numpy (or a
+refusal with PALIAL_ERR_ARG), batched factors against numpy Cholesky and a
+Doolittle elimination, QR by its residuals and numpy's solution -- and, on
+the race-checking variant, every access against the stream order.
 
     python tests/emu/fuzz.py SEED COUNT        (FUZZ_VARIANT=race|asan|plain)
 
@@ -92,6 +94,66 @@ def solve(rng, it, N, b, MT):
                 f"used={rep.precision_used}")
 
 
+def doolittle(a):
+    """No-pivot LU (L unit lower strictly below, U upper), in a copy."""
+    a = a.copy()
+    for k in range(a.shape[0] - 1):
+        a[k + 1:, k] /= a[k, k]
+        a[k + 1:, k + 1:] -= np.outer(a[k + 1:, k], a[k, k + 1:])
+    return a
+
+
+def batched(rng, it):
+    count = rng.randint(1, 12)
+    n = np.array([rng.choice([1, 2, 17, 32, 33, 64, 100, 129, 200]) for _ in range(count)], dtype=np.int32)
+    kind = np.array([rng.randint(0, 1) for _ in range(count)], dtype=np.int32)
+    prec = rng.choice([P.PREC_FP64, P.PREC_FP32])
+    grid, no_graph = rng.choice([0, 1, 3, 8]), rng.random() < 0.5
+    bt = P.Batch(n, kind, precision=prec)
+    try:
+        bt.generate(it)
+        mats0, _ = bt.get()
+        rhs = [np.random.default_rng(it + i).standard_normal(m) for i, m in enumerate(n)]
+        bt.set(rhs=rhs)
+        bt.factor(solve=True, no_graph=no_graph, grid=grid)
+        mats, xs = bt.get()
+    finally:
+        bt.free()
+    tol = 1e-12 if prec == P.PREC_FP64 else 2e-5
+    worst = 0.0
+    for a0, f, x, b, k in zip(mats0, mats, xs, rhs, kind):
+        a0 = a0.astype(np.float64)
+        ref = np.linalg.cholesky(a0).T if k == 0 else doolittle(a0)
+        got = np.triu(f.astype(np.float64)) if k == 0 else f.astype(np.float64)
+        xe = np.linalg.solve(a0, b)
+        worst = max(worst, np.abs(got - ref).max() / np.abs(ref).max(),
+                    np.abs(x.astype(np.float64) - xe).max() / (10 * max(1.0, np.abs(xe).max())))
+    ok = (bt.info == 0).all() and worst <= tol
+    return ok, f"count={count} n={n.tolist()} kinds={kind.tolist()} prec={prec} grid={grid} no_graph={no_graph} " \
+               f"err={worst:.1e}"
+
+
+def qr_uses(rng, it, N, b):
+    """QR, then the least-squares solve and Q formed by the NoTrans application."""
+    a = oracle.gen("dense", N, b, seed=it, shift=float(N))
+    A = oracle.to_dense(a, N, b)
+    x, s = a.copy(), np.zeros(N * N)
+    P.qr(x, s, b=b, no_graph=rng.random() < 0.5)
+    nrhs = rng.choice([1, 2, 4])
+    B = np.random.default_rng(it).standard_normal((N, nrhs))
+    X, _ = P.qr_solve(x, s, B, b=b)
+    err_x = np.abs(X - np.linalg.solve(A, B)).max() / np.abs(X).max()
+    Qt = oracle.gen("identity", N, b)
+    P.qr_apply(x, s, Qt, trans=P.NOTRANS, b=b, N=N)
+    Q = oracle.to_dense(Qt, N, b)
+    R = np.triu(oracle.to_dense(x, N, b))
+    eps = np.finfo(float).eps
+    r1 = np.linalg.norm(A - Q @ R) / (N * eps * np.linalg.norm(A))
+    r2 = np.linalg.norm(np.eye(N) - Q.T @ Q) / (N * eps)
+    ok = err_x < 1e-10 and r1 < 10 and r2 < 10
+    return ok, f"N={N} b={b} nrhs={nrhs} err_x={err_x:.1e} |A-QR|={r1:.2f} |I-QtQ|={r2:.2f}"
+
+
 def main(seed, count):
     P.init(0)
     L = P.lib()
@@ -100,14 +162,21 @@ def main(seed, count):
     rng = random.Random(seed)
     fails = 0
     for it in range(count):
-        kind = rng.choice(["factorization", "factorization", "solve", "solve", "solve"])
-        # solves take tile sizes b % 64 == 0 (palial_hip_solve); factorizations any
+        kind = rng.choice(["factorization", "factorization", "solve", "solve", "solve", "batched", "qr"])
+        # solves take tile sizes b % 64 == 0 (palial_hip_solve, palial_hip_qr_solve); factorizations any
         b = rng.choice([32, 64, 96, 128, 128, 256] if kind == "factorization" else [64, 128, 128, 256])
         MT = rng.randint(1, 7 if b <= 128 else 4)
         N = b * MT
         t0 = time.time()
         try:
-            ok, desc = factorization(rng, it, N, b) if kind == "factorization" else solve(rng, it, N, b, MT)
+            if kind == "factorization":
+                ok, desc = factorization(rng, it, N, b)
+            elif kind == "solve":
+                ok, desc = solve(rng, it, N, b, MT)
+            elif kind == "batched":
+                ok, desc = batched(rng, it)
+            else:
+                ok, desc = qr_uses(rng, it, N, b)
         except Exception as ex:      # noqa: BLE001 -- reported as a failure of this configuration
             ok, desc = False, f"N={N} b={b}: {ex!r}"
         n, text = races(L)
