@@ -1,9 +1,9 @@
 """Autotuner: tile size, precision tier and schedule (task priority /
 look-ahead) of the tiled factorizations and solves, ranked by the task-time
 cost model (palial_amd.costmodel), with the exhaustive search as fallback and
-as the judge of the model's predictions.
+as the check of the model's predictions.
 
-    python -m palial_amd.autotune --algo cholesky --n 8192 [--solve] [--judge]
+    python -m palial_amd.autotune --algo cholesky --n 8192 [--solve] [--measure-all]
 
 Candidates:
   factorizations (FP64): b in {128, 256, 384, 512, 768, 1024} dividing N (QR:
@@ -24,7 +24,7 @@ Procedure:
      in the subset, and fall back to measuring every candidate when the
      predictions of the measured candidates (median) or of the pick are off
      by more than `max_error`;
-  4. judge=True also measures every candidate, to report what the model's
+  4. measure_all=True also measures every candidate, to report what the model's
      pick costs against the exhaustive best, the prediction error over all
      candidates and the measurement time the model saved;
   5. solves: for the pick, the sweep block (tiles per correction-sweep step,
@@ -173,7 +173,7 @@ def model_error(pred, measured):
     return float(np.median([abs(pred[c] / v - 1.0) for c, v in measured.items()]))
 
 
-def _report(cands, pred, measured, meas_time, pick, fallback, judge_time=None, exhaustive=None):
+def _report(cands, pred, measured, meas_time, pick, fallback, all_time=None, exhaustive=None):
     out = {"candidates": len(cands), "measured": len(measured), "pick": list(pick), "fallback": fallback,
            "predicted_ms": {f"{c[0]}/{c[1]}": round(pred[c], 3) for c in cands},
            "measured_ms": {f"{c[0]}/{c[1]}": round(v, 3) for c, v in measured.items()},
@@ -186,12 +186,12 @@ def _report(cands, pred, measured, meas_time, pick, fallback, judge_time=None, e
                     "pick_loss": round(exhaustive[pick] / exhaustive[best] - 1.0, 4),
                     "median_abs_error": round(float(np.median(errs)), 4),
                     "max_abs_error": round(float(np.max(errs)), 4),
-                    "exhaustive_measure_s": round(judge_time, 3),
-                    "measure_time_saved_s": round(judge_time - meas_time, 3)})
+                    "exhaustive_measure_s": round(all_time, 3),
+                    "measure_time_saved_s": round(all_time - meas_time, 3)})
     return out
 
 
-def tune_factor(P, algo, N, tiles=TILES, epochs=300, seed=0, max_error=0.25, judge=False):
+def tune_factor(P, algo, N, tiles=TILES, epochs=300, seed=0, max_error=0.25, measure_all=False):
     """Best (b, schedule) of the FP64 factorization; see the module docstring."""
     cands = factor_candidates(algo, N, tiles)
     default_sched = 8 if algo == "qr" else 1
@@ -223,14 +223,14 @@ def tune_factor(P, algo, N, tiles=TILES, epochs=300, seed=0, max_error=0.25, jud
                     measured[c] = fb.run(c[0], c[1])[0]
                     t_meas += time.perf_counter() - t0
             pick = min(cands, key=measured.get)
-        exhaustive, t_judge = None, None
-        if judge:
-            exhaustive, t_judge = {}, 0.0
+        exhaustive, t_all = None, None
+        if measure_all:
+            exhaustive, t_all = {}, 0.0
             for c in cands:
                 t0 = time.perf_counter()
                 exhaustive[c] = fb.run(c[0], c[1])[0]
-                t_judge += time.perf_counter() - t0
-        out = _report(cands, pred, measured, t_meas, pick, fallback, t_judge, exhaustive)
+                t_all += time.perf_counter() - t0
+        out = _report(cands, pred, measured, t_meas, pick, fallback, t_all, exhaustive)
         out.update({"algo": algo, "N": N, "train_samples": nsamp, "train_loss": [float(loss[0]), float(loss[-1])],
                     "scale": scale, "second": "schedule"})
         return out
@@ -263,7 +263,7 @@ def tune_knobs(P, sb, b, tier, N, reps=3):
     return (min(res, key=res.get) if res else (1, 0)), res
 
 
-def tune_solve(P, A, algo, tiles=SOLVE_TILES, tiers=TIERS, epochs=300, seed=0, max_error=0.25, judge=False,
+def tune_solve(P, A, algo, tiles=SOLVE_TILES, tiers=TIERS, epochs=300, seed=0, max_error=0.25, measure_all=False,
                knobs=True):
     """Best (factor tile, tier) of a solve of device matrix A: predicted time =
     model factorization time + the refinement time measured for the tier.
@@ -309,14 +309,14 @@ def tune_solve(P, A, algo, tiles=SOLVE_TILES, tiers=TIERS, epochs=300, seed=0, m
                     measured[c], _, info[c] = sb.run(c[0], c[1])
                     t_meas += time.perf_counter() - t0
             pick = min(cands, key=measured.get)
-        exhaustive, t_judge = None, None
-        if judge:
-            exhaustive, t_judge = {}, 0.0
+        exhaustive, t_all = None, None
+        if measure_all:
+            exhaustive, t_all = {}, 0.0
             for c in cands:
                 t0 = time.perf_counter()
                 exhaustive[c] = sb.run(c[0], c[1])[0]
-                t_judge += time.perf_counter() - t0
-        out = _report(cands, pred, measured, t_meas, pick, fallback, t_judge, exhaustive)
+                t_all += time.perf_counter() - t0
+        out = _report(cands, pred, measured, t_meas, pick, fallback, t_all, exhaustive)
         out.update({"algo": algo, "N": N, "train_samples": nsamp, "train_loss": [float(loss[0]), float(loss[-1])],
                     "scale": scale, "second": "tier", "refine_ms": refine})
         if knobs:
@@ -356,7 +356,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--solve", action="store_true", help="tune the solve (tile x tier) instead of the factorization")
     ap.add_argument("--kappa", type=float, default=1e2)
-    ap.add_argument("--judge", action="store_true", help="also measure every candidate")
+    ap.add_argument("--measure-all", action="store_true", help="also measure every candidate")
     args = ap.parse_args(argv)
     import palial_amd as P
     P.init(0)
@@ -366,11 +366,11 @@ def main(argv=None):
         gen.fill(A, args.kappa)
         gen.free()
         try:
-            res = tune_solve(P, A, args.algo, judge=args.judge)
+            res = tune_solve(P, A, args.algo, measure_all=args.measure_all)
         finally:
             A.free()
     else:
-        res = tune_factor(P, args.algo, args.n, judge=args.judge)
+        res = tune_factor(P, args.algo, args.n, measure_all=args.measure_all)
     print(json.dumps(res))
     return 0
 

@@ -45,9 +45,9 @@ def test_training_fits_a_smooth_target(hip):
 @pytest.mark.parametrize("algo", ["cholesky", "lu", "qr"])
 def test_autotune_factorization(hip, algo):
     """Trained on a measured subset, the model picks (b, schedule); the
-    exhaustive search measures every candidate to judge it."""
+    exhaustive search measures every candidate to check it."""
     from palial_amd import autotune
-    res = autotune.tune_factor(hip, algo, 8192, judge=True)
+    res = autotune.tune_factor(hip, algo, 8192, measure_all=True)
     print(f"autotune {algo}: pick {res['pick']} {res['pick_ms']} ms vs exhaustive {res['exhaustive_pick']} "
           f"{res['exhaustive_best_ms']} ms (loss {res['pick_loss']:.3f}), median |error| "
           f"{res['median_abs_error']:.3f}, measured {res['measured']}/{res['candidates']}, saved "
@@ -67,7 +67,7 @@ def test_autotune_solve(hip):
     A = hip.DeviceMatrix(N, 512)
     try:
         gen.fill(A, 1e2)
-        res = autotune.tune_solve(hip, A, "cholesky", judge=True, knobs=False)   # knobs: wip_gpu_sweep_block.py
+        res = autotune.tune_solve(hip, A, "cholesky", measure_all=True, knobs=False)   # knobs: wip_gpu_sweep_block.py
     finally:
         A.free()
         gen.free()
