@@ -77,12 +77,16 @@ struct emu_event {
     double t_ms = 0.0;
     std::vector<uint32_t> vc;
 };
-// a recorded stream operation: what it is (a kernel's address, or a copy or
-// fill) and the code that performs it
+// a captured node: a stream operation (what it is -- a kernel's address, or a
+// copy or fill -- and the code that performs it), or an event record or wait,
+// with the stream it was issued on
 struct emu_op {
     const void *kfn;
     const char *what;
     std::function<void()> run;
+    int type = 0;                     // 0 operation, 1 event record, 2 event wait
+    emu_stream *stream = nullptr;
+    emu_event *event = nullptr;
 };
 struct emu_graph { std::vector<emu_op> ops; };
 
@@ -102,6 +106,8 @@ void set_workgroup(unsigned wg);
 void barrier_opened();
 void wave_synced(int wave);
 void dma_issued(const void *lds, size_t n);
+void outside_capture(const void *kfn, const char *what);
+bool drop_this_wait();
 void access(const void *p, size_t n, int kind);
 void *dev_alloc(size_t bytes);
 bool dev_free(void *p);
@@ -119,6 +125,8 @@ inline void set_workgroup(unsigned) {}
 inline void barrier_opened() {}
 inline void wave_synced(int) {}
 inline void dma_issued(const void *, size_t) {}
+inline void outside_capture(const void *, const char *) {}
+inline bool drop_this_wait() { return false; }
 inline void access(const void *, size_t, int) {}
 inline void record(hipEvent_t, hipStream_t) {}
 inline bool wait(hipStream_t, hipEvent_t) { return true; }
@@ -162,6 +170,11 @@ hipError_t g_last_err = hipSuccess;
 std::map<const void *, int> g_dyn_limit;
 emu_graph *g_capture = nullptr;
 hipStream_t g_capture_stream = nullptr;
+// streams taking part in the open capture (its origin, and every stream that
+// waited for an event recorded in it) and the events recorded in it
+std::vector<hipStream_t> g_cap_streams;
+std::vector<hipEvent_t> g_cap_events;
+bool in_capture(hipStream_t s) { return std::find(g_cap_streams.begin(), g_cap_streams.end(), s) != g_cap_streams.end(); }
 unsigned long long g_tick = 0;
 auto g_t0 = std::chrono::steady_clock::now();
 
@@ -355,7 +368,12 @@ double now_ms()
 hipError_t stream_op(hipStream_t s, const void *kfn, const char *what, std::function<void()> op)
 {
     if (g_capture) {
-        g_capture->ops.push_back({kfn, what, std::move(op)});
+        // work on a stream that is not part of the capture would run at once
+        // on the device instead of joining the graph
+        if (!in_capture(s)) race::outside_capture(kfn, what);
+        emu_op n{kfn, what, std::move(op)};
+        n.stream = s;
+        g_capture->ops.push_back(std::move(n));
         return hipSuccess;
     }
     race::begin_op(s, kfn, what);
@@ -723,7 +741,18 @@ hipError_t hipStreamSynchronize(hipStream_t s)
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned)
 {
     if (!e) return hipErrorInvalidValue;
-    if (!g_capture) race::wait(s, e);   // a captured graph replays its ops in capture order
+    if (race::drop_this_wait()) return hipSuccess;     // the checker's planted mistake
+    if (g_capture && std::find(g_cap_events.begin(), g_cap_events.end(), e) != g_cap_events.end()) {
+        // a graph edge; the waiting stream joins the capture
+        if (!in_capture(s)) g_cap_streams.push_back(s);
+        emu_op n{nullptr, "wait", nullptr};
+        n.type = 2;
+        n.stream = s;
+        n.event = e;
+        g_capture->ops.push_back(std::move(n));
+        return hipSuccess;
+    }
+    race::wait(s, e);
     return hipSuccess;
 }
 hipError_t hipEventCreate(hipEvent_t *e)
@@ -740,7 +769,15 @@ hipError_t hipEventDestroy(hipEvent_t e)
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s)
 {
     if (!e) return hipErrorInvalidValue;
-    if (g_capture) return hipSuccess;
+    if (g_capture && in_capture(s)) {
+        if (std::find(g_cap_events.begin(), g_cap_events.end(), e) == g_cap_events.end()) g_cap_events.push_back(e);
+        emu_op n{nullptr, "record", nullptr};
+        n.type = 1;
+        n.stream = s;
+        n.event = e;
+        g_capture->ops.push_back(std::move(n));
+        return hipSuccess;
+    }
     e->t_ms = now_ms();
     race::record(e, s);
     return hipSuccess;
@@ -762,6 +799,8 @@ hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode)
     if (g_capture) return hipErrorStreamCaptureUnsupported;
     g_capture = new emu_graph();
     g_capture_stream = s;
+    g_cap_streams.assign(1, s);
+    g_cap_events.clear();
     return hipSuccess;
 }
 hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g)
@@ -770,6 +809,8 @@ hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g)
     *g = g_capture;
     g_capture = nullptr;
     g_capture_stream = nullptr;
+    g_cap_streams.clear();
+    g_cap_events.clear();
     return hipSuccess;
 }
 hipError_t hipGraphInstantiate(hipGraphExec_t *x, hipGraph_t g, hipGraphNode_t *, char *, size_t)
@@ -780,8 +821,53 @@ hipError_t hipGraphInstantiate(hipGraphExec_t *x, hipGraph_t g, hipGraphNode_t *
 hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s)
 {
     if (!x) return hipErrorInvalidValue;
-    // the recorded ops one after the other on the launch stream
-    for (auto &op : x->ops) stream_op(s, op.kfn, op.what, op.run);
+#ifdef HIPEMU_RACE
+    // The nodes in capture order, each captured stream replaced by a fresh
+    // stream that starts after the launch stream's earlier work and each
+    // captured event by a fresh event, so that the graph keeps exactly its
+    // captured edges; the launch stream then waits for all of them.
+    std::vector<std::pair<hipStream_t, hipStream_t>> vs;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ve;
+    hipEvent_t start;
+    (void)hipEventCreate(&start);
+    race::record(start, s);
+    auto vstream = [&](hipStream_t o) {
+        for (auto &p : vs)
+            if (p.first == o) return p.second;
+        hipStream_t v;
+        (void)hipStreamCreateWithFlags(&v, hipStreamNonBlocking);
+        race::wait(v, start);
+        vs.push_back({o, v});
+        return v;
+    };
+    auto vevent = [&](hipEvent_t o) {
+        for (auto &p : ve)
+            if (p.first == o) return p.second;
+        hipEvent_t v;
+        (void)hipEventCreate(&v);
+        ve.push_back({o, v});
+        return v;
+    };
+    for (auto &op : x->ops) {
+        if (op.type == 1) race::record(vevent(op.event), vstream(op.stream));
+        else if (op.type == 2) race::wait(vstream(op.stream), vevent(op.event));
+        else stream_op(vstream(op.stream), op.kfn, op.what, op.run);
+    }
+    for (auto &p : vs) {
+        hipEvent_t end;
+        (void)hipEventCreate(&end);
+        race::record(end, p.second);
+        race::wait(s, end);
+        (void)hipEventDestroy(end);
+        (void)hipStreamDestroy(p.second);
+    }
+    for (auto &p : ve) (void)hipEventDestroy(p.second);
+    (void)hipEventDestroy(start);
+#else
+    // the recorded operations one after the other on the launch stream
+    for (auto &op : x->ops)
+        if (op.type == 0) stream_op(s, op.kfn, op.what, op.run);
+#endif
     return hipSuccess;
 }
 hipError_t hipGraphExecDestroy(hipGraphExec_t x)
@@ -848,9 +934,11 @@ long hipemu_ticks(void) { return (long)g_tick; }
 //   * lanes of one wave that pass a value through memory with no wave
 //     operation between, in an order whose outcome differs between the
 //     lockstep device and the emulator's lane-by-lane execution (lane_check).
+//   * a captured graph keeps its edges: its streams and events are replayed
+//     as fresh ones forked from the launch stream (hipGraphLaunch), and work
+//     issued during a capture on a stream outside it is reported.
 // What it does not see: accesses wider than 16 bytes in one instruction (the
-// instrumentation skips them), and ops inside a captured graph (replayed in
-// capture order on one stream; the same schedule issued eagerly is checked).
+// instrumentation skips them).
 #ifdef HIPEMU_RACE
 namespace emu {
 namespace race {
@@ -1010,6 +1098,8 @@ std::string where(uintptr_t a)
     return buf;
 }
 
+const char *g_report_what = nullptr;                   // kind 15: the operation
+
 void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1, uint32_t wg2)
 {
     static const char *const names[] = {"read after an unordered write", "write after an unordered write",
@@ -1024,7 +1114,8 @@ void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1
                                         "read of what another lane of the wave wrote, with no wave operation between",
                                         "write over what another lane of the wave read, with no wave operation between",
                                         "two lanes of one wave write one location, with no wave operation between",
-                                        "LDS read of a global_load_lds destination before its s_waitcnt vmcnt"};
+                                        "LDS read of a global_load_lds destination before its s_waitcnt vmcnt",
+                                        "operation issued during a stream capture on a stream outside it"};
     g_count++;
     auto key = std::make_tuple(first, second, kind);
     if (g_seen.count(key) || g_seen.size() >= 64) return;
@@ -1032,7 +1123,9 @@ void report(int kind, uint32_t first, uint32_t second, uintptr_t a, uint32_t wg1
     const bool dev = a - g_base < DEV_SPAN;
     std::string m = std::string(names[kind]) + " at " +
                     (kind >= 7 && !dev ? std::string("an LDS byte") : where(a)) + ": ";
-    if (kind == 14) {
+    if (kind == 15) {
+        m = std::string(names[kind]) + ": " + (g_report_what ? g_report_what : "?");
+    } else if (kind == 14) {
         char buf[96];
         snprintf(buf, sizeof buf, " workgroup %u, lane %d of the reading wave", g_wg, g_lane->lin);
         m += op_name(second) + buf;
@@ -1315,6 +1408,16 @@ void wave_synced(int wave)
     if (wave < MAX_WAVES) new_interval(wave);
 }
 
+void outside_capture(const void *kfn, const char *what)
+{
+    std::string k = kfn ? kname(kfn) : std::string(what);
+    if (k == "?") k = what;
+    static std::string held;
+    held = k;
+    g_report_what = held.c_str();
+    report(15, 0, 0, 0, 0, 0);
+}
+
 void dma_issued(const void *lds, size_t n)
 {
     if (!g_lane || !n) return;
@@ -1335,9 +1438,10 @@ void record(hipEvent_t e, hipStream_t s)
     e->vc = t->vc;
 }
 
+bool drop_this_wait() { return g_drop_wait > 0 && --g_drop_wait == 0; }
+
 bool wait(hipStream_t s, hipEvent_t e)
 {
-    if (g_drop_wait > 0 && --g_drop_wait == 0) return false;
     emu_stream *t = st(s);
     join(t->vc, g_host);
     join(t->vc, e->vc);

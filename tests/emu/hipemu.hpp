@@ -21,8 +21,8 @@
 //     products summed in FP64 and rounded once to the accumulator type (FP64:
 //     one fused multiply-add per K step).
 // Streams execute synchronously in host issue order; hipStreamBeginCapture
-// records the launches and copies issued until hipStreamEndCapture, and
-// hipGraphLaunch replays them.  Dynamic LDS is a heap block of exactly the
+// records the launches, copies and event edges issued until
+// hipStreamEndCapture, and hipGraphLaunch replays them.  Dynamic LDS is a heap block of exactly the
 // launch's size and hipMalloc returns blocks of exactly the requested size,
 // so an AddressSanitizer build reports any access outside them.  The
 // race-checking build (HIPEMU_RACE) instead checks every access to device

@@ -119,6 +119,8 @@ L.hipemu_races.restype = C.c_long
 N, b = 512, 128
 rhs = np.random.default_rng(1).standard_normal((N, 1))
 def run(algo, drop, prec=None):
+    graph = algo.endswith("-graph")            # the FP64 factorization as captured and replayed
+    algo = algo.replace("-graph", "")
     a = oracle.gen("sym" if algo == "cholesky" else "dense", N, b, seed=3, shift=0.0 if algo == "qr" else N)
     L.hipemu_race_reset()
     L.hipemu_drop_wait(drop)
@@ -127,11 +129,11 @@ def run(algo, drop, prec=None):
     elif algo == "qr":
         P.qr(a, np.zeros(N * N), b=b, no_graph=True)
     else:
-        getattr(P, algo)(a, b=b, no_graph=True)
+        getattr(P, algo)(a, b=b, no_graph=not graph)
     L.hipemu_drop_wait(0)
     return L.hipemu_races()
 for algo, prec, drops in (("cholesky", None, (3, 5)), ("lu", None, (3,)), ("qr", None, (17, 18)),
-                          ("cholesky", 2, (6,))):
+                          ("cholesky", 2, (6,)), ("cholesky-graph", None, (3,))):
     print(algo, prec, run(algo, 0, prec), *[run(algo, d, prec) for d in drops], flush=True)
 L.hipemu_race_reset()
 assert L.hipemu_plant_wg_race() == 0
@@ -145,8 +147,10 @@ print("WG", L.hipemu_races(), buf.value.decode().replace("\n", " | "))
 def test_checker_reports_planted_races(race_lib):
     """The check has teeth.  With one event wait of a schedule left out
     (hipemu_drop_wait: the harness's planted mistake) the FP64 Cholesky, LU
-    and QR factorizations and the BF16 Cholesky solve report racing accesses,
-    and the same runs with every wait in place report none; a launch in which
+    and QR factorizations, the BF16 Cholesky solve and the FP64 Cholesky
+    captured into a graph (the wait left out of the capture, so the replay
+    lacks that edge) report racing accesses, and the same runs with every
+    wait in place report none; a launch in which
     two workgroups store to one element, or one reads another's store, is
     reported as a race between workgroups; a wave that reads what another
     wave stored to LDS with no barrier between, a workgroup that reads LDS it
@@ -159,7 +163,7 @@ def test_checker_reports_planted_races(race_lib):
     assert r.returncode == 0, r.stderr[-3000:]
     lines = [l.split() for l in r.stdout.splitlines()]
     runs = [l for l in lines if l[0] != "WG"]
-    assert len(runs) == 4, r.stdout
+    assert len(runs) == 5, r.stdout
     for l in runs:
         clean, dropped = int(l[2]), [int(x) for x in l[3:]]
         assert clean == 0 and all(d > 0 for d in dropped), l
