@@ -1,7 +1,9 @@
 """Random configurations of the library's API on the CPU build of its HIP
 sources (tests/emu): factorizations of every algorithm and schedule, solves
-of every precision tier, sweep block and paired-update setting, batched
-factorizations, and QR least-squares solves and Q formation, on
+of every precision tier, sweep block and paired-update setting (also on
+matrices of condition number up to 1e9, through breakdown, shifted
+refactorization, GMRES-IR and the FP64 fallback), batched factorizations,
+and QR least-squares solves and Q formation, on
 small matrices with tile sizes from 32 to 256 (solves: multiples of 64, as
 palial_hip_solve requires).  Each result is checked -- factors against the
 oracle to 1e-10, solves to backward error 2e-14 recomputed with numpy (or a
@@ -94,6 +96,28 @@ def solve(rng, it, N, b, MT):
                 f"used={rep.precision_used}")
 
 
+def conditioned(rng, it):
+    """A solve of Q diag(s) Q^T with condition number 1e2 to 1e9 in a random
+    tier: a low-precision factor may break down, be shifted, refine with
+    GMRES-IR or hand over to FP64 -- the answer must still reach 2e-14."""
+    b = 128
+    N = b * rng.randint(2, 5)
+    kappa = rng.choice([1e2, 1e5, 1e7, 1e9])
+    algo = rng.choice(["cholesky", "lu"])
+    prec = rng.choice([P.PREC_FP32, P.PREC_BF16, P.PREC_FP8, P.PREC_BF16X3])
+    g = np.random.default_rng(it)
+    Q, _ = np.linalg.qr(g.standard_normal((N, N)))
+    A = (Q * np.geomspace(1.0, 1.0 / kappa, N)) @ Q.T
+    A = (A + A.T) / 2
+    a = oracle.from_dense(A, N, b)
+    B = g.standard_normal((N, rng.choice([1, 3])))
+    X, rep, rc = P.solve(a, B, algo=algo, precision=prec, b=b, factor_tile=b, check=False)
+    eta = backward_error(A, X, B) if rc == 0 else float("nan")
+    return rc == 0 and eta <= 2e-14, (f"{algo} prec={prec} N={N} kappa={kappa:g} nrhs={B.shape[1]} rc={rc} "
+                                      f"eta={eta:.1e} used={rep.precision_used} shifts={rep.shifts} "
+                                      f"gmres={rep.gmres_iterations}")
+
+
 def doolittle(a):
     """No-pivot LU (L unit lower strictly below, U upper), in a copy."""
     a = a.copy()
@@ -162,7 +186,8 @@ def main(seed, count):
     rng = random.Random(seed)
     fails = 0
     for it in range(count):
-        kind = rng.choice(["factorization", "factorization", "solve", "solve", "solve", "batched", "qr"])
+        kind = rng.choice(["factorization", "factorization", "solve", "solve", "solve", "batched", "qr",
+                           "conditioned"])
         # solves take tile sizes b % 64 == 0 (palial_hip_solve, palial_hip_qr_solve); factorizations any
         b = rng.choice([32, 64, 96, 128, 128, 256] if kind == "factorization" else [64, 128, 128, 256])
         MT = rng.randint(1, 7 if b <= 128 else 4)
@@ -175,6 +200,8 @@ def main(seed, count):
                 ok, desc = solve(rng, it, N, b, MT)
             elif kind == "batched":
                 ok, desc = batched(rng, it)
+            elif kind == "conditioned":
+                ok, desc = conditioned(rng, it)
             else:
                 ok, desc = qr_uses(rng, it, N, b)
         except Exception as ex:      # noqa: BLE001 -- reported as a failure of this configuration
