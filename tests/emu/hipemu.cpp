@@ -930,10 +930,14 @@ long hipemu_ticks(void) { return (long)g_tick; }
 //   * LDS (the __shared__ arrays, gathered in section emu_lds, and a launch's
 //     dynamic block): an access that conflicts with another wave's access
 //     in the same barrier interval is a race, and a read of a granule this
-//     workgroup has not written reads what an earlier workgroup left.
+//     workgroup has not written reads what an earlier workgroup left;
 //   * lanes of one wave that pass a value through memory with no wave
 //     operation between, in an order whose outcome differs between the
-//     lockstep device and the emulator's lane-by-lane execution (lane_check).
+//     lockstep device and the emulator's lane-by-lane execution (lane_check);
+//   * an LDS read of a global_load_lds destination whose load the issuing
+//     lane has not yet retired with s_waitcnt vmcnt (dma_issued, waitcnt);
+//   * a vector access at an address that is not a multiple of its size
+//     ends the run (aligned);
 //   * a captured graph keeps its edges: its streams and events are replayed
 //     as fresh ones forked from the launch stream (hipGraphLaunch), and work
 //     issued during a capture on a stream outside it is reported.
