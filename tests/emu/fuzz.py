@@ -13,6 +13,10 @@ the race-checking variant, every access against the stream order.
 
     python tests/emu/fuzz.py SEED COUNT        (FUZZ_VARIANT=race|asan|plain)
 
+FUZZ_ROWS=LO-HI draws the tile rows of factorizations and solves from LO..HI
+instead (with tile sizes 32 and 64 for factorizations, 64 for solves), for
+deeper schedules.
+
 One line per configuration ("ok" or "BAD"), then "fails <n>".
 """
 import ctypes as C
@@ -35,6 +39,7 @@ P.LIB_PATH = emu_build.build(VARIANT)
 import oracle  # noqa: E402
 
 ERR_ARG = -1
+ROWS = tuple(int(x) for x in os.environ["FUZZ_ROWS"].split("-")) if os.environ.get("FUZZ_ROWS") else None
 
 
 def races(L):
@@ -189,8 +194,12 @@ def main(seed, count):
         kind = rng.choice(["factorization", "factorization", "solve", "solve", "solve", "batched", "qr",
                            "conditioned"])
         # solves take tile sizes b % 64 == 0 (palial_hip_solve, palial_hip_qr_solve); factorizations any
-        b = rng.choice([32, 64, 96, 128, 128, 256] if kind == "factorization" else [64, 128, 128, 256])
-        MT = rng.randint(1, 7 if b <= 128 else 4)
+        if ROWS and kind in ("factorization", "solve"):
+            b = rng.choice([32, 64]) if kind == "factorization" else 64
+            MT = rng.randint(*ROWS)
+        else:
+            b = rng.choice([32, 64, 96, 128, 128, 256] if kind == "factorization" else [64, 128, 128, 256])
+            MT = rng.randint(1, 7 if b <= 128 else 4)
         N = b * MT
         t0 = time.time()
         try:
